@@ -1,0 +1,388 @@
+/*
+ * shiro_hip.h - C ABI of the MI355X (gfx950) path-tracing backend.
+ *
+ * This is the drop-in boundary for Shiro's GPU backend slot.  Shiro's
+ * renderer talks to a backend through the virtual interface
+ *   class RenderBackend { GetStatus(); RenderSampleBatch(const RenderRequest&); }
+ * (reference include/shiro/backend/RenderBackend.h:12-34), and its GPU
+ * backend additionally answers SupportsScene() (include/shiro/backend/optix/
+ * OptixBackend.h:16) which Renderer::Impl::ResolveBackend consults
+ * (src/render/Renderer.cpp:16-31).  The C++ adapter HipBackend
+ * (include/shiro/backend/hip/HipBackend.h) implements that interface on top
+ * of the functions below; every other caller (Python/ctypes, tests, bench)
+ * binds these symbols directly.
+ *
+ * Conventions
+ *  - Plain C, POD structs, no exceptions.  Every function that can fail
+ *    returns a shiro_hip_result; shiro_hip_last_error() gives the message.
+ *  - Vectors are float[3] in the reference's Vec3f order (x, y, z).
+ *  - Frame buffers are row major, pixel (x, y) at y*width + x, with y = 0
+ *    the bottom row (reference CpuPathTracer.cpp:1633-1636).
+ *  - A context owns one HIP stream on one device; calls on one context are
+ *    serialised by an internal mutex, like the reference backends
+ *    (CpuPathTracer.h:26-31, OptixBackend.h:24-26).
+ */
+#ifndef SHIRO_HIP_H
+#define SHIRO_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SHIRO_HIP_ABI_VERSION 1u
+
+typedef enum shiro_hip_result {
+    SHIRO_HIP_OK = 0,
+    SHIRO_HIP_NO_DEVICE = 1,   /* no HIP device / init failed (OptixBackend.cpp:1049-1058) */
+    SHIRO_HIP_OOM = 2,         /* device allocation failed */
+    SHIRO_HIP_CANCELLED = 3,   /* cancel flag observed between launches */
+    SHIRO_HIP_INVALID = 4,     /* invalid argument or scene */
+    SHIRO_HIP_NO_SCENE = 5,    /* render requested before a scene upload */
+    SHIRO_HIP_LAUNCH = 6       /* a kernel launch or HIP runtime call failed */
+} shiro_hip_result;
+
+/* Environment map layout, reference Renderer.h:18-22 (EnvironmentMapLayout). */
+typedef enum shiro_hip_env_layout {
+    SHIRO_HIP_ENV_AUTOMATIC = 0,
+    SHIRO_HIP_ENV_LATLONG = 1,
+    SHIRO_HIP_ENV_ANGULAR = 2
+} shiro_hip_env_layout;
+
+/* Transport form used by render calls. */
+typedef enum shiro_hip_transport {
+    SHIRO_HIP_TRANSPORT_MEGAKERNEL = 0, /* one fused TracePath per thread */
+    SHIRO_HIP_TRANSPORT_WAVEFRONT = 1   /* queue-driven stage kernels */
+} shiro_hip_transport;
+
+/* PbrMaterial, reference Renderer.h:56-102; same fields, same order.
+ * Booleans are uint32_t (0/1) to keep the layout compiler independent.
+ * The two texture indices are an extension for per-texel parameters
+ * (-1 = none); the reference has no textures. */
+typedef struct shiro_hip_material {
+    float baseColor[3];
+    float emissionColor[3];
+    float specularColor[3];
+    float transmissionColor[3];
+    float transmissionScatter[3];
+    float coatColor[3];
+    float subsurfaceColor[3];
+    float subsurfaceRadius[3];
+    float sheenColor[3];
+    float normalOverride[3];
+    float coatNormalOverride[3];
+    float tangentOverride[3];
+    float baseWeight;
+    float emissionStrength;
+    float specularWeight;
+    float metallic;
+    float roughness;
+    float diffuseRoughness;
+    float opacity;
+    float specularAnisotropy;
+    float specularRotation;
+    float coatWeight;
+    float coatRoughness;
+    float coatIor;
+    float coatAnisotropy;
+    float coatRotation;
+    float sheen;
+    float sheenRoughness;
+    float subsurface;
+    float subsurfaceScale;
+    float subsurfaceAnisotropy;
+    float transmission;
+    float transmissionDepth;
+    float transmissionScatterAnisotropy;
+    float transmissionDispersion;
+    float transmissionExtraRoughness;
+    float ior;
+    float coatAffectColor;
+    float coatAffectRoughness;
+    float thinFilmThickness;
+    float thinFilmIor;
+    uint32_t thinWalled;
+    uint32_t hasNormalOverride;
+    uint32_t hasCoatNormalOverride;
+    uint32_t hasTangentOverride;
+    int32_t baseColorTexture;
+    int32_t roughnessTexture;
+} shiro_hip_material;
+
+/* TriangleMesh, reference Renderer.h:104-109.  normals are used only when
+ * normalCount == vertexCount (CpuPathTracer.cpp:148-151).  uvs (2 floats per
+ * vertex) are an extension used only by textured materials.  Triangles are
+ * read as consecutive index triples; a trailing partial triple is ignored
+ * (CpuPathTracer.cpp:434). */
+typedef struct shiro_hip_mesh {
+    const float* positions;
+    const float* normals;
+    const float* uvs;
+    const uint32_t* indices;
+    uint32_t vertexCount;
+    uint32_t normalCount;
+    uint32_t indexCount;
+    uint32_t materialIndex;
+} shiro_hip_mesh;
+
+/* DirectionalLight, reference Renderer.h:111-114: direction of travel. */
+typedef struct shiro_hip_directional_light {
+    float direction[3];
+    float radiance[3];
+} shiro_hip_directional_light;
+
+/* DomeLight, reference Renderer.h:116-124.  pixels == NULL means a dome
+ * without an environment texture (constant radiance, uniform sampling,
+ * CpuPathTracer.cpp:327-384).  pixels holds width*height RGB triples, row 0
+ * is the top of the map (theta = 0). */
+typedef struct shiro_hip_dome_light {
+    float radiance[3];
+    float right[3];
+    float up[3];
+    float forward[3];
+    const float* pixels;
+    uint32_t width;
+    uint32_t height;
+    uint32_t layout; /* shiro_hip_env_layout */
+    uint32_t reserved;
+} shiro_hip_dome_light;
+
+/* Single-channel or RGB float texture for the texture extension. */
+typedef struct shiro_hip_texture {
+    const float* texels;
+    uint32_t width;
+    uint32_t height;
+    uint32_t channels; /* 1 or 3 */
+    uint32_t reserved;
+} shiro_hip_texture;
+
+/* Scene, reference Renderer.h:126-133. */
+typedef struct shiro_hip_scene {
+    const shiro_hip_material* materials;
+    uint32_t materialCount;
+    const shiro_hip_mesh* meshes;
+    uint32_t meshCount;
+    const shiro_hip_directional_light* distantLights;
+    uint32_t distantLightCount;
+    const shiro_hip_dome_light* domeLights;
+    uint32_t domeLightCount;
+    float environmentTop[3];
+    float environmentBottom[3];
+    const shiro_hip_texture* textures;
+    uint32_t textureCount;
+} shiro_hip_scene;
+
+/* Camera, reference Renderer.h:47-54. */
+typedef struct shiro_hip_camera {
+    float position[3];
+    float forward[3];
+    float right[3];
+    float up[3];
+    float verticalFovDegrees;
+    float aspectRatio;
+} shiro_hip_camera;
+
+/* RenderSettings, reference Renderer.h:29-45 (same fields and defaults;
+ * see shiro_hip_default_settings).  russianRouletteDepth is an extension:
+ * the reference starts roulette at depth 2 (CpuPathTracer.cpp:2006), which is
+ * the default here.  transport selects the kernel form. */
+typedef struct shiro_hip_settings {
+    uint32_t width;
+    uint32_t height;
+    uint32_t samplesPerPixel;
+    uint32_t samplesPerUpdate;
+    uint32_t domeLightSamples;
+    uint32_t maxDepth;
+    uint32_t diffuseDepth;
+    uint32_t specularDepth;
+    uint32_t threadLimit;
+    uint32_t maxSubdivLevel;
+    uint32_t backend;
+    uint32_t backgroundVisible;
+    uint32_t enableHeadlight;
+    uint32_t enableCaustics;
+    uint32_t enableSss;
+    uint32_t russianRouletteDepth;
+    uint32_t transport;
+    uint32_t reserved;
+} shiro_hip_settings;
+
+/* FrameBuffer, reference FrameBuffer.h:10-37: AoS beauty RGBA (float4),
+ * albedo RGB, normal RGB (0.5*n + 0.5), depth (min hit distance, +inf when
+ * no sample hit).  Pointers are host or device memory depending on the call. */
+typedef struct shiro_hip_frame {
+    float* beauty;
+    float* albedo;
+    float* normal;
+    float* depth;
+    uint32_t width;
+    uint32_t height;
+} shiro_hip_frame;
+
+/* BackendStatus, reference RenderBackend.h:21-26. */
+typedef struct shiro_hip_status {
+    uint32_t kind;       /* BackendKind::Gpu = 1 */
+    const char* name;    /* "hip" */
+    uint32_t available;
+    uint32_t usesGpu;
+    char deviceName[64];
+    uint32_t computeUnits;
+} shiro_hip_status;
+
+/* Opaque backend context: device, stream, device-resident scene. */
+typedef struct shiro_hip_ctx shiro_hip_ctx;
+
+/* Fill settings with the reference defaults (Renderer.h:29-45). */
+void shiro_hip_default_settings(shiro_hip_settings* settings);
+/* Fill a material with the reference defaults (Renderer.h:56-102). */
+void shiro_hip_default_material(shiro_hip_material* material);
+
+/* Context creation; replaces OptixBackend::EnsureRuntime's device/stream
+ * set-up (OptixBackend.cpp:1071-1139). */
+int shiro_hip_create(int device, shiro_hip_ctx** out);
+void shiro_hip_destroy(shiro_hip_ctx* ctx);
+const char* shiro_hip_last_error(const shiro_hip_ctx* ctx);
+
+/* GetStatus, RenderBackend.h:30 / OptixBackend.cpp:1035-1060.  Works with
+ * ctx == NULL (probes the device without creating a context). */
+int shiro_hip_get_status(const shiro_hip_ctx* ctx, shiro_hip_status* out);
+
+/* SupportsScene, OptixBackend.h:16: 1 if the scene can be rendered. */
+int shiro_hip_supports_scene(const shiro_hip_scene* scene);
+
+/* Scene upload: flattens meshes, builds the BVH4 on the host (binned SAH),
+ * builds dome importance tables and uploads everything.  Replaces
+ * OptixBackend::UploadSceneData + GAS build (OptixBackend.cpp:751-1026) and
+ * BuildAccelerationScene (CpuPathTracer.cpp:477-598).  The scene is copied;
+ * the caller may free it after the call. */
+int shiro_hip_upload_scene(shiro_hip_ctx* ctx, const shiro_hip_scene* scene);
+
+/* RenderSampleBatch, RenderBackend.h:33 / CpuPathTracer.cpp:2043-2170:
+ * renders samples [sampleStart, sampleStart + sampleCount) of every pixel and
+ * writes the batch means into host buffers `out` (sized settings->width x
+ * settings->height).  cancel (may be NULL) is polled between launches; when
+ * it becomes non-zero the call returns SHIRO_HIP_CANCELLED and `out` holds a
+ * cleared frame. */
+int shiro_hip_render_batch(
+    shiro_hip_ctx* ctx,
+    const shiro_hip_camera* camera,
+    const shiro_hip_settings* settings,
+    uint32_t sampleStart,
+    uint32_t sampleCount,
+    const volatile int32_t* cancel,
+    shiro_hip_frame* out);
+
+/* Same as shiro_hip_render_batch, but `out` holds device pointers and the
+ * work is enqueued on the context stream without a host copy.  When
+ * `synchronize` is non-zero the call waits for completion. */
+int shiro_hip_render_batch_device(
+    shiro_hip_ctx* ctx,
+    const shiro_hip_camera* camera,
+    const shiro_hip_settings* settings,
+    uint32_t sampleStart,
+    uint32_t sampleCount,
+    shiro_hip_frame* out,
+    int synchronize);
+
+/* Device-side FrameAccumulator (Renderer.cpp:47-110).  accumulate adds the
+ * batch frame held in device buffers `batch` with weight sampleCount;
+ * resolve writes the resolved frame into device buffers `out`. */
+int shiro_hip_accum_reset(shiro_hip_ctx* ctx, uint32_t width, uint32_t height);
+int shiro_hip_accum_add(shiro_hip_ctx* ctx, const shiro_hip_frame* batch, uint32_t sampleCount);
+int shiro_hip_accum_resolve(shiro_hip_ctx* ctx, shiro_hip_frame* out);
+
+/* Stream and timing.  last_render_ms is the HIP-event time of the most
+ * recent render call's kernels on the context stream. */
+void* shiro_hip_stream(shiro_hip_ctx* ctx);
+int shiro_hip_synchronize(shiro_hip_ctx* ctx);
+int shiro_hip_last_render_ms(const shiro_hip_ctx* ctx, float* ms);
+
+/* Scene statistics after upload (triangles, BVH nodes, depth, bytes). */
+typedef struct shiro_hip_scene_info {
+    uint64_t triangleCount;
+    uint64_t bvhNodeCount;
+    uint64_t bvhLeafCount;
+    uint32_t bvhDepth;
+    uint32_t maxLeafSize;
+    uint64_t deviceBytes;
+    double buildSeconds;
+    double uploadSeconds;
+} shiro_hip_scene_info;
+int shiro_hip_scene_info_get(const shiro_hip_ctx* ctx, shiro_hip_scene_info* out);
+
+/* ---- Entry points exposing single stages of the hot path (device-side
+ * counterparts of the reference functions named; inputs/outputs are host
+ * arrays, n elements). ---- */
+
+/* OpenQmcSampler fallback LowDiscrepancySample (OpenQmcSampler.cpp:56-71)
+ * over pixels [x0,x0+w) x [y0,y0+h), dims [dim0, dim0+dimCount) for one
+ * sample index; out[((y-y0)*w + (x-x0))*dimCount + d]. */
+int shiro_hip_qmc_fill(
+    shiro_hip_ctx* ctx,
+    uint32_t x0, uint32_t y0, uint32_t w, uint32_t h,
+    uint32_t sampleIndex, uint32_t dim0, uint32_t dimCount,
+    float* out);
+
+/* Closest hit of arbitrary rays (IntersectScene, CpuPathTracer.cpp:684-695).
+ * rays: n x 8 floats (origin xyz, direction xyz, tMin, tMax).  prim receives
+ * the global triangle index (meshes concatenated in order, 0xffffffff on a
+ * miss), t the hit distance, uv the barycentrics of vertices 1 and 2. */
+int shiro_hip_intersect(
+    shiro_hip_ctx* ctx, const float* rays, uint32_t n,
+    uint32_t* prim, float* t, float* uv);
+
+/* Any-hit test (Occluded, CpuPathTracer.cpp:697-709): out[i] = 1 if hit. */
+int shiro_hip_occluded(
+    shiro_hip_ctx* ctx, const float* rays, uint32_t n, uint8_t* out);
+
+/* Primary visibility: camera ray for `sampleIndex` at every pixel
+ * (GenerateCameraRay, CpuPathTracer.cpp:1633-1648) and its closest hit. */
+int shiro_hip_trace_primary(
+    shiro_hip_ctx* ctx, const shiro_hip_camera* camera,
+    const shiro_hip_settings* settings, uint32_t sampleIndex,
+    uint32_t* prim, float* t);
+
+/* EvaluateBsdf / EvaluateBsdfPdf (CpuPathTracer.cpp:1232-1354) for n
+ * tuples: material index into the uploaded scene, shading normal, coat
+ * normal, view and light directions (3 floats each). */
+int shiro_hip_eval_bsdf(
+    shiro_hip_ctx* ctx, const uint32_t* materialIndex,
+    const float* normal, const float* coatNormal,
+    const float* view, const float* light, uint32_t n,
+    float* value, float* pdf);
+
+/* Dome light i of the uploaded scene: EnvironmentMap::Sample
+ * (EnvironmentMap.cpp:109-163) for n sample pairs u (2 floats each), giving
+ * local direction, pdf and radiance; and EnvironmentMap::Pdf / Evaluate
+ * (:94-107, :165-183) of local directions. */
+int shiro_hip_env_sample(
+    shiro_hip_ctx* ctx, uint32_t domeIndex, const float* u, uint32_t n,
+    float* direction, float* pdf, float* radiance);
+int shiro_hip_env_eval(
+    shiro_hip_ctx* ctx, uint32_t domeIndex, const float* direction,
+    uint32_t n, float* pdf, float* radiance);
+
+/* Traversal counters for the roofline: runs the render once with fetch
+ * counting enabled and returns totals over the frame. */
+typedef struct shiro_hip_fetch_counts {
+    uint64_t samples;
+    uint64_t rays;          /* closest-hit rays */
+    uint64_t shadowRays;    /* any-hit rays */
+    uint64_t nodeFetches;   /* BVH4 nodes read (128 B each) */
+    uint64_t triFetches;    /* triangles tested (48 B each) */
+    uint64_t envTaps;       /* environment texel reads (16 B each) */
+    uint64_t cdfSteps;      /* importance table reads (4 B each) */
+    uint64_t hitFetches;    /* per-hit vertex/normal/material fetches */
+} shiro_hip_fetch_counts;
+int shiro_hip_count_fetches(
+    shiro_hip_ctx* ctx, const shiro_hip_camera* camera,
+    const shiro_hip_settings* settings, uint32_t sampleStart,
+    uint32_t sampleCount, shiro_hip_fetch_counts* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SHIRO_HIP_H */

@@ -1,0 +1,2103 @@
+// TEST INFRASTRUCTURE ONLY.
+//
+// CPU oracle for the MI355X path-tracing backend: a restatement, written for
+// this project, of the reference CPU path tracer's algorithm --
+//   src/backend/cpu/CpuPathTracer.cpp:57-2170   (transport, BSDF, lights)
+//   src/render/OpenQmcSampler.cpp:20-71,106-135 (fallback QMC sampler)
+//   src/render/EnvironmentMap.cpp:17-47,94-330  (dome map lookup + IS)
+//   src/render/Renderer.cpp:47-110              (FrameAccumulator)
+// with the same fp32 operation order wherever rounding matters.  It is the
+// checker the HIP kernels are compared against and the CPU baseline timed
+// beside them; the product library never links it.  Only tests, the smoke
+// check and the benchmark's CPU-baseline leg load it.
+//
+// Intersection has two modes: brute force Moller-Trumbore over all triangles
+// in scene order (the reference's build without Embree,
+// CpuPathTracer.cpp:386-475) and a binary SAH BVH over the same triangle test
+// with the same tie rule (first triangle in scene order wins an exact tie),
+// which stands in for Embree (absent here) and is what the CPU baseline runs.
+//
+// Pinning (see tests/test_oracle_*.py): every exported function is compared
+// against the reference sources compiled under oracle/_ref (fixtures in
+// tests/golden/) and against closed forms (radical inverse in base 2 is bit
+// reversal, Lambert reduction of the BSDF, GGX normalisation, CDF invariants,
+// brute force vs BVH).
+
+#include "oracle_math.h"
+
+#include "../include/shiro_hip.h"
+
+#include <algorithm>
+#include <array>
+#include <atomic>
+#include <cfloat>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+namespace orc {
+
+// ---------------------------------------------------------------------------
+// Constants (CpuPathTracer.cpp:59-77, EnvironmentMap.cpp:19-21)
+// ---------------------------------------------------------------------------
+constexpr float kPi = 3.14159265358979323846f;
+constexpr float kTwoPi = 2.0f * kPi;
+constexpr float kInvPi = 1.0f / kPi;
+constexpr float kInvFourPi = 1.0f / (4.0f * kPi);
+constexpr float kEps = 1.0e-4f;             // ray epsilon
+constexpr float kDirectClamp = 24.0f;       // NEE sample clamp
+constexpr float kThroughputClamp = 12.0f;
+constexpr float kDeltaRoughness = 0.02f;
+constexpr uint32_t kBranchesPerDepth = 12u;  // sampler branches per bounce
+constexpr uint32_t kDomeBranchBase = 256u;
+constexpr uint32_t kDomeBranchDepth = 512u;
+constexpr uint32_t kDomeBranchLight = 128u;
+constexpr uint32_t kDomeBranchSample = 2u;
+constexpr uint32_t kEmissiveBranchBase = 8192u;
+constexpr uint32_t kEmissiveBranchDepth = 8u;
+constexpr uint32_t kGuideTheta = 8u;
+constexpr uint32_t kGuidePhi = 16u;
+constexpr uint32_t kGuideBins = kGuideTheta * kGuidePhi;
+constexpr float kGuideMix = 0.5f;
+constexpr float kGuideMinTotal = 1.0f;
+
+// ---------------------------------------------------------------------------
+// QMC sampler, fallback branch (OpenQmcSampler.cpp:20-71, 110-135)
+// ---------------------------------------------------------------------------
+constexpr uint32_t kPrimes[24] = {2u,  3u,  5u,  7u,  11u, 13u, 17u, 19u, 23u, 29u, 31u, 37u,
+                                  41u, 43u, 47u, 53u, 59u, 61u, 67u, 71u, 73u, 79u, 83u, 89u};
+constexpr float kR2Step[2] = {0.7548776662466927f, 0.5698402909980532f};
+
+// OpenQmcSampler.cpp:30-37 -- xorshift-multiply integer finaliser.
+inline uint32_t mix32(uint32_t h) {
+    h ^= h >> 16;
+    h *= 0x7feb352du;
+    h ^= h >> 15;
+    h *= 0x846ca68bu;
+    h ^= h >> 16;
+    return h;
+}
+
+// OpenQmcSampler.cpp:39-41 -- low 24 bits of the hash as a float in [0,1).
+inline float hash_to_unit(uint32_t key) {
+    return static_cast<float>(mix32(key) & 0x00ffffffu) / 16777216.0f;
+}
+
+// OpenQmcSampler.cpp:43-54 -- van der Corput digits accumulated in fp32,
+// most significant digit first, the weight multiplied by 1/base each step.
+inline float radical_inverse(uint32_t base, uint32_t n) {
+    const float invBase = 1.0f / static_cast<float>(base);
+    float weight = invBase;
+    float acc = 0.0f;
+    for (; n > 0u; n /= base) {
+        acc += static_cast<float>(n % base) * weight;
+        weight *= invBase;
+    }
+    return acc;
+}
+
+// OpenQmcSampler.cpp:56-71.
+float qmc_value(uint32_t px, uint32_t py, uint32_t s, uint32_t dim) {
+    if (dim < 2u) {
+        const float v = 0.5f + kR2Step[dim] * static_cast<float>(s);
+        return v - std::floor(v);
+    }
+    const float seq = radical_inverse(kPrimes[dim % 24u], s + 1u);
+    const uint32_t key = (px * 0x8da6b343u) ^ (py * 0xd8163841u) ^ (dim * 0xcb1ab31fu) ^ 0x9e3779b9u;
+    const float v = seq + hash_to_unit(key);
+    return v - std::floor(v);
+}
+
+// OpenQmcSampler::Next2D / Next1D: branch b -> dims 2b, 2b+1; Next1D is the
+// first of the pair (OpenQmcSampler.cpp:110-112, 129-134).
+struct Sampler {
+    uint32_t px, py, s;
+    V2 next2(uint32_t b) const { return {qmc_value(px, py, s, 2u * b), qmc_value(px, py, s, 2u * b + 1u)}; }
+    float next1(uint32_t b) const { return next2(b).x; }
+};
+
+// ---------------------------------------------------------------------------
+// Environment map (EnvironmentMap.cpp)
+// ---------------------------------------------------------------------------
+struct Env {
+    uint32_t w = 0, h = 0;
+    uint32_t layout = SHIRO_HIP_ENV_LATLONG;  // Automatic resolves to LatLong (:43-45)
+    std::vector<V3> px;
+    std::vector<float> rowCdf;   // h+1
+    std::vector<float> condCdf;  // h*(w+1)
+};
+
+inline float env_lum(V3 c) { return c.x * 0.2126f + c.y * 0.7152f + c.z * 0.0722f; }
+
+inline float wrap01(float v) {
+    v = std::fmod(v, 1.0f);
+    return v < 0.0f ? v + 1.0f : v;
+}
+
+// EnvironmentMap.cpp:185-236: per row, weights lum*sin(theta_centre) summed
+// left to right, normalised by the row sum (uniform if zero); the row CDF
+// sums the (post-fallback) row weights and is normalised by the total.
+void env_build_tables(Env& e) {
+    e.rowCdf.clear();
+    e.condCdf.clear();
+    if (e.px.empty() || e.w == 0 || e.h == 0 || e.layout != SHIRO_HIP_ENV_LATLONG) return;
+    e.rowCdf.assign(e.h + 1u, 0.0f);
+    e.condCdf.assign(static_cast<size_t>(e.h) * (e.w + 1u), 0.0f);
+    float total = 0.0f;
+    for (uint32_t y = 0; y < e.h; ++y) {
+        const float theta = (static_cast<float>(y) + 0.5f) * kPi / static_cast<float>(e.h);
+        const float st = std::max(std::sin(theta), 1.0e-6f);
+        float* row = &e.condCdf[static_cast<size_t>(y) * (e.w + 1u)];
+        float sum = 0.0f;
+        for (uint32_t x = 0; x < e.w; ++x) {
+            sum += std::max(0.0f, env_lum(e.px[static_cast<size_t>(y) * e.w + x])) * st;
+            row[x + 1u] = sum;
+        }
+        if (sum > 0.0f) {
+            for (uint32_t x = 1; x <= e.w; ++x) row[x] /= sum;
+        } else {
+            for (uint32_t x = 1; x <= e.w; ++x) row[x] = static_cast<float>(x) / static_cast<float>(e.w);
+            sum = 1.0f;
+        }
+        total += sum;
+        e.rowCdf[y + 1u] = total;
+    }
+    if (total <= 0.0f) {
+        for (uint32_t y = 1; y <= e.h; ++y) e.rowCdf[y] = static_cast<float>(y) / static_cast<float>(e.h);
+        return;
+    }
+    for (uint32_t y = 1; y <= e.h; ++y) e.rowCdf[y] /= total;
+}
+
+// EnvironmentMap.cpp:266-300: u wraps, v clamps, half-texel offset.
+V3 env_bilinear(const Env& e, float u, float v) {
+    if (e.px.empty() || e.w == 0 || e.h == 0) return {};
+    u = wrap01(u) * static_cast<float>(e.w) - 0.5f;
+    v = clampf(v, 0.0f, 1.0f) * static_cast<float>(e.h) - 0.5f;
+    const int x0 = static_cast<int>(std::floor(u));
+    const int y0 = static_cast<int>(std::floor(v));
+    const int x1 = x0 + 1;
+    const int y1 = std::min(y0 + 1, static_cast<int>(e.h) - 1);
+    const float fx = u - static_cast<float>(x0);
+    const float fy = v - static_cast<float>(y0);
+    const int W = static_cast<int>(e.w);
+    auto texel = [&](int x, int y) {
+        int xx = x % W;
+        if (xx < 0) xx += W;
+        const int yy = std::clamp(y, 0, static_cast<int>(e.h) - 1);
+        return e.px[static_cast<size_t>(yy) * e.w + static_cast<size_t>(xx)];
+    };
+    const V3 top = lerp(texel(x0, y0), texel(x1, y0), fx);
+    const V3 bot = lerp(texel(x0, y1), texel(x1, y1), fx);
+    return lerp(top, bot, fy);
+}
+
+// EnvironmentMap.cpp:238-245.
+V3 env_eval_latlong(const Env& e, V3 dir) {
+    const V3 d = normalize(dir);
+    const float phi = std::atan2(d.x, d.z);
+    const float theta = std::acos(clampf(d.y, -1.0f, 1.0f));
+    return env_bilinear(e, wrap01(phi / kTwoPi), clampf(theta / kPi, 0.0f, 1.0f));
+}
+
+// EnvironmentMap.cpp:247-264.
+V3 env_eval_angular(const Env& e, V3 dir) {
+    const V3 d = normalize(dir);
+    const float radial = std::sqrt(d.x * d.x + d.y * d.y);
+    if (radial <= 1.0e-6f) return env_bilinear(e, 0.5f, 0.5f);
+    const float r = std::acos(clampf(d.z, -1.0f, 1.0f)) / kPi;
+    const float u = 0.5f + 0.5f * r * (d.x / radial);
+    const float v = 0.5f + 0.5f * r * (d.y / radial);
+    if (u < 0.0f || u > 1.0f || v < 0.0f || v > 1.0f) return {};
+    return env_bilinear(e, u, v);
+}
+
+// EnvironmentMap.cpp:94-107.
+V3 env_eval(const Env& e, V3 dir) {
+    if (e.px.empty()) return {};
+    return e.layout == SHIRO_HIP_ENV_ANGULAR ? env_eval_angular(e, dir) : env_eval_latlong(e, dir);
+}
+
+// EnvironmentMap.cpp:315-330: lower_bound over cdf[begin+1 .. end] of the
+// clamped sample; returns the bucket (relative to begin) and its mass.
+uint32_t env_sample_cdf(const std::vector<float>& cdf, uint32_t begin, uint32_t end, float u, float* mass) {
+    const float key = clampf(u, 0.0f, std::nextafter(1.0f, 0.0f));
+    const auto it = std::lower_bound(cdf.begin() + (begin + 1u), cdf.begin() + (end + 1u), key);
+    const uint32_t idx = static_cast<uint32_t>(it - cdf.begin()) - 1u;
+    *mass = cdf[idx + 1u] - cdf[idx];
+    return idx - begin;
+}
+
+// EnvironmentMap.cpp:302-313.
+float env_latlong_pdf(const Env& e, uint32_t x, uint32_t y, float sinTheta) {
+    if (e.rowCdf.empty() || e.condCdf.empty() || e.w == 0 || e.h == 0 || sinTheta <= 1.0e-6f) return 0.0f;
+    const float rowMass = e.rowCdf[y + 1u] - e.rowCdf[y];
+    const size_t o = static_cast<size_t>(y) * (e.w + 1u);
+    const float colMass = e.condCdf[o + x + 1u] - e.condCdf[o + x];
+    return rowMass * colMass * static_cast<float>(e.w * e.h) / (2.0f * kPi * kPi * sinTheta);
+}
+
+// Uniform sphere direction (EnvironmentMap.cpp:32-41; same formula as
+// CpuPathTracer.cpp:304-313).
+V3 uniform_sphere(V2 u) {
+    const float y = 1.0f - 2.0f * u.y;
+    const float r = std::sqrt(std::max(0.0f, 1.0f - y * y));
+    const float phi = kTwoPi * u.x;
+    return {std::sin(phi) * r, y, std::cos(phi) * r};
+}
+
+// EnvironmentMap.cpp:109-163: row then column from the tables; direction
+// through the pixel centre; pdf from the centre's sin(theta).
+V3 env_sample(const Env& e, V2 u, V3* dirOut, float* pdfOut) {
+    if (e.px.empty()) {
+        *dirOut = {0.0f, 1.0f, 0.0f};
+        *pdfOut = 0.0f;
+        return {};
+    }
+    if (e.layout == SHIRO_HIP_ENV_LATLONG && !e.rowCdf.empty() && e.w > 0 && e.h > 0) {
+        float rowMass = 0.0f, colMass = 0.0f;
+        const uint32_t row = env_sample_cdf(e.rowCdf, 0, e.h, u.y, &rowMass);
+        const uint32_t off = row * (e.w + 1u);
+        const uint32_t col = env_sample_cdf(e.condCdf, off, off + e.w, u.x, &colMass);
+        const float uu = (static_cast<float>(col) + 0.5f) / static_cast<float>(e.w);
+        const float vv = (static_cast<float>(row) + 0.5f) / static_cast<float>(e.h);
+        const float theta = vv * kPi;
+        const float phi = uu * kTwoPi;
+        const float st = std::sin(theta);
+        const V3 d{std::sin(phi) * st, std::cos(theta), std::cos(phi) * st};
+        *dirOut = d;
+        *pdfOut = st > 1.0e-6f ? rowMass * colMass * static_cast<float>(e.w * e.h) / (2.0f * kPi * kPi * st) : 0.0f;
+        return env_eval(e, d);
+    }
+    const V3 d = uniform_sphere(u);
+    *dirOut = d;
+    *pdfOut = kInvFourPi;
+    return env_eval(e, d);
+}
+
+// EnvironmentMap.cpp:165-183: pdf of a direction uses its own sin(theta).
+float env_pdf(const Env& e, V3 dir) {
+    if (e.px.empty()) return 0.0f;
+    if (e.layout != SHIRO_HIP_ENV_LATLONG || e.rowCdf.empty() || e.w == 0 || e.h == 0) return kInvFourPi;
+    const V3 d = normalize(dir);
+    const float phi = std::atan2(d.x, d.z);
+    const float theta = std::acos(clampf(d.y, -1.0f, 1.0f));
+    const float u = wrap01(phi / kTwoPi);
+    const float v = clampf(theta / kPi, 0.0f, 1.0f);
+    const uint32_t x = std::min(static_cast<uint32_t>(u * static_cast<float>(e.w)), e.w - 1u);
+    const uint32_t y = std::min(static_cast<uint32_t>(v * static_cast<float>(e.h)), e.h - 1u);
+    return env_latlong_pdf(e, x, y, std::sin(theta));
+}
+
+// ---------------------------------------------------------------------------
+// Scene
+// ---------------------------------------------------------------------------
+using Material = shiro_hip_material;
+
+struct Mesh {
+    std::vector<V3> pos;
+    std::vector<V3> nrm;
+    std::vector<uint32_t> idx;
+    uint32_t material = 0;
+};
+
+struct Dome {
+    V3 radiance{1, 1, 1}, right{1, 0, 0}, up{0, 1, 0}, forward{0, 0, 1};
+    std::shared_ptr<Env> env;  // null: constant dome
+};
+
+struct DirLight {
+    V3 direction, radiance;
+};
+
+// Emissive triangle list of BuildAccelerationScene (CpuPathTracer.cpp:22-30, 482-522).
+struct EmissiveTri {
+    uint32_t mesh = 0, prim = 0;
+    float area = 0.0f, weight = 0.0f, cumulative = 0.0f;
+    V3 radiance, normal;
+};
+
+struct BvhNode {
+    float lo[3], hi[3];
+    uint32_t left = 0;   // internal: left child (right = left + 1); leaf: first ref
+    uint32_t count = 0;  // 0 = internal
+};
+
+struct Scene {
+    std::vector<Material> materials;
+    std::vector<Mesh> meshes;
+    std::vector<DirLight> dirs;
+    std::vector<Dome> domes;
+    V3 envTop, envBottom;
+    std::vector<EmissiveTri> emissive;
+    float emissiveSum = 0.0f;
+    // BVH over global triangle references (mesh, prim), ordered by scene order.
+    std::vector<uint32_t> refMesh, refPrim;  // per global triangle
+    std::vector<uint32_t> bvhRefs;            // leaf order -> global triangle
+    std::vector<BvhNode> bvh;
+    bool useBvh = true;
+    // Path-guide weights persisting across batches (CpuPathTracer.h:31-33).
+    std::vector<float> guideWeights;
+    std::mutex guideMutex;
+};
+
+Material default_material() {
+    Material m;
+    std::memset(&m, 0, sizeof(m));
+    auto set3 = [](float* d, float a, float b, float c) { d[0] = a; d[1] = b; d[2] = c; };
+    set3(m.baseColor, 0.8f, 0.8f, 0.8f);
+    set3(m.specularColor, 1, 1, 1);
+    set3(m.transmissionColor, 1, 1, 1);
+    set3(m.coatColor, 1, 1, 1);
+    set3(m.subsurfaceColor, 1, 1, 1);
+    set3(m.subsurfaceRadius, 1, 1, 1);
+    set3(m.sheenColor, 1, 1, 1);
+    m.baseWeight = 1.0f;
+    m.specularWeight = 1.0f;
+    m.roughness = 0.5f;
+    m.opacity = 1.0f;
+    m.coatRoughness = 0.03f;
+    m.coatIor = 1.5f;
+    m.sheenRoughness = 0.3f;
+    m.subsurfaceScale = 1.0f;
+    m.ior = 1.5f;
+    m.thinFilmIor = 1.5f;
+    m.baseColorTexture = -1;
+    m.roughnessTexture = -1;
+    return m;
+}
+
+// ---------------------------------------------------------------------------
+// Triangle helpers (CpuPathTracer.cpp:126-253)
+// ---------------------------------------------------------------------------
+V3 tri_geometric_normal(const Mesh& m, uint32_t prim) {
+    const size_t o = static_cast<size_t>(prim) * 3u;
+    if (o + 2u >= m.idx.size()) return {0.0f, 1.0f, 0.0f};
+    const uint32_t a = m.idx[o], b = m.idx[o + 1], c = m.idx[o + 2];
+    if (a >= m.pos.size() || b >= m.pos.size() || c >= m.pos.size()) return {0.0f, 1.0f, 0.0f};
+    return normalize(cross(m.pos[b] - m.pos[a], m.pos[c] - m.pos[a]));
+}
+
+V3 tri_shading_normal(const Mesh& m, uint32_t prim, float b0, float b1, float b2) {
+    const size_t o = static_cast<size_t>(prim) * 3u;
+    if (o + 2u >= m.idx.size()) return {0.0f, 1.0f, 0.0f};
+    const uint32_t a = m.idx[o], b = m.idx[o + 1], c = m.idx[o + 2];
+    if (m.nrm.size() == m.pos.size() && a < m.nrm.size() && b < m.nrm.size() && c < m.nrm.size()) {
+        const V3 n = m.nrm[a] * b0 + m.nrm[b] * b1 + m.nrm[c] * b2;
+        if (length(n) > 0.0f) return normalize(n);
+    }
+    return tri_geometric_normal(m, prim);
+}
+
+bool tri_vertices(const Mesh& m, uint32_t prim, V3* p0, V3* p1, V3* p2) {
+    const size_t o = static_cast<size_t>(prim) * 3u;
+    if (o + 2u >= m.idx.size()) return false;
+    const uint32_t a = m.idx[o], b = m.idx[o + 1], c = m.idx[o + 2];
+    if (a >= m.pos.size() || b >= m.pos.size() || c >= m.pos.size()) return false;
+    *p0 = m.pos[a];
+    *p1 = m.pos[b];
+    *p2 = m.pos[c];
+    return true;
+}
+
+float tri_area(const Mesh& m, uint32_t prim, V3* normal) {
+    V3 p0, p1, p2;
+    if (!tri_vertices(m, prim, &p0, &p1, &p2)) {
+        *normal = {0.0f, 1.0f, 0.0f};
+        return 0.0f;
+    }
+    const V3 av = cross(p1 - p0, p2 - p0);
+    const float twice = length(av);
+    if (twice <= 1.0e-8f) {
+        *normal = {0.0f, 1.0f, 0.0f};
+        return 0.0f;
+    }
+    *normal = av / twice;
+    return twice * 0.5f;
+}
+
+// ---------------------------------------------------------------------------
+// Sampling helpers (CpuPathTracer.cpp:255-384)
+// ---------------------------------------------------------------------------
+// Tangent = normalize(cross(axis, n)), axis z unless |n.z| >= 0.999 (:255-258).
+inline V3 tangent_of(V3 n) {
+    const V3 axis = std::fabs(n.z) < 0.999f ? V3{0.0f, 0.0f, 1.0f} : V3{0.0f, 1.0f, 0.0f};
+    return normalize(cross(axis, n));
+}
+
+V3 to_local(V3 n, V3 d) {
+    const V3 t = tangent_of(n);
+    const V3 b = cross(n, t);
+    return {dot(d, t), dot(d, b), dot(d, n)};
+}
+
+V3 from_local(V3 n, V3 l) {
+    const V3 t = tangent_of(n);
+    const V3 b = cross(n, t);
+    return normalize(t * l.x + b * l.y + n * l.z);
+}
+
+// Malley's method in the tangent frame (:276-286).
+V3 sample_cosine(V3 n, V2 u) {
+    const float phi = 2.0f * kPi * u.x;
+    const float r = std::sqrt(u.y);
+    const float lx = r * std::cos(phi);
+    const float ly = r * std::sin(phi);
+    const float lz = std::sqrt(std::max(0.0f, 1.0f - u.y));
+    const V3 t = tangent_of(n);
+    const V3 b = cross(n, t);
+    return normalize(t * lx + b * ly + n * lz);
+}
+
+// GGX normal-distribution half-vector sampling (:288-302).
+V3 sample_ggx_half(V3 n, V2 u, float roughness) {
+    const float a = std::max(0.02f, roughness * roughness);
+    const float a2 = a * a;
+    const float phi = 2.0f * kPi * u.x;
+    const float ct = std::sqrt(std::max(0.0f, (1.0f - u.y) / std::max(1.0e-6f, 1.0f + (a2 - 1.0f) * u.y)));
+    const float st = std::sqrt(std::max(0.0f, 1.0f - ct * ct));
+    const float lx = st * std::cos(phi);
+    const float ly = st * std::sin(phi);
+    const V3 t = tangent_of(n);
+    const V3 b = cross(n, t);
+    return normalize(t * lx + b * ly + n * ct);
+}
+
+V3 world_to_dome(const Dome& d, V3 v) { return normalize({dot(v, d.right), dot(v, d.up), dot(v, d.forward)}); }
+V3 dome_to_world(const Dome& d, V3 v) { return normalize(d.right * v.x + d.up * v.y + d.forward * v.z); }
+
+V3 dome_radiance(const Dome& d, V3 dir) {
+    if (d.env) return env_eval(*d.env, world_to_dome(d, dir)) * d.radiance;
+    return d.radiance;
+}
+
+// EnvironmentRadiance (:334-345): domes if any, else the sky gradient.
+V3 environment_radiance(const Scene& s, V3 dir) {
+    if (!s.domes.empty()) {
+        V3 sum{};
+        for (const Dome& d : s.domes) sum = sum + dome_radiance(d, dir);
+        return sum;
+    }
+    const float t = clampf(dir.y * 0.5f + 0.5f, 0.0f, 1.0f);
+    return lerp(s.envBottom, s.envTop, t);
+}
+
+float dome_pdf(const Dome& d, V3 dir) {
+    if (d.env) return env_pdf(*d.env, world_to_dome(d, dir));
+    return kInvFourPi;
+}
+
+bool dome_sample(const Dome& d, V2 u, V3* dir, V3* rad, float* pdf) {
+    V3 ld{};
+    float lp = 0.0f;
+    V3 lr{};
+    if (d.env) {
+        lr = env_sample(*d.env, u, &ld, &lp);
+    } else {
+        ld = uniform_sphere(u);
+        lp = kInvFourPi;
+        lr = {1.0f, 1.0f, 1.0f};
+    }
+    if (lp <= 0.0f) return false;
+    *dir = dome_to_world(d, ld);
+    *rad = lr * d.radiance;
+    *pdf = lp;
+    return true;
+}
+
+// ---------------------------------------------------------------------------
+// Intersection
+// ---------------------------------------------------------------------------
+struct Ray {
+    V3 o, d;
+    float tmin = kEps, tmax = FLT_MAX;
+};
+
+struct Hit {
+    bool hit = false;
+    float t = FLT_MAX;
+    V3 p, ng, ns;
+    uint32_t mesh = std::numeric_limits<uint32_t>::max();
+    uint32_t prim = std::numeric_limits<uint32_t>::max();
+    uint32_t material = 0;
+};
+
+// Moller-Trumbore with the reference's acceptance rules (:386-428):
+// |det| < 1e-8 rejects, u in [0,1], v >= 0, u+v <= 1, t in [tmin, tmax].
+inline bool mt_test(const Ray& r, V3 p0, V3 p1, V3 p2, float* t, float* u, float* v) {
+    const V3 e1 = p1 - p0;
+    const V3 e2 = p2 - p0;
+    const V3 pv = cross(r.d, e2);
+    const float det = dot(e1, pv);
+    if (std::fabs(det) < 1.0e-8f) return false;
+    const float inv = 1.0f / det;
+    const V3 tv = r.o - p0;
+    const float uu = dot(tv, pv) * inv;
+    if (uu < 0.0f || uu > 1.0f) return false;
+    const V3 qv = cross(tv, e1);
+    const float vv = dot(r.d, qv) * inv;
+    if (vv < 0.0f || uu + vv > 1.0f) return false;
+    const float tt = dot(e2, qv) * inv;
+    if (tt < r.tmin || tt > r.tmax) return false;
+    *t = tt;
+    *u = uu;
+    *v = vv;
+    return true;
+}
+
+struct Candidate {
+    float t = FLT_MAX, u = 0.0f, v = 0.0f;
+    uint32_t global = std::numeric_limits<uint32_t>::max();
+};
+
+// Fills a Hit from the winning triangle (:446-454).
+bool finish_hit(const Scene& s, const Ray& r, const Candidate& c, Hit* out) {
+    if (c.global == std::numeric_limits<uint32_t>::max()) return false;
+    const uint32_t mi = s.refMesh[c.global], pi = s.refPrim[c.global];
+    const Mesh& m = s.meshes[mi];
+    const V3 p0 = m.pos[m.idx[pi * 3u]], p1 = m.pos[m.idx[pi * 3u + 1u]], p2 = m.pos[m.idx[pi * 3u + 2u]];
+    Hit h;
+    h.hit = true;
+    h.t = c.t;
+    h.p = r.o + r.d * c.t;
+    h.ng = normalize(cross(p1 - p0, p2 - p0));
+    h.ns = tri_shading_normal(m, pi, 1.0f - c.u - c.v, c.u, c.v);
+    h.mesh = mi;
+    h.prim = pi;
+    h.material = m.material;
+    *out = h;
+    return true;
+}
+
+inline void test_global(const Scene& s, const Ray& r, uint32_t g, Candidate* best) {
+    const Mesh& m = s.meshes[s.refMesh[g]];
+    const uint32_t o = s.refPrim[g] * 3u;
+    float t, u, v;
+    if (!mt_test(r, m.pos[m.idx[o]], m.pos[m.idx[o + 1u]], m.pos[m.idx[o + 2u]], &t, &u, &v)) return;
+    // Scene order wins an exact tie, as in the in-order brute-force scan (:443).
+    if (t < best->t || (t == best->t && g < best->global)) {
+        best->t = t;
+        best->u = u;
+        best->v = v;
+        best->global = g;
+    }
+}
+
+inline bool slab(const BvhNode& n, const Ray& r, V3 inv, float tmax, float* tnear) {
+    float t0 = r.tmin, t1 = tmax;
+    const float o[3] = {r.o.x, r.o.y, r.o.z};
+    const float id[3] = {inv.x, inv.y, inv.z};
+    for (int a = 0; a < 3; ++a) {
+        float ta = (n.lo[a] - o[a]) * id[a];
+        float tb = (n.hi[a] - o[a]) * id[a];
+        if (ta > tb) std::swap(ta, tb);
+        // NaN (0 * inf) on an axis the ray is parallel to is ignored.
+        if (ta == ta) t0 = std::max(t0, ta);
+        if (tb == tb) t1 = std::min(t1, tb);
+    }
+    // Small relative slack so that rounding never drops a box whose
+    // triangle test would pass; the triangle test is the one that decides.
+    t1 = t1 * 1.0000005f + 1e-7f;
+    *tnear = t0;
+    return t0 <= t1;
+}
+
+bool intersect_brute(const Scene& s, const Ray& r, Hit* out) {
+    Candidate best;
+    const uint32_t n = static_cast<uint32_t>(s.refMesh.size());
+    for (uint32_t g = 0; g < n; ++g) test_global(s, r, g, &best);
+    return finish_hit(s, r, best, out);
+}
+
+bool intersect_bvh(const Scene& s, const Ray& r, Hit* out) {
+    Candidate best;
+    if (s.bvh.empty()) return false;
+    const V3 inv{1.0f / r.d.x, 1.0f / r.d.y, 1.0f / r.d.z};
+    uint32_t stack[128];
+    int sp = 0;
+    stack[sp++] = 0;
+    while (sp > 0) {
+        const BvhNode& n = s.bvh[stack[--sp]];
+        float tn;
+        if (!slab(n, r, inv, std::min(r.tmax, best.t), &tn)) continue;
+        if (n.count > 0) {
+            for (uint32_t i = 0; i < n.count; ++i) test_global(s, r, s.bvhRefs[n.left + i], &best);
+        } else {
+            stack[sp++] = n.left + 1u;
+            stack[sp++] = n.left;
+        }
+    }
+    return finish_hit(s, r, best, out);
+}
+
+bool intersect(const Scene& s, const Ray& r, Hit* out) {
+    return s.useBvh ? intersect_bvh(s, r, out) : intersect_brute(s, r, out);
+}
+
+// Occluded (:468-475, 697-709): any hit in [kEps, maxDist].
+bool occluded(const Scene& s, V3 o, V3 d, float maxDist = FLT_MAX) {
+    Ray r;
+    r.o = o;
+    r.d = d;
+    r.tmax = maxDist;
+    if (!s.useBvh) {
+        const uint32_t n = static_cast<uint32_t>(s.refMesh.size());
+        for (uint32_t g = 0; g < n; ++g) {
+            const Mesh& m = s.meshes[s.refMesh[g]];
+            const uint32_t io = s.refPrim[g] * 3u;
+            float t, u, v;
+            if (mt_test(r, m.pos[m.idx[io]], m.pos[m.idx[io + 1u]], m.pos[m.idx[io + 2u]], &t, &u, &v)) return true;
+        }
+        return false;
+    }
+    if (s.bvh.empty()) return false;
+    const V3 inv{1.0f / d.x, 1.0f / d.y, 1.0f / d.z};
+    uint32_t stack[128];
+    int sp = 0;
+    stack[sp++] = 0;
+    while (sp > 0) {
+        const BvhNode& n = s.bvh[stack[--sp]];
+        float tn;
+        if (!slab(n, r, inv, r.tmax, &tn)) continue;
+        if (n.count > 0) {
+            for (uint32_t i = 0; i < n.count; ++i) {
+                const uint32_t g = s.bvhRefs[n.left + i];
+                const Mesh& m = s.meshes[s.refMesh[g]];
+                const uint32_t io = s.refPrim[g] * 3u;
+                float t, u, v;
+                if (mt_test(r, m.pos[m.idx[io]], m.pos[m.idx[io + 1u]], m.pos[m.idx[io + 2u]], &t, &u, &v)) return true;
+            }
+        } else {
+            stack[sp++] = n.left + 1u;
+            stack[sp++] = n.left;
+        }
+    }
+    return false;
+}
+
+// Binary BVH, binned SAH (16 bins) on centroids, leaves of <= 4 triangles.
+// Depth is bounded (the traversal stack holds 128 entries).
+struct BuildRef {
+    float lo[3], hi[3], c[3];
+    uint32_t g;
+};
+
+void bvh_build(Scene& s) {
+    s.bvh.clear();
+    s.bvhRefs.clear();
+    const uint32_t n = static_cast<uint32_t>(s.refMesh.size());
+    if (n == 0) return;
+    std::vector<BuildRef> refs(n);
+    for (uint32_t g = 0; g < n; ++g) {
+        const Mesh& m = s.meshes[s.refMesh[g]];
+        const uint32_t o = s.refPrim[g] * 3u;
+        const V3 p[3] = {m.pos[m.idx[o]], m.pos[m.idx[o + 1u]], m.pos[m.idx[o + 2u]]};
+        BuildRef& r = refs[g];
+        r.g = g;
+        for (int a = 0; a < 3; ++a) {
+            const float va = (&p[0].x)[a], vb = (&p[1].x)[a], vc = (&p[2].x)[a];
+            r.lo[a] = std::min(va, std::min(vb, vc));
+            r.hi[a] = std::max(va, std::max(vb, vc));
+            r.c[a] = 0.5f * (r.lo[a] + r.hi[a]);
+        }
+    }
+    struct Task {
+        uint32_t node, begin, end, depth;
+    };
+    s.bvh.reserve(2u * n / 2u + 1u);
+    s.bvh.push_back({});
+    std::vector<Task> tasks{{0u, 0u, n, 0u}};
+    constexpr int kBins = 16;
+    while (!tasks.empty()) {
+        const Task tk = tasks.back();
+        tasks.pop_back();
+        BvhNode node;
+        float clo[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, chi[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+        for (int a = 0; a < 3; ++a) {
+            node.lo[a] = FLT_MAX;
+            node.hi[a] = -FLT_MAX;
+        }
+        for (uint32_t i = tk.begin; i < tk.end; ++i) {
+            for (int a = 0; a < 3; ++a) {
+                node.lo[a] = std::min(node.lo[a], refs[i].lo[a]);
+                node.hi[a] = std::max(node.hi[a], refs[i].hi[a]);
+                clo[a] = std::min(clo[a], refs[i].c[a]);
+                chi[a] = std::max(chi[a], refs[i].c[a]);
+            }
+        }
+        // Pad the box by a few ulps of its coordinates so that rounding in
+        // the slab test never rejects a box whose triangle test would pass.
+        for (int a = 0; a < 3; ++a) {
+            const float pad = (std::fabs(node.lo[a]) + std::fabs(node.hi[a])) * 2.0e-6f + 1.0e-9f;
+            node.lo[a] -= pad;
+            node.hi[a] += pad;
+        }
+        const uint32_t count = tk.end - tk.begin;
+        auto make_leaf = [&]() {
+            node.left = static_cast<uint32_t>(s.bvhRefs.size());
+            node.count = count;
+            for (uint32_t i = tk.begin; i < tk.end; ++i) s.bvhRefs.push_back(refs[i].g);
+            s.bvh[tk.node] = node;
+        };
+        if (count <= 4u) {
+            make_leaf();
+            continue;
+        }
+        int bestAxis = -1;
+        int bestSplit = 0;
+        float bestCost = FLT_MAX;
+        for (int a = 0; a < 3; ++a) {
+            const float ext = chi[a] - clo[a];
+            if (!(ext > 0.0f)) continue;
+            float blo[kBins][3], bhi[kBins][3];
+            uint32_t bcnt[kBins] = {};
+            for (int b = 0; b < kBins; ++b)
+                for (int k = 0; k < 3; ++k) {
+                    blo[b][k] = FLT_MAX;
+                    bhi[b][k] = -FLT_MAX;
+                }
+            for (uint32_t i = tk.begin; i < tk.end; ++i) {
+                int b = static_cast<int>((refs[i].c[a] - clo[a]) / ext * kBins);
+                b = std::clamp(b, 0, kBins - 1);
+                ++bcnt[b];
+                for (int k = 0; k < 3; ++k) {
+                    blo[b][k] = std::min(blo[b][k], refs[i].lo[k]);
+                    bhi[b][k] = std::max(bhi[b][k], refs[i].hi[k]);
+                }
+            }
+            auto area = [](const float* lo, const float* hi) {
+                const float dx = hi[0] - lo[0], dy = hi[1] - lo[1], dz = hi[2] - lo[2];
+                return dx * dy + dy * dz + dz * dx;
+            };
+            float rightArea[kBins];
+            uint32_t rightCnt[kBins];
+            float rlo[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, rhi[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+            uint32_t rc = 0;
+            for (int b = kBins - 1; b > 0; --b) {
+                rc += bcnt[b];
+                for (int k = 0; k < 3; ++k) {
+                    rlo[k] = std::min(rlo[k], blo[b][k]);
+                    rhi[k] = std::max(rhi[k], bhi[b][k]);
+                }
+                rightArea[b] = rc ? area(rlo, rhi) : 0.0f;
+                rightCnt[b] = rc;
+            }
+            float llo[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, lhi[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+            uint32_t lc = 0;
+            for (int b = 0; b < kBins - 1; ++b) {
+                lc += bcnt[b];
+                for (int k = 0; k < 3; ++k) {
+                    llo[k] = std::min(llo[k], blo[b][k]);
+                    lhi[k] = std::max(lhi[k], bhi[b][k]);
+                }
+                if (lc == 0 || rightCnt[b + 1] == 0) continue;
+                const float cost = area(llo, lhi) * lc + rightArea[b + 1] * rightCnt[b + 1];
+                if (cost < bestCost) {
+                    bestCost = cost;
+                    bestAxis = a;
+                    bestSplit = b;
+                }
+            }
+        }
+        uint32_t mid;
+        if (bestAxis < 0 || tk.depth >= 60u) {
+            if (count <= 16u || tk.depth >= 60u) {
+                make_leaf();
+                continue;
+            }
+            // All centroids coincide: split the range in half.
+            mid = tk.begin + count / 2u;
+        } else {
+            const float ext = chi[bestAxis] - clo[bestAxis];
+            auto it = std::partition(refs.begin() + tk.begin, refs.begin() + tk.end, [&](const BuildRef& r) {
+                int b = static_cast<int>((r.c[bestAxis] - clo[bestAxis]) / ext * kBins);
+                b = std::clamp(b, 0, kBins - 1);
+                return b <= bestSplit;
+            });
+            mid = static_cast<uint32_t>(it - refs.begin());
+            if (mid == tk.begin || mid == tk.end) mid = tk.begin + count / 2u;
+        }
+        node.left = static_cast<uint32_t>(s.bvh.size());
+        node.count = 0;
+        s.bvh[tk.node] = node;
+        s.bvh.push_back({});
+        s.bvh.push_back({});
+        tasks.push_back({node.left + 1u, mid, tk.end, tk.depth + 1u});
+        tasks.push_back({node.left, tk.begin, mid, tk.depth + 1u});
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Material model (CpuPathTracer.cpp:841-1441)
+// ---------------------------------------------------------------------------
+inline float sq(float v) { return v * v; }
+inline float luminance(V3 v) { return 0.2126f * v.x + 0.7152f * v.y + 0.0722f * v.z; }
+
+inline V3 safe_normalize(V3 v, V3 fallback) {
+    const float len = length(v);
+    return len > 0.0f ? v / len : fallback;
+}
+
+// Isotropic roughness adjustment used for sampling (:845-850).
+float roughness_sampling(float r, float aniso, float rot) {
+    const float a = clampf(aniso, -0.95f, 0.95f);
+    const float phase = std::cos(rot * 2.0f * kPi);
+    return clampf(r * (1.0f - 0.35f * a * phase), 0.02f, 1.0f);
+}
+
+V3 projected_tangent(const Material& m, V3 n) {
+    if (m.hasTangentOverride) {
+        const V3 t0 = v3(m.tangentOverride);
+        const V3 t = t0 - n * dot(t0, n);
+        if (length(t) > 0.0f) return normalize(t);
+    }
+    return tangent_of(n);
+}
+
+// Azimuth of the half vector in the tangent frame (:862-879).
+float anisotropy_phase(V3 n, V3 t, V3 v, V3 l, float rot) {
+    const V3 td = safe_normalize(t - n * dot(t, n), tangent_of(n));
+    const V3 bt = normalize(cross(n, td));
+    const V3 h = safe_normalize(v + l, n);
+    const V3 hp = h - n * dot(n, h);
+    if (length(hp) <= 1.0e-6f) return std::cos(rot * 2.0f * kPi);
+    const V3 hd = normalize(hp);
+    const float ang = std::atan2(dot(hd, bt), dot(hd, td));
+    return std::cos(ang - rot * 2.0f * kPi);
+}
+
+// Direction-dependent roughness adjustment used for evaluation (:881-893).
+float roughness_eval(float r, float aniso, float rot, V3 n, V3 t, V3 v, V3 l) {
+    const float a = clampf(aniso, -0.95f, 0.95f);
+    const float phase = anisotropy_phase(n, t, v, l, rot);
+    return clampf(r * (1.0f - 0.35f * a * phase), 0.02f, 1.0f);
+}
+
+float f0_scalar(float ior) {
+    const float i = std::max(ior, 1.0f);
+    return sq((i - 1.0f) / (i + 1.0f));
+}
+
+V3 diffuse_color(const Material& m) {
+    const float metal = clampf(m.metallic, 0.0f, 1.0f);
+    const float trans = clampf(m.transmission, 0.0f, 1.0f);
+    const float sss = clampf(m.subsurface, 0.0f, 1.0f);
+    const float blend = clampf(max3(v3(m.subsurfaceRadius)) / 3.0f, 0.0f, 1.0f);
+    const V3 tint = lerp(v3(m.baseColor), v3(m.subsurfaceColor), blend);
+    const V3 base = lerp(v3(m.baseColor), tint, sss);
+    return base * (clampf(m.baseWeight, 0.0f, 1.0f) * (1.0f - metal) * (1.0f - trans));
+}
+
+float base_roughness(const Material& m) {
+    return clampf(m.roughness + m.coatAffectRoughness * m.coatWeight * m.coatRoughness, 0.0f, 1.0f);
+}
+
+V3 specular_f0(const Material& m) {
+    const float metal = clampf(m.metallic, 0.0f, 1.0f);
+    const V3 diel = v3(m.specularColor) * (clampf(m.specularWeight, 0.0f, 1.0f) * f0_scalar(m.ior));
+    return lerp(diel, v3(m.baseColor), metal);
+}
+
+V3 coat_f0(const Material& m) { return v3(m.coatColor) * (clampf(m.coatWeight, 0.0f, 1.0f) * f0_scalar(m.coatIor)); }
+
+float transmission_roughness(const Material& m) { return clampf(m.roughness + m.transmissionExtraRoughness, 0.0f, 1.0f); }
+
+// Three-band interference tint (:934-954).
+V3 thin_film_color(float thickness, float ior, float vDotH) {
+    if (thickness <= 0.0f) return {1.0f, 1.0f, 1.0f};
+    const float i = std::max(ior, 1.0f);
+    const float path = 2.0f * i * thickness * clampf(vDotH, 0.05f, 1.0f);
+    auto band = [&](float wl) { return 0.5f + 0.5f * std::cos(4.0f * kPi * path / std::max(wl, 1.0f)); };
+    const V3 irid{band(650.0f), band(510.0f), band(475.0f)};
+    const float strength = clampf(thickness / 1200.0f, 0.0f, 1.0f);
+    return lerp(V3{1.0f, 1.0f, 1.0f}, (V3{1.0f, 1.0f, 1.0f} + irid) * 0.5f, strength);
+}
+
+V3 apply_thin_film(V3 lobe, const Material& m, V3 n, V3 v, V3 l) {
+    if (m.thinFilmThickness <= 0.0f) return lobe;
+    const V3 h = safe_normalize(v + l, n);
+    return lobe * thin_film_color(m.thinFilmThickness, m.thinFilmIor, clampf(dot(v, h), 0.0f, 1.0f));
+}
+
+V3 resolve_normal(const Material& m, V3 fallback, bool coat) {
+    const bool has = coat ? m.hasCoatNormalOverride : m.hasNormalOverride;
+    if (!has) return fallback;
+    const V3 c = coat ? v3(m.coatNormalOverride) : v3(m.normalOverride);
+    if (length(c) <= 0.0f) return fallback;
+    V3 n = normalize(c);
+    if (dot(n, fallback) < 0.0f) n = n * -1.0f;
+    return n;
+}
+
+float ggx_d(float nDotH, float roughness) {
+    const float a = std::max(0.02f, roughness * roughness);
+    const float a2 = a * a;
+    const float den = std::max(1.0e-6f, sq(nDotH * nDotH * (a2 - 1.0f) + 1.0f));
+    return a2 / (kPi * den);
+}
+
+float ggx_g2(float nDotV, float nDotL, float roughness) {
+    const float a = std::max(0.02f, roughness * roughness);
+    const float a2 = a * a;
+    auto lambda = [&](float c) {
+        const float cc = std::max(c, 1.0e-6f);
+        return (-1.0f + std::sqrt(1.0f + a2 * (1.0f - cc * cc) / (cc * cc))) * 0.5f;
+    };
+    return 1.0f / (1.0f + lambda(nDotV) + lambda(nDotL));
+}
+
+V3 schlick(V3 f0, float c) {
+    const float x = clampf(1.0f - c, 0.0f, 1.0f);
+    const float x5 = x * x * x * x * x;
+    return f0 + (V3{1.0f, 1.0f, 1.0f} - f0) * x5;
+}
+
+// Unpolarised dielectric Fresnel with side swap and TIR (:1011-1033).
+float fresnel_dielectric(float cosI, float etaI, float etaT) {
+    float c = clampf(cosI, -1.0f, 1.0f);
+    float ei = std::max(etaI, 1.0e-6f);
+    float et = std::max(etaT, 1.0e-6f);
+    if (c <= 0.0f) {
+        std::swap(ei, et);
+        c = std::fabs(c);
+    }
+    const float si = std::sqrt(std::max(0.0f, 1.0f - c * c));
+    const float stt = ei / et * si;
+    if (stt >= 1.0f) return 1.0f;
+    const float ct = std::sqrt(std::max(0.0f, 1.0f - stt * stt));
+    const float rpa = ((et * c) - (ei * ct)) / std::max((et * c) + (ei * ct), 1.0e-6f);
+    const float rpe = ((ei * c) - (et * ct)) / std::max((ei * c) + (et * ct), 1.0e-6f);
+    return 0.5f * (rpa * rpa + rpe * rpe);
+}
+
+V3 one_minus_clamped(V3 v) {
+    return {clampf(1.0f - v.x, 0.0f, 1.0f), clampf(1.0f - v.y, 0.0f, 1.0f), clampf(1.0f - v.z, 0.0f, 1.0f)};
+}
+
+V3 coat_tint(const Material& m) {
+    const float k = clampf(m.coatAffectColor * m.coatWeight, 0.0f, 1.0f);
+    return lerp(V3{1.0f, 1.0f, 1.0f}, clamp3(v3(m.coatColor), 0.0f, 1.0f), k);
+}
+
+V3 layer_transmittance(V3 f0, float nDotV, float nDotL) {
+    const V3 fv = schlick(f0, clampf(nDotV, 0.0f, 1.0f));
+    const V3 fl = schlick(f0, clampf(nDotL, 0.0f, 1.0f));
+    return one_minus_clamped((fv + fl) * 0.5f);
+}
+
+V3 interface_transmittance(V3 f0, float ca, float cb) {
+    const V3 fa = schlick(f0, clampf(std::fabs(ca), 0.0f, 1.0f));
+    const V3 fb = schlick(f0, clampf(std::fabs(cb), 0.0f, 1.0f));
+    return one_minus_clamped((fa + fb) * 0.5f);
+}
+
+V3 coat_under_transmittance(const Material& m, V3 cn, V3 v, V3 l) {
+    return coat_tint(m) * layer_transmittance(coat_f0(m), dot(cn, v), dot(cn, l));
+}
+
+V3 coat_transmission_scale(const Material& m, V3 cn, V3 v, V3 l) {
+    return coat_tint(m) * interface_transmittance(coat_f0(m), dot(cn, v), dot(cn, l));
+}
+
+// Microfacet reflection F*D*G/(4 n.v n.l) (:1096-1115).
+V3 specular_lobe(V3 n, V3 v, V3 l, V3 f0, float r) {
+    const float nv = clampf(dot(n, v), 0.0f, 1.0f);
+    const float nl = clampf(dot(n, l), 0.0f, 1.0f);
+    if (nv <= 0.0f || nl <= 0.0f) return {};
+    const V3 h = safe_normalize(v + l, n);
+    const float nh = clampf(dot(n, h), 0.0f, 1.0f);
+    const float vh = clampf(dot(v, h), 0.0f, 1.0f);
+    const float D = ggx_d(nh, r);
+    const float G = ggx_g2(nv, nl, r);
+    return schlick(f0, vh) * (D * G / std::max(4.0f * nv * nl, 1.0e-6f));
+}
+
+// D(h) n.h / (4 v.h) (:1117-1132).
+float specular_pdf(V3 n, V3 v, V3 l, float r) {
+    const float nv = clampf(dot(n, v), 0.0f, 1.0f);
+    const float nl = clampf(dot(n, l), 0.0f, 1.0f);
+    if (nv <= 0.0f || nl <= 0.0f) return 0.0f;
+    const V3 h = safe_normalize(v + l, n);
+    const float nh = clampf(dot(n, h), 0.0f, 1.0f);
+    const float vh = clampf(dot(v, h), 1.0e-6f, 1.0f);
+    return ggx_d(nh, r) * nh / std::max(4.0f * vh, 1.0e-6f);
+}
+
+V3 transmission_tint(const Material& m) {
+    if (!m.thinWalled && m.transmissionDepth > 0.0f) return {1.0f, 1.0f, 1.0f};
+    return clamp3(v3(m.transmissionColor), 0.0f, 1.0f);
+}
+
+// Rough dielectric transmission (Walter et al. 2007 form) (:1136-1182).
+V3 transmission_lobe(const Material& m, V3 n, V3 cn, V3 v, V3 l, float etaI, float etaT, float r) {
+    const float cwo = dot(n, v);
+    const float cwi = dot(n, l);
+    const float acwo = std::fabs(cwo), acwi = std::fabs(cwi);
+    if (acwo <= 0.0f || acwi <= 0.0f || cwo * cwi >= 0.0f) return {};
+    const float eta = etaT / std::max(etaI, 1.0e-6f);
+    V3 h = safe_normalize(v + l * eta, n);
+    if (dot(n, h) < 0.0f) h = h * -1.0f;
+    const float oh = dot(v, h);
+    const float ih = dot(l, h);
+    if (oh * ih >= 0.0f) return {};
+    const float nh = clampf(dot(n, h), 0.0f, 1.0f);
+    const float sden = oh + eta * ih;
+    if (nh <= 0.0f || std::fabs(sden) <= 1.0e-6f) return {};
+    const float D = ggx_d(nh, r);
+    const float G = ggx_g2(acwo, acwi, r);
+    const float F = 1.0f - fresnel_dielectric(std::fabs(oh), etaI, etaT);
+    const V3 scale = transmission_tint(m) * coat_transmission_scale(m, cn, v, l) *
+                     (clampf(m.transmission, 0.0f, 1.0f) * (1.0f - clampf(m.metallic, 0.0f, 1.0f)));
+    const float factor = std::fabs(D * G * std::fabs(ih) * std::fabs(oh) / std::max(acwo * acwi * sq(sden), 1.0e-6f));
+    const float transport = 1.0f / std::max(eta * eta, 1.0e-6f);
+    return scale * F * (factor * transport);
+}
+
+float transmission_pdf(V3 n, V3 v, V3 l, float eta, float r) {
+    const float cwo = dot(n, v);
+    const float cwi = dot(n, l);
+    if (std::fabs(cwo) <= 0.0f || std::fabs(cwi) <= 0.0f || cwo * cwi >= 0.0f) return 0.0f;
+    V3 h = safe_normalize(v + l * eta, n);
+    if (dot(n, h) < 0.0f) h = h * -1.0f;
+    const float oh = dot(v, h);
+    const float ih = dot(l, h);
+    const float sden = oh + eta * ih;
+    if (oh * ih >= 0.0f || std::fabs(sden) <= 1.0e-6f) return 0.0f;
+    const float nh = clampf(dot(n, h), 0.0f, 1.0f);
+    const float pwh = ggx_d(nh, r) * nh;
+    const float jac = std::fabs((eta * eta * ih) / sq(sden));
+    return pwh * jac;
+}
+
+float power_heuristic(float a, float b) {
+    const float a2 = a * a, b2 = b * b;
+    return a2 / std::max(a2 + b2, 1.0e-6f);
+}
+
+V3 clamp_max_component(V3 v, float limit) {
+    const float c = max3(v);
+    if (c <= limit || c <= 0.0f) return v;
+    return v * (limit / c);
+}
+
+// Lobe selection weights (:1376-1403).
+float w_diffuse(const Material& m) {
+    const V3 uc = coat_tint(m) * one_minus_clamped(coat_f0(m));
+    const V3 ub = one_minus_clamped(specular_f0(m));
+    return std::max(0.0f, max3(diffuse_color(m) * uc * ub));
+}
+
+float w_base_specular(const Material& m) {
+    const V3 uc = coat_tint(m) * one_minus_clamped(coat_f0(m));
+    return std::max(0.0f, max3(uc * specular_f0(m)));
+}
+
+float w_coat(const Material& m) { return std::max(0.0f, max3(coat_f0(m))); }
+
+float w_transmission(const Material& m) {
+    const float tr = clampf(m.transmission, 0.0f, 1.0f);
+    const float metal = clampf(m.metallic, 0.0f, 1.0f);
+    if (tr <= 0.0f || metal >= 0.999f) return 0.0f;
+    const V3 uc = coat_tint(m) * one_minus_clamped(coat_f0(m));
+    const V3 ub = one_minus_clamped(specular_f0(m));
+    const V3 tint = clamp3(v3(m.transmissionColor), 0.0f, 1.0f);
+    return std::max(0.0f, max3(tint * uc * ub)) * tr * (1.0f - metal);
+}
+
+// Reflective mixture pdf (:1232-1273).
+float bsdf_pdf(const Material& m, V3 n, V3 cn, V3 v, V3 l) {
+    const float nl = clampf(dot(n, l), 0.0f, 1.0f);
+    if (nl <= 0.0f) return 0.0f;
+    const float wd = w_diffuse(m), ws = w_base_specular(m), wc = w_coat(m);
+    const float wr = wd + ws + wc;
+    if (wr <= 0.0f) return 0.0f;
+    const float pd = wd / wr, ps = ws / wr, pc = wc / wr;
+    const float rs =
+        roughness_eval(base_roughness(m), m.specularAnisotropy, m.specularRotation, n, projected_tangent(m, n), v, l);
+    const float rc = roughness_eval(m.coatRoughness, m.coatAnisotropy, m.coatRotation, cn, projected_tangent(m, cn), v, l);
+    return pd * nl * kInvPi + ps * specular_pdf(n, v, l, rs) + pc * specular_pdf(cn, v, l, rc);
+}
+
+// Layered BSDF value (:1275-1354).
+V3 bsdf_eval(const Material& m, V3 n, V3 cn, V3 v, V3 l) {
+    const float nv = clampf(dot(n, v), 0.0f, 1.0f);
+    const float nl = clampf(dot(n, l), 0.0f, 1.0f);
+    if (nv <= 0.0f || nl <= 0.0f) return {};
+    const V3 f0b = specular_f0(m);
+    const V3 f0c = coat_f0(m);
+    const V3 underCoat = coat_under_transmittance(m, cn, v, l);
+    const V3 underBase = layer_transmittance(f0b, nv, nl);
+    const V3 t = projected_tangent(m, n);
+    const V3 tc = projected_tangent(m, cn);
+    const V3 vp = normalize(v - n * nv);
+    const V3 lp = normalize(l - n * nl);
+    const float cosPhi = length(vp) > 0.0f && length(lp) > 0.0f ? clampf(dot(vp, lp), -1.0f, 1.0f) : 0.0f;
+    const float sg = clampf(m.diffuseRoughness, 0.0f, 1.0f);
+    const float sg2 = sg * sg;
+    const float A = 1.0f - (sg2 / (2.0f * (sg2 + 0.33f)));
+    const float B = 0.45f * sg2 / (sg2 + 0.09f);
+    const float sinAlpha = std::max(std::sqrt(std::max(0.0f, 1.0f - nv * nv)), std::sqrt(std::max(0.0f, 1.0f - nl * nl)));
+    const float tanBeta = std::min(std::sqrt(std::max(0.0f, 1.0f - nv * nv)) / std::max(nv, 1.0e-6f),
+                                   std::sqrt(std::max(0.0f, 1.0f - nl * nl)) / std::max(nl, 1.0e-6f));
+    const float wrap = clampf(m.subsurface * m.subsurfaceScale *
+                                  (0.35f + 0.15f * clampf(m.subsurfaceAnisotropy, -1.0f, 1.0f) +
+                                   0.1f * clampf(max3(v3(m.subsurfaceRadius)) / 3.0f, 0.0f, 1.0f)),
+                              0.0f, 0.9f);
+    const float wnl = clampf((nl + wrap) / (1.0f + wrap), 0.0f, 1.0f);
+    const float dterm = (A + B * std::max(0.0f, cosPhi) * sinAlpha * tanBeta) * (wnl / std::max(nl, 1.0e-6f));
+    const V3 diffuse = diffuse_color(m) * underCoat * underBase * (kInvPi * dterm);
+    const float rs = roughness_eval(base_roughness(m), m.specularAnisotropy, m.specularRotation, n, t, v, l);
+    const V3 spec = underCoat * apply_thin_film(specular_lobe(n, v, l, f0b, rs), m, n, v, l);
+    const float rc = roughness_eval(m.coatRoughness, m.coatAnisotropy, m.coatRotation, cn, tc, v, l);
+    const V3 coat = apply_thin_film(specular_lobe(cn, v, l, f0c, rc), m, cn, v, l);
+    const V3 h = safe_normalize(v + l, n);
+    const float sheenW = clampf(m.sheen, 0.0f, 1.0f);
+    const float lh = clampf(dot(l, h), 0.0f, 1.0f);
+    const float sheenF = std::pow(1.0f - lh, 5.0f) * (1.0f - 0.5f * clampf(m.sheenRoughness, 0.0f, 1.0f));
+    const V3 sheen = v3(m.sheenColor) * sheenW * sheenF * underCoat * underBase;
+    return diffuse + spec + coat + sheen;
+}
+
+// Snell refraction, false on total internal reflection (:1356-1374).
+bool refract_dir(V3 i, V3 n, float eta, V3* out) {
+    const float c = clampf(-dot(i, n), -1.0f, 1.0f);
+    const float s2 = std::max(0.0f, 1.0f - c * c);
+    const float k = 1.0f - eta * eta * s2;
+    if (k <= 0.0f) return false;
+    *out = normalize(i * eta + n * (eta * c - std::sqrt(k)));
+    return true;
+}
+
+float pane_transmittance(float R) {
+    const float r = clampf(R, 0.0f, 1.0f);
+    return clampf((1.0f - r) / std::max(1.0f + r, 1.0e-6f), 0.0f, 1.0f);
+}
+
+// Beer-Lambert extinction of the interior medium (:1420-1437).
+V3 medium_extinction(const Material& m) {
+    V3 e{std::max(m.transmissionScatter[0], 0.0f), std::max(m.transmissionScatter[1], 0.0f),
+         std::max(m.transmissionScatter[2], 0.0f)};
+    if (m.thinWalled || m.transmissionDepth <= 0.0f) return e;
+    const float depth = std::max(m.transmissionDepth, 1.0e-3f);
+    const V3 tint = clamp3(v3(m.transmissionColor), 1.0e-4f, 1.0f);
+    e.x += -std::log(tint.x) / depth;
+    e.y += -std::log(tint.y) / depth;
+    e.z += -std::log(tint.z) / depth;
+    return e;
+}
+
+V3 medium_attenuation(V3 ext, float dist) {
+    return {std::exp(-ext.x * dist), std::exp(-ext.y * dist), std::exp(-ext.z * dist)};
+}
+
+// ---------------------------------------------------------------------------
+// Emissive triangles (CpuPathTracer.cpp:1443-1495)
+// ---------------------------------------------------------------------------
+const EmissiveTri* pick_emissive(const Scene& s, float u) {
+    if (s.emissive.empty() || s.emissiveSum <= 0.0f) return nullptr;
+    const float target = clampf(u, 0.0f, 0.99999994f) * s.emissiveSum;
+    auto it = std::lower_bound(s.emissive.begin(), s.emissive.end(), target,
+                               [](const EmissiveTri& e, float w) { return e.cumulative < w; });
+    if (it == s.emissive.end()) return &s.emissive.back();
+    return &*it;
+}
+
+const EmissiveTri* find_emissive(const Scene& s, uint32_t mesh, uint32_t prim) {
+    for (const EmissiveTri& e : s.emissive)
+        if (e.mesh == mesh && e.prim == prim) return &e;
+    return nullptr;
+}
+
+float emissive_pdf(const Scene& s, uint32_t mesh, uint32_t prim, V3 origin, V3 pos) {
+    const EmissiveTri* e = find_emissive(s, mesh, prim);
+    if (!e || e->area <= 0.0f || s.emissiveSum <= 0.0f) return 0.0f;
+    const V3 d = pos - origin;
+    const float d2 = dot(d, d);
+    if (d2 <= 1.0e-10f) return 0.0f;
+    const V3 ld = normalize(d);
+    const float lc = std::max(std::fabs(dot(e->normal, ld * -1.0f)), 1.0e-6f);
+    const float sel = e->weight / s.emissiveSum;
+    return sel / e->area * d2 / lc;
+}
+
+float env_pdf_sq_sum(const Scene& s, V3 dir) {
+    float acc = 0.0f;
+    for (const Dome& d : s.domes) {
+        const float p = dome_pdf(d, dir);
+        acc += p * p;
+    }
+    return acc;
+}
+
+float power_heuristic_many(float bsdfPdf, float sqSum) {
+    const float b2 = bsdfPdf * bsdfPdf;
+    return b2 / std::max(b2 + sqSum, 1.0e-6f);
+}
+
+// ---------------------------------------------------------------------------
+// Path guide (CpuPathTracer.cpp:719-834)
+// ---------------------------------------------------------------------------
+struct Guide {
+    std::array<float, kGuideBins> mass{}, cdf{};
+    float total = 0.0f;
+};
+
+struct GuideRecord {
+    uint32_t bin;
+    V3 throughput;
+};
+
+float guide_bin_solid_angle(uint32_t ti) {
+    const float t0 = (static_cast<float>(ti) / static_cast<float>(kGuideTheta)) * (0.5f * kPi);
+    const float t1 = (static_cast<float>(ti + 1u) / static_cast<float>(kGuideTheta)) * (0.5f * kPi);
+    const float dphi = (2.0f * kPi) / static_cast<float>(kGuidePhi);
+    return (std::cos(t0) - std::cos(t1)) * dphi;
+}
+
+uint32_t guide_bin(V3 n, V3 d) {
+    const V3 l = to_local(n, d);
+    const float z = clampf(l.z, 0.0f, 1.0f);
+    const float theta = std::acos(z);
+    float phi = std::atan2(l.y, l.x);
+    if (phi < 0.0f) phi += 2.0f * kPi;
+    const uint32_t ti = std::min(kGuideTheta - 1u, static_cast<uint32_t>(theta / (0.5f * kPi) * static_cast<float>(kGuideTheta)));
+    const uint32_t pi = std::min(kGuidePhi - 1u, static_cast<uint32_t>(phi / (2.0f * kPi) * static_cast<float>(kGuidePhi)));
+    return ti * kGuidePhi + pi;
+}
+
+Guide guide_snapshot(const std::vector<float>& w) {
+    Guide g;
+    if (w.size() != kGuideBins) return g;
+    float c = 0.0f;
+    for (uint32_t b = 0; b < kGuideBins; ++b) {
+        const float m = std::max(w[b], 0.0f);
+        g.mass[b] = m;
+        c += m;
+        g.cdf[b] = c;
+    }
+    g.total = c;
+    return g;
+}
+
+float guide_pdf(const Guide& g, V3 n, V3 d, uint32_t* bin) {
+    if (g.total <= kGuideMinTotal) {
+        *bin = 0;
+        return 0.0f;
+    }
+    const uint32_t b = guide_bin(n, d);
+    *bin = b;
+    const float m = g.mass[b];
+    if (m <= 0.0f) return 0.0f;
+    return (m / g.total) / std::max(guide_bin_solid_angle(b / kGuidePhi), 1.0e-6f);
+}
+
+bool guide_sample(const Guide& g, V3 n, float sel, V2 shape, V3* dir, float* pdf, uint32_t* bin) {
+    if (g.total <= kGuideMinTotal) return false;
+    const float target = std::max(clampf(sel, 0.0f, 0.99999994f) * g.total, 1.0e-6f);
+    const auto it = std::lower_bound(g.cdf.begin(), g.cdf.end(), target);
+    const uint32_t b = it != g.cdf.end() ? static_cast<uint32_t>(it - g.cdf.begin()) : (kGuideBins - 1u);
+    const uint32_t ti = b / kGuidePhi, pi = b % kGuidePhi;
+    const float t0 = (static_cast<float>(ti) / static_cast<float>(kGuideTheta)) * (0.5f * kPi);
+    const float t1 = (static_cast<float>(ti + 1u) / static_cast<float>(kGuideTheta)) * (0.5f * kPi);
+    const float c0 = std::cos(t0), c1 = std::cos(t1);
+    const float ct = c0 + (c1 - c0) * clampf(shape.x, 0.0f, 1.0f);
+    const float st = std::sqrt(std::max(0.0f, 1.0f - ct * ct));
+    const float phi0 = (static_cast<float>(pi) / static_cast<float>(kGuidePhi)) * (2.0f * kPi);
+    const float dphi = (2.0f * kPi) / static_cast<float>(kGuidePhi);
+    const float phi = phi0 + clampf(shape.y, 0.0f, 1.0f) * dphi;
+    *dir = from_local(n, {std::cos(phi) * st, std::sin(phi) * st, ct});
+    *pdf = (g.mass[b] / g.total) / std::max(guide_bin_solid_angle(ti), 1.0e-6f);
+    *bin = b;
+    return true;
+}
+
+void guide_accumulate(const std::vector<GuideRecord>& recs, V3 delta, std::array<float, kGuideBins>* upd) {
+    if (!upd) return;
+    for (const GuideRecord& r : recs) {
+        if (r.bin >= kGuideBins) continue;
+        (*upd)[r.bin] += std::max(0.0f, luminance(r.throughput * delta));
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Direct lighting (CpuPathTracer.cpp:1511-1631)
+// ---------------------------------------------------------------------------
+struct Settings {
+    uint32_t width, height, domeLightSamples, maxDepth, diffuseDepth, specularDepth, threadLimit;
+    bool backgroundVisible;
+    uint32_t rrDepth;
+};
+
+V3 direct_lighting(const Scene& s, const Settings& st, const Hit& hit, V3 n, V3 cn, const Material& m, V3 v,
+                   const Sampler& smp, uint32_t depth) {
+    V3 L{};
+    const V3 origin = hit.p + n * kEps;
+    for (const DirLight& dl : s.dirs) {
+        const V3 ld = normalize(dl.direction * -1.0f);
+        const float nl = clampf(dot(n, ld), 0.0f, 1.0f);
+        if (nl <= 0.0f || occluded(s, origin, ld)) continue;
+        L = L + clamp_max_component(bsdf_eval(m, n, cn, v, ld) * dl.radiance * nl, kDirectClamp);
+    }
+    const uint32_t K = std::max(1u, st.domeLightSamples);
+    for (size_t li = 0; li < s.domes.size(); ++li) {
+        V3 acc{};
+        for (uint32_t k = 0; k < K; ++k) {
+            V3 ld{}, lr{};
+            float lp = 0.0f;
+            const uint32_t branch =
+                kDomeBranchBase + depth * kDomeBranchDepth + static_cast<uint32_t>(li) * kDomeBranchLight + k * kDomeBranchSample;
+            if (!dome_sample(s.domes[li], smp.next2(branch), &ld, &lr, &lp) || lp <= 0.0f) continue;
+            const float nl = clampf(dot(n, ld), 0.0f, 1.0f);
+            if (nl <= 0.0f || occluded(s, origin, ld)) continue;
+            const float bp = bsdf_pdf(m, n, cn, v, ld);
+            const float w = power_heuristic(lp, bp);
+            acc = acc + clamp_max_component(bsdf_eval(m, n, cn, v, ld) * lr * (nl * w / lp), kDirectClamp);
+        }
+        L = L + acc / static_cast<float>(K);
+    }
+    if (!s.emissive.empty() && s.emissiveSum > 0.0f) {
+        const uint32_t base = kEmissiveBranchBase + depth * kEmissiveBranchDepth;
+        const EmissiveTri* e = pick_emissive(s, smp.next1(base));
+        if (e && !(e->mesh == hit.mesh && e->prim == hit.prim) && e->mesh < s.meshes.size()) {
+            const Mesh& em = s.meshes[e->mesh];
+            const V2 u = smp.next2(base + 1u);
+            const float su = std::sqrt(clampf(u.x, 0.0f, 1.0f));
+            const float b0 = 1.0f - su;
+            const float b1 = clampf(u.y, 0.0f, 1.0f) * su;
+            const float b2 = 1.0f - b0 - b1;
+            V3 p0, p1, p2, lpos{};
+            if (tri_vertices(em, e->prim, &p0, &p1, &p2)) lpos = p0 * b0 + p1 * b1 + p2 * b2;
+            const V3 tl = lpos - hit.p;
+            const float d2 = dot(tl, tl);
+            if (d2 > 1.0e-10f) {
+                const float dist = std::sqrt(d2);
+                const V3 ld = tl / dist;
+                const float nl = clampf(dot(n, ld), 0.0f, 1.0f);
+                const float lc = std::fabs(dot(e->normal, ld * -1.0f));
+                if (nl > 0.0f && lc > 0.0f && !occluded(s, origin, ld, std::max(dist - 2.0f * kEps, kEps))) {
+                    const float sel = e->weight / s.emissiveSum;
+                    const float pa = sel / std::max(e->area, 1.0e-6f);
+                    const float lp = pa * d2 / std::max(lc, 1.0e-6f);
+                    if (lp > 0.0f) {
+                        const float bp = bsdf_pdf(m, n, cn, v, ld);
+                        const float w = power_heuristic(lp, bp);
+                        L = L + clamp_max_component(bsdf_eval(m, n, cn, v, ld) * e->radiance * (nl * w / lp), kDirectClamp);
+                    }
+                }
+            }
+        }
+    }
+    return L;
+}
+
+// ---------------------------------------------------------------------------
+// Camera and integrator (CpuPathTracer.cpp:1633-2017)
+// ---------------------------------------------------------------------------
+struct Camera {
+    V3 pos, fwd, right, up;
+    float vfov, aspect;
+};
+
+Ray camera_ray(const Camera& c, const Settings& st, uint32_t x, uint32_t y, V2 j) {
+    const float fx = ((static_cast<float>(x) + j.x) / static_cast<float>(st.width)) * 2.0f - 1.0f;
+    const float fy = ((static_cast<float>(y) + j.y) / static_cast<float>(st.height)) * 2.0f - 1.0f;
+    const float th = std::tan(c.vfov * 0.5f * kPi / 180.0f);
+    const V3 d = c.fwd + c.right * (fx * c.aspect * th) + c.up * (fy * th);
+    Ray r;
+    r.o = c.pos;
+    r.d = normalize(d);
+    return r;
+}
+
+struct PathResult {
+    V3 radiance, albedo, normal;
+    float depth = std::numeric_limits<float>::infinity();
+};
+
+enum Bounce { kNone, kDiffuse, kReflective, kTransmission };
+
+PathResult trace_path(const Scene& s, const Settings& st, Ray ray, const Sampler& smp, const Guide* guide,
+                      std::array<float, kGuideBins>* upd) {
+    PathResult res;
+    V3 T{1.0f, 1.0f, 1.0f};
+    uint32_t nDiffuse = 0, nSpecular = 0;
+    Bounce prevType = kNone;
+    bool prevDelta = true;
+    float prevPdf = 0.0f;
+    bool inside = false, mediumOn = false;
+    V3 ext{};
+    std::vector<GuideRecord> recs;
+    for (uint32_t depth = 0; depth < st.maxDepth; ++depth) {
+        const uint32_t b0 = depth * kBranchesPerDepth;
+        Hit hit;
+        if (!intersect(s, ray, &hit)) {
+            if (depth > 0 || st.backgroundVisible) {
+                const V3 Le = environment_radiance(s, ray.d);
+                V3 dL;
+                if (s.domes.empty() || depth == 0 || prevDelta || prevType == kTransmission) {
+                    dL = T * Le;
+                } else {
+                    dL = T * Le * power_heuristic_many(prevPdf, env_pdf_sq_sum(s, ray.d));
+                }
+                guide_accumulate(recs, dL, upd);
+                res.radiance = res.radiance + dL;
+            }
+            break;
+        }
+        if (mediumOn) T = T * medium_attenuation(ext, hit.t);
+        const Material m = hit.material < s.materials.size() ? s.materials[hit.material] : default_material();
+        const bool front = dot(hit.ng, ray.d) < 0.0f;
+        const V3 ng = front ? hit.ng : hit.ng * -1.0f;
+        V3 nbase = front ? hit.ns : hit.ns * -1.0f;
+        if (dot(nbase, ng) < 0.0f) nbase = nbase * -1.0f;
+        V3 n = resolve_normal(m, nbase, false);
+        if (dot(n, ng) < 0.0f) n = n * -1.0f;
+        const V3 cn = resolve_normal(m, n, true);
+        const V3 v = ray.d * -1.0f;
+        const float opacity = clampf(m.opacity, 0.0f, 1.0f);
+        if (opacity < 0.999f) {
+            if (smp.next1(b0) > opacity) {
+                ray.o = hit.p + ray.d * kEps;
+                continue;
+            }
+            T = T / std::max(opacity, 1.0e-4f);
+        }
+        if (depth == 0) {
+            res.albedo = diffuse_color(m);
+            res.normal = n * 0.5f + V3{0.5f, 0.5f, 0.5f};
+            res.depth = hit.t;
+        }
+        if (m.emissionStrength > 0.0f) {
+            const V3 Le{std::max(m.emissionColor[0] * m.emissionStrength, 0.0f),
+                        std::max(m.emissionColor[1] * m.emissionStrength, 0.0f),
+                        std::max(m.emissionColor[2] * m.emissionStrength, 0.0f)};
+            V3 dL;
+            if (depth == 0 || prevDelta || prevType == kTransmission) {
+                dL = T * Le;
+            } else {
+                const float lp = emissive_pdf(s, hit.mesh, hit.prim, ray.o, hit.p);
+                dL = T * Le * (lp > 0.0f ? power_heuristic(prevPdf, lp) : 1.0f);
+            }
+            guide_accumulate(recs, dL, upd);
+            res.radiance = res.radiance + dL;
+        }
+        const V3 direct = T * direct_lighting(s, st, hit, n, cn, m, v, smp, depth);
+        guide_accumulate(recs, direct, upd);
+        res.radiance = res.radiance + direct;
+
+        const float wd = w_diffuse(m);
+        float ws = w_base_specular(m);
+        const float wc = w_coat(m);
+        const float wtRaw = w_transmission(m);
+        float wt = wtRaw;
+        if (wtRaw > 0.0f) {
+            const float ei = inside ? std::max(m.ior, 1.0f) : 1.0f;
+            const float et = inside ? 1.0f : std::max(m.ior, 1.0f);
+            const float R = fresnel_dielectric(std::fabs(dot(ng, v)), ei, et);
+            const float pass = m.thinWalled ? pane_transmittance(R) : (1.0f - R);
+            wt = wtRaw * pass;
+            ws += wtRaw * (1.0f - pass);
+        }
+        const float wr = wd + ws + wc;
+        const float wsum = wr + wt;
+        if (wsum <= 0.0f) break;
+        const float sel = smp.next1(b0 + 1u) * wsum;
+        const V2 du = smp.next2(b0 + 2u);
+
+        V3 next{}, f{};
+        float pdf = 1.0f;
+        Bounce type = kDiffuse;
+        bool delta = false, pickT = false, pickS = false, pickC = false;
+        if (sel < wt) {
+            pickT = true;
+        } else {
+            const float rs = sel - wt;
+            if (rs < wd) {
+            } else if (rs < wd + ws) {
+                pickS = true;
+            } else {
+                pickC = true;
+            }
+        }
+
+        if (pickT) {
+            if (nSpecular >= st.specularDepth) break;
+            ++nSpecular;
+            type = kTransmission;
+            const float pT = std::max(wt / wsum, 1.0e-6f);
+            const float rT = m.thinWalled ? 0.0f
+                                          : roughness_sampling(transmission_roughness(m), m.specularAnisotropy,
+                                                               m.specularRotation);
+            if (m.thinWalled) {
+                next = ray.d;
+                const float R = fresnel_dielectric(std::fabs(dot(ng, v)), 1.0f, std::max(m.ior, 1.0f));
+                const float pane = pane_transmittance(R);
+                const V3 scale = transmission_tint(m) * coat_transmission_scale(m, cn, v, next) *
+                                 (clampf(m.transmission, 0.0f, 1.0f) * (1.0f - clampf(m.metallic, 0.0f, 1.0f)));
+                const float ac = std::max(std::fabs(dot(ng, next)), 1.0e-4f);
+                f = scale * pane / ac;
+                pdf = pT;
+                delta = true;
+            } else {
+                const float ei = inside ? std::max(m.ior, 1.0f) : 1.0f;
+                const float et = inside ? 1.0f : std::max(m.ior, 1.0f);
+                const float etaRatio = ei / et;
+                const float relEta = et / std::max(ei, 1.0f);
+                const float R = fresnel_dielectric(std::fabs(dot(ng, v)), ei, et);
+                const float mt = clampf(m.transmission, 0.0f, 1.0f) * (1.0f - clampf(m.metallic, 0.0f, 1.0f));
+                if (rT <= kDeltaRoughness) {
+                    if (!refract_dir(ray.d, ng, etaRatio, &next)) {
+                        next = normalize(reflect(ray.d, ng));
+                        if (wr <= 0.0f) break;
+                        f = bsdf_eval(m, n, cn, v, next);
+                        pdf = std::max((wr / wsum) * bsdf_pdf(m, n, cn, v, next), 1.0e-6f);
+                        type = kReflective;
+                        delta = true;
+                        pickT = false;
+                        pickS = true;
+                    } else {
+                        const V3 scale = transmission_tint(m) * coat_transmission_scale(m, cn, v, next) * mt;
+                        const float ac = std::max(std::fabs(dot(ng, next)), 1.0e-4f);
+                        const float transport = 1.0f / std::max(relEta * relEta, 1.0e-6f);
+                        f = scale * (1.0f - R) * transport / ac;
+                        pdf = pT;
+                        inside = !inside;
+                        if (inside) {
+                            ext = medium_extinction(m);
+                            mediumOn = m.transmissionDepth > 0.0f || max3(v3(m.transmissionScatter)) > 0.0f;
+                        } else {
+                            mediumOn = false;
+                        }
+                        delta = true;
+                    }
+                } else {
+                    V3 mn = sample_ggx_half(ng, du, rT);
+                    if (dot(mn, v) < 0.0f) mn = mn * -1.0f;
+                    if (!refract_dir(ray.d, mn, etaRatio, &next)) {
+                        next = normalize(reflect(ray.d, mn));
+                        if (wr <= 0.0f) break;
+                        f = bsdf_eval(m, n, cn, v, next);
+                        pdf = std::max((wr / wsum) * bsdf_pdf(m, n, cn, v, next), 1.0e-6f);
+                        type = kReflective;
+                        delta = false;
+                        pickT = false;
+                        pickS = true;
+                    } else {
+                        f = transmission_lobe(m, ng, cn, v, next, ei, et, rT);
+                        pdf = std::max(pT * transmission_pdf(ng, v, next, et / ei, rT), 1.0e-6f);
+                        inside = !inside;
+                        if (inside) {
+                            ext = medium_extinction(m);
+                            mediumOn = m.transmissionDepth > 0.0f || max3(v3(m.transmissionScatter)) > 0.0f;
+                        } else {
+                            mediumOn = false;
+                        }
+                        delta = rT <= kDeltaRoughness;
+                    }
+                }
+            }
+        } else if (pickS || pickC) {
+            if (nSpecular >= st.specularDepth) break;
+            ++nSpecular;
+            type = kReflective;
+            const V3 sn = pickC ? cn : n;
+            const float sr = pickC ? roughness_sampling(m.coatRoughness, m.coatAnisotropy, m.coatRotation)
+                                   : roughness_sampling(base_roughness(m), m.specularAnisotropy, m.specularRotation);
+            const V3 h = sample_ggx_half(sn, du, sr);
+            next = normalize(reflect(ray.d, h));
+            const float rc = clampf(dot(sn, next), 0.0f, 1.0f);
+            if (rc <= 0.0f) break;
+            pdf = std::max((wr / wsum) * bsdf_pdf(m, n, cn, v, next), 1.0e-6f);
+            f = bsdf_eval(m, n, cn, v, next);
+            delta = sr <= kDeltaRoughness;
+        } else {
+            if (nDiffuse >= st.diffuseDepth) break;
+            ++nDiffuse;
+            uint32_t gbin = 0;
+            const bool useGuide = guide && guide->total > kGuideMinTotal;
+            float gpdf = 0.0f;
+            const float pCos = useGuide ? (1.0f - kGuideMix) : 1.0f;
+            const float pGuide = useGuide ? kGuideMix : 0.0f;
+            const bool sampleGuide = useGuide && smp.next1(b0 + 5u) < pGuide;
+            if (sampleGuide &&
+                !guide_sample(*guide, n, smp.next1(b0 + 6u), smp.next2(b0 + 7u), &next, &gpdf, &gbin)) {
+                next = sample_cosine(n, du);
+                gpdf = guide_pdf(*guide, n, next, &gbin);
+            } else if (!sampleGuide) {
+                next = sample_cosine(n, du);
+                if (useGuide) gpdf = guide_pdf(*guide, n, next, &gbin);
+            }
+            const float cpdf = clampf(dot(n, next), 0.0f, 1.0f) * kInvPi;
+            pdf = std::max((wr / wsum) * (pCos * cpdf + pGuide * gpdf), 1.0e-6f);
+            f = bsdf_eval(m, n, cn, v, next);
+            const float pc = clampf(dot(n, next), 0.0f, 1.0f);
+            const V3 preview = clamp_max_component(T * f * (pc / pdf), kThroughputClamp);
+            if (useGuide && upd) recs.push_back({gbin, preview});
+        }
+
+        const float cosT = pickT ? std::max(std::fabs(dot(ng, next)), 1.0e-4f) : clampf(dot(n, next), 0.0f, 1.0f);
+        if (cosT <= 0.0f) break;
+        T = clamp_max_component(T * f * (cosT / pdf), kThroughputClamp);
+        const V3 rn = pickT ? ng : n;
+        const float side = dot(rn, next) >= 0.0f ? 1.0f : -1.0f;
+        ray.o = hit.p + rn * (kEps * side);
+        ray.d = next;
+        prevType = type;
+        prevDelta = delta;
+        prevPdf = pdf;
+        if (depth >= st.rrDepth) {
+            const float q = clampf(max3(T), 0.05f, 0.95f);
+            if (smp.next1(b0 + 4u) > q) break;
+            T = T / q;
+        }
+    }
+    return res;
+}
+
+// ---------------------------------------------------------------------------
+// Batch render (CpuPathTracer.cpp:2043-2170)
+// ---------------------------------------------------------------------------
+struct Frame {
+    float* beauty;
+    float* albedo;
+    float* normal;
+    float* depth;
+};
+
+Settings to_settings(const shiro_hip_settings& s) {
+    Settings o;
+    o.width = s.width;
+    o.height = s.height;
+    o.domeLightSamples = s.domeLightSamples;
+    o.maxDepth = s.maxDepth;
+    o.diffuseDepth = s.diffuseDepth;
+    o.specularDepth = s.specularDepth;
+    o.threadLimit = s.threadLimit;
+    o.backgroundVisible = s.backgroundVisible != 0;
+    o.rrDepth = s.russianRouletteDepth;
+    return o;
+}
+
+Camera to_camera(const shiro_hip_camera& c) {
+    return {v3(c.position), v3(c.forward), v3(c.right), v3(c.up), c.verticalFovDegrees, c.aspectRatio};
+}
+
+// Renders rows [y0, y1) x columns [x0, x1) of one batch.  Returns false if
+// cancelled.
+bool render_rows(const Scene& s, const Settings& st, const Camera& cam, uint32_t start, uint32_t count,
+                 const Guide* guide, std::array<float, kGuideBins>* upd, uint32_t y0, uint32_t y1, uint32_t x0,
+                 uint32_t x1, const volatile int32_t* cancel, const Frame& out) {
+    for (uint32_t y = y0; y < y1; ++y) {
+        if (cancel && *cancel) return false;
+        for (uint32_t x = x0; x < x1; ++x) {
+            V4 beauty{};
+            V3 albedo{}, normal{};
+            float depth = std::numeric_limits<float>::infinity();
+            for (uint32_t i = 0; i < count; ++i) {
+                if (cancel && *cancel) return false;
+                const Sampler smp{x, y, start + i};
+                const Ray r = camera_ray(cam, st, x, y, smp.next2(0));
+                const PathResult pr = trace_path(s, st, r, smp, guide, upd);
+                beauty = beauty + V4{pr.radiance.x, pr.radiance.y, pr.radiance.z,
+                                     pr.depth < std::numeric_limits<float>::infinity() ? 1.0f : 0.0f};
+                albedo = albedo + pr.albedo;
+                normal = normal + pr.normal;
+                if (pr.depth < depth) depth = pr.depth;
+            }
+            const float inv = 1.0f / static_cast<float>(count);
+            const V4 b = beauty / static_cast<float>(count);
+            const V3 a = albedo * inv;
+            const V3 nn = normal * inv;
+            const size_t p = static_cast<size_t>(y) * st.width + x;
+            out.beauty[p * 4 + 0] = b.x;
+            out.beauty[p * 4 + 1] = b.y;
+            out.beauty[p * 4 + 2] = b.z;
+            out.beauty[p * 4 + 3] = b.w;
+            out.albedo[p * 3 + 0] = a.x;
+            out.albedo[p * 3 + 1] = a.y;
+            out.albedo[p * 3 + 2] = a.z;
+            out.normal[p * 3 + 0] = nn.x;
+            out.normal[p * 3 + 1] = nn.y;
+            out.normal[p * 3 + 2] = nn.z;
+            out.depth[p] = depth;
+        }
+    }
+    return true;
+}
+
+}  // namespace orc
+
+// ---------------------------------------------------------------------------
+// C ABI used by the tests (ctypes) and the CPU-baseline leg of the benchmark.
+// ---------------------------------------------------------------------------
+using namespace orc;
+
+extern "C" {
+
+float oracle_qmc(uint32_t px, uint32_t py, uint32_t s, uint32_t dim) { return qmc_value(px, py, s, dim); }
+
+void oracle_qmc_fill(uint32_t x0, uint32_t y0, uint32_t w, uint32_t h, uint32_t s, uint32_t dim0, uint32_t dimCount,
+                     float* out) {
+    for (uint32_t y = 0; y < h; ++y)
+        for (uint32_t x = 0; x < w; ++x)
+            for (uint32_t d = 0; d < dimCount; ++d)
+                out[(static_cast<size_t>(y) * w + x) * dimCount + d] = qmc_value(x0 + x, y0 + y, s, dim0 + d);
+}
+
+float oracle_radical_inverse(uint32_t base, uint32_t n) { return radical_inverse(base, n); }
+
+void* oracle_scene_create(const shiro_hip_scene* in, int useBvh) {
+    auto* s = new Scene();
+    s->useBvh = useBvh != 0;
+    s->materials.assign(in->materials, in->materials + in->materialCount);
+    s->envTop = v3(in->environmentTop);
+    s->envBottom = v3(in->environmentBottom);
+    for (uint32_t i = 0; i < in->distantLightCount; ++i)
+        s->dirs.push_back({v3(in->distantLights[i].direction), v3(in->distantLights[i].radiance)});
+    for (uint32_t i = 0; i < in->domeLightCount; ++i) {
+        const shiro_hip_dome_light& d = in->domeLights[i];
+        Dome dm;
+        dm.radiance = v3(d.radiance);
+        dm.right = v3(d.right);
+        dm.up = v3(d.up);
+        dm.forward = v3(d.forward);
+        if (d.pixels && d.width > 0 && d.height > 0) {
+            auto e = std::make_shared<Env>();
+            e->w = d.width;
+            e->h = d.height;
+            e->layout = d.layout == SHIRO_HIP_ENV_ANGULAR ? SHIRO_HIP_ENV_ANGULAR : SHIRO_HIP_ENV_LATLONG;
+            e->px.resize(static_cast<size_t>(d.width) * d.height);
+            for (size_t p = 0; p < e->px.size(); ++p) e->px[p] = v3(d.pixels + 3 * p);
+            env_build_tables(*e);
+            dm.env = e;
+        }
+        s->domes.push_back(dm);
+    }
+    s->meshes.resize(in->meshCount);
+    for (uint32_t mi = 0; mi < in->meshCount; ++mi) {
+        const shiro_hip_mesh& src = in->meshes[mi];
+        Mesh& m = s->meshes[mi];
+        m.material = src.materialIndex;
+        m.pos.resize(src.vertexCount);
+        for (uint32_t v = 0; v < src.vertexCount; ++v) m.pos[v] = v3(src.positions + 3 * v);
+        if (src.normals) {
+            m.nrm.resize(src.normalCount);
+            for (uint32_t v = 0; v < src.normalCount; ++v) m.nrm[v] = v3(src.normals + 3 * v);
+        }
+        m.idx.assign(src.indices, src.indices + src.indexCount);
+        for (uint32_t p = 0; 3u * p + 2u < src.indexCount; ++p) {
+            s->refMesh.push_back(mi);
+            s->refPrim.push_back(p);
+        }
+    }
+    // Emissive triangle list, mesh order then primitive order (:482-522).
+    for (uint32_t mi = 0; mi < s->meshes.size(); ++mi) {
+        const Mesh& m = s->meshes[mi];
+        if (m.idx.size() < 3 || m.idx.size() % 3 != 0) continue;
+        const Material mat = m.material < s->materials.size() ? s->materials[m.material] : default_material();
+        const V3 rad{std::max(mat.emissionColor[0] * mat.emissionStrength, 0.0f),
+                     std::max(mat.emissionColor[1] * mat.emissionStrength, 0.0f),
+                     std::max(mat.emissionColor[2] * mat.emissionStrength, 0.0f)};
+        const float rw = std::max(0.2126f * rad.x + 0.7152f * rad.y + 0.0722f * rad.z, 0.0f);
+        if (rw <= 0.0f) continue;
+        const uint32_t np = static_cast<uint32_t>(m.idx.size() / 3u);
+        for (uint32_t p = 0; p < np; ++p) {
+            V3 gn;
+            const float area = tri_area(m, p, &gn);
+            if (area <= 0.0f) continue;
+            const float w = area * rw;
+            s->emissiveSum += w;
+            s->emissive.push_back({mi, p, area, w, s->emissiveSum, rad, gn});
+        }
+    }
+    if (s->useBvh) bvh_build(*s);
+    return s;
+}
+
+void oracle_scene_destroy(void* h) { delete static_cast<Scene*>(h); }
+
+void oracle_scene_set_bvh(void* h, int useBvh) {
+    Scene* s = static_cast<Scene*>(h);
+    s->useBvh = useBvh != 0;
+    if (s->useBvh && s->bvh.empty()) bvh_build(*s);
+}
+
+// RenderSampleBatch with the reference's row-band threading (:2064-2156).
+// threads == 0 selects hardware_concurrency()-1 like the reference (or
+// settings.threadLimit when set).  Returns 0, or 3 when cancelled.
+int oracle_render(void* h, const shiro_hip_camera* camera, const shiro_hip_settings* settings, uint32_t sampleStart,
+                  uint32_t sampleCount, uint32_t threads, const volatile int32_t* cancel, shiro_hip_frame* out) {
+    Scene& s = *static_cast<Scene*>(h);
+    const Settings st = to_settings(*settings);
+    const Camera cam = to_camera(*camera);
+    const size_t np = static_cast<size_t>(st.width) * st.height;
+    std::fill(out->beauty, out->beauty + np * 4, 0.0f);
+    std::fill(out->albedo, out->albedo + np * 3, 0.0f);
+    std::fill(out->normal, out->normal + np * 3, 0.0f);
+    std::fill(out->depth, out->depth + np, std::numeric_limits<float>::infinity());
+    if (sampleCount == 0) return 0;
+    Guide guide;
+    {
+        std::lock_guard<std::mutex> lock(s.guideMutex);
+        if (sampleStart == 0u || s.guideWeights.size() != kGuideBins) s.guideWeights.assign(kGuideBins, 0.0f);
+        guide = guide_snapshot(s.guideWeights);
+    }
+    const uint32_t hw = std::max(1u, std::thread::hardware_concurrency());
+    uint32_t n = threads;
+    if (n == 0) n = st.threadLimit > 0 ? std::max(1u, std::min(st.threadLimit, hw)) : (hw > 1 ? hw - 1 : 1);
+    const uint32_t rows = std::max(1u, st.height / n);
+    std::vector<std::array<float, kGuideBins>> upd(n);
+    for (auto& u : upd) u.fill(0.0f);
+    const Frame f{out->beauty, out->albedo, out->normal, out->depth};
+    std::vector<std::thread> pool;
+    std::atomic<bool> cancelled{false};
+    for (uint32_t t = 0; t < n; ++t) {
+        const uint32_t y0 = t * rows;
+        if (y0 >= st.height) break;
+        const uint32_t y1 = t + 1 == n ? st.height : std::min(st.height, y0 + rows);
+        pool.emplace_back([&, t, y0, y1]() {
+            if (!render_rows(s, st, cam, sampleStart, sampleCount, &guide, &upd[t], y0, y1, 0, st.width, cancel, f))
+                cancelled = true;
+        });
+    }
+    for (auto& th : pool) th.join();
+    {
+        std::lock_guard<std::mutex> lock(s.guideMutex);
+        for (const auto& u : upd)
+            for (uint32_t b = 0; b < kGuideBins; ++b) s.guideWeights[b] += u[b];
+    }
+    return cancelled ? 3 : 0;
+}
+
+// Renders a rectangle [x0,x0+w) x [y0,y0+h) of the frame with `threads`
+// threads splitting rows; the frame buffers are full-size.  Used for bounded
+// CPU-baseline timing.  Guide disabled (single batch semantics).
+int oracle_render_region(void* h, const shiro_hip_camera* camera, const shiro_hip_settings* settings,
+                         uint32_t sampleStart, uint32_t sampleCount, uint32_t x0, uint32_t y0, uint32_t w, uint32_t hgt,
+                         uint32_t threads, shiro_hip_frame* out) {
+    Scene& s = *static_cast<Scene*>(h);
+    const Settings st = to_settings(*settings);
+    const Camera cam = to_camera(*camera);
+    const Frame f{out->beauty, out->albedo, out->normal, out->depth};
+    const uint32_t n = std::max(1u, threads);
+    std::atomic<uint32_t> nextRow{y0};
+    std::vector<std::thread> pool;
+    for (uint32_t t = 0; t < n; ++t) {
+        pool.emplace_back([&]() {
+            for (;;) {
+                const uint32_t y = nextRow.fetch_add(1);
+                if (y >= y0 + hgt) break;
+                render_rows(s, st, cam, sampleStart, sampleCount, nullptr, nullptr, y, y + 1, x0, x0 + w, nullptr, f);
+            }
+        });
+    }
+    for (auto& th : pool) th.join();
+    return 0;
+}
+
+// Device-side FrameAccumulator counterpart (Renderer.cpp:47-110) on host
+// arrays: sums are updated in place; resolve writes the frame.
+void oracle_accum_add(uint32_t np, const float* beauty, const float* albedo, const float* normal, const float* depth,
+                      uint32_t count, float* bSum, float* aSum, float* nSum, float* dSum, uint32_t* dCount) {
+    const float w = static_cast<float>(count);
+    for (uint32_t p = 0; p < np; ++p) {
+        for (int c = 0; c < 4; ++c) bSum[p * 4 + c] = bSum[p * 4 + c] + beauty[p * 4 + c] * w;
+        for (int c = 0; c < 3; ++c) aSum[p * 3 + c] = aSum[p * 3 + c] + albedo[p * 3 + c] * w;
+        for (int c = 0; c < 3; ++c) nSum[p * 3 + c] = nSum[p * 3 + c] + normal[p * 3 + c] * w;
+        if (depth[p] < std::numeric_limits<float>::infinity()) {
+            dSum[p] += depth[p] * w;
+            dCount[p] += count;
+        }
+    }
+}
+
+void oracle_accum_resolve(uint32_t np, uint32_t total, const float* bSum, const float* aSum, const float* nSum,
+                          const float* dSum, const uint32_t* dCount, float* beauty, float* albedo, float* normal,
+                          float* depth) {
+    if (total == 0) {
+        std::fill(beauty, beauty + np * 4, 0.0f);
+        std::fill(albedo, albedo + np * 3, 0.0f);
+        std::fill(normal, normal + np * 3, 0.0f);
+        std::fill(depth, depth + np, std::numeric_limits<float>::infinity());
+        return;
+    }
+    const float inv = 1.0f / static_cast<float>(total);
+    for (uint32_t p = 0; p < np; ++p) {
+        for (int c = 0; c < 4; ++c) beauty[p * 4 + c] = bSum[p * 4 + c] * inv;
+        for (int c = 0; c < 3; ++c) albedo[p * 3 + c] = aSum[p * 3 + c] * inv;
+        for (int c = 0; c < 3; ++c) normal[p * 3 + c] = nSum[p * 3 + c] * inv;
+        depth[p] = dCount[p] > 0 ? dSum[p] / static_cast<float>(dCount[p]) : std::numeric_limits<float>::infinity();
+    }
+}
+
+// Closest hit for rays (n x 8 floats), as shiro_hip_intersect.
+void oracle_intersect(void* h, const float* rays, uint32_t n, uint32_t* prim, float* t, float* uv) {
+    const Scene& s = *static_cast<Scene*>(h);
+    for (uint32_t i = 0; i < n; ++i) {
+        Ray r;
+        r.o = v3(rays + 8 * i);
+        r.d = v3(rays + 8 * i + 3);
+        r.tmin = rays[8 * i + 6];
+        r.tmax = rays[8 * i + 7];
+        Candidate best;
+        if (s.useBvh) {
+            // Reuse the traversal but keep the candidate (global index).
+            Hit hh;
+            if (intersect_bvh(s, r, &hh)) {
+                // Recover the global index from (mesh, prim).
+                uint32_t g = 0;
+                for (uint32_t mi = 0; mi < hh.mesh; ++mi) g += static_cast<uint32_t>(s.meshes[mi].idx.size() / 3u);
+                prim[i] = g + hh.prim;
+                t[i] = hh.t;
+            } else {
+                prim[i] = 0xffffffffu;
+                t[i] = FLT_MAX;
+            }
+            uv[2 * i] = 0.0f;
+            uv[2 * i + 1] = 0.0f;
+            continue;
+        }
+        const uint32_t ng = static_cast<uint32_t>(s.refMesh.size());
+        for (uint32_t g = 0; g < ng; ++g) test_global(s, r, g, &best);
+        prim[i] = best.global;
+        t[i] = best.t;
+        uv[2 * i] = best.u;
+        uv[2 * i + 1] = best.v;
+    }
+}
+
+void oracle_occluded(void* h, const float* rays, uint32_t n, uint8_t* out) {
+    const Scene& s = *static_cast<Scene*>(h);
+    for (uint32_t i = 0; i < n; ++i) out[i] = occluded(s, v3(rays + 8 * i), v3(rays + 8 * i + 3), rays[8 * i + 7]) ? 1 : 0;
+}
+
+// Camera ray of `sampleIndex` at every pixel and its closest hit.
+void oracle_trace_primary(void* h, const shiro_hip_camera* camera, const shiro_hip_settings* settings,
+                          uint32_t sampleIndex, uint32_t* prim, float* t) {
+    const Scene& s = *static_cast<Scene*>(h);
+    const Settings st = to_settings(*settings);
+    const Camera cam = to_camera(*camera);
+    std::vector<uint32_t> offs(s.meshes.size() + 1, 0);
+    for (size_t mi = 0; mi < s.meshes.size(); ++mi) offs[mi + 1] = offs[mi] + static_cast<uint32_t>(s.meshes[mi].idx.size() / 3u);
+    const uint32_t hw = std::max(1u, std::thread::hardware_concurrency());
+    std::vector<std::thread> pool;
+    for (uint32_t tIdx = 0; tIdx < hw; ++tIdx) {
+        pool.emplace_back([&, tIdx]() {
+            for (uint32_t y = tIdx; y < st.height; y += hw)
+                for (uint32_t x = 0; x < st.width; ++x) {
+                    const Sampler smp{x, y, sampleIndex};
+                    const Ray r = camera_ray(cam, st, x, y, smp.next2(0));
+                    Hit hh;
+                    const size_t p = static_cast<size_t>(y) * st.width + x;
+                    if (intersect(s, r, &hh)) {
+                        prim[p] = offs[hh.mesh] + hh.prim;
+                        t[p] = hh.t;
+                    } else {
+                        prim[p] = 0xffffffffu;
+                        t[p] = FLT_MAX;
+                    }
+                }
+        });
+    }
+    for (auto& th : pool) th.join();
+}
+
+// EvaluateBsdf / EvaluateBsdfPdf on explicit tuples.
+void oracle_eval_bsdf(const shiro_hip_material* mats, const uint32_t* matIndex, const float* n, const float* cn,
+                      const float* v, const float* l, uint32_t count, float* value, float* pdf) {
+    for (uint32_t i = 0; i < count; ++i) {
+        const Material& m = mats[matIndex[i]];
+        const V3 f = bsdf_eval(m, v3(n + 3 * i), v3(cn + 3 * i), v3(v + 3 * i), v3(l + 3 * i));
+        value[3 * i] = f.x;
+        value[3 * i + 1] = f.y;
+        value[3 * i + 2] = f.z;
+        pdf[i] = bsdf_pdf(m, v3(n + 3 * i), v3(cn + 3 * i), v3(v + 3 * i), v3(l + 3 * i));
+    }
+}
+
+// Lobe sampling weights (diffuse, base specular, coat, transmission) and the
+// diffuse colour of a material.
+void oracle_lobe_weights(const shiro_hip_material* m, float* w4, float* diffuse3) {
+    w4[0] = w_diffuse(*m);
+    w4[1] = w_base_specular(*m);
+    w4[2] = w_coat(*m);
+    w4[3] = w_transmission(*m);
+    const V3 d = diffuse_color(*m);
+    diffuse3[0] = d.x;
+    diffuse3[1] = d.y;
+    diffuse3[2] = d.z;
+}
+
+// Scalar BSDF building blocks for closed-form checks.
+float oracle_ggx_d(float nDotH, float roughness) { return ggx_d(nDotH, roughness); }
+float oracle_ggx_g2(float nv, float nl, float roughness) { return ggx_g2(nv, nl, roughness); }
+float oracle_fresnel_dielectric(float c, float ei, float et) { return fresnel_dielectric(c, ei, et); }
+float oracle_specular_pdf(const float* n, const float* v, const float* l, float r) {
+    return specular_pdf(v3(n), v3(v), v3(l), r);
+}
+void oracle_sample_ggx_half(const float* n, float u0, float u1, float r, float* h) {
+    const V3 o = sample_ggx_half(v3(n), {u0, u1}, r);
+    h[0] = o.x;
+    h[1] = o.y;
+    h[2] = o.z;
+}
+void oracle_sample_cosine(const float* n, float u0, float u1, float* d) {
+    const V3 o = sample_cosine(v3(n), {u0, u1});
+    d[0] = o.x;
+    d[1] = o.y;
+    d[2] = o.z;
+}
+
+// Environment-map tables and sampling on a dome of a created scene.
+int oracle_env_tables(void* h, uint32_t dome, float* rowCdf, float* condCdf) {
+    const Scene& s = *static_cast<Scene*>(h);
+    if (dome >= s.domes.size() || !s.domes[dome].env) return 1;
+    const Env& e = *s.domes[dome].env;
+    if (e.rowCdf.empty()) return 2;
+    std::copy(e.rowCdf.begin(), e.rowCdf.end(), rowCdf);
+    std::copy(e.condCdf.begin(), e.condCdf.end(), condCdf);
+    return 0;
+}
+
+int oracle_env_sample(void* h, uint32_t dome, const float* u, uint32_t n, float* dir, float* pdf, float* rgb) {
+    const Scene& s = *static_cast<Scene*>(h);
+    if (dome >= s.domes.size() || !s.domes[dome].env) return 1;
+    const Env& e = *s.domes[dome].env;
+    for (uint32_t i = 0; i < n; ++i) {
+        V3 d;
+        float p;
+        const V3 c = env_sample(e, {u[2 * i], u[2 * i + 1]}, &d, &p);
+        dir[3 * i] = d.x;
+        dir[3 * i + 1] = d.y;
+        dir[3 * i + 2] = d.z;
+        pdf[i] = p;
+        rgb[3 * i] = c.x;
+        rgb[3 * i + 1] = c.y;
+        rgb[3 * i + 2] = c.z;
+    }
+    return 0;
+}
+
+int oracle_env_eval(void* h, uint32_t dome, const float* dir, uint32_t n, float* pdf, float* rgb) {
+    const Scene& s = *static_cast<Scene*>(h);
+    if (dome >= s.domes.size() || !s.domes[dome].env) return 1;
+    const Env& e = *s.domes[dome].env;
+    for (uint32_t i = 0; i < n; ++i) {
+        const V3 d = v3(dir + 3 * i);
+        pdf[i] = env_pdf(e, d);
+        const V3 c = env_eval(e, d);
+        rgb[3 * i] = c.x;
+        rgb[3 * i + 1] = c.y;
+        rgb[3 * i + 2] = c.z;
+    }
+    return 0;
+}
+
+uint32_t oracle_bvh_node_count(void* h) { return static_cast<uint32_t>(static_cast<Scene*>(h)->bvh.size()); }
+
+}  // extern "C"
