@@ -1,0 +1,164 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X path-tracing backend: Msamples/s of
+RenderSampleBatch on one GPU.
+
+A step is one full-frame batch (every pixel, all samples of the workload)
+rendered into device-resident AOV buffers (inputs and outputs stay in HBM;
+no host copies inside the timed region).  Default workload: c2 -- 1920x1080,
+64 spp, max depth 8, 181,920 triangles, 2048x1024 importance-sampled dome,
+standard_surface materials (see shiro_amd/workloads.py).  Prints one JSON
+line with the throughput, the roofline of the render kernel and a CPU
+baseline (the CPU oracle on this host's cores, bounded sample).
+
+    python bench.py [--steps K] [--warmup W] [--workload c2|c1|c3]
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+NODE_BYTES, TRI_BYTES, TEXEL_BYTES, CDF_BYTES, HIT_BYTES = 128, 48, 16, 4, 96
+
+
+def workload_desc(wl):
+    s = wl.settings
+    return (f"{wl.name}: {s.width}x{s.height}, {s.samplesPerPixel} spp, max depth {s.maxDepth}, "
+            f"{wl.scene.triangle_count} triangles, {len(wl.scene.domeLights)} dome / "
+            f"{len(wl.scene.distantLights)} directional lights")
+
+
+def cpu_baseline(wl, target_seconds, threads):
+    """Times the CPU oracle (binary SAH BVH, this host's cores) on a band of
+    full-width rows of the same frame at the full spp."""
+    from oracle.bindings import OracleScene
+    t0 = time.perf_counter()
+    orc = OracleScene(wl.scene, bvh=True)
+    build_s = time.perf_counter() - t0
+    st = wl.settings
+    spp = st.samplesPerPixel
+    rows = 2
+    y0 = st.height // 2
+    t0 = time.perf_counter()
+    orc.render_region(wl.camera, st, 0, y0, st.width, rows, 0, spp, threads)
+    dt = max(time.perf_counter() - t0, 1e-3)
+    rows2 = int(min(st.height // 2, max(rows, rows * target_seconds / dt)))
+    y0 = (st.height - rows2) // 2
+    t0 = time.perf_counter()
+    orc.render_region(wl.camera, st, 0, y0, st.width, rows2, 0, spp, threads)
+    dt = time.perf_counter() - t0
+    samples = st.width * rows2 * spp
+    orc.close()
+    return {
+        "value": samples / dt / 1e6,
+        "unit": "Msamples/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"rows {y0}-{y0 + rows2 - 1} of the same frame ({st.width}x{rows2} px x {spp} spp = "
+                  f"{samples} samples) in {dt:.1f} s; CPU oracle with a binary SAH BVH "
+                  f"(built in {build_s:.1f} s, not timed), row-parallel threads",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3"])
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-count", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this backend has no multi-GPU path "
+              "(a frame shards by sample range or by tile; see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from shiro_amd import workloads
+    from shiro_amd.hip import Context
+
+    wl = workloads.WORKLOADS[args.workload]()
+    st = wl.settings
+    print(f"# workload {workload_desc(wl)}", file=sys.stderr, flush=True)
+    ctx = Context(0)
+    t0 = time.perf_counter()
+    ctx.upload(wl.scene)
+    upload_s = time.perf_counter() - t0
+    info = ctx.scene_info()
+    print(f"# BVH4 nodes {info.bvhNodeCount} leaves {info.bvhLeafCount} depth {info.bvhDepth} "
+          f"build {info.buildSeconds:.2f}s upload {info.uploadSeconds:.2f}s", file=sys.stderr, flush=True)
+    npx = st.width * st.height
+    dev = torch.device("cuda:0")
+    bufs = [torch.empty(npx * k, dtype=torch.float32, device=dev) for k in (4, 3, 3, 1)]
+    ptrs = [b.data_ptr() for b in bufs]
+    torch.cuda.synchronize()
+    for i in range(args.warmup):
+        ctx.render_device(wl.camera, st, ptrs, 0, st.samplesPerPixel, synchronize=True)
+        print(f"# warmup {i} {ctx.last_render_ms():.1f} ms", file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    ctx.synchronize()
+    gpu_ms = []
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        ctx.render_device(wl.camera, st, ptrs, 0, st.samplesPerPixel, synchronize=True)
+        gpu_ms.append(ctx.last_render_ms())
+        print(f"# step {i} {gpu_ms[-1]:.1f} ms", file=sys.stderr, flush=True)
+    ctx.synchronize()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    samples = npx * st.samplesPerPixel
+    value = samples * args.steps / wall / 1e6
+    ms_per_step = wall * 1e3 / args.steps
+    frame = bufs[0].view(-1, 4)
+    ok = bool(torch.isfinite(frame).all().item())
+
+    # Roofline of the render kernel: algorithmic bytes per sample from the
+    # fetch counters (BVH4 nodes 128 B, triangles 48 B, hit data 96 B,
+    # environment texels 16 B, CDF entries 4 B) plus the AOV write (44 B per
+    # pixel per batch), over the kernel's event-timed duration.
+    roofline = None
+    if not args.no_count:
+        cspp = min(4, st.samplesPerPixel)
+        c = ctx.count_fetches(wl.camera, st, 0, cspp)
+        per_sample = (c.nodeFetches * NODE_BYTES + c.triFetches * TRI_BYTES + c.hitFetches * HIT_BYTES +
+                      c.envTaps * TEXEL_BYTES + c.cdfSteps * CDF_BYTES) / max(c.samples, 1) + 44.0 / st.samplesPerPixel
+        slice_spp = max(1, min(st.samplesPerPixel, (8 << 20) // npx))
+        launches = math.ceil(st.samplesPerPixel / slice_spp)
+        avg_launch_ms = sum(gpu_ms) / len(gpu_ms) / launches
+        bytes_per_launch = per_sample * npx * slice_spp
+        achieved = bytes_per_launch / (avg_launch_ms * 1e-3) / 1e9
+        roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                    "kernel": "k_render<false> (megakernel)", "launches_per_step": launches,
+                    "avg_launch_ms": round(avg_launch_ms, 3), "bytes_per_launch": int(bytes_per_launch),
+                    "bytes_per_sample": round(per_sample, 1),
+                    "per_sample": {"rays": c.rays / c.samples, "shadow_rays": c.shadowRays / c.samples,
+                                   "node_fetches": c.nodeFetches / c.samples, "tri_tests": c.triFetches / c.samples,
+                                   "env_taps": c.envTaps / c.samples, "cdf_steps": c.cdfSteps / c.samples}}
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(wl, args.cpu_seconds, args.cpu_threads)
+    out = {
+        "metric": "Msamples/s", "value": round(value, 3), "unit": "Msamples/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 2),
+        "gpu_ms_per_step": round(sum(gpu_ms) / len(gpu_ms), 2),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded procedural scene and dome map)",
+        "config": {"workload": workload_desc(wl), "transport": "megakernel", "samples_per_step": samples,
+                   "bvh": f"BVH4 {info.bvhNodeCount} nodes, depth {info.bvhDepth}",
+                   "scene_upload_s": round(upload_s, 2), "finite_output": ok},
+        "roofline": roofline, "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

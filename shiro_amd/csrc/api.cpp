@@ -1,0 +1,730 @@
+// C ABI of the backend (include/shiro_hip.h): context, scene upload, batch
+// rendering, device accumulation and the single-stage entry points.
+//
+// The context owns one HIP stream on one device and the device-resident
+// scene.  Calls on a context are serialised by a mutex, as the reference
+// backends serialise RenderSampleBatch (CpuPathTracer.h:26-31,
+// OptixBackend.h:24-26).  Errors are reported as shiro_hip_result codes with
+// a message retrievable through shiro_hip_last_error; nothing throws across
+// the ABI.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/shiro_hip.h"
+#include "bvh_build.h"
+#include "host_scene.h"
+#include "kernels.h"
+
+using namespace shiro_hip;
+
+namespace {
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    // Grows to at least `n` bytes (contents not preserved).
+    hipError_t reserve(size_t n) {
+        if (n <= bytes && p) return hipSuccess;
+        release();
+        if (n == 0) return hipSuccess;
+        hipError_t e = hipMalloc(&p, n);
+        if (e != hipSuccess) {
+            p = nullptr;
+            return e;
+        }
+        bytes = n;
+        return hipSuccess;
+    }
+    template <typename T>
+    T* as() const {
+        return static_cast<T*>(p);
+    }
+};
+
+struct EnvBuffers {
+    DevBuf texels, row, cond;
+};
+
+}  // namespace
+
+struct shiro_hip_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::mutex mu;
+    std::string err;
+    float lastMs = 0.0f;
+    // Scene.
+    bool hasScene = false;
+    bool emissive = false;
+    DevBuf nodes, tris, hitdata, materials, derived;
+    EnvBuffers env[kMaxDomes];
+    DevScene dev{};
+    shiro_hip_scene_info info{};
+    // Per-batch scratch.
+    DevBuf sumB, sumA, sumN, sumD;
+    DevBuf frB, frA, frN, frD;
+    DevBuf counters;
+    // Accumulator.
+    DevBuf acB, acA, acN, acD, acC;
+    uint32_t acW = 0, acH = 0, acTotal = 0;
+};
+
+namespace {
+
+int fail(shiro_hip_ctx* c, int code, const std::string& msg) {
+    if (c) c->err = msg;
+    return code;
+}
+
+int hip_fail(shiro_hip_ctx* c, hipError_t e, const char* what) {
+    const int code = e == hipErrorOutOfMemory ? SHIRO_HIP_OOM : SHIRO_HIP_LAUNCH;
+    return fail(c, code, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define HIPC(expr, what)                          \
+    do {                                          \
+        hipError_t e_ = (expr);                   \
+        if (e_ != hipSuccess) return hip_fail(ctx, e_, what); \
+    } while (0)
+
+DevCamera to_dev(const shiro_hip_camera& c) {
+    DevCamera d;
+    std::memcpy(d.position, c.position, 12);
+    std::memcpy(d.forward, c.forward, 12);
+    std::memcpy(d.right, c.right, 12);
+    std::memcpy(d.up, c.up, 12);
+    d.vfov = c.verticalFovDegrees;
+    d.aspect = c.aspectRatio;
+    return d;
+}
+
+DevSettings to_dev(const shiro_hip_settings& s) {
+    DevSettings d;
+    d.width = s.width;
+    d.height = s.height;
+    d.domeLightSamples = s.domeLightSamples;
+    d.maxDepth = s.maxDepth;
+    d.diffuseDepth = s.diffuseDepth;
+    d.specularDepth = s.specularDepth;
+    d.backgroundVisible = s.backgroundVisible;
+    d.rrDepth = s.russianRouletteDepth;
+    return d;
+}
+
+template <typename T>
+hipError_t upload(DevBuf& b, const T* src, size_t count, hipStream_t s) {
+    hipError_t e = b.reserve(std::max<size_t>(count * sizeof(T), 16));
+    if (e != hipSuccess) return e;
+    if (count == 0) return hipSuccess;
+    e = hipMemcpyAsync(b.p, src, count * sizeof(T), hipMemcpyHostToDevice, s);
+    if (e != hipSuccess) return e;
+    return hipStreamSynchronize(s);
+}
+
+// Samples per launch: bound a launch to roughly 8M paths so that the cancel
+// flag is polled at a useful rate.
+uint32_t slice_samples(uint64_t pixels, uint32_t count) {
+    if (pixels == 0) return count;
+    const uint64_t per = std::max<uint64_t>(1, (8ull << 20) / pixels);
+    return static_cast<uint32_t>(std::min<uint64_t>(count, per));
+}
+
+int check_settings(shiro_hip_ctx* ctx, const shiro_hip_settings* s) {
+    if (!s) return fail(ctx, SHIRO_HIP_INVALID, "settings is null");
+    if (s->width == 0 || s->height == 0) return fail(ctx, SHIRO_HIP_INVALID, "zero-sized frame");
+    if (static_cast<uint64_t>(s->width) * s->height > (1ull << 31))
+        return fail(ctx, SHIRO_HIP_INVALID, "frame larger than 2^31 pixels");
+    return SHIRO_HIP_OK;
+}
+
+// Enqueues the slices of one batch; `count` > 0.  frame holds device
+// pointers.  If cancel is non-null the host flag is polled between slices.
+int enqueue_batch(shiro_hip_ctx* ctx, const shiro_hip_camera& cam, const shiro_hip_settings& st, uint32_t start,
+                  uint32_t count, const DevFrame& frame, bool countFetches, const volatile int32_t* cancel) {
+    const uint64_t np = static_cast<uint64_t>(st.width) * st.height;
+    const uint32_t slice = slice_samples(np, count);
+    RenderParams P{};
+    P.scene = ctx->dev;
+    P.camera = to_dev(cam);
+    P.settings = to_dev(st);
+    if (slice < count) {
+        HIPC(ctx->sumB.reserve(np * 16), "alloc sums");
+        HIPC(ctx->sumA.reserve(np * 16), "alloc sums");
+        HIPC(ctx->sumN.reserve(np * 16), "alloc sums");
+        HIPC(ctx->sumD.reserve(np * 4), "alloc sums");
+    }
+    P.sums = DevBatchSums{ctx->sumB.as<float4>(), ctx->sumA.as<float4>(), ctx->sumN.as<float4>(), ctx->sumD.as<float>()};
+    P.frame = frame;
+    P.sampleStart = start;
+    P.batchCount = count;
+    P.tilesX = (st.width + 7u) / 8u;
+    P.counters = countFetches ? ctx->counters.as<DevCounters>() : nullptr;
+    HIPC(hipEventRecord(ctx->ev0, ctx->stream), "event");
+    for (uint32_t b = 0; b < count; b += slice) {
+        if (cancel && *cancel) {
+            (void)hipStreamSynchronize(ctx->stream);
+            return fail(ctx, SHIRO_HIP_CANCELLED, "cancelled");
+        }
+        P.sliceBegin = b;
+        P.sliceCount = std::min(slice, count - b);
+        HIPC(launch_render(P, countFetches, ctx->stream), "render launch");
+        if (cancel) HIPC(hipStreamSynchronize(ctx->stream), "render");
+    }
+    HIPC(hipEventRecord(ctx->ev1, ctx->stream), "event");
+    return SHIRO_HIP_OK;
+}
+
+int finish_timing(shiro_hip_ctx* ctx) {
+    HIPC(hipEventSynchronize(ctx->ev1), "render");
+    float ms = 0.0f;
+    HIPC(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1), "timing");
+    ctx->lastMs = ms;
+    return SHIRO_HIP_OK;
+}
+
+void clear_host_frame(const shiro_hip_settings& st, shiro_hip_frame* out) {
+    const size_t np = static_cast<size_t>(st.width) * st.height;
+    if (out->beauty) std::fill(out->beauty, out->beauty + np * 4, 0.0f);
+    if (out->albedo) std::fill(out->albedo, out->albedo + np * 3, 0.0f);
+    if (out->normal) std::fill(out->normal, out->normal + np * 3, 0.0f);
+    if (out->depth) std::fill(out->depth, out->depth + np, std::numeric_limits<float>::infinity());
+}
+
+// Scratch device array for the single-stage entry points.
+struct Tmp {
+    DevBuf b;
+    template <typename T>
+    hipError_t in(const T* src, size_t n, hipStream_t s) {
+        hipError_t e = b.reserve(std::max<size_t>(n * sizeof(T), 16));
+        if (e != hipSuccess || n == 0) return e;
+        return hipMemcpyAsync(b.p, src, n * sizeof(T), hipMemcpyHostToDevice, s);
+    }
+    template <typename T>
+    hipError_t alloc(size_t n) {
+        return b.reserve(std::max<size_t>(n * sizeof(T), 16));
+    }
+    template <typename T>
+    hipError_t out(T* dst, size_t n, hipStream_t s) {
+        if (n == 0) return hipSuccess;
+        return hipMemcpyAsync(dst, b.p, n * sizeof(T), hipMemcpyDeviceToHost, s);
+    }
+};
+
+}  // namespace
+
+extern "C" {
+
+void shiro_hip_default_settings(shiro_hip_settings* s) {
+    std::memset(s, 0, sizeof(*s));
+    s->width = 1280;
+    s->height = 720;
+    s->samplesPerPixel = 32;
+    s->samplesPerUpdate = 1;
+    s->domeLightSamples = 1;
+    s->maxDepth = 3;
+    s->diffuseDepth = 2;
+    s->specularDepth = 2;
+    s->maxSubdivLevel = 2;
+    s->backend = 0;
+    s->backgroundVisible = 1;
+    s->enableHeadlight = 1;
+    s->enableCaustics = 1;
+    s->enableSss = 1;
+    s->russianRouletteDepth = 2;
+    s->transport = SHIRO_HIP_TRANSPORT_MEGAKERNEL;
+}
+
+void shiro_hip_default_material(shiro_hip_material* m) { *m = default_material(); }
+
+int shiro_hip_create(int device, shiro_hip_ctx** out) {
+    if (!out) return SHIRO_HIP_INVALID;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return SHIRO_HIP_NO_DEVICE;
+    if (device < 0 || device >= n) return SHIRO_HIP_INVALID;
+    auto* ctx = new shiro_hip_ctx();
+    ctx->device = device;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess ||
+        ctx->counters.reserve(sizeof(DevCounters)) != hipSuccess) {
+        delete ctx;
+        return SHIRO_HIP_NO_DEVICE;
+    }
+    *out = ctx;
+    return SHIRO_HIP_OK;
+}
+
+void shiro_hip_destroy(shiro_hip_ctx* ctx) {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
+    if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+const char* shiro_hip_last_error(const shiro_hip_ctx* ctx) { return ctx ? ctx->err.c_str() : "no context"; }
+
+int shiro_hip_get_status(const shiro_hip_ctx* ctx, shiro_hip_status* out) {
+    if (!out) return SHIRO_HIP_INVALID;
+    std::memset(out, 0, sizeof(*out));
+    out->kind = 1;
+    out->name = "hip";
+    out->usesGpu = 1;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return SHIRO_HIP_OK;
+    const int dev = ctx ? ctx->device : 0;
+    hipDeviceProp_t p;
+    if (hipGetDeviceProperties(&p, dev) != hipSuccess) return SHIRO_HIP_OK;
+    out->available = 1;
+    std::snprintf(out->deviceName, sizeof(out->deviceName), "%s", p.gcnArchName);
+    out->computeUnits = static_cast<uint32_t>(p.multiProcessorCount);
+    return SHIRO_HIP_OK;
+}
+
+int shiro_hip_supports_scene(const shiro_hip_scene* s) {
+    if (!s) return 0;
+    if (s->distantLightCount > static_cast<uint32_t>(kMaxDirLights)) return 0;
+    if (s->domeLightCount > static_cast<uint32_t>(kMaxDomes)) return 0;
+    // Emissive-surface lights are not sampled on the device yet.
+    for (uint32_t i = 0; i < s->meshCount; ++i) {
+        const uint32_t m = s->meshes[i].materialIndex;
+        if (m < s->materialCount && s->materials[m].emissionStrength > 0.0f && s->meshes[i].indexCount >= 3) return 0;
+    }
+    return 1;
+}
+
+int shiro_hip_upload_scene(shiro_hip_ctx* ctx, const shiro_hip_scene* s) {
+    if (!ctx) return SHIRO_HIP_INVALID;
+    if (!s) return fail(ctx, SHIRO_HIP_INVALID, "scene is null");
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    HIPC(hipSetDevice(ctx->device), "set device");
+    ctx->hasScene = false;
+    const auto t0 = std::chrono::steady_clock::now();
+    FlatScene flat;
+    std::string err;
+    if (!flatten_scene(*s, &flat, &err)) return fail(ctx, SHIRO_HIP_INVALID, err);
+    const size_t nt = flat.material.size();
+    Bvh4 bvh;
+    if (!build_bvh4(flat.tri.data(), nt, kMaxBvhDepth, &bvh, &err)) return fail(ctx, SHIRO_HIP_INVALID, err);
+    if (!validate_bvh4(bvh, nt, kMaxBvhDepth, &err)) return fail(ctx, SHIRO_HIP_INVALID, "BVH check: " + err);
+    const auto t1 = std::chrono::steady_clock::now();
+    // Triangles and hit data in leaf order.
+    std::vector<float> tris(nt * 12), hit(nt * 12);
+    for (size_t slot = 0; slot < nt; ++slot) {
+        const uint32_t g = bvh.order[slot];
+        float* t = &tris[slot * 12];
+        float* h = &hit[slot * 12];
+        const float* v = &flat.tri[static_cast<size_t>(g) * 9];
+        const float* n = &flat.nrm[static_cast<size_t>(g) * 9];
+        const uint32_t endFlag = bvh.leafEnd[slot] ? kLeafEndFlag : 0u;
+        const uint32_t flags = flat.hasNormals[g] ? kTriHasNormals : 0u;
+        for (int k = 0; k < 3; ++k) {
+            std::memcpy(t + 4 * k, v + 3 * k, 12);
+            std::memcpy(h + 4 * k, n + 3 * k, 12);
+        }
+        std::memcpy(t + 3, &g, 4);
+        std::memcpy(t + 7, &endFlag, 4);
+        t[11] = 0.0f;
+        std::memcpy(h + 3, &flat.material[g], 4);
+        std::memcpy(h + 7, &flags, 4);
+        h[11] = 0.0f;
+    }
+    // Materials (+ the default material used for out-of-range indices).
+    std::vector<shiro_hip_material> mats(s->materials, s->materials + s->materialCount);
+    mats.push_back(default_material());
+    std::vector<MaterialDerived> der(mats.size());
+    for (size_t i = 0; i < mats.size(); ++i) der[i] = derive_material(mats[i]);
+    HIPC(upload(ctx->nodes, bvh.nodes.data(), bvh.nodes.size(), ctx->stream), "upload nodes");
+    HIPC(upload(ctx->tris, tris.data(), tris.size(), ctx->stream), "upload triangles");
+    HIPC(upload(ctx->hitdata, hit.data(), hit.size(), ctx->stream), "upload hit data");
+    HIPC(upload(ctx->materials, mats.data(), mats.size(), ctx->stream), "upload materials");
+    HIPC(upload(ctx->derived, der.data(), der.size(), ctx->stream), "upload materials");
+    DevScene d{};
+    d.nodes = ctx->nodes.as<float4>();
+    d.tris = ctx->tris.as<float4>();
+    d.hitdata = ctx->hitdata.as<float4>();
+    d.materials = ctx->materials.as<shiro_hip_material>();
+    d.derived = ctx->derived.as<MaterialDerived>();
+    d.nodeCount = static_cast<uint32_t>(bvh.nodeCount());
+    d.triCount = static_cast<uint32_t>(nt);
+    d.materialCount = static_cast<uint32_t>(mats.size());
+    uint64_t bytes = ctx->nodes.bytes + ctx->tris.bytes + ctx->hitdata.bytes + ctx->materials.bytes + ctx->derived.bytes;
+    // Lights.
+    d.dirCount = s->distantLightCount;
+    for (uint32_t i = 0; i < s->distantLightCount; ++i) {
+        std::memcpy(d.dirs[i].direction, s->distantLights[i].direction, 12);
+        std::memcpy(d.dirs[i].radiance, s->distantLights[i].radiance, 12);
+    }
+    d.domeCount = s->domeLightCount;
+    for (uint32_t i = 0; i < s->domeLightCount; ++i) {
+        const shiro_hip_dome_light& L = s->domeLights[i];
+        DevDome& D = d.domes[i];
+        std::memcpy(D.radiance, L.radiance, 12);
+        std::memcpy(D.right, L.right, 12);
+        std::memcpy(D.up, L.up, 12);
+        std::memcpy(D.forward, L.forward, 12);
+        D.env = -1;
+        if (L.pixels && L.width > 0 && L.height > 0) {
+            const uint32_t layout = L.layout == SHIRO_HIP_ENV_ANGULAR ? SHIRO_HIP_ENV_ANGULAR : SHIRO_HIP_ENV_LATLONG;
+            const size_t np = static_cast<size_t>(L.width) * L.height;
+            std::vector<float> tex(np * 4);
+            for (size_t p = 0; p < np; ++p) {
+                tex[4 * p] = L.pixels[3 * p];
+                tex[4 * p + 1] = L.pixels[3 * p + 1];
+                tex[4 * p + 2] = L.pixels[3 * p + 2];
+                tex[4 * p + 3] = 0.0f;
+            }
+            std::vector<float> row, cond;
+            const bool table = build_env_tables(L.pixels, L.width, L.height, layout, &row, &cond);
+            HIPC(upload(ctx->env[i].texels, tex.data(), tex.size(), ctx->stream), "upload env");
+            if (table) {
+                HIPC(upload(ctx->env[i].row, row.data(), row.size(), ctx->stream), "upload env");
+                HIPC(upload(ctx->env[i].cond, cond.data(), cond.size(), ctx->stream), "upload env");
+            }
+            DevEnv& E = d.envs[i];
+            E.texels = ctx->env[i].texels.as<float4>();
+            E.rowCdf = table ? ctx->env[i].row.as<float>() : nullptr;
+            E.condCdf = table ? ctx->env[i].cond.as<float>() : nullptr;
+            E.w = L.width;
+            E.h = L.height;
+            E.layout = layout;
+            E.hasTable = table ? 1u : 0u;
+            D.env = static_cast<int32_t>(i);
+            bytes += ctx->env[i].texels.bytes + (table ? ctx->env[i].row.bytes + ctx->env[i].cond.bytes : 0);
+        }
+    }
+    std::memcpy(d.envTop, s->environmentTop, 12);
+    std::memcpy(d.envBottom, s->environmentBottom, 12);
+    ctx->dev = d;
+    ctx->emissive = flat.emissive;
+    const auto t2 = std::chrono::steady_clock::now();
+    ctx->info.triangleCount = nt;
+    ctx->info.bvhNodeCount = bvh.nodeCount();
+    ctx->info.bvhLeafCount = bvh.leafCount;
+    ctx->info.bvhDepth = bvh.depth;
+    ctx->info.maxLeafSize = bvh.maxLeafSize;
+    ctx->info.deviceBytes = bytes;
+    ctx->info.buildSeconds = std::chrono::duration<double>(t1 - t0).count();
+    ctx->info.uploadSeconds = std::chrono::duration<double>(t2 - t1).count();
+    ctx->hasScene = true;
+    return SHIRO_HIP_OK;
+}
+
+int shiro_hip_scene_info_get(const shiro_hip_ctx* ctx, shiro_hip_scene_info* out) {
+    if (!ctx || !out) return SHIRO_HIP_INVALID;
+    if (!ctx->hasScene) return SHIRO_HIP_NO_SCENE;
+    *out = ctx->info;
+    return SHIRO_HIP_OK;
+}
+
+int shiro_hip_render_batch(shiro_hip_ctx* ctx, const shiro_hip_camera* camera, const shiro_hip_settings* settings,
+                           uint32_t sampleStart, uint32_t sampleCount, const volatile int32_t* cancel,
+                           shiro_hip_frame* out) {
+    if (!ctx) return SHIRO_HIP_INVALID;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    if (!camera || !out || !out->beauty || !out->albedo || !out->normal || !out->depth)
+        return fail(ctx, SHIRO_HIP_INVALID, "null camera or frame buffer");
+    int rc = check_settings(ctx, settings);
+    if (rc) return rc;
+    clear_host_frame(*settings, out);
+    if (!ctx->hasScene) return fail(ctx, SHIRO_HIP_NO_SCENE, "no scene uploaded");
+    if (sampleCount == 0) return SHIRO_HIP_OK;
+    HIPC(hipSetDevice(ctx->device), "set device");
+    const uint64_t np = static_cast<uint64_t>(settings->width) * settings->height;
+    HIPC(ctx->frB.reserve(np * 16), "alloc frame");
+    HIPC(ctx->frA.reserve(np * 12), "alloc frame");
+    HIPC(ctx->frN.reserve(np * 12), "alloc frame");
+    HIPC(ctx->frD.reserve(np * 4), "alloc frame");
+    const DevFrame f{ctx->frB.as<float>(), ctx->frA.as<float>(), ctx->frN.as<float>(), ctx->frD.as<float>()};
+    rc = enqueue_batch(ctx, *camera, *settings, sampleStart, sampleCount, f, false, cancel);
+    if (rc) {
+        clear_host_frame(*settings, out);
+        return rc;
+    }
+    rc = finish_timing(ctx);
+    if (rc) return rc;
+    HIPC(hipMemcpyAsync(out->beauty, f.beauty, np * 16, hipMemcpyDeviceToHost, ctx->stream), "readback");
+    HIPC(hipMemcpyAsync(out->albedo, f.albedo, np * 12, hipMemcpyDeviceToHost, ctx->stream), "readback");
+    HIPC(hipMemcpyAsync(out->normal, f.normal, np * 12, hipMemcpyDeviceToHost, ctx->stream), "readback");
+    HIPC(hipMemcpyAsync(out->depth, f.depth, np * 4, hipMemcpyDeviceToHost, ctx->stream), "readback");
+    HIPC(hipStreamSynchronize(ctx->stream), "readback");
+    return SHIRO_HIP_OK;
+}
+
+int shiro_hip_render_batch_device(shiro_hip_ctx* ctx, const shiro_hip_camera* camera,
+                                  const shiro_hip_settings* settings, uint32_t sampleStart, uint32_t sampleCount,
+                                  shiro_hip_frame* out, int synchronize) {
+    if (!ctx) return SHIRO_HIP_INVALID;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    if (!camera || !out || !out->beauty || !out->albedo || !out->normal || !out->depth)
+        return fail(ctx, SHIRO_HIP_INVALID, "null camera or frame buffer");
+    int rc = check_settings(ctx, settings);
+    if (rc) return rc;
+    if (!ctx->hasScene) return fail(ctx, SHIRO_HIP_NO_SCENE, "no scene uploaded");
+    if (sampleCount == 0) return SHIRO_HIP_OK;
+    HIPC(hipSetDevice(ctx->device), "set device");
+    const DevFrame f{out->beauty, out->albedo, out->normal, out->depth};
+    rc = enqueue_batch(ctx, *camera, *settings, sampleStart, sampleCount, f, false, nullptr);
+    if (rc) return rc;
+    if (synchronize) return finish_timing(ctx);
+    return SHIRO_HIP_OK;
+}
+
+int shiro_hip_accum_reset(shiro_hip_ctx* ctx, uint32_t width, uint32_t height) {
+    if (!ctx) return SHIRO_HIP_INVALID;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    HIPC(hipSetDevice(ctx->device), "set device");
+    const uint64_t np = static_cast<uint64_t>(width) * height;
+    HIPC(ctx->acB.reserve(std::max<uint64_t>(np * 16, 16)), "alloc accumulator");
+    HIPC(ctx->acA.reserve(std::max<uint64_t>(np * 12, 16)), "alloc accumulator");
+    HIPC(ctx->acN.reserve(std::max<uint64_t>(np * 12, 16)), "alloc accumulator");
+    HIPC(ctx->acD.reserve(std::max<uint64_t>(np * 4, 16)), "alloc accumulator");
+    HIPC(ctx->acC.reserve(std::max<uint64_t>(np * 4, 16)), "alloc accumulator");
+    HIPC(hipMemsetAsync(ctx->acB.p, 0, np * 16, ctx->stream), "clear accumulator");
+    HIPC(hipMemsetAsync(ctx->acA.p, 0, np * 12, ctx->stream), "clear accumulator");
+    HIPC(hipMemsetAsync(ctx->acN.p, 0, np * 12, ctx->stream), "clear accumulator");
+    HIPC(hipMemsetAsync(ctx->acD.p, 0, np * 4, ctx->stream), "clear accumulator");
+    HIPC(hipMemsetAsync(ctx->acC.p, 0, np * 4, ctx->stream), "clear accumulator");
+    HIPC(hipStreamSynchronize(ctx->stream), "clear accumulator");
+    ctx->acW = width;
+    ctx->acH = height;
+    ctx->acTotal = 0;
+    return SHIRO_HIP_OK;
+}
+
+int shiro_hip_accum_add(shiro_hip_ctx* ctx, const shiro_hip_frame* batch, uint32_t sampleCount) {
+    if (!ctx) return SHIRO_HIP_INVALID;
+    if (!batch || batch->width == 0 || batch->height == 0 || sampleCount == 0) return SHIRO_HIP_OK;
+    if (batch->width != ctx->acW || batch->height != ctx->acH) {
+        // FrameAccumulator resets on a size change (Renderer.cpp:65-67).
+        const int rc = shiro_hip_accum_reset(ctx, batch->width, batch->height);
+        if (rc) return rc;
+    }
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    const AccumParams A{ctx->acB.as<float>(), ctx->acA.as<float>(), ctx->acN.as<float>(), ctx->acD.as<float>(),
+                        ctx->acC.as<uint32_t>()};
+    const DevFrame f{batch->beauty, batch->albedo, batch->normal, batch->depth};
+    HIPC(launch_accum_add(A, f, sampleCount, ctx->acW * ctx->acH, ctx->stream), "accumulate");
+    ctx->acTotal += sampleCount;
+    return SHIRO_HIP_OK;
+}
+
+int shiro_hip_accum_resolve(shiro_hip_ctx* ctx, shiro_hip_frame* out) {
+    if (!ctx || !out) return SHIRO_HIP_INVALID;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    if (out->width != ctx->acW || out->height != ctx->acH) return fail(ctx, SHIRO_HIP_INVALID, "frame size mismatch");
+    const AccumParams A{ctx->acB.as<float>(), ctx->acA.as<float>(), ctx->acN.as<float>(), ctx->acD.as<float>(),
+                        ctx->acC.as<uint32_t>()};
+    const DevFrame f{out->beauty, out->albedo, out->normal, out->depth};
+    HIPC(launch_accum_resolve(A, f, ctx->acTotal, ctx->acW * ctx->acH, ctx->stream), "resolve");
+    HIPC(hipStreamSynchronize(ctx->stream), "resolve");
+    return SHIRO_HIP_OK;
+}
+
+void* shiro_hip_stream(shiro_hip_ctx* ctx) { return ctx ? static_cast<void*>(ctx->stream) : nullptr; }
+
+int shiro_hip_synchronize(shiro_hip_ctx* ctx) {
+    if (!ctx) return SHIRO_HIP_INVALID;
+    HIPC(hipStreamSynchronize(ctx->stream), "synchronize");
+    return SHIRO_HIP_OK;
+}
+
+int shiro_hip_last_render_ms(const shiro_hip_ctx* ctx, float* ms) {
+    if (!ctx || !ms) return SHIRO_HIP_INVALID;
+    *ms = ctx->lastMs;
+    return SHIRO_HIP_OK;
+}
+
+int shiro_hip_qmc_fill(shiro_hip_ctx* ctx, uint32_t x0, uint32_t y0, uint32_t w, uint32_t h, uint32_t sampleIndex,
+                       uint32_t dim0, uint32_t dimCount, float* out) {
+    if (!ctx || !out) return SHIRO_HIP_INVALID;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    HIPC(hipSetDevice(ctx->device), "set device");
+    const size_t n = static_cast<size_t>(w) * h * dimCount;
+    Tmp o;
+    HIPC(o.alloc<float>(n), "alloc");
+    HIPC(launch_qmc_fill(x0, y0, w, h, sampleIndex, dim0, dimCount, o.b.as<float>(), ctx->stream), "qmc launch");
+    HIPC(o.out(out, n, ctx->stream), "readback");
+    HIPC(hipStreamSynchronize(ctx->stream), "qmc");
+    return SHIRO_HIP_OK;
+}
+
+int shiro_hip_intersect(shiro_hip_ctx* ctx, const float* rays, uint32_t n, uint32_t* prim, float* t, float* uv) {
+    if (!ctx || !rays || !prim || !t || !uv) return SHIRO_HIP_INVALID;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    if (!ctx->hasScene) return fail(ctx, SHIRO_HIP_NO_SCENE, "no scene uploaded");
+    HIPC(hipSetDevice(ctx->device), "set device");
+    Tmp r, p, tt, u;
+    HIPC(r.in(rays, static_cast<size_t>(n) * 8, ctx->stream), "upload rays");
+    HIPC(p.alloc<uint32_t>(n), "alloc");
+    HIPC(tt.alloc<float>(n), "alloc");
+    HIPC(u.alloc<float>(2ull * n), "alloc");
+    HIPC(launch_intersect(ctx->dev, r.b.as<float>(), n, p.b.as<uint32_t>(), tt.b.as<float>(), u.b.as<float>(),
+                          ctx->stream),
+         "intersect launch");
+    HIPC(p.out(prim, n, ctx->stream), "readback");
+    HIPC(tt.out(t, n, ctx->stream), "readback");
+    HIPC(u.out(uv, 2ull * n, ctx->stream), "readback");
+    HIPC(hipStreamSynchronize(ctx->stream), "intersect");
+    return SHIRO_HIP_OK;
+}
+
+int shiro_hip_occluded(shiro_hip_ctx* ctx, const float* rays, uint32_t n, uint8_t* out) {
+    if (!ctx || !rays || !out) return SHIRO_HIP_INVALID;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    if (!ctx->hasScene) return fail(ctx, SHIRO_HIP_NO_SCENE, "no scene uploaded");
+    HIPC(hipSetDevice(ctx->device), "set device");
+    Tmp r, o;
+    HIPC(r.in(rays, static_cast<size_t>(n) * 8, ctx->stream), "upload rays");
+    HIPC(o.alloc<uint8_t>(n), "alloc");
+    HIPC(launch_occluded(ctx->dev, r.b.as<float>(), n, o.b.as<uint8_t>(), ctx->stream), "occluded launch");
+    HIPC(o.out(out, n, ctx->stream), "readback");
+    HIPC(hipStreamSynchronize(ctx->stream), "occluded");
+    return SHIRO_HIP_OK;
+}
+
+int shiro_hip_trace_primary(shiro_hip_ctx* ctx, const shiro_hip_camera* camera, const shiro_hip_settings* settings,
+                            uint32_t sampleIndex, uint32_t* prim, float* t) {
+    if (!ctx || !camera || !prim || !t) return SHIRO_HIP_INVALID;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    int rc = check_settings(ctx, settings);
+    if (rc) return rc;
+    if (!ctx->hasScene) return fail(ctx, SHIRO_HIP_NO_SCENE, "no scene uploaded");
+    HIPC(hipSetDevice(ctx->device), "set device");
+    const size_t n = static_cast<size_t>(settings->width) * settings->height;
+    Tmp p, tt;
+    HIPC(p.alloc<uint32_t>(n), "alloc");
+    HIPC(tt.alloc<float>(n), "alloc");
+    HIPC(launch_trace_primary(ctx->dev, to_dev(*camera), to_dev(*settings), sampleIndex, p.b.as<uint32_t>(),
+                              tt.b.as<float>(), ctx->stream),
+         "primary launch");
+    HIPC(p.out(prim, n, ctx->stream), "readback");
+    HIPC(tt.out(t, n, ctx->stream), "readback");
+    HIPC(hipStreamSynchronize(ctx->stream), "primary");
+    return SHIRO_HIP_OK;
+}
+
+int shiro_hip_eval_bsdf(shiro_hip_ctx* ctx, const uint32_t* materialIndex, const float* normal, const float* coatNormal,
+                        const float* view, const float* light, uint32_t n, float* value, float* pdf) {
+    if (!ctx || !materialIndex || !normal || !coatNormal || !view || !light || !value || !pdf) return SHIRO_HIP_INVALID;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    if (!ctx->hasScene) return fail(ctx, SHIRO_HIP_NO_SCENE, "no scene uploaded");
+    for (uint32_t i = 0; i < n; ++i)
+        if (materialIndex[i] >= ctx->dev.materialCount) return fail(ctx, SHIRO_HIP_INVALID, "material index out of range");
+    HIPC(hipSetDevice(ctx->device), "set device");
+    Tmp mi, a, b, c, d, v, p;
+    HIPC(mi.in(materialIndex, n, ctx->stream), "upload");
+    HIPC(a.in(normal, 3ull * n, ctx->stream), "upload");
+    HIPC(b.in(coatNormal, 3ull * n, ctx->stream), "upload");
+    HIPC(c.in(view, 3ull * n, ctx->stream), "upload");
+    HIPC(d.in(light, 3ull * n, ctx->stream), "upload");
+    HIPC(v.alloc<float>(3ull * n), "alloc");
+    HIPC(p.alloc<float>(n), "alloc");
+    HIPC(launch_eval_bsdf(ctx->dev, mi.b.as<uint32_t>(), a.b.as<float>(), b.b.as<float>(), c.b.as<float>(),
+                          d.b.as<float>(), n, v.b.as<float>(), p.b.as<float>(), ctx->stream),
+         "bsdf launch");
+    HIPC(v.out(value, 3ull * n, ctx->stream), "readback");
+    HIPC(p.out(pdf, n, ctx->stream), "readback");
+    HIPC(hipStreamSynchronize(ctx->stream), "bsdf");
+    return SHIRO_HIP_OK;
+}
+
+int shiro_hip_env_sample(shiro_hip_ctx* ctx, uint32_t domeIndex, const float* u, uint32_t n, float* direction,
+                         float* pdf, float* radiance) {
+    if (!ctx || !u || !direction || !pdf || !radiance) return SHIRO_HIP_INVALID;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    if (!ctx->hasScene) return fail(ctx, SHIRO_HIP_NO_SCENE, "no scene uploaded");
+    if (domeIndex >= ctx->dev.domeCount || ctx->dev.domes[domeIndex].env < 0)
+        return fail(ctx, SHIRO_HIP_INVALID, "dome has no environment map");
+    HIPC(hipSetDevice(ctx->device), "set device");
+    Tmp uu, d, p, r;
+    HIPC(uu.in(u, 2ull * n, ctx->stream), "upload");
+    HIPC(d.alloc<float>(3ull * n), "alloc");
+    HIPC(p.alloc<float>(n), "alloc");
+    HIPC(r.alloc<float>(3ull * n), "alloc");
+    HIPC(launch_env_sample(ctx->dev.envs[ctx->dev.domes[domeIndex].env], uu.b.as<float>(), n, d.b.as<float>(),
+                           p.b.as<float>(), r.b.as<float>(), ctx->stream),
+         "env launch");
+    HIPC(d.out(direction, 3ull * n, ctx->stream), "readback");
+    HIPC(p.out(pdf, n, ctx->stream), "readback");
+    HIPC(r.out(radiance, 3ull * n, ctx->stream), "readback");
+    HIPC(hipStreamSynchronize(ctx->stream), "env");
+    return SHIRO_HIP_OK;
+}
+
+int shiro_hip_env_eval(shiro_hip_ctx* ctx, uint32_t domeIndex, const float* direction, uint32_t n, float* pdf,
+                       float* radiance) {
+    if (!ctx || !direction || !pdf || !radiance) return SHIRO_HIP_INVALID;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    if (!ctx->hasScene) return fail(ctx, SHIRO_HIP_NO_SCENE, "no scene uploaded");
+    if (domeIndex >= ctx->dev.domeCount || ctx->dev.domes[domeIndex].env < 0)
+        return fail(ctx, SHIRO_HIP_INVALID, "dome has no environment map");
+    HIPC(hipSetDevice(ctx->device), "set device");
+    Tmp d, p, r;
+    HIPC(d.in(direction, 3ull * n, ctx->stream), "upload");
+    HIPC(p.alloc<float>(n), "alloc");
+    HIPC(r.alloc<float>(3ull * n), "alloc");
+    HIPC(launch_env_eval(ctx->dev.envs[ctx->dev.domes[domeIndex].env], d.b.as<float>(), n, p.b.as<float>(),
+                         r.b.as<float>(), ctx->stream),
+         "env launch");
+    HIPC(p.out(pdf, n, ctx->stream), "readback");
+    HIPC(r.out(radiance, 3ull * n, ctx->stream), "readback");
+    HIPC(hipStreamSynchronize(ctx->stream), "env");
+    return SHIRO_HIP_OK;
+}
+
+int shiro_hip_count_fetches(shiro_hip_ctx* ctx, const shiro_hip_camera* camera, const shiro_hip_settings* settings,
+                            uint32_t sampleStart, uint32_t sampleCount, shiro_hip_fetch_counts* out) {
+    if (!ctx || !camera || !out) return SHIRO_HIP_INVALID;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    int rc = check_settings(ctx, settings);
+    if (rc) return rc;
+    if (!ctx->hasScene) return fail(ctx, SHIRO_HIP_NO_SCENE, "no scene uploaded");
+    std::memset(out, 0, sizeof(*out));
+    if (sampleCount == 0) return SHIRO_HIP_OK;
+    HIPC(hipSetDevice(ctx->device), "set device");
+    const uint64_t np = static_cast<uint64_t>(settings->width) * settings->height;
+    HIPC(ctx->frB.reserve(np * 16), "alloc frame");
+    HIPC(ctx->frA.reserve(np * 12), "alloc frame");
+    HIPC(ctx->frN.reserve(np * 12), "alloc frame");
+    HIPC(ctx->frD.reserve(np * 4), "alloc frame");
+    HIPC(hipMemsetAsync(ctx->counters.p, 0, sizeof(DevCounters), ctx->stream), "clear counters");
+    const DevFrame f{ctx->frB.as<float>(), ctx->frA.as<float>(), ctx->frN.as<float>(), ctx->frD.as<float>()};
+    rc = enqueue_batch(ctx, *camera, *settings, sampleStart, sampleCount, f, true, nullptr);
+    if (rc) return rc;
+    DevCounters c;
+    HIPC(hipMemcpyAsync(&c, ctx->counters.p, sizeof(c), hipMemcpyDeviceToHost, ctx->stream), "readback");
+    HIPC(hipStreamSynchronize(ctx->stream), "count");
+    out->samples = c.samples;
+    out->rays = c.rays;
+    out->shadowRays = c.shadowRays;
+    out->nodeFetches = c.nodeFetches;
+    out->triFetches = c.triFetches;
+    out->envTaps = c.envTaps;
+    out->cdfSteps = c.cdfSteps;
+    out->hitFetches = c.hitFetches;
+    return SHIRO_HIP_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,296 @@
+// BVH4 traversal on gfx950 with a watertight ray/triangle test.
+//
+// Replaces Embree's rtcIntersect1 / rtcOccluded1 (reference
+// CpuPathTracer.cpp:601-681) and the brute-force fallback (:386-475).
+//  - Nodes are 128 B (one L2 line): the four child boxes in SoA form, child
+//    words and leaf counts (layout.h).  A thread loads a node with eight
+//    16-byte loads.
+//  - Closest hit: front-to-back ordered traversal; the hit children of a
+//    node are sorted by entry distance with a five-comparator network, the
+//    nearest is visited next and the rest are pushed (farthest first) with
+//    their entry distances, so popped entries beyond the current hit are
+//    skipped.  Exact ties in distance resolve to the lower global triangle
+//    index, which is the first-in-scene-order rule of the reference's scan.
+//  - Any hit: unordered, returns at the first accepted triangle.
+//  - The triangle test is the watertight test of Woop, Benthin and Wald
+//    (JCGT 2013): shear to the ray's dominant axis, edge functions U, V, W
+//    (recomputed in fp64 when one is exactly zero), so rays through shared
+//    edges and vertices of a closed mesh cannot slip through.
+//  - The stack holds kStackSize entries per thread; the uploader rejects
+//    trees deeper than kMaxBvhDepth so that it cannot overflow.
+#pragma once
+
+#include "dev_math.h"
+#include "layout.h"
+
+namespace shiro_hip {
+
+struct Ray {
+    V3 o, d;
+    float tmin, tmax;
+};
+
+// Per-ray constants of the watertight test and the slab test.
+struct RayPre {
+    V3 o;
+    V3 inv;            // 1/d with zero components replaced by tiny values
+    int kx, ky, kz;
+    float Sx, Sy, Sz;
+    float tmin;
+};
+
+DEV float comp(V3 v, int k) { return k == 0 ? v.x : (k == 1 ? v.y : v.z); }
+
+DEV RayPre precompute(const Ray& r) {
+    RayPre p;
+    p.o = r.o;
+    const float ax = fabsf(r.d.x), ay = fabsf(r.d.y), az = fabsf(r.d.z);
+    int kz = (ax >= ay && ax >= az) ? 0 : (ay >= az ? 1 : 2);
+    int kx = kz == 2 ? 0 : kz + 1;
+    int ky = kx == 2 ? 0 : kx + 1;
+    const float dz = comp(r.d, kz);
+    if (dz < 0.0f) {
+        const int t = kx;
+        kx = ky;
+        ky = t;
+    }
+    p.kx = kx;
+    p.ky = ky;
+    p.kz = kz;
+    p.Sx = comp(r.d, kx) / dz;
+    p.Sy = comp(r.d, ky) / dz;
+    p.Sz = 1.0f / dz;
+    const float tiny = 1.0e-30f;
+    p.inv = V3{1.0f / (ax > tiny ? r.d.x : copysignf(tiny, r.d.x)), 1.0f / (ay > tiny ? r.d.y : copysignf(tiny, r.d.y)),
+               1.0f / (az > tiny ? r.d.z : copysignf(tiny, r.d.z))};
+    p.tmin = r.tmin;
+    return p;
+}
+
+// Watertight test; on success t in [tmin, tmax] and barycentrics b1, b2 of
+// vertices 1 and 2.
+DEV bool tri_test(const RayPre& r, V3 v0, V3 v1, V3 v2, float tmax, float* tOut, float* b1Out, float* b2Out) {
+    const V3 A = v0 - r.o, B = v1 - r.o, C = v2 - r.o;
+    const float Akz = comp(A, r.kz), Bkz = comp(B, r.kz), Ckz = comp(C, r.kz);
+    const float Ax = comp(A, r.kx) - r.Sx * Akz;
+    const float Ay = comp(A, r.ky) - r.Sy * Akz;
+    const float Bx = comp(B, r.kx) - r.Sx * Bkz;
+    const float By = comp(B, r.ky) - r.Sy * Bkz;
+    const float Cx = comp(C, r.kx) - r.Sx * Ckz;
+    const float Cy = comp(C, r.ky) - r.Sy * Ckz;
+    float U = Cx * By - Cy * Bx;
+    float V = Ax * Cy - Ay * Cx;
+    float W = Bx * Ay - By * Ax;
+    if (U == 0.0f || V == 0.0f || W == 0.0f) {
+        U = static_cast<float>(static_cast<double>(Cx) * By - static_cast<double>(Cy) * Bx);
+        V = static_cast<float>(static_cast<double>(Ax) * Cy - static_cast<double>(Ay) * Cx);
+        W = static_cast<float>(static_cast<double>(Bx) * Ay - static_cast<double>(By) * Ax);
+    }
+    if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f)) return false;
+    const float det = U + V + W;
+    if (det == 0.0f) return false;
+    const float T = U * (r.Sz * Akz) + V * (r.Sz * Bkz) + W * (r.Sz * Ckz);
+    const float inv = 1.0f / det;
+    const float t = T * inv;
+    if (!(t >= r.tmin && t <= tmax)) return false;
+    *tOut = t;
+    *b1Out = V * inv;
+    *b2Out = W * inv;
+    return true;
+}
+
+struct HitRec {
+    float t;
+    float b1, b2;
+    uint32_t slot;   // leaf-order triangle slot, kNoHit on a miss
+    uint32_t prim;   // global triangle index
+};
+
+template <bool COUNT>
+struct TravCount {
+    unsigned long long* nodes;
+    unsigned long long* tris;
+    DEV void node() const {
+        if (COUNT) *nodes += 1;
+    }
+    DEV void tri() const {
+        if (COUNT) *tris += 1;
+    }
+};
+
+// Slab test of the four children of a node; returns a 4-bit hit mask and
+// the entry distances.  Near and far planes are chosen by the sign of the
+// ray direction, so an inverted box (lo = +inf, hi = -inf: an empty child
+// slot) always misses; the inverse direction is finite, so no NaN arises.
+DEV uint32_t node_boxes(const float4* __restrict__ nd, const RayPre& r, float tmax, float tn[4]) {
+    const float4 lox = nd[0], hix = nd[1], loy = nd[2], hiy = nd[3], loz = nd[4], hiz = nd[5];
+    const bool px = r.inv.x >= 0.0f, py = r.inv.y >= 0.0f, pz = r.inv.z >= 0.0f;
+    const float4 nx = px ? lox : hix, fx = px ? hix : lox;
+    const float4 ny = py ? loy : hiy, fy = py ? hiy : loy;
+    const float4 nz = pz ? loz : hiz, fz = pz ? hiz : loz;
+    const float ox = r.o.x, oy = r.o.y, oz = r.o.z;
+    const float ix = r.inv.x, iy = r.inv.y, iz = r.inv.z;
+    const float NX[4] = {nx.x, nx.y, nx.z, nx.w}, FX[4] = {fx.x, fx.y, fx.z, fx.w};
+    const float NY[4] = {ny.x, ny.y, ny.z, ny.w}, FY[4] = {fy.x, fy.y, fy.z, fy.w};
+    const float NZ[4] = {nz.x, nz.y, nz.z, nz.w}, FZ[4] = {fz.x, fz.y, fz.z, fz.w};
+    uint32_t mask = 0;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        const float t0 = fmaxf(fmaxf((NX[c] - ox) * ix, (NY[c] - oy) * iy), fmaxf((NZ[c] - oz) * iz, r.tmin));
+        const float t1 = fminf(fminf((FX[c] - ox) * ix, (FY[c] - oy) * iy), fminf((FZ[c] - oz) * iz, tmax));
+        tn[c] = t0;
+        mask |= (t0 <= t1 ? 1u : 0u) << c;
+    }
+    return mask;
+}
+
+DEV void cswap(float& ka, uint32_t& va, float& kb, uint32_t& vb) {
+    if (kb < ka) {
+        const float tk = ka;
+        ka = kb;
+        kb = tk;
+        const uint32_t tv = va;
+        va = vb;
+        vb = tv;
+    }
+}
+
+// Tests the triangles of the leaf starting at `slot` (the last one carries
+// the end-of-leaf flag in tris[3*slot+1].w) against the current best hit.
+template <bool COUNT>
+DEV void leaf_closest(const DevScene& S, const RayPre& r, uint32_t slot, HitRec& best, const TravCount<COUNT>& cnt) {
+    for (;;) {
+        const float4* tp = S.tris + static_cast<size_t>(slot) * 3u;
+        const float4 a = tp[0], b = tp[1], c = tp[2];
+        cnt.tri();
+        float t, b1, b2;
+        if (tri_test(r, xyz(a), xyz(b), xyz(c), best.t, &t, &b1, &b2)) {
+            const uint32_t prim = __float_as_uint(a.w);
+            // t <= best.t here; an exact tie goes to the lower index.
+            if (t < best.t || prim < best.prim) {
+                best.t = t;
+                best.b1 = b1;
+                best.b2 = b2;
+                best.slot = slot;
+                best.prim = prim;
+            }
+        }
+        if (__float_as_uint(b.w) & kLeafEndFlag) break;
+        ++slot;
+    }
+}
+
+template <bool COUNT>
+DEV HitRec intersect_closest(const DevScene& S, const Ray& ray, const TravCount<COUNT>& cnt) {
+    HitRec best;
+    best.t = ray.tmax;
+    best.b1 = best.b2 = 0.0f;
+    best.slot = kNoHit;
+    best.prim = kNoHit;
+    if (S.nodeCount == 0) return best;
+    const RayPre r = precompute(ray);
+    uint32_t stackNode[kStackSize];
+    float stackT[kStackSize];
+    int sp = 0;
+    uint32_t cur = 0u;  // the root is always an internal node
+    for (;;) {
+        if (cur & kLeafBit) {
+            leaf_closest(S, r, cur & ~kLeafBit, best, cnt);
+        } else {
+            const float4* nd = S.nodes + static_cast<size_t>(cur) * 8u;
+            cnt.node();
+            float tn[4];
+            const uint32_t mask = node_boxes(nd, r, best.t, tn);
+            if (mask) {
+                const float4 cw = nd[6];
+                const uint32_t word[4] = {__float_as_uint(cw.x), __float_as_uint(cw.y), __float_as_uint(cw.z),
+                                          __float_as_uint(cw.w)};
+                float k[4];
+                uint32_t v[4];
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    k[c] = (mask >> c) & 1u ? tn[c] : kInf;
+                    v[c] = word[c];
+                }
+                cswap(k[0], v[0], k[1], v[1]);
+                cswap(k[2], v[2], k[3], v[3]);
+                cswap(k[0], v[0], k[2], v[2]);
+                cswap(k[1], v[1], k[3], v[3]);
+                cswap(k[1], v[1], k[2], v[2]);
+                const int nh = __popc(mask);
+                // Push the farther hits (farthest first), descend into the nearest.
+                for (int i = nh - 1; i >= 1; --i) {
+                    stackNode[sp] = v[i];
+                    stackT[sp] = k[i];
+                    ++sp;
+                }
+                cur = v[0];
+                continue;
+            }
+        }
+        // Pop the nearest pending entry that can still beat the best hit.
+        bool found = false;
+        while (sp > 0) {
+            --sp;
+            if (stackT[sp] <= best.t) {
+                cur = stackNode[sp];
+                found = true;
+                break;
+            }
+        }
+        if (!found) break;
+    }
+    return best;
+}
+
+// Any hit in [tmin, tmax]: Occluded (CpuPathTracer.cpp:659-681, 697-709).
+template <bool COUNT>
+DEV bool intersect_any(const DevScene& S, const Ray& ray, const TravCount<COUNT>& cnt) {
+    if (S.nodeCount == 0) return false;
+    const RayPre r = precompute(ray);
+    uint32_t stackNode[kStackSize];
+    int sp = 0;
+    uint32_t cur = 0u;
+    for (;;) {
+        if (cur & kLeafBit) {
+            uint32_t slot = cur & ~kLeafBit;
+            for (;;) {
+                const float4* tp = S.tris + static_cast<size_t>(slot) * 3u;
+                const float4 a = tp[0], b = tp[1], c = tp[2];
+                cnt.tri();
+                float t, b1, b2;
+                if (tri_test(r, xyz(a), xyz(b), xyz(c), ray.tmax, &t, &b1, &b2)) return true;
+                if (__float_as_uint(b.w) & kLeafEndFlag) break;
+                ++slot;
+            }
+        } else {
+            const float4* nd = S.nodes + static_cast<size_t>(cur) * 8u;
+            cnt.node();
+            float tn[4];
+            const uint32_t mask = node_boxes(nd, r, ray.tmax, tn);
+            if (mask) {
+                const float4 cw = nd[6];
+                const uint32_t word[4] = {__float_as_uint(cw.x), __float_as_uint(cw.y), __float_as_uint(cw.z),
+                                          __float_as_uint(cw.w)};
+                int first = -1;
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    if ((mask >> c) & 1u) {
+                        if (first < 0) {
+                            first = c;
+                        } else {
+                            stackNode[sp++] = word[c];
+                        }
+                    }
+                }
+                cur = word[first];
+                continue;
+            }
+        }
+        if (sp == 0) break;
+        cur = stackNode[--sp];
+    }
+    return false;
+}
+
+}  // namespace shiro_hip

@@ -1,0 +1,254 @@
+// Host-side scene preparation; see host_scene.h.  The material expressions
+// are evaluated in the same fp32 order as Shiro's CPU path so that the
+// per-material constants uploaded to the device equal what the CPU path
+// computes inline (this file is built with -ffp-contract=off).
+#include "host_scene.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+namespace shiro_hip {
+
+namespace {
+
+constexpr float kPiH = 3.14159265358979323846f;
+
+struct H3 {
+    float x, y, z;
+};
+inline H3 h3(const float* p) { return {p[0], p[1], p[2]}; }
+inline H3 operator+(H3 a, H3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+inline H3 operator-(H3 a, H3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+inline H3 operator*(H3 a, float s) { return {a.x * s, a.y * s, a.z * s}; }
+inline H3 operator*(H3 a, H3 b) { return {a.x * b.x, a.y * b.y, a.z * b.z}; }
+inline float clampf(float v, float lo, float hi) { return std::clamp(v, lo, hi); }
+inline H3 clamp3(H3 v, float lo, float hi) { return {clampf(v.x, lo, hi), clampf(v.y, lo, hi), clampf(v.z, lo, hi)}; }
+inline H3 lerp(H3 a, H3 b, float t) { return a * (1.0f - t) + b * t; }
+inline float max3(H3 v) { return std::max(v.x, std::max(v.y, v.z)); }
+inline void st3(float* d, H3 v) {
+    d[0] = v.x;
+    d[1] = v.y;
+    d[2] = v.z;
+}
+
+float f0_scalar(float ior) {
+    const float i = std::max(ior, 1.0f);
+    const float r = (i - 1.0f) / (i + 1.0f);
+    return r * r;
+}
+
+H3 one_minus_clamped(H3 v) {
+    return {clampf(1.0f - v.x, 0.0f, 1.0f), clampf(1.0f - v.y, 0.0f, 1.0f), clampf(1.0f - v.z, 0.0f, 1.0f)};
+}
+
+// Isotropic AdjustedRoughness (CpuPathTracer.cpp:845-850).
+float sampling_roughness(float r, float aniso, float rot) {
+    const float a = clampf(aniso, -0.95f, 0.95f);
+    const float phase = std::cos(rot * 2.0f * kPiH);
+    return clampf(r * (1.0f - 0.35f * a * phase), 0.02f, 1.0f);
+}
+
+}  // namespace
+
+shiro_hip_material default_material() {
+    shiro_hip_material m;
+    std::memset(&m, 0, sizeof(m));
+    auto set3 = [](float* d, float a, float b, float c) {
+        d[0] = a;
+        d[1] = b;
+        d[2] = c;
+    };
+    set3(m.baseColor, 0.8f, 0.8f, 0.8f);
+    set3(m.specularColor, 1.0f, 1.0f, 1.0f);
+    set3(m.transmissionColor, 1.0f, 1.0f, 1.0f);
+    set3(m.coatColor, 1.0f, 1.0f, 1.0f);
+    set3(m.subsurfaceColor, 1.0f, 1.0f, 1.0f);
+    set3(m.subsurfaceRadius, 1.0f, 1.0f, 1.0f);
+    set3(m.sheenColor, 1.0f, 1.0f, 1.0f);
+    m.baseWeight = 1.0f;
+    m.specularWeight = 1.0f;
+    m.roughness = 0.5f;
+    m.opacity = 1.0f;
+    m.coatRoughness = 0.03f;
+    m.coatIor = 1.5f;
+    m.sheenRoughness = 0.3f;
+    m.subsurfaceScale = 1.0f;
+    m.ior = 1.5f;
+    m.thinFilmIor = 1.5f;
+    m.baseColorTexture = -1;
+    m.roughnessTexture = -1;
+    return m;
+}
+
+MaterialDerived derive_material(const shiro_hip_material& m) {
+    MaterialDerived d;
+    std::memset(&d, 0, sizeof(d));
+    const float metal = clampf(m.metallic, 0.0f, 1.0f);
+    const float trans = clampf(m.transmission, 0.0f, 1.0f);
+    // DiffuseColor (:900-908).
+    const float sss = clampf(m.subsurface, 0.0f, 1.0f);
+    const float blend = clampf(max3(h3(m.subsurfaceRadius)) / 3.0f, 0.0f, 1.0f);
+    const H3 tint = lerp(h3(m.baseColor), h3(m.subsurfaceColor), blend);
+    const H3 base = lerp(h3(m.baseColor), tint, sss);
+    const H3 diffuse = base * (clampf(m.baseWeight, 0.0f, 1.0f) * (1.0f - metal) * (1.0f - trans));
+    st3(d.diffuseColor, diffuse);
+    // SpecularF0 / CoatF0 (:918-928).
+    const H3 diel = h3(m.specularColor) * (clampf(m.specularWeight, 0.0f, 1.0f) * f0_scalar(m.ior));
+    const H3 f0b = lerp(diel, h3(m.baseColor), metal);
+    st3(d.specularF0, f0b);
+    const H3 f0c = h3(m.coatColor) * (clampf(m.coatWeight, 0.0f, 1.0f) * f0_scalar(m.coatIor));
+    st3(d.coatF0, f0c);
+    // CoatUnderlayerTint (:1042-1045).
+    const float k = clampf(m.coatAffectColor * m.coatWeight, 0.0f, 1.0f);
+    const H3 ctint = lerp(H3{1.0f, 1.0f, 1.0f}, clamp3(h3(m.coatColor), 0.0f, 1.0f), k);
+    st3(d.coatTint, ctint);
+    // Lobe selection weights (:1376-1403).
+    const H3 uc = ctint * one_minus_clamped(f0c);
+    const H3 ub = one_minus_clamped(f0b);
+    d.wDiffuse = std::max(0.0f, max3(diffuse * uc * ub));
+    d.wBaseSpecular = std::max(0.0f, max3(uc * f0b));
+    d.wCoat = std::max(0.0f, max3(f0c));
+    if (trans <= 0.0f || metal >= 0.999f) {
+        d.wTransmission = 0.0f;
+    } else {
+        const H3 tt = clamp3(h3(m.transmissionColor), 0.0f, 1.0f);
+        d.wTransmission = std::max(0.0f, max3(tt * uc * ub)) * trans * (1.0f - metal);
+    }
+    // Roughnesses (:911-916, 930-932).
+    d.baseRoughness = clampf(m.roughness + m.coatAffectRoughness * m.coatWeight * m.coatRoughness, 0.0f, 1.0f);
+    d.transmissionRoughness = clampf(m.roughness + m.transmissionExtraRoughness, 0.0f, 1.0f);
+    d.sampleRoughnessBase = sampling_roughness(d.baseRoughness, m.specularAnisotropy, m.specularRotation);
+    d.sampleRoughnessCoat = sampling_roughness(m.coatRoughness, m.coatAnisotropy, m.coatRotation);
+    d.sampleRoughnessTrans = sampling_roughness(d.transmissionRoughness, m.specularAnisotropy, m.specularRotation);
+    // Emission (:1728-1733).
+    d.emission[0] = std::max(m.emissionColor[0] * m.emissionStrength, 0.0f);
+    d.emission[1] = std::max(m.emissionColor[1] * m.emissionStrength, 0.0f);
+    d.emission[2] = std::max(m.emissionColor[2] * m.emissionStrength, 0.0f);
+    // MediumExtinction (:1420-1437).
+    H3 ext{std::max(m.transmissionScatter[0], 0.0f), std::max(m.transmissionScatter[1], 0.0f),
+           std::max(m.transmissionScatter[2], 0.0f)};
+    if (!m.thinWalled && m.transmissionDepth > 0.0f) {
+        const float depth = std::max(m.transmissionDepth, 1.0e-3f);
+        const H3 t = clamp3(h3(m.transmissionColor), 1.0e-4f, 1.0f);
+        ext.x += -std::log(t.x) / depth;
+        ext.y += -std::log(t.y) / depth;
+        ext.z += -std::log(t.z) / depth;
+    }
+    st3(d.mediumExtinction, ext);
+    // TransmissionSurfaceTint (:1405-1410).
+    if (!m.thinWalled && m.transmissionDepth > 0.0f) {
+        st3(d.transmissionTint, H3{1.0f, 1.0f, 1.0f});
+    } else {
+        st3(d.transmissionTint, clamp3(h3(m.transmissionColor), 0.0f, 1.0f));
+    }
+    uint32_t f = 0;
+    if (clampf(m.specularAnisotropy, -0.95f, 0.95f) != 0.0f || clampf(m.coatAnisotropy, -0.95f, 0.95f) != 0.0f)
+        f |= kMatAnisotropic;
+    if (m.thinFilmThickness > 0.0f) f |= kMatThinFilm;
+    if (m.sheen > 0.0f) f |= kMatSheen;
+    if (m.hasNormalOverride) f |= kMatNormalOverride;
+    if (m.hasCoatNormalOverride) f |= kMatCoatNormalOverride;
+    if (m.hasTangentOverride) f |= kMatTangentOverride;
+    if (m.emissionStrength > 0.0f) f |= kMatEmissive;
+    if (clampf(m.opacity, 0.0f, 1.0f) < 0.999f) f |= kMatOpacity;
+    d.flags = f;
+    return d;
+}
+
+bool build_env_tables(const float* px, uint32_t w, uint32_t h, uint32_t layout, std::vector<float>* rowCdf,
+                      std::vector<float>* condCdf) {
+    rowCdf->clear();
+    condCdf->clear();
+    if (!px || w == 0 || h == 0 || layout != SHIRO_HIP_ENV_LATLONG) return false;
+    rowCdf->assign(h + 1u, 0.0f);
+    condCdf->assign(static_cast<size_t>(h) * (w + 1u), 0.0f);
+    float total = 0.0f;
+    for (uint32_t y = 0; y < h; ++y) {
+        const float theta = (static_cast<float>(y) + 0.5f) * kPiH / static_cast<float>(h);
+        const float st = std::max(std::sin(theta), 1.0e-6f);
+        float* row = condCdf->data() + static_cast<size_t>(y) * (w + 1u);
+        float sum = 0.0f;
+        for (uint32_t x = 0; x < w; ++x) {
+            const float* c = px + 3 * (static_cast<size_t>(y) * w + x);
+            const float lum = c[0] * 0.2126f + c[1] * 0.7152f + c[2] * 0.0722f;
+            sum += std::max(0.0f, lum) * st;
+            row[x + 1u] = sum;
+        }
+        if (sum > 0.0f) {
+            for (uint32_t x = 1; x <= w; ++x) row[x] /= sum;
+        } else {
+            for (uint32_t x = 1; x <= w; ++x) row[x] = static_cast<float>(x) / static_cast<float>(w);
+            sum = 1.0f;
+        }
+        total += sum;
+        (*rowCdf)[y + 1u] = total;
+    }
+    if (total <= 0.0f) {
+        for (uint32_t y = 1; y <= h; ++y) (*rowCdf)[y] = static_cast<float>(y) / static_cast<float>(h);
+        return true;
+    }
+    for (uint32_t y = 1; y <= h; ++y) (*rowCdf)[y] /= total;
+    return true;
+}
+
+bool flatten_scene(const shiro_hip_scene& s, FlatScene* out, std::string* err) {
+    *out = FlatScene();
+    if (s.materialCount > 0 && !s.materials) {
+        *err = "materials is null";
+        return false;
+    }
+    if (s.meshCount > 0 && !s.meshes) {
+        *err = "meshes is null";
+        return false;
+    }
+    if (s.distantLightCount > static_cast<uint32_t>(kMaxDirLights)) {
+        *err = "too many distant lights (max " + std::to_string(kMaxDirLights) + ")";
+        return false;
+    }
+    if (s.domeLightCount > static_cast<uint32_t>(kMaxDomes)) {
+        *err = "too many dome lights (max " + std::to_string(kMaxDomes) + ")";
+        return false;
+    }
+    size_t total = 0;
+    for (uint32_t i = 0; i < s.meshCount; ++i) total += s.meshes[i].indexCount / 3u;
+    out->tri.reserve(total * 9);
+    out->material.reserve(total);
+    out->hasNormals.reserve(total);
+    out->nrm.reserve(total * 9);
+    for (uint32_t mi = 0; mi < s.meshCount; ++mi) {
+        const shiro_hip_mesh& m = s.meshes[mi];
+        const uint32_t nt = m.indexCount / 3u;
+        if (nt == 0) continue;
+        if (!m.positions || !m.indices) {
+            *err = "mesh " + std::to_string(mi) + " has null positions or indices";
+            return false;
+        }
+        const bool normals = m.normals && m.normalCount == m.vertexCount;
+        const uint32_t mat = m.materialIndex < s.materialCount ? m.materialIndex : s.materialCount;
+        const shiro_hip_material mm = m.materialIndex < s.materialCount ? s.materials[m.materialIndex] : default_material();
+        if (mm.emissionStrength > 0.0f) out->emissive = true;
+        for (uint32_t p = 0; p < nt; ++p) {
+            const uint32_t* ix = m.indices + 3u * p;
+            for (int k = 0; k < 3; ++k) {
+                if (ix[k] >= m.vertexCount) {
+                    *err = "mesh " + std::to_string(mi) + " index out of range";
+                    return false;
+                }
+                const float* v = m.positions + 3u * ix[k];
+                out->tri.insert(out->tri.end(), v, v + 3);
+                if (normals) {
+                    const float* n = m.normals + 3u * ix[k];
+                    out->nrm.insert(out->nrm.end(), n, n + 3);
+                } else {
+                    out->nrm.insert(out->nrm.end(), {0.0f, 0.0f, 0.0f});
+                }
+            }
+            out->hasNormals.push_back(normals ? 1 : 0);
+            out->material.push_back(mat);
+        }
+    }
+    return true;
+}
+
+}  // namespace shiro_hip

@@ -1,0 +1,41 @@
+// Host-side scene preparation for the device layout (layout.h).
+#pragma once
+
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "layout.h"
+
+namespace shiro_hip {
+
+// Reference PbrMaterial defaults (Renderer.h:56-102).
+shiro_hip_material default_material();
+
+// Material-only constants with the reference's expressions
+// (CpuPathTracer.cpp:841-1441): lobe weights, F0s, tints, isotropic
+// sampling roughnesses, medium extinction.
+MaterialDerived derive_material(const shiro_hip_material& m);
+
+// EnvironmentMap::BuildImportanceTable (EnvironmentMap.cpp:185-236):
+// sequential fp32 prefix sums in the reference's order.  pixels is w*h RGB.
+// Returns false (tables empty) when the layout is not lat-long.
+bool build_env_tables(const float* pixels, uint32_t w, uint32_t h, uint32_t layout, std::vector<float>* rowCdf,
+                      std::vector<float>* condCdf);
+
+// Flattened triangle soup of a scene in scene order (meshes concatenated,
+// full index triples only, CpuPathTracer.cpp:434).
+struct FlatScene {
+    std::vector<float> tri;        // 9 floats per triangle
+    std::vector<float> nrm;        // 9 floats per triangle (vertex normals), when hasNormals
+    std::vector<uint8_t> hasNormals;
+    std::vector<uint32_t> material; // material index per triangle (default material = materialCount)
+    bool emissive = false;          // any emissive material in use
+};
+
+// Validates and flattens; returns false with a message on invalid input
+// (index out of range, null arrays, too many lights).
+bool flatten_scene(const shiro_hip_scene& s, FlatScene* out, std::string* err);
+
+}  // namespace shiro_hip

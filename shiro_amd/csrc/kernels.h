@@ -1,0 +1,36 @@
+// Host-side declarations of the kernel launchers (kernels.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "layout.h"
+
+namespace shiro_hip {
+
+// Device FrameAccumulator state (Renderer.cpp:47-58): AoS sums like the
+// reference's vectors.
+struct AccumParams {
+    float* beauty;          // 4 per pixel
+    float* albedo;          // 3
+    float* normal;          // 3
+    float* depthSum;        // 1
+    uint32_t* depthCount;   // 1
+};
+
+hipError_t launch_render(const RenderParams& P, bool count, hipStream_t stream);
+hipError_t launch_accum_add(const AccumParams& A, const DevFrame& f, uint32_t count, uint32_t np, hipStream_t s);
+hipError_t launch_accum_resolve(const AccumParams& A, const DevFrame& f, uint32_t total, uint32_t np, hipStream_t s);
+hipError_t launch_qmc_fill(uint32_t x0, uint32_t y0, uint32_t w, uint32_t h, uint32_t s, uint32_t dim0,
+                           uint32_t dimCount, float* out, hipStream_t st);
+hipError_t launch_intersect(const DevScene& S, const float* rays, uint32_t n, uint32_t* prim, float* t, float* uv,
+                            hipStream_t s);
+hipError_t launch_occluded(const DevScene& S, const float* rays, uint32_t n, uint8_t* out, hipStream_t s);
+hipError_t launch_trace_primary(const DevScene& S, const DevCamera& cam, const DevSettings& st, uint32_t sample,
+                                uint32_t* prim, float* t, hipStream_t s);
+hipError_t launch_eval_bsdf(const DevScene& S, const uint32_t* mi, const float* n, const float* cn, const float* v,
+                            const float* l, uint32_t count, float* value, float* pdf, hipStream_t s);
+hipError_t launch_env_sample(const DevEnv& e, const float* u, uint32_t n, float* dir, float* pdf, float* rgb,
+                             hipStream_t s);
+hipError_t launch_env_eval(const DevEnv& e, const float* dir, uint32_t n, float* pdf, float* rgb, hipStream_t s);
+
+}  // namespace shiro_hip

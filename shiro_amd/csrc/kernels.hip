@@ -1,0 +1,328 @@
+// gfx950 kernels of the path-tracing backend and their host launchers.
+//
+//  k_render        the megakernel: one thread per pixel runs a slice of the
+//                  batch's samples through the full TracePath and keeps the
+//                  per-pixel running sums (beauty + hit count, albedo,
+//                  normal, depth minimum) in HBM between slice launches; the
+//                  last slice writes the batch means like
+//                  CpuPathTracer::RenderSampleBatch (CpuPathTracer.cpp:2088-2137).
+//                  One 64-lane wave covers an 8x8 pixel tile; tiles are
+//                  dealt to the eight XCDs in contiguous ranges so that
+//                  neighbouring tiles share an L2.
+//  k_accum_*       FrameAccumulator on the device (Renderer.cpp:47-110).
+//  k_qmc_fill, k_intersect, k_occluded, k_trace_primary, k_eval_bsdf,
+//  k_env_sample, k_env_eval   single stages of the path for parity tests.
+#include <hip/hip_runtime.h>
+
+#include "dev_path.h"
+#include "kernels.h"
+
+namespace shiro_hip {
+
+// Tile index for a block: hardware deals consecutive blocks round-robin to
+// the 8 XCDs; remap so that XCD k gets the k-th contiguous range of tiles.
+DEV uint32_t xcd_tile(uint32_t bid, uint32_t nblocks) {
+    const uint32_t per = (nblocks + 7u) / 8u;
+    const uint32_t xcd = bid % 8u, slot = bid / 8u;
+    const uint32_t full = nblocks % 8u == 0u ? 8u : nblocks % 8u;  // XCDs with `per` blocks
+    // Blocks [0, full*per) belong to XCDs that get `per` tiles, the rest get per-1.
+    uint32_t base;
+    if (xcd < full) {
+        base = xcd * per;
+    } else {
+        base = full * per + (xcd - full) * (per - 1u);
+    }
+    return base + slot;
+}
+
+template <bool COUNT>
+__global__ __launch_bounds__(64) void k_render(RenderParams P) {
+    const uint32_t nblocks = gridDim.x;
+    const uint32_t tile = xcd_tile(blockIdx.x, nblocks);
+    const uint32_t lane = threadIdx.x;
+    const uint32_t x = (tile % P.tilesX) * 8u + (lane & 7u);
+    const uint32_t y = (tile / P.tilesX) * 8u + (lane >> 3);
+    const DevSettings& st = P.settings;
+    if (x >= st.width || y >= st.height) return;
+    const size_t pix = static_cast<size_t>(y) * st.width + x;
+
+    float4 beauty, albedo, normal;
+    float depth;
+    if (P.sliceBegin == 0u) {
+        beauty = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        albedo = normal = beauty;
+        depth = kInf;
+    } else {
+        beauty = P.sums.beauty[pix];
+        albedo = P.sums.albedo[pix];
+        normal = P.sums.normal[pix];
+        depth = P.sums.depth[pix];
+    }
+    Counters<COUNT> cnt;
+    for (uint32_t i = 0; i < P.sliceCount; ++i) {
+        const Sampler smp{x, y, P.sampleStart + P.sliceBegin + i};
+        const Ray r = camera_ray(P.camera, st, x, y, smp.next2(0u));
+        const PathResult pr = trace_path(P.scene, st, r, smp, cnt);
+        beauty.x = beauty.x + pr.radiance.x;
+        beauty.y = beauty.y + pr.radiance.y;
+        beauty.z = beauty.z + pr.radiance.z;
+        beauty.w = beauty.w + (pr.depth < kInf ? 1.0f : 0.0f);
+        albedo.x = albedo.x + pr.albedo.x;
+        albedo.y = albedo.y + pr.albedo.y;
+        albedo.z = albedo.z + pr.albedo.z;
+        normal.x = normal.x + pr.normal.x;
+        normal.y = normal.y + pr.normal.y;
+        normal.z = normal.z + pr.normal.z;
+        if (pr.depth < depth) depth = pr.depth;
+    }
+    if (COUNT) {
+        DevCounters* c = P.counters;
+        atomicAdd(&c->samples, static_cast<unsigned long long>(P.sliceCount));
+        atomicAdd(&c->rays, cnt.rays);
+        atomicAdd(&c->shadowRays, cnt.shadowRays);
+        atomicAdd(&c->nodeFetches, cnt.nodes);
+        atomicAdd(&c->triFetches, cnt.tris);
+        atomicAdd(&c->envTaps, cnt.taps);
+        atomicAdd(&c->cdfSteps, cnt.steps);
+        atomicAdd(&c->hitFetches, cnt.hits);
+    }
+    if (P.sliceBegin + P.sliceCount < P.batchCount) {
+        P.sums.beauty[pix] = beauty;
+        P.sums.albedo[pix] = albedo;
+        P.sums.normal[pix] = normal;
+        P.sums.depth[pix] = depth;
+        return;
+    }
+    // Batch means: beauty divided by the count, albedo/normal scaled by its
+    // reciprocal (CpuPathTracer.cpp:2132-2137).
+    const float n = static_cast<float>(P.batchCount);
+    const float inv = 1.0f / n;
+    float* b = P.frame.beauty + pix * 4u;
+    b[0] = beauty.x / n;
+    b[1] = beauty.y / n;
+    b[2] = beauty.z / n;
+    b[3] = beauty.w / n;
+    float* a = P.frame.albedo + pix * 3u;
+    a[0] = albedo.x * inv;
+    a[1] = albedo.y * inv;
+    a[2] = albedo.z * inv;
+    float* nn = P.frame.normal + pix * 3u;
+    nn[0] = normal.x * inv;
+    nn[1] = normal.y * inv;
+    nn[2] = normal.z * inv;
+    P.frame.depth[pix] = depth;
+}
+
+// FrameAccumulator::Accumulate (Renderer.cpp:60-83).
+__global__ void k_accum_add(AccumParams A, DevFrame f, uint32_t count, uint32_t np) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= np) return;
+    const float w = static_cast<float>(count);
+#pragma unroll
+    for (int c = 0; c < 4; ++c) A.beauty[p * 4 + c] = A.beauty[p * 4 + c] + f.beauty[p * 4 + c] * w;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) A.albedo[p * 3 + c] = A.albedo[p * 3 + c] + f.albedo[p * 3 + c] * w;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) A.normal[p * 3 + c] = A.normal[p * 3 + c] + f.normal[p * 3 + c] * w;
+    const float d = f.depth[p];
+    if (d < kInf) {
+        A.depthSum[p] += d * w;
+        A.depthCount[p] += count;
+    }
+}
+
+// FrameAccumulator::Resolve (Renderer.cpp:85-110).
+__global__ void k_accum_resolve(AccumParams A, DevFrame f, uint32_t total, uint32_t np) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= np) return;
+    if (total == 0u) {
+        for (int c = 0; c < 4; ++c) f.beauty[p * 4 + c] = 0.0f;
+        for (int c = 0; c < 3; ++c) f.albedo[p * 3 + c] = 0.0f;
+        for (int c = 0; c < 3; ++c) f.normal[p * 3 + c] = 0.0f;
+        f.depth[p] = kInf;
+        return;
+    }
+    const float inv = 1.0f / static_cast<float>(total);
+#pragma unroll
+    for (int c = 0; c < 4; ++c) f.beauty[p * 4 + c] = A.beauty[p * 4 + c] * inv;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) f.albedo[p * 3 + c] = A.albedo[p * 3 + c] * inv;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) f.normal[p * 3 + c] = A.normal[p * 3 + c] * inv;
+    f.depth[p] = A.depthCount[p] > 0u ? A.depthSum[p] / static_cast<float>(A.depthCount[p]) : kInf;
+}
+
+__global__ void k_qmc_fill(uint32_t x0, uint32_t y0, uint32_t w, uint32_t h, uint32_t s, uint32_t dim0,
+                           uint32_t dimCount, float* out) {
+    const uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const uint64_t total = static_cast<uint64_t>(w) * h * dimCount;
+    if (i >= total) return;
+    const uint32_t d = static_cast<uint32_t>(i % dimCount);
+    const uint64_t p = i / dimCount;
+    const uint32_t x = static_cast<uint32_t>(p % w), y = static_cast<uint32_t>(p / w);
+    out[i] = qmc_value(x0 + x, y0 + y, s, dim0 + d);
+}
+
+__global__ void k_intersect(DevScene S, const float* rays, uint32_t n, uint32_t* prim, float* t, float* uv) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float* q = rays + static_cast<size_t>(i) * 8u;
+    Ray r;
+    r.o = V3{q[0], q[1], q[2]};
+    r.d = V3{q[3], q[4], q[5]};
+    r.tmin = q[6];
+    r.tmax = q[7];
+    Counters<false> c;
+    const HitRec h = intersect_closest(S, r, c.trav());
+    prim[i] = h.prim;
+    t[i] = h.slot == kNoHit ? kFltMax : h.t;
+    uv[2 * i] = h.b1;
+    uv[2 * i + 1] = h.b2;
+}
+
+__global__ void k_occluded(DevScene S, const float* rays, uint32_t n, uint8_t* out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float* q = rays + static_cast<size_t>(i) * 8u;
+    Ray r;
+    r.o = V3{q[0], q[1], q[2]};
+    r.d = V3{q[3], q[4], q[5]};
+    r.tmin = q[6];
+    r.tmax = q[7];
+    Counters<false> c;
+    out[i] = intersect_any(S, r, c.trav()) ? 1u : 0u;
+}
+
+__global__ __launch_bounds__(64) void k_trace_primary(DevScene S, DevCamera cam, DevSettings st, uint32_t s,
+                                                      uint32_t* prim, float* t) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= st.width * st.height) return;
+    const uint32_t x = i % st.width, y = i / st.width;
+    const Sampler smp{x, y, s};
+    const Ray r = camera_ray(cam, st, x, y, smp.next2(0u));
+    Counters<false> c;
+    const HitRec h = intersect_closest(S, r, c.trav());
+    prim[i] = h.prim;
+    t[i] = h.slot == kNoHit ? kFltMax : h.t;
+}
+
+__global__ void k_eval_bsdf(DevScene S, const uint32_t* mi, const float* n, const float* cn, const float* v,
+                            const float* l, uint32_t count, float* value, float* pdf) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    const uint32_t m = mi[i];
+    const Mat M{S.materials + m, S.derived + m};
+    const V3 N = ld3(n + 3 * i), CN = ld3(cn + 3 * i), Vv = ld3(v + 3 * i), L = ld3(l + 3 * i);
+    const V3 f = bsdf_eval(M, N, CN, Vv, L);
+    value[3 * i] = f.x;
+    value[3 * i + 1] = f.y;
+    value[3 * i + 2] = f.z;
+    pdf[i] = bsdf_pdf(M, N, CN, Vv, L);
+}
+
+__global__ void k_env_sample(DevEnv e, const float* u, uint32_t n, float* dir, float* pdf, float* rgb) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    V3 d;
+    float p;
+    unsigned long long a = 0, b = 0;
+    const V3 c = env_sample(e, V2{u[2 * i], u[2 * i + 1]}, &d, &p, EnvCount<false>{&a, &b});
+    dir[3 * i] = d.x;
+    dir[3 * i + 1] = d.y;
+    dir[3 * i + 2] = d.z;
+    pdf[i] = p;
+    rgb[3 * i] = c.x;
+    rgb[3 * i + 1] = c.y;
+    rgb[3 * i + 2] = c.z;
+}
+
+__global__ void k_env_eval(DevEnv e, const float* dir, uint32_t n, float* pdf, float* rgb) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    unsigned long long a = 0, b = 0;
+    const EnvCount<false> ec{&a, &b};
+    const V3 d = ld3(dir + 3 * i);
+    pdf[i] = env_pdf(e, d, ec);
+    const V3 c = env_eval(e, d, ec);
+    rgb[3 * i] = c.x;
+    rgb[3 * i + 1] = c.y;
+    rgb[3 * i + 2] = c.z;
+}
+
+static inline uint32_t blocks_for(uint64_t n, uint32_t bs) { return static_cast<uint32_t>((n + bs - 1) / bs); }
+
+hipError_t launch_render(const RenderParams& P, bool count, hipStream_t stream) {
+    const uint32_t tilesY = (P.settings.height + 7u) / 8u;
+    const uint32_t nb = P.tilesX * tilesY;
+    if (nb == 0) return hipSuccess;
+    if (count) {
+        hipLaunchKernelGGL(k_render<true>, dim3(nb), dim3(64), 0, stream, P);
+    } else {
+        hipLaunchKernelGGL(k_render<false>, dim3(nb), dim3(64), 0, stream, P);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_accum_add(const AccumParams& A, const DevFrame& f, uint32_t count, uint32_t np, hipStream_t s) {
+    if (np == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_accum_add, dim3(blocks_for(np, 256)), dim3(256), 0, s, A, f, count, np);
+    return hipGetLastError();
+}
+
+hipError_t launch_accum_resolve(const AccumParams& A, const DevFrame& f, uint32_t total, uint32_t np, hipStream_t s) {
+    if (np == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_accum_resolve, dim3(blocks_for(np, 256)), dim3(256), 0, s, A, f, total, np);
+    return hipGetLastError();
+}
+
+hipError_t launch_qmc_fill(uint32_t x0, uint32_t y0, uint32_t w, uint32_t h, uint32_t s, uint32_t dim0,
+                           uint32_t dimCount, float* out, hipStream_t st) {
+    const uint64_t n = static_cast<uint64_t>(w) * h * dimCount;
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_qmc_fill, dim3(blocks_for(n, 256)), dim3(256), 0, st, x0, y0, w, h, s, dim0, dimCount, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_intersect(const DevScene& S, const float* rays, uint32_t n, uint32_t* prim, float* t, float* uv,
+                            hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_intersect, dim3(blocks_for(n, 64)), dim3(64), 0, s, S, rays, n, prim, t, uv);
+    return hipGetLastError();
+}
+
+hipError_t launch_occluded(const DevScene& S, const float* rays, uint32_t n, uint8_t* out, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_occluded, dim3(blocks_for(n, 64)), dim3(64), 0, s, S, rays, n, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_trace_primary(const DevScene& S, const DevCamera& cam, const DevSettings& st, uint32_t sample,
+                                uint32_t* prim, float* t, hipStream_t s) {
+    const uint64_t n = static_cast<uint64_t>(st.width) * st.height;
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_trace_primary, dim3(blocks_for(n, 64)), dim3(64), 0, s, S, cam, st, sample, prim, t);
+    return hipGetLastError();
+}
+
+hipError_t launch_eval_bsdf(const DevScene& S, const uint32_t* mi, const float* n, const float* cn, const float* v,
+                            const float* l, uint32_t count, float* value, float* pdf, hipStream_t s) {
+    if (count == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_eval_bsdf, dim3(blocks_for(count, 64)), dim3(64), 0, s, S, mi, n, cn, v, l, count, value,
+                       pdf);
+    return hipGetLastError();
+}
+
+hipError_t launch_env_sample(const DevEnv& e, const float* u, uint32_t n, float* dir, float* pdf, float* rgb,
+                             hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_env_sample, dim3(blocks_for(n, 64)), dim3(64), 0, s, e, u, n, dir, pdf, rgb);
+    return hipGetLastError();
+}
+
+hipError_t launch_env_eval(const DevEnv& e, const float* dir, uint32_t n, float* pdf, float* rgb, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_env_eval, dim3(blocks_for(n, 64)), dim3(64), 0, s, e, dir, n, pdf, rgb);
+    return hipGetLastError();
+}
+
+}  // namespace shiro_hip

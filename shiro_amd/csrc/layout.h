@@ -1,0 +1,162 @@
+// Device-resident scene layout shared by the host uploader and the kernels.
+//
+// HBM layout (all arrays 16-byte aligned, one allocation each):
+//   nodes    BVH4, 128 B per node = 8 x float4:
+//              [0] lo.x of children 0..3   [1] hi.x   [2] lo.y   [3] hi.y
+//              [4] lo.z                    [5] hi.z
+//              [6] child word (uint bits): internal -> node index,
+//                  leaf -> kLeafBit | first triangle slot, empty -> kEmpty
+//                  (empty children have an inverted box and never hit)
+//              [7] leaf triangle count per child (uint bits; statistics)
+//            One node is one 128-byte cache line of gfx950's L2.
+//   tris     48 B per triangle in leaf order = 3 x float4:
+//              {v0.xyz, global prim id bits}, {v1.xyz, end-of-leaf flag},
+//              {v2.xyz, 0}
+//            read only by the watertight test during traversal.
+//   hitdata  48 B per triangle in leaf order = 3 x float4:
+//              {n0.xyz, material index bits}, {n1.xyz, flags}, {n2.xyz, 0}
+//            read once per closest hit (shading normal + material).
+//   materials  shiro_hip_material array (276 B each) + a derived table
+//              (MaterialDerived) precomputed on the host.
+//   env      per dome: texels as float4 (RGB + pad), rowCdf (H+1),
+//            condCdf (H x (W+1)), all fp32.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/shiro_hip.h"
+
+namespace shiro_hip {
+
+constexpr uint32_t kLeafBit = 0x80000000u;
+constexpr uint32_t kEmpty = 0xffffffffu;
+constexpr uint32_t kNoHit = 0xffffffffu;
+constexpr uint32_t kTriHasNormals = 1u;  // hitdata[1].w flag
+constexpr uint32_t kLeafEndFlag = 1u;    // tris[3*slot+1].w: last triangle of its leaf
+
+// Traversal stack capacity per thread; the host rejects BVHs whose depth
+// could overflow it (3 pushes per level + 1).
+constexpr int kStackSize = 96;
+constexpr uint32_t kMaxBvhDepth = (kStackSize - 1) / 3;
+
+constexpr int kMaxDomes = 4;
+constexpr int kMaxDirLights = 8;
+
+struct DevEnv {
+    const float4* texels;   // w*h, row 0 = theta 0
+    const float* rowCdf;    // h+1 (null if no table)
+    const float* condCdf;   // h*(w+1)
+    uint32_t w, h;
+    uint32_t layout;        // SHIRO_HIP_ENV_LATLONG or _ANGULAR
+    uint32_t hasTable;
+};
+
+struct DevDome {
+    float radiance[3], right[3], up[3], forward[3];
+    int32_t env;            // index into DevScene::envs, -1 = constant dome
+    uint32_t pad;
+};
+
+struct DevDirLight {
+    float direction[3];     // direction of travel (as given)
+    float radiance[3];
+};
+
+// Per-material constants computed on the host with the reference's own
+// expressions (they depend only on the material): lobe weights and colours.
+struct MaterialDerived {
+    float diffuseColor[3];
+    float specularF0[3];
+    float coatF0[3];
+    float coatTint[3];
+    float wDiffuse, wBaseSpecular, wCoat, wTransmission;
+    float baseRoughness;
+    float transmissionRoughness;
+    float sampleRoughnessBase;   // isotropic AdjustedRoughness(baseRoughness,...)
+    float sampleRoughnessCoat;
+    float sampleRoughnessTrans;
+    float emission[3];           // max(emissionColor*strength, 0)
+    float mediumExtinction[3];
+    float transmissionTint[3];
+    uint32_t flags;              // kMat* bits
+};
+
+constexpr uint32_t kMatAnisotropic = 1u;   // specular or coat anisotropy != 0
+constexpr uint32_t kMatThinFilm = 2u;
+constexpr uint32_t kMatSheen = 4u;
+constexpr uint32_t kMatNormalOverride = 8u;
+constexpr uint32_t kMatCoatNormalOverride = 16u;
+constexpr uint32_t kMatTangentOverride = 32u;
+constexpr uint32_t kMatEmissive = 64u;
+constexpr uint32_t kMatOpacity = 128u;     // opacity < 0.999
+
+struct DevScene {
+    const float4* nodes;
+    const float4* tris;
+    const float4* hitdata;
+    const shiro_hip_material* materials;
+    const MaterialDerived* derived;
+    uint32_t nodeCount;
+    uint32_t triCount;
+    uint32_t materialCount;
+    uint32_t rootIsLeaf;          // single-leaf scenes: root word in rootWord
+    uint32_t rootWord;
+    uint32_t rootCount;
+    DevEnv envs[kMaxDomes];
+    DevDome domes[kMaxDomes];
+    DevDirLight dirs[kMaxDirLights];
+    uint32_t domeCount;
+    uint32_t dirCount;
+    float envTop[3];
+    float envBottom[3];
+};
+
+struct DevCamera {
+    float position[3], forward[3], right[3], up[3];
+    float vfov, aspect;
+};
+
+struct DevSettings {
+    uint32_t width, height;
+    uint32_t domeLightSamples;
+    uint32_t maxDepth, diffuseDepth, specularDepth;
+    uint32_t backgroundVisible;
+    uint32_t rrDepth;
+};
+
+// Running per-pixel sums of one batch across sample-slice launches.
+struct DevBatchSums {
+    float4* beauty;     // rgb sum + hit count
+    float4* albedo;     // rgb sum
+    float4* normal;     // rgb sum
+    float* depth;       // running min
+};
+
+struct DevFrame {
+    float* beauty;      // 4 floats per pixel
+    float* albedo;      // 3
+    float* normal;      // 3
+    float* depth;       // 1
+};
+
+// Fetch counters (roofline accounting); all 64-bit.
+struct DevCounters {
+    unsigned long long samples, rays, shadowRays, nodeFetches, triFetches, envTaps, cdfSteps, hitFetches;
+};
+
+struct RenderParams {
+    DevScene scene;
+    DevCamera camera;
+    DevSettings settings;
+    DevBatchSums sums;
+    DevFrame frame;
+    uint32_t sampleStart;     // global index of the first sample of the batch
+    uint32_t sliceBegin;      // first sample (relative) of this launch
+    uint32_t sliceCount;      // samples in this launch
+    uint32_t batchCount;      // samples in the whole batch
+    uint32_t tilesX;          // 8x8 tiles per row
+    DevCounters* counters;    // null unless counting
+};
+
+}  // namespace shiro_hip

@@ -1,0 +1,335 @@
+"""ctypes binding of libshiro_hip.so and a Python HipBackend mirroring Shiro's
+RenderBackend interface (reference include/shiro/backend/RenderBackend.h:12-34):
+
+    backend = HipBackend()
+    backend.GetStatus()                       -> BackendStatus
+    backend.SupportsScene(scene)              -> bool   (OptixBackend.h:16)
+    backend.RenderSampleBatch(RenderRequest)  -> FrameBuffer
+
+The library is required: importing works without it, but constructing a
+context raises if libshiro_hip.so is missing or no HIP device is present --
+there is no CPU fallback in this package.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import hashlib
+import os
+from dataclasses import dataclass
+from typing import Optional
+
+import numpy as np
+
+from . import abi
+from .scene import Camera, FrameBuffer, RenderSettings, Scene, marshal
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libshiro_hip.so")
+
+_P = C.POINTER
+_u = C.c_uint32
+_fp, _up = _P(C.c_float), _P(C.c_uint32)
+_lib = None
+
+
+class HipError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{abi.RESULT_NAMES.get(code, code)}: {msg}")
+        self.code = code
+
+
+def load_library(path: str = LIB_PATH):
+    """Loads libshiro_hip.so and declares its signatures (raises if absent)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise FileNotFoundError(
+            f"{path} is not built; run __graft_entry__.build() or `make -C shiro_amd/csrc`")
+    # One HIP runtime per process: when PyTorch is installed, load it first so
+    # that the library binds to the same libamdhip64 as torch's tensors.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
+    L = C.CDLL(path)
+
+    def d(name, res, *args):
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = list(args)
+
+    V = C.c_void_p
+    d("shiro_hip_default_settings", None, _P(abi.Settings))
+    d("shiro_hip_default_material", None, _P(abi.Material))
+    d("shiro_hip_create", C.c_int, C.c_int, _P(V))
+    d("shiro_hip_destroy", None, V)
+    d("shiro_hip_last_error", C.c_char_p, V)
+    d("shiro_hip_get_status", C.c_int, V, _P(abi.Status))
+    d("shiro_hip_supports_scene", C.c_int, _P(abi.Scene))
+    d("shiro_hip_upload_scene", C.c_int, V, _P(abi.Scene))
+    d("shiro_hip_render_batch", C.c_int, V, _P(abi.Camera), _P(abi.Settings), _u, _u, _P(C.c_int32), _P(abi.Frame))
+    d("shiro_hip_render_batch_device", C.c_int, V, _P(abi.Camera), _P(abi.Settings), _u, _u, _P(abi.Frame), C.c_int)
+    d("shiro_hip_accum_reset", C.c_int, V, _u, _u)
+    d("shiro_hip_accum_add", C.c_int, V, _P(abi.Frame), _u)
+    d("shiro_hip_accum_resolve", C.c_int, V, _P(abi.Frame))
+    d("shiro_hip_stream", V, V)
+    d("shiro_hip_synchronize", C.c_int, V)
+    d("shiro_hip_last_render_ms", C.c_int, V, _P(C.c_float))
+    d("shiro_hip_scene_info_get", C.c_int, V, _P(abi.SceneInfo))
+    d("shiro_hip_qmc_fill", C.c_int, V, _u, _u, _u, _u, _u, _u, _u, _fp)
+    d("shiro_hip_intersect", C.c_int, V, _fp, _u, _up, _fp, _fp)
+    d("shiro_hip_occluded", C.c_int, V, _fp, _u, _P(C.c_uint8))
+    d("shiro_hip_trace_primary", C.c_int, V, _P(abi.Camera), _P(abi.Settings), _u, _up, _fp)
+    d("shiro_hip_eval_bsdf", C.c_int, V, _up, _fp, _fp, _fp, _fp, _u, _fp, _fp)
+    d("shiro_hip_env_sample", C.c_int, V, _u, _fp, _u, _fp, _fp, _fp)
+    d("shiro_hip_env_eval", C.c_int, V, _u, _fp, _u, _fp, _fp)
+    d("shiro_hip_count_fetches", C.c_int, V, _P(abi.Camera), _P(abi.Settings), _u, _u, _P(abi.FetchCounts))
+    _lib = L
+    return L
+
+
+def scene_signature(scene: Scene) -> str:
+    """Content hash of a scene (the reference re-uploads when its FNV-1a scene
+    signature changes, OptixBackend.cpp:150-232)."""
+    h = hashlib.blake2b(digest_size=16)
+    for m in scene.meshes:
+        for a in (m.positions, m.normals, m.indices, m.uvs):
+            if a is not None:
+                h.update(np.ascontiguousarray(a).tobytes())
+        h.update(int(m.materialIndex).to_bytes(4, "little"))
+    for mat in scene.materials:
+        h.update(bytes(mat.to_c()))
+    for d in scene.distantLights:
+        h.update(np.asarray(d.direction + d.radiance, np.float32).tobytes())
+    for d in scene.domeLights:
+        h.update(np.asarray(d.radiance + d.right + d.up + d.forward, np.float32).tobytes())
+        h.update(int(d.layout).to_bytes(4, "little"))
+        if d.pixels is not None:
+            h.update(np.ascontiguousarray(d.pixels).tobytes())
+    h.update(np.asarray(tuple(scene.environmentTop) + tuple(scene.environmentBottom), np.float32).tobytes())
+    return h.hexdigest()
+
+
+@dataclass
+class BackendStatus:
+    """RenderBackend.h:21-26."""
+    kind: int = 1
+    name: str = "hip"
+    available: bool = False
+    usesGpu: bool = True
+    deviceName: str = ""
+    computeUnits: int = 0
+
+
+@dataclass
+class RenderRequest:
+    """RenderBackend.h:12-19."""
+    scene: Scene
+    camera: Camera
+    settings: RenderSettings
+    sampleStart: int = 0
+    sampleCount: int = 1
+    cancel: Optional[C.Array] = None  # ctypes c_int32 flag, non-zero = cancel
+
+
+class Context:
+    """One libshiro_hip context (device, stream, device-resident scene)."""
+
+    def __init__(self, device: int = 0):
+        self.L = load_library()
+        self.h = C.c_void_p()
+        rc = self.L.shiro_hip_create(device, C.byref(self.h))
+        if rc != 0:
+            raise HipError(rc, "cannot create a HIP context (no device?)")
+        self._sig = None
+        self._keep = None
+
+    def close(self):
+        if self.h:
+            self.L.shiro_hip_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        if rc != 0:
+            raise HipError(rc, self.L.shiro_hip_last_error(self.h).decode())
+        return rc
+
+    def upload(self, scene: Scene, force: bool = False):
+        sig = scene_signature(scene)
+        if sig == self._sig and not force:
+            return
+        m = marshal(scene)
+        self._check(self.L.shiro_hip_upload_scene(self.h, m.ptr))
+        self._sig = sig
+
+    def scene_info(self) -> abi.SceneInfo:
+        info = abi.SceneInfo()
+        self._check(self.L.shiro_hip_scene_info_get(self.h, C.byref(info)))
+        return info
+
+    def render(self, camera: Camera, settings: RenderSettings, sample_start=0, sample_count=None,
+               cancel=None) -> FrameBuffer:
+        sample_count = settings.samplesPerPixel if sample_count is None else sample_count
+        fb = FrameBuffer(settings.width, settings.height)
+        f = fb.to_c()
+        cam, st = camera.to_c(), settings.to_c()
+        rc = self.L.shiro_hip_render_batch(self.h, C.byref(cam), C.byref(st), sample_start, sample_count,
+                                           cancel, C.byref(f))
+        fb.rc = rc
+        if rc not in (0, 3):
+            self._check(rc)
+        return fb
+
+    def render_device(self, camera: Camera, settings: RenderSettings, frame_ptrs, sample_start=0,
+                      sample_count=None, synchronize=True):
+        """frame_ptrs: (beauty, albedo, normal, depth) device addresses."""
+        sample_count = settings.samplesPerPixel if sample_count is None else sample_count
+        f = abi.Frame()
+        f.beauty, f.albedo, f.normal, f.depth = [C.cast(C.c_void_p(p), _fp) for p in frame_ptrs]
+        f.width, f.height = settings.width, settings.height
+        cam, st = camera.to_c(), settings.to_c()
+        self._check(self.L.shiro_hip_render_batch_device(self.h, C.byref(cam), C.byref(st), sample_start,
+                                                         sample_count, C.byref(f), int(synchronize)))
+
+    def last_render_ms(self) -> float:
+        ms = C.c_float()
+        self._check(self.L.shiro_hip_last_render_ms(self.h, C.byref(ms)))
+        return ms.value
+
+    def synchronize(self):
+        self._check(self.L.shiro_hip_synchronize(self.h))
+
+    def stream(self) -> int:
+        return int(self.L.shiro_hip_stream(self.h) or 0)
+
+    def accum_reset(self, w, h):
+        self._check(self.L.shiro_hip_accum_reset(self.h, w, h))
+
+    def accum_add(self, frame_ptrs, w, h, count):
+        f = abi.Frame()
+        f.beauty, f.albedo, f.normal, f.depth = [C.cast(C.c_void_p(p), _fp) for p in frame_ptrs]
+        f.width, f.height = w, h
+        self._check(self.L.shiro_hip_accum_add(self.h, C.byref(f), count))
+
+    def accum_resolve(self, frame_ptrs, w, h):
+        f = abi.Frame()
+        f.beauty, f.albedo, f.normal, f.depth = [C.cast(C.c_void_p(p), _fp) for p in frame_ptrs]
+        f.width, f.height = w, h
+        self._check(self.L.shiro_hip_accum_resolve(self.h, C.byref(f)))
+
+    # ---- single stages ----
+    def qmc_fill(self, x0, y0, w, h, sample, dim0, dim_count) -> np.ndarray:
+        out = np.zeros(w * h * dim_count, np.float32)
+        self._check(self.L.shiro_hip_qmc_fill(self.h, x0, y0, w, h, sample, dim0, dim_count, abi.fp(out)))
+        return out.reshape(h, w, dim_count)
+
+    def intersect(self, rays):
+        rays = np.ascontiguousarray(rays, np.float32)
+        n = len(rays)
+        prim = np.zeros(n, np.uint32)
+        t = np.zeros(n, np.float32)
+        uv = np.zeros((n, 2), np.float32)
+        self._check(self.L.shiro_hip_intersect(self.h, abi.fp(rays), n, abi.up(prim), abi.fp(t), abi.fp(uv)))
+        return prim, t, uv
+
+    def occluded(self, rays):
+        rays = np.ascontiguousarray(rays, np.float32)
+        out = np.zeros(len(rays), np.uint8)
+        self._check(self.L.shiro_hip_occluded(self.h, abi.fp(rays), len(rays), out.ctypes.data_as(_P(C.c_uint8))))
+        return out
+
+    def trace_primary(self, camera, settings, sample_index=0):
+        n = settings.width * settings.height
+        prim = np.zeros(n, np.uint32)
+        t = np.zeros(n, np.float32)
+        cam, st = camera.to_c(), settings.to_c()
+        self._check(self.L.shiro_hip_trace_primary(self.h, C.byref(cam), C.byref(st), sample_index, abi.up(prim),
+                                                   abi.fp(t)))
+        return prim, t
+
+    def eval_bsdf(self, mat_index, n, cn, v, l):
+        mi = np.ascontiguousarray(mat_index, np.uint32)
+        arrs = [np.ascontiguousarray(a, np.float32) for a in (n, cn, v, l)]
+        cnt = len(mi)
+        val = np.zeros((cnt, 3), np.float32)
+        pdf = np.zeros(cnt, np.float32)
+        self._check(self.L.shiro_hip_eval_bsdf(self.h, abi.up(mi), *[abi.fp(a) for a in arrs], cnt, abi.fp(val),
+                                               abi.fp(pdf)))
+        return val, pdf
+
+    def env_sample(self, u, dome=0):
+        u = np.ascontiguousarray(u, np.float32)
+        n = len(u)
+        d = np.zeros((n, 3), np.float32)
+        p = np.zeros(n, np.float32)
+        c = np.zeros((n, 3), np.float32)
+        self._check(self.L.shiro_hip_env_sample(self.h, dome, abi.fp(u), n, abi.fp(d), abi.fp(p), abi.fp(c)))
+        return d, p, c
+
+    def env_eval(self, dirs, dome=0):
+        dirs = np.ascontiguousarray(dirs, np.float32)
+        n = len(dirs)
+        p = np.zeros(n, np.float32)
+        c = np.zeros((n, 3), np.float32)
+        self._check(self.L.shiro_hip_env_eval(self.h, dome, abi.fp(dirs), n, abi.fp(p), abi.fp(c)))
+        return p, c
+
+    def count_fetches(self, camera, settings, sample_start=0, sample_count=None) -> abi.FetchCounts:
+        sample_count = settings.samplesPerPixel if sample_count is None else sample_count
+        out = abi.FetchCounts()
+        cam, st = camera.to_c(), settings.to_c()
+        self._check(self.L.shiro_hip_count_fetches(self.h, C.byref(cam), C.byref(st), sample_start, sample_count,
+                                                   C.byref(out)))
+        return out
+
+
+class HipBackend:
+    """RenderBackend (RenderBackend.h:28-34) over libshiro_hip.  Like the
+    reference's GPU backend it reports unavailability instead of raising,
+    returns a cleared frame on failure, and caches the device scene keyed by
+    its content hash."""
+
+    def __init__(self, device: int = 0):
+        self.device = device
+        self._ctx: Optional[Context] = None
+        self._attempted = False
+
+    def _runtime(self) -> Optional[Context]:
+        if self._ctx is None and not self._attempted:
+            self._attempted = True
+            try:
+                self._ctx = Context(self.device)
+            except (OSError, HipError):
+                self._ctx = None
+        return self._ctx
+
+    def GetStatus(self) -> BackendStatus:
+        L = load_library()
+        st = abi.Status()
+        L.shiro_hip_get_status(None, C.byref(st))
+        return BackendStatus(kind=st.kind, name=(st.name or b"hip").decode(), available=bool(st.available),
+                             usesGpu=bool(st.usesGpu), deviceName=st.deviceName.decode(),
+                             computeUnits=st.computeUnits)
+
+    def SupportsScene(self, scene: Scene) -> bool:
+        m = marshal(scene)
+        return bool(load_library().shiro_hip_supports_scene(m.ptr))
+
+    def RenderSampleBatch(self, request: RenderRequest) -> FrameBuffer:
+        st = request.settings
+        ctx = self._runtime()
+        if ctx is None:
+            return FrameBuffer(st.width, st.height)
+        try:
+            ctx.upload(request.scene)
+            return ctx.render(request.camera, st, request.sampleStart, request.sampleCount, request.cancel)
+        except HipError as e:
+            import sys
+            print(f"shiro_hip: {e}", file=sys.stderr)
+            return FrameBuffer(st.width, st.height)

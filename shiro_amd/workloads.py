@@ -202,24 +202,48 @@ def make_mesh(p, ix, material: int) -> TriangleMesh:
 # ---------------------------------------------------------------------------
 # Environment map
 # ---------------------------------------------------------------------------
+def _upsample(img: np.ndarray, height: int, width: int) -> np.ndarray:
+    """Bilinear upsampling of an (h, w, c) grid of texel centres to
+    (height, width, c); u wraps, v clamps."""
+    h, w = img.shape[:2]
+    v = np.clip((np.arange(height) + 0.5) / height * h - 0.5, 0, h - 1)
+    u = (np.arange(width) + 0.5) / width * w - 0.5
+    v0 = np.floor(v).astype(np.int64)
+    v1 = np.minimum(v0 + 1, h - 1)
+    fv = (v - v0)[:, None, None]
+    u0 = np.floor(u).astype(np.int64)
+    fu = (u - u0)[None, :, None]
+    u1 = (u0 + 1) % w
+    u0 = u0 % w
+    top = img[v0][:, u0] * (1 - fu) + img[v0][:, u1] * fu
+    bot = img[v1][:, u0] * (1 - fu) + img[v1][:, u1] * fu
+    return top * (1 - fv) + bot * fv
+
+
 def procedural_dome(width: int, height: int, seed: int, hot_spots: int = 3, peak: float = 1.0e4,
                     spot_width_deg: float = 1.5) -> np.ndarray:
     """(H, W, 3) fp32 lat-long map: log-normal background (log radiance is a
     smooth field with mean 0 and standard deviation 1 per channel, slightly
-    decorrelated between channels) plus Gaussian hot spots of the given peak
-    in the upper hemisphere.  Row 0 is theta = 0 (+Y)."""
+    decorrelated between channels; evaluated on a grid 4x coarser and
+    upsampled bilinearly) plus Gaussian hot spots of the given peak in the
+    upper hemisphere.  Row 0 is theta = 0 (+Y)."""
     rng = np.random.default_rng(seed)
-    theta = (np.arange(height) + 0.5) / height * np.pi
-    phi = (np.arange(width) + 0.5) / width * 2 * np.pi
-    st, ct = np.sin(theta)[:, None], np.cos(theta)[:, None]
-    d = np.stack(np.broadcast_arrays(np.sin(phi)[None] * st, ct, np.cos(phi)[None] * st), -1)
-    flat = d.reshape(-1, 3)
-    img = np.empty((height * width, 3))
-    base = ValueNoise3(seed).fbm(flat * 3.0 + 11.0, 5)
+
+    def dirs(h, w):
+        theta = (np.arange(h) + 0.5) / h * np.pi
+        phi = (np.arange(w) + 0.5) / w * 2 * np.pi
+        st, ct = np.sin(theta)[:, None], np.cos(theta)[:, None]
+        return np.stack(np.broadcast_arrays(np.sin(phi)[None] * st, ct, np.cos(phi)[None] * st), -1).reshape(-1, 3)
+
+    ch, cw = max(2, height // 4), max(4, width // 4)
+    coarse = dirs(ch, cw)
+    base = ValueNoise3(seed).fbm(coarse * 3.0 + 11.0, 5)
+    log_rad = np.empty((ch * cw, 3))
     for c in range(3):
-        n = 0.85 * base + 0.15 * ValueNoise3(seed + 101 + c).fbm(flat * 3.0 + 7.0, 5)
-        n = (n - n.mean()) / (n.std() + 1e-12)
-        img[:, c] = np.exp(n)
+        n = 0.85 * base + 0.15 * ValueNoise3(seed + 101 + c).fbm(coarse * 3.0 + 7.0, 5)
+        log_rad[:, c] = (n - n.mean()) / (n.std() + 1e-12)
+    img = np.exp(_upsample(log_rad.reshape(ch, cw, 3), height, width)).reshape(-1, 3)
+    flat = dirs(height, width)
     sw = np.deg2rad(spot_width_deg)
     for _ in range(hot_spots):
         th = rng.uniform(0.15, 0.45) * np.pi

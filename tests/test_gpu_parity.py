@@ -1,0 +1,227 @@
+"""Parity of the HIP backend (through the C ABI) with the CPU oracle.
+
+Bars (tolerances written in each test):
+  - QMC samples: bit-exact.
+  - Closest hit / primary visibility: triangle ids agree on >= 99.99% of rays
+    (the rest are ties at shared edges, where the watertight test and the
+    oracle's Moller-Trumbore test may pick different neighbours).
+  - BSDF / environment: fp32, relative 2e-5 (the device's sin/cos/atan2/acos
+    differ from glibc by ulps).
+  - Images: beauty relMSE <= 1e-3 against the oracle at equal spp and seeds.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from helpers import c1_mini, c2_mini, c3_mini, random_rays, relmse
+from oracle.bindings import Accumulator, OracleScene, oracle_fns
+from shiro_amd.scene import Camera, DirectionalLight, PbrMaterial, RenderSettings, Scene, TriangleMesh
+
+pytestmark = pytest.mark.gpu
+
+
+def _log(msg):
+    print(msg, flush=True)
+
+
+def test_qmc_bit_exact(hip_ctx):
+    f = oracle_fns()
+    for s in (0, 1, 7, 1000, 65535, 1 << 20):
+        _log(f"qmc sample {s}")
+        g = hip_ctx.qmc_fill(3, 5, 32, 16, s, 0, 96)
+        ref = np.zeros(32 * 16 * 96, np.float32)
+        from shiro_amd import abi
+        f.qmc_fill(3, 5, 32, 16, s, 0, 96, abi.fp(ref))
+        assert np.array_equal(g.reshape(-1).view(np.uint32), ref.view(np.uint32)), f"sample {s}"
+
+
+@pytest.fixture(scope="module")
+def c2m():
+    wl = c2_mini()
+    return wl, OracleScene(wl.scene, bvh=True)
+
+
+def test_intersect_vs_oracle(hip_ctx, c2m):
+    wl, orc = c2m
+    _log("intersect c2-mini")
+    hip_ctx.upload(wl.scene)
+    rays = random_rays(wl.scene, 100000, seed=1)
+    gp, gt, _ = hip_ctx.intersect(rays)
+    op, ot, _ = orc.intersect(rays)
+    agree = np.mean(gp == op)
+    assert agree >= 0.9999, agree
+    hit = (gp == op) & (op != 0xFFFFFFFF)
+    assert np.allclose(gt[hit], ot[hit], rtol=1e-4, atol=1e-5)
+    occ_rays = rays.copy()
+    occ_rays[:, 7] = np.random.default_rng(2).uniform(0.1, 60.0, len(rays))
+    go = hip_ctx.occluded(occ_rays)
+    oo = orc.occluded(occ_rays)
+    assert np.mean(go == oo) >= 0.9999
+
+
+def test_primary_visibility(hip_ctx, c2m):
+    wl, orc = c2m
+    _log("primary visibility c2 480x270")
+    hip_ctx.upload(wl.scene)
+    st = RenderSettings(**{**wl.settings.__dict__, "width": 480, "height": 270})
+    cam = Camera(**{**wl.camera.__dict__, "aspectRatio": 480 / 270})
+    for s in (0, 3):
+        gp, gt = hip_ctx.trace_primary(cam, st, s)
+        op, ot = orc.trace_primary(cam, st, s)
+        assert np.mean(gp == op) >= 0.9999
+
+
+def _bsdf_materials():
+    rng = np.random.default_rng(5)
+    mats = []
+    for i in range(24):
+        m = PbrMaterial()
+        m.baseColor = tuple(rng.uniform(0, 1, 3))
+        m.metallic = float(rng.choice([0.0, 1.0, rng.uniform()]))
+        m.roughness = float(rng.uniform(0.0, 1.0))
+        m.diffuseRoughness = float(rng.uniform(0, 1)) if i % 3 == 0 else 0.0
+        m.coatWeight = float(rng.uniform()) if i % 4 == 1 else 0.0
+        m.coatRoughness = float(rng.uniform(0, 0.5))
+        m.specularAnisotropy = float(rng.uniform(-1, 1)) if i % 5 == 2 else 0.0
+        m.specularRotation = float(rng.uniform())
+        m.sheen = float(rng.uniform()) if i % 6 == 3 else 0.0
+        m.thinFilmThickness = float(rng.uniform(100, 900)) if i % 7 == 4 else 0.0
+        m.subsurface = float(rng.uniform()) if i % 8 == 5 else 0.0
+        m.transmission = float(rng.uniform()) if i % 3 == 2 else 0.0
+        mats.append(m)
+    return mats
+
+
+def test_bsdf_vs_oracle(hip_ctx):
+    from oracle.bindings import eval_bsdf
+    mats = _bsdf_materials()
+    quad = np.array([[0, 0, 0], [1, 0, 0], [0, 0, 1]], np.float32)
+    sc = Scene(materials=mats, meshes=[TriangleMesh(quad, np.zeros((0, 3), np.float32), np.array([0, 1, 2], np.uint32))])
+    hip_ctx.upload(sc)
+    rng = np.random.default_rng(6)
+    n = 20000
+
+    def unit(k):
+        v = rng.normal(size=(k, 3))
+        return (v / np.linalg.norm(v, axis=1, keepdims=True)).astype(np.float32)
+
+    N = unit(n)
+    CN = N.copy()
+    V = unit(n)
+    V = np.where((np.sum(V * N, 1) < 0)[:, None], -V, V).astype(np.float32)
+    L = unit(n)
+    mi = rng.integers(0, len(mats), n).astype(np.uint32)
+    gv, gp = hip_ctx.eval_bsdf(mi, N, CN, V, L)
+    ov, op = eval_bsdf(oracle_fns(), mats, mi, N, CN, V, L)
+    assert np.allclose(gv, ov, rtol=2e-5, atol=1e-6), np.max(np.abs(gv - ov))
+    assert np.allclose(gp, op, rtol=2e-5, atol=1e-6), np.max(np.abs(gp - op))
+
+
+def test_env_vs_oracle(hip_ctx, c2m):
+    wl, orc = c2m
+    hip_ctx.upload(wl.scene)
+    rng = np.random.default_rng(7)
+    u = rng.random((50000, 2)).astype(np.float32)
+    gd, gp, gc = hip_ctx.env_sample(u)
+    od, op, oc = orc.env_sample(u)
+    assert np.allclose(gd, od, rtol=2e-5, atol=2e-6)
+    assert np.allclose(gp, op, rtol=2e-5, atol=1e-7)
+    assert np.allclose(gc, oc, rtol=2e-4, atol=1e-5)
+    d = rng.normal(size=(50000, 3)).astype(np.float32)
+    gp2, gc2 = hip_ctx.env_eval(d)
+    op2, oc2 = orc.env_eval(d)
+    # A direction within an ulp of a texel boundary may fall on either side.
+    assert np.mean(np.isclose(gp2, op2, rtol=2e-5, atol=1e-7)) >= 0.999
+    assert np.mean(np.all(np.isclose(gc2, oc2, rtol=2e-4, atol=1e-5), axis=1)) >= 0.999
+
+
+@pytest.mark.parametrize("make", [c1_mini, c2_mini, c3_mini], ids=["c1", "c2", "c3"])
+def test_render_relmse(hip_ctx, make):
+    wl = make()
+    _log(f"render {wl.name} {wl.settings.width}x{wl.settings.height} {wl.settings.samplesPerPixel}spp")
+    hip_ctx.upload(wl.scene)
+    g = hip_ctx.render(wl.camera, wl.settings)
+    o = OracleScene(wl.scene, bvh=True).render(wl.camera, wl.settings)
+    assert np.all(np.isfinite(g.beauty))
+    e = relmse(g.beauty[:, :3], o.beauty[:, :3])
+    _log(f"relMSE {wl.name} = {e:.3e}")
+    assert e <= 1e-3, e
+    assert np.mean(g.beauty[:, 3] == o.beauty[:, 3]) >= 0.999
+    assert np.mean(np.all(np.isclose(g.albedo, o.albedo, atol=1e-6), axis=1)) >= 0.999
+    assert np.mean(np.all(np.isclose(g.normal, o.normal, atol=1e-4), axis=1)) >= 0.999
+    fin = np.isfinite(o.depth) & np.isfinite(g.depth)
+    assert np.mean(np.isfinite(g.depth) == np.isfinite(o.depth)) >= 0.999
+    assert np.allclose(g.depth[fin], o.depth[fin], rtol=1e-4)
+
+
+def test_render_sliced_batches(hip_ctx):
+    """A batch larger than one launch slice carries its running sums across
+    launches; the result equals the oracle's single batch (compared on a
+    band of rows the oracle renders in seconds)."""
+    wl = c1_mini(size=512, spp=40)  # 512^2 * 40 samples span two launches
+    _log("render sliced c1 512x512 40spp")
+    hip_ctx.upload(wl.scene)
+    g = hip_ctx.render(wl.camera, wl.settings)
+    y0, rows = 200, 24
+    o = OracleScene(wl.scene, bvh=True).render_region(wl.camera, wl.settings, 0, y0, 512, rows, threads=8)
+    band = slice(y0 * 512, (y0 + rows) * 512)
+    assert relmse(g.beauty[band, :3], o.beauty[band, :3]) <= 1e-3
+
+
+def test_device_accumulator(hip_ctx):
+    import torch
+    wl = c1_mini(size=48, spp=4)
+    hip_ctx.upload(wl.scene)
+    w, h = wl.settings.width, wl.settings.height
+    dev = torch.device("cuda:0")
+    bufs = [torch.zeros(w * h * k, device=dev) for k in (4, 3, 3, 1)]
+    out = [torch.zeros(w * h * k, device=dev) for k in (4, 3, 3, 1)]
+    ptr = [b.data_ptr() for b in bufs]
+    optr = [b.data_ptr() for b in out]
+    acc = Accumulator(w, h)
+    hip_ctx.accum_reset(w, h)
+    start = 0
+    for count in (1, 3, 2):
+        hip_ctx.render_device(wl.camera, wl.settings, ptr, start, count)
+        hip_ctx.accum_add(ptr, w, h, count)
+        fb = hip_ctx.render(wl.camera, wl.settings, start, count)
+        acc.add(fb, count)
+        start += count
+    hip_ctx.accum_resolve(optr, w, h)
+    torch.cuda.synchronize()
+    ref = acc.resolve()
+    assert np.array_equal(out[0].cpu().numpy().reshape(-1, 4), ref.beauty)
+    assert np.array_equal(out[3].cpu().numpy(), ref.depth)
+
+
+def test_cancel_returns_cleared_frame(hip_ctx):
+    wl = c1_mini(size=32, spp=2)
+    hip_ctx.upload(wl.scene)
+    flag = C.c_int32(1)
+    fb = hip_ctx.render(wl.camera, wl.settings, cancel=C.pointer(flag))
+    assert fb.rc == 3
+    assert not fb.beauty.any() and np.all(np.isinf(fb.depth))
+
+
+def test_probe_scene(hip_ctx):
+    """The reference's GPU smoke probe (tools/optix_probe.cpp:7-80): one
+    triangle, one directional light, gradient sky, 8x8 at 1 spp; the centre
+    pixel must be lit, covered and at finite depth."""
+    sc = Scene(environmentTop=(0.2, 0.4, 0.8), environmentBottom=(0.05, 0.05, 0.05))
+    m = PbrMaterial()
+    m.baseColor = (0.8, 0.2, 0.2)
+    m.roughness = 0.4
+    sc.materials.append(m)
+    sc.meshes.append(TriangleMesh(np.array([[-1, -1, 0], [1, -1, 0], [0, 1, 0]], np.float32),
+                                  np.array([[0, 0, 1]] * 3, np.float32), np.array([0, 1, 2], np.uint32), 0))
+    sc.distantLights.append(DirectionalLight((0.0, -1.0, -1.0), (2.0, 2.0, 2.0)))
+    st = RenderSettings(width=8, height=8, samplesPerPixel=1, samplesPerUpdate=1)
+    cam = Camera(aspectRatio=1.0)
+    hip_ctx.upload(sc)
+    fb = hip_ctx.render(cam, st, 0, 1)
+    c = fb.beauty[4 * 8 + 4]
+    assert c[3] > 0 and (c[0] != 0 or c[1] != 0 or c[2] != 0)
+    assert fb.depth[4 * 8 + 4] < 1e10
+    o = OracleScene(sc, bvh=False).render(cam, st, 0, 1)
+    assert np.allclose(fb.beauty, o.beauty, rtol=1e-5, atol=1e-6)
