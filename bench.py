@@ -42,8 +42,9 @@ def cpu_baseline(wl, target_seconds, threads):
     build_s = time.perf_counter() - t0
     st = wl.settings
     spp = st.samplesPerPixel
-    rows = 2
+    rows = 4
     y0 = st.height // 2
+    orc.render_region(wl.camera, st, 0, y0, st.width, 1, 0, 1, threads)  # warm caches and threads
     t0 = time.perf_counter()
     orc.render_region(wl.camera, st, 0, y0, st.width, rows, 0, spp, threads)
     dt = max(time.perf_counter() - t0, 1e-3)
@@ -75,7 +76,19 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-count", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only copy MIB MiB device-to-device 3 times (calibrates HBM byte counters)")
     args = ap.parse_args()
+    if args.copy_probe:
+        import torch
+        n = args.copy_probe * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": n * 4, "repeats": 3}), flush=True)
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: this backend has no multi-GPU path "
               "(a frame shards by sample range or by tile; see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
@@ -135,8 +148,14 @@ def main():
         avg_launch_ms = sum(gpu_ms) / len(gpu_ms) / launches
         bytes_per_launch = per_sample * npx * slice_spp
         achieved = bytes_per_launch / (avg_launch_ms * 1e-3) / 1e9
+        traffic, traffic_src = None, None
+        tpath = os.path.join(ROOT, "profiles", "traffic.json")
+        if os.path.exists(tpath):
+            rec = json.load(open(tpath)).get(f"{args.workload}/megakernel")
+            if rec:
+                traffic, traffic_src = rec["bytes_per_launch"], rec["source"]
         roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                    "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic, "traffic_source": traffic_src,
                     "kernel": "k_render<false> (megakernel)", "launches_per_step": launches,
                     "avg_launch_ms": round(avg_launch_ms, 3), "bytes_per_launch": int(bytes_per_launch),
                     "bytes_per_sample": round(per_sample, 1),
