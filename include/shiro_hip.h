@@ -262,16 +262,17 @@ int shiro_hip_upload_scene(shiro_hip_ctx* ctx, const shiro_hip_scene* scene);
 /* RenderSampleBatch, RenderBackend.h:33 / CpuPathTracer.cpp:2043-2170:
  * renders samples [sampleStart, sampleStart + sampleCount) of every pixel and
  * writes the batch means into host buffers `out` (sized settings->width x
- * settings->height).  cancel (may be NULL) is polled between launches; when
- * it becomes non-zero the call returns SHIRO_HIP_CANCELLED and `out` holds a
- * cleared frame. */
+ * settings->height).  cancel (may be NULL) is a byte polled between launches
+ * -- the object representation of the reference's std::atomic<bool> cancel
+ * flag (RenderBackend.h:18) -- and when it becomes non-zero the call returns
+ * SHIRO_HIP_CANCELLED and `out` holds a cleared frame. */
 int shiro_hip_render_batch(
     shiro_hip_ctx* ctx,
     const shiro_hip_camera* camera,
     const shiro_hip_settings* settings,
     uint32_t sampleStart,
     uint32_t sampleCount,
-    const volatile int32_t* cancel,
+    const volatile uint8_t* cancel,
     shiro_hip_frame* out);
 
 /* Same as shiro_hip_render_batch, but `out` holds device pointers and the

@@ -131,7 +131,7 @@ class OracleScene(_SceneBase):
         f = fb.to_c()
         cam, st = camera.to_c(), settings.to_c()
         rc = self._fn("render", C.c_int, C.c_void_p, _P(abi.Camera), _P(abi.Settings), _u, _u, _u,
-                      _P(C.c_int32), _P(abi.Frame))(
+                      _P(C.c_uint8), _P(abi.Frame))(
             self.h, C.byref(cam), C.byref(st), sample_start, sample_count, threads, cancel, C.byref(f))
         fb.rc = rc
         return fb

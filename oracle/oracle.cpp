@@ -1691,7 +1691,7 @@ Camera to_camera(const shiro_hip_camera& c) {
 // cancelled.
 bool render_rows(const Scene& s, const Settings& st, const Camera& cam, uint32_t start, uint32_t count,
                  const Guide* guide, std::array<float, kGuideBins>* upd, uint32_t y0, uint32_t y1, uint32_t x0,
-                 uint32_t x1, const volatile int32_t* cancel, const Frame& out) {
+                 uint32_t x1, const volatile uint8_t* cancel, const Frame& out) {
     for (uint32_t y = y0; y < y1; ++y) {
         if (cancel && *cancel) return false;
         for (uint32_t x = x0; x < x1; ++x) {
@@ -1831,7 +1831,7 @@ void oracle_scene_set_bvh(void* h, int useBvh) {
 // threads == 0 selects hardware_concurrency()-1 like the reference (or
 // settings.threadLimit when set).  Returns 0, or 3 when cancelled.
 int oracle_render(void* h, const shiro_hip_camera* camera, const shiro_hip_settings* settings, uint32_t sampleStart,
-                  uint32_t sampleCount, uint32_t threads, const volatile int32_t* cancel, shiro_hip_frame* out) {
+                  uint32_t sampleCount, uint32_t threads, const volatile uint8_t* cancel, shiro_hip_frame* out) {
     Scene& s = *static_cast<Scene*>(h);
     const Settings st = to_settings(*settings);
     const Camera cam = to_camera(*camera);

@@ -159,7 +159,7 @@ int check_settings(shiro_hip_ctx* ctx, const shiro_hip_settings* s) {
 // Enqueues the slices of one batch; `count` > 0.  frame holds device
 // pointers.  If cancel is non-null the host flag is polled between slices.
 int enqueue_batch(shiro_hip_ctx* ctx, const shiro_hip_camera& cam, const shiro_hip_settings& st, uint32_t start,
-                  uint32_t count, const DevFrame& frame, bool countFetches, const volatile int32_t* cancel) {
+                  uint32_t count, const DevFrame& frame, bool countFetches, const volatile uint8_t* cancel) {
     const uint64_t np = static_cast<uint64_t>(st.width) * st.height;
     const uint32_t slice = slice_samples(np, count);
     RenderParams P{};
@@ -439,7 +439,7 @@ int shiro_hip_scene_info_get(const shiro_hip_ctx* ctx, shiro_hip_scene_info* out
 }
 
 int shiro_hip_render_batch(shiro_hip_ctx* ctx, const shiro_hip_camera* camera, const shiro_hip_settings* settings,
-                           uint32_t sampleStart, uint32_t sampleCount, const volatile int32_t* cancel,
+                           uint32_t sampleStart, uint32_t sampleCount, const volatile uint8_t* cancel,
                            shiro_hip_frame* out) {
     if (!ctx) return SHIRO_HIP_INVALID;
     std::lock_guard<std::mutex> lock(ctx->mu);

@@ -67,7 +67,7 @@ def load_library(path: str = LIB_PATH):
     d("shiro_hip_get_status", C.c_int, V, _P(abi.Status))
     d("shiro_hip_supports_scene", C.c_int, _P(abi.Scene))
     d("shiro_hip_upload_scene", C.c_int, V, _P(abi.Scene))
-    d("shiro_hip_render_batch", C.c_int, V, _P(abi.Camera), _P(abi.Settings), _u, _u, _P(C.c_int32), _P(abi.Frame))
+    d("shiro_hip_render_batch", C.c_int, V, _P(abi.Camera), _P(abi.Settings), _u, _u, _P(C.c_uint8), _P(abi.Frame))
     d("shiro_hip_render_batch_device", C.c_int, V, _P(abi.Camera), _P(abi.Settings), _u, _u, _P(abi.Frame), C.c_int)
     d("shiro_hip_accum_reset", C.c_int, V, _u, _u)
     d("shiro_hip_accum_add", C.c_int, V, _P(abi.Frame), _u)
@@ -129,7 +129,7 @@ class RenderRequest:
     settings: RenderSettings
     sampleStart: int = 0
     sampleCount: int = 1
-    cancel: Optional[C.Array] = None  # ctypes c_int32 flag, non-zero = cancel
+    cancel: Optional[object] = None  # pointer to a ctypes c_uint8 flag, non-zero = cancel
 
 
 class Context:
