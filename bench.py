@@ -72,6 +72,7 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3"])
+    ap.add_argument("--transport", default="wavefront", choices=["megakernel", "wavefront"])
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
@@ -98,8 +99,10 @@ def main():
     from shiro_amd import workloads
     from shiro_amd.hip import Context
 
+    from shiro_amd import abi
     wl = workloads.WORKLOADS[args.workload]()
     st = wl.settings
+    st.transport = abi.TRANSPORT_WAVEFRONT if args.transport == "wavefront" else abi.TRANSPORT_MEGAKERNEL
     print(f"# workload {workload_desc(wl)}", file=sys.stderr, flush=True)
     ctx = Context(0)
     t0 = time.perf_counter()
@@ -171,7 +174,7 @@ def main():
         "gpu_ms_per_step": round(sum(gpu_ms) / len(gpu_ms), 2),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic (seeded procedural scene and dome map)",
-        "config": {"workload": workload_desc(wl), "transport": "megakernel", "samples_per_step": samples,
+        "config": {"workload": workload_desc(wl), "transport": args.transport, "samples_per_step": samples,
                    "bvh": f"BVH4 {info.bvhNodeCount} nodes, depth {info.bvhDepth}",
                    "scene_upload_s": round(upload_s, 2), "finite_output": ok},
         "roofline": roofline, "cpu_baseline": cpu,

@@ -83,6 +83,8 @@ struct shiro_hip_ctx {
     DevBuf sumB, sumA, sumN, sumD;
     DevBuf frB, frA, frN, frD;
     DevBuf counters;
+    // Wavefront state (WfBuffers arrays) and queues.
+    DevBuf wf[14];
     // Accumulator.
     DevBuf acB, acA, acN, acD, acC;
     uint32_t acW = 0, acH = 0, acTotal = 0;
@@ -153,6 +155,92 @@ int check_settings(shiro_hip_ctx* ctx, const shiro_hip_settings* s) {
     if (s->width == 0 || s->height == 0) return fail(ctx, SHIRO_HIP_INVALID, "zero-sized frame");
     if (static_cast<uint64_t>(s->width) * s->height > (1ull << 31))
         return fail(ctx, SHIRO_HIP_INVALID, "frame larger than 2^31 pixels");
+    if (s->transport > SHIRO_HIP_TRANSPORT_WAVEFRONT) return fail(ctx, SHIRO_HIP_INVALID, "unknown transport");
+    if (s->maxDepth > 255u || s->diffuseDepth > 255u || s->specularDepth > 255u)
+        return fail(ctx, SHIRO_HIP_INVALID, "depth limits above 255 are not supported");
+    if (s->domeLightSamples > 64u) return fail(ctx, SHIRO_HIP_INVALID, "domeLightSamples above 64 is not supported");
+    return SHIRO_HIP_OK;
+}
+
+// Wavefront form of one batch: passes of S samples for every pixel, S
+// chosen so that a pass holds about kWavefrontPaths paths (and at most
+// kWavefrontStateBytes of path state).
+constexpr uint64_t kWavefrontPaths = 4ull << 20;
+constexpr uint64_t kWavefrontStateBytes = 6ull << 30;
+
+int enqueue_wavefront(shiro_hip_ctx* ctx, const shiro_hip_camera& cam, const shiro_hip_settings& st, uint32_t start,
+                      uint32_t count, const DevFrame& frame, const volatile uint8_t* cancel) {
+    const uint64_t np = static_cast<uint64_t>(st.width) * st.height;
+    const uint32_t tilesX = (st.width + 7u) / 8u, tilesY = (st.height + 7u) / 8u;
+    const uint64_t perSample = static_cast<uint64_t>(tilesX) * tilesY * 64u;
+    const uint32_t K = st.domeLightSamples > 1u ? st.domeLightSamples : 1u;
+    const uint32_t M = std::max(1u, ctx->dev.dirCount + ctx->dev.domeCount * K);
+    const uint64_t bytesPerPath = 10u * 16u + 12u + static_cast<uint64_t>(M) * 32u;
+    uint64_t S = std::max<uint64_t>(1, kWavefrontPaths / perSample);
+    S = std::min<uint64_t>(S, std::max<uint64_t>(1, kWavefrontStateBytes / (bytesPerPath * perSample)));
+    S = std::min<uint64_t>(S, count);
+    const uint64_t cap = S * perSample;
+    if (cap >= (1ull << 32) / std::max<uint64_t>(1, 2ull * M))
+        return fail(ctx, SHIRO_HIP_INVALID, "frame too large for the wavefront path state");
+    for (int i = 0; i < 8; ++i) HIPC(ctx->wf[i].reserve(cap * 16), "alloc wavefront state");
+    HIPC(ctx->wf[8].reserve(cap * 16), "alloc wavefront state");
+    HIPC(ctx->wf[9].reserve(cap * 16), "alloc wavefront state");
+    HIPC(ctx->wf[10].reserve(cap * M * 32), "alloc wavefront state");
+    HIPC(ctx->wf[11].reserve(cap * 4), "alloc wavefront queues");
+    HIPC(ctx->wf[12].reserve(cap * 4), "alloc wavefront queues");
+    HIPC(ctx->wf[13].reserve(cap * 4), "alloc wavefront queues");
+    const uint32_t slots = 2u * st.maxDepth + 1u;
+    HIPC(ctx->counters.reserve(std::max<size_t>(sizeof(DevCounters), slots * 4u)), "alloc counters");
+    if (S < count) {
+        HIPC(ctx->sumB.reserve(np * 16), "alloc sums");
+        HIPC(ctx->sumA.reserve(np * 16), "alloc sums");
+        HIPC(ctx->sumN.reserve(np * 16), "alloc sums");
+        HIPC(ctx->sumD.reserve(np * 4), "alloc sums");
+    }
+    WfParams P{};
+    P.scene = ctx->dev;
+    P.camera = to_dev(cam);
+    P.settings = to_dev(st);
+    P.sums = DevBatchSums{ctx->sumB.as<float4>(), ctx->sumA.as<float4>(), ctx->sumN.as<float4>(), ctx->sumD.as<float>()};
+    P.frame = frame;
+    WfBuffers& B = P.B;
+    B.o = ctx->wf[0].as<float4>();
+    B.d = ctx->wf[1].as<float4>();
+    B.T = ctx->wf[2].as<float4>();
+    B.L = ctx->wf[3].as<float4>();
+    B.E = ctx->wf[4].as<float4>();
+    B.A = ctx->wf[5].as<float4>();
+    B.N = ctx->wf[6].as<float4>();
+    B.H = ctx->wf[7].as<float4>();
+    B.so = ctx->wf[8].as<float4>();
+    B.st = ctx->wf[9].as<float4>();
+    B.sr = ctx->wf[10].as<float4>();
+    B.qa = ctx->wf[11].as<uint32_t>();
+    B.qb = ctx->wf[12].as<uint32_t>();
+    B.qs = ctx->wf[13].as<uint32_t>();
+    B.counts = ctx->counters.as<uint32_t>();
+    B.countSlots = slots;
+    B.shadowCountBase = st.maxDepth + 1u;
+    B.maxShadow = M;
+    B.cap = static_cast<uint32_t>(cap);
+    P.sampleStart = start;
+    P.batchCount = count;
+    P.tilesX = tilesX;
+    P.tiles = tilesX * tilesY;
+    WfLaunch L{};
+    L.maxBlocks = 16384;
+    HIPC(hipEventRecord(ctx->ev0, ctx->stream), "event");
+    for (uint32_t b = 0; b < count; b += static_cast<uint32_t>(S)) {
+        if (cancel && *cancel) {
+            (void)hipStreamSynchronize(ctx->stream);
+            return fail(ctx, SHIRO_HIP_CANCELLED, "cancelled");
+        }
+        P.sliceBegin = b;
+        P.sliceCount = std::min(static_cast<uint32_t>(S), count - b);
+        HIPC(launch_wavefront_pass(P, L, ctx->stream), "wavefront launch");
+        if (cancel) HIPC(hipStreamSynchronize(ctx->stream), "wavefront");
+    }
+    HIPC(hipEventRecord(ctx->ev1, ctx->stream), "event");
     return SHIRO_HIP_OK;
 }
 
@@ -160,6 +248,8 @@ int check_settings(shiro_hip_ctx* ctx, const shiro_hip_settings* s) {
 // pointers.  If cancel is non-null the host flag is polled between slices.
 int enqueue_batch(shiro_hip_ctx* ctx, const shiro_hip_camera& cam, const shiro_hip_settings& st, uint32_t start,
                   uint32_t count, const DevFrame& frame, bool countFetches, const volatile uint8_t* cancel) {
+    if (st.transport == SHIRO_HIP_TRANSPORT_WAVEFRONT && !countFetches)
+        return enqueue_wavefront(ctx, cam, st, start, count, frame, cancel);
     const uint64_t np = static_cast<uint64_t>(st.width) * st.height;
     const uint32_t slice = slice_samples(np, count);
     RenderParams P{};

@@ -94,39 +94,6 @@ DEV bool dome_sample(const DevScene& S, const DevDome& d, V2 u, V3* dir, V3* rad
     return true;
 }
 
-// DirectLighting (CpuPathTracer.cpp:1511-1631) for directional and dome
-// lights.
-template <bool COUNT>
-DEV V3 direct_lighting(const DevScene& S, const DevSettings& st, V3 p, V3 n, V3 cn, const Mat& M, V3 v,
-                       const Sampler& smp, uint32_t depth, Counters<COUNT>& c) {
-    V3 L{0.0f, 0.0f, 0.0f};
-    const V3 origin = p + n * kRayEps;
-    for (uint32_t i = 0; i < S.dirCount; ++i) {
-        const DevDirLight& dl = S.dirs[i];
-        const V3 ld = normalize(ld3(dl.direction) * -1.0f);
-        const float nl = clampf(dot(n, ld), 0.0f, 1.0f);
-        if (nl <= 0.0f || occluded(S, origin, ld, kFltMax, c)) continue;
-        L = L + clamp_max_component(bsdf_eval(M, n, cn, v, ld) * ld3(dl.radiance) * nl, kDirectClamp);
-    }
-    const uint32_t K = st.domeLightSamples > 1u ? st.domeLightSamples : 1u;
-    for (uint32_t li = 0; li < S.domeCount; ++li) {
-        V3 acc{0.0f, 0.0f, 0.0f};
-        for (uint32_t k = 0; k < K; ++k) {
-            V3 ld, lr;
-            float lp = 0.0f;
-            const uint32_t branch = 256u + depth * 512u + li * 128u + k * 2u;
-            if (!dome_sample(S, S.domes[li], smp.next2(branch), &ld, &lr, &lp, c) || lp <= 0.0f) continue;
-            const float nl = clampf(dot(n, ld), 0.0f, 1.0f);
-            if (nl <= 0.0f || occluded(S, origin, ld, kFltMax, c)) continue;
-            const float bp = bsdf_pdf(M, n, cn, v, ld);
-            const float w = power_heuristic(lp, bp);
-            acc = acc + clamp_max_component(bsdf_eval(M, n, cn, v, ld) * lr * (nl * w / lp), kDirectClamp);
-        }
-        L = L + acc / static_cast<float>(K);
-    }
-    return L;
-}
-
 // GenerateCameraRay (CpuPathTracer.cpp:1633-1648).
 DEV Ray camera_ray(const DevCamera& c, const DevSettings& st, uint32_t x, uint32_t y, V2 j) {
     const float fx = ((static_cast<float>(x) + j.x) / static_cast<float>(st.width)) * 2.0f - 1.0f;
@@ -140,11 +107,6 @@ DEV Ray camera_ray(const DevCamera& c, const DevSettings& st, uint32_t x, uint32
     r.tmax = kFltMax;
     return r;
 }
-
-struct PathResult {
-    V3 radiance, albedo, normal;
-    float depth;
-};
 
 // Surface data of a closest hit (CpuPathTracer.cpp:126-168, 446-454).
 struct SurfaceHit {
@@ -172,214 +134,384 @@ DEV SurfaceHit fetch_hit(const DevScene& S, const Ray& ray, const HitRec& h) {
 
 enum BounceType : uint32_t { kBounceNone = 0, kBounceDiffuse, kBounceReflective, kBounceTransmission };
 
-// TracePath (CpuPathTracer.cpp:1650-2017) without the path guide and
-// emissive-triangle lights (the uploader reports scenes with emissive
-// surfaces as unsupported, so the caller falls back to the CPU backend).
+// Per-path state carried between bounces (TracePath's locals,
+// CpuPathTracer.cpp:1659-1670) plus the first-hit AOVs.
+struct PathState {
+    V3 o, d;          // current ray (tMin = 1e-4, tMax = FLT_MAX)
+    V3 T;             // throughput
+    V3 L;             // radiance gathered so far
+    V3 ext;           // interior medium extinction
+    float prevPdf;
+    uint32_t bits;    // see kBits*
+    V3 albedo, normal;
+    float depth;      // first-hit distance, +inf if none
+};
+
+// PathState::bits layout.
+constexpr uint32_t kBitsTypeMask = 3u;        // previous BounceType
+constexpr uint32_t kBitsPrevDelta = 4u;
+constexpr uint32_t kBitsInside = 8u;
+constexpr uint32_t kBitsMediumOn = 16u;
+constexpr uint32_t kBitsDiffuseShift = 8u;    // diffuse bounce count (8 bits)
+constexpr uint32_t kBitsSpecularShift = 16u;  // specular bounce count (8 bits)
+
+DEV PathState init_path(const DevCamera& cam, const DevSettings& st, uint32_t x, uint32_t y, const Sampler& smp) {
+    PathState ps;
+    const Ray r = camera_ray(cam, st, x, y, smp.next2(0u));
+    ps.o = r.o;
+    ps.d = r.d;
+    ps.T = V3{1.0f, 1.0f, 1.0f};
+    ps.L = V3{0.0f, 0.0f, 0.0f};
+    ps.ext = V3{0.0f, 0.0f, 0.0f};
+    ps.prevPdf = 0.0f;
+    ps.bits = kBitsPrevDelta;  // previousBounceDelta starts true
+    ps.albedo = ps.normal = V3{0.0f, 0.0f, 0.0f};
+    ps.depth = kInf;
+    return ps;
+}
+
+DEV Ray path_ray(const PathState& ps) {
+    Ray r;
+    r.o = ps.o;
+    r.d = ps.d;
+    r.tmin = kRayEps;
+    r.tmax = kFltMax;
+    return r;
+}
+
+// Environment seen by a ray that left the scene (CpuPathTracer.cpp:1675-1693).
 template <bool COUNT>
-DEV PathResult trace_path(const DevScene& S, const DevSettings& st, Ray ray, const Sampler& smp, Counters<COUNT>& c) {
-    PathResult res;
-    res.radiance = res.albedo = res.normal = V3{0.0f, 0.0f, 0.0f};
-    res.depth = kInf;
-    V3 T{1.0f, 1.0f, 1.0f};
-    uint32_t nDiffuse = 0, nSpecular = 0;
-    uint32_t prevType = kBounceNone;
-    bool prevDelta = true;
-    float prevPdf = 0.0f;
-    bool inside = false, mediumOn = false;
-    V3 ext{0.0f, 0.0f, 0.0f};
+DEV void handle_miss(const DevScene& S, const DevSettings& st, uint32_t depth, PathState& ps, Counters<COUNT>& c) {
+    if (depth > 0 || st.backgroundVisible) {
+        const V3 Le = environment_radiance(S, ps.d, c);
+        const uint32_t prevType = ps.bits & kBitsTypeMask;
+        V3 dL;
+        if (S.domeCount == 0 || depth == 0 || (ps.bits & kBitsPrevDelta) || prevType == kBounceTransmission) {
+            dL = ps.T * Le;
+        } else {
+            const float sq2 = env_pdf_sq_sum(S, ps.d, c);
+            const float b2 = ps.prevPdf * ps.prevPdf;
+            dL = ps.T * Le * (b2 / fmaxs(b2 + sq2, 1.0e-6f));
+        }
+        ps.L = ps.L + dL;
+    }
+}
+
+// Shading frame at a closest hit.
+struct Surface {
+    V3 p, ng, n, cn, v;
+    float t;
+    uint32_t material;
+};
+
+// Everything of a bounce up to direct lighting (CpuPathTracer.cpp:1695-1748):
+// medium attenuation, normal orientation, opacity, first-hit AOVs, emission.
+// Returns false when the opacity test lets the ray pass through (the ray is
+// advanced and the bounce ends without lighting, as the reference's
+// `continue`).
+DEV bool shade_hit(const DevScene& S, uint32_t depth, const Sampler& smp, const Ray& ray, const HitRec& h,
+                   PathState& ps, Surface& sf) {
+    const SurfaceHit hit = fetch_hit(S, ray, h);
+    if (ps.bits & kBitsMediumOn) {
+        ps.T = ps.T * V3{expf(-ps.ext.x * h.t), expf(-ps.ext.y * h.t), expf(-ps.ext.z * h.t)};
+    }
+    const shiro_hip_material& m = S.materials[hit.material];
+    const MaterialDerived& md = S.derived[hit.material];
+    const bool front = dot(hit.ng, ray.d) < 0.0f;
+    const V3 ng = front ? hit.ng : hit.ng * -1.0f;
+    V3 nbase = front ? hit.ns : hit.ns * -1.0f;
+    if (dot(nbase, ng) < 0.0f) nbase = nbase * -1.0f;
+    V3 n = nbase;
+    if (md.flags & kMatNormalOverride) n = resolve_normal(m, nbase, false);
+    if (dot(n, ng) < 0.0f) n = n * -1.0f;
+    V3 cn = n;
+    if (md.flags & kMatCoatNormalOverride) cn = resolve_normal(m, n, true);
+    if (md.flags & kMatOpacity) {
+        const float opacity = clampf(m.opacity, 0.0f, 1.0f);
+        if (smp.next1(depth * kBranchesPerDepth) > opacity) {
+            ps.o = hit.p + ray.d * kRayEps;
+            return false;
+        }
+        ps.T = ps.T / fmaxs(opacity, 1.0e-4f);
+    }
+    if (depth == 0) {
+        ps.albedo = ld3(md.diffuseColor);
+        ps.normal = n * 0.5f + V3{0.5f, 0.5f, 0.5f};
+        ps.depth = h.t;
+    }
+    if (md.flags & kMatEmissive) {
+        // Emission seen directly, through a delta or transmission bounce
+        // (CpuPathTracer.cpp:1728-1748); other cases need emissive-surface
+        // lights, which the uploader reports as unsupported.
+        const uint32_t prevType = ps.bits & kBitsTypeMask;
+        if (depth == 0 || (ps.bits & kBitsPrevDelta) || prevType == kBounceTransmission) {
+            ps.L = ps.L + ps.T * ld3(md.emission);
+        }
+    }
+    sf.p = hit.p;
+    sf.ng = ng;
+    sf.n = n;
+    sf.cn = cn;
+    sf.v = ray.d * -1.0f;
+    sf.t = h.t;
+    sf.material = hit.material;
+    return true;
+}
+
+// Number of light samples of DirectLighting at one hit: each directional
+// light once, each dome domeLightSamples times (CpuPathTracer.cpp:1529-1576).
+DEV uint32_t direct_sample_count(const DevScene& S, const DevSettings& st) {
+    const uint32_t K = st.domeLightSamples > 1u ? st.domeLightSamples : 1u;
+    return S.dirCount + S.domeCount * K;
+}
+
+// Light sample j of DirectLighting: the shadow ray (from p + n*eps) and the
+// contribution it carries if unoccluded.  group 0 = directional light,
+// 1 + i = dome i.  Returns false when the sample needs no shadow ray.
+template <bool COUNT>
+DEV bool direct_sample(const DevScene& S, const DevSettings& st, const Surface& sf, const Mat& M, const Sampler& smp,
+                       uint32_t depth, uint32_t j, V3* dirOut, V3* contrib, uint32_t* group, Counters<COUNT>& c) {
+    if (j < S.dirCount) {
+        const DevDirLight& dl = S.dirs[j];
+        const V3 ld = normalize(ld3(dl.direction) * -1.0f);
+        const float nl = clampf(dot(sf.n, ld), 0.0f, 1.0f);
+        if (nl <= 0.0f) return false;
+        *dirOut = ld;
+        *contrib = clamp_max_component(bsdf_eval(M, sf.n, sf.cn, sf.v, ld) * ld3(dl.radiance) * nl, kDirectClamp);
+        *group = 0u;
+        return true;
+    }
+    const uint32_t K = st.domeLightSamples > 1u ? st.domeLightSamples : 1u;
+    const uint32_t jj = j - S.dirCount;
+    const uint32_t li = jj / K, k = jj - li * K;
+    V3 ld, lr;
+    float lp = 0.0f;
+    const uint32_t branch = 256u + depth * 512u + li * 128u + k * 2u;
+    if (!dome_sample(S, S.domes[li], smp.next2(branch), &ld, &lr, &lp, c) || lp <= 0.0f) return false;
+    const float nl = clampf(dot(sf.n, ld), 0.0f, 1.0f);
+    if (nl <= 0.0f) return false;
+    const float bp = bsdf_pdf(M, sf.n, sf.cn, sf.v, ld);
+    const float w = power_heuristic(lp, bp);
+    *dirOut = ld;
+    *contrib = clamp_max_component(bsdf_eval(M, sf.n, sf.cn, sf.v, ld) * lr * (nl * w / lp), kDirectClamp);
+    *group = 1u + li;
+    return true;
+}
+
+// Sums visible light samples in DirectLighting's order: directional lights
+// add directly; the samples of dome i are summed and divided by K when the
+// next group starts.
+struct DirectSum {
+    V3 L, acc;
+    int dome;
+    float K;
+    DEV void init(const DevSettings& st) {
+        L = acc = V3{0.0f, 0.0f, 0.0f};
+        dome = -1;
+        K = static_cast<float>(st.domeLightSamples > 1u ? st.domeLightSamples : 1u);
+    }
+    DEV void flush() {
+        if (dome >= 0) L = L + acc / K;
+        acc = V3{0.0f, 0.0f, 0.0f};
+        dome = -1;
+    }
+    DEV void add(uint32_t group, V3 c, bool visible) {
+        if (group == 0u) {
+            if (visible) L = L + c;
+            return;
+        }
+        const int d = static_cast<int>(group) - 1;
+        if (d != dome) {
+            flush();
+            dome = d;
+        }
+        if (visible) acc = acc + c;
+    }
+};
+
+// Lobe choice, BSDF sampling, throughput update and Russian roulette
+// (CpuPathTracer.cpp:1750-2013).  Returns false when the path ends.
+DEV bool sample_bounce(const DevScene& S, const DevSettings& st, uint32_t depth, const Sampler& smp,
+                       const Surface& sf, PathState& ps) {
+    const Mat M{S.materials + sf.material, S.derived + sf.material};
+    const shiro_hip_material& m = *M.m;
+    const MaterialDerived& md = *M.d;
+    const uint32_t b0 = depth * kBranchesPerDepth;
+    const V3 n = sf.n, cn = sf.cn, ng = sf.ng, v = sf.v;
+    const V3 rd = sf.v * -1.0f;
+    bool inside = (ps.bits & kBitsInside) != 0u;
+    bool mediumOn = (ps.bits & kBitsMediumOn) != 0u;
+    uint32_t nDiffuse = (ps.bits >> kBitsDiffuseShift) & 0xffu;
+    uint32_t nSpecular = (ps.bits >> kBitsSpecularShift) & 0xffu;
+
+    const float wd = md.wDiffuse;
+    float ws = md.wBaseSpecular;
+    const float wc = md.wCoat;
+    const float wtRaw = md.wTransmission;
+    float wt = wtRaw;
+    if (wtRaw > 0.0f) {
+        const float ei = inside ? fmaxs(m.ior, 1.0f) : 1.0f;
+        const float et = inside ? 1.0f : fmaxs(m.ior, 1.0f);
+        const float R = fresnel_dielectric(fabsf(dot(ng, v)), ei, et);
+        const float pass = m.thinWalled ? pane_transmittance(R) : (1.0f - R);
+        wt = wtRaw * pass;
+        ws += wtRaw * (1.0f - pass);
+    }
+    const float wr = wd + ws + wc;
+    const float wsum = wr + wt;
+    if (wsum <= 0.0f) return false;
+    const float sel = smp.next1(b0 + 1u) * wsum;
+    const V2 du = smp.next2(b0 + 2u);
+
+    V3 next{0.0f, 0.0f, 0.0f}, f{0.0f, 0.0f, 0.0f};
+    float pdf = 1.0f;
+    uint32_t type = kBounceDiffuse;
+    bool delta = false, pickT = false, pickS = false, pickC = false;
+    if (sel < wt) {
+        pickT = true;
+    } else {
+        const float rs = sel - wt;
+        if (rs < wd) {
+        } else if (rs < wd + ws) {
+            pickS = true;
+        } else {
+            pickC = true;
+        }
+    }
+
+    if (pickT) {
+        if (nSpecular >= st.specularDepth) return false;
+        ++nSpecular;
+        type = kBounceTransmission;
+        const float pT = fmaxs(wt / wsum, 1.0e-6f);
+        const float rT = m.thinWalled ? 0.0f : md.sampleRoughnessTrans;
+        const float mt = clampf(m.transmission, 0.0f, 1.0f) * (1.0f - clampf(m.metallic, 0.0f, 1.0f));
+        if (m.thinWalled) {
+            next = rd;
+            const float R = fresnel_dielectric(fabsf(dot(ng, v)), 1.0f, fmaxs(m.ior, 1.0f));
+            const float pane = pane_transmittance(R);
+            const V3 scale = ld3(md.transmissionTint) * coat_transmission_scale(M, cn, v, next) * mt;
+            const float ac = fmaxs(fabsf(dot(ng, next)), 1.0e-4f);
+            f = scale * pane / ac;
+            pdf = pT;
+            delta = true;
+        } else {
+            const float ei = inside ? fmaxs(m.ior, 1.0f) : 1.0f;
+            const float et = inside ? 1.0f : fmaxs(m.ior, 1.0f);
+            const float etaRatio = ei / et;
+            const float relEta = et / fmaxs(ei, 1.0f);
+            const float R = fresnel_dielectric(fabsf(dot(ng, v)), ei, et);
+            V3 mn = ng;
+            if (rT > kDeltaRoughness) {
+                mn = sample_ggx_half(ng, du, rT);
+                if (dot(mn, v) < 0.0f) mn = mn * -1.0f;
+            }
+            if (!refract_dir(rd, mn, etaRatio, &next)) {
+                next = normalize(reflect(rd, mn));
+                if (wr <= 0.0f) return false;
+                f = bsdf_eval(M, n, cn, v, next);
+                pdf = fmaxs((wr / wsum) * bsdf_pdf(M, n, cn, v, next), 1.0e-6f);
+                type = kBounceReflective;
+                delta = rT <= kDeltaRoughness;
+                pickT = false;
+                pickS = true;
+            } else {
+                if (rT <= kDeltaRoughness) {
+                    const V3 scale = ld3(md.transmissionTint) * coat_transmission_scale(M, cn, v, next) * mt;
+                    const float ac = fmaxs(fabsf(dot(ng, next)), 1.0e-4f);
+                    const float transport = 1.0f / fmaxs(relEta * relEta, 1.0e-6f);
+                    f = scale * (1.0f - R) * transport / ac;
+                    pdf = pT;
+                    delta = true;
+                } else {
+                    f = transmission_lobe(M, ng, cn, v, next, ei, et, rT);
+                    pdf = fmaxs(pT * transmission_pdf(ng, v, next, et / ei, rT), 1.0e-6f);
+                    delta = false;
+                }
+                inside = !inside;
+                if (inside) {
+                    ps.ext = ld3(md.mediumExtinction);
+                    mediumOn = m.transmissionDepth > 0.0f || max3(ld3(m.transmissionScatter)) > 0.0f;
+                } else {
+                    mediumOn = false;
+                }
+            }
+        }
+    } else if (pickS || pickC) {
+        if (nSpecular >= st.specularDepth) return false;
+        ++nSpecular;
+        type = kBounceReflective;
+        const V3 sn = pickC ? cn : n;
+        const float sr = pickC ? md.sampleRoughnessCoat : md.sampleRoughnessBase;
+        const V3 hv = sample_ggx_half(sn, du, sr);
+        next = normalize(reflect(rd, hv));
+        const float rc = clampf(dot(sn, next), 0.0f, 1.0f);
+        if (rc <= 0.0f) return false;
+        pdf = fmaxs((wr / wsum) * bsdf_pdf(M, n, cn, v, next), 1.0e-6f);
+        f = bsdf_eval(M, n, cn, v, next);
+        delta = sr <= kDeltaRoughness;
+    } else {
+        if (nDiffuse >= st.diffuseDepth) return false;
+        ++nDiffuse;
+        // Guide-free diffuse sampling: cosine pdf times the reflective share
+        // (CpuPathTracer.cpp:1948-1988 with no guide snapshot).
+        next = sample_cosine(n, du);
+        const float cpdf = clampf(dot(n, next), 0.0f, 1.0f) * kInvPi;
+        pdf = fmaxs((wr / wsum) * (1.0f * cpdf + 0.0f * 0.0f), 1.0e-6f);
+        f = bsdf_eval(M, n, cn, v, next);
+    }
+
+    const float cosT = pickT ? fmaxs(fabsf(dot(ng, next)), 1.0e-4f) : clampf(dot(n, next), 0.0f, 1.0f);
+    if (cosT <= 0.0f) return false;
+    ps.T = clamp_max_component(ps.T * f * (cosT / pdf), kThroughputClamp);
+    const V3 rn = pickT ? ng : n;
+    const float side = dot(rn, next) >= 0.0f ? 1.0f : -1.0f;
+    ps.o = sf.p + rn * (kRayEps * side);
+    ps.d = next;
+    ps.prevPdf = pdf;
+    ps.bits = type | (delta ? kBitsPrevDelta : 0u) | (inside ? kBitsInside : 0u) | (mediumOn ? kBitsMediumOn : 0u) |
+              (nDiffuse << kBitsDiffuseShift) | (nSpecular << kBitsSpecularShift);
+    if (depth >= st.rrDepth) {
+        const float q = clampf(max3(ps.T), 0.05f, 0.95f);
+        if (smp.next1(b0 + 4u) > q) return false;
+        ps.T = ps.T / q;
+    }
+    return true;
+}
+
+// TracePath (CpuPathTracer.cpp:1650-2017) in one thread: the megakernel
+// form.  Shadow rays are traced as soon as they are generated.
+template <bool COUNT>
+DEV PathState trace_path(const DevScene& S, const DevSettings& st, const DevCamera& cam, uint32_t x, uint32_t y,
+                         const Sampler& smp, Counters<COUNT>& c) {
+    PathState ps = init_path(cam, st, x, y, smp);
+    const uint32_t nLight = direct_sample_count(S, st);
     for (uint32_t depth = 0; depth < st.maxDepth; ++depth) {
-        const uint32_t b0 = depth * kBranchesPerDepth;
+        const Ray ray = path_ray(ps);
         if (COUNT) c.rays += 1;
         const HitRec h = intersect_closest(S, ray, c.trav());
         if (h.slot == kNoHit) {
-            if (depth > 0 || st.backgroundVisible) {
-                const V3 Le = environment_radiance(S, ray.d, c);
-                V3 dL;
-                if (S.domeCount == 0 || depth == 0 || prevDelta || prevType == kBounceTransmission) {
-                    dL = T * Le;
-                } else {
-                    const float sq2 = env_pdf_sq_sum(S, ray.d, c);
-                    const float b2 = prevPdf * prevPdf;
-                    dL = T * Le * (b2 / fmaxs(b2 + sq2, 1.0e-6f));
-                }
-                res.radiance = res.radiance + dL;
-            }
+            handle_miss(S, st, depth, ps, c);
             break;
         }
         if (COUNT) c.hits += 1;
-        const SurfaceHit hit = fetch_hit(S, ray, h);
-        if (mediumOn) {
-            T = T * V3{expf(-ext.x * h.t), expf(-ext.y * h.t), expf(-ext.z * h.t)};
+        Surface sf;
+        if (!shade_hit(S, depth, smp, ray, h, ps, sf)) continue;
+        const Mat M{S.materials + sf.material, S.derived + sf.material};
+        DirectSum ds;
+        ds.init(st);
+        const V3 origin = sf.p + sf.n * kRayEps;
+        for (uint32_t j = 0; j < nLight; ++j) {
+            V3 ld, contrib;
+            uint32_t group;
+            if (!direct_sample(S, st, sf, M, smp, depth, j, &ld, &contrib, &group, c)) continue;
+            ds.add(group, contrib, !occluded(S, origin, ld, kFltMax, c));
         }
-        const Mat M{S.materials + hit.material, S.derived + hit.material};
-        const shiro_hip_material& m = *M.m;
-        const MaterialDerived& md = *M.d;
-        const bool front = dot(hit.ng, ray.d) < 0.0f;
-        const V3 ng = front ? hit.ng : hit.ng * -1.0f;
-        V3 nbase = front ? hit.ns : hit.ns * -1.0f;
-        if (dot(nbase, ng) < 0.0f) nbase = nbase * -1.0f;
-        V3 n = nbase;
-        if (md.flags & kMatNormalOverride) n = resolve_normal(m, nbase, false);
-        if (dot(n, ng) < 0.0f) n = n * -1.0f;
-        V3 cn = n;
-        if (md.flags & kMatCoatNormalOverride) cn = resolve_normal(m, n, true);
-        const V3 v = ray.d * -1.0f;
-        if (md.flags & kMatOpacity) {
-            const float opacity = clampf(m.opacity, 0.0f, 1.0f);
-            if (smp.next1(b0) > opacity) {
-                ray.o = hit.p + ray.d * kRayEps;
-                continue;
-            }
-            T = T / fmaxs(opacity, 1.0e-4f);
-        }
-        if (depth == 0) {
-            res.albedo = ld3(md.diffuseColor);
-            res.normal = n * 0.5f + V3{0.5f, 0.5f, 0.5f};
-            res.depth = h.t;
-        }
-        if (md.flags & kMatEmissive) {
-            // Emission seen directly, through a delta or transmission
-            // bounce (CpuPathTracer.cpp:1728-1748).
-            if (depth == 0 || prevDelta || prevType == kBounceTransmission) {
-                res.radiance = res.radiance + T * ld3(md.emission);
-            }
-        }
-        res.radiance = res.radiance + T * direct_lighting(S, st, hit.p, n, cn, M, v, smp, depth, c);
-
-        const float wd = md.wDiffuse;
-        float ws = md.wBaseSpecular;
-        const float wc = md.wCoat;
-        const float wtRaw = md.wTransmission;
-        float wt = wtRaw;
-        if (wtRaw > 0.0f) {
-            const float ei = inside ? fmaxs(m.ior, 1.0f) : 1.0f;
-            const float et = inside ? 1.0f : fmaxs(m.ior, 1.0f);
-            const float R = fresnel_dielectric(fabsf(dot(ng, v)), ei, et);
-            const float pass = m.thinWalled ? pane_transmittance(R) : (1.0f - R);
-            wt = wtRaw * pass;
-            ws += wtRaw * (1.0f - pass);
-        }
-        const float wr = wd + ws + wc;
-        const float wsum = wr + wt;
-        if (wsum <= 0.0f) break;
-        const float sel = smp.next1(b0 + 1u) * wsum;
-        const V2 du = smp.next2(b0 + 2u);
-
-        V3 next{0.0f, 0.0f, 0.0f}, f{0.0f, 0.0f, 0.0f};
-        float pdf = 1.0f;
-        uint32_t type = kBounceDiffuse;
-        bool delta = false, pickT = false, pickS = false, pickC = false;
-        if (sel < wt) {
-            pickT = true;
-        } else {
-            const float rs = sel - wt;
-            if (rs < wd) {
-            } else if (rs < wd + ws) {
-                pickS = true;
-            } else {
-                pickC = true;
-            }
-        }
-
-        if (pickT) {
-            if (nSpecular >= st.specularDepth) break;
-            ++nSpecular;
-            type = kBounceTransmission;
-            const float pT = fmaxs(wt / wsum, 1.0e-6f);
-            const float rT = m.thinWalled ? 0.0f : md.sampleRoughnessTrans;
-            const float mt = clampf(m.transmission, 0.0f, 1.0f) * (1.0f - clampf(m.metallic, 0.0f, 1.0f));
-            if (m.thinWalled) {
-                next = ray.d;
-                const float R = fresnel_dielectric(fabsf(dot(ng, v)), 1.0f, fmaxs(m.ior, 1.0f));
-                const float pane = pane_transmittance(R);
-                const V3 scale = ld3(md.transmissionTint) * coat_transmission_scale(M, cn, v, next) * mt;
-                const float ac = fmaxs(fabsf(dot(ng, next)), 1.0e-4f);
-                f = scale * pane / ac;
-                pdf = pT;
-                delta = true;
-            } else {
-                const float ei = inside ? fmaxs(m.ior, 1.0f) : 1.0f;
-                const float et = inside ? 1.0f : fmaxs(m.ior, 1.0f);
-                const float etaRatio = ei / et;
-                const float relEta = et / fmaxs(ei, 1.0f);
-                const float R = fresnel_dielectric(fabsf(dot(ng, v)), ei, et);
-                bool refracted;
-                V3 mn = ng;
-                if (rT > kDeltaRoughness) {
-                    mn = sample_ggx_half(ng, du, rT);
-                    if (dot(mn, v) < 0.0f) mn = mn * -1.0f;
-                }
-                refracted = refract_dir(ray.d, mn, etaRatio, &next);
-                if (!refracted) {
-                    next = normalize(reflect(ray.d, mn));
-                    if (wr <= 0.0f) break;
-                    f = bsdf_eval(M, n, cn, v, next);
-                    pdf = fmaxs((wr / wsum) * bsdf_pdf(M, n, cn, v, next), 1.0e-6f);
-                    type = kBounceReflective;
-                    delta = rT <= kDeltaRoughness;
-                    pickT = false;
-                    pickS = true;
-                } else {
-                    if (rT <= kDeltaRoughness) {
-                        const V3 scale = ld3(md.transmissionTint) * coat_transmission_scale(M, cn, v, next) * mt;
-                        const float ac = fmaxs(fabsf(dot(ng, next)), 1.0e-4f);
-                        const float transport = 1.0f / fmaxs(relEta * relEta, 1.0e-6f);
-                        f = scale * (1.0f - R) * transport / ac;
-                        pdf = pT;
-                        delta = true;
-                    } else {
-                        f = transmission_lobe(M, ng, cn, v, next, ei, et, rT);
-                        pdf = fmaxs(pT * transmission_pdf(ng, v, next, et / ei, rT), 1.0e-6f);
-                        delta = false;
-                    }
-                    inside = !inside;
-                    if (inside) {
-                        ext = ld3(md.mediumExtinction);
-                        mediumOn = m.transmissionDepth > 0.0f || max3(ld3(m.transmissionScatter)) > 0.0f;
-                    } else {
-                        mediumOn = false;
-                    }
-                }
-            }
-        } else if (pickS || pickC) {
-            if (nSpecular >= st.specularDepth) break;
-            ++nSpecular;
-            type = kBounceReflective;
-            const V3 sn = pickC ? cn : n;
-            const float sr = pickC ? md.sampleRoughnessCoat : md.sampleRoughnessBase;
-            const V3 hv = sample_ggx_half(sn, du, sr);
-            next = normalize(reflect(ray.d, hv));
-            const float rc = clampf(dot(sn, next), 0.0f, 1.0f);
-            if (rc <= 0.0f) break;
-            pdf = fmaxs((wr / wsum) * bsdf_pdf(M, n, cn, v, next), 1.0e-6f);
-            f = bsdf_eval(M, n, cn, v, next);
-            delta = sr <= kDeltaRoughness;
-        } else {
-            if (nDiffuse >= st.diffuseDepth) break;
-            ++nDiffuse;
-            next = sample_cosine(n, du);
-            const float cpdf = clampf(dot(n, next), 0.0f, 1.0f) * kInvPi;
-            pdf = fmaxs((wr / wsum) * (1.0f * cpdf + 0.0f * 0.0f), 1.0e-6f);
-            f = bsdf_eval(M, n, cn, v, next);
-        }
-
-        const float cosT = pickT ? fmaxs(fabsf(dot(ng, next)), 1.0e-4f) : clampf(dot(n, next), 0.0f, 1.0f);
-        if (cosT <= 0.0f) break;
-        T = clamp_max_component(T * f * (cosT / pdf), kThroughputClamp);
-        const V3 rn = pickT ? ng : n;
-        const float side = dot(rn, next) >= 0.0f ? 1.0f : -1.0f;
-        ray.o = hit.p + rn * (kRayEps * side);
-        ray.d = next;
-        prevType = type;
-        prevDelta = delta;
-        prevPdf = pdf;
-        if (depth >= st.rrDepth) {
-            const float q = clampf(max3(T), 0.05f, 0.95f);
-            if (smp.next1(b0 + 4u) > q) break;
-            T = T / q;
-        }
+        ds.flush();
+        ps.L = ps.L + ps.T * ds.L;
+        if (!sample_bounce(S, st, depth, smp, sf, ps)) break;
     }
-    return res;
+    return ps;
 }
 
 }  // namespace shiro_hip

@@ -17,7 +17,37 @@ struct AccumParams {
     uint32_t* depthCount;   // 1
 };
 
+// Wavefront path state (structure of arrays, `cap` paths) and queues.
+struct WfBuffers {
+    float4 *o, *d, *T, *L, *E, *A, *N, *H;  // ray origin|prevPdf, dir|bits, throughput, radiance,
+                                            // extinction, albedo|first-hit t, normal, hit record
+    float4 *so, *st;                        // shadow header: origin|count, throughput at the hit
+    float4* sr;                             // shadow rays: maxShadow x {dir|group, contribution}
+    uint32_t *qa, *qb, *qs;                 // ray queues (ping-pong) and shadow queue
+    uint32_t* counts;                       // queue lengths: rays per depth, then shadows per depth
+    uint32_t countSlots;
+    uint32_t shadowCountBase;
+    uint32_t maxShadow;
+    uint32_t cap;
+};
+
+struct WfParams {
+    DevScene scene;
+    DevCamera camera;
+    DevSettings settings;
+    DevBatchSums sums;
+    DevFrame frame;
+    WfBuffers B;
+    uint32_t sampleStart, sliceBegin, sliceCount, batchCount;
+    uint32_t tilesX, tiles;
+};
+
+struct WfLaunch {
+    uint32_t maxBlocks;  // grid of the grid-stride stage kernels
+};
+
 hipError_t launch_render(const RenderParams& P, bool count, hipStream_t stream);
+hipError_t launch_wavefront_pass(const WfParams& P, const WfLaunch& L, hipStream_t s);
 hipError_t launch_accum_add(const AccumParams& A, const DevFrame& f, uint32_t count, uint32_t np, hipStream_t s);
 hipError_t launch_accum_resolve(const AccumParams& A, const DevFrame& f, uint32_t total, uint32_t np, hipStream_t s);
 hipError_t launch_qmc_fill(uint32_t x0, uint32_t y0, uint32_t w, uint32_t h, uint32_t s, uint32_t dim0,

@@ -61,11 +61,10 @@ __global__ __launch_bounds__(64) void k_render(RenderParams P) {
     Counters<COUNT> cnt;
     for (uint32_t i = 0; i < P.sliceCount; ++i) {
         const Sampler smp{x, y, P.sampleStart + P.sliceBegin + i};
-        const Ray r = camera_ray(P.camera, st, x, y, smp.next2(0u));
-        const PathResult pr = trace_path(P.scene, st, r, smp, cnt);
-        beauty.x = beauty.x + pr.radiance.x;
-        beauty.y = beauty.y + pr.radiance.y;
-        beauty.z = beauty.z + pr.radiance.z;
+        const PathState pr = trace_path(P.scene, st, P.camera, x, y, smp, cnt);
+        beauty.x = beauty.x + pr.L.x;
+        beauty.y = beauty.y + pr.L.y;
+        beauty.z = beauty.z + pr.L.z;
         beauty.w = beauty.w + (pr.depth < kInf ? 1.0f : 0.0f);
         albedo.x = albedo.x + pr.albedo.x;
         albedo.y = albedo.y + pr.albedo.y;

@@ -1,0 +1,281 @@
+// Wavefront transport: the path integrator split into stage kernels over
+// path state kept in HBM (structure of arrays), with ray queues compacted by
+// wave ballots.  Per pass (a slice of samples for every pixel):
+//
+//   k_wf_reset     clear the queue counters of the pass
+//   k_wf_generate  camera rays for every (pixel, sample) path -> ray queue
+//   per depth d:
+//     k_wf_extend  closest hit of every queued ray (BVH4 traversal only)
+//     k_wf_shade   surface terms, light samples -> shadow rays, BSDF
+//                  sampling and roulette -> next ray queue
+//     k_wf_shadow  any-hit tests of the shadow rays; visible contributions
+//                  are summed per path in DirectLighting's order
+//   k_wf_accum     per-pixel sums over the pass's samples, in sample order
+//
+// Every stage uses the same device functions as the megakernel
+// (dev_path.h), shadow contributions are applied in the reference's light
+// order before the next bounce, and pixels accumulate samples in order, so
+// both transports produce the same per-sample radiance up to the float
+// rounding of identical operations.  Grids are fixed (grid-stride loops read
+// the queue lengths from device memory), so no host round trip is needed
+// between stages.
+#include <hip/hip_runtime.h>
+
+#include "dev_path.h"
+#include "kernels.h"
+
+namespace shiro_hip {
+
+DEV uint32_t lane_id() { return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u)); }
+
+// Wave-aggregated append: one atomic per wave, slots in lane order.
+DEV void wave_push(bool push, uint32_t value, uint32_t* queue, uint32_t* count) {
+    const unsigned long long mask = __ballot(push);
+    if (mask == 0ull) return;
+    const uint32_t lane = lane_id();
+    const int leader = __ffsll(static_cast<long long>(mask)) - 1;
+    uint32_t base = 0;
+    if (static_cast<int>(lane) == leader) base = atomicAdd(count, static_cast<uint32_t>(__popcll(mask)));
+    base = __shfl(base, leader);
+    if (push) {
+        const unsigned long long below = mask & ((1ull << lane) - 1ull);
+        queue[base + static_cast<uint32_t>(__popcll(below))] = value;
+    }
+}
+
+struct PathCoord {
+    uint32_t x, y, s;
+};
+
+DEV PathCoord path_coord(const WfParams& P, uint32_t p) {
+    const uint32_t per = P.tiles * 64u;
+    const uint32_t s = p / per;
+    const uint32_t q = p - s * per;
+    const uint32_t tile = q >> 6, lane = q & 63u;
+    return PathCoord{(tile % P.tilesX) * 8u + (lane & 7u), (tile / P.tilesX) * 8u + (lane >> 3), s};
+}
+
+DEV Sampler path_sampler(const WfParams& P, const PathCoord& c) {
+    return Sampler{c.x, c.y, P.sampleStart + P.sliceBegin + c.s};
+}
+
+DEV PathState load_state(const WfBuffers& B, uint32_t p) {
+    PathState ps;
+    const float4 o = B.o[p], d = B.d[p], T = B.T[p], L = B.L[p], E = B.E[p], A = B.A[p], N = B.N[p];
+    ps.o = xyz(o);
+    ps.prevPdf = o.w;
+    ps.d = xyz(d);
+    ps.bits = __float_as_uint(d.w);
+    ps.T = xyz(T);
+    ps.L = xyz(L);
+    ps.ext = xyz(E);
+    ps.albedo = xyz(A);
+    ps.depth = A.w;
+    ps.normal = xyz(N);
+    return ps;
+}
+
+DEV void store_state(const WfBuffers& B, uint32_t p, const PathState& ps) {
+    B.o[p] = make_float4(ps.o.x, ps.o.y, ps.o.z, ps.prevPdf);
+    B.d[p] = make_float4(ps.d.x, ps.d.y, ps.d.z, __uint_as_float(ps.bits));
+    B.T[p] = make_float4(ps.T.x, ps.T.y, ps.T.z, 0.0f);
+    B.L[p] = make_float4(ps.L.x, ps.L.y, ps.L.z, 0.0f);
+    B.E[p] = make_float4(ps.ext.x, ps.ext.y, ps.ext.z, 0.0f);
+    B.A[p] = make_float4(ps.albedo.x, ps.albedo.y, ps.albedo.z, ps.depth);
+    B.N[p] = make_float4(ps.normal.x, ps.normal.y, ps.normal.z, 0.0f);
+}
+
+__global__ void k_wf_reset(uint32_t* counts, uint32_t n) {
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) counts[i] = 0u;
+}
+
+__global__ __launch_bounds__(64) void k_wf_generate(WfParams P) {
+    const uint32_t total = P.sliceCount * P.tiles * 64u;
+    for (uint32_t base = blockIdx.x * 64u; base < total; base += gridDim.x * 64u) {
+        const uint32_t p = base + threadIdx.x;
+        bool valid = false;
+        if (p < total) {
+            const PathCoord c = path_coord(P, p);
+            valid = c.x < P.settings.width && c.y < P.settings.height;
+            if (valid) store_state(P.B, p, init_path(P.camera, P.settings, c.x, c.y, path_sampler(P, c)));
+        }
+        wave_push(valid, p, P.B.qa, &P.B.counts[0]);
+    }
+}
+
+__global__ __launch_bounds__(64) void k_wf_extend(WfParams P, uint32_t depth) {
+    const uint32_t* in = (depth & 1u) ? P.B.qb : P.B.qa;
+    const uint32_t n = P.B.counts[depth];
+    for (uint32_t i = blockIdx.x * 64u + threadIdx.x; i < n; i += gridDim.x * 64u) {
+        const uint32_t p = in[i];
+        const float4 o = P.B.o[p], d = P.B.d[p];
+        Ray r;
+        r.o = xyz(o);
+        r.d = xyz(d);
+        r.tmin = kRayEps;
+        r.tmax = kFltMax;
+        Counters<false> c;
+        const HitRec h = intersect_closest(P.scene, r, c.trav());
+        P.B.H[p] = make_float4(h.t, h.b1, h.b2, __uint_as_float(h.slot));
+    }
+}
+
+__global__ __launch_bounds__(64) void k_wf_shade(WfParams P, uint32_t depth) {
+    const uint32_t* in = (depth & 1u) ? P.B.qb : P.B.qa;
+    uint32_t* out = (depth & 1u) ? P.B.qa : P.B.qb;
+    const uint32_t n = P.B.counts[depth];
+    const uint32_t nLight = direct_sample_count(P.scene, P.settings);
+    const bool more = depth + 1u < P.settings.maxDepth;
+    for (uint32_t base = blockIdx.x * 64u; base < n; base += gridDim.x * 64u) {
+        const uint32_t i = base + threadIdx.x;
+        bool alive = false, shadow = false;
+        uint32_t p = 0;
+        if (i < n) {
+            p = in[i];
+            const PathCoord pc = path_coord(P, p);
+            const Sampler smp = path_sampler(P, pc);
+            PathState ps = load_state(P.B, p);
+            const float4 hh = P.B.H[p];
+            HitRec h;
+            h.t = hh.x;
+            h.b1 = hh.y;
+            h.b2 = hh.z;
+            h.slot = __float_as_uint(hh.w);
+            Counters<false> c;
+            if (h.slot == kNoHit) {
+                handle_miss(P.scene, P.settings, depth, ps, c);
+            } else {
+                const Ray ray = path_ray(ps);
+                Surface sf;
+                if (!shade_hit(P.scene, depth, smp, ray, h, ps, sf)) {
+                    alive = true;
+                } else {
+                    const Mat M{P.scene.materials + sf.material, P.scene.derived + sf.material};
+                    uint32_t ns = 0;
+                    float4* sr = P.B.sr + static_cast<size_t>(p) * P.B.maxShadow * 2u;
+                    for (uint32_t j = 0; j < nLight; ++j) {
+                        V3 ld, contrib;
+                        uint32_t group;
+                        if (!direct_sample(P.scene, P.settings, sf, M, smp, depth, j, &ld, &contrib, &group, c)) continue;
+                        sr[2u * ns] = make_float4(ld.x, ld.y, ld.z, __uint_as_float(group));
+                        sr[2u * ns + 1u] = make_float4(contrib.x, contrib.y, contrib.z, 0.0f);
+                        ++ns;
+                    }
+                    if (ns > 0u) {
+                        const V3 origin = sf.p + sf.n * kRayEps;
+                        P.B.so[p] = make_float4(origin.x, origin.y, origin.z, __uint_as_float(ns));
+                        P.B.st[p] = make_float4(ps.T.x, ps.T.y, ps.T.z, 0.0f);
+                        shadow = true;
+                    }
+                    alive = sample_bounce(P.scene, P.settings, depth, smp, sf, ps);
+                }
+            }
+            store_state(P.B, p, ps);
+        }
+        wave_push(shadow, p, P.B.qs, &P.B.counts[P.B.shadowCountBase + depth]);
+        wave_push(alive && more, p, out, &P.B.counts[depth + 1u]);
+    }
+}
+
+__global__ __launch_bounds__(64) void k_wf_shadow(WfParams P, uint32_t depth) {
+    const uint32_t n = P.B.counts[P.B.shadowCountBase + depth];
+    for (uint32_t i = blockIdx.x * 64u + threadIdx.x; i < n; i += gridDim.x * 64u) {
+        const uint32_t p = P.B.qs[i];
+        const float4 so = P.B.so[p];
+        const V3 origin = xyz(so);
+        const uint32_t ns = __float_as_uint(so.w);
+        const float4* sr = P.B.sr + static_cast<size_t>(p) * P.B.maxShadow * 2u;
+        DirectSum ds;
+        ds.init(P.settings);
+        Counters<false> c;
+        for (uint32_t j = 0; j < ns; ++j) {
+            const float4 a = sr[2u * j], b = sr[2u * j + 1u];
+            ds.add(__float_as_uint(a.w), xyz(b), !occluded(P.scene, origin, xyz(a), kFltMax, c));
+        }
+        ds.flush();
+        const float4 L = P.B.L[p];
+        const V3 Lv = xyz(L) + xyz(P.B.st[p]) * ds.L;
+        P.B.L[p] = make_float4(Lv.x, Lv.y, Lv.z, L.w);
+    }
+}
+
+// Per-pixel sums over the pass's samples in sample order, then the batch
+// means on the last pass (CpuPathTracer.cpp:2098-2137).
+__global__ __launch_bounds__(64) void k_wf_accum(WfParams P) {
+    const uint32_t tile = blockIdx.x;
+    const uint32_t lane = threadIdx.x;
+    const uint32_t x = (tile % P.tilesX) * 8u + (lane & 7u);
+    const uint32_t y = (tile / P.tilesX) * 8u + (lane >> 3);
+    const DevSettings& st = P.settings;
+    if (x >= st.width || y >= st.height) return;
+    const size_t pix = static_cast<size_t>(y) * st.width + x;
+    float4 beauty, albedo, normal;
+    float depth;
+    if (P.sliceBegin == 0u) {
+        beauty = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        albedo = normal = beauty;
+        depth = kInf;
+    } else {
+        beauty = P.sums.beauty[pix];
+        albedo = P.sums.albedo[pix];
+        normal = P.sums.normal[pix];
+        depth = P.sums.depth[pix];
+    }
+    const uint32_t per = P.tiles * 64u;
+    for (uint32_t s = 0; s < P.sliceCount; ++s) {
+        const uint32_t p = s * per + tile * 64u + lane;
+        const float4 L = P.B.L[p], A = P.B.A[p], N = P.B.N[p];
+        beauty.x = beauty.x + L.x;
+        beauty.y = beauty.y + L.y;
+        beauty.z = beauty.z + L.z;
+        beauty.w = beauty.w + (A.w < kInf ? 1.0f : 0.0f);
+        albedo.x = albedo.x + A.x;
+        albedo.y = albedo.y + A.y;
+        albedo.z = albedo.z + A.z;
+        normal.x = normal.x + N.x;
+        normal.y = normal.y + N.y;
+        normal.z = normal.z + N.z;
+        if (A.w < depth) depth = A.w;
+    }
+    if (P.sliceBegin + P.sliceCount < P.batchCount) {
+        P.sums.beauty[pix] = beauty;
+        P.sums.albedo[pix] = albedo;
+        P.sums.normal[pix] = normal;
+        P.sums.depth[pix] = depth;
+        return;
+    }
+    const float nf = static_cast<float>(P.batchCount);
+    const float inv = 1.0f / nf;
+    float* b = P.frame.beauty + pix * 4u;
+    b[0] = beauty.x / nf;
+    b[1] = beauty.y / nf;
+    b[2] = beauty.z / nf;
+    b[3] = beauty.w / nf;
+    float* a = P.frame.albedo + pix * 3u;
+    a[0] = albedo.x * inv;
+    a[1] = albedo.y * inv;
+    a[2] = albedo.z * inv;
+    float* nn = P.frame.normal + pix * 3u;
+    nn[0] = normal.x * inv;
+    nn[1] = normal.y * inv;
+    nn[2] = normal.z * inv;
+    P.frame.depth[pix] = depth;
+}
+
+hipError_t launch_wavefront_pass(const WfParams& P, const WfLaunch& L, hipStream_t s) {
+    const uint32_t total = P.sliceCount * P.tiles * 64u;
+    if (total == 0) return hipSuccess;
+    const uint32_t genBlocks = (total + 63u) / 64u;
+    const uint32_t blocks = genBlocks < L.maxBlocks ? genBlocks : L.maxBlocks;
+    hipLaunchKernelGGL(k_wf_reset, dim3(1), dim3(256), 0, s, P.B.counts, P.B.countSlots);
+    hipLaunchKernelGGL(k_wf_generate, dim3(blocks), dim3(64), 0, s, P);
+    for (uint32_t d = 0; d < P.settings.maxDepth; ++d) {
+        hipLaunchKernelGGL(k_wf_extend, dim3(blocks), dim3(64), 0, s, P, d);
+        hipLaunchKernelGGL(k_wf_shade, dim3(blocks), dim3(64), 0, s, P, d);
+        hipLaunchKernelGGL(k_wf_shadow, dim3(blocks), dim3(64), 0, s, P, d);
+    }
+    hipLaunchKernelGGL(k_wf_accum, dim3(P.tiles), dim3(64), 0, s, P);
+    return hipGetLastError();
+}
+
+}  // namespace shiro_hip
