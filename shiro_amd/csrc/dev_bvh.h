@@ -122,22 +122,27 @@ struct TravCount {
 // the entry distances.  Near and far planes are chosen by the sign of the
 // ray direction, so an inverted box (lo = +inf, hi = -inf: an empty child
 // slot) always misses; the inverse direction is finite, so no NaN arises.
+// Plane distances are fused multiply-adds (plane * inv - origin * inv): the
+// boxes are padded on the host, so the box test only has to be conservative,
+// and the triangle test alone decides hits.
 DEV uint32_t node_boxes(const float4* __restrict__ nd, const RayPre& r, float tmax, float tn[4]) {
     const float4 lox = nd[0], hix = nd[1], loy = nd[2], hiy = nd[3], loz = nd[4], hiz = nd[5];
     const bool px = r.inv.x >= 0.0f, py = r.inv.y >= 0.0f, pz = r.inv.z >= 0.0f;
     const float4 nx = px ? lox : hix, fx = px ? hix : lox;
     const float4 ny = py ? loy : hiy, fy = py ? hiy : loy;
     const float4 nz = pz ? loz : hiz, fz = pz ? hiz : loz;
-    const float ox = r.o.x, oy = r.o.y, oz = r.o.z;
     const float ix = r.inv.x, iy = r.inv.y, iz = r.inv.z;
+    const float ox = -r.o.x * ix, oy = -r.o.y * iy, oz = -r.o.z * iz;
     const float NX[4] = {nx.x, nx.y, nx.z, nx.w}, FX[4] = {fx.x, fx.y, fx.z, fx.w};
     const float NY[4] = {ny.x, ny.y, ny.z, ny.w}, FY[4] = {fy.x, fy.y, fy.z, fy.w};
     const float NZ[4] = {nz.x, nz.y, nz.z, nz.w}, FZ[4] = {fz.x, fz.y, fz.z, fz.w};
     uint32_t mask = 0;
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
-        const float t0 = fmaxf(fmaxf((NX[c] - ox) * ix, (NY[c] - oy) * iy), fmaxf((NZ[c] - oz) * iz, r.tmin));
-        const float t1 = fminf(fminf((FX[c] - ox) * ix, (FY[c] - oy) * iy), fminf((FZ[c] - oz) * iz, tmax));
+        const float t0 = fmaxf(fmaxf(__builtin_fmaf(NX[c], ix, ox), __builtin_fmaf(NY[c], iy, oy)),
+                               fmaxf(__builtin_fmaf(NZ[c], iz, oz), r.tmin));
+        const float t1 = fminf(fminf(__builtin_fmaf(FX[c], ix, ox), __builtin_fmaf(FY[c], iy, oy)),
+                               fminf(__builtin_fmaf(FZ[c], iz, oz), tmax));
         tn[c] = t0;
         mask |= (t0 <= t1 ? 1u : 0u) << c;
     }
@@ -180,6 +185,10 @@ DEV void leaf_closest(const DevScene& S, const RayPre& r, uint32_t slot, HitRec&
     }
 }
 
+// Closest hit.  "While-while" loop: each lane descends internal nodes
+// until it reaches a leaf (or runs out of work), and only then do the lanes
+// intersect leaf triangles, so node steps and triangle tests of different
+// lanes are not interleaved within one iteration.
 template <bool COUNT>
 DEV HitRec intersect_closest(const DevScene& S, const Ray& ray, const TravCount<COUNT>& cnt) {
     HitRec best;
@@ -194,24 +203,18 @@ DEV HitRec intersect_closest(const DevScene& S, const Ray& ray, const TravCount<
     int sp = 0;
     uint32_t cur = 0u;  // the root is always an internal node
     for (;;) {
-        if (cur & kLeafBit) {
-            leaf_closest(S, r, cur & ~kLeafBit, best, cnt);
-        } else {
+        // Node phase.
+        while (cur != kEmpty && !(cur & kLeafBit)) {
             const float4* nd = S.nodes + static_cast<size_t>(cur) * 8u;
             cnt.node();
             float tn[4];
             const uint32_t mask = node_boxes(nd, r, best.t, tn);
             if (mask) {
                 const float4 cw = nd[6];
-                const uint32_t word[4] = {__float_as_uint(cw.x), __float_as_uint(cw.y), __float_as_uint(cw.z),
-                                          __float_as_uint(cw.w)};
-                float k[4];
-                uint32_t v[4];
-#pragma unroll
-                for (int c = 0; c < 4; ++c) {
-                    k[c] = (mask >> c) & 1u ? tn[c] : kInf;
-                    v[c] = word[c];
-                }
+                float k[4] = {(mask & 1u) ? tn[0] : kInf, (mask & 2u) ? tn[1] : kInf, (mask & 4u) ? tn[2] : kInf,
+                              (mask & 8u) ? tn[3] : kInf};
+                uint32_t v[4] = {__float_as_uint(cw.x), __float_as_uint(cw.y), __float_as_uint(cw.z),
+                                 __float_as_uint(cw.w)};
                 cswap(k[0], v[0], k[1], v[1]);
                 cswap(k[2], v[2], k[3], v[3]);
                 cswap(k[0], v[0], k[2], v[2]);
@@ -225,25 +228,35 @@ DEV HitRec intersect_closest(const DevScene& S, const Ray& ray, const TravCount<
                     ++sp;
                 }
                 cur = v[0];
-                continue;
+            } else {
+                cur = kEmpty;
+                while (sp > 0) {
+                    --sp;
+                    if (stackT[sp] <= best.t) {
+                        cur = stackNode[sp];
+                        break;
+                    }
+                }
             }
         }
-        // Pop the nearest pending entry that can still beat the best hit.
-        bool found = false;
+        if (cur == kEmpty) break;
+        // Leaf phase.
+        leaf_closest(S, r, cur & ~kLeafBit, best, cnt);
+        cur = kEmpty;
         while (sp > 0) {
             --sp;
             if (stackT[sp] <= best.t) {
                 cur = stackNode[sp];
-                found = true;
                 break;
             }
         }
-        if (!found) break;
+        if (cur == kEmpty) break;
     }
     return best;
 }
 
 // Any hit in [tmin, tmax]: Occluded (CpuPathTracer.cpp:659-681, 697-709).
+// Same while-while structure, unordered, first accepted triangle ends it.
 template <bool COUNT>
 DEV bool intersect_any(const DevScene& S, const Ray& ray, const TravCount<COUNT>& cnt) {
     if (S.nodeCount == 0) return false;
@@ -252,18 +265,7 @@ DEV bool intersect_any(const DevScene& S, const Ray& ray, const TravCount<COUNT>
     int sp = 0;
     uint32_t cur = 0u;
     for (;;) {
-        if (cur & kLeafBit) {
-            uint32_t slot = cur & ~kLeafBit;
-            for (;;) {
-                const float4* tp = S.tris + static_cast<size_t>(slot) * 3u;
-                const float4 a = tp[0], b = tp[1], c = tp[2];
-                cnt.tri();
-                float t, b1, b2;
-                if (tri_test(r, xyz(a), xyz(b), xyz(c), ray.tmax, &t, &b1, &b2)) return true;
-                if (__float_as_uint(b.w) & kLeafEndFlag) break;
-                ++slot;
-            }
-        } else {
+        while (cur != kEmpty && !(cur & kLeafBit)) {
             const float4* nd = S.nodes + static_cast<size_t>(cur) * 8u;
             cnt.node();
             float tn[4];
@@ -284,11 +286,22 @@ DEV bool intersect_any(const DevScene& S, const Ray& ray, const TravCount<COUNT>
                     }
                 }
                 cur = word[first];
-                continue;
+            } else {
+                cur = sp > 0 ? stackNode[--sp] : kEmpty;
             }
         }
-        if (sp == 0) break;
-        cur = stackNode[--sp];
+        if (cur == kEmpty) break;
+        uint32_t slot = cur & ~kLeafBit;
+        for (;;) {
+            const float4* tp = S.tris + static_cast<size_t>(slot) * 3u;
+            const float4 a = tp[0], b = tp[1], c = tp[2];
+            cnt.tri();
+            float t, b1, b2;
+            if (tri_test(r, xyz(a), xyz(b), xyz(c), ray.tmax, &t, &b1, &b2)) return true;
+            if (__float_as_uint(b.w) & kLeafEndFlag) break;
+            ++slot;
+        }
+        cur = sp > 0 ? stackNode[--sp] : kEmpty;
     }
     return false;
 }

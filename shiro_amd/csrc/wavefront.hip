@@ -3,9 +3,10 @@
 // wave ballots.  Per pass (a slice of samples for every pixel):
 //
 //   k_wf_reset     clear the queue counters of the pass
-//   k_wf_generate  camera rays for every (pixel, sample) path -> ray queue
 //   per depth d:
-//     k_wf_extend  closest hit of every queued ray (BVH4 traversal only)
+//     k_wf_extend  closest hit of every queued ray (BVH4 traversal only); at
+//                  depth 0 every (pixel, sample) path of the pass, its camera
+//                  ray computed in place (no queue, no stored ray)
 //     k_wf_shade   surface terms, light samples -> shadow rays, BSDF
 //                  sampling and roulette -> next ray queue
 //     k_wf_shadow  any-hit tests of the shadow rays; visible contributions
@@ -59,61 +60,77 @@ DEV Sampler path_sampler(const WfParams& P, const PathCoord& c) {
     return Sampler{c.x, c.y, P.sampleStart + P.sliceBegin + c.s};
 }
 
+// Loads what a bounce reads: ray, throughput, radiance, flags; the medium
+// extinction only inside a medium.  First-hit AOVs are written once at depth
+// 0 and never read back here.
 DEV PathState load_state(const WfBuffers& B, uint32_t p) {
     PathState ps;
-    const float4 o = B.o[p], d = B.d[p], T = B.T[p], L = B.L[p], E = B.E[p], A = B.A[p], N = B.N[p];
+    const float4 o = B.o[p], d = B.d[p], T = B.T[p], L = B.L[p];
     ps.o = xyz(o);
     ps.prevPdf = o.w;
     ps.d = xyz(d);
     ps.bits = __float_as_uint(d.w);
     ps.T = xyz(T);
     ps.L = xyz(L);
-    ps.ext = xyz(E);
-    ps.albedo = xyz(A);
-    ps.depth = A.w;
-    ps.normal = xyz(N);
+    ps.ext = V3{0.0f, 0.0f, 0.0f};
+    if (ps.bits & kBitsMediumOn) ps.ext = xyz(B.E[p]);
+    ps.albedo = ps.normal = V3{0.0f, 0.0f, 0.0f};
+    ps.depth = kInf;
     return ps;
 }
 
-DEV void store_state(const WfBuffers& B, uint32_t p, const PathState& ps) {
-    B.o[p] = make_float4(ps.o.x, ps.o.y, ps.o.z, ps.prevPdf);
-    B.d[p] = make_float4(ps.d.x, ps.d.y, ps.d.z, __uint_as_float(ps.bits));
-    B.T[p] = make_float4(ps.T.x, ps.T.y, ps.T.z, 0.0f);
+// Stores the state after a bounce.  `alive` paths keep their ray and
+// throughput; every path keeps its radiance; the AOVs are stored at depth 0;
+// the extinction only when it was (re)set by entering a medium.
+DEV void store_state(const WfBuffers& B, uint32_t p, const PathState& ps, uint32_t depth, bool alive,
+                     bool extChanged) {
+    if (alive) {
+        B.o[p] = make_float4(ps.o.x, ps.o.y, ps.o.z, ps.prevPdf);
+        B.d[p] = make_float4(ps.d.x, ps.d.y, ps.d.z, __uint_as_float(ps.bits));
+        B.T[p] = make_float4(ps.T.x, ps.T.y, ps.T.z, 0.0f);
+        if (extChanged) B.E[p] = make_float4(ps.ext.x, ps.ext.y, ps.ext.z, 0.0f);
+    }
     B.L[p] = make_float4(ps.L.x, ps.L.y, ps.L.z, 0.0f);
-    B.E[p] = make_float4(ps.ext.x, ps.ext.y, ps.ext.z, 0.0f);
-    B.A[p] = make_float4(ps.albedo.x, ps.albedo.y, ps.albedo.z, ps.depth);
-    B.N[p] = make_float4(ps.normal.x, ps.normal.y, ps.normal.z, 0.0f);
+    if (depth == 0u) {
+        B.A[p] = make_float4(ps.albedo.x, ps.albedo.y, ps.albedo.z, ps.depth);
+        B.N[p] = make_float4(ps.normal.x, ps.normal.y, ps.normal.z, 0.0f);
+    }
 }
 
 __global__ void k_wf_reset(uint32_t* counts, uint32_t n) {
     for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) counts[i] = 0u;
 }
 
-__global__ __launch_bounds__(64) void k_wf_generate(WfParams P) {
-    const uint32_t total = P.sliceCount * P.tiles * 64u;
-    for (uint32_t base = blockIdx.x * 64u; base < total; base += gridDim.x * 64u) {
-        const uint32_t p = base + threadIdx.x;
-        bool valid = false;
-        if (p < total) {
-            const PathCoord c = path_coord(P, p);
-            valid = c.x < P.settings.width && c.y < P.settings.height;
-            if (valid) store_state(P.B, p, init_path(P.camera, P.settings, c.x, c.y, path_sampler(P, c)));
-        }
-        wave_push(valid, p, P.B.qa, &P.B.counts[0]);
-    }
+DEV bool path_valid(const WfParams& P, const PathCoord& c) {
+    return c.x < P.settings.width && c.y < P.settings.height;
+}
+
+// Number of work items of stage `depth`: every path of the pass at depth 0,
+// the length of the ray queue afterwards.
+DEV uint32_t stage_items(const WfParams& P, uint32_t depth) {
+    return depth == 0u ? P.sliceCount * P.tiles * 64u : P.B.counts[depth];
 }
 
 __global__ __launch_bounds__(64) void k_wf_extend(WfParams P, uint32_t depth) {
     const uint32_t* in = (depth & 1u) ? P.B.qb : P.B.qa;
-    const uint32_t n = P.B.counts[depth];
+    const uint32_t n = stage_items(P, depth);
     for (uint32_t i = blockIdx.x * 64u + threadIdx.x; i < n; i += gridDim.x * 64u) {
-        const uint32_t p = in[i];
-        const float4 o = P.B.o[p], d = P.B.d[p];
         Ray r;
-        r.o = xyz(o);
-        r.d = xyz(d);
-        r.tmin = kRayEps;
-        r.tmax = kFltMax;
+        uint32_t p;
+        if (depth == 0u) {
+            p = i;
+            const PathCoord c = path_coord(P, p);
+            if (!path_valid(P, c)) continue;
+            const Sampler smp = path_sampler(P, c);
+            r = camera_ray(P.camera, P.settings, c.x, c.y, smp.next2(0u));
+        } else {
+            p = in[i];
+            const float4 o = P.B.o[p], d = P.B.d[p];
+            r.o = xyz(o);
+            r.d = xyz(d);
+            r.tmin = kRayEps;
+            r.tmax = kFltMax;
+        }
         Counters<false> c;
         const HitRec h = intersect_closest(P.scene, r, c.trav());
         P.B.H[p] = make_float4(h.t, h.b1, h.b2, __uint_as_float(h.slot));
@@ -123,18 +140,23 @@ __global__ __launch_bounds__(64) void k_wf_extend(WfParams P, uint32_t depth) {
 __global__ __launch_bounds__(64) void k_wf_shade(WfParams P, uint32_t depth) {
     const uint32_t* in = (depth & 1u) ? P.B.qb : P.B.qa;
     uint32_t* out = (depth & 1u) ? P.B.qa : P.B.qb;
-    const uint32_t n = P.B.counts[depth];
+    const uint32_t n = stage_items(P, depth);
     const uint32_t nLight = direct_sample_count(P.scene, P.settings);
     const bool more = depth + 1u < P.settings.maxDepth;
     for (uint32_t base = blockIdx.x * 64u; base < n; base += gridDim.x * 64u) {
         const uint32_t i = base + threadIdx.x;
-        bool alive = false, shadow = false;
+        bool alive = false, shadow = false, extChanged = false;
         uint32_t p = 0;
-        if (i < n) {
-            p = in[i];
-            const PathCoord pc = path_coord(P, p);
+        bool work = i < n;
+        PathCoord pc{};
+        if (work) {
+            p = depth == 0u ? i : in[i];
+            pc = path_coord(P, p);
+            work = depth > 0u || path_valid(P, pc);
+        }
+        if (work) {
             const Sampler smp = path_sampler(P, pc);
-            PathState ps = load_state(P.B, p);
+            PathState ps = depth == 0u ? init_path(P.camera, P.settings, pc.x, pc.y, smp) : load_state(P.B, p);
             const float4 hh = P.B.H[p];
             HitRec h;
             h.t = hh.x;
@@ -167,10 +189,12 @@ __global__ __launch_bounds__(64) void k_wf_shade(WfParams P, uint32_t depth) {
                         P.B.st[p] = make_float4(ps.T.x, ps.T.y, ps.T.z, 0.0f);
                         shadow = true;
                     }
+                    const uint32_t insideBefore = ps.bits & kBitsInside;
                     alive = sample_bounce(P.scene, P.settings, depth, smp, sf, ps);
+                    extChanged = (ps.bits & kBitsInside) != insideBefore && (ps.bits & kBitsInside);
                 }
             }
-            store_state(P.B, p, ps);
+            store_state(P.B, p, ps, depth, alive, extChanged);
         }
         wave_push(shadow, p, P.B.qs, &P.B.counts[P.B.shadowCountBase + depth]);
         wave_push(alive && more, p, out, &P.B.counts[depth + 1u]);
@@ -265,10 +289,9 @@ __global__ __launch_bounds__(64) void k_wf_accum(WfParams P) {
 hipError_t launch_wavefront_pass(const WfParams& P, const WfLaunch& L, hipStream_t s) {
     const uint32_t total = P.sliceCount * P.tiles * 64u;
     if (total == 0) return hipSuccess;
-    const uint32_t genBlocks = (total + 63u) / 64u;
-    const uint32_t blocks = genBlocks < L.maxBlocks ? genBlocks : L.maxBlocks;
+    const uint32_t fullBlocks = (total + 63u) / 64u;
+    const uint32_t blocks = fullBlocks < L.maxBlocks ? fullBlocks : L.maxBlocks;
     hipLaunchKernelGGL(k_wf_reset, dim3(1), dim3(256), 0, s, P.B.counts, P.B.countSlots);
-    hipLaunchKernelGGL(k_wf_generate, dim3(blocks), dim3(64), 0, s, P);
     for (uint32_t d = 0; d < P.settings.maxDepth; ++d) {
         hipLaunchKernelGGL(k_wf_extend, dim3(blocks), dim3(64), 0, s, P, d);
         hipLaunchKernelGGL(k_wf_shade, dim3(blocks), dim3(64), 0, s, P, d);
