@@ -77,6 +77,8 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-count", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default wavefront tuning")
+    ap.add_argument("--fetch-mode", type=int, default=None, help="force the traversal fetch mode (0, 1, 2)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device 3 times (calibrates HBM byte counters)")
     args = ap.parse_args()
@@ -112,6 +114,19 @@ def main():
     print(f"# BVH4 nodes {info.bvhNodeCount} leaves {info.bvhLeafCount} depth {info.bvhDepth} "
           f"build {info.buildSeconds:.2f}s upload {info.uploadSeconds:.2f}s", file=sys.stderr, flush=True)
     npx = st.width * st.height
+    tuning = None
+    if args.transport == "wavefront":
+        if args.fetch_mode is not None:
+            ctx.set_tuning(fetchMode=args.fetch_mode)
+        elif not args.no_autotune:
+            trial = max(1, min(st.samplesPerPixel, (16 << 20) // npx))
+            t0 = time.perf_counter()
+            _, report = ctx.autotune(wl.camera, st, trial)
+            print(f"# autotune ({trial} spp trials, {time.perf_counter() - t0:.1f} s):\n" + report, file=sys.stderr,
+                  flush=True)
+        t = ctx.get_tuning()
+        tuning = {"pathsPerPass": t.pathsPerPass, "fetchMode": t.fetchMode, "traceBlocks": t.traceBlocks,
+                  "stageBlocks": t.stageBlocks, "autotuned": args.fetch_mode is None and not args.no_autotune}
     dev = torch.device("cuda:0")
     bufs = [torch.empty(npx * k, dtype=torch.float32, device=dev) for k in (4, 3, 3, 1)]
     ptrs = [b.data_ptr() for b in bufs]
@@ -174,7 +189,8 @@ def main():
         "gpu_ms_per_step": round(sum(gpu_ms) / len(gpu_ms), 2),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic (seeded procedural scene and dome map)",
-        "config": {"workload": workload_desc(wl), "transport": args.transport, "samples_per_step": samples,
+        "config": {"workload": workload_desc(wl), "transport": args.transport, "tuning": tuning,
+                   "samples_per_step": samples,
                    "bvh": f"BVH4 {info.bvhNodeCount} nodes, depth {info.bvhDepth}",
                    "scene_upload_s": round(upload_s, 2), "finite_output": ok},
         "roofline": roofline, "cpu_baseline": cpu,

@@ -382,6 +382,27 @@ int shiro_hip_count_fetches(
     const shiro_hip_settings* settings, uint32_t sampleStart,
     uint32_t sampleCount, shiro_hip_fetch_counts* out);
 
+/* ---- Wavefront tuning (no counterpart in the reference: knobs of the
+ * wavefront transport, set by the caller or chosen by shiro_hip_autotune). ---- */
+typedef struct shiro_hip_tuning {
+    uint32_t pathsPerPass;  /* paths in flight per pass (default 4M) */
+    uint32_t fetchMode;     /* traversal work distribution: 0 wave rounds, 1 lane refill,
+                               2 lane refill from per-wave pools of the queue (default 0) */
+    uint32_t traceBlocks;   /* grid of the fetchMode 2 traversal kernels (default 32 x CUs) */
+    uint32_t stageBlocks;   /* grid cap of the grid-stride stage kernels (default 16384) */
+    uint32_t reserved[4];
+} shiro_hip_tuning;
+int shiro_hip_set_tuning(shiro_hip_ctx* ctx, const shiro_hip_tuning* tuning);
+int shiro_hip_get_tuning(const shiro_hip_ctx* ctx, shiro_hip_tuning* tuning);
+
+/* Times candidate tunings on `trialSamples` samples of the given frame
+ * (wavefront transport, uploaded scene), keeps the fastest in the context
+ * and returns it; `report` (may be NULL) receives one line per candidate. */
+int shiro_hip_autotune(
+    shiro_hip_ctx* ctx, const shiro_hip_camera* camera,
+    const shiro_hip_settings* settings, uint32_t trialSamples,
+    shiro_hip_tuning* best, char* report, size_t reportSize);
+
 #ifdef __cplusplus
 }
 #endif

@@ -128,6 +128,13 @@ class FetchCounts(C.Structure):
     ]
 
 
+class Tuning(C.Structure):
+    _fields_ = [
+        ("pathsPerPass", C.c_uint32), ("fetchMode", C.c_uint32), ("traceBlocks", C.c_uint32),
+        ("stageBlocks", C.c_uint32), ("reserved", C.c_uint32 * 4),
+    ]
+
+
 RESULT_NAMES = {
     0: "OK", 1: "NO_DEVICE", 2: "OOM", 3: "CANCELLED", 4: "INVALID",
     5: "NO_SCENE", 6: "LAUNCH",
@@ -147,6 +154,7 @@ EXPORTS = [
     "shiro_hip_scene_info_get", "shiro_hip_qmc_fill", "shiro_hip_intersect",
     "shiro_hip_occluded", "shiro_hip_trace_primary", "shiro_hip_eval_bsdf",
     "shiro_hip_env_sample", "shiro_hip_env_eval", "shiro_hip_count_fetches",
+    "shiro_hip_set_tuning", "shiro_hip_get_tuning", "shiro_hip_autotune",
 ]
 
 

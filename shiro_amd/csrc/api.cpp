@@ -84,10 +84,12 @@ struct shiro_hip_ctx {
     DevBuf frB, frA, frN, frD;
     DevBuf counters;
     // Wavefront state (WfBuffers arrays) and queues.
-    DevBuf wf[14];
+    DevBuf wf[16];
     // Accumulator.
     DevBuf acB, acA, acN, acD, acC;
     uint32_t acW = 0, acH = 0, acTotal = 0;
+    // Wavefront tuning (shiro_hip_set_tuning / shiro_hip_autotune).
+    shiro_hip_tuning tuning{4u << 20, 0u, 8192u, 16384u, {0, 0, 0, 0}};
 };
 
 namespace {
@@ -165,7 +167,6 @@ int check_settings(shiro_hip_ctx* ctx, const shiro_hip_settings* s) {
 // Wavefront form of one batch: passes of S samples for every pixel, S
 // chosen so that a pass holds about kWavefrontPaths paths (and at most
 // kWavefrontStateBytes of path state).
-constexpr uint64_t kWavefrontPaths = 4ull << 20;
 constexpr uint64_t kWavefrontStateBytes = 6ull << 30;
 
 int enqueue_wavefront(shiro_hip_ctx* ctx, const shiro_hip_camera& cam, const shiro_hip_settings& st, uint32_t start,
@@ -176,7 +177,7 @@ int enqueue_wavefront(shiro_hip_ctx* ctx, const shiro_hip_camera& cam, const shi
     const uint32_t K = st.domeLightSamples > 1u ? st.domeLightSamples : 1u;
     const uint32_t M = std::max(1u, ctx->dev.dirCount + ctx->dev.domeCount * K);
     const uint64_t bytesPerPath = 10u * 16u + 12u + static_cast<uint64_t>(M) * 32u;
-    uint64_t S = std::max<uint64_t>(1, kWavefrontPaths / perSample);
+    uint64_t S = std::max<uint64_t>(1, ctx->tuning.pathsPerPass / perSample);
     S = std::min<uint64_t>(S, std::max<uint64_t>(1, kWavefrontStateBytes / (bytesPerPath * perSample)));
     S = std::min<uint64_t>(S, count);
     const uint64_t cap = S * perSample;
@@ -189,7 +190,9 @@ int enqueue_wavefront(shiro_hip_ctx* ctx, const shiro_hip_camera& cam, const shi
     HIPC(ctx->wf[11].reserve(cap * 4), "alloc wavefront queues");
     HIPC(ctx->wf[12].reserve(cap * 4), "alloc wavefront queues");
     HIPC(ctx->wf[13].reserve(cap * 4), "alloc wavefront queues");
-    const uint32_t slots = 2u * st.maxDepth + 1u;
+    HIPC(ctx->wf[14].reserve(cap * M * 4), "alloc wavefront queues");
+    HIPC(ctx->wf[15].reserve(cap * M * 4), "alloc wavefront queues");
+    const uint32_t slots = 5u * st.maxDepth + 1u;
     HIPC(ctx->counters.reserve(std::max<size_t>(sizeof(DevCounters), slots * 4u)), "alloc counters");
     if (S < count) {
         HIPC(ctx->sumB.reserve(np * 16), "alloc sums");
@@ -219,8 +222,12 @@ int enqueue_wavefront(shiro_hip_ctx* ctx, const shiro_hip_camera& cam, const shi
     B.qb = ctx->wf[12].as<uint32_t>();
     B.qs = ctx->wf[13].as<uint32_t>();
     B.counts = ctx->counters.as<uint32_t>();
+    B.qsr = ctx->wf[14].as<uint32_t>();
+    B.svis = ctx->wf[15].as<uint32_t>();
     B.countSlots = slots;
-    B.shadowCountBase = st.maxDepth + 1u;
+    B.shadowRayBase = st.maxDepth + 1u;
+    B.shadowPathBase = B.shadowRayBase + st.maxDepth;
+    B.fetchBase = B.shadowPathBase + st.maxDepth;
     B.maxShadow = M;
     B.cap = static_cast<uint32_t>(cap);
     P.sampleStart = start;
@@ -228,7 +235,9 @@ int enqueue_wavefront(shiro_hip_ctx* ctx, const shiro_hip_camera& cam, const shi
     P.tilesX = tilesX;
     P.tiles = tilesX * tilesY;
     WfLaunch L{};
-    L.maxBlocks = 16384;
+    L.maxBlocks = ctx->tuning.stageBlocks;
+    L.traceBlocks = ctx->tuning.traceBlocks;
+    L.fetchMode = ctx->tuning.fetchMode;
     HIPC(hipEventRecord(ctx->ev0, ctx->stream), "event");
     for (uint32_t b = 0; b < count; b += static_cast<uint32_t>(S)) {
         if (cancel && *cancel) {
@@ -359,6 +368,9 @@ int shiro_hip_create(int device, shiro_hip_ctx** out) {
         delete ctx;
         return SHIRO_HIP_NO_DEVICE;
     }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
+        ctx->tuning.traceBlocks = static_cast<uint32_t>(prop.multiProcessorCount) * 32u;
     *out = ctx;
     return SHIRO_HIP_OK;
 }
@@ -781,6 +793,88 @@ int shiro_hip_env_eval(shiro_hip_ctx* ctx, uint32_t domeIndex, const float* dire
     HIPC(p.out(pdf, n, ctx->stream), "readback");
     HIPC(r.out(radiance, 3ull * n, ctx->stream), "readback");
     HIPC(hipStreamSynchronize(ctx->stream), "env");
+    return SHIRO_HIP_OK;
+}
+
+int shiro_hip_set_tuning(shiro_hip_ctx* ctx, const shiro_hip_tuning* t) {
+    if (!ctx || !t) return SHIRO_HIP_INVALID;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    if (t->pathsPerPass == 0 || t->fetchMode > 2u || t->traceBlocks == 0 || t->stageBlocks == 0)
+        return fail(ctx, SHIRO_HIP_INVALID, "invalid tuning");
+    ctx->tuning = *t;
+    return SHIRO_HIP_OK;
+}
+
+int shiro_hip_get_tuning(const shiro_hip_ctx* ctx, shiro_hip_tuning* t) {
+    if (!ctx || !t) return SHIRO_HIP_INVALID;
+    *t = ctx->tuning;
+    return SHIRO_HIP_OK;
+}
+
+int shiro_hip_autotune(shiro_hip_ctx* ctx, const shiro_hip_camera* camera, const shiro_hip_settings* settings,
+                       uint32_t trialSamples, shiro_hip_tuning* best, char* report, size_t reportSize) {
+    if (!ctx || !camera) return SHIRO_HIP_INVALID;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    int rc = check_settings(ctx, settings);
+    if (rc) return rc;
+    if (!ctx->hasScene) return fail(ctx, SHIRO_HIP_NO_SCENE, "no scene uploaded");
+    HIPC(hipSetDevice(ctx->device), "set device");
+    shiro_hip_settings st = *settings;
+    st.transport = SHIRO_HIP_TRANSPORT_WAVEFRONT;
+    const uint32_t spp = std::max(1u, trialSamples);
+    const uint64_t np = static_cast<uint64_t>(st.width) * st.height;
+    HIPC(ctx->frB.reserve(np * 16), "alloc frame");
+    HIPC(ctx->frA.reserve(np * 12), "alloc frame");
+    HIPC(ctx->frN.reserve(np * 12), "alloc frame");
+    HIPC(ctx->frD.reserve(np * 4), "alloc frame");
+    const DevFrame f{ctx->frB.as<float>(), ctx->frA.as<float>(), ctx->frN.as<float>(), ctx->frD.as<float>()};
+    const shiro_hip_tuning base = ctx->tuning;
+    const uint32_t cus = std::max(1u, base.traceBlocks / 32u);
+    std::vector<shiro_hip_tuning> cands;
+    for (uint32_t paths : {1u << 20, 2u << 20, 4u << 20, 8u << 20}) {
+        for (uint32_t mode : {0u, 1u, 2u}) {
+            for (uint32_t tb : {16u, 32u, 64u}) {
+                if (mode != 2u && tb != 32u) continue;
+                shiro_hip_tuning t = base;
+                t.pathsPerPass = paths;
+                t.fetchMode = mode;
+                t.traceBlocks = cus * tb;
+                cands.push_back(t);
+            }
+        }
+    }
+    std::string log;
+    float bestMs = 0.0f;
+    shiro_hip_tuning bestT = base;
+    for (size_t i = 0; i < cands.size(); ++i) {
+        ctx->tuning = cands[i];
+        // One untimed warm run, then the timed run.
+        float ms = 0.0f;
+        for (int rep = 0; rep < 2; ++rep) {
+            rc = enqueue_batch(ctx, *camera, st, 0, spp, f, false, nullptr);
+            if (rc) {
+                ctx->tuning = base;
+                return rc;
+            }
+            rc = finish_timing(ctx);
+            if (rc) {
+                ctx->tuning = base;
+                return rc;
+            }
+            ms = ctx->lastMs;
+        }
+        char line[160];
+        std::snprintf(line, sizeof(line), "pathsPerPass %u fetchMode %u traceBlocks %u: %.3f ms\n",
+                      cands[i].pathsPerPass, cands[i].fetchMode, cands[i].traceBlocks, ms);
+        log += line;
+        if (i == 0 || ms < bestMs) {
+            bestMs = ms;
+            bestT = cands[i];
+        }
+    }
+    ctx->tuning = bestT;
+    if (best) *best = bestT;
+    if (report && reportSize > 0) std::snprintf(report, reportSize, "%s", log.c_str());
     return SHIRO_HIP_OK;
 }
 

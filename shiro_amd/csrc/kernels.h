@@ -23,10 +23,15 @@ struct WfBuffers {
                                             // extinction, albedo|first-hit t, normal, hit record
     float4 *so, *st;                        // shadow header: origin|count, throughput at the hit
     float4* sr;                             // shadow rays: maxShadow x {dir|group, contribution}
-    uint32_t *qa, *qb, *qs;                 // ray queues (ping-pong) and shadow queue
-    uint32_t* counts;                       // queue lengths: rays per depth, then shadows per depth
+    uint32_t *qa, *qb;                      // ray queues (ping-pong)
+    uint32_t* qs;                           // paths with shadow rays
+    uint32_t* qsr;                          // shadow rays (path * maxShadow + index)
+    uint32_t* svis;                         // per shadow ray: 1 if unoccluded
+    uint32_t* counts;                       // queue lengths and work counters, see k*Base
     uint32_t countSlots;
-    uint32_t shadowCountBase;
+    uint32_t shadowRayBase;                 // counts[shadowRayBase + d]: shadow rays at depth d
+    uint32_t shadowPathBase;                // counts[shadowPathBase + d]: paths with shadow rays
+    uint32_t fetchBase;                     // counts[fetchBase + d], [+ maxDepth + d]: work fetched
     uint32_t maxShadow;
     uint32_t cap;
 };
@@ -43,7 +48,9 @@ struct WfParams {
 };
 
 struct WfLaunch {
-    uint32_t maxBlocks;  // grid of the grid-stride stage kernels
+    uint32_t maxBlocks;    // grid of the grid-stride stage kernels
+    uint32_t traceBlocks;  // grid of the persistent traversal kernels
+    uint32_t fetchMode;    // traversal work distribution, see k_wf_trace
 };
 
 hipError_t launch_render(const RenderParams& P, bool count, hipStream_t stream);

@@ -4,13 +4,19 @@
 //
 //   k_wf_reset     clear the queue counters of the pass
 //   per depth d:
-//     k_wf_extend  closest hit of every queued ray (BVH4 traversal only); at
-//                  depth 0 every (pixel, sample) path of the pass, its camera
-//                  ray computed in place (no queue, no stored ray)
+//     k_wf_trace<closest>  closest hit of every queued ray; at depth 0 every
+//                  (pixel, sample) path of the pass, its camera ray computed
+//                  in place (no queue, no stored ray)
 //     k_wf_shade   surface terms, light samples -> shadow rays, BSDF
 //                  sampling and roulette -> next ray queue
-//     k_wf_shadow  any-hit tests of the shadow rays; visible contributions
-//                  are summed per path in DirectLighting's order
+//     k_wf_trace<any>  any-hit test of every queued shadow ray
+//     k_wf_resolve visible contributions summed per path in DirectLighting's
+//                  order and added to the path radiance
+//
+// The traversal kernels are persistent: a fixed grid whose lanes fetch rays
+// from the queue with one wave-aggregated atomic, and a lane whose ray is
+// done fetches the next one at once instead of idling until the longest ray
+// of its wave ends (dynamic ray fetch, Aila and Laine 2009).
 //   k_wf_accum     per-pixel sums over the pass's samples, in sample order
 //
 // Every stage uses the same device functions as the megakernel
@@ -111,29 +117,206 @@ DEV uint32_t stage_items(const WfParams& P, uint32_t depth) {
     return depth == 0u ? P.sliceCount * P.tiles * 64u : P.B.counts[depth];
 }
 
-__global__ __launch_bounds__(64) void k_wf_extend(WfParams P, uint32_t depth) {
-    const uint32_t* in = (depth & 1u) ? P.B.qb : P.B.qa;
-    const uint32_t n = stage_items(P, depth);
-    for (uint32_t i = blockIdx.x * 64u + threadIdx.x; i < n; i += gridDim.x * 64u) {
-        Ray r;
-        uint32_t p;
-        if (depth == 0u) {
-            p = i;
-            const PathCoord c = path_coord(P, p);
-            if (!path_valid(P, c)) continue;
-            const Sampler smp = path_sampler(P, c);
-            r = camera_ray(P.camera, P.settings, c.x, c.y, smp.next2(0u));
-        } else {
-            p = in[i];
-            const float4 o = P.B.o[p], d = P.B.d[p];
-            r.o = xyz(o);
-            r.d = xyz(d);
-            r.tmin = kRayEps;
-            r.tmax = kFltMax;
+// Work distribution of the persistent kernels: each wave owns a pool of
+// queue positions [next, end) taken 64 at a time from the global counter, so
+// lanes refill from the pool without a global atomic most of the time.
+struct WavePool {
+    uint32_t next = 0, end = 0;
+    // Hands the wanting lanes consecutive positions; positions >= n mean the
+    // queue is exhausted.
+    DEV uint32_t fetch(bool want, uint32_t* counter) {
+        const unsigned long long mask = __ballot(want);
+        if (mask == 0ull) return 0xffffffffu;
+        const uint32_t lane = lane_id();
+        const uint32_t nWant = static_cast<uint32_t>(__popcll(mask));
+        const uint32_t avail = end - next;
+        uint32_t chunk = 0;
+        if (nWant > avail) {
+            const int leader = __ffsll(static_cast<long long>(mask)) - 1;
+            if (static_cast<int>(lane) == leader) chunk = atomicAdd(counter, 64u);
+            chunk = __shfl(chunk, leader);
         }
-        Counters<false> c;
-        const HitRec h = intersect_closest(P.scene, r, c.trav());
-        P.B.H[p] = make_float4(h.t, h.b1, h.b2, __uint_as_float(h.slot));
+        const uint32_t rank = static_cast<uint32_t>(__popcll(mask & ((1ull << lane) - 1ull)));
+        const uint32_t idx = rank < avail ? next + rank : chunk + (rank - avail);
+        if (nWant > avail) {
+            next = chunk + (nWant - avail);
+            end = chunk + 64u;
+        } else {
+            next += nWant;
+        }
+        return idx;
+    }
+};
+
+// Persistent traversal.  ANY = false: closest hit of ray-queue entries (all
+// paths at depth 0), result in H.  ANY = true: any hit of shadow-queue
+// entries, result in svis.  The traversal itself is the one of dev_bvh.h
+// (same node test, child order, tie rule and triangle test) reorganised so
+// that each lane can restart with a new ray.
+// FETCH: 0 = wave-synchronous rounds (a wave takes new rays only when all
+// its lanes are done), 1 = lanes refill as their rays finish (static item
+// order), 2 = lanes refill from per-wave pools of the global queue.
+template <bool ANY, int FETCH>
+__global__ __launch_bounds__(64) void k_wf_trace(WfParams P, uint32_t depth) {
+    const uint32_t md = P.settings.maxDepth;
+    const uint32_t n = ANY ? P.B.counts[P.B.shadowRayBase + depth] : stage_items(P, depth);
+    uint32_t* counter = &P.B.counts[P.B.fetchBase + (ANY ? md : 0u) + depth];
+    const uint32_t* in = (depth & 1u) ? P.B.qb : P.B.qa;
+    const DevScene& S = P.scene;
+    if (S.nodeCount == 0) {
+        // Empty scene: every ray misses.
+        for (uint32_t i = blockIdx.x * 64u + threadIdx.x; i < n; i += gridDim.x * 64u) {
+            if (ANY) {
+                P.B.svis[P.B.qsr[i]] = 1u;
+            } else {
+                const uint32_t p = depth == 0u ? i : in[i];
+                P.B.H[p] = make_float4(kFltMax, 0.0f, 0.0f, __uint_as_float(kNoHit));
+            }
+        }
+        return;
+    }
+    uint32_t stackNode[kStackSize];
+    float stackT[kStackSize];
+    int sp = 0;
+    uint32_t cur = kEmpty;
+    uint32_t item = 0;
+    bool active = false, exhausted = false;
+    RayPre r;
+    float tmax = kFltMax;
+    HitRec best;
+    best.t = kFltMax;
+    best.b1 = best.b2 = 0.0f;
+    best.slot = best.prim = kNoHit;
+    TravCount<false> cnt{nullptr, nullptr};
+    WavePool pool;
+    // Static assignment: lane i of the grid takes items i, i + G, i + 2G...
+    uint32_t nextStatic = blockIdx.x * 64u + threadIdx.x;
+    for (;;) {
+        const bool want = !active && !exhausted && (FETCH != 0 || !__any(active));
+        uint32_t idx;
+        if (FETCH == 2) {
+            idx = pool.fetch(want, counter);
+        } else {
+            idx = nextStatic;
+            if (want) nextStatic += gridDim.x * 64u;
+        }
+        if (want) {
+            if (idx >= n) {
+                exhausted = true;
+            } else {
+                item = idx;
+                Ray ray;
+                bool valid = true;
+                if (ANY) {
+                    const uint32_t rid = P.B.qsr[idx];
+                    const uint32_t p = rid / P.B.maxShadow;
+                    ray.o = xyz(P.B.so[p]);
+                    ray.d = xyz(P.B.sr[2u * static_cast<size_t>(rid)]);
+                    item = rid;
+                } else if (depth == 0u) {
+                    const PathCoord c = path_coord(P, idx);
+                    valid = path_valid(P, c);
+                    if (valid) ray = camera_ray(P.camera, P.settings, c.x, c.y, path_sampler(P, c).next2(0u));
+                } else {
+                    item = in[idx];
+                    ray.o = xyz(P.B.o[item]);
+                    ray.d = xyz(P.B.d[item]);
+                }
+                if (valid) {
+                    ray.tmin = kRayEps;
+                    ray.tmax = kFltMax;
+                    r = precompute(ray);
+                    tmax = kFltMax;
+                    best.t = kFltMax;
+                    best.b1 = best.b2 = 0.0f;
+                    best.slot = best.prim = kNoHit;
+                    sp = 0;
+                    cur = 0u;
+                    active = true;
+                }
+            }
+        }
+        // A lane is exhausted once the queue is; an inactive lane that is not
+        // exhausted (it fetched an off-image path) fetches again.
+        if (__all(exhausted)) break;
+        if (!active) continue;
+        bool finished = false, occluded = false;
+        // Node phase.
+        while (cur != kEmpty && !(cur & kLeafBit)) {
+            const float4* nd = S.nodes + static_cast<size_t>(cur) * 8u;
+            float tn[4];
+            const uint32_t mask = node_boxes(nd, r, ANY ? tmax : best.t, tn);
+            if (mask) {
+                const float4 cw = nd[6];
+                float k[4] = {(mask & 1u) ? tn[0] : kInf, (mask & 2u) ? tn[1] : kInf, (mask & 4u) ? tn[2] : kInf,
+                              (mask & 8u) ? tn[3] : kInf};
+                uint32_t v[4] = {__float_as_uint(cw.x), __float_as_uint(cw.y), __float_as_uint(cw.z),
+                                 __float_as_uint(cw.w)};
+                cswap(k[0], v[0], k[1], v[1]);
+                cswap(k[2], v[2], k[3], v[3]);
+                cswap(k[0], v[0], k[2], v[2]);
+                cswap(k[1], v[1], k[3], v[3]);
+                cswap(k[1], v[1], k[2], v[2]);
+                const int nh = __popc(mask);
+                for (int i = nh - 1; i >= 1; --i) {
+                    stackNode[sp] = v[i];
+                    stackT[sp] = k[i];
+                    ++sp;
+                }
+                cur = v[0];
+            } else {
+                cur = kEmpty;
+                while (sp > 0) {
+                    --sp;
+                    if (ANY || stackT[sp] <= best.t) {
+                        cur = stackNode[sp];
+                        break;
+                    }
+                }
+            }
+        }
+        if (cur == kEmpty) {
+            finished = true;
+        } else {
+            // Leaf phase.
+            if (ANY) {
+                uint32_t slot = cur & ~kLeafBit;
+                for (;;) {
+                    const float4* tp = S.tris + static_cast<size_t>(slot) * 3u;
+                    const float4 a = tp[0], b = tp[1], c = tp[2];
+                    float t, b1, b2;
+                    if (tri_test(r, xyz(a), xyz(b), xyz(c), tmax, &t, &b1, &b2)) {
+                        occluded = true;
+                        break;
+                    }
+                    if (__float_as_uint(b.w) & kLeafEndFlag) break;
+                    ++slot;
+                }
+            } else {
+                leaf_closest(S, r, cur & ~kLeafBit, best, cnt);
+            }
+            cur = kEmpty;
+            if (occluded) {
+                finished = true;
+            } else {
+                while (sp > 0) {
+                    --sp;
+                    if (ANY || stackT[sp] <= best.t) {
+                        cur = stackNode[sp];
+                        break;
+                    }
+                }
+                if (cur == kEmpty) finished = true;
+            }
+        }
+        if (finished) {
+            if (ANY) {
+                P.B.svis[item] = occluded ? 0u : 1u;
+            } else {
+                P.B.H[item] = make_float4(best.t, best.b1, best.b2, __uint_as_float(best.slot));
+            }
+            active = false;
+        }
     }
 }
 
@@ -181,6 +364,7 @@ __global__ __launch_bounds__(64) void k_wf_shade(WfParams P, uint32_t depth) {
                         if (!direct_sample(P.scene, P.settings, sf, M, smp, depth, j, &ld, &contrib, &group, c)) continue;
                         sr[2u * ns] = make_float4(ld.x, ld.y, ld.z, __uint_as_float(group));
                         sr[2u * ns + 1u] = make_float4(contrib.x, contrib.y, contrib.z, 0.0f);
+                        wave_push(true, p * P.B.maxShadow + ns, P.B.qsr, &P.B.counts[P.B.shadowRayBase + depth]);
                         ++ns;
                     }
                     if (ns > 0u) {
@@ -196,25 +380,25 @@ __global__ __launch_bounds__(64) void k_wf_shade(WfParams P, uint32_t depth) {
             }
             store_state(P.B, p, ps, depth, alive, extChanged);
         }
-        wave_push(shadow, p, P.B.qs, &P.B.counts[P.B.shadowCountBase + depth]);
+        wave_push(shadow, p, P.B.qs, &P.B.counts[P.B.shadowPathBase + depth]);
         wave_push(alive && more, p, out, &P.B.counts[depth + 1u]);
     }
 }
 
-__global__ __launch_bounds__(64) void k_wf_shadow(WfParams P, uint32_t depth) {
-    const uint32_t n = P.B.counts[P.B.shadowCountBase + depth];
+// Adds the visible light samples of each shaded path, in DirectLighting's
+// order, times the throughput at the hit (CpuPathTracer.cpp:1750-1762).
+__global__ __launch_bounds__(64) void k_wf_resolve(WfParams P, uint32_t depth) {
+    const uint32_t n = P.B.counts[P.B.shadowPathBase + depth];
     for (uint32_t i = blockIdx.x * 64u + threadIdx.x; i < n; i += gridDim.x * 64u) {
         const uint32_t p = P.B.qs[i];
-        const float4 so = P.B.so[p];
-        const V3 origin = xyz(so);
-        const uint32_t ns = __float_as_uint(so.w);
+        const uint32_t ns = __float_as_uint(P.B.so[p].w);
         const float4* sr = P.B.sr + static_cast<size_t>(p) * P.B.maxShadow * 2u;
+        const uint32_t* vis = P.B.svis + static_cast<size_t>(p) * P.B.maxShadow;
         DirectSum ds;
         ds.init(P.settings);
-        Counters<false> c;
         for (uint32_t j = 0; j < ns; ++j) {
             const float4 a = sr[2u * j], b = sr[2u * j + 1u];
-            ds.add(__float_as_uint(a.w), xyz(b), !occluded(P.scene, origin, xyz(a), kFltMax, c));
+            ds.add(__float_as_uint(a.w), xyz(b), vis[j] != 0u);
         }
         ds.flush();
         const float4 L = P.B.L[p];
@@ -292,10 +476,20 @@ hipError_t launch_wavefront_pass(const WfParams& P, const WfLaunch& L, hipStream
     const uint32_t fullBlocks = (total + 63u) / 64u;
     const uint32_t blocks = fullBlocks < L.maxBlocks ? fullBlocks : L.maxBlocks;
     hipLaunchKernelGGL(k_wf_reset, dim3(1), dim3(256), 0, s, P.B.counts, P.B.countSlots);
+    const uint32_t tblocks = fullBlocks < L.traceBlocks ? fullBlocks : L.traceBlocks;
     for (uint32_t d = 0; d < P.settings.maxDepth; ++d) {
-        hipLaunchKernelGGL(k_wf_extend, dim3(blocks), dim3(64), 0, s, P, d);
+        switch (L.fetchMode) {
+            case 2: hipLaunchKernelGGL((k_wf_trace<false, 2>), dim3(tblocks), dim3(64), 0, s, P, d); break;
+            case 1: hipLaunchKernelGGL((k_wf_trace<false, 1>), dim3(blocks), dim3(64), 0, s, P, d); break;
+            default: hipLaunchKernelGGL((k_wf_trace<false, 0>), dim3(blocks), dim3(64), 0, s, P, d); break;
+        }
         hipLaunchKernelGGL(k_wf_shade, dim3(blocks), dim3(64), 0, s, P, d);
-        hipLaunchKernelGGL(k_wf_shadow, dim3(blocks), dim3(64), 0, s, P, d);
+        switch (L.fetchMode) {
+            case 2: hipLaunchKernelGGL((k_wf_trace<true, 2>), dim3(tblocks), dim3(64), 0, s, P, d); break;
+            case 1: hipLaunchKernelGGL((k_wf_trace<true, 1>), dim3(blocks), dim3(64), 0, s, P, d); break;
+            default: hipLaunchKernelGGL((k_wf_trace<true, 0>), dim3(blocks), dim3(64), 0, s, P, d); break;
+        }
+        hipLaunchKernelGGL(k_wf_resolve, dim3(blocks), dim3(64), 0, s, P, d);
     }
     hipLaunchKernelGGL(k_wf_accum, dim3(P.tiles), dim3(64), 0, s, P);
     return hipGetLastError();

@@ -84,6 +84,10 @@ def load_library(path: str = LIB_PATH):
     d("shiro_hip_env_sample", C.c_int, V, _u, _fp, _u, _fp, _fp, _fp)
     d("shiro_hip_env_eval", C.c_int, V, _u, _fp, _u, _fp, _fp)
     d("shiro_hip_count_fetches", C.c_int, V, _P(abi.Camera), _P(abi.Settings), _u, _u, _P(abi.FetchCounts))
+    d("shiro_hip_set_tuning", C.c_int, V, _P(abi.Tuning))
+    d("shiro_hip_get_tuning", C.c_int, V, _P(abi.Tuning))
+    d("shiro_hip_autotune", C.c_int, V, _P(abi.Camera), _P(abi.Settings), _u, _P(abi.Tuning), C.c_char_p,
+      C.c_size_t)
     _lib = L
     return L
 
@@ -279,6 +283,27 @@ class Context:
         c = np.zeros((n, 3), np.float32)
         self._check(self.L.shiro_hip_env_eval(self.h, dome, abi.fp(dirs), n, abi.fp(p), abi.fp(c)))
         return p, c
+
+    def get_tuning(self) -> abi.Tuning:
+        t = abi.Tuning()
+        self._check(self.L.shiro_hip_get_tuning(self.h, C.byref(t)))
+        return t
+
+    def set_tuning(self, **kw):
+        t = self.get_tuning()
+        for k, v in kw.items():
+            setattr(t, k, int(v))
+        self._check(self.L.shiro_hip_set_tuning(self.h, C.byref(t)))
+
+    def autotune(self, camera, settings, trial_samples=2):
+        """Chooses paths per pass, traversal fetch mode and grid for this
+        frame; returns (tuning, report)."""
+        t = abi.Tuning()
+        buf = C.create_string_buffer(8192)
+        cam, st = camera.to_c(), settings.to_c()
+        self._check(self.L.shiro_hip_autotune(self.h, C.byref(cam), C.byref(st), trial_samples, C.byref(t), buf,
+                                              len(buf)))
+        return t, buf.value.decode()
 
     def count_fetches(self, camera, settings, sample_start=0, sample_count=None) -> abi.FetchCounts:
         sample_count = settings.samplesPerPixel if sample_count is None else sample_count

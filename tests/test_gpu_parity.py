@@ -283,3 +283,36 @@ def test_wavefront_multi_pass(hip_ctx):
     wf = hip_ctx.render(wl.camera, st)
     assert np.array_equal(wf.beauty, mk.beauty)
     assert np.array_equal(wf.depth, mk.depth)
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2])
+def test_wavefront_fetch_modes_identical(hip_ctx, mode):
+    """The traversal work distribution (tuning) does not change the frame."""
+    from shiro_amd import abi
+    wl = c3_mini(width=64, spp=4)
+    hip_ctx.upload(wl.scene)
+    mk = hip_ctx.render(wl.camera, wl.settings)
+    saved = hip_ctx.get_tuning()
+    try:
+        hip_ctx.set_tuning(fetchMode=mode, pathsPerPass=1 << 14)
+        st = RenderSettings(**{**wl.settings.__dict__, "transport": abi.TRANSPORT_WAVEFRONT})
+        wf = hip_ctx.render(wl.camera, st)
+    finally:
+        hip_ctx.set_tuning(**{k: getattr(saved, k) for k in ("pathsPerPass", "fetchMode", "traceBlocks",
+                                                               "stageBlocks")})
+    assert np.array_equal(wf.beauty, mk.beauty)
+
+
+def test_autotune_keeps_result(hip_ctx):
+    from shiro_amd import abi
+    wl = c2_mini(width=128, spp=4)
+    hip_ctx.upload(wl.scene)
+    st = RenderSettings(**{**wl.settings.__dict__, "transport": abi.TRANSPORT_WAVEFRONT})
+    ref = hip_ctx.render(wl.camera, st)
+    saved = hip_ctx.get_tuning()
+    t, report = hip_ctx.autotune(wl.camera, st, 2)
+    _log(report)
+    assert report.count("ms") >= 10 and t.fetchMode in (0, 1, 2)
+    out = hip_ctx.render(wl.camera, st)
+    hip_ctx.set_tuning(**{k: getattr(saved, k) for k in ("pathsPerPass", "fetchMode", "traceBlocks", "stageBlocks")})
+    assert np.array_equal(out.beauty, ref.beauty)
