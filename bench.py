@@ -14,7 +14,6 @@ baseline (the CPU oracle on this host's cores, bounded sample).
 """
 import argparse
 import json
-import math
 import os
 import sys
 import time
@@ -151,35 +150,58 @@ def main():
     frame = bufs[0].view(-1, 4)
     ok = bool(torch.isfinite(frame).all().item())
 
-    # Roofline of the render kernel: algorithmic bytes per sample from the
-    # fetch counters (BVH4 nodes 128 B, triangles 48 B, hit data 96 B,
-    # environment texels 16 B, CDF entries 4 B) plus the AOV write (44 B per
-    # pixel per batch), over the kernel's event-timed duration.
+    # Roofline of the dominant render kernel.  Algorithmic bytes come from
+    # the fetch counters (same BVH, same traversal order): BVH4 nodes 128 B,
+    # triangles 48 B, hit data 96 B, environment texels 16 B, CDF entries
+    # 4 B, AOV write 44 B per pixel per batch.  The wavefront's closest-hit
+    # kernel is credited with the closest-hit rays' node and triangle bytes,
+    # its any-hit kernel with the shadow rays'; queue and path-state traffic
+    # earns nothing.  Kernel durations come from events recorded after every
+    # launch on the render stream, in one extra (untimed) render call.
     roofline = None
+    stages = None
     if not args.no_count:
         cspp = min(4, st.samplesPerPixel)
         c = ctx.count_fetches(wl.camera, st, 0, cspp)
-        per_sample = (c.nodeFetches * NODE_BYTES + c.triFetches * TRI_BYTES + c.hitFetches * HIT_BYTES +
-                      c.envTaps * TEXEL_BYTES + c.cdfSteps * CDF_BYTES) / max(c.samples, 1) + 44.0 / st.samplesPerPixel
-        slice_spp = max(1, min(st.samplesPerPixel, (8 << 20) // npx))
-        launches = math.ceil(st.samplesPerPixel / slice_spp)
-        avg_launch_ms = sum(gpu_ms) / len(gpu_ms) / launches
-        bytes_per_launch = per_sample * npx * slice_spp
+        ns = max(c.samples, 1)
+        closest = (c.nodeFetches * NODE_BYTES + c.triFetches * TRI_BYTES) / ns
+        shadow = (c.shadowNodeFetches * NODE_BYTES + c.shadowTriFetches * TRI_BYTES) / ns
+        other = (c.hitFetches * HIT_BYTES + c.envTaps * TEXEL_BYTES + c.cdfSteps * CDF_BYTES) / ns
+        per_sample = closest + shadow + other + 44.0 / st.samplesPerPixel
+        ctx.set_stage_timing(True)
+        ctx.render_device(wl.camera, st, ptrs, 0, st.samplesPerPixel, synchronize=True)
+        stages = ctx.stage_times()
+        ctx.set_stage_timing(False)
+        if args.transport == "wavefront":
+            dom = max(("extend", "shadow", "shade"), key=lambda k: stages.get(k, {"ms": 0.0})["ms"])
+            names = {"extend": f"k_wf_trace<false,{tuning['fetchMode']}> (closest hit)",
+                     "shadow": f"k_wf_trace<true,{tuning['fetchMode']}> (any hit)",
+                     "shade": "k_wf_shade"}
+            credit = {"extend": closest, "shadow": shadow, "shade": other}[dom]
+        else:
+            dom, credit = "megakernel", per_sample
+            names = {"megakernel": "k_render<false> (megakernel)"}
+        stage_ms, launches = stages[dom]["ms"], stages[dom]["launches"]
+        bytes_per_launch = credit * samples / launches
+        avg_launch_ms = stage_ms / launches
         achieved = bytes_per_launch / (avg_launch_ms * 1e-3) / 1e9
         traffic, traffic_src = None, None
         tpath = os.path.join(ROOT, "profiles", "traffic.json")
         if os.path.exists(tpath):
-            rec = json.load(open(tpath)).get(f"{args.workload}/megakernel")
+            rec = json.load(open(tpath)).get(f"{args.workload}/{args.transport}/{dom}")
             if rec:
                 traffic, traffic_src = rec["bytes_per_launch"], rec["source"]
         roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic, "traffic_source": traffic_src,
-                    "kernel": "k_render<false> (megakernel)", "launches_per_step": launches,
-                    "avg_launch_ms": round(avg_launch_ms, 3), "bytes_per_launch": int(bytes_per_launch),
-                    "bytes_per_sample": round(per_sample, 1),
-                    "per_sample": {"rays": c.rays / c.samples, "shadow_rays": c.shadowRays / c.samples,
-                                   "node_fetches": c.nodeFetches / c.samples, "tri_tests": c.triFetches / c.samples,
-                                   "env_taps": c.envTaps / c.samples, "cdf_steps": c.cdfSteps / c.samples}}
+                    "kernel": names[dom], "launches_per_step": launches,
+                    "avg_launch_ms": round(avg_launch_ms, 4), "bytes_per_launch": int(bytes_per_launch),
+                    "bytes_per_sample": {"kernel": round(credit, 1), "path": round(per_sample, 1)},
+                    "stage_ms": {k: round(v["ms"], 2) for k, v in stages.items() if k != "complete"},
+                    "per_sample": {"rays": c.rays / ns, "shadow_rays": c.shadowRays / ns,
+                                   "node_fetches": c.nodeFetches / ns, "tri_tests": c.triFetches / ns,
+                                   "shadow_node_fetches": c.shadowNodeFetches / ns,
+                                   "shadow_tri_tests": c.shadowTriFetches / ns,
+                                   "env_taps": c.envTaps / ns, "cdf_steps": c.cdfSteps / ns}}
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(wl, args.cpu_seconds, args.cpu_threads)

@@ -371,11 +371,13 @@ typedef struct shiro_hip_fetch_counts {
     uint64_t samples;
     uint64_t rays;          /* closest-hit rays */
     uint64_t shadowRays;    /* any-hit rays */
-    uint64_t nodeFetches;   /* BVH4 nodes read (128 B each) */
-    uint64_t triFetches;    /* triangles tested (48 B each) */
+    uint64_t nodeFetches;   /* BVH4 nodes read by closest-hit rays (128 B each) */
+    uint64_t triFetches;    /* triangles tested by closest-hit rays (48 B each) */
     uint64_t envTaps;       /* environment texel reads (16 B each) */
     uint64_t cdfSteps;      /* importance table reads (4 B each) */
     uint64_t hitFetches;    /* per-hit vertex/normal/material fetches */
+    uint64_t shadowNodeFetches;  /* BVH4 nodes read by any-hit rays */
+    uint64_t shadowTriFetches;   /* triangles tested by any-hit rays */
 } shiro_hip_fetch_counts;
 int shiro_hip_count_fetches(
     shiro_hip_ctx* ctx, const shiro_hip_camera* camera,
@@ -402,6 +404,30 @@ int shiro_hip_autotune(
     shiro_hip_ctx* ctx, const shiro_hip_camera* camera,
     const shiro_hip_settings* settings, uint32_t trialSamples,
     shiro_hip_tuning* best, char* report, size_t reportSize);
+
+/* ---- Per-kernel timing (no counterpart in the reference: measurement of
+ * the render kernels for roofline accounting). ---- */
+enum {
+    SHIRO_HIP_STAGE_RESET = 0,      /* wavefront queue counters cleared */
+    SHIRO_HIP_STAGE_EXTEND = 1,     /* closest-hit traversal of the path queue */
+    SHIRO_HIP_STAGE_SHADE = 2,      /* surface/miss terms, light samples, bounce */
+    SHIRO_HIP_STAGE_SHADOW = 3,     /* any-hit traversal of the shadow-ray queue */
+    SHIRO_HIP_STAGE_RESOLVE = 4,    /* ordered sum of unoccluded light samples */
+    SHIRO_HIP_STAGE_ACCUM = 5,      /* per-pixel sample sums of the pass */
+    SHIRO_HIP_STAGE_MEGAKERNEL = 6, /* one megakernel launch */
+    SHIRO_HIP_STAGE_COUNT = 8
+};
+typedef struct shiro_hip_stage_times {
+    float ms[SHIRO_HIP_STAGE_COUNT];          /* summed kernel time per stage */
+    uint32_t launches[SHIRO_HIP_STAGE_COUNT]; /* launches per stage */
+    uint32_t complete;                        /* 0 if the event pool ran out */
+    uint32_t reserved;
+} shiro_hip_stage_times;
+/* While enabled, every render call records an event after each kernel
+ * launch on the context's stream; shiro_hip_get_stage_times returns the
+ * per-stage sums of the last synchronised render call. */
+int shiro_hip_set_stage_timing(shiro_hip_ctx* ctx, uint32_t enable);
+int shiro_hip_get_stage_times(shiro_hip_ctx* ctx, shiro_hip_stage_times* out);
 
 #ifdef __cplusplus
 }

@@ -125,6 +125,7 @@ class FetchCounts(C.Structure):
         ("samples", C.c_uint64), ("rays", C.c_uint64), ("shadowRays", C.c_uint64),
         ("nodeFetches", C.c_uint64), ("triFetches", C.c_uint64),
         ("envTaps", C.c_uint64), ("cdfSteps", C.c_uint64), ("hitFetches", C.c_uint64),
+        ("shadowNodeFetches", C.c_uint64), ("shadowTriFetches", C.c_uint64),
     ]
 
 
@@ -132,6 +133,16 @@ class Tuning(C.Structure):
     _fields_ = [
         ("pathsPerPass", C.c_uint32), ("fetchMode", C.c_uint32), ("traceBlocks", C.c_uint32),
         ("stageBlocks", C.c_uint32), ("reserved", C.c_uint32 * 4),
+    ]
+
+
+STAGE_NAMES = ["reset", "extend", "shade", "shadow", "resolve", "accum", "megakernel", "unused"]
+
+
+class StageTimes(C.Structure):
+    _fields_ = [
+        ("ms", C.c_float * 8), ("launches", C.c_uint32 * 8),
+        ("complete", C.c_uint32), ("reserved", C.c_uint32),
     ]
 
 
@@ -155,6 +166,7 @@ EXPORTS = [
     "shiro_hip_occluded", "shiro_hip_trace_primary", "shiro_hip_eval_bsdf",
     "shiro_hip_env_sample", "shiro_hip_env_eval", "shiro_hip_count_fetches",
     "shiro_hip_set_tuning", "shiro_hip_get_tuning", "shiro_hip_autotune",
+    "shiro_hip_set_stage_timing", "shiro_hip_get_stage_times",
 ]
 
 

@@ -90,6 +90,13 @@ struct shiro_hip_ctx {
     uint32_t acW = 0, acH = 0, acTotal = 0;
     // Wavefront tuning (shiro_hip_set_tuning / shiro_hip_autotune).
     shiro_hip_tuning tuning{4u << 20, 0u, 8192u, 16384u, {0, 0, 0, 0}};
+    // Stage timing (shiro_hip_set_stage_timing): events recorded after each
+    // launch of the last render call.
+    bool stageTiming = false;
+    std::vector<hipEvent_t> stageEv;
+    std::vector<uint8_t> stageKind;
+    StageMarks marks;
+    shiro_hip_stage_times stageTimes{};
 };
 
 namespace {
@@ -169,6 +176,17 @@ int check_settings(shiro_hip_ctx* ctx, const shiro_hip_settings* s) {
 // kWavefrontStateBytes of path state).
 constexpr uint64_t kWavefrontStateBytes = 6ull << 30;
 
+// Starts the event list of a render call in stage-timing mode: the first
+// event marks the start of the first kernel.
+void begin_marks(shiro_hip_ctx* ctx) {
+    ctx->marks = StageMarks{};
+    if (!ctx->stageTiming) return;
+    ctx->marks.ev = ctx->stageEv.data();
+    ctx->marks.kind = ctx->stageKind.data();
+    ctx->marks.cap = static_cast<uint32_t>(ctx->stageEv.size());
+    ctx->marks.mark(0xFF, ctx->stream);
+}
+
 int enqueue_wavefront(shiro_hip_ctx* ctx, const shiro_hip_camera& cam, const shiro_hip_settings& st, uint32_t start,
                       uint32_t count, const DevFrame& frame, const volatile uint8_t* cancel) {
     const uint64_t np = static_cast<uint64_t>(st.width) * st.height;
@@ -239,6 +257,7 @@ int enqueue_wavefront(shiro_hip_ctx* ctx, const shiro_hip_camera& cam, const shi
     L.traceBlocks = ctx->tuning.traceBlocks;
     L.fetchMode = ctx->tuning.fetchMode;
     HIPC(hipEventRecord(ctx->ev0, ctx->stream), "event");
+    begin_marks(ctx);
     for (uint32_t b = 0; b < count; b += static_cast<uint32_t>(S)) {
         if (cancel && *cancel) {
             (void)hipStreamSynchronize(ctx->stream);
@@ -246,7 +265,7 @@ int enqueue_wavefront(shiro_hip_ctx* ctx, const shiro_hip_camera& cam, const shi
         }
         P.sliceBegin = b;
         P.sliceCount = std::min(static_cast<uint32_t>(S), count - b);
-        HIPC(launch_wavefront_pass(P, L, ctx->stream), "wavefront launch");
+        HIPC(launch_wavefront_pass(P, L, ctx->stream, ctx->stageTiming ? &ctx->marks : nullptr), "wavefront launch");
         if (cancel) HIPC(hipStreamSynchronize(ctx->stream), "wavefront");
     }
     HIPC(hipEventRecord(ctx->ev1, ctx->stream), "event");
@@ -278,6 +297,7 @@ int enqueue_batch(shiro_hip_ctx* ctx, const shiro_hip_camera& cam, const shiro_h
     P.tilesX = (st.width + 7u) / 8u;
     P.counters = countFetches ? ctx->counters.as<DevCounters>() : nullptr;
     HIPC(hipEventRecord(ctx->ev0, ctx->stream), "event");
+    begin_marks(ctx);
     for (uint32_t b = 0; b < count; b += slice) {
         if (cancel && *cancel) {
             (void)hipStreamSynchronize(ctx->stream);
@@ -286,6 +306,7 @@ int enqueue_batch(shiro_hip_ctx* ctx, const shiro_hip_camera& cam, const shiro_h
         P.sliceBegin = b;
         P.sliceCount = std::min(slice, count - b);
         HIPC(launch_render(P, countFetches, ctx->stream), "render launch");
+        if (ctx->stageTiming) ctx->marks.mark(SHIRO_HIP_STAGE_MEGAKERNEL, ctx->stream);
         if (cancel) HIPC(hipStreamSynchronize(ctx->stream), "render");
     }
     HIPC(hipEventRecord(ctx->ev1, ctx->stream), "event");
@@ -297,6 +318,21 @@ int finish_timing(shiro_hip_ctx* ctx) {
     float ms = 0.0f;
     HIPC(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1), "timing");
     ctx->lastMs = ms;
+    if (ctx->stageTiming && ctx->marks.ev) {
+        shiro_hip_stage_times& t = ctx->stageTimes;
+        t = shiro_hip_stage_times{};
+        for (uint32_t i = 1; i < ctx->marks.n; ++i) {
+            float d = 0.0f;
+            HIPC(hipEventElapsedTime(&d, ctx->marks.ev[i - 1], ctx->marks.ev[i]), "stage timing");
+            const uint8_t k = ctx->marks.kind[i];
+            if (k < SHIRO_HIP_STAGE_COUNT) {
+                t.ms[k] += d;
+                t.launches[k] += 1u;
+            }
+        }
+        t.complete = ctx->marks.overflow ? 0u : 1u;
+        ctx->marks = StageMarks{};
+    }
     return SHIRO_HIP_OK;
 }
 
@@ -379,6 +415,7 @@ void shiro_hip_destroy(shiro_hip_ctx* ctx) {
     if (!ctx) return;
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    for (hipEvent_t e : ctx->stageEv) (void)hipEventDestroy(e);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -908,6 +945,34 @@ int shiro_hip_count_fetches(shiro_hip_ctx* ctx, const shiro_hip_camera* camera, 
     out->envTaps = c.envTaps;
     out->cdfSteps = c.cdfSteps;
     out->hitFetches = c.hitFetches;
+    out->shadowNodeFetches = c.shadowNodeFetches;
+    out->shadowTriFetches = c.shadowTriFetches;
+    return SHIRO_HIP_OK;
+}
+
+int shiro_hip_set_stage_timing(shiro_hip_ctx* ctx, uint32_t enable) {
+    if (!ctx) return SHIRO_HIP_INVALID;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    HIPC(hipSetDevice(ctx->device), "set device");
+    if (enable && ctx->stageEv.empty()) {
+        constexpr uint32_t kStageEvents = 8192;
+        ctx->stageEv.reserve(kStageEvents);
+        for (uint32_t i = 0; i < kStageEvents; ++i) {
+            hipEvent_t e = nullptr;
+            HIPC(hipEventCreate(&e), "stage events");
+            ctx->stageEv.push_back(e);
+        }
+        ctx->stageKind.assign(kStageEvents, 0);
+    }
+    ctx->stageTiming = enable != 0;
+    ctx->stageTimes = shiro_hip_stage_times{};
+    return SHIRO_HIP_OK;
+}
+
+int shiro_hip_get_stage_times(shiro_hip_ctx* ctx, shiro_hip_stage_times* out) {
+    if (!ctx || !out) return SHIRO_HIP_INVALID;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    *out = ctx->stageTimes;
     return SHIRO_HIP_OK;
 }
 

@@ -21,7 +21,9 @@ constexpr uint32_t kBranchesPerDepth = 12u;
 template <bool COUNT>
 struct Counters {
     unsigned long long rays = 0, shadowRays = 0, nodes = 0, tris = 0, taps = 0, steps = 0, hits = 0;
+    unsigned long long shadowNodes = 0, shadowTris = 0;
     DEV TravCount<COUNT> trav() { return TravCount<COUNT>{&nodes, &tris}; }
+    DEV TravCount<COUNT> shadow_trav() { return TravCount<COUNT>{&shadowNodes, &shadowTris}; }
     DEV EnvCount<COUNT> env() { return EnvCount<COUNT>{&taps, &steps}; }
 };
 
@@ -33,7 +35,7 @@ DEV bool occluded(const DevScene& S, V3 o, V3 d, float maxDist, Counters<COUNT>&
     r.d = d;
     r.tmin = kRayEps;
     r.tmax = maxDist;
-    return intersect_any(S, r, c.trav());
+    return intersect_any(S, r, c.shadow_trav());
 }
 
 DEV V3 world_to_dome(const DevDome& d, V3 v) {

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include "layout.h"
+#include "../../include/shiro_hip.h"
 
 namespace shiro_hip {
 
@@ -53,8 +54,26 @@ struct WfLaunch {
     uint32_t fetchMode;    // traversal work distribution, see k_wf_trace
 };
 
+// Optional per-kernel timing: an event recorded after each launch, tagged
+// with the stage (SHIRO_HIP_STAGE_*) whose kernel it closes.
+struct StageMarks {
+    hipEvent_t* ev = nullptr;
+    uint8_t* kind = nullptr;
+    uint32_t n = 0, cap = 0;
+    bool overflow = false;
+    void mark(uint8_t k, hipStream_t s) {
+        if (!ev) return;
+        if (n == cap) {
+            overflow = true;
+            return;
+        }
+        (void)hipEventRecord(ev[n], s);
+        kind[n++] = k;
+    }
+};
+
 hipError_t launch_render(const RenderParams& P, bool count, hipStream_t stream);
-hipError_t launch_wavefront_pass(const WfParams& P, const WfLaunch& L, hipStream_t s);
+hipError_t launch_wavefront_pass(const WfParams& P, const WfLaunch& L, hipStream_t s, StageMarks* marks);
 hipError_t launch_accum_add(const AccumParams& A, const DevFrame& f, uint32_t count, uint32_t np, hipStream_t s);
 hipError_t launch_accum_resolve(const AccumParams& A, const DevFrame& f, uint32_t total, uint32_t np, hipStream_t s);
 hipError_t launch_qmc_fill(uint32_t x0, uint32_t y0, uint32_t w, uint32_t h, uint32_t s, uint32_t dim0,

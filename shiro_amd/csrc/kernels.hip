@@ -84,6 +84,8 @@ __global__ __launch_bounds__(64) void k_render(RenderParams P) {
         atomicAdd(&c->envTaps, cnt.taps);
         atomicAdd(&c->cdfSteps, cnt.steps);
         atomicAdd(&c->hitFetches, cnt.hits);
+        atomicAdd(&c->shadowNodeFetches, cnt.shadowNodes);
+        atomicAdd(&c->shadowTriFetches, cnt.shadowTris);
     }
     if (P.sliceBegin + P.sliceCount < P.batchCount) {
         P.sums.beauty[pix] = beauty;

@@ -143,6 +143,7 @@ struct DevFrame {
 // Fetch counters (roofline accounting); all 64-bit.
 struct DevCounters {
     unsigned long long samples, rays, shadowRays, nodeFetches, triFetches, envTaps, cdfSteps, hitFetches;
+    unsigned long long shadowNodeFetches, shadowTriFetches;
 };
 
 struct RenderParams {

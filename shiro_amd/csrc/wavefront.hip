@@ -470,12 +470,15 @@ __global__ __launch_bounds__(64) void k_wf_accum(WfParams P) {
     P.frame.depth[pix] = depth;
 }
 
-hipError_t launch_wavefront_pass(const WfParams& P, const WfLaunch& L, hipStream_t s) {
+hipError_t launch_wavefront_pass(const WfParams& P, const WfLaunch& L, hipStream_t s, StageMarks* marks) {
     const uint32_t total = P.sliceCount * P.tiles * 64u;
     if (total == 0) return hipSuccess;
+    StageMarks none;
+    StageMarks& m = marks ? *marks : none;
     const uint32_t fullBlocks = (total + 63u) / 64u;
     const uint32_t blocks = fullBlocks < L.maxBlocks ? fullBlocks : L.maxBlocks;
     hipLaunchKernelGGL(k_wf_reset, dim3(1), dim3(256), 0, s, P.B.counts, P.B.countSlots);
+    m.mark(SHIRO_HIP_STAGE_RESET, s);
     const uint32_t tblocks = fullBlocks < L.traceBlocks ? fullBlocks : L.traceBlocks;
     for (uint32_t d = 0; d < P.settings.maxDepth; ++d) {
         switch (L.fetchMode) {
@@ -483,15 +486,20 @@ hipError_t launch_wavefront_pass(const WfParams& P, const WfLaunch& L, hipStream
             case 1: hipLaunchKernelGGL((k_wf_trace<false, 1>), dim3(blocks), dim3(64), 0, s, P, d); break;
             default: hipLaunchKernelGGL((k_wf_trace<false, 0>), dim3(blocks), dim3(64), 0, s, P, d); break;
         }
+        m.mark(SHIRO_HIP_STAGE_EXTEND, s);
         hipLaunchKernelGGL(k_wf_shade, dim3(blocks), dim3(64), 0, s, P, d);
+        m.mark(SHIRO_HIP_STAGE_SHADE, s);
         switch (L.fetchMode) {
             case 2: hipLaunchKernelGGL((k_wf_trace<true, 2>), dim3(tblocks), dim3(64), 0, s, P, d); break;
             case 1: hipLaunchKernelGGL((k_wf_trace<true, 1>), dim3(blocks), dim3(64), 0, s, P, d); break;
             default: hipLaunchKernelGGL((k_wf_trace<true, 0>), dim3(blocks), dim3(64), 0, s, P, d); break;
         }
+        m.mark(SHIRO_HIP_STAGE_SHADOW, s);
         hipLaunchKernelGGL(k_wf_resolve, dim3(blocks), dim3(64), 0, s, P, d);
+        m.mark(SHIRO_HIP_STAGE_RESOLVE, s);
     }
     hipLaunchKernelGGL(k_wf_accum, dim3(P.tiles), dim3(64), 0, s, P);
+    m.mark(SHIRO_HIP_STAGE_ACCUM, s);
     return hipGetLastError();
 }
 

@@ -313,6 +313,17 @@ class Context:
                                                    C.byref(out)))
         return out
 
+    def set_stage_timing(self, enable=True):
+        self._check(self.L.shiro_hip_set_stage_timing(self.h, 1 if enable else 0))
+
+    def stage_times(self) -> dict:
+        """Per-stage kernel milliseconds and launch counts of the last
+        synchronised render call (stage-timing mode)."""
+        out = abi.StageTimes()
+        self._check(self.L.shiro_hip_get_stage_times(self.h, C.byref(out)))
+        return {name: {"ms": out.ms[i], "launches": out.launches[i]}
+                for i, name in enumerate(abi.STAGE_NAMES) if out.launches[i]} | {"complete": bool(out.complete)}
+
 
 class HipBackend:
     """RenderBackend (RenderBackend.h:28-34) over libshiro_hip.  Like the
