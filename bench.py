@@ -22,7 +22,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-NODE_BYTES, TRI_BYTES, TEXEL_BYTES, CDF_BYTES, HIT_BYTES = 128, 48, 16, 4, 96
+NODE_BYTES, TRI_BYTES, TEXEL_BYTES, CDF_BYTES, HIT_BYTES = 64, 48, 16, 4, 96
 
 
 def workload_desc(wl):
@@ -151,7 +151,7 @@ def main():
     ok = bool(torch.isfinite(frame).all().item())
 
     # Roofline of the dominant render kernel.  Algorithmic bytes come from
-    # the fetch counters (same BVH, same traversal order): BVH4 nodes 128 B,
+    # the fetch counters (same BVH, same traversal order): BVH4 nodes 64 B,
     # triangles 48 B, hit data 96 B, environment texels 16 B, CDF entries
     # 4 B, AOV write 44 B per pixel per batch.  The wavefront's closest-hit
     # kernel is credited with the closest-hit rays' node and triangle bytes,

@@ -466,7 +466,7 @@ int shiro_hip_upload_scene(shiro_hip_ctx* ctx, const shiro_hip_scene* s) {
     const size_t nt = flat.material.size();
     Bvh4 bvh;
     if (!build_bvh4(flat.tri.data(), nt, kMaxBvhDepth, &bvh, &err)) return fail(ctx, SHIRO_HIP_INVALID, err);
-    if (!validate_bvh4(bvh, nt, kMaxBvhDepth, &err)) return fail(ctx, SHIRO_HIP_INVALID, "BVH check: " + err);
+    if (!validate_bvh4(bvh, flat.tri.data(), nt, kMaxBvhDepth, &err)) return fail(ctx, SHIRO_HIP_INVALID, "BVH check: " + err);
     const auto t1 = std::chrono::steady_clock::now();
     // Triangles and hit data in leaf order.
     std::vector<float> tris(nt * 12), hit(nt * 12);

@@ -161,14 +161,58 @@ void pad(Box& b) {
     }
 }
 
+// Quantises a node's child boxes to 8 bits per plane relative to their
+// union (layout.h): origin = union.lo, per-axis scale 2^e with
+// 255 * 2^e >= extent, lo rounded down and hi rounded up, so every
+// dequantised box contains the exact one.
+void quantize_node(const Box* boxes, int nk, const uint32_t* words, float* nd) {
+    Box u;
+    for (int k = 0; k < nk; ++k) u.grow(boxes[k]);
+    uint32_t q[6] = {0, 0, 0, 0, 0, 0};  // lo.x hi.x lo.y hi.y lo.z hi.z, byte k = child k
+    uint32_t expo = 0;
+    for (int a = 0; a < 3; ++a) {
+        const double origin = u.lo[a];
+        const double extent = static_cast<double>(u.hi[a]) - origin;
+        int e = extent > 0.0 ? static_cast<int>(std::ceil(std::log2(extent / 255.0))) : -100;
+        e = std::max(-100, std::min(100, e));
+        while (std::ldexp(255.0, e) < extent) ++e;
+        const double s = std::ldexp(1.0, e);
+        for (int k = 0; k < 4; ++k) {
+            uint32_t lo = 0xffu, hi = 0u;  // empty: inverted, and masked out by the valid bits
+            if (k < nk) {
+                double ql = std::floor((static_cast<double>(boxes[k].lo[a]) - origin) / s);
+                double qh = std::ceil((static_cast<double>(boxes[k].hi[a]) - origin) / s);
+                ql = std::max(0.0, std::min(255.0, ql));
+                qh = std::max(0.0, std::min(255.0, qh));
+                while (ql > 0.0 && origin + ql * s > boxes[k].lo[a]) ql -= 1.0;
+                while (qh < 255.0 && origin + qh * s < boxes[k].hi[a]) qh += 1.0;
+                lo = static_cast<uint32_t>(ql);
+                hi = static_cast<uint32_t>(qh);
+            }
+            q[2 * a] |= lo << (8 * k);
+            q[2 * a + 1] |= hi << (8 * k);
+        }
+        expo |= static_cast<uint32_t>(e + 128) << (8 * a);
+        nd[a] = u.lo[a];
+    }
+    uint32_t valid = 0;
+    for (int k = 0; k < nk; ++k) valid |= 1u << k;
+    expo |= valid << 24;
+    std::memcpy(&nd[3], &expo, 4);
+    std::memcpy(&nd[4], words, 16);
+    std::memcpy(&nd[8], q, 24);
+    nd[14] = nd[15] = 0.0f;
+}
+
 struct Collapser {
     const Builder& B;
     Bvh4* out;
+    float slack = 0.0f;   // absolute widening of every box (scene-scale rounding margin)
     uint32_t maxDepth = 0;
 
     uint32_t alloc() {
-        const uint32_t id = static_cast<uint32_t>(out->nodes.size() / 32u);
-        out->nodes.resize(out->nodes.size() + 32u, 0.0f);
+        const uint32_t id = static_cast<uint32_t>(out->nodes.size() / kNodeFloats);
+        out->nodes.resize(out->nodes.size() + kNodeFloats, 0.0f);
         return id;
     }
 
@@ -202,39 +246,27 @@ struct Collapser {
         uint32_t childId[4] = {kEmpty, kEmpty, kEmpty, kEmpty};
         for (int k = 0; k < nk; ++k)
             if (B.nodes[kids[k]].count == 0) childId[k] = alloc();
-        uint32_t sub = 0;
-        float* nd = &out->nodes[static_cast<size_t>(id) * 32u];
-        for (int k = 0; k < 4; ++k) {
-            float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
-            uint32_t word = kEmpty, cnt = 0;
-            if (k < nk) {
-                const Node2& c = B.nodes[kids[k]];
-                Box b = c.box;
-                pad(b);
-                for (int a = 0; a < 3; ++a) {
-                    lo[a] = b.lo[a];
-                    hi[a] = b.hi[a];
-                }
-                if (c.count > 0) {
-                    word = kLeafBit | c.left;
-                    cnt = c.count;
-                    out->leafCount += 1;
-                    out->maxLeafSize = std::max(out->maxLeafSize, c.count);
-                    out->leafEnd[c.left + c.count - 1] = 1;
-                } else {
-                    word = childId[k];
-                }
+        Box boxes[4];
+        uint32_t words[4] = {kEmpty, kEmpty, kEmpty, kEmpty};
+        for (int k = 0; k < nk; ++k) {
+            const Node2& c = B.nodes[kids[k]];
+            boxes[k] = c.box;
+            pad(boxes[k]);
+            for (int a = 0; a < 3; ++a) {
+                boxes[k].lo[a] -= slack;
+                boxes[k].hi[a] += slack;
             }
-            nd = &out->nodes[static_cast<size_t>(id) * 32u];
-            nd[0 + k] = lo[0];
-            nd[4 + k] = hi[0];
-            nd[8 + k] = lo[1];
-            nd[12 + k] = hi[1];
-            nd[16 + k] = lo[2];
-            nd[20 + k] = hi[2];
-            std::memcpy(&nd[24 + k], &word, 4);
-            std::memcpy(&nd[28 + k], &cnt, 4);
+            if (c.count > 0) {
+                words[k] = kLeafBit | c.left;
+                out->leafCount += 1;
+                out->maxLeafSize = std::max(out->maxLeafSize, c.count);
+                out->leafEnd[c.left + c.count - 1] = 1;
+            } else {
+                words[k] = childId[k];
+            }
         }
+        quantize_node(boxes, nk, words, &out->nodes[static_cast<size_t>(id) * kNodeFloats]);
+        uint32_t sub = 0;
         for (int k = 0; k < nk; ++k)
             if (childId[k] != kEmpty) sub = std::max(sub, emit(childId[k], kids[k]));
         return sub + 1;
@@ -273,8 +305,13 @@ bool build_bvh4(const float* tris, size_t n, uint32_t maxDepth, Bvh4* out, std::
     out->order.resize(n);
     for (size_t i = 0; i < n; ++i) out->order[i] = B.refs[i].tri;
     out->leafEnd.assign(n, 0);
-    Collapser C{B, out};
-    out->nodes.reserve(32u * (n / 2 + 4));
+    // Rounding margin of the device slab test, which forms plane distances
+    // from differences of scene coordinates: 16 ulps of the largest one.
+    float scale = 0.0f;
+    for (int a = 0; a < 3; ++a)
+        scale = std::max(scale, std::max(std::fabs(B.nodes[0].box.lo[a]), std::fabs(B.nodes[0].box.hi[a])));
+    Collapser C{B, out, scale * 16.0f * FLT_EPSILON};
+    out->nodes.reserve(kNodeFloats * (n / 2 + 4));
     const uint32_t root = C.alloc();
     out->depth = C.emit(root, 0);
     if (out->depth > maxDepth) {
@@ -285,59 +322,98 @@ bool build_bvh4(const float* tris, size_t n, uint32_t maxDepth, Bvh4* out, std::
     return true;
 }
 
-bool validate_bvh4(const Bvh4& b, size_t n, uint32_t maxDepth, std::string* err) {
+namespace {
+
+// Dequantised box of child k of a node, in double precision.
+void child_box(const float* nd, int k, double lo[3], double hi[3]) {
+    uint32_t ex, q[6];
+    std::memcpy(&ex, &nd[3], 4);
+    std::memcpy(q, &nd[8], 24);
+    for (int a = 0; a < 3; ++a) {
+        const double s = std::ldexp(1.0, static_cast<int>((ex >> (8 * a)) & 0xffu) - 128);
+        lo[a] = nd[a] + static_cast<double>((q[2 * a] >> (8 * k)) & 0xffu) * s;
+        hi[a] = nd[a] + static_cast<double>((q[2 * a + 1] >> (8 * k)) & 0xffu) * s;
+    }
+}
+
+struct Checker {
+    const Bvh4& b;
+    const float* tris;
+    size_t n;
+    uint32_t maxDepth;
+    uint64_t nn;
+    std::vector<uint8_t> seen, slotSeen;
+    std::string* err;
+
+    bool fail(const char* m) {
+        *err = m;
+        return false;
+    }
+
+    // Visits node `id`; grows lo/hi by the vertices of every triangle below
+    // it and checks that each child's box contains its triangles.
+    bool visit(uint32_t id, uint32_t depth, double lo[3], double hi[3]) {
+        if (depth > maxDepth) return fail("depth bound exceeded");
+        const float* nd = &b.nodes[static_cast<size_t>(id) * kNodeFloats];
+        uint32_t ex;
+        std::memcpy(&ex, &nd[3], 4);
+        for (int k = 0; k < 4; ++k) {
+            uint32_t w;
+            std::memcpy(&w, &nd[4 + k], 4);
+            const bool valid = (ex >> (24 + k)) & 1u;
+            if (w == kEmpty) {
+                if (valid) return fail("empty child marked valid");
+                continue;
+            }
+            if (!valid) return fail("child not marked valid");
+            double clo[3] = {INFINITY, INFINITY, INFINITY}, chi[3] = {-INFINITY, -INFINITY, -INFINITY};
+            if (w & kLeafBit) {
+                uint32_t slot = w & ~kLeafBit;
+                for (;;) {
+                    if (slot >= n) return fail("leaf runs past the triangle array");
+                    if (slotSeen[slot]) return fail("triangle slot referenced twice");
+                    slotSeen[slot] = 1;
+                    const float* t = tris + 9 * static_cast<size_t>(b.order[slot]);
+                    for (int v = 0; v < 3; ++v)
+                        for (int a = 0; a < 3; ++a) {
+                            clo[a] = std::min(clo[a], static_cast<double>(t[3 * v + a]));
+                            chi[a] = std::max(chi[a], static_cast<double>(t[3 * v + a]));
+                        }
+                    if (b.leafEnd[slot]) break;
+                    ++slot;
+                }
+            } else {
+                if (w == 0u || w >= nn || seen[w]) return fail("bad or shared child node index");
+                seen[w] = 1;
+                if (!visit(w, depth + 1, clo, chi)) return false;
+            }
+            double qlo[3], qhi[3];
+            child_box(nd, k, qlo, qhi);
+            for (int a = 0; a < 3; ++a) {
+                if (!(qlo[a] <= clo[a] && qhi[a] >= chi[a])) return fail("quantised child box does not contain its triangles");
+                lo[a] = std::min(lo[a], clo[a]);
+                hi[a] = std::max(hi[a], chi[a]);
+            }
+        }
+        return true;
+    }
+};
+
+}  // namespace
+
+bool validate_bvh4(const Bvh4& b, const float* tris, size_t n, uint32_t maxDepth, std::string* err) {
     const uint64_t nn = b.nodeCount();
     if (n == 0) return nn == 0 || (*err = "nodes without triangles", false);
     if (nn == 0) {
         *err = "no nodes";
         return false;
     }
-    std::vector<uint8_t> seen(nn, 0);
-    std::vector<uint8_t> slotSeen(n, 0);
-    struct Item {
-        uint32_t node, depth;
-    };
-    std::vector<Item> todo{{0u, 1u}};
-    seen[0] = 1;
-    while (!todo.empty()) {
-        const Item it = todo.back();
-        todo.pop_back();
-        if (it.depth > maxDepth) {
-            *err = "depth bound exceeded";
-            return false;
-        }
-        const float* nd = &b.nodes[static_cast<size_t>(it.node) * 32u];
-        for (int k = 0; k < 4; ++k) {
-            uint32_t w;
-            std::memcpy(&w, &nd[24 + k], 4);
-            if (w == kEmpty) continue;
-            if (w & kLeafBit) {
-                uint32_t slot = w & ~kLeafBit;
-                for (;;) {
-                    if (slot >= n) {
-                        *err = "leaf runs past the triangle array";
-                        return false;
-                    }
-                    if (slotSeen[slot]) {
-                        *err = "triangle slot referenced twice";
-                        return false;
-                    }
-                    slotSeen[slot] = 1;
-                    if (b.leafEnd[slot]) break;
-                    ++slot;
-                }
-            } else {
-                if (w == 0u || w >= nn || seen[w]) {
-                    *err = "bad or shared child node index";
-                    return false;
-                }
-                seen[w] = 1;
-                todo.push_back({w, it.depth + 1u});
-            }
-        }
-    }
+    Checker c{b, tris, n, maxDepth, nn, std::vector<uint8_t>(nn, 0), std::vector<uint8_t>(n, 0), err};
+    c.seen[0] = 1;
+    double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    if (!c.visit(0u, 1u, lo, hi)) return false;
     for (size_t i = 0; i < n; ++i)
-        if (!slotSeen[i]) {
+        if (!c.slotSeen[i]) {
             *err = "triangle slot not referenced by any leaf";
             return false;
         }

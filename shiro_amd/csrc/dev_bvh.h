@@ -2,16 +2,16 @@
 //
 // Replaces Embree's rtcIntersect1 / rtcOccluded1 (reference
 // CpuPathTracer.cpp:601-681) and the brute-force fallback (:386-475).
-//  - Nodes are 128 B (one L2 line): the four child boxes in SoA form, child
-//    words and leaf counts (layout.h).  A thread loads a node with eight
-//    16-byte loads.
+//  - Nodes are 64 B: an fp32 origin, per-axis power-of-two scales, child
+//    words and the child boxes quantised to 8 bits per plane (layout.h).  A
+//    thread loads a node with four 16-byte loads.
 //  - Closest hit: front-to-back ordered traversal; the hit children of a
 //    node are sorted by entry distance with a five-comparator network, the
 //    nearest is visited next and the rest are pushed (farthest first) with
 //    their entry distances, so popped entries beyond the current hit are
 //    skipped.  Exact ties in distance resolve to the lower global triangle
 //    index, which is the first-in-scene-order rule of the reference's scan.
-//  - Any hit: unordered, returns at the first accepted triangle.
+//  - Any hit: same child order, returns at the first accepted triangle.
 //  - The triangle test is the watertight test of Woop, Benthin and Wald
 //    (JCGT 2013): shear to the ray's dominant axis, edge functions U, V, W
 //    (recomputed in fp64 when one is exactly zero), so rays through shared
@@ -118,35 +118,51 @@ struct TravCount {
     }
 };
 
-// Slab test of the four children of a node; returns a 4-bit hit mask and
-// the entry distances.  Near and far planes are chosen by the sign of the
-// ray direction, so an inverted box (lo = +inf, hi = -inf: an empty child
-// slot) always misses; the inverse direction is finite, so no NaN arises.
-// Plane distances are fused multiply-adds (plane * inv - origin * inv): the
-// boxes are padded on the host, so the box test only has to be conservative,
-// and the triangle test alone decides hits.
-DEV uint32_t node_boxes(const float4* __restrict__ nd, const RayPre& r, float tmax, float tn[4]) {
-    const float4 lox = nd[0], hix = nd[1], loy = nd[2], hiy = nd[3], loz = nd[4], hiz = nd[5];
-    const bool px = r.inv.x >= 0.0f, py = r.inv.y >= 0.0f, pz = r.inv.z >= 0.0f;
-    const float4 nx = px ? lox : hix, fx = px ? hix : lox;
-    const float4 ny = py ? loy : hiy, fy = py ? hiy : loy;
-    const float4 nz = pz ? loz : hiz, fz = pz ? hiz : loz;
-    const float ix = r.inv.x, iy = r.inv.y, iz = r.inv.z;
-    const float ox = -r.o.x * ix, oy = -r.o.y * iy, oz = -r.o.z * iz;
-    const float NX[4] = {nx.x, nx.y, nx.z, nx.w}, FX[4] = {fx.x, fx.y, fx.z, fx.w};
-    const float NY[4] = {ny.x, ny.y, ny.z, ny.w}, FY[4] = {fy.x, fy.y, fy.z, fy.w};
-    const float NZ[4] = {nz.x, nz.y, nz.z, nz.w}, FZ[4] = {fz.x, fz.y, fz.z, fz.w};
+DEV float byte_as_float(uint32_t w, int k) { return static_cast<float>((w >> (8 * k)) & 0xffu); }
+
+// Slab test of the four children of a quantised node (layout.h); returns a
+// 4-bit hit mask, the entry distances and the child words.  Per axis the
+// plane distance is q * (2^e / d) + (origin - o) / d: one byte-to-float
+// conversion and one FMA per plane.  Near and far planes are chosen by the
+// sign of the ray direction; empty children are masked out by the node's
+// valid bits.  The boxes are widened on the host by more than the rounding
+// of these expressions, so the test is conservative and the triangle test
+// alone decides hits.
+DEV uint32_t node_boxes(const float4* __restrict__ nd, const RayPre& r, float tmax, float tn[4], uint32_t w[4]) {
+    const float4 h = nd[0];
+    const float4 cw = nd[1];
+    const float4 qa = nd[2];
+    const float4 qb = nd[3];
+    const uint32_t ex = __float_as_uint(h.w);
+    const float ax = ldexpf(r.inv.x, static_cast<int>(ex & 0xffu) - 128);
+    const float ay = ldexpf(r.inv.y, static_cast<int>((ex >> 8) & 0xffu) - 128);
+    const float az = ldexpf(r.inv.z, static_cast<int>((ex >> 16) & 0xffu) - 128);
+    const float bx = (h.x - r.o.x) * r.inv.x;
+    const float by = (h.y - r.o.y) * r.inv.y;
+    const float bz = (h.z - r.o.z) * r.inv.z;
+    const uint32_t lx = __float_as_uint(qa.x), hx = __float_as_uint(qa.y);
+    const uint32_t ly = __float_as_uint(qa.z), hy = __float_as_uint(qa.w);
+    const uint32_t lz = __float_as_uint(qb.x), hz = __float_as_uint(qb.y);
+    const uint32_t nx = r.inv.x >= 0.0f ? lx : hx, fx = r.inv.x >= 0.0f ? hx : lx;
+    const uint32_t ny = r.inv.y >= 0.0f ? ly : hy, fy = r.inv.y >= 0.0f ? hy : ly;
+    const uint32_t nz = r.inv.z >= 0.0f ? lz : hz, fz = r.inv.z >= 0.0f ? hz : lz;
     uint32_t mask = 0;
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
-        const float t0 = fmaxf(fmaxf(__builtin_fmaf(NX[c], ix, ox), __builtin_fmaf(NY[c], iy, oy)),
-                               fmaxf(__builtin_fmaf(NZ[c], iz, oz), r.tmin));
-        const float t1 = fminf(fminf(__builtin_fmaf(FX[c], ix, ox), __builtin_fmaf(FY[c], iy, oy)),
-                               fminf(__builtin_fmaf(FZ[c], iz, oz), tmax));
+        const float t0 = fmaxf(fmaxf(__builtin_fmaf(byte_as_float(nx, c), ax, bx),
+                                     __builtin_fmaf(byte_as_float(ny, c), ay, by)),
+                               fmaxf(__builtin_fmaf(byte_as_float(nz, c), az, bz), r.tmin));
+        const float t1 = fminf(fminf(__builtin_fmaf(byte_as_float(fx, c), ax, bx),
+                                     __builtin_fmaf(byte_as_float(fy, c), ay, by)),
+                               fminf(__builtin_fmaf(byte_as_float(fz, c), az, bz), tmax));
         tn[c] = t0;
         mask |= (t0 <= t1 ? 1u : 0u) << c;
     }
-    return mask;
+    w[0] = __float_as_uint(cw.x);
+    w[1] = __float_as_uint(cw.y);
+    w[2] = __float_as_uint(cw.z);
+    w[3] = __float_as_uint(cw.w);
+    return mask & (ex >> 24);
 }
 
 DEV void cswap(float& ka, uint32_t& va, float& kb, uint32_t& vb) {
@@ -158,6 +174,39 @@ DEV void cswap(float& ka, uint32_t& va, float& kb, uint32_t& vb) {
         va = vb;
         vb = tv;
     }
+}
+
+// Orders the hit children of a node by entry distance (five-comparator
+// network), pushes all but the nearest (farthest first) and returns the
+// nearest.  `mask` is non-zero.
+DEV uint32_t push_sorted(uint32_t mask, const float tn[4], const uint32_t w[4], uint32_t* stackNode, float* stackT,
+                         int& sp) {
+    float k[4] = {(mask & 1u) ? tn[0] : kInf, (mask & 2u) ? tn[1] : kInf, (mask & 4u) ? tn[2] : kInf,
+                  (mask & 8u) ? tn[3] : kInf};
+    uint32_t v[4] = {w[0], w[1], w[2], w[3]};
+    cswap(k[0], v[0], k[1], v[1]);
+    cswap(k[2], v[2], k[3], v[3]);
+    cswap(k[0], v[0], k[2], v[2]);
+    cswap(k[1], v[1], k[3], v[3]);
+    cswap(k[1], v[1], k[2], v[2]);
+    const int nh = __popc(mask);
+    for (int i = nh - 1; i >= 1; --i) {
+        stackNode[sp] = v[i];
+        stackT[sp] = k[i];
+        ++sp;
+    }
+    return v[0];
+}
+
+// Pops the next stack entry; a closest-hit traversal drops entries whose
+// entry distance is beyond the current hit.  kEmpty when none is left.
+template <bool ANY>
+DEV uint32_t pop_entry(const uint32_t* stackNode, const float* stackT, int& sp, float tcull) {
+    while (sp > 0) {
+        --sp;
+        if (ANY || stackT[sp] <= tcull) return stackNode[sp];
+    }
+    return kEmpty;
 }
 
 // Tests the triangles of the leaf starting at `slot` (the last one carries
@@ -205,29 +254,13 @@ DEV HitRec intersect_closest(const DevScene& S, const Ray& ray, const TravCount<
     for (;;) {
         // Node phase.
         while (cur != kEmpty && !(cur & kLeafBit)) {
-            const float4* nd = S.nodes + static_cast<size_t>(cur) * 8u;
+            const float4* nd = S.nodes + static_cast<size_t>(cur) * 4u;
             cnt.node();
             float tn[4];
-            const uint32_t mask = node_boxes(nd, r, best.t, tn);
+            uint32_t w[4];
+            const uint32_t mask = node_boxes(nd, r, best.t, tn, w);
             if (mask) {
-                const float4 cw = nd[6];
-                float k[4] = {(mask & 1u) ? tn[0] : kInf, (mask & 2u) ? tn[1] : kInf, (mask & 4u) ? tn[2] : kInf,
-                              (mask & 8u) ? tn[3] : kInf};
-                uint32_t v[4] = {__float_as_uint(cw.x), __float_as_uint(cw.y), __float_as_uint(cw.z),
-                                 __float_as_uint(cw.w)};
-                cswap(k[0], v[0], k[1], v[1]);
-                cswap(k[2], v[2], k[3], v[3]);
-                cswap(k[0], v[0], k[2], v[2]);
-                cswap(k[1], v[1], k[3], v[3]);
-                cswap(k[1], v[1], k[2], v[2]);
-                const int nh = __popc(mask);
-                // Push the farther hits (farthest first), descend into the nearest.
-                for (int i = nh - 1; i >= 1; --i) {
-                    stackNode[sp] = v[i];
-                    stackT[sp] = k[i];
-                    ++sp;
-                }
-                cur = v[0];
+                cur = push_sorted(mask, tn, w, stackNode, stackT, sp);
             } else {
                 cur = kEmpty;
                 while (sp > 0) {
@@ -256,36 +289,26 @@ DEV HitRec intersect_closest(const DevScene& S, const Ray& ray, const TravCount<
 }
 
 // Any hit in [tmin, tmax]: Occluded (CpuPathTracer.cpp:659-681, 697-709).
-// Same while-while structure, unordered, first accepted triangle ends it.
+// Same while-while structure and near-first child order as the closest-hit
+// traversal (an occluder near the origin is found early), no distance
+// culling of popped entries, first accepted triangle ends it.
 template <bool COUNT>
 DEV bool intersect_any(const DevScene& S, const Ray& ray, const TravCount<COUNT>& cnt) {
     if (S.nodeCount == 0) return false;
     const RayPre r = precompute(ray);
     uint32_t stackNode[kStackSize];
+    float stackT[kStackSize];
     int sp = 0;
     uint32_t cur = 0u;
     for (;;) {
         while (cur != kEmpty && !(cur & kLeafBit)) {
-            const float4* nd = S.nodes + static_cast<size_t>(cur) * 8u;
+            const float4* nd = S.nodes + static_cast<size_t>(cur) * 4u;
             cnt.node();
             float tn[4];
-            const uint32_t mask = node_boxes(nd, r, ray.tmax, tn);
+            uint32_t w[4];
+            const uint32_t mask = node_boxes(nd, r, ray.tmax, tn, w);
             if (mask) {
-                const float4 cw = nd[6];
-                const uint32_t word[4] = {__float_as_uint(cw.x), __float_as_uint(cw.y), __float_as_uint(cw.z),
-                                          __float_as_uint(cw.w)};
-                int first = -1;
-#pragma unroll
-                for (int c = 0; c < 4; ++c) {
-                    if ((mask >> c) & 1u) {
-                        if (first < 0) {
-                            first = c;
-                        } else {
-                            stackNode[sp++] = word[c];
-                        }
-                    }
-                }
-                cur = word[first];
+                cur = push_sorted(mask, tn, w, stackNode, stackT, sp);
             } else {
                 cur = sp > 0 ? stackNode[--sp] : kEmpty;
             }

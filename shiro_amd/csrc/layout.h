@@ -1,14 +1,17 @@
 // Device-resident scene layout shared by the host uploader and the kernels.
 //
 // HBM layout (all arrays 16-byte aligned, one allocation each):
-//   nodes    BVH4, 128 B per node = 8 x float4:
-//              [0] lo.x of children 0..3   [1] hi.x   [2] lo.y   [3] hi.y
-//              [4] lo.z                    [5] hi.z
-//              [6] child word (uint bits): internal -> node index,
+//   nodes    BVH4 with quantised child boxes, 64 B per node = 4 x float4:
+//              [0] origin.xyz (fp32, the union of the child boxes' lo
+//                  corner), w = bits: byte a (a = x, y, z) = e_a + 128 with
+//                  plane scale 2^e_a; byte 3 = valid-child mask
+//              [1] child word (uint bits): internal -> node index,
 //                  leaf -> kLeafBit | first triangle slot, empty -> kEmpty
-//                  (empty children have an inverted box and never hit)
-//              [7] leaf triangle count per child (uint bits; statistics)
-//            One node is one 128-byte cache line of gfx950's L2.
+//              [2] quantised planes, one uint per plane kind with byte k
+//                  for child k: lo.x, hi.x, lo.y, hi.y
+//              [3] lo.z, hi.z, 0, 0
+//            A child's box is origin + q * 2^e per axis, rounded outwards
+//            on the host.  Two sibling nodes share one 128-byte L2 line.
 //   tris     48 B per triangle in leaf order = 3 x float4:
 //              {v0.xyz, global prim id bits}, {v1.xyz, end-of-leaf flag},
 //              {v2.xyz, 0}
@@ -39,6 +42,8 @@ constexpr uint32_t kLeafEndFlag = 1u;    // tris[3*slot+1].w: last triangle of i
 // could overflow it (3 pushes per level + 1).
 constexpr int kStackSize = 96;
 constexpr uint32_t kMaxBvhDepth = (kStackSize - 1) / 3;
+
+constexpr size_t kNodeFloats = 16;   // floats per BVH4 node (64 B)
 
 constexpr int kMaxDomes = 4;
 constexpr int kMaxDirLights = 8;

@@ -240,47 +240,34 @@ __global__ __launch_bounds__(64) void k_wf_trace(WfParams P, uint32_t depth) {
         // exhausted (it fetched an off-image path) fetches again.
         if (__all(exhausted)) break;
         if (!active) continue;
-        bool finished = false, occluded = false;
-        // Node phase.
-        while (cur != kEmpty && !(cur & kLeafBit)) {
-            const float4* nd = S.nodes + static_cast<size_t>(cur) * 8u;
-            float tn[4];
-            const uint32_t mask = node_boxes(nd, r, ANY ? tmax : best.t, tn);
-            if (mask) {
-                const float4 cw = nd[6];
-                float k[4] = {(mask & 1u) ? tn[0] : kInf, (mask & 2u) ? tn[1] : kInf, (mask & 4u) ? tn[2] : kInf,
-                              (mask & 8u) ? tn[3] : kInf};
-                uint32_t v[4] = {__float_as_uint(cw.x), __float_as_uint(cw.y), __float_as_uint(cw.z),
-                                 __float_as_uint(cw.w)};
-                cswap(k[0], v[0], k[1], v[1]);
-                cswap(k[2], v[2], k[3], v[3]);
-                cswap(k[0], v[0], k[2], v[2]);
-                cswap(k[1], v[1], k[3], v[3]);
-                cswap(k[1], v[1], k[2], v[2]);
-                const int nh = __popc(mask);
-                for (int i = nh - 1; i >= 1; --i) {
-                    stackNode[sp] = v[i];
-                    stackT[sp] = k[i];
-                    ++sp;
-                }
-                cur = v[0];
-            } else {
-                cur = kEmpty;
-                while (sp > 0) {
-                    --sp;
-                    if (ANY || stackT[sp] <= best.t) {
-                        cur = stackNode[sp];
-                        break;
-                    }
+        bool occluded = false;
+        // Node phase with postponed leaves: the first leaf a lane reaches is
+        // held and the lane keeps descending (from the next stack entry)
+        // until every active lane holds a leaf or has nothing left, so
+        // lanes do not idle while others are still looking for a leaf.
+        uint32_t leaf = kEmpty;
+        for (;;) {
+            if (cur != kEmpty && !(cur & kLeafBit)) {
+                const float4* nd = S.nodes + static_cast<size_t>(cur) * 4u;
+                float tn[4];
+                uint32_t w[4];
+                const uint32_t mask = node_boxes(nd, r, ANY ? tmax : best.t, tn, w);
+                if (mask) {
+                    cur = push_sorted(mask, tn, w, stackNode, stackT, sp);
+                } else {
+                    cur = pop_entry<ANY>(stackNode, stackT, sp, best.t);
                 }
             }
+            if (leaf == kEmpty && cur != kEmpty && (cur & kLeafBit)) {
+                leaf = cur;
+                cur = pop_entry<ANY>(stackNode, stackT, sp, best.t);
+            }
+            if (!__any(leaf == kEmpty && cur != kEmpty)) break;
         }
-        if (cur == kEmpty) {
-            finished = true;
-        } else {
-            // Leaf phase.
+        // Leaf phase.
+        if (leaf != kEmpty) {
             if (ANY) {
-                uint32_t slot = cur & ~kLeafBit;
+                uint32_t slot = leaf & ~kLeafBit;
                 for (;;) {
                     const float4* tp = S.tris + static_cast<size_t>(slot) * 3u;
                     const float4 a = tp[0], b = tp[1], c = tp[2];
@@ -293,22 +280,12 @@ __global__ __launch_bounds__(64) void k_wf_trace(WfParams P, uint32_t depth) {
                     ++slot;
                 }
             } else {
-                leaf_closest(S, r, cur & ~kLeafBit, best, cnt);
-            }
-            cur = kEmpty;
-            if (occluded) {
-                finished = true;
-            } else {
-                while (sp > 0) {
-                    --sp;
-                    if (ANY || stackT[sp] <= best.t) {
-                        cur = stackNode[sp];
-                        break;
-                    }
-                }
-                if (cur == kEmpty) finished = true;
+                leaf_closest(S, r, leaf & ~kLeafBit, best, cnt);
             }
         }
+        // The next entry was popped when the leaf was postponed; the stack is
+        // empty when it is kEmpty.
+        const bool finished = occluded || cur == kEmpty;
         if (finished) {
             if (ANY) {
                 P.B.svis[item] = occluded ? 0u : 1u;
