@@ -65,12 +65,92 @@ def cpu_baseline(wl, target_seconds, threads):
     }
 
 
+def bench_c5(args):
+    """Gradient evaluations per second of the texture extension's
+    differentiable render (c5: 256x256, 32 spp forward + 32 spp path-replay
+    adjoint, depth 4, 50,002 triangles, 128x128 base-colour and roughness
+    textures): one evaluation = textured forward render, MSE to a target
+    image, adjoint into both textures, one Adam update, as a PyTorch loop."""
+    import numpy as np
+    import torch
+    from shiro_amd import workloads
+    from shiro_amd.diff import TextureRenderer
+    wl = workloads.c5()
+    st = wl.settings
+    alb_t, rough_t = wl.textures_truth
+    alb0, rough0 = wl.textures_init
+    from shiro_amd.scene import Texture
+    import copy
+    scene = copy.copy(wl.scene)
+    scene.textures = [Texture(alb_t), Texture(rough_t)]
+    print(f"# workload {workload_desc(wl)}", file=sys.stderr, flush=True)
+    r = TextureRenderer(scene, wl.camera, st)
+    with torch.no_grad():
+        target = r.render(torch.from_numpy(alb_t).cuda(), torch.from_numpy(rough_t).cuda()).clone()
+    albedo = torch.from_numpy(alb0.copy()).cuda().requires_grad_(True)
+    rough = torch.from_numpy(rough0.copy()).cuda().requires_grad_(True)
+    opt = torch.optim.Adam([albedo, rough], lr=0.02)
+    spp = st.samplesPerPixel
+    losses, fwd_ms, adj_ms = [], [], []
+
+    def step(i):
+        opt.zero_grad()
+        img = r.render(albedo, rough)
+        fwd_ms.append(r.ctx.last_render_ms())
+        loss = ((img - target) ** 2).mean()
+        loss.backward()
+        adj_ms.append(r.ctx.last_render_ms())
+        opt.step()
+        with torch.no_grad():
+            albedo.clamp_(0.0, 1.0)
+            rough.clamp_(0.05, 1.0)
+        losses.append(loss.item())
+
+    for i in range(args.warmup):
+        step(i)
+    torch.cuda.synchronize()
+    fwd_ms.clear()
+    adj_ms.clear()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        step(args.warmup + i)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    value = args.steps / wall
+    cpu = None
+    if not args.no_cpu:
+        from oracle.bindings import OracleScene
+        orc = OracleScene(scene, bvh=True)
+        rows = max(8, min(st.height, int(st.height * args.cpu_seconds / 40.0)))
+        y0 = (st.height - rows) // 2
+        dimg = np.zeros((st.height, st.width, 4), np.float32)
+        dimg[y0:y0 + rows, :, :3] = 1.0 / (st.width * st.height * 3)
+        t0 = time.perf_counter()
+        orc.render_region(wl.camera, st, 0, y0, st.width, rows, 0, spp, args.cpu_threads)
+        orc.render_adjoint(wl.camera, st, dimg, wanted=[0, 1], threads=args.cpu_threads)
+        dt = time.perf_counter() - t0
+        cpu = {"value": rows / st.height / dt, "unit": "grads/s", "cores": args.cpu_threads, "kind": "port",
+               "sample": f"rows {y0}-{y0 + rows - 1} ({st.width}x{rows} px, {spp} spp forward + {spp} spp adjoint)"
+                         f" in {dt:.1f} s, scaled to the full frame; CPU oracle, row-parallel threads"}
+    out = {
+        "metric": "grads/s", "value": round(value, 3), "unit": "grads/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(wall * 1e3 / args.steps, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded sphere, target textures and dome)",
+        "config": {"workload": workload_desc(wl), "textures": "128x128x3 base colour + 128x128x1 roughness",
+                   "forward_ms": round(sum(fwd_ms) / len(fwd_ms), 3), "adjoint_ms": round(sum(adj_ms) / len(adj_ms), 3),
+                   "loss_first": losses[0], "loss_last": losses[-1]},
+        "roofline": None, "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3"])
+    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "c5"])
     ap.add_argument("--transport", default="wavefront", choices=["megakernel", "wavefront"])
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--cpu-threads", type=int, default=16)
@@ -96,6 +176,9 @@ def main():
               "(a frame shards by sample range or by tile; see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
+    if args.workload == "c5":
+        bench_c5(args)
+        return
     import torch
     from shiro_amd import workloads
     from shiro_amd.hip import Context

@@ -149,12 +149,19 @@ typedef struct shiro_hip_dome_light {
     uint32_t reserved;
 } shiro_hip_dome_light;
 
-/* Single-channel or RGB float texture for the texture extension. */
+/* Float texture of the texture extension (no counterpart in the reference).
+ * texels[(y * width + x) * channels + c], row 0 at v = 0.  A textured
+ * material replaces baseColor by channels 0..2 of its baseColorTexture (a
+ * one-channel texture gives grey) and roughness by channel 0 of its
+ * roughnessTexture, looked up at the hit's interpolated UV with bilinear
+ * filtering (texel centres at (i + 0.5) / size) and repeat addressing; every
+ * quantity the material derives from them (diffuse colour, F0, lobe weights,
+ * roughnesses) follows per hit. */
 typedef struct shiro_hip_texture {
     const float* texels;
     uint32_t width;
     uint32_t height;
-    uint32_t channels; /* 1 or 3 */
+    uint32_t channels; /* 1 to 4 */
     uint32_t reserved;
 } shiro_hip_texture;
 
@@ -311,6 +318,36 @@ typedef struct shiro_hip_scene_info {
     double buildSeconds;
     double uploadSeconds;
 } shiro_hip_scene_info;
+/* Replaces the texels of texture `index` of the uploaded scene (same size
+ * and channel count) without rebuilding anything.  flags:
+ * SHIRO_HIP_TEXELS_DEVICE: `texels` is device memory (e.g. a torch tensor);
+ * SHIRO_HIP_TEXELS_EVAL_ONLY: only the copy that shading evaluates changes,
+ * sampling decisions (lobe weights, sampling roughness, pdfs) keep reading
+ * the previous texels - a frozen-sampling render whose finite differences
+ * the adjoint reproduces.  Synchronous.  No counterpart in the reference. */
+enum { SHIRO_HIP_TEXELS_DEVICE = 1, SHIRO_HIP_TEXELS_EVAL_ONLY = 2 };
+int shiro_hip_update_texture(
+    shiro_hip_ctx* ctx, uint32_t index, const float* texels, uint32_t flags);
+
+/* Path-replay adjoint of a render with respect to texture texels (texture
+ * extension; no counterpart in the reference).  Replays the paths of
+ * samples [sampleStart, sampleStart + sampleCount) of the uploaded scene,
+ * camera and settings (maxDepth <= 16) and ADDS to dTexels[i] (device
+ * memory, same layout as texture i's texels; NULL entries or i >=
+ * dTexelsCount are skipped) the gradient sum_{pixel, c} dBeauty[pixel][c] *
+ * d beauty[pixel][c] / d texel, where beauty is the batch mean the render
+ * returns and dBeauty (device, W*H*4 floats, alpha ignored) is the caller's
+ * adjoint image.  Sampling is detached (lobe choices, directions, pdfs and
+ * MIS weights are constants), so this is the exact gradient of the frozen-
+ * sampling render (SHIRO_HIP_TEXELS_EVAL_ONLY) at the current texels.
+ * Textured materials must have no specular anisotropy, thin film, sheen or
+ * transmission.  Synchronous. */
+int shiro_hip_render_adjoint(
+    shiro_hip_ctx* ctx, const shiro_hip_camera* camera,
+    const shiro_hip_settings* settings, uint32_t sampleStart,
+    uint32_t sampleCount, const float* dBeauty, float* const* dTexels,
+    uint32_t dTexelsCount);
+
 int shiro_hip_scene_info_get(const shiro_hip_ctx* ctx, shiro_hip_scene_info* out);
 
 /* ---- Entry points exposing single stages of the hot path (device-side

@@ -172,6 +172,27 @@ class OracleScene(_SceneBase):
             self.h, C.byref(cam), C.byref(st), sample_index, abi.up(prim), abi.fp(t))
         return prim, t
 
+    # ---- texture extension ----
+    def update_texture(self, index, texels, eval_only=False):
+        arr = np.ascontiguousarray(texels, np.float32)
+        rc = self._fn("update_texture", C.c_int, C.c_void_p, _u, _fp, C.c_int)(self.h, index, abi.fp(arr),
+                                                                                int(eval_only))
+        if rc:
+            raise IndexError(index)
+
+    def render_adjoint(self, camera, settings, dbeauty, wanted, sample_start=0, sample_count=None, threads=8):
+        """Texel gradients of sum(dbeauty * beauty); `wanted` lists texture
+        indices; returns {index: gradient array shaped like the texels}."""
+        sample_count = settings.samplesPerPixel if sample_count is None else sample_count
+        db = np.ascontiguousarray(dbeauty, np.float32).reshape(-1)
+        ntex = len(self.scene.textures)
+        grads = {i: np.zeros(np.shape(self.scene.textures[i].texels), np.float32) for i in wanted}
+        ptrs = (_fp * max(1, ntex))(*[abi.fp(grads[i]) if i in grads else None for i in range(ntex)])
+        cam, st = camera.to_c(), settings.to_c()
+        self._fn("render_adjoint", C.c_int, C.c_void_p, _P(abi.Camera), _P(abi.Settings), _u, _u, _fp, _P(_fp), _u,
+                 _u)(self.h, C.byref(cam), C.byref(st), sample_start, sample_count, abi.fp(db), ptrs, ntex, threads)
+        return grads
+
 
 class ReferenceScene(_SceneBase):
     """A scene loaded into the reference's CPU path tracer (brute force)."""
@@ -233,6 +254,7 @@ def oracle_fns():
     f.accum_add = _bind(lib, "oracle_accum_add", None, _u, _fp, _fp, _fp, _fp, _u, _fp, _fp, _fp, _fp, _up)
     f.accum_resolve = _bind(lib, "oracle_accum_resolve", None, _u, _u, _fp, _fp, _fp, _fp, _up, _fp, _fp, _fp,
                             _fp)
+    f.bsdf_grad = _bind(lib, "oracle_bsdf_grad", None, _P(abi.Material), _up, _fp, _fp, _fp, _fp, _u, _fp, _fp)
     return f
 
 
@@ -254,6 +276,19 @@ def eval_bsdf(fns, materials, mat_index, n, cn, v, l):
     fns.eval_bsdf(C.cast(mats, _P(abi.Material)), abi.up(mi), *[abi.fp(a) for a in arrs], cnt, abi.fp(val),
                   abi.fp(pdf))
     return val, pdf
+
+
+def bsdf_grad(materials, mat_index, n, cn, v, l):
+    """Oracle d f / d baseColor and d f / d roughness over tuples."""
+    fns = oracle_fns()
+    mats = (abi.Material * len(materials))(*[m.to_c() for m in materials])
+    mi = np.ascontiguousarray(mat_index, np.uint32)
+    arrs = [np.ascontiguousarray(a, np.float32) for a in (n, cn, v, l)]
+    cnt = len(mi)
+    db = np.zeros((cnt, 3), np.float32)
+    dr = np.zeros((cnt, 3), np.float32)
+    fns.bsdf_grad(C.cast(mats, _P(abi.Material)), abi.up(mi), *[abi.fp(a) for a in arrs], cnt, abi.fp(db), abi.fp(dr))
+    return db, dr
 
 
 class Accumulator:

@@ -303,7 +303,16 @@ struct Mesh {
     std::vector<V3> pos;
     std::vector<V3> nrm;
     std::vector<uint32_t> idx;
+    std::vector<V2> uv;   // texture extension: per-vertex UVs (empty: none)
     uint32_t material = 0;
+};
+
+// Texture extension (no reference counterpart): texels shading evaluates
+// and, while frozen, the texels sampling decisions read.
+struct Tex {
+    std::vector<float> eval, sampling;
+    uint32_t w = 0, h = 0, ch = 0;
+    bool frozen = false;
 };
 
 struct Dome {
@@ -341,6 +350,8 @@ struct Scene {
     std::vector<uint32_t> bvhRefs;            // leaf order -> global triangle
     std::vector<BvhNode> bvh;
     bool useBvh = true;
+    std::vector<Tex> textures;
+    bool anyUv = false;
     // Path-guide weights persisting across batches (CpuPathTracer.h:31-33).
     std::vector<float> guideWeights;
     std::mutex guideMutex;
@@ -525,6 +536,7 @@ struct Hit {
     uint32_t mesh = std::numeric_limits<uint32_t>::max();
     uint32_t prim = std::numeric_limits<uint32_t>::max();
     uint32_t material = 0;
+    float b1 = 0.0f, b2 = 0.0f;   // barycentrics of vertices 1 and 2
 };
 
 // Moller-Trumbore with the reference's acceptance rules (:386-428):
@@ -570,6 +582,8 @@ bool finish_hit(const Scene& s, const Ray& r, const Candidate& c, Hit* out) {
     h.mesh = mi;
     h.prim = pi;
     h.material = m.material;
+    h.b1 = c.u;
+    h.b2 = c.v;
     *out = h;
     return true;
 }
@@ -1164,6 +1178,174 @@ V3 bsdf_eval(const Material& m, V3 n, V3 cn, V3 v, V3 l) {
 }
 
 // Snell refraction, false on total internal reflection (:1356-1374).
+// ---------------------------------------------------------------------------
+// Texture extension (no counterpart in the reference).  A textured material
+// replaces baseColor by channels 0..2 of its baseColorTexture (grey for one
+// channel) and roughness by channel 0 of its roughnessTexture, looked up at
+// the hit's interpolated UV with bilinear filtering (texel centres at
+// (i + 0.5) / size, row 0 at v = 0) and repeat addressing; every expression
+// above then runs on the modified material.  `me` is the material shading
+// evaluates, `ms` the one sampling decisions read (they differ only while a
+// texture's sampling texels are frozen).
+// ---------------------------------------------------------------------------
+struct TexTap {
+    uint32_t i[4];
+    float w[4];     // bilinear weights (the derivatives of the lookup)
+    float tx, ty;   // fractional position between texel centres
+};
+
+uint32_t wrap_index(int i, uint32_t n) {
+    const int m = static_cast<int>(n);
+    const int r = i % m;
+    return static_cast<uint32_t>(r < 0 ? r + m : r);
+}
+
+TexTap tex_tap(uint32_t w, uint32_t h, V2 uv) {
+    float x = uv.x * static_cast<float>(w) - 0.5f;
+    float y = uv.y * static_cast<float>(h) - 0.5f;
+    x = x == x ? clampf(x, -16777216.0f, 16777216.0f) : 0.0f;
+    y = y == y ? clampf(y, -16777216.0f, 16777216.0f) : 0.0f;
+    const float fx = std::floor(x), fy = std::floor(y);
+    const float tx = x - fx, ty = y - fy;
+    const uint32_t x0 = wrap_index(static_cast<int>(fx), w), y0 = wrap_index(static_cast<int>(fy), h);
+    const uint32_t x1 = x0 + 1u == w ? 0u : x0 + 1u, y1 = y0 + 1u == h ? 0u : y0 + 1u;
+    TexTap t;
+    t.i[0] = y0 * w + x0;
+    t.i[1] = y0 * w + x1;
+    t.i[2] = y1 * w + x0;
+    t.i[3] = y1 * w + x1;
+    t.w[0] = (1.0f - tx) * (1.0f - ty);
+    t.w[1] = tx * (1.0f - ty);
+    t.w[2] = (1.0f - tx) * ty;
+    t.w[3] = tx * ty;
+    t.tx = tx;
+    t.ty = ty;
+    return t;
+}
+
+// Two lerps along x, one along y (exact for equal texels).
+float tex_channel(const std::vector<float>& texels, uint32_t ch, const TexTap& t, uint32_t c) {
+    const float v0 = texels[t.i[0] * ch + c], v1 = texels[t.i[1] * ch + c];
+    const float v2 = texels[t.i[2] * ch + c], v3 = texels[t.i[3] * ch + c];
+    const float a = v0 + (v1 - v0) * t.tx;
+    const float b = v2 + (v3 - v2) * t.tx;
+    return a + (b - a) * t.ty;
+}
+
+V3 tex_rgb(const std::vector<float>& texels, uint32_t ch, const TexTap& t) {
+    const float r = tex_channel(texels, ch, t, 0u);
+    if (ch < 3u) return {r, r, r};
+    return {r, tex_channel(texels, ch, t, 1u), tex_channel(texels, ch, t, 2u)};
+}
+
+V2 hit_uv(const Scene& s, const Hit& hit) {
+    const Mesh& m = s.meshes[hit.mesh];
+    if (m.uv.empty()) return {0.0f, 0.0f};
+    const V2 a = m.uv[m.idx[hit.prim * 3u]], b = m.uv[m.idx[hit.prim * 3u + 1u]], c = m.uv[m.idx[hit.prim * 3u + 2u]];
+    const float b0 = 1.0f - hit.b1 - hit.b2;
+    return {a.x * b0 + b.x * hit.b1 + c.x * hit.b2, a.y * b0 + b.y * hit.b1 + c.y * hit.b2};
+}
+
+bool textured(const Material& m) { return m.baseColorTexture >= 0 || m.roughnessTexture >= 0; }
+
+void texture_materials(const Scene& s, V2 uv, Material* me, Material* ms) {
+    if (me->baseColorTexture >= 0) {
+        const Tex& t = s.textures[me->baseColorTexture];
+        const TexTap tap = tex_tap(t.w, t.h, uv);
+        const V3 e = tex_rgb(t.eval, t.ch, tap);
+        const V3 q = t.frozen ? tex_rgb(t.sampling, t.ch, tap) : e;
+        me->baseColor[0] = e.x, me->baseColor[1] = e.y, me->baseColor[2] = e.z;
+        ms->baseColor[0] = q.x, ms->baseColor[1] = q.y, ms->baseColor[2] = q.z;
+    }
+    if (me->roughnessTexture >= 0) {
+        const Tex& t = s.textures[me->roughnessTexture];
+        const TexTap tap = tex_tap(t.w, t.h, uv);
+        me->roughness = tex_channel(t.eval, t.ch, tap, 0u);
+        ms->roughness = t.frozen ? tex_channel(t.sampling, t.ch, tap, 0u) : me->roughness;
+    }
+}
+
+// d f / d baseColor (per channel) and d f / d roughness of bsdf_eval at
+// fixed directions, for textured materials without specular anisotropy,
+// thin film, sheen or transmission.  Clamps give zero derivative where
+// active.  Pinned against central differences of bsdf_eval.
+struct BsdfGrad {
+    V3 dBase, dRough;
+};
+
+inline float schlick_weight(float c) {
+    const float x = clampf(1.0f - c, 0.0f, 1.0f);
+    return x * x * x * x * x;
+}
+
+inline float smith_lambda_da2(float c, float a2) {
+    const float cc = std::max(c, 1.0e-6f);
+    const float t = (1.0f - cc * cc) / (cc * cc);
+    return t / (4.0f * std::sqrt(1.0f + a2 * t));
+}
+
+BsdfGrad bsdf_grad(const Material& m, V3 n, V3 cn, V3 v, V3 l) {
+    BsdfGrad g{{0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f}};
+    const float nv = clampf(dot(n, v), 0.0f, 1.0f);
+    const float nl = clampf(dot(n, l), 0.0f, 1.0f);
+    if (nv <= 0.0f || nl <= 0.0f) return g;
+    const V3 f0b = specular_f0(m);
+    const V3 uC = coat_under_transmittance(m, cn, v, l);
+    const float wv = schlick_weight(nv), wl = schlick_weight(nl);
+    const V3 sumF = (schlick(f0b, nv) + schlick(f0b, nl)) * 0.5f;
+    const V3 uB = one_minus_clamped(sumF);
+    const float dUB = -0.5f * ((1.0f - wv) + (1.0f - wl));
+    const V3 dUBdF0{sumF.x > 0.0f && sumF.x < 1.0f ? dUB : 0.0f, sumF.y > 0.0f && sumF.y < 1.0f ? dUB : 0.0f,
+                    sumF.z > 0.0f && sumF.z < 1.0f ? dUB : 0.0f};
+    const V3 vp = normalize(v - n * nv);
+    const V3 lp = normalize(l - n * nl);
+    const float cosPhi = length(vp) > 0.0f && length(lp) > 0.0f ? clampf(dot(vp, lp), -1.0f, 1.0f) : 0.0f;
+    const float sg = clampf(m.diffuseRoughness, 0.0f, 1.0f);
+    const float sg2 = sg * sg;
+    const float A = 1.0f - (sg2 / (2.0f * (sg2 + 0.33f)));
+    const float B = 0.45f * sg2 / (sg2 + 0.09f);
+    const float sv = std::sqrt(std::max(0.0f, 1.0f - nv * nv)), sl = std::sqrt(std::max(0.0f, 1.0f - nl * nl));
+    const float tanBeta = std::min(sv / std::max(nv, 1.0e-6f), sl / std::max(nl, 1.0e-6f));
+    const float wrap = clampf(m.subsurface * m.subsurfaceScale *
+                                  (0.35f + 0.15f * clampf(m.subsurfaceAnisotropy, -1.0f, 1.0f) +
+                                   0.1f * clampf(max3(v3(m.subsurfaceRadius)) / 3.0f, 0.0f, 1.0f)),
+                              0.0f, 0.9f);
+    const float wnl = clampf((nl + wrap) / (1.0f + wrap), 0.0f, 1.0f);
+    const float kd = kInvPi * (A + B * std::max(0.0f, cosPhi) * std::max(sv, sl) * tanBeta) * (wnl / std::max(nl, 1.0e-6f));
+    const float metal = clampf(m.metallic, 0.0f, 1.0f);
+    const float trans = clampf(m.transmission, 0.0f, 1.0f);
+    const float sss = clampf(m.subsurface, 0.0f, 1.0f);
+    const float blend = clampf(max3(v3(m.subsurfaceRadius)) / 3.0f, 0.0f, 1.0f);
+    const float kDiff = ((1.0f - sss) + sss * (1.0f - blend)) *
+                        (clampf(m.baseWeight, 0.0f, 1.0f) * (1.0f - metal) * (1.0f - trans));
+    const V3 dc = diffuse_color(m);
+    const V3 h = safe_normalize(v + l, n);
+    const float nh = clampf(dot(n, h), 0.0f, 1.0f);
+    const float vh = clampf(dot(v, h), 0.0f, 1.0f);
+    const float bR = base_roughness(m);
+    const float rs = clampf(bR, 0.02f, 1.0f);
+    const float a = std::max(0.02f, rs * rs);
+    const float a2 = a * a;
+    const float q = nh * nh * (a2 - 1.0f) + 1.0f;
+    const float D = a2 / (kPi * std::max(1.0e-6f, q * q));
+    const float G = ggx_g2(nv, nl, rs);
+    const float den4 = std::max(4.0f * nv * nl, 1.0e-6f);
+    const float wh = schlick_weight(vh);
+    const V3 F = schlick(f0b, vh);
+    const float dSpecdF0 = (1.0f - wh) * D * G / den4;
+    g.dBase = {kDiff * uC.x * uB.x * kd + dc.x * uC.x * dUBdF0.x * metal * kd + uC.x * dSpecdF0 * metal,
+               kDiff * uC.y * uB.y * kd + dc.y * uC.y * dUBdF0.y * metal * kd + uC.y * dSpecdF0 * metal,
+               kDiff * uC.z * uB.z * kd + dc.z * uC.z * dUBdF0.z * metal * kd + uC.z * dSpecdF0 * metal};
+    if (bR > 0.02f && bR < 1.0f && rs * rs > 0.02f) {
+        const float dDda2 = q * q > 1.0e-6f ? (1.0f / (kPi * q * q)) * (1.0f - 2.0f * a2 * nh * nh / q)
+                                             : 1.0f / (kPi * 1.0e-6f);
+        const float dGda2 = -G * G * (smith_lambda_da2(nv, a2) + smith_lambda_da2(nl, a2));
+        const float dS = (dDda2 * G + D * dGda2) * (2.0f * a * 2.0f * rs) / den4;
+        g.dRough = uC * F * dS;
+    }
+    return g;
+}
+
 bool refract_dir(V3 i, V3 n, float eta, V3* out) {
     const float c = clampf(-dot(i, n), -1.0f, 1.0f);
     const float s2 = std::max(0.0f, 1.0f - c * c);
@@ -1333,15 +1515,106 @@ struct Settings {
     uint32_t rrDepth;
 };
 
-V3 direct_lighting(const Scene& s, const Settings& st, const Hit& hit, V3 n, V3 cn, const Material& m, V3 v,
-                   const Sampler& smp, uint32_t depth) {
+// Adjoint of one sample (texture extension, no reference counterpart): the
+// recorded vertices of the path and the texel-gradient buffers.  The
+// algorithm is the device's (shiro_amd/csrc/adjoint.hip): sampling is
+// detached; T_{j+1} = RR(clamp_max_component(T_j f_j c_j, 12)); light
+// samples add their gradients as they are traced; a reverse sweep carries
+// dLoss/dT through the clamp and the roulette division.
+struct AdjVtx {
+    V3 T, att, C, f, dfa, dfr;
+    V2 uv;
+    float c = 0.0f;
+    uint32_t material = 0, kind = 0, flags = 0;   // kind 0 bounce, 1 end, 2 pass, 3 miss
+};
+
+struct Adjoint {
+    V3 a;                                   // dLoss / d(sample radiance)
+    V3 T;                                   // throughput at the current vertex
+    std::vector<AdjVtx> tape;
+    std::vector<std::vector<float>>* grads; // per texture (empty: not wanted)
+};
+
+V3 clamp_max_vjp(V3 v, float lim, V3 g) {
+    const float c = max3(v);
+    if (c <= lim || c <= 0.0f) return g;
+    V3 r = g * (lim / c);
+    const float corr = dot(g, v) * lim / (c * c);
+    if (!(v.x < std::max(v.y, v.z))) {
+        r.x -= corr;
+    } else if (!(v.y < v.z)) {
+        r.y -= corr;
+    } else {
+        r.z -= corr;
+    }
+    return r;
+}
+
+V3 roulette_vjp(V3 y, V3 g) {
+    const float m = max3(y);
+    const float q = clampf(m, 0.05f, 0.95f);
+    V3 r = g / q;
+    if (!(m < 0.05f) && !(0.95f < m)) {
+        const float corr = dot(g, y) / (q * q);
+        if (!(y.x < std::max(y.y, y.z))) {
+            r.x -= corr;
+        } else if (!(y.y < y.z)) {
+            r.y -= corr;
+        } else {
+            r.z -= corr;
+        }
+    }
+    return r;
+}
+
+void scatter_tex(std::vector<float>& grad, const Tex& t, V2 uv, const float* g, uint32_t n) {
+    const TexTap tap = tex_tap(t.w, t.h, uv);
+    for (int k = 0; k < 4; ++k)
+        for (uint32_t c = 0; c < n; ++c) grad[static_cast<size_t>(tap.i[k]) * t.ch + c] += g[c] * tap.w[k];
+}
+
+void scatter_grad(const Scene& s, Adjoint& adj, const Material& m, V2 uv, V3 gBase, float gRough) {
+    auto& G = *adj.grads;
+    if (m.baseColorTexture >= 0 && static_cast<size_t>(m.baseColorTexture) < G.size() && !G[m.baseColorTexture].empty()) {
+        const Tex& t = s.textures[m.baseColorTexture];
+        if (t.ch >= 3u) {
+            const float g[3] = {gBase.x, gBase.y, gBase.z};
+            scatter_tex(G[m.baseColorTexture], t, uv, g, 3u);
+        } else {
+            const float g[1] = {gBase.x + gBase.y + gBase.z};
+            scatter_tex(G[m.baseColorTexture], t, uv, g, 1u);
+        }
+    }
+    if (m.roughnessTexture >= 0 && static_cast<size_t>(m.roughnessTexture) < G.size() && !G[m.roughnessTexture].empty()) {
+        const float g[1] = {gRough};
+        scatter_tex(G[m.roughnessTexture], s.textures[m.roughnessTexture], uv, g, 1u);
+    }
+}
+
+// Gradient of one visible light sample contrib = clamp(f * lf, 24) that
+// enters the sum with weight 1 / K.
+void light_grad(Adjoint* adj, const Material& me, V3 n, V3 cn, V3 v, V3 ld, V3 f, V3 lf, float invK, V3* gBase,
+                float* gRough) {
+    if (!adj || !textured(me)) return;
+    const V3 gc = adj->a * adj->T * invK;
+    const V3 gf = clamp_max_vjp(f * lf, kDirectClamp, gc) * lf;
+    const BsdfGrad bg = bsdf_grad(me, n, cn, v, ld);
+    *gBase = *gBase + gf * bg.dBase;
+    *gRough += dot(gf, bg.dRough);
+}
+
+V3 direct_lighting(const Scene& s, const Settings& st, const Hit& hit, V3 n, V3 cn, const Material& m,
+                   const Material& ms, V3 v, const Sampler& smp, uint32_t depth, Adjoint* adj, V3* gBase,
+                   float* gRough) {
     V3 L{};
     const V3 origin = hit.p + n * kEps;
     for (const DirLight& dl : s.dirs) {
         const V3 ld = normalize(dl.direction * -1.0f);
         const float nl = clampf(dot(n, ld), 0.0f, 1.0f);
         if (nl <= 0.0f || occluded(s, origin, ld)) continue;
-        L = L + clamp_max_component(bsdf_eval(m, n, cn, v, ld) * dl.radiance * nl, kDirectClamp);
+        const V3 f = bsdf_eval(m, n, cn, v, ld);
+        L = L + clamp_max_component(f * dl.radiance * nl, kDirectClamp);
+        light_grad(adj, m, n, cn, v, ld, f, dl.radiance * nl, 1.0f, gBase, gRough);
     }
     const uint32_t K = std::max(1u, st.domeLightSamples);
     for (size_t li = 0; li < s.domes.size(); ++li) {
@@ -1354,9 +1627,11 @@ V3 direct_lighting(const Scene& s, const Settings& st, const Hit& hit, V3 n, V3 
             if (!dome_sample(s.domes[li], smp.next2(branch), &ld, &lr, &lp) || lp <= 0.0f) continue;
             const float nl = clampf(dot(n, ld), 0.0f, 1.0f);
             if (nl <= 0.0f || occluded(s, origin, ld)) continue;
-            const float bp = bsdf_pdf(m, n, cn, v, ld);
+            const float bp = bsdf_pdf(ms, n, cn, v, ld);
             const float w = power_heuristic(lp, bp);
-            acc = acc + clamp_max_component(bsdf_eval(m, n, cn, v, ld) * lr * (nl * w / lp), kDirectClamp);
+            const V3 f = bsdf_eval(m, n, cn, v, ld);
+            acc = acc + clamp_max_component(f * lr * (nl * w / lp), kDirectClamp);
+            light_grad(adj, m, n, cn, v, ld, f, lr * (nl * w / lp), 1.0f / static_cast<float>(K), gBase, gRough);
         }
         L = L + acc / static_cast<float>(K);
     }
@@ -1384,9 +1659,11 @@ V3 direct_lighting(const Scene& s, const Settings& st, const Hit& hit, V3 n, V3 
                     const float pa = sel / std::max(e->area, 1.0e-6f);
                     const float lp = pa * d2 / std::max(lc, 1.0e-6f);
                     if (lp > 0.0f) {
-                        const float bp = bsdf_pdf(m, n, cn, v, ld);
+                        const float bp = bsdf_pdf(ms, n, cn, v, ld);
                         const float w = power_heuristic(lp, bp);
-                        L = L + clamp_max_component(bsdf_eval(m, n, cn, v, ld) * e->radiance * (nl * w / lp), kDirectClamp);
+                        const V3 f = bsdf_eval(m, n, cn, v, ld);
+                        L = L + clamp_max_component(f * e->radiance * (nl * w / lp), kDirectClamp);
+                        light_grad(adj, m, n, cn, v, ld, f, e->radiance * (nl * w / lp), 1.0f, gBase, gRough);
                     }
                 }
             }
@@ -1422,7 +1699,7 @@ struct PathResult {
 enum Bounce { kNone, kDiffuse, kReflective, kTransmission };
 
 PathResult trace_path(const Scene& s, const Settings& st, Ray ray, const Sampler& smp, const Guide* guide,
-                      std::array<float, kGuideBins>* upd) {
+                      std::array<float, kGuideBins>* upd, Adjoint* adj = nullptr) {
     PathResult res;
     V3 T{1.0f, 1.0f, 1.0f};
     uint32_t nDiffuse = 0, nSpecular = 0;
@@ -1436,21 +1713,47 @@ PathResult trace_path(const Scene& s, const Settings& st, Ray ray, const Sampler
         const uint32_t b0 = depth * kBranchesPerDepth;
         Hit hit;
         if (!intersect(s, ray, &hit)) {
+            V3 factor{};
             if (depth > 0 || st.backgroundVisible) {
                 const V3 Le = environment_radiance(s, ray.d);
                 V3 dL;
                 if (s.domes.empty() || depth == 0 || prevDelta || prevType == kTransmission) {
                     dL = T * Le;
+                    factor = Le;
                 } else {
-                    dL = T * Le * power_heuristic_many(prevPdf, env_pdf_sq_sum(s, ray.d));
+                    const float w = power_heuristic_many(prevPdf, env_pdf_sq_sum(s, ray.d));
+                    dL = T * Le * w;
+                    factor = Le * w;
                 }
                 guide_accumulate(recs, dL, upd);
                 res.radiance = res.radiance + dL;
             }
+            if (adj) {
+                AdjVtx vx;
+                vx.kind = 3;
+                vx.T = T;
+                vx.att = {1.0f, 1.0f, 1.0f};
+                vx.C = factor;
+                adj->tape.push_back(vx);
+            }
             break;
         }
+        const V3 Tin = T;
         if (mediumOn) T = T * medium_attenuation(ext, hit.t);
         const Material m = hit.material < s.materials.size() ? s.materials[hit.material] : default_material();
+        // Texture extension: me is evaluated, ms drives sampling decisions.
+        Material me = m, ms = m;
+        V2 uv{0.0f, 0.0f};
+        if (textured(m)) {
+            uv = hit_uv(s, hit);
+            texture_materials(s, uv, &me, &ms);
+        }
+        AdjVtx* vx = nullptr;
+        if (adj) {
+            adj->tape.push_back(AdjVtx{});
+            vx = &adj->tape.back();
+            vx->kind = 1;
+        }
         const bool front = dot(hit.ng, ray.d) < 0.0f;
         const V3 ng = front ? hit.ng : hit.ng * -1.0f;
         V3 nbase = front ? hit.ns : hit.ns * -1.0f;
@@ -1463,12 +1766,24 @@ PathResult trace_path(const Scene& s, const Settings& st, Ray ray, const Sampler
         if (opacity < 0.999f) {
             if (smp.next1(b0) > opacity) {
                 ray.o = hit.p + ray.d * kEps;
+                if (vx) {
+                    vx->kind = 2;
+                    vx->T = T;
+                    vx->att = {Tin.x != 0.0f ? T.x / Tin.x : 0.0f, Tin.y != 0.0f ? T.y / Tin.y : 0.0f,
+                               Tin.z != 0.0f ? T.z / Tin.z : 0.0f};
+                }
                 continue;
             }
             T = T / std::max(opacity, 1.0e-4f);
         }
+        if (vx) {
+            vx->T = T;
+            vx->att = {Tin.x != 0.0f ? T.x / Tin.x : 0.0f, Tin.y != 0.0f ? T.y / Tin.y : 0.0f,
+                       Tin.z != 0.0f ? T.z / Tin.z : 0.0f};
+            adj->T = T;
+        }
         if (depth == 0) {
-            res.albedo = diffuse_color(m);
+            res.albedo = diffuse_color(me);
             res.normal = n * 0.5f + V3{0.5f, 0.5f, 0.5f};
             res.depth = hit.t;
         }
@@ -1479,21 +1794,31 @@ PathResult trace_path(const Scene& s, const Settings& st, Ray ray, const Sampler
             V3 dL;
             if (depth == 0 || prevDelta || prevType == kTransmission) {
                 dL = T * Le;
+                if (vx) vx->C = Le;
             } else {
                 const float lp = emissive_pdf(s, hit.mesh, hit.prim, ray.o, hit.p);
-                dL = T * Le * (lp > 0.0f ? power_heuristic(prevPdf, lp) : 1.0f);
+                const float w = lp > 0.0f ? power_heuristic(prevPdf, lp) : 1.0f;
+                dL = T * Le * w;
+                if (vx) vx->C = Le * w;
             }
             guide_accumulate(recs, dL, upd);
             res.radiance = res.radiance + dL;
         }
-        const V3 direct = T * direct_lighting(s, st, hit, n, cn, m, v, smp, depth);
+        V3 gBase{};
+        float gRough = 0.0f;
+        const V3 dlight = direct_lighting(s, st, hit, n, cn, me, ms, v, smp, depth, adj, &gBase, &gRough);
+        const V3 direct = T * dlight;
         guide_accumulate(recs, direct, upd);
         res.radiance = res.radiance + direct;
+        if (vx) {
+            vx->C = vx->C + dlight;
+            if (textured(m)) scatter_grad(s, *adj, m, uv, gBase, gRough);
+        }
 
-        const float wd = w_diffuse(m);
-        float ws = w_base_specular(m);
-        const float wc = w_coat(m);
-        const float wtRaw = w_transmission(m);
+        const float wd = w_diffuse(ms);
+        float ws = w_base_specular(ms);
+        const float wc = w_coat(ms);
+        const float wtRaw = w_transmission(ms);
         float wt = wtRaw;
         if (wtRaw > 0.0f) {
             const float ei = inside ? std::max(m.ior, 1.0f) : 1.0f;
@@ -1531,7 +1856,7 @@ PathResult trace_path(const Scene& s, const Settings& st, Ray ray, const Sampler
             type = kTransmission;
             const float pT = std::max(wt / wsum, 1.0e-6f);
             const float rT = m.thinWalled ? 0.0f
-                                          : roughness_sampling(transmission_roughness(m), m.specularAnisotropy,
+                                          : roughness_sampling(transmission_roughness(ms), m.specularAnisotropy,
                                                                m.specularRotation);
             if (m.thinWalled) {
                 next = ray.d;
@@ -1554,8 +1879,8 @@ PathResult trace_path(const Scene& s, const Settings& st, Ray ray, const Sampler
                     if (!refract_dir(ray.d, ng, etaRatio, &next)) {
                         next = normalize(reflect(ray.d, ng));
                         if (wr <= 0.0f) break;
-                        f = bsdf_eval(m, n, cn, v, next);
-                        pdf = std::max((wr / wsum) * bsdf_pdf(m, n, cn, v, next), 1.0e-6f);
+                        f = bsdf_eval(me, n, cn, v, next);
+                        pdf = std::max((wr / wsum) * bsdf_pdf(ms, n, cn, v, next), 1.0e-6f);
                         type = kReflective;
                         delta = true;
                         pickT = false;
@@ -1581,14 +1906,14 @@ PathResult trace_path(const Scene& s, const Settings& st, Ray ray, const Sampler
                     if (!refract_dir(ray.d, mn, etaRatio, &next)) {
                         next = normalize(reflect(ray.d, mn));
                         if (wr <= 0.0f) break;
-                        f = bsdf_eval(m, n, cn, v, next);
-                        pdf = std::max((wr / wsum) * bsdf_pdf(m, n, cn, v, next), 1.0e-6f);
+                        f = bsdf_eval(me, n, cn, v, next);
+                        pdf = std::max((wr / wsum) * bsdf_pdf(ms, n, cn, v, next), 1.0e-6f);
                         type = kReflective;
                         delta = false;
                         pickT = false;
                         pickS = true;
                     } else {
-                        f = transmission_lobe(m, ng, cn, v, next, ei, et, rT);
+                        f = transmission_lobe(me, ng, cn, v, next, ei, et, rT);
                         pdf = std::max(pT * transmission_pdf(ng, v, next, et / ei, rT), 1.0e-6f);
                         inside = !inside;
                         if (inside) {
@@ -1607,13 +1932,13 @@ PathResult trace_path(const Scene& s, const Settings& st, Ray ray, const Sampler
             type = kReflective;
             const V3 sn = pickC ? cn : n;
             const float sr = pickC ? roughness_sampling(m.coatRoughness, m.coatAnisotropy, m.coatRotation)
-                                   : roughness_sampling(base_roughness(m), m.specularAnisotropy, m.specularRotation);
+                                   : roughness_sampling(base_roughness(ms), m.specularAnisotropy, m.specularRotation);
             const V3 h = sample_ggx_half(sn, du, sr);
             next = normalize(reflect(ray.d, h));
             const float rc = clampf(dot(sn, next), 0.0f, 1.0f);
             if (rc <= 0.0f) break;
-            pdf = std::max((wr / wsum) * bsdf_pdf(m, n, cn, v, next), 1.0e-6f);
-            f = bsdf_eval(m, n, cn, v, next);
+            pdf = std::max((wr / wsum) * bsdf_pdf(ms, n, cn, v, next), 1.0e-6f);
+            f = bsdf_eval(me, n, cn, v, next);
             delta = sr <= kDeltaRoughness;
         } else {
             if (nDiffuse >= st.diffuseDepth) break;
@@ -1634,7 +1959,7 @@ PathResult trace_path(const Scene& s, const Settings& st, Ray ray, const Sampler
             }
             const float cpdf = clampf(dot(n, next), 0.0f, 1.0f) * kInvPi;
             pdf = std::max((wr / wsum) * (pCos * cpdf + pGuide * gpdf), 1.0e-6f);
-            f = bsdf_eval(m, n, cn, v, next);
+            f = bsdf_eval(me, n, cn, v, next);
             const float pc = clampf(dot(n, next), 0.0f, 1.0f);
             const V3 preview = clamp_max_component(T * f * (pc / pdf), kThroughputClamp);
             if (useGuide && upd) recs.push_back({gbin, preview});
@@ -1642,6 +1967,18 @@ PathResult trace_path(const Scene& s, const Settings& st, Ray ray, const Sampler
 
         const float cosT = pickT ? std::max(std::fabs(dot(ng, next)), 1.0e-4f) : clampf(dot(n, next), 0.0f, 1.0f);
         if (cosT <= 0.0f) break;
+        if (vx) {
+            vx->f = f;
+            vx->c = cosT / pdf;
+            if (!pickT && textured(m)) {
+                const BsdfGrad bg = bsdf_grad(me, n, cn, v, next);
+                vx->dfa = bg.dBase;
+                vx->dfr = bg.dRough;
+                vx->uv = uv;
+                vx->material = hit.material;
+                vx->flags = 2u;
+            }
+        }
         T = clamp_max_component(T * f * (cosT / pdf), kThroughputClamp);
         const V3 rn = pickT ? ng : n;
         const float side = dot(rn, next) >= 0.0f ? 1.0f : -1.0f;
@@ -1654,6 +1991,31 @@ PathResult trace_path(const Scene& s, const Settings& st, Ray ray, const Sampler
             const float q = clampf(max3(T), 0.05f, 0.95f);
             if (smp.next1(b0 + 4u) > q) break;
             T = T / q;
+            if (vx) vx->flags |= 1u;
+        }
+        if (vx) vx->kind = 0;
+    }
+    if (adj) {
+        V3 g{};
+        for (size_t k = adj->tape.size(); k-- > 0;) {
+            const AdjVtx& V = adj->tape[k];
+            V3 gT{};
+            if (V.kind == 0) {
+                const V3 y = V.T * V.f * V.c;
+                V3 gy = g;
+                if (V.flags & 1u) gy = roulette_vjp(clamp_max_component(y, kThroughputClamp), gy);
+                gy = clamp_max_vjp(y, kThroughputClamp, gy);
+                gT = gy * V.f * V.c;
+                if (V.flags & 2u) {
+                    const V3 gf = gy * V.T * V.c;
+                    const Material& mm = V.material < s.materials.size() ? s.materials[V.material] : default_material();
+                    scatter_grad(s, *adj, mm, V.uv, gf * V.dfa, dot(gf, V.dfr));
+                }
+            } else if (V.kind == 2) {
+                gT = g;
+            }
+            gT = gT + adj->a * V.C;
+            g = gT * V.att;
         }
     }
     return res;
@@ -1778,6 +2140,15 @@ void* oracle_scene_create(const shiro_hip_scene* in, int useBvh) {
         }
         s->domes.push_back(dm);
     }
+    for (uint32_t i = 0; i < in->textureCount; ++i) {
+        const shiro_hip_texture& t = in->textures[i];
+        Tex tx;
+        tx.w = t.width;
+        tx.h = t.height;
+        tx.ch = t.channels;
+        tx.eval.assign(t.texels, t.texels + static_cast<size_t>(t.width) * t.height * t.channels);
+        s->textures.push_back(std::move(tx));
+    }
     s->meshes.resize(in->meshCount);
     for (uint32_t mi = 0; mi < in->meshCount; ++mi) {
         const shiro_hip_mesh& src = in->meshes[mi];
@@ -1790,6 +2161,11 @@ void* oracle_scene_create(const shiro_hip_scene* in, int useBvh) {
             for (uint32_t v = 0; v < src.normalCount; ++v) m.nrm[v] = v3(src.normals + 3 * v);
         }
         m.idx.assign(src.indices, src.indices + src.indexCount);
+        if (src.uvs) {
+            m.uv.resize(src.vertexCount);
+            for (uint32_t v = 0; v < src.vertexCount; ++v) m.uv[v] = {src.uvs[2 * v], src.uvs[2 * v + 1]};
+            s->anyUv = true;
+        }
         for (uint32_t p = 0; 3u * p + 2u < src.indexCount; ++p) {
             s->refMesh.push_back(mi);
             s->refPrim.push_back(p);
@@ -2099,5 +2475,92 @@ int oracle_env_eval(void* h, uint32_t dome, const float* dir, uint32_t n, float*
 }
 
 uint32_t oracle_bvh_node_count(void* h) { return static_cast<uint32_t>(static_cast<Scene*>(h)->bvh.size()); }
+
+// ---- Texture extension and adjoint (no reference counterpart) ----
+
+// Replaces texture `index`'s texels; evalOnly keeps the texels sampling reads
+// (the frozen-sampling render the adjoint differentiates).
+int oracle_update_texture(void* h, uint32_t index, const float* texels, int evalOnly) {
+    Scene& s = *static_cast<Scene*>(h);
+    if (index >= s.textures.size()) return 1;
+    Tex& t = s.textures[index];
+    if (evalOnly) {
+        if (!t.frozen) t.sampling = t.eval;
+        t.frozen = true;
+    } else {
+        t.frozen = false;
+        t.sampling.clear();
+    }
+    std::copy(texels, texels + t.eval.size(), t.eval.begin());
+    return 0;
+}
+
+// Adds to grads[i] (size of texture i; null = skip) the gradient of
+// sum_{pixel, c} dBeauty[pixel*4 + c] * beauty[pixel][c] over samples
+// [sampleStart, sampleStart + sampleCount), beauty being the batch mean.
+// Row bands over `threads` threads, each with its own gradient buffers,
+// summed in thread order at the end.
+int oracle_render_adjoint(void* h, const shiro_hip_camera* camera, const shiro_hip_settings* settings,
+                          uint32_t sampleStart, uint32_t sampleCount, const float* dBeauty, float* const* grads,
+                          uint32_t gradCount, uint32_t threads) {
+    const Scene& s = *static_cast<Scene*>(h);
+    const Settings st = to_settings(*settings);
+    const Camera cam = to_camera(*camera);
+    if (sampleCount == 0) return 0;
+    const uint32_t n = std::max(1u, threads);
+    const uint32_t rows = std::max(1u, (st.height + n - 1) / n);
+    std::vector<std::vector<std::vector<float>>> local(n);
+    for (auto& L : local) {
+        L.resize(s.textures.size());
+        for (uint32_t i = 0; i < gradCount && i < s.textures.size(); ++i)
+            if (grads[i]) L[i].assign(s.textures[i].eval.size(), 0.0f);
+    }
+    std::vector<std::thread> pool;
+    for (uint32_t t = 0; t < n; ++t) {
+        const uint32_t y0 = t * rows;
+        if (y0 >= st.height) break;
+        const uint32_t y1 = std::min(st.height, y0 + rows);
+        pool.emplace_back([&, t, y0, y1]() {
+            Adjoint adj;
+            adj.grads = &local[t];
+            const float inv = 1.0f / static_cast<float>(sampleCount);
+            for (uint32_t y = y0; y < y1; ++y)
+                for (uint32_t x = 0; x < st.width; ++x) {
+                    const float* db = dBeauty + (static_cast<size_t>(y) * st.width + x) * 4;
+                    adj.a = {db[0] * inv, db[1] * inv, db[2] * inv};
+                    if (adj.a.x == 0.0f && adj.a.y == 0.0f && adj.a.z == 0.0f) continue;
+                    for (uint32_t i = 0; i < sampleCount; ++i) {
+                        const Sampler smp{x, y, sampleStart + i};
+                        const Ray r = camera_ray(cam, st, x, y, smp.next2(0));
+                        adj.tape.clear();
+                        adj.T = {1.0f, 1.0f, 1.0f};
+                        trace_path(s, st, r, smp, nullptr, nullptr, &adj);
+                    }
+                }
+        });
+    }
+    for (auto& th : pool) th.join();
+    for (uint32_t i = 0; i < gradCount && i < s.textures.size(); ++i) {
+        if (!grads[i]) continue;
+        for (const auto& L : local)
+            for (size_t k = 0; k < L[i].size(); ++k) grads[i][k] += L[i][k];
+    }
+    return 0;
+}
+
+// bsdf_grad for `count` tuples (layout as oracle_eval_bsdf); dBase and
+// dRough receive 3 floats per tuple.
+void oracle_bsdf_grad(const shiro_hip_material* mats, const uint32_t* matIndex, const float* n, const float* cn,
+                      const float* v, const float* l, uint32_t count, float* dBase, float* dRough) {
+    for (uint32_t i = 0; i < count; ++i) {
+        const BsdfGrad g = bsdf_grad(mats[matIndex[i]], v3(n + 3 * i), v3(cn + 3 * i), v3(v + 3 * i), v3(l + 3 * i));
+        dBase[3 * i] = g.dBase.x;
+        dBase[3 * i + 1] = g.dBase.y;
+        dBase[3 * i + 2] = g.dBase.z;
+        dRough[3 * i] = g.dRough.x;
+        dRough[3 * i + 1] = g.dRough.y;
+        dRough[3 * i + 2] = g.dRough.z;
+    }
+}
 
 }  // extern "C"

@@ -152,6 +152,7 @@ RESULT_NAMES = {
 }
 
 ENV_AUTOMATIC, ENV_LATLONG, ENV_ANGULAR = 0, 1, 2
+TEXELS_DEVICE, TEXELS_EVAL_ONLY = 1, 2
 TRANSPORT_MEGAKERNEL, TRANSPORT_WAVEFRONT = 0, 1
 
 # Every symbol include/shiro_hip.h declares (checked by the ABI test).
@@ -167,6 +168,7 @@ EXPORTS = [
     "shiro_hip_env_sample", "shiro_hip_env_eval", "shiro_hip_count_fetches",
     "shiro_hip_set_tuning", "shiro_hip_get_tuning", "shiro_hip_autotune",
     "shiro_hip_set_stage_timing", "shiro_hip_get_stage_times",
+    "shiro_hip_update_texture", "shiro_hip_render_adjoint",
 ]
 
 

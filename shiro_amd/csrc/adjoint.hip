@@ -1,0 +1,239 @@
+// Path-replay adjoint of the render with respect to texture texels (texture
+// extension; no counterpart in the reference).
+//
+// The forward estimate of one sample is L = sum_j T_j * C_j, where C_j is
+// what vertex j adds per unit of throughput (emission, the clamped and
+// averaged light samples of DirectLighting, or the environment on a miss)
+// and T_{j+1} = RR(clamp_max_component(T_j * f_j * c_j, 12)) with the
+// roulette division RR and c_j = cos / pdf.  Sampling is detached: lobe
+// choices, directions, pdfs and MIS weights are constants (they read the
+// sampling copy of the texels), so the derivative is that of the
+// frozen-sampling render, whose finite differences it reproduces.
+//
+// One thread replays one (pixel, sample) path with the forward code
+// (intersect_closest, shade_hit<true>, direct_sample, occluded,
+// sample_bounce: the same decisions as the textured render), records each
+// vertex (throughput, attenuation, C_j, bounce factors, d f_j / d theta at
+// the sampled direction), adds the light-sample gradients on the way, then
+// walks the vertices backwards propagating dLoss/dT_j through the clamp and
+// the roulette division.  Texel gradients are scattered with atomics through
+// the bilinear footprint of each hit.
+#include "kernels.h"
+
+#include "dev_path.h"
+
+namespace shiro_hip {
+
+namespace {
+
+enum VtxKind : uint32_t { kVtxBounce = 0, kVtxEnd = 1, kVtxPass = 2, kVtxMiss = 3 };
+
+struct Vtx {
+    V3 T;          // throughput at the vertex after medium attenuation and opacity
+    V3 att;        // T / throughput on arrival
+    V3 C;          // radiance added per unit of T
+    V3 f;          // bounce: BSDF value
+    V3 dfa, dfr;   // bounce: d f / d base colour (per channel), d f / d roughness
+    V2 uv;
+    float c;       // bounce: cos / pdf
+    uint32_t material;
+    uint32_t kind;
+    uint32_t flags;  // 1: roulette ran, 2: textured bounce gradient recorded
+};
+
+// Vector-Jacobian product of clamp_max_component(v, lim).
+DEV V3 clamp_max_vjp(V3 v, float lim, V3 g) {
+    const float c = max3(v);
+    if (c <= lim || c <= 0.0f) return g;
+    const float s = lim / c;
+    V3 r = g * s;
+    const float corr = dot(g, v) * lim / (c * c);
+    // max3 returns the first of equal maxima in x, y, z order.
+    if (!(v.x < fmaxs(v.y, v.z))) {
+        r.x -= corr;
+    } else if (!(v.y < v.z)) {
+        r.y -= corr;
+    } else {
+        r.z -= corr;
+    }
+    return r;
+}
+
+// Vector-Jacobian product of y / clamp(max3(y), 0.05, 0.95).
+DEV V3 roulette_vjp(V3 y, V3 g) {
+    const float m = max3(y);
+    const float q = clampf(m, 0.05f, 0.95f);
+    V3 r = g / q;
+    if (!(m < 0.05f) && !(0.95f < m)) {
+        const float corr = dot(g, y) / (q * q);
+        if (!(y.x < fmaxs(y.y, y.z))) {
+            r.x -= corr;
+        } else if (!(y.y < y.z)) {
+            r.y -= corr;
+        } else {
+            r.z -= corr;
+        }
+    }
+    return r;
+}
+
+DEV void scatter(float* grad, const DevTexture& t, V2 uv, const float* g, uint32_t n) {
+    const TexTap tap = tex_tap(t.w, t.h, uv);
+    for (int k = 0; k < 4; ++k) {
+        if (tap.w[k] == 0.0f) continue;
+        for (uint32_t c = 0; c < n; ++c)
+            if (g[c] != 0.0f) atomicAdd(grad + static_cast<size_t>(tap.i[k]) * t.channels + c, g[c] * tap.w[k]);
+    }
+}
+
+// Adds dLoss/dbaseColor and dLoss/droughness at a hit to the texel
+// gradients of the material's textures.
+DEV void scatter_grad(const AdjointParams& P, uint32_t material, V2 uv, V3 gBase, float gRough) {
+    const DevScene& S = P.scene;
+    const shiro_hip_material& m = S.materials[material];
+    if (m.baseColorTexture >= 0 && P.grad[m.baseColorTexture]) {
+        const DevTexture& t = S.textures[m.baseColorTexture];
+        if (t.channels >= 3u) {
+            const float g[3] = {gBase.x, gBase.y, gBase.z};
+            scatter(P.grad[m.baseColorTexture], t, uv, g, 3u);
+        } else {
+            const float g[1] = {gBase.x + gBase.y + gBase.z};
+            scatter(P.grad[m.baseColorTexture], t, uv, g, 1u);
+        }
+    }
+    if (m.roughnessTexture >= 0 && P.grad[m.roughnessTexture]) {
+        const float g[1] = {gRough};
+        scatter(P.grad[m.roughnessTexture], S.textures[m.roughnessTexture], uv, g, 1u);
+    }
+}
+
+DEV V3 ratio(V3 a, V3 b) {
+    return V3{b.x != 0.0f ? a.x / b.x : 0.0f, b.y != 0.0f ? a.y / b.y : 0.0f, b.z != 0.0f ? a.z / b.z : 0.0f};
+}
+
+}  // namespace
+
+// Grid: one 64-thread block per (8x8 tile, sample); tile-major within a
+// sample.
+__global__ __launch_bounds__(64) void k_adjoint(AdjointParams P) {
+    const uint32_t tile = blockIdx.x % P.tiles;
+    const uint32_t s = blockIdx.x / P.tiles;
+    const uint32_t x = (tile % P.tilesX) * 8u + (threadIdx.x & 7u);
+    const uint32_t y = (tile / P.tilesX) * 8u + (threadIdx.x >> 3);
+    const DevSettings& st = P.settings;
+    const DevScene& S = P.scene;
+    if (x >= st.width || y >= st.height || s >= P.sampleCount) return;
+    const float4 db = P.dBeauty[static_cast<size_t>(y) * st.width + x];
+    const float inv = 1.0f / static_cast<float>(P.sampleCount);
+    const V3 a{db.x * inv, db.y * inv, db.z * inv};
+    if (a.x == 0.0f && a.y == 0.0f && a.z == 0.0f) return;
+    const Sampler smp{x, y, P.sampleStart + s};
+    PathState ps = init_path(P.camera, st, x, y, smp);
+    const uint32_t nLight = direct_sample_count(S, st);
+    Counters<false> cnt;
+    Vtx tape[kAdjointMaxDepth];
+    uint32_t nv = 0;
+    for (uint32_t depth = 0; depth < st.maxDepth; ++depth) {
+        const Ray ray = path_ray(ps);
+        const HitRec h = intersect_closest(S, ray, cnt.trav());
+        Vtx& V = tape[nv++];
+        V.flags = 0u;
+        if (h.slot == kNoHit) {
+            V.kind = kVtxMiss;
+            V.T = ps.T;
+            V.att = V3{1.0f, 1.0f, 1.0f};
+            handle_miss(S, st, depth, ps, cnt, &V.C);
+            break;
+        }
+        const V3 Tin = ps.T;
+        Surface sf;
+        TexState ts;
+        const bool lit = shade_hit<true>(S, depth, smp, ray, h, ps, sf, &ts);
+        V.T = ps.T;
+        V.att = ratio(ps.T, Tin);
+        V.C = V3{0.0f, 0.0f, 0.0f};
+        if (!lit) {
+            V.kind = kVtxPass;
+            continue;
+        }
+        const Mat& M = sf.M;
+        const bool textured = (S.derived[sf.material].flags & kMatTextured) != 0u;
+        const uint32_t prevType = ps.bits & kBitsTypeMask;
+        if ((M.d->flags & kMatEmissive) && (depth == 0 || (ps.bits & kBitsPrevDelta) || prevType == kBounceTransmission))
+            V.C = ld3(M.d->emission);
+        // Light samples: forward sum and the gradient of each visible one.
+        DirectSum ds;
+        ds.init(st);
+        const V3 origin = sf.p + sf.n * kRayEps;
+        V3 gBase{0.0f, 0.0f, 0.0f};
+        float gRough = 0.0f;
+        const V3 aT = a * ps.T;
+        for (uint32_t j = 0; j < nLight; ++j) {
+            V3 ld, contrib, f, lf;
+            uint32_t group;
+            if (!direct_sample(S, st, sf, M, smp, depth, j, &ld, &contrib, &group, cnt, &f, &lf)) continue;
+            const bool visible = !occluded(S, origin, ld, kFltMax, cnt);
+            ds.add(group, contrib, visible);
+            if (visible && textured) {
+                const V3 gc = group == 0u ? aT : aT / ds.K;
+                const V3 gf = clamp_max_vjp(f * lf, kDirectClamp, gc) * lf;
+                const BsdfGrad bg = bsdf_grad(M, sf.n, sf.cn, sf.v, ld);
+                gBase = gBase + gf * bg.dBase;
+                gRough += dot(gf, bg.dRough);
+            }
+        }
+        ds.flush();
+        ps.L = ps.L + ps.T * ds.L;
+        V.C = V.C + ds.L;
+        if (textured) scatter_grad(P, sf.material, sf.uv, gBase, gRough);
+        BounceRec br;
+        const bool alive = sample_bounce(S, st, depth, smp, sf, ps, &br);
+        if (!alive) {
+            V.kind = kVtxEnd;
+            break;
+        }
+        V.kind = kVtxBounce;
+        V.f = br.f;
+        V.c = br.c;
+        V.flags = br.rr ? 1u : 0u;
+        if (textured && br.reflective) {
+            const BsdfGrad bg = bsdf_grad(M, sf.n, sf.cn, sf.v, br.next);
+            V.dfa = bg.dBase;
+            V.dfr = bg.dRough;
+            V.uv = sf.uv;
+            V.material = sf.material;
+            V.flags |= 2u;
+        }
+    }
+    // Reverse sweep: g = dLoss / dT of the vertex after the current one.
+    V3 g{0.0f, 0.0f, 0.0f};
+    for (uint32_t k = nv; k-- > 0;) {
+        const Vtx& V = tape[k];
+        V3 gT{0.0f, 0.0f, 0.0f};
+        if (V.kind == kVtxBounce) {
+            const V3 y = V.T * V.f * V.c;
+            V3 gy = g;
+            if (V.flags & 1u) gy = roulette_vjp(clamp_max_component(y, kThroughputClamp), gy);
+            gy = clamp_max_vjp(y, kThroughputClamp, gy);
+            gT = gy * V.f * V.c;
+            if (V.flags & 2u) {
+                const V3 gf = gy * V.T * V.c;
+                scatter_grad(P, V.material, V.uv, gf * V.dfa, dot(gf, V.dfr));
+            }
+        } else if (V.kind == kVtxPass) {
+            gT = g;
+        }
+        gT = gT + a * V.C;
+        g = gT * V.att;
+    }
+}
+
+hipError_t launch_adjoint(const AdjointParams& P, hipStream_t s) {
+    const uint64_t blocks = static_cast<uint64_t>(P.tiles) * P.sampleCount;
+    if (blocks == 0) return hipSuccess;
+    if (blocks > 0x7fffffffull) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_adjoint, dim3(static_cast<uint32_t>(blocks)), dim3(64), 0, s, P);
+    return hipGetLastError();
+}
+
+}  // namespace shiro_hip

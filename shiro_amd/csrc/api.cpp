@@ -15,6 +15,7 @@
 #include <cstdio>
 #include <cstring>
 #include <limits>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -63,6 +64,15 @@ struct EnvBuffers {
     DevBuf texels, row, cond;
 };
 
+// Texture extension: the texels, and a second copy that sampling reads
+// while frozen (shiro_hip_update_texture with EVAL_ONLY).
+struct TexBuffers {
+    DevBuf eval, sampling;
+    uint32_t w = 0, h = 0, channels = 0;
+    bool frozen = false;
+    size_t count() const { return static_cast<size_t>(w) * h * channels; }
+};
+
 }  // namespace
 
 struct shiro_hip_ctx {
@@ -75,8 +85,11 @@ struct shiro_hip_ctx {
     // Scene.
     bool hasScene = false;
     bool emissive = false;
-    DevBuf nodes, tris, hitdata, materials, derived;
+    DevBuf nodes, tris, hitdata, materials, derived, uvs;
     EnvBuffers env[kMaxDomes];
+    std::vector<std::unique_ptr<TexBuffers>> tex;
+    DevBuf texTable;
+    std::vector<shiro_hip_material> mats;   // host copy of the uploaded materials
     DevScene dev{};
     shiro_hip_scene_info info{};
     // Per-batch scratch.
@@ -492,6 +505,7 @@ int shiro_hip_upload_scene(shiro_hip_ctx* ctx, const shiro_hip_scene* s) {
     // Materials (+ the default material used for out-of-range indices).
     std::vector<shiro_hip_material> mats(s->materials, s->materials + s->materialCount);
     mats.push_back(default_material());
+    ctx->mats = mats;
     std::vector<MaterialDerived> der(mats.size());
     for (size_t i = 0; i < mats.size(); ++i) der[i] = derive_material(mats[i]);
     HIPC(upload(ctx->nodes, bvh.nodes.data(), bvh.nodes.size(), ctx->stream), "upload nodes");
@@ -499,7 +513,30 @@ int shiro_hip_upload_scene(shiro_hip_ctx* ctx, const shiro_hip_scene* s) {
     HIPC(upload(ctx->hitdata, hit.data(), hit.size(), ctx->stream), "upload hit data");
     HIPC(upload(ctx->materials, mats.data(), mats.size(), ctx->stream), "upload materials");
     HIPC(upload(ctx->derived, der.data(), der.size(), ctx->stream), "upload materials");
+    if (flat.hasUvs) {
+        std::vector<float> uv(nt * 6);
+        for (size_t slot = 0; slot < nt; ++slot)
+            std::memcpy(&uv[slot * 6], &flat.uv[static_cast<size_t>(bvh.order[slot]) * 6], 24);
+        HIPC(upload(ctx->uvs, uv.data(), uv.size(), ctx->stream), "upload uvs");
+    }
+    ctx->tex.clear();
+    for (uint32_t i = 0; i < s->textureCount; ++i) ctx->tex.push_back(std::make_unique<TexBuffers>());
+    std::vector<DevTexture> table(s->textureCount);
+    for (uint32_t i = 0; i < s->textureCount; ++i) {
+        const shiro_hip_texture& t = s->textures[i];
+        TexBuffers& b = *ctx->tex[i];
+        b.w = t.width;
+        b.h = t.height;
+        b.channels = t.channels;
+        HIPC(upload(b.eval, t.texels, b.count(), ctx->stream), "upload texture");
+        table[i] = DevTexture{b.eval.as<float>(), b.eval.as<float>(), b.w, b.h, b.channels, 0u};
+    }
+    if (!table.empty()) HIPC(upload(ctx->texTable, table.data(), table.size(), ctx->stream), "upload textures");
     DevScene d{};
+    d.uvs = flat.hasUvs ? ctx->uvs.as<float2>() : nullptr;
+    d.textures = table.empty() ? nullptr : ctx->texTable.as<DevTexture>();
+    d.textureCount = s->textureCount;
+    d.textured = flat.textured ? 1u : 0u;
     d.nodes = ctx->nodes.as<float4>();
     d.tris = ctx->tris.as<float4>();
     d.hitdata = ctx->hitdata.as<float4>();
@@ -509,6 +546,8 @@ int shiro_hip_upload_scene(shiro_hip_ctx* ctx, const shiro_hip_scene* s) {
     d.triCount = static_cast<uint32_t>(nt);
     d.materialCount = static_cast<uint32_t>(mats.size());
     uint64_t bytes = ctx->nodes.bytes + ctx->tris.bytes + ctx->hitdata.bytes + ctx->materials.bytes + ctx->derived.bytes;
+    if (flat.hasUvs) bytes += ctx->uvs.bytes;
+    for (const auto& b : ctx->tex) bytes += b->eval.bytes;
     // Lights.
     d.dirCount = s->distantLightCount;
     for (uint32_t i = 0; i < s->distantLightCount; ++i) {
@@ -567,6 +606,34 @@ int shiro_hip_upload_scene(shiro_hip_ctx* ctx, const shiro_hip_scene* s) {
     ctx->info.buildSeconds = std::chrono::duration<double>(t1 - t0).count();
     ctx->info.uploadSeconds = std::chrono::duration<double>(t2 - t1).count();
     ctx->hasScene = true;
+    return SHIRO_HIP_OK;
+}
+
+int shiro_hip_update_texture(shiro_hip_ctx* ctx, uint32_t index, const float* texels, uint32_t flags) {
+    if (!ctx) return SHIRO_HIP_INVALID;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    if (!ctx->hasScene) return fail(ctx, SHIRO_HIP_NO_SCENE, "no scene uploaded");
+    if (index >= ctx->tex.size() || !texels) return fail(ctx, SHIRO_HIP_INVALID, "bad texture index or null texels");
+    HIPC(hipSetDevice(ctx->device), "set device");
+    TexBuffers& b = *ctx->tex[index];
+    const size_t bytes = b.count() * sizeof(float);
+    const hipMemcpyKind kind = (flags & SHIRO_HIP_TEXELS_DEVICE) ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+    const bool freeze = (flags & SHIRO_HIP_TEXELS_EVAL_ONLY) != 0;
+    if (freeze && !b.frozen) {
+        // Keep the current texels for sampling.
+        HIPC(b.sampling.reserve(bytes), "alloc texture");
+        HIPC(hipMemcpyAsync(b.sampling.p, b.eval.p, bytes, hipMemcpyDeviceToDevice, ctx->stream), "freeze texture");
+    }
+    HIPC(hipMemcpyAsync(b.eval.p, texels, bytes, kind, ctx->stream), "update texture");
+    if (freeze != b.frozen) {
+        b.frozen = freeze;
+        const DevTexture entry{b.eval.as<float>(), freeze ? b.sampling.as<float>() : b.eval.as<float>(), b.w, b.h,
+                               b.channels, 0u};
+        HIPC(hipMemcpyAsync(ctx->texTable.as<DevTexture>() + index, &entry, sizeof(entry), hipMemcpyHostToDevice,
+                            ctx->stream),
+             "update texture");
+    }
+    HIPC(hipStreamSynchronize(ctx->stream), "update texture");
     return SHIRO_HIP_OK;
 }
 
@@ -974,6 +1041,49 @@ int shiro_hip_get_stage_times(shiro_hip_ctx* ctx, shiro_hip_stage_times* out) {
     std::lock_guard<std::mutex> lock(ctx->mu);
     *out = ctx->stageTimes;
     return SHIRO_HIP_OK;
+}
+
+int shiro_hip_render_adjoint(shiro_hip_ctx* ctx, const shiro_hip_camera* camera, const shiro_hip_settings* settings,
+                             uint32_t sampleStart, uint32_t sampleCount, const float* dBeauty, float* const* dTexels,
+                             uint32_t dTexelsCount) {
+    if (!ctx) return SHIRO_HIP_INVALID;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    if (!camera || !dBeauty) return fail(ctx, SHIRO_HIP_INVALID, "null camera or adjoint image");
+    int rc = check_settings(ctx, settings);
+    if (rc) return rc;
+    if (!ctx->hasScene) return fail(ctx, SHIRO_HIP_NO_SCENE, "no scene uploaded");
+    if (settings->maxDepth > kAdjointMaxDepth)
+        return fail(ctx, SHIRO_HIP_INVALID, "adjoint supports maxDepth <= " + std::to_string(kAdjointMaxDepth));
+    if (dTexelsCount > ctx->tex.size() || dTexelsCount > kAdjointMaxTextures || (dTexelsCount > 0 && !dTexels))
+        return fail(ctx, SHIRO_HIP_INVALID, "more texel gradients than textures");
+    if (ctx->tex.size() > kAdjointMaxTextures)
+        return fail(ctx, SHIRO_HIP_INVALID, "adjoint supports at most " + std::to_string(kAdjointMaxTextures) +
+                                                " textures");
+    for (size_t i = 0; i < ctx->mats.size(); ++i) {
+        const shiro_hip_material& m = ctx->mats[i];
+        if (m.baseColorTexture < 0 && m.roughnessTexture < 0) continue;
+        if (m.specularAnisotropy != 0.0f || m.thinFilmThickness > 0.0f || m.sheen > 0.0f || m.transmission > 0.0f)
+            return fail(ctx, SHIRO_HIP_INVALID,
+                        "material " + std::to_string(i) +
+                            " is textured but outside the differentiable class (anisotropy, thin film, sheen or "
+                            "transmission)");
+    }
+    if (sampleCount == 0) return SHIRO_HIP_OK;
+    HIPC(hipSetDevice(ctx->device), "set device");
+    AdjointParams P{};
+    P.scene = ctx->dev;
+    P.camera = to_dev(*camera);
+    P.settings = to_dev(*settings);
+    P.dBeauty = reinterpret_cast<const float4*>(dBeauty);
+    for (uint32_t i = 0; i < dTexelsCount; ++i) P.grad[i] = dTexels[i];
+    P.sampleStart = sampleStart;
+    P.sampleCount = sampleCount;
+    P.tilesX = (settings->width + 7u) / 8u;
+    P.tiles = P.tilesX * ((settings->height + 7u) / 8u);
+    HIPC(hipEventRecord(ctx->ev0, ctx->stream), "event");
+    HIPC(launch_adjoint(P, ctx->stream), "adjoint launch");
+    HIPC(hipEventRecord(ctx->ev1, ctx->stream), "event");
+    return finish_timing(ctx);
 }
 
 }  // extern "C"

@@ -13,9 +13,14 @@
 
 namespace shiro_hip {
 
+// A material at a hit: its parameters, the derived constants shading
+// evaluates (d) and those sampling decisions and pdfs read (ds).  d and ds
+// are the same table entry except at hits on textured materials whose
+// sampling texels were frozen (layout.h DevTexture).
 struct Mat {
     const shiro_hip_material* m;
     const MaterialDerived* d;
+    const MaterialDerived* ds;
 };
 
 DEV V3 tangent_of(V3 n) {
@@ -161,11 +166,11 @@ DEV float bsdf_pdf(const Mat& M, V3 n, V3 cn, V3 v, V3 l) {
     const float nl = clampf(dot(n, l), 0.0f, 1.0f);
     if (nl <= 0.0f) return 0.0f;
     const shiro_hip_material& m = *M.m;
-    const float wd = M.d->wDiffuse, ws = M.d->wBaseSpecular, wc = M.d->wCoat;
+    const float wd = M.ds->wDiffuse, ws = M.ds->wBaseSpecular, wc = M.ds->wCoat;
     const float wr = wd + ws + wc;
     if (wr <= 0.0f) return 0.0f;
     const float pd = wd / wr, ps = ws / wr, pc = wc / wr;
-    const float rs = roughness_eval(M.d->baseRoughness, m.specularAnisotropy, m.specularRotation, m, n, v, l);
+    const float rs = roughness_eval(M.ds->baseRoughness, m.specularAnisotropy, m.specularRotation, m, n, v, l);
     float coatTerm = 0.0f;
     if (pc != 0.0f) {
         const float rc = roughness_eval(m.coatRoughness, m.coatAnisotropy, m.coatRotation, m, cn, v, l);
@@ -216,6 +221,105 @@ DEV V3 bsdf_eval(const Mat& M, V3 n, V3 cn, V3 v, V3 l) {
         out = out + ld3(m.sheenColor) * sheenW * sheenF * underCoat * underBase;
     }
     return out;
+}
+
+// Derivatives of bsdf_eval at fixed directions with respect to the base
+// colour (per channel: d f_c / d baseColor_c) and the roughness
+// (d f_c / d roughness), for the differentiable class of the texture
+// extension: no specular anisotropy, thin film, sheen or transmission
+// (shiro_hip_render_adjoint rejects other textured materials).  The base
+// colour enters through the diffuse colour and, for metals, the base F0 (in
+// the layer transmittance and the Fresnel term); the roughness through the
+// GGX D and the height-correlated G2.  Clamps contribute no derivative
+// where they are active.
+struct BsdfGrad {
+    V3 dBase, dRough;
+};
+
+DEV float schlick_weight(float c) {
+    const float x = clampf(1.0f - c, 0.0f, 1.0f);
+    return x * x * x * x * x;
+}
+
+// d Lambda(c) / d a2 of smith_lambda.
+DEV float smith_lambda_da2(float c, float a2) {
+    const float cc = fmaxs(c, 1.0e-6f);
+    const float t = (1.0f - cc * cc) / (cc * cc);
+    return t / (4.0f * sqrtf(1.0f + a2 * t));
+}
+
+DEV BsdfGrad bsdf_grad(const Mat& M, V3 n, V3 cn, V3 v, V3 l) {
+    BsdfGrad g{V3{0.0f, 0.0f, 0.0f}, V3{0.0f, 0.0f, 0.0f}};
+    const float nv = clampf(dot(n, v), 0.0f, 1.0f);
+    const float nl = clampf(dot(n, l), 0.0f, 1.0f);
+    if (nv <= 0.0f || nl <= 0.0f) return g;
+    const shiro_hip_material& m = *M.m;
+    const MaterialDerived& d = *M.d;
+    const V3 f0b = ld3(d.specularF0);
+    const V3 uC = ld3(d.coatTint) * layer_transmittance(ld3(d.coatF0), dot(cn, v), dot(cn, l));
+    // Layer transmittance of the base and its derivative in F0.
+    const float wv = schlick_weight(nv), wl = schlick_weight(nl);
+    const V3 Fv = schlick(f0b, nv), Fl = schlick(f0b, nl);
+    const V3 sumF = (Fv + Fl) * 0.5f;
+    const V3 uB = one_minus_clamped(sumF);
+    const float dUB = -0.5f * ((1.0f - wv) + (1.0f - wl));
+    const V3 dUBdF0{sumF.x > 0.0f && sumF.x < 1.0f ? dUB : 0.0f, sumF.y > 0.0f && sumF.y < 1.0f ? dUB : 0.0f,
+                    sumF.z > 0.0f && sumF.z < 1.0f ? dUB : 0.0f};
+    // Diffuse factor (dterm / pi) as in bsdf_eval.
+    const V3 vp = normalize(v - n * nv);
+    const V3 lp = normalize(l - n * nl);
+    const float cosPhi = len(vp) > 0.0f && len(lp) > 0.0f ? clampf(dot(vp, lp), -1.0f, 1.0f) : 0.0f;
+    const float sg = clampf(m.diffuseRoughness, 0.0f, 1.0f);
+    const float sg2 = sg * sg;
+    const float A = 1.0f - (sg2 / (2.0f * (sg2 + 0.33f)));
+    const float B = 0.45f * sg2 / (sg2 + 0.09f);
+    const float sv = sqrtf(fmaxs(0.0f, 1.0f - nv * nv));
+    const float sl = sqrtf(fmaxs(0.0f, 1.0f - nl * nl));
+    const float tanBeta = fmins(sv / fmaxs(nv, 1.0e-6f), sl / fmaxs(nl, 1.0e-6f));
+    const float wrap = clampf(m.subsurface * m.subsurfaceScale *
+                                  (0.35f + 0.15f * clampf(m.subsurfaceAnisotropy, -1.0f, 1.0f) +
+                                   0.1f * clampf(max3(ld3(m.subsurfaceRadius)) / 3.0f, 0.0f, 1.0f)),
+                              0.0f, 0.9f);
+    const float wnl = clampf((nl + wrap) / (1.0f + wrap), 0.0f, 1.0f);
+    const float kd = kInvPi * (A + B * fmaxs(0.0f, cosPhi) * fmaxs(sv, sl) * tanBeta) * (wnl / fmaxs(nl, 1.0e-6f));
+    const float metal = clampf(m.metallic, 0.0f, 1.0f);
+    const float trans = clampf(m.transmission, 0.0f, 1.0f);
+    const float sss = clampf(m.subsurface, 0.0f, 1.0f);
+    const float blend = clampf(max3(ld3(m.subsurfaceRadius)) / 3.0f, 0.0f, 1.0f);
+    const float kDiff = ((1.0f - sss) + sss * (1.0f - blend)) *
+                        (clampf(m.baseWeight, 0.0f, 1.0f) * (1.0f - metal) * (1.0f - trans));
+    const V3 dc = ld3(d.diffuseColor);
+    // Specular lobe pieces.
+    const V3 h = safe_normalize(v + l, n);
+    const float nh = clampf(dot(n, h), 0.0f, 1.0f);
+    const float vh = clampf(dot(v, h), 0.0f, 1.0f);
+    const float bR = d.baseRoughness;
+    const float rs = clampf(bR, 0.02f, 1.0f);
+    const float a = fmaxs(0.02f, rs * rs);
+    const float a2 = a * a;
+    const float q = nh * nh * (a2 - 1.0f) + 1.0f;
+    const float den = fmaxs(1.0e-6f, q * q);
+    const float D = a2 / (kPi * den);
+    const float G = ggx_g2(nv, nl, rs);
+    const float den4 = fmaxs(4.0f * nv * nl, 1.0e-6f);
+    const float wh = schlick_weight(vh);
+    const V3 F = schlick(f0b, vh);
+    // d/d base.
+    const float dSpecdF0 = (1.0f - wh) * D * G / den4;
+    g.dBase = V3{kDiff * uC.x * uB.x * kd + dc.x * uC.x * dUBdF0.x * metal * kd + uC.x * dSpecdF0 * metal,
+                 kDiff * uC.y * uB.y * kd + dc.y * uC.y * dUBdF0.y * metal * kd + uC.y * dSpecdF0 * metal,
+                 kDiff * uC.z * uB.z * kd + dc.z * uC.z * dUBdF0.z * metal * kd + uC.z * dSpecdF0 * metal};
+    // d/d roughness: baseRoughness = clamp(r + coat term, 0, 1), rs = clamp(bR, 0.02, 1),
+    // a = max(0.02, rs^2), a2 = a^2.
+    if (bR > 0.02f && bR < 1.0f && rs * rs > 0.02f) {
+        const float dDda2 = q * q > 1.0e-6f ? (1.0f / (kPi * q * q)) * (1.0f - 2.0f * a2 * nh * nh / q)
+                                             : 1.0f / (kPi * 1.0e-6f);
+        const float dGda2 = -G * G * (smith_lambda_da2(nv, a2) + smith_lambda_da2(nl, a2));
+        const float da2drs = 2.0f * a * 2.0f * rs;
+        const float dS = (dDda2 * G + D * dGda2) * da2drs / den4;
+        g.dRough = uC * F * dS;
+    }
+    return g;
 }
 
 // Rough dielectric transmission (CpuPathTracer.cpp:1136-1182).

@@ -10,6 +10,7 @@
 #include "dev_bvh.h"
 #include "dev_env.h"
 #include "dev_sampler.h"
+#include "dev_texture.h"
 
 namespace shiro_hip {
 
@@ -182,20 +183,28 @@ DEV Ray path_ray(const PathState& ps) {
 }
 
 // Environment seen by a ray that left the scene (CpuPathTracer.cpp:1675-1693).
+// `factor` (optional) receives Le times the MIS weight: the radiance added
+// per unit of throughput.
 template <bool COUNT>
-DEV void handle_miss(const DevScene& S, const DevSettings& st, uint32_t depth, PathState& ps, Counters<COUNT>& c) {
+DEV void handle_miss(const DevScene& S, const DevSettings& st, uint32_t depth, PathState& ps, Counters<COUNT>& c,
+                     V3* factor = nullptr) {
     if (depth > 0 || st.backgroundVisible) {
         const V3 Le = environment_radiance(S, ps.d, c);
         const uint32_t prevType = ps.bits & kBitsTypeMask;
         V3 dL;
         if (S.domeCount == 0 || depth == 0 || (ps.bits & kBitsPrevDelta) || prevType == kBounceTransmission) {
             dL = ps.T * Le;
+            if (factor) *factor = Le;
         } else {
             const float sq2 = env_pdf_sq_sum(S, ps.d, c);
             const float b2 = ps.prevPdf * ps.prevPdf;
-            dL = ps.T * Le * (b2 / fmaxs(b2 + sq2, 1.0e-6f));
+            const float w = b2 / fmaxs(b2 + sq2, 1.0e-6f);
+            dL = ps.T * Le * w;
+            if (factor) *factor = Le * w;
         }
         ps.L = ps.L + dL;
+    } else if (factor) {
+        *factor = V3{0.0f, 0.0f, 0.0f};
     }
 }
 
@@ -204,21 +213,33 @@ struct Surface {
     V3 p, ng, n, cn, v;
     float t;
     uint32_t material;
+    V2 uv;            // interpolated UV (textured materials only)
+    Mat M;
 };
 
 // Everything of a bounce up to direct lighting (CpuPathTracer.cpp:1695-1748):
 // medium attenuation, normal orientation, opacity, first-hit AOVs, emission.
 // Returns false when the opacity test lets the ray pass through (the ray is
 // advanced and the bounce ends without lighting, as the reference's
-// `continue`).
+// `continue`).  TEX: the scene has textured materials; their per-hit
+// constants are derived into *ts, which sf.M then points to.
+template <bool TEX>
 DEV bool shade_hit(const DevScene& S, uint32_t depth, const Sampler& smp, const Ray& ray, const HitRec& h,
-                   PathState& ps, Surface& sf) {
+                   PathState& ps, Surface& sf, TexState* ts) {
     const SurfaceHit hit = fetch_hit(S, ray, h);
     if (ps.bits & kBitsMediumOn) {
         ps.T = ps.T * V3{expf(-ps.ext.x * h.t), expf(-ps.ext.y * h.t), expf(-ps.ext.z * h.t)};
     }
     const shiro_hip_material& m = S.materials[hit.material];
-    const MaterialDerived& md = S.derived[hit.material];
+    sf.M = Mat{&m, S.derived + hit.material, S.derived + hit.material};
+    sf.uv = V2{0.0f, 0.0f};
+    if (TEX && (S.derived[hit.material].flags & kMatTextured)) {
+        sf.uv = hit_uv(S, h.slot, h.b1, h.b2);
+        texture_material(S, m, S.derived[hit.material], sf.uv, *ts);
+        sf.M.d = &ts->eval;
+        sf.M.ds = &ts->samp;
+    }
+    const MaterialDerived& md = *sf.M.d;
     const bool front = dot(hit.ng, ray.d) < 0.0f;
     const V3 ng = front ? hit.ng : hit.ng * -1.0f;
     V3 nbase = front ? hit.ns : hit.ns * -1.0f;
@@ -270,16 +291,22 @@ DEV uint32_t direct_sample_count(const DevScene& S, const DevSettings& st) {
 // Light sample j of DirectLighting: the shadow ray (from p + n*eps) and the
 // contribution it carries if unoccluded.  group 0 = directional light,
 // 1 + i = dome i.  Returns false when the sample needs no shadow ray.
+// `fOut` / `lfOut` (optional) receive the BSDF value and the light factor,
+// contrib = clamp_max_component(f * lf, 24).
 template <bool COUNT>
 DEV bool direct_sample(const DevScene& S, const DevSettings& st, const Surface& sf, const Mat& M, const Sampler& smp,
-                       uint32_t depth, uint32_t j, V3* dirOut, V3* contrib, uint32_t* group, Counters<COUNT>& c) {
+                       uint32_t depth, uint32_t j, V3* dirOut, V3* contrib, uint32_t* group, Counters<COUNT>& c,
+                       V3* fOut = nullptr, V3* lfOut = nullptr) {
     if (j < S.dirCount) {
         const DevDirLight& dl = S.dirs[j];
         const V3 ld = normalize(ld3(dl.direction) * -1.0f);
         const float nl = clampf(dot(sf.n, ld), 0.0f, 1.0f);
         if (nl <= 0.0f) return false;
         *dirOut = ld;
-        *contrib = clamp_max_component(bsdf_eval(M, sf.n, sf.cn, sf.v, ld) * ld3(dl.radiance) * nl, kDirectClamp);
+        const V3 f = bsdf_eval(M, sf.n, sf.cn, sf.v, ld);
+        *contrib = clamp_max_component(f * ld3(dl.radiance) * nl, kDirectClamp);
+        if (fOut) *fOut = f;
+        if (lfOut) *lfOut = ld3(dl.radiance) * nl;
         *group = 0u;
         return true;
     }
@@ -295,7 +322,10 @@ DEV bool direct_sample(const DevScene& S, const DevSettings& st, const Surface& 
     const float bp = bsdf_pdf(M, sf.n, sf.cn, sf.v, ld);
     const float w = power_heuristic(lp, bp);
     *dirOut = ld;
-    *contrib = clamp_max_component(bsdf_eval(M, sf.n, sf.cn, sf.v, ld) * lr * (nl * w / lp), kDirectClamp);
+    const V3 f = bsdf_eval(M, sf.n, sf.cn, sf.v, ld);
+    *contrib = clamp_max_component(f * lr * (nl * w / lp), kDirectClamp);
+    if (fOut) *fOut = f;
+    if (lfOut) *lfOut = lr * (nl * w / lp);
     *group = 1u + li;
     return true;
 }
@@ -331,13 +361,25 @@ struct DirectSum {
     }
 };
 
+// What a bounce did to the throughput, for the adjoint:
+// T' = clamp_max_component(T * f * c, 12), then T' / q when roulette ran.
+struct BounceRec {
+    V3 f;            // BSDF value of the sampled direction
+    float c;         // cos / pdf
+    V3 next;         // sampled direction
+    bool reflective; // f is bsdf_eval(v, next) (not a transmission lobe)
+    bool rr;         // roulette ran (and the path survived)
+};
+
 // Lobe choice, BSDF sampling, throughput update and Russian roulette
-// (CpuPathTracer.cpp:1750-2013).  Returns false when the path ends.
+// (CpuPathTracer.cpp:1750-2013).  Returns false when the path ends.  `rec`
+// (optional) receives the throughput factors of a surviving bounce.
 DEV bool sample_bounce(const DevScene& S, const DevSettings& st, uint32_t depth, const Sampler& smp,
-                       const Surface& sf, PathState& ps) {
-    const Mat M{S.materials + sf.material, S.derived + sf.material};
+                       const Surface& sf, PathState& ps, BounceRec* rec = nullptr) {
+    const Mat& M = sf.M;
     const shiro_hip_material& m = *M.m;
     const MaterialDerived& md = *M.d;
+    const MaterialDerived& ms = *M.ds;
     const uint32_t b0 = depth * kBranchesPerDepth;
     const V3 n = sf.n, cn = sf.cn, ng = sf.ng, v = sf.v;
     const V3 rd = sf.v * -1.0f;
@@ -346,10 +388,10 @@ DEV bool sample_bounce(const DevScene& S, const DevSettings& st, uint32_t depth,
     uint32_t nDiffuse = (ps.bits >> kBitsDiffuseShift) & 0xffu;
     uint32_t nSpecular = (ps.bits >> kBitsSpecularShift) & 0xffu;
 
-    const float wd = md.wDiffuse;
-    float ws = md.wBaseSpecular;
-    const float wc = md.wCoat;
-    const float wtRaw = md.wTransmission;
+    const float wd = ms.wDiffuse;
+    float ws = ms.wBaseSpecular;
+    const float wc = ms.wCoat;
+    const float wtRaw = ms.wTransmission;
     float wt = wtRaw;
     if (wtRaw > 0.0f) {
         const float ei = inside ? fmaxs(m.ior, 1.0f) : 1.0f;
@@ -386,7 +428,7 @@ DEV bool sample_bounce(const DevScene& S, const DevSettings& st, uint32_t depth,
         ++nSpecular;
         type = kBounceTransmission;
         const float pT = fmaxs(wt / wsum, 1.0e-6f);
-        const float rT = m.thinWalled ? 0.0f : md.sampleRoughnessTrans;
+        const float rT = m.thinWalled ? 0.0f : ms.sampleRoughnessTrans;
         const float mt = clampf(m.transmission, 0.0f, 1.0f) * (1.0f - clampf(m.metallic, 0.0f, 1.0f));
         if (m.thinWalled) {
             next = rd;
@@ -444,7 +486,7 @@ DEV bool sample_bounce(const DevScene& S, const DevSettings& st, uint32_t depth,
         ++nSpecular;
         type = kBounceReflective;
         const V3 sn = pickC ? cn : n;
-        const float sr = pickC ? md.sampleRoughnessCoat : md.sampleRoughnessBase;
+        const float sr = pickC ? ms.sampleRoughnessCoat : ms.sampleRoughnessBase;
         const V3 hv = sample_ggx_half(sn, du, sr);
         next = normalize(reflect(rd, hv));
         const float rc = clampf(dot(sn, next), 0.0f, 1.0f);
@@ -465,6 +507,13 @@ DEV bool sample_bounce(const DevScene& S, const DevSettings& st, uint32_t depth,
 
     const float cosT = pickT ? fmaxs(fabsf(dot(ng, next)), 1.0e-4f) : clampf(dot(n, next), 0.0f, 1.0f);
     if (cosT <= 0.0f) return false;
+    if (rec) {
+        rec->f = f;
+        rec->c = cosT / pdf;
+        rec->next = next;
+        rec->reflective = !pickT;
+        rec->rr = false;
+    }
     ps.T = clamp_max_component(ps.T * f * (cosT / pdf), kThroughputClamp);
     const V3 rn = pickT ? ng : n;
     const float side = dot(rn, next) >= 0.0f ? 1.0f : -1.0f;
@@ -477,13 +526,14 @@ DEV bool sample_bounce(const DevScene& S, const DevSettings& st, uint32_t depth,
         const float q = clampf(max3(ps.T), 0.05f, 0.95f);
         if (smp.next1(b0 + 4u) > q) return false;
         ps.T = ps.T / q;
+        if (rec) rec->rr = true;
     }
     return true;
 }
 
 // TracePath (CpuPathTracer.cpp:1650-2017) in one thread: the megakernel
 // form.  Shadow rays are traced as soon as they are generated.
-template <bool COUNT>
+template <bool COUNT, bool TEX>
 DEV PathState trace_path(const DevScene& S, const DevSettings& st, const DevCamera& cam, uint32_t x, uint32_t y,
                          const Sampler& smp, Counters<COUNT>& c) {
     PathState ps = init_path(cam, st, x, y, smp);
@@ -498,8 +548,9 @@ DEV PathState trace_path(const DevScene& S, const DevSettings& st, const DevCame
         }
         if (COUNT) c.hits += 1;
         Surface sf;
-        if (!shade_hit(S, depth, smp, ray, h, ps, sf)) continue;
-        const Mat M{S.materials + sf.material, S.derived + sf.material};
+        TexState ts;
+        if (!shade_hit<TEX>(S, depth, smp, ray, h, ps, sf, &ts)) continue;
+        const Mat& M = sf.M;
         DirectSum ds;
         ds.init(st);
         const V3 origin = sf.p + sf.n * kRayEps;

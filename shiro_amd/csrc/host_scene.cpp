@@ -42,12 +42,15 @@ H3 one_minus_clamped(H3 v) {
     return {clampf(1.0f - v.x, 0.0f, 1.0f), clampf(1.0f - v.y, 0.0f, 1.0f), clampf(1.0f - v.z, 0.0f, 1.0f)};
 }
 
-// Isotropic AdjustedRoughness (CpuPathTracer.cpp:845-850).
-float sampling_roughness(float r, float aniso, float rot) {
+// The factor of the isotropic AdjustedRoughness (CpuPathTracer.cpp:845-850):
+// AdjustedRoughness = clamp(r * factor, 0.02, 1).
+float sampling_scale(float aniso, float rot) {
     const float a = clampf(aniso, -0.95f, 0.95f);
     const float phase = std::cos(rot * 2.0f * kPiH);
-    return clampf(r * (1.0f - 0.35f * a * phase), 0.02f, 1.0f);
+    return 1.0f - 0.35f * a * phase;
 }
+
+float sampling_roughness(float r, float aniso, float rot) { return clampf(r * sampling_scale(aniso, rot), 0.02f, 1.0f); }
 
 }  // namespace
 
@@ -121,6 +124,8 @@ MaterialDerived derive_material(const shiro_hip_material& m) {
     d.sampleRoughnessBase = sampling_roughness(d.baseRoughness, m.specularAnisotropy, m.specularRotation);
     d.sampleRoughnessCoat = sampling_roughness(m.coatRoughness, m.coatAnisotropy, m.coatRotation);
     d.sampleRoughnessTrans = sampling_roughness(d.transmissionRoughness, m.specularAnisotropy, m.specularRotation);
+    d.sampleScaleBase = sampling_scale(m.specularAnisotropy, m.specularRotation);
+    d.sampleScaleTrans = d.sampleScaleBase;
     // Emission (:1728-1733).
     d.emission[0] = std::max(m.emissionColor[0] * m.emissionStrength, 0.0f);
     d.emission[1] = std::max(m.emissionColor[1] * m.emissionStrength, 0.0f);
@@ -152,6 +157,7 @@ MaterialDerived derive_material(const shiro_hip_material& m) {
     if (m.hasTangentOverride) f |= kMatTangentOverride;
     if (m.emissionStrength > 0.0f) f |= kMatEmissive;
     if (clampf(m.opacity, 0.0f, 1.0f) < 0.999f) f |= kMatOpacity;
+    if (m.baseColorTexture >= 0 || m.roughnessTexture >= 0) f |= kMatTextured;
     d.flags = f;
     return d;
 }
@@ -216,6 +222,29 @@ bool flatten_scene(const shiro_hip_scene& s, FlatScene* out, std::string* err) {
     out->material.reserve(total);
     out->hasNormals.reserve(total);
     out->nrm.reserve(total * 9);
+    for (uint32_t mi = 0; mi < s.meshCount; ++mi)
+        if (s.meshes[mi].uvs && s.meshes[mi].indexCount >= 3u) out->hasUvs = true;
+    if (out->hasUvs) out->uv.reserve(total * 6);
+    if (s.textureCount > 0 && !s.textures) {
+        *err = "textures is null";
+        return false;
+    }
+    for (uint32_t ti = 0; ti < s.textureCount; ++ti) {
+        const shiro_hip_texture& t = s.textures[ti];
+        if (!t.texels || t.width == 0 || t.height == 0 || t.channels == 0 || t.channels > 4) {
+            *err = "texture " + std::to_string(ti) + " is empty or has 0 or more than 4 channels";
+            return false;
+        }
+    }
+    for (uint32_t i = 0; i < s.materialCount; ++i) {
+        const shiro_hip_material& m = s.materials[i];
+        if (m.baseColorTexture >= static_cast<int32_t>(s.textureCount) ||
+            m.roughnessTexture >= static_cast<int32_t>(s.textureCount)) {
+            *err = "material " + std::to_string(i) + " names a missing texture";
+            return false;
+        }
+        if (m.baseColorTexture >= 0 || m.roughnessTexture >= 0) out->textured = true;
+    }
     for (uint32_t mi = 0; mi < s.meshCount; ++mi) {
         const shiro_hip_mesh& m = s.meshes[mi];
         const uint32_t nt = m.indexCount / 3u;
@@ -237,6 +266,14 @@ bool flatten_scene(const shiro_hip_scene& s, FlatScene* out, std::string* err) {
                 }
                 const float* v = m.positions + 3u * ix[k];
                 out->tri.insert(out->tri.end(), v, v + 3);
+                if (out->hasUvs) {
+                    if (m.uvs) {
+                        out->uv.push_back(m.uvs[2u * ix[k]]);
+                        out->uv.push_back(m.uvs[2u * ix[k] + 1u]);
+                    } else {
+                        out->uv.insert(out->uv.end(), {0.0f, 0.0f});
+                    }
+                }
                 if (normals) {
                     const float* n = m.normals + 3u * ix[k];
                     out->nrm.insert(out->nrm.end(), n, n + 3);

@@ -31,11 +31,15 @@ struct FlatScene {
     std::vector<float> nrm;        // 9 floats per triangle (vertex normals), when hasNormals
     std::vector<uint8_t> hasNormals;
     std::vector<uint32_t> material; // material index per triangle (default material = materialCount)
+    std::vector<float> uv;          // 6 floats per triangle (vertex UVs, 0 where a mesh has none), when hasUvs
+    bool hasUvs = false;            // any mesh with UVs
+    bool textured = false;          // any material with a texture
     bool emissive = false;          // any emissive material in use
 };
 
 // Validates and flattens; returns false with a message on invalid input
-// (index out of range, null arrays, too many lights).
+// (index out of range, null arrays, too many lights, a material naming a
+// missing texture, an empty texture).
 bool flatten_scene(const shiro_hip_scene& s, FlatScene* out, std::string* err);
 
 }  // namespace shiro_hip

@@ -72,7 +72,21 @@ struct StageMarks {
     }
 };
 
+// Path-replay adjoint (adjoint.hip).
+constexpr uint32_t kAdjointMaxDepth = 16;
+constexpr uint32_t kAdjointMaxTextures = 16;
+struct AdjointParams {
+    DevScene scene;
+    DevCamera camera;
+    DevSettings settings;
+    const float4* dBeauty;                // W*H adjoint of the beauty mean (rgb; a ignored)
+    float* grad[kAdjointMaxTextures];     // texel gradients per texture (null: not wanted)
+    uint32_t sampleStart, sampleCount;
+    uint32_t tilesX, tiles;
+};
+
 hipError_t launch_render(const RenderParams& P, bool count, hipStream_t stream);
+hipError_t launch_adjoint(const AdjointParams& P, hipStream_t s);
 hipError_t launch_wavefront_pass(const WfParams& P, const WfLaunch& L, hipStream_t s, StageMarks* marks);
 hipError_t launch_accum_add(const AccumParams& A, const DevFrame& f, uint32_t count, uint32_t np, hipStream_t s);
 hipError_t launch_accum_resolve(const AccumParams& A, const DevFrame& f, uint32_t total, uint32_t np, hipStream_t s);

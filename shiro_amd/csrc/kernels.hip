@@ -35,7 +35,7 @@ DEV uint32_t xcd_tile(uint32_t bid, uint32_t nblocks) {
     return base + slot;
 }
 
-template <bool COUNT>
+template <bool COUNT, bool TEX>
 __global__ __launch_bounds__(64) void k_render(RenderParams P) {
     const uint32_t nblocks = gridDim.x;
     const uint32_t tile = xcd_tile(blockIdx.x, nblocks);
@@ -61,7 +61,7 @@ __global__ __launch_bounds__(64) void k_render(RenderParams P) {
     Counters<COUNT> cnt;
     for (uint32_t i = 0; i < P.sliceCount; ++i) {
         const Sampler smp{x, y, P.sampleStart + P.sliceBegin + i};
-        const PathState pr = trace_path(P.scene, st, P.camera, x, y, smp, cnt);
+        const PathState pr = trace_path<COUNT, TEX>(P.scene, st, P.camera, x, y, smp, cnt);
         beauty.x = beauty.x + pr.L.x;
         beauty.y = beauty.y + pr.L.y;
         beauty.z = beauty.z + pr.L.z;
@@ -212,7 +212,7 @@ __global__ void k_eval_bsdf(DevScene S, const uint32_t* mi, const float* n, cons
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= count) return;
     const uint32_t m = mi[i];
-    const Mat M{S.materials + m, S.derived + m};
+    const Mat M{S.materials + m, S.derived + m, S.derived + m};
     const V3 N = ld3(n + 3 * i), CN = ld3(cn + 3 * i), Vv = ld3(v + 3 * i), L = ld3(l + 3 * i);
     const V3 f = bsdf_eval(M, N, CN, Vv, L);
     value[3 * i] = f.x;
@@ -256,10 +256,19 @@ hipError_t launch_render(const RenderParams& P, bool count, hipStream_t stream) 
     const uint32_t tilesY = (P.settings.height + 7u) / 8u;
     const uint32_t nb = P.tilesX * tilesY;
     if (nb == 0) return hipSuccess;
+    const bool tex = P.scene.textured != 0u;
     if (count) {
-        hipLaunchKernelGGL(k_render<true>, dim3(nb), dim3(64), 0, stream, P);
+        if (tex) {
+            hipLaunchKernelGGL((k_render<true, true>), dim3(nb), dim3(64), 0, stream, P);
+        } else {
+            hipLaunchKernelGGL((k_render<true, false>), dim3(nb), dim3(64), 0, stream, P);
+        }
     } else {
-        hipLaunchKernelGGL(k_render<false>, dim3(nb), dim3(64), 0, stream, P);
+        if (tex) {
+            hipLaunchKernelGGL((k_render<false, true>), dim3(nb), dim3(64), 0, stream, P);
+        } else {
+            hipLaunchKernelGGL((k_render<false, false>), dim3(nb), dim3(64), 0, stream, P);
+        }
     }
     return hipGetLastError();
 }

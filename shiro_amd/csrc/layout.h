@@ -21,6 +21,9 @@
 //            read once per closest hit (shading normal + material).
 //   materials  shiro_hip_material array (276 B each) + a derived table
 //              (MaterialDerived) precomputed on the host.
+//   uvs      optional, 24 B per triangle in leaf order (3 x float2), read
+//            only at hits on textured materials.
+//   textures optional float texels (w x h x channels, row 0 at v = 0).
 //   env      per dome: texels as float4 (RGB + pad), rowCdf (H+1),
 //            condCdf (H x (W+1)), all fp32.
 #pragma once
@@ -84,6 +87,8 @@ struct MaterialDerived {
     float emission[3];           // max(emissionColor*strength, 0)
     float mediumExtinction[3];
     float transmissionTint[3];
+    float sampleScaleBase;       // 1 - 0.35 * anisotropy * cos(2 pi rotation), base lobe
+    float sampleScaleTrans;      // the same for the transmission lobe
     uint32_t flags;              // kMat* bits
 };
 
@@ -95,6 +100,17 @@ constexpr uint32_t kMatCoatNormalOverride = 16u;
 constexpr uint32_t kMatTangentOverride = 32u;
 constexpr uint32_t kMatEmissive = 64u;
 constexpr uint32_t kMatOpacity = 128u;     // opacity < 0.999
+constexpr uint32_t kMatTextured = 256u;    // baseColorTexture or roughnessTexture >= 0
+
+// A float texture of the texture extension.  Two copies of the texels: the
+// one shading evaluates and the one sampling decisions read (lobe weights,
+// sampling roughness, pdfs).  They are the same array unless a caller froze
+// the sampling copy (shiro_hip_update_texture with EVAL_ONLY).
+struct DevTexture {
+    const float* eval;
+    const float* sampling;
+    uint32_t w, h, channels, pad;
+};
 
 struct DevScene {
     const float4* nodes;
@@ -102,6 +118,10 @@ struct DevScene {
     const float4* hitdata;
     const shiro_hip_material* materials;
     const MaterialDerived* derived;
+    const float2* uvs;            // 3 per triangle in leaf order, null without UVs
+    const DevTexture* textures;   // null without textures
+    uint32_t textureCount;
+    uint32_t textured;            // any material with kMatTextured
     uint32_t nodeCount;
     uint32_t triCount;
     uint32_t materialCount;

@@ -297,6 +297,7 @@ __global__ __launch_bounds__(64) void k_wf_trace(WfParams P, uint32_t depth) {
     }
 }
 
+template <bool TEX>
 __global__ __launch_bounds__(64) void k_wf_shade(WfParams P, uint32_t depth) {
     const uint32_t* in = (depth & 1u) ? P.B.qb : P.B.qa;
     uint32_t* out = (depth & 1u) ? P.B.qa : P.B.qb;
@@ -329,10 +330,11 @@ __global__ __launch_bounds__(64) void k_wf_shade(WfParams P, uint32_t depth) {
             } else {
                 const Ray ray = path_ray(ps);
                 Surface sf;
-                if (!shade_hit(P.scene, depth, smp, ray, h, ps, sf)) {
+                TexState ts;
+                if (!shade_hit<TEX>(P.scene, depth, smp, ray, h, ps, sf, &ts)) {
                     alive = true;
                 } else {
-                    const Mat M{P.scene.materials + sf.material, P.scene.derived + sf.material};
+                    const Mat& M = sf.M;
                     uint32_t ns = 0;
                     float4* sr = P.B.sr + static_cast<size_t>(p) * P.B.maxShadow * 2u;
                     for (uint32_t j = 0; j < nLight; ++j) {
@@ -464,7 +466,11 @@ hipError_t launch_wavefront_pass(const WfParams& P, const WfLaunch& L, hipStream
             default: hipLaunchKernelGGL((k_wf_trace<false, 0>), dim3(blocks), dim3(64), 0, s, P, d); break;
         }
         m.mark(SHIRO_HIP_STAGE_EXTEND, s);
-        hipLaunchKernelGGL(k_wf_shade, dim3(blocks), dim3(64), 0, s, P, d);
+        if (P.scene.textured) {
+            hipLaunchKernelGGL(k_wf_shade<true>, dim3(blocks), dim3(64), 0, s, P, d);
+        } else {
+            hipLaunchKernelGGL(k_wf_shade<false>, dim3(blocks), dim3(64), 0, s, P, d);
+        }
         m.mark(SHIRO_HIP_STAGE_SHADE, s);
         switch (L.fetchMode) {
             case 2: hipLaunchKernelGGL((k_wf_trace<true, 2>), dim3(tblocks), dim3(64), 0, s, P, d); break;

@@ -88,6 +88,10 @@ def load_library(path: str = LIB_PATH):
     d("shiro_hip_get_tuning", C.c_int, V, _P(abi.Tuning))
     d("shiro_hip_autotune", C.c_int, V, _P(abi.Camera), _P(abi.Settings), _u, _P(abi.Tuning), C.c_char_p,
       C.c_size_t)
+    d("shiro_hip_set_stage_timing", C.c_int, V, _u)
+    d("shiro_hip_get_stage_times", C.c_int, V, _P(abi.StageTimes))
+    d("shiro_hip_update_texture", C.c_int, V, _u, _fp, _u)
+    d("shiro_hip_render_adjoint", C.c_int, V, _P(abi.Camera), _P(abi.Settings), _u, _u, _fp, _P(_fp), _u)
     _lib = L
     return L
 
@@ -111,6 +115,9 @@ def scene_signature(scene: Scene) -> str:
         if d.pixels is not None:
             h.update(np.ascontiguousarray(d.pixels).tobytes())
     h.update(np.asarray(tuple(scene.environmentTop) + tuple(scene.environmentBottom), np.float32).tobytes())
+    for t in scene.textures:
+        h.update(np.asarray(np.shape(t.texels), np.int64).tobytes())
+        h.update(np.ascontiguousarray(t.texels, np.float32).tobytes())
     return h.hexdigest()
 
 
@@ -312,6 +319,33 @@ class Context:
         self._check(self.L.shiro_hip_count_fetches(self.h, C.byref(cam), C.byref(st), sample_start, sample_count,
                                                    C.byref(out)))
         return out
+
+    # ---- texture extension ----
+    def update_texture(self, index, texels, eval_only=False):
+        """Replaces texture `index`'s texels: a float32 numpy array, or a
+        device address (int) of the same size.  eval_only freezes the texels
+        sampling reads (finite differences of the frozen-sampling render)."""
+        flags = abi.TEXELS_EVAL_ONLY if eval_only else 0
+        if isinstance(texels, int):
+            ptr = C.cast(C.c_void_p(texels), _fp)
+            flags |= abi.TEXELS_DEVICE
+            keep = None
+        else:
+            keep = np.ascontiguousarray(texels, np.float32)
+            ptr = abi.fp(keep)
+        self._check(self.L.shiro_hip_update_texture(self.h, index, ptr, flags))
+        del keep
+        self._sig = None  # the device scene no longer matches the uploaded one
+
+    def render_adjoint(self, camera, settings, dbeauty_ptr, grad_ptrs, sample_start=0, sample_count=None):
+        """Adds the texel gradients of sum(dbeauty * beauty) to the device
+        buffers grad_ptrs[i] (0 = skip) for texture i; dbeauty_ptr is a
+        device W*H*4 float image."""
+        sample_count = settings.samplesPerPixel if sample_count is None else sample_count
+        arr = (_fp * max(1, len(grad_ptrs)))(*[C.cast(C.c_void_p(p), _fp) if p else None for p in grad_ptrs])
+        cam, st = camera.to_c(), settings.to_c()
+        self._check(self.L.shiro_hip_render_adjoint(self.h, C.byref(cam), C.byref(st), sample_start, sample_count,
+                                                    C.cast(C.c_void_p(dbeauty_ptr), _fp), arr, len(grad_ptrs)))
 
     def set_stage_timing(self, enable=True):
         self._check(self.L.shiro_hip_set_stage_timing(self.h, 1 if enable else 0))

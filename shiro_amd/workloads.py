@@ -19,6 +19,12 @@ Workloads (sizes exact):
       geodesic spheres (12,500 tris) + 250 displaced boxes (7,500 tris) =
       5,000,000 triangles at random transforms in a 100-unit cube; 40% diffuse,
       30% GGX metal, 20% dielectric, 10% coated; c2's dome + one directional.
+  c5  256x256, 32 spp, depth 4 (texture extension): UV-mapped geodesic
+      sphere (frequency 50, 50,000 tris) on a 2-triangle ground; the sphere's
+      base colour (128x128x3) and roughness (128x128x1) are textures; a
+      256x128 dome.  `textures_truth` holds the seeded target texels,
+      `textures_init` the (non-grey, jittered) starting point of an
+      optimisation.
 Every workload accepts a `scale` that shrinks the image (and spp where
 stated) for parity tests at sizes the oracle finishes in seconds.
 """
@@ -29,7 +35,7 @@ from typing import Tuple
 
 import numpy as np
 
-from .scene import (Camera, DirectionalLight, DomeLight, PbrMaterial, RenderSettings, Scene,
+from .scene import (Camera, DirectionalLight, DomeLight, PbrMaterial, RenderSettings, Scene, Texture,
                     TriangleMesh, standard_surface, usd_preview_surface)
 
 BASE_SEED = 0x5348524F
@@ -386,4 +392,60 @@ def c3(scale: float = 1.0, spp: int = 16, seed: int = BASE_SEED, objects: int = 
     return Workload("c3", scene, cam, _settings(w, h, spp, 16, rr=3))
 
 
-WORKLOADS = {"c1": c1, "c2": c2, "c3": c3}
+def sphere_uvs(p: np.ndarray) -> np.ndarray:
+    """Lat-long UVs of points on a sphere around the origin: u = longitude,
+    v = polar angle / pi (0 at +Y)."""
+    d = p / np.linalg.norm(p, axis=1, keepdims=True)
+    u = np.arctan2(d[:, 0], d[:, 2]) / (2 * np.pi) + 0.5
+    v = np.arccos(np.clip(d[:, 1], -1.0, 1.0)) / np.pi
+    return np.stack([u, v], 1).astype(np.float32)
+
+
+def c5_textures(seed: int, size: int = 128):
+    """Seeded target texels: smooth base colour in [0.1, 0.9]^3 and
+    roughness in [0.15, 0.85] (value-noise fields on the texture plane)."""
+    yy, xx = np.meshgrid((np.arange(size) + 0.5) / size, (np.arange(size) + 0.5) / size, indexing="ij")
+    pts = np.stack([xx.ravel() * 6.0, yy.ravel() * 6.0, np.zeros(size * size)], 1)
+    cols = []
+    for c in range(3):
+        n = ValueNoise3(seed + 500 + c).fbm(pts + c * 17.0, 4)
+        cols.append(0.5 + 0.4 * np.tanh(1.5 * n))
+    albedo = np.stack(cols, 1).reshape(size, size, 3).astype(np.float32)
+    r = ValueNoise3(seed + 600).fbm(pts * 0.7 + 3.0, 4)
+    rough = (0.5 + 0.35 * np.tanh(1.5 * r)).reshape(size, size, 1).astype(np.float32)
+    return albedo, rough
+
+
+def c5(scale: float = 1.0, spp: int = 32, seed: int = BASE_SEED, frequency: int = 50, tex_size: int = 128,
+       env_size=(256, 128)) -> Workload:
+    p, ix = geodesic_sphere(frequency)
+    p = p.astype(np.float32)
+    sphere = make_mesh(p + np.array([0.0, 1.0, 0.0], np.float32), ix, 0)
+    sphere.uvs = sphere_uvs(p)
+    quad = np.array([[-4, 0, -4], [4, 0, -4], [4, 0, 4], [-4, 0, 4]], np.float32)
+    qi = np.array([0, 2, 1, 0, 3, 2], np.uint32)
+    mat = usd_preview_surface((0.5, 0.5, 0.5), roughness=0.5)
+    mat.baseColorTexture, mat.roughnessTexture = 0, 1
+    ground = usd_preview_surface((0.35, 0.35, 0.35), roughness=0.7)
+    albedo, rough = c5_textures(seed + 5, tex_size)
+    # Starting texels: a non-grey base colour with a small seeded jitter.
+    # With grey texels every throughput channel ties, and the max in the
+    # reference's clamp_max_component has no derivative at a tie.
+    jr = np.random.default_rng(seed + 7)
+    init_albedo = (np.array([0.55, 0.5, 0.45], np.float32) +
+                   jr.uniform(-0.02, 0.02, albedo.shape)).astype(np.float32)
+    init_rough = (0.5 + jr.uniform(-0.02, 0.02, rough.shape)).astype(np.float32)
+    scene = Scene(materials=[mat, ground], meshes=[sphere, make_mesh(quad, qi, 1)],
+                  textures=[Texture(albedo.copy()), Texture(rough.copy())])
+    ew, eh = env_size
+    scene.domeLights.append(DomeLight(pixels=procedural_dome(ew, eh, seed + 6, hot_spots=2, peak=40.0,
+                                                             spot_width_deg=6.0)))
+    w = h = max(8, int(round(256 * scale)))
+    cam = Camera.look_at((0.0, 1.6, 3.6), (0.0, 0.9, 0.0), fov=40.0, aspect=1.0)
+    wl = Workload("c5", scene, cam, _settings(w, h, spp, 4, rr=4))
+    wl.textures_truth = (albedo, rough)
+    wl.textures_init = (init_albedo, init_rough)
+    return wl
+
+
+WORKLOADS = {"c1": c1, "c2": c2, "c3": c3, "c5": c5}

@@ -40,3 +40,17 @@ def random_rays(scene, n, seed=0):
     r[:, 6] = 1e-4
     r[:, 7] = np.float32(3.402823466e38)
     return r
+
+
+def c5_mini(size=32, spp=4, frequency=12, tex=16, env=(64, 32)):
+    """c5's textured sphere at a size the oracle renders in seconds."""
+    return workloads.c5(scale=size / 256, spp=spp, frequency=frequency, tex_size=tex, env_size=env)
+
+
+def c5_scene_with(wl, albedo, rough):
+    """The c5 scene with its two textures replaced."""
+    import copy
+    from shiro_amd.scene import Texture
+    scene = copy.copy(wl.scene)
+    scene.textures = [Texture(np.ascontiguousarray(albedo, np.float32)), Texture(np.ascontiguousarray(rough, np.float32))]
+    return scene
