@@ -65,6 +65,71 @@ def cpu_baseline(wl, target_seconds, threads):
     }
 
 
+def bench_c4(args):
+    """Aggregate Msamples/s of the c4 batch (32 scenes, 256-1024 px square,
+    4/16/64 spp, 10k-500k triangles, own 1024x512 dome each) rendered by one
+    persistent batched launch (autotuned grid), with the same batch as a
+    replayed hipGraph of per-scene megakernel launches and as sequential
+    launches reported beside it."""
+    import torch
+    from shiro_amd import abi, workloads
+    from shiro_amd.hip import Context
+    t0 = time.perf_counter()
+    wls = workloads.c4()
+    gen_s = time.perf_counter() - t0
+    ctxs, frames = [], []
+    t0 = time.perf_counter()
+    for wl in wls:
+        c = Context(0)
+        c.upload(wl.scene)
+        ctxs.append(c)
+        n = wl.settings.width * wl.settings.height
+        frames.append([torch.empty(n * k, dtype=torch.float32, device="cuda") for k in (4, 3, 3, 1)])
+    upload_s = time.perf_counter() - t0
+    launcher = Context(0)
+    items = Context.batch_items([(c, wl.camera, wl.settings, [b.data_ptr() for b in f], 0, None)
+                                 for wl, c, f in zip(wls, ctxs, frames)])
+    samples = sum(wl.samples for wl in wls)
+    tris = sum(wl.scene.triangle_count for wl in wls)
+    print(f"# c4: {len(wls)} scenes, {tris} triangles, {samples} samples per batch; generated in {gen_s:.1f} s, "
+          f"built and uploaded in {upload_s:.1f} s", file=sys.stderr, flush=True)
+    best, report = launcher.autotune_many(items)
+    print("# batch autotune:\n" + report, file=sys.stderr, flush=True)
+    modes = {}
+    for name, mode in (("sequential", abi.MANY_SEQUENTIAL), ("graph", abi.MANY_GRAPH),
+                       ("wavefront", abi.MANY_WAVEFRONT)):
+        launcher.render_many(items, mode)
+        modes[name] = round(launcher.render_many(items, mode), 3)
+        print(f"# {name}: {modes[name]} ms", file=sys.stderr, flush=True)
+    for i in range(args.warmup):
+        launcher.render_many(items, abi.MANY_PERSISTENT)
+    torch.cuda.synchronize()
+    gpu_ms = []
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        gpu_ms.append(launcher.render_many(items, abi.MANY_PERSISTENT))
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    modes["persistent"] = round(sum(gpu_ms) / len(gpu_ms), 3)
+    cpu = None
+    if not args.no_cpu:
+        wl = sorted(wls, key=lambda w: w.samples)[len(wls) // 2]
+        cpu = cpu_baseline(wl, args.cpu_seconds, args.cpu_threads)
+        cpu["sample"] = f"{wl.name}: " + cpu["sample"]
+    out = {
+        "metric": "Msamples/s", "value": round(samples * args.steps / wall / 1e6, 3), "unit": "Msamples/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(wall * 1e3 / args.steps, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (32 seeded scenes and domes)",
+        "config": {"workload": f"c4: {len(wls)} scenes, {tris} triangles, {samples} samples per step",
+                   "batch_ms": modes,
+                   "speedup_vs_sequential": round(modes["sequential"] / modes["persistent"], 3),
+                   "tuning": {"batchBlocksPerCU": best.batchBlocksPerCU}},
+        "roofline": None, "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def bench_c5(args):
     """Gradient evaluations per second of the texture extension's
     differentiable render (c5: 256x256, 32 spp forward + 32 spp path-replay
@@ -150,7 +215,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "c5"])
+    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "c4", "c5"])
     ap.add_argument("--transport", default="wavefront", choices=["megakernel", "wavefront"])
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--cpu-threads", type=int, default=16)
@@ -178,6 +243,9 @@ def main():
 
     if args.workload == "c5":
         bench_c5(args)
+        return
+    if args.workload == "c4":
+        bench_c4(args)
         return
     import torch
     from shiro_amd import workloads

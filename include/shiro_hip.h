@@ -429,7 +429,8 @@ typedef struct shiro_hip_tuning {
                                2 lane refill from per-wave pools of the queue (default 0) */
     uint32_t traceBlocks;   /* grid of the fetchMode 2 traversal kernels (default 32 x CUs) */
     uint32_t stageBlocks;   /* grid cap of the grid-stride stage kernels (default 16384) */
-    uint32_t reserved[4];
+    uint32_t batchBlocksPerCU; /* one-wave blocks per CU of the batched persistent kernel (default 8) */
+    uint32_t reserved[3];
 } shiro_hip_tuning;
 int shiro_hip_set_tuning(shiro_hip_ctx* ctx, const shiro_hip_tuning* tuning);
 int shiro_hip_get_tuning(const shiro_hip_ctx* ctx, shiro_hip_tuning* tuning);
@@ -440,6 +441,38 @@ int shiro_hip_get_tuning(const shiro_hip_ctx* ctx, shiro_hip_tuning* tuning);
 int shiro_hip_autotune(
     shiro_hip_ctx* ctx, const shiro_hip_camera* camera,
     const shiro_hip_settings* settings, uint32_t trialSamples,
+    shiro_hip_tuning* best, char* report, size_t reportSize);
+
+/* ---- Batched multi-scene rendering (no counterpart in the reference: one
+ * launch for the sample batches of many scenes, e.g. a farm of small
+ * frames). ---- */
+typedef struct shiro_hip_batch_item {
+    const shiro_hip_ctx* scene;   /* context holding the uploaded scene (same device) */
+    shiro_hip_camera camera;
+    shiro_hip_settings settings;  /* transport is ignored */
+    uint32_t sampleStart;
+    uint32_t sampleCount;
+    shiro_hip_frame frame;        /* device memory, settings.width x height */
+} shiro_hip_batch_item;
+enum {
+    SHIRO_HIP_MANY_PERSISTENT = 0, /* one persistent megakernel over (item, tile) units, longest first */
+    SHIRO_HIP_MANY_GRAPH = 1,      /* one megakernel launch per item, captured in a hipGraph and replayed */
+    SHIRO_HIP_MANY_SEQUENTIAL = 2, /* one megakernel launch per item, back to back on the stream */
+    SHIRO_HIP_MANY_WAVEFRONT = 3   /* the wavefront form of each item in turn (ctx's tuning and buffers) */
+};
+/* Renders every item's batch (means into its device frame, like
+ * shiro_hip_render_batch_device; the frames are identical in every mode)
+ * on ctx's stream and waits.  shiro_hip_last_render_ms times the kernels
+ * (for the graph mode the replay, not the capture).  The scene contexts
+ * must not be changed while the call runs. */
+int shiro_hip_render_many(
+    shiro_hip_ctx* ctx, const shiro_hip_batch_item* items, uint32_t count,
+    uint32_t mode);
+/* Times the persistent mode for blocks-per-CU in {1, 2, 4, 8, 16, 32} (a
+ * warm run and a timed run each), keeps the fastest in ctx's tuning and
+ * returns it; `report` (may be NULL) receives one line per candidate. */
+int shiro_hip_autotune_many(
+    shiro_hip_ctx* ctx, const shiro_hip_batch_item* items, uint32_t count,
     shiro_hip_tuning* best, char* report, size_t reportSize);
 
 /* ---- Per-kernel timing (no counterpart in the reference: measurement of

@@ -132,8 +132,18 @@ class FetchCounts(C.Structure):
 class Tuning(C.Structure):
     _fields_ = [
         ("pathsPerPass", C.c_uint32), ("fetchMode", C.c_uint32), ("traceBlocks", C.c_uint32),
-        ("stageBlocks", C.c_uint32), ("reserved", C.c_uint32 * 4),
+        ("stageBlocks", C.c_uint32), ("batchBlocksPerCU", C.c_uint32), ("reserved", C.c_uint32 * 3),
     ]
+
+
+class BatchItem(C.Structure):
+    _fields_ = [
+        ("scene", C.c_void_p), ("camera", Camera), ("settings", Settings),
+        ("sampleStart", C.c_uint32), ("sampleCount", C.c_uint32), ("frame", Frame),
+    ]
+
+
+MANY_PERSISTENT, MANY_GRAPH, MANY_SEQUENTIAL, MANY_WAVEFRONT = 0, 1, 2, 3
 
 
 STAGE_NAMES = ["reset", "extend", "shade", "shadow", "resolve", "accum", "megakernel", "unused"]
@@ -169,6 +179,7 @@ EXPORTS = [
     "shiro_hip_set_tuning", "shiro_hip_get_tuning", "shiro_hip_autotune",
     "shiro_hip_set_stage_timing", "shiro_hip_get_stage_times",
     "shiro_hip_update_texture", "shiro_hip_render_adjoint",
+    "shiro_hip_render_many", "shiro_hip_autotune_many",
 ]
 
 

@@ -102,7 +102,12 @@ struct shiro_hip_ctx {
     DevBuf acB, acA, acN, acD, acC;
     uint32_t acW = 0, acH = 0, acTotal = 0;
     // Wavefront tuning (shiro_hip_set_tuning / shiro_hip_autotune).
-    shiro_hip_tuning tuning{4u << 20, 0u, 8192u, 16384u, {0, 0, 0, 0}};
+    shiro_hip_tuning tuning{4u << 20, 0u, 8192u, 16384u, 8u, {0, 0, 0}};
+    uint32_t computeUnits = 256;
+    // Batched rendering: items, work list and the work counter.
+    DevBuf manyItems, manyWork, manyCounter;
+    std::vector<uint32_t> manyDepth;   // BVH depth per item of the current batch (cost estimate)
+    std::vector<const shiro_hip_batch_item*> manyItemsHost;  // non-empty items of the current batch
     // Stage timing (shiro_hip_set_stage_timing): events recorded after each
     // launch of the last render call.
     bool stageTiming = false;
@@ -200,13 +205,16 @@ void begin_marks(shiro_hip_ctx* ctx) {
     ctx->marks.mark(0xFF, ctx->stream);
 }
 
-int enqueue_wavefront(shiro_hip_ctx* ctx, const shiro_hip_camera& cam, const shiro_hip_settings& st, uint32_t start,
-                      uint32_t count, const DevFrame& frame, const volatile uint8_t* cancel) {
+// `scene` is ctx's own scene except for batched renders; `events` records
+// ev0 / ev1 around the passes.
+int enqueue_wavefront(shiro_hip_ctx* ctx, const DevScene& scene, const shiro_hip_camera& cam,
+                      const shiro_hip_settings& st, uint32_t start, uint32_t count, const DevFrame& frame,
+                      const volatile uint8_t* cancel, bool events = true) {
     const uint64_t np = static_cast<uint64_t>(st.width) * st.height;
     const uint32_t tilesX = (st.width + 7u) / 8u, tilesY = (st.height + 7u) / 8u;
     const uint64_t perSample = static_cast<uint64_t>(tilesX) * tilesY * 64u;
     const uint32_t K = st.domeLightSamples > 1u ? st.domeLightSamples : 1u;
-    const uint32_t M = std::max(1u, ctx->dev.dirCount + ctx->dev.domeCount * K);
+    const uint32_t M = std::max(1u, scene.dirCount + scene.domeCount * K);
     const uint64_t bytesPerPath = 10u * 16u + 12u + static_cast<uint64_t>(M) * 32u;
     uint64_t S = std::max<uint64_t>(1, ctx->tuning.pathsPerPass / perSample);
     S = std::min<uint64_t>(S, std::max<uint64_t>(1, kWavefrontStateBytes / (bytesPerPath * perSample)));
@@ -232,7 +240,7 @@ int enqueue_wavefront(shiro_hip_ctx* ctx, const shiro_hip_camera& cam, const shi
         HIPC(ctx->sumD.reserve(np * 4), "alloc sums");
     }
     WfParams P{};
-    P.scene = ctx->dev;
+    P.scene = scene;
     P.camera = to_dev(cam);
     P.settings = to_dev(st);
     P.sums = DevBatchSums{ctx->sumB.as<float4>(), ctx->sumA.as<float4>(), ctx->sumN.as<float4>(), ctx->sumD.as<float>()};
@@ -269,8 +277,10 @@ int enqueue_wavefront(shiro_hip_ctx* ctx, const shiro_hip_camera& cam, const shi
     L.maxBlocks = ctx->tuning.stageBlocks;
     L.traceBlocks = ctx->tuning.traceBlocks;
     L.fetchMode = ctx->tuning.fetchMode;
-    HIPC(hipEventRecord(ctx->ev0, ctx->stream), "event");
-    begin_marks(ctx);
+    if (events) {
+        HIPC(hipEventRecord(ctx->ev0, ctx->stream), "event");
+        begin_marks(ctx);
+    }
     for (uint32_t b = 0; b < count; b += static_cast<uint32_t>(S)) {
         if (cancel && *cancel) {
             (void)hipStreamSynchronize(ctx->stream);
@@ -278,10 +288,11 @@ int enqueue_wavefront(shiro_hip_ctx* ctx, const shiro_hip_camera& cam, const shi
         }
         P.sliceBegin = b;
         P.sliceCount = std::min(static_cast<uint32_t>(S), count - b);
-        HIPC(launch_wavefront_pass(P, L, ctx->stream, ctx->stageTiming ? &ctx->marks : nullptr), "wavefront launch");
+        HIPC(launch_wavefront_pass(P, L, ctx->stream, ctx->stageTiming && events ? &ctx->marks : nullptr),
+             "wavefront launch");
         if (cancel) HIPC(hipStreamSynchronize(ctx->stream), "wavefront");
     }
-    HIPC(hipEventRecord(ctx->ev1, ctx->stream), "event");
+    if (events) HIPC(hipEventRecord(ctx->ev1, ctx->stream), "event");
     return SHIRO_HIP_OK;
 }
 
@@ -290,7 +301,7 @@ int enqueue_wavefront(shiro_hip_ctx* ctx, const shiro_hip_camera& cam, const shi
 int enqueue_batch(shiro_hip_ctx* ctx, const shiro_hip_camera& cam, const shiro_hip_settings& st, uint32_t start,
                   uint32_t count, const DevFrame& frame, bool countFetches, const volatile uint8_t* cancel) {
     if (st.transport == SHIRO_HIP_TRANSPORT_WAVEFRONT && !countFetches)
-        return enqueue_wavefront(ctx, cam, st, start, count, frame, cancel);
+        return enqueue_wavefront(ctx, ctx->dev, cam, st, start, count, frame, cancel);
     const uint64_t np = static_cast<uint64_t>(st.width) * st.height;
     const uint32_t slice = slice_samples(np, count);
     RenderParams P{};
@@ -418,8 +429,10 @@ int shiro_hip_create(int device, shiro_hip_ctx** out) {
         return SHIRO_HIP_NO_DEVICE;
     }
     hipDeviceProp_t prop;
-    if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0) {
         ctx->tuning.traceBlocks = static_cast<uint32_t>(prop.multiProcessorCount) * 32u;
+        ctx->computeUnits = static_cast<uint32_t>(prop.multiProcessorCount);
+    }
     *out = ctx;
     return SHIRO_HIP_OK;
 }
@@ -903,7 +916,8 @@ int shiro_hip_env_eval(shiro_hip_ctx* ctx, uint32_t domeIndex, const float* dire
 int shiro_hip_set_tuning(shiro_hip_ctx* ctx, const shiro_hip_tuning* t) {
     if (!ctx || !t) return SHIRO_HIP_INVALID;
     std::lock_guard<std::mutex> lock(ctx->mu);
-    if (t->pathsPerPass == 0 || t->fetchMode > 2u || t->traceBlocks == 0 || t->stageBlocks == 0)
+    if (t->pathsPerPass == 0 || t->fetchMode > 2u || t->traceBlocks == 0 || t->stageBlocks == 0 ||
+        t->batchBlocksPerCU == 0 || t->batchBlocksPerCU > 64u)
         return fail(ctx, SHIRO_HIP_INVALID, "invalid tuning");
     ctx->tuning = *t;
     return SHIRO_HIP_OK;
@@ -1084,6 +1098,182 @@ int shiro_hip_render_adjoint(shiro_hip_ctx* ctx, const shiro_hip_camera* camera,
     HIPC(launch_adjoint(P, ctx->stream), "adjoint launch");
     HIPC(hipEventRecord(ctx->ev1, ctx->stream), "event");
     return finish_timing(ctx);
+}
+
+namespace {
+
+// Validates the items and fills the per-item megakernel parameters.
+int many_params(shiro_hip_ctx* ctx, const shiro_hip_batch_item* items, uint32_t count, std::vector<RenderParams>* out,
+                bool* tex) {
+    out->clear();
+    *tex = false;
+    for (uint32_t i = 0; i < count; ++i) {
+        const shiro_hip_batch_item& it = items[i];
+        int rc = check_settings(ctx, &it.settings);
+        if (rc) return rc;
+        if (!it.scene || !it.scene->hasScene) return fail(ctx, SHIRO_HIP_NO_SCENE, "batch item without a scene");
+        if (it.scene->device != ctx->device) return fail(ctx, SHIRO_HIP_INVALID, "batch item on another device");
+        if (!it.frame.beauty || !it.frame.albedo || !it.frame.normal || !it.frame.depth)
+            return fail(ctx, SHIRO_HIP_INVALID, "batch item with a null frame buffer");
+        if (it.sampleCount == 0) continue;
+        RenderParams P{};
+        P.scene = it.scene->dev;
+        P.camera = to_dev(it.camera);
+        P.settings = to_dev(it.settings);
+        P.frame = DevFrame{it.frame.beauty, it.frame.albedo, it.frame.normal, it.frame.depth};
+        P.sampleStart = it.sampleStart;
+        P.sliceBegin = 0;
+        P.sliceCount = it.sampleCount;
+        P.batchCount = it.sampleCount;
+        P.tilesX = (it.settings.width + 7u) / 8u;
+        P.counters = nullptr;
+        if (P.scene.textured) *tex = true;
+        out->push_back(P);
+    }
+    return SHIRO_HIP_OK;
+}
+
+// Enqueues one batched render in `mode` between ev0 and ev1.
+int enqueue_many(shiro_hip_ctx* ctx, const std::vector<RenderParams>& P, bool tex, uint32_t mode) {
+    if (mode == SHIRO_HIP_MANY_PERSISTENT) {
+        // Work units (item, tile), items in decreasing estimated cost
+        // (samples per tile times BVH depth), tiles of an item contiguous.
+        std::vector<uint32_t> order(P.size());
+        for (uint32_t i = 0; i < order.size(); ++i) order[i] = i;
+        auto cost = [&](uint32_t i) {
+            return static_cast<double>(P[i].batchCount) * (4.0 + static_cast<double>(ctx->manyDepth[i]));
+        };
+        std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return cost(a) > cost(b); });
+        std::vector<uint2> work;
+        for (uint32_t i : order) {
+            const uint32_t tiles = P[i].tilesX * ((P[i].settings.height + 7u) / 8u);
+            for (uint32_t t = 0; t < tiles; ++t) work.push_back(make_uint2(i, t));
+        }
+        HIPC(upload(ctx->manyItems, P.data(), P.size(), ctx->stream), "upload batch");
+        HIPC(upload(ctx->manyWork, work.data(), work.size(), ctx->stream), "upload batch");
+        HIPC(ctx->manyCounter.reserve(256), "alloc batch counter");
+        const uint32_t blocks = std::max(1u, ctx->computeUnits * std::max(1u, ctx->tuning.batchBlocksPerCU));
+        HIPC(hipEventRecord(ctx->ev0, ctx->stream), "event");
+        HIPC(launch_render_many(ctx->manyItems.as<RenderParams>(), ctx->manyWork.as<uint2>(),
+                                static_cast<uint32_t>(work.size()), ctx->manyCounter.as<uint32_t>(), blocks, tex,
+                                ctx->stream),
+             "batched launch");
+        HIPC(hipEventRecord(ctx->ev1, ctx->stream), "event");
+        return SHIRO_HIP_OK;
+    }
+    if (mode == SHIRO_HIP_MANY_SEQUENTIAL) {
+        HIPC(hipEventRecord(ctx->ev0, ctx->stream), "event");
+        for (const RenderParams& q : P) HIPC(launch_render(q, false, ctx->stream), "render launch");
+        HIPC(hipEventRecord(ctx->ev1, ctx->stream), "event");
+        return SHIRO_HIP_OK;
+    }
+    if (mode == SHIRO_HIP_MANY_WAVEFRONT) {
+        HIPC(hipEventRecord(ctx->ev0, ctx->stream), "event");
+        for (size_t i = 0; i < P.size(); ++i) {
+            const shiro_hip_batch_item& it = *ctx->manyItemsHost[i];
+            const int rc = enqueue_wavefront(ctx, P[i].scene, it.camera, it.settings, it.sampleStart, it.sampleCount,
+                                             P[i].frame, nullptr, false);
+            if (rc) return rc;
+        }
+        HIPC(hipEventRecord(ctx->ev1, ctx->stream), "event");
+        return SHIRO_HIP_OK;
+    }
+    // Graph: capture the per-item launches, instantiate, replay.
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    HIPC(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal), "graph capture");
+    hipError_t e = hipSuccess;
+    for (const RenderParams& q : P) {
+        e = launch_render(q, false, ctx->stream);
+        if (e != hipSuccess) break;
+    }
+    const hipError_t e2 = hipStreamEndCapture(ctx->stream, &graph);
+    if (e != hipSuccess) return hip_fail(ctx, e, "graph capture");
+    if (e2 != hipSuccess) return hip_fail(ctx, e2, "graph capture");
+    e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        (void)hipGraphDestroy(graph);
+        return hip_fail(ctx, e, "graph instantiate");
+    }
+    e = hipEventRecord(ctx->ev0, ctx->stream);
+    if (e == hipSuccess) e = hipGraphLaunch(exec, ctx->stream);
+    if (e == hipSuccess) e = hipEventRecord(ctx->ev1, ctx->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(ctx->ev1);
+    (void)hipGraphExecDestroy(exec);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) return hip_fail(ctx, e, "graph launch");
+    return SHIRO_HIP_OK;
+}
+
+}  // namespace
+
+int shiro_hip_render_many(shiro_hip_ctx* ctx, const shiro_hip_batch_item* items, uint32_t count, uint32_t mode) {
+    if (!ctx) return SHIRO_HIP_INVALID;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    if (count > 0 && !items) return fail(ctx, SHIRO_HIP_INVALID, "items is null");
+    if (mode > SHIRO_HIP_MANY_WAVEFRONT) return fail(ctx, SHIRO_HIP_INVALID, "unknown batch mode");
+    HIPC(hipSetDevice(ctx->device), "set device");
+    std::vector<RenderParams> P;
+    bool tex = false;
+    int rc = many_params(ctx, items, count, &P, &tex);
+    if (rc) return rc;
+    ctx->lastMs = 0.0f;
+    if (P.empty()) return SHIRO_HIP_OK;
+    ctx->manyDepth.clear();
+    ctx->manyItemsHost.clear();
+    for (uint32_t i = 0; i < count; ++i)
+        if (items[i].sampleCount > 0) {
+            ctx->manyDepth.push_back(items[i].scene->info.bvhDepth);
+            ctx->manyItemsHost.push_back(items + i);
+        }
+    rc = enqueue_many(ctx, P, tex, mode);
+    if (rc) return rc;
+    return finish_timing(ctx);
+}
+
+int shiro_hip_autotune_many(shiro_hip_ctx* ctx, const shiro_hip_batch_item* items, uint32_t count,
+                            shiro_hip_tuning* best, char* report, size_t reportSize) {
+    if (!ctx) return SHIRO_HIP_INVALID;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    if (count > 0 && !items) return fail(ctx, SHIRO_HIP_INVALID, "items is null");
+    HIPC(hipSetDevice(ctx->device), "set device");
+    std::vector<RenderParams> P;
+    bool tex = false;
+    int rc = many_params(ctx, items, count, &P, &tex);
+    if (rc) return rc;
+    ctx->manyDepth.clear();
+    for (uint32_t i = 0; i < count; ++i)
+        if (items[i].sampleCount > 0) ctx->manyDepth.push_back(items[i].scene->info.bvhDepth);
+    const shiro_hip_tuning base = ctx->tuning;
+    shiro_hip_tuning bestT = base;
+    float bestMs = 0.0f;
+    std::string log;
+    bool first = true;
+    for (uint32_t bpc : {1u, 2u, 4u, 8u, 16u, 32u}) {
+        ctx->tuning.batchBlocksPerCU = bpc;
+        float ms = 0.0f;
+        for (int rep = 0; rep < 2 && !P.empty(); ++rep) {
+            rc = enqueue_many(ctx, P, tex, SHIRO_HIP_MANY_PERSISTENT);
+            if (!rc) rc = finish_timing(ctx);
+            if (rc) {
+                ctx->tuning = base;
+                return rc;
+            }
+            ms = ctx->lastMs;
+        }
+        char line[96];
+        std::snprintf(line, sizeof(line), "batchBlocksPerCU %u: %.3f ms\n", bpc, ms);
+        log += line;
+        if (first || ms < bestMs) {
+            bestMs = ms;
+            bestT = ctx->tuning;
+            first = false;
+        }
+    }
+    ctx->tuning = bestT;
+    if (best) *best = bestT;
+    if (report && reportSize > 0) std::snprintf(report, reportSize, "%s", log.c_str());
+    return SHIRO_HIP_OK;
 }
 
 }  // extern "C"

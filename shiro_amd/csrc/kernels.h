@@ -86,6 +86,9 @@ struct AdjointParams {
 };
 
 hipError_t launch_render(const RenderParams& P, bool count, hipStream_t stream);
+// items / work / counter in device memory; work = (item, tile) pairs.
+hipError_t launch_render_many(const RenderParams* items, const uint2* work, uint32_t workCount, uint32_t* counter,
+                              uint32_t blocks, bool tex, hipStream_t s);
 hipError_t launch_adjoint(const AdjointParams& P, hipStream_t s);
 hipError_t launch_wavefront_pass(const WfParams& P, const WfLaunch& L, hipStream_t s, StageMarks* marks);
 hipError_t launch_accum_add(const AccumParams& A, const DevFrame& f, uint32_t count, uint32_t np, hipStream_t s);

@@ -9,6 +9,8 @@
 //                  One 64-lane wave covers an 8x8 pixel tile; tiles are
 //                  dealt to the eight XCDs in contiguous ranges so that
 //                  neighbouring tiles share an L2.
+//  k_render_many   the batched form: persistent blocks take (item, tile)
+//                  units of many scenes' batches from one cost-sorted list.
 //  k_accum_*       FrameAccumulator on the device (Renderer.cpp:47-110).
 //  k_qmc_fill, k_intersect, k_occluded, k_trace_primary, k_eval_bsdf,
 //  k_env_sample, k_env_eval   single stages of the path for parity tests.
@@ -35,11 +37,9 @@ DEV uint32_t xcd_tile(uint32_t bid, uint32_t nblocks) {
     return base + slot;
 }
 
+// The megakernel's work for one 8x8 tile of one batch (one lane per pixel).
 template <bool COUNT, bool TEX>
-__global__ __launch_bounds__(64) void k_render(RenderParams P) {
-    const uint32_t nblocks = gridDim.x;
-    const uint32_t tile = xcd_tile(blockIdx.x, nblocks);
-    const uint32_t lane = threadIdx.x;
+DEV void render_tile(const RenderParams& P, uint32_t tile, uint32_t lane) {
     const uint32_t x = (tile % P.tilesX) * 8u + (lane & 7u);
     const uint32_t y = (tile / P.tilesX) * 8u + (lane >> 3);
     const DevSettings& st = P.settings;
@@ -112,6 +112,35 @@ __global__ __launch_bounds__(64) void k_render(RenderParams P) {
     nn[1] = normal.y * inv;
     nn[2] = normal.z * inv;
     P.frame.depth[pix] = depth;
+}
+
+template <bool COUNT, bool TEX>
+__global__ __launch_bounds__(64) void k_render(RenderParams P) {
+    render_tile<COUNT, TEX>(P, xcd_tile(blockIdx.x, gridDim.x), threadIdx.x);
+}
+
+// Batched multi-scene megakernel: persistent one-wave blocks take work
+// units (batch item, tile) from a list sorted by decreasing estimated cost
+// with one atomic per unit, so large units start first and the tail is
+// short.  Each unit is exactly the k_render tile of its item (all samples of
+// the batch in one pass), so the frames equal single-scene renders bit for
+// bit.
+template <bool TEX>
+__global__ __launch_bounds__(64) void k_render_many(const RenderParams* __restrict__ items,
+                                                    const uint2* __restrict__ work, uint32_t workCount,
+                                                    uint32_t* counter) {
+    for (;;) {
+        uint32_t w = 0u;
+        if (threadIdx.x == 0u) w = atomicAdd(counter, 1u);
+        w = __shfl(w, 0);
+        if (w >= workCount) return;
+        const uint2 u = work[w];
+        render_tile<false, TEX>(items[u.x], u.y, threadIdx.x);
+    }
+}
+
+__global__ void k_clear_u32(uint32_t* p, uint32_t n) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0u;
 }
 
 // FrameAccumulator::Accumulate (Renderer.cpp:60-83).
@@ -269,6 +298,18 @@ hipError_t launch_render(const RenderParams& P, bool count, hipStream_t stream) 
         } else {
             hipLaunchKernelGGL((k_render<false, false>), dim3(nb), dim3(64), 0, stream, P);
         }
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_render_many(const RenderParams* items, const uint2* work, uint32_t workCount, uint32_t* counter,
+                              uint32_t blocks, bool tex, hipStream_t s) {
+    if (workCount == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_clear_u32, dim3(1), dim3(64), 0, s, counter, 1u);
+    if (tex) {
+        hipLaunchKernelGGL(k_render_many<true>, dim3(blocks), dim3(64), 0, s, items, work, workCount, counter);
+    } else {
+        hipLaunchKernelGGL(k_render_many<false>, dim3(blocks), dim3(64), 0, s, items, work, workCount, counter);
     }
     return hipGetLastError();
 }

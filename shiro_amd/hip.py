@@ -92,6 +92,8 @@ def load_library(path: str = LIB_PATH):
     d("shiro_hip_get_stage_times", C.c_int, V, _P(abi.StageTimes))
     d("shiro_hip_update_texture", C.c_int, V, _u, _fp, _u)
     d("shiro_hip_render_adjoint", C.c_int, V, _P(abi.Camera), _P(abi.Settings), _u, _u, _fp, _P(_fp), _u)
+    d("shiro_hip_render_many", C.c_int, V, _P(abi.BatchItem), _u, _u)
+    d("shiro_hip_autotune_many", C.c_int, V, _P(abi.BatchItem), _u, _P(abi.Tuning), C.c_char_p, C.c_size_t)
     _lib = L
     return L
 
@@ -319,6 +321,36 @@ class Context:
         self._check(self.L.shiro_hip_count_fetches(self.h, C.byref(cam), C.byref(st), sample_start, sample_count,
                                                    C.byref(out)))
         return out
+
+    # ---- batched multi-scene rendering ----
+    @staticmethod
+    def batch_items(entries):
+        """entries: (scene Context, Camera, RenderSettings, frame_ptrs,
+        sample_start, sample_count) tuples -> a ctypes BatchItem array."""
+        arr = (abi.BatchItem * max(1, len(entries)))()
+        for i, (sctx, cam, st, ptrs, start, count) in enumerate(entries):
+            it = arr[i]
+            it.scene = sctx.h.value
+            it.camera = cam.to_c()
+            it.settings = st.to_c()
+            it.sampleStart = start
+            it.sampleCount = st.samplesPerPixel if count is None else count
+            it.frame.beauty, it.frame.albedo, it.frame.normal, it.frame.depth = [
+                C.cast(C.c_void_p(p), _fp) for p in ptrs]
+            it.frame.width, it.frame.height = st.width, st.height
+        return arr
+
+    def render_many(self, items, mode=abi.MANY_PERSISTENT):
+        """Renders a BatchItem array (see batch_items) on this context's
+        stream; returns the kernel milliseconds."""
+        self._check(self.L.shiro_hip_render_many(self.h, items, len(items), mode))
+        return self.last_render_ms()
+
+    def autotune_many(self, items):
+        best = abi.Tuning()
+        buf = C.create_string_buffer(2048)
+        self._check(self.L.shiro_hip_autotune_many(self.h, items, len(items), C.byref(best), buf, len(buf)))
+        return best, buf.value.decode()
 
     # ---- texture extension ----
     def update_texture(self, index, texels, eval_only=False):

@@ -19,6 +19,10 @@ Workloads (sizes exact):
       geodesic spheres (12,500 tris) + 250 displaced boxes (7,500 tris) =
       5,000,000 triangles at random transforms in a 100-unit cube; 40% diffuse,
       30% GGX metal, 20% dielectric, 10% coated; c2's dome + one directional.
+  c4  32 scenes in one batch (batched multi-scene rendering): square
+      frames of side U[256, 1024], spp in {4, 16, 64}, triangles log-uniform
+      in [10k, 500k] (a value-noise heightfield plus 1,280-triangle spheres,
+      c2's standard_surface mix), each with its own 1024x512 dome; depth 8.
   c5  256x256, 32 spp, depth 4 (texture extension): UV-mapped geodesic
       sphere (frequency 50, 50,000 tris) on a 2-triangle ground; the sphere's
       base colour (128x128x3) and roughness (128x128x1) are textures; a
@@ -390,6 +394,60 @@ def c3(scale: float = 1.0, spp: int = 16, seed: int = BASE_SEED, objects: int = 
     h = max(8, int(round(2160 * scale)))
     cam = Camera.look_at((0.0, 40.0, 140.0), (0.0, 0.0, 0.0), fov=45.0, aspect=w / h)
     return Workload("c3", scene, cam, _settings(w, h, spp, 16, rr=3))
+
+
+def c4_scene(index: int, seed: int = BASE_SEED, env_size=(1024, 512), scale: float = 1.0,
+             spp_scale: float = 1.0, tri_scale: float = 1.0) -> Workload:
+    """Scene `index` of the c4 batch (parameters drawn from a per-index
+    seed); the scales shrink frames, spp and triangle counts for tests."""
+    rng = np.random.default_rng(seed + 4000 + index)
+    side = int(rng.integers(256, 1025))
+    spp = int(rng.choice([4, 16, 64]))
+    tris = int(np.exp(rng.uniform(np.log(10_000), np.log(500_000))) * tri_scale)
+    noise = ValueNoise3(seed + 4100 + index)
+    sphere_tris = 1280
+    nsph = int(min(64, max(1, 0.3 * tris // sphere_tris)))
+    quads = max(8, (tris - nsph * sphere_tris) // 2)
+    nx = max(2, int(np.sqrt(quads * 1.25)))
+    nz = max(2, quads // nx)
+    xs = np.linspace(-50, 50, nx + 1)
+    zs = np.linspace(-40, 40, nz + 1)
+    gx, gz = np.meshgrid(xs, zs, indexing="ij")
+    pts = np.stack([gx.reshape(-1), np.zeros(gx.size), gz.reshape(-1)], 1)
+    pts[:, 1] = 3.0 * noise.fbm(pts * 0.06, 4)
+    qi, qj = np.meshgrid(np.arange(nx), np.arange(nz), indexing="ij")
+    qi, qj = qi.reshape(-1), qj.reshape(-1)
+    a = qi * (nz + 1) + qj
+    c = (qi + 1) * (nz + 1) + qj + 1
+    hf_idx = np.concatenate([np.stack([a, a + 1, c], 1), np.stack([a, c, c - 1], 1)]).reshape(-1)
+
+    def mat():
+        return standard_surface(base_color=tuple(rng.uniform(0, 1, 3)), metalness=float(rng.integers(0, 2)),
+                                specular_roughness=float(rng.uniform(0.05, 1.0)))
+
+    mats = [mat()]
+    meshes = [make_mesh(pts, hf_idx, 0)]
+    sp, si = geodesic_sphere(8)
+    for k in range(nsph):
+        rad = rng.uniform(1.0, 3.0)
+        cx, cz = rng.uniform(-40, 40), rng.uniform(-30, 30)
+        cy = 3.0 * noise.fbm(np.array([[cx, 0.0, cz]]) * 0.06, 4)[0] + rad * 0.8
+        mats.append(mat())
+        meshes.append(make_mesh(sp * rad + np.array([cx, cy, cz]), si, k + 1))
+    scene = Scene(materials=mats, meshes=meshes)
+    ew, eh = env_size
+    scene.domeLights.append(DomeLight(pixels=procedural_dome(ew, eh, seed + 4200 + index)))
+    w = h = max(8, int(round(side * scale)))
+    spp = max(1, int(round(spp * spp_scale)))
+    az = rng.uniform(0, 2 * np.pi)
+    eye = (60.0 * np.sin(az), 25.0, 60.0 * np.cos(az))
+    cam = Camera.look_at(eye, (0.0, 0.0, 0.0), fov=45.0, aspect=1.0)
+    return Workload(f"c4[{index}]", scene, cam, _settings(w, h, spp, 8))
+
+
+def c4(count: int = 32, seed: int = BASE_SEED, **kw):
+    """The c4 batch: `count` scenes (see c4_scene)."""
+    return [c4_scene(i, seed, **kw) for i in range(count)]
 
 
 def sphere_uvs(p: np.ndarray) -> np.ndarray:
