@@ -148,7 +148,8 @@ __global__ __launch_bounds__(64) void k_adjoint(AdjointParams P) {
         const V3 Tin = ps.T;
         Surface sf;
         TexState ts;
-        const bool lit = shade_hit<true>(S, depth, smp, ray, h, ps, sf, &ts);
+        V3 emitted;
+        const bool lit = shade_hit<true>(S, depth, smp, ray, h, ps, sf, &ts, &emitted);
         V.T = ps.T;
         V.att = ratio(ps.T, Tin);
         V.C = V3{0.0f, 0.0f, 0.0f};
@@ -158,9 +159,7 @@ __global__ __launch_bounds__(64) void k_adjoint(AdjointParams P) {
         }
         const Mat& M = sf.M;
         const bool textured = (S.derived[sf.material].flags & kMatTextured) != 0u;
-        const uint32_t prevType = ps.bits & kBitsTypeMask;
-        if ((M.d->flags & kMatEmissive) && (depth == 0 || (ps.bits & kBitsPrevDelta) || prevType == kBounceTransmission))
-            V.C = ld3(M.d->emission);
+        V.C = emitted;
         // Light samples: forward sum and the gradient of each visible one.
         DirectSum ds;
         ds.init(st);
@@ -171,8 +170,9 @@ __global__ __launch_bounds__(64) void k_adjoint(AdjointParams P) {
         for (uint32_t j = 0; j < nLight; ++j) {
             V3 ld, contrib, f, lf;
             uint32_t group;
-            if (!direct_sample(S, st, sf, M, smp, depth, j, &ld, &contrib, &group, cnt, &f, &lf)) continue;
-            const bool visible = !occluded(S, origin, ld, kFltMax, cnt);
+            float maxDist;
+            if (!direct_sample(S, st, sf, M, smp, depth, j, &ld, &contrib, &group, &maxDist, cnt, &f, &lf)) continue;
+            const bool visible = !occluded(S, origin, ld, maxDist, cnt);
             ds.add(group, contrib, visible);
             if (visible && textured) {
                 const V3 gc = group == 0u ? aT : aT / ds.K;

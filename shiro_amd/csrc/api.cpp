@@ -85,7 +85,7 @@ struct shiro_hip_ctx {
     // Scene.
     bool hasScene = false;
     bool emissive = false;
-    DevBuf nodes, tris, hitdata, materials, derived, uvs;
+    DevBuf nodes, tris, hitdata, materials, derived, uvs, lightList, lightCdf;
     EnvBuffers env[kMaxDomes];
     std::vector<std::unique_ptr<TexBuffers>> tex;
     DevBuf texTable;
@@ -214,7 +214,10 @@ int enqueue_wavefront(shiro_hip_ctx* ctx, const DevScene& scene, const shiro_hip
     const uint32_t tilesX = (st.width + 7u) / 8u, tilesY = (st.height + 7u) / 8u;
     const uint64_t perSample = static_cast<uint64_t>(tilesX) * tilesY * 64u;
     const uint32_t K = st.domeLightSamples > 1u ? st.domeLightSamples : 1u;
-    const uint32_t M = std::max(1u, scene.dirCount + scene.domeCount * K);
+    // Light samples per hit: the device's direct_sample_count (dev_path.h).
+    const uint32_t lights = scene.dirCount + scene.domeCount * K +
+                            (scene.emissiveCount > 0u && scene.emissiveSum > 0.0f ? 1u : 0u);
+    const uint32_t M = std::max(1u, lights);
     const uint64_t bytesPerPath = 10u * 16u + 12u + static_cast<uint64_t>(M) * 32u;
     uint64_t S = std::max<uint64_t>(1, ctx->tuning.pathsPerPass / perSample);
     S = std::min<uint64_t>(S, std::max<uint64_t>(1, kWavefrontStateBytes / (bytesPerPath * perSample)));
@@ -471,11 +474,6 @@ int shiro_hip_supports_scene(const shiro_hip_scene* s) {
     if (!s) return 0;
     if (s->distantLightCount > static_cast<uint32_t>(kMaxDirLights)) return 0;
     if (s->domeLightCount > static_cast<uint32_t>(kMaxDomes)) return 0;
-    // Emissive-surface lights are not sampled on the device yet.
-    for (uint32_t i = 0; i < s->meshCount; ++i) {
-        const uint32_t m = s->meshes[i].materialIndex;
-        if (m < s->materialCount && s->materials[m].emissionStrength > 0.0f && s->meshes[i].indexCount >= 3) return 0;
-    }
     return 1;
 }
 
@@ -494,6 +492,13 @@ int shiro_hip_upload_scene(shiro_hip_ctx* ctx, const shiro_hip_scene* s) {
     if (!build_bvh4(flat.tri.data(), nt, kMaxBvhDepth, &bvh, &err)) return fail(ctx, SHIRO_HIP_INVALID, err);
     if (!validate_bvh4(bvh, flat.tri.data(), nt, kMaxBvhDepth, &err)) return fail(ctx, SHIRO_HIP_INVALID, "BVH check: " + err);
     const auto t1 = std::chrono::steady_clock::now();
+    // Emissive-triangle lights and the triangle -> light map.
+    std::vector<EmissiveEntry> lights;
+    float lightSum = 0.0f;
+    build_emissive(*s, &lights, &lightSum);
+    std::vector<uint32_t> lightOf(nt, 0u);
+    for (size_t i = 0; i < lights.size(); ++i)
+        if (lights[i].tri < nt) lightOf[lights[i].tri] = static_cast<uint32_t>(i + 1);
     // Triangles and hit data in leaf order.
     std::vector<float> tris(nt * 12), hit(nt * 12);
     for (size_t slot = 0; slot < nt; ++slot) {
@@ -513,7 +518,7 @@ int shiro_hip_upload_scene(shiro_hip_ctx* ctx, const shiro_hip_scene* s) {
         t[11] = 0.0f;
         std::memcpy(h + 3, &flat.material[g], 4);
         std::memcpy(h + 7, &flags, 4);
-        h[11] = 0.0f;
+        std::memcpy(h + 11, &lightOf[g], 4);
     }
     // Materials (+ the default material used for out-of-range indices).
     std::vector<shiro_hip_material> mats(s->materials, s->materials + s->materialCount);
@@ -526,6 +531,26 @@ int shiro_hip_upload_scene(shiro_hip_ctx* ctx, const shiro_hip_scene* s) {
     HIPC(upload(ctx->hitdata, hit.data(), hit.size(), ctx->stream), "upload hit data");
     HIPC(upload(ctx->materials, mats.data(), mats.size(), ctx->stream), "upload materials");
     HIPC(upload(ctx->derived, der.data(), der.size(), ctx->stream), "upload materials");
+    if (!lights.empty()) {
+        std::vector<float> L(lights.size() * 20), cdf(lights.size());
+        for (size_t i = 0; i < lights.size(); ++i) {
+            const EmissiveEntry& e = lights[i];
+            float* q = &L[i * 20];
+            std::memcpy(q, e.p, 12);
+            q[3] = e.area;
+            std::memcpy(q + 4, e.p + 3, 12);
+            q[7] = e.weight;
+            std::memcpy(q + 8, e.p + 6, 12);
+            std::memcpy(q + 11, &e.tri, 4);
+            std::memcpy(q + 12, e.normal, 12);
+            q[15] = 0.0f;
+            std::memcpy(q + 16, e.radiance, 12);
+            q[19] = 0.0f;
+            cdf[i] = e.cumulative;
+        }
+        HIPC(upload(ctx->lightList, L.data(), L.size(), ctx->stream), "upload lights");
+        HIPC(upload(ctx->lightCdf, cdf.data(), cdf.size(), ctx->stream), "upload lights");
+    }
     if (flat.hasUvs) {
         std::vector<float> uv(nt * 6);
         for (size_t slot = 0; slot < nt; ++slot)
@@ -547,6 +572,10 @@ int shiro_hip_upload_scene(shiro_hip_ctx* ctx, const shiro_hip_scene* s) {
     if (!table.empty()) HIPC(upload(ctx->texTable, table.data(), table.size(), ctx->stream), "upload textures");
     DevScene d{};
     d.uvs = flat.hasUvs ? ctx->uvs.as<float2>() : nullptr;
+    d.emissive = lights.empty() ? nullptr : ctx->lightList.as<float4>();
+    d.emissiveCdf = lights.empty() ? nullptr : ctx->lightCdf.as<float>();
+    d.emissiveCount = static_cast<uint32_t>(lights.size());
+    d.emissiveSum = lightSum;
     d.textures = table.empty() ? nullptr : ctx->texTable.as<DevTexture>();
     d.textureCount = s->textureCount;
     d.textured = flat.textured ? 1u : 0u;
@@ -560,6 +589,7 @@ int shiro_hip_upload_scene(shiro_hip_ctx* ctx, const shiro_hip_scene* s) {
     d.materialCount = static_cast<uint32_t>(mats.size());
     uint64_t bytes = ctx->nodes.bytes + ctx->tris.bytes + ctx->hitdata.bytes + ctx->materials.bytes + ctx->derived.bytes;
     if (flat.hasUvs) bytes += ctx->uvs.bytes;
+    if (!lights.empty()) bytes += ctx->lightList.bytes + ctx->lightCdf.bytes;
     for (const auto& b : ctx->tex) bytes += b->eval.bytes;
     // Lights.
     d.dirCount = s->distantLightCount;

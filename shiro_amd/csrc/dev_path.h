@@ -115,6 +115,8 @@ DEV Ray camera_ray(const DevCamera& c, const DevSettings& st, uint32_t x, uint32
 struct SurfaceHit {
     V3 p, ng, ns;
     uint32_t material;
+    uint32_t prim;    // global triangle index
+    uint32_t light;   // emissive-list index + 1, 0 if the triangle is not a light
 };
 
 DEV SurfaceHit fetch_hit(const DevScene& S, const Ray& ray, const HitRec& h) {
@@ -126,6 +128,8 @@ DEV SurfaceHit fetch_hit(const DevScene& S, const Ray& ray, const HitRec& h) {
     const float4* hd = S.hitdata + static_cast<size_t>(h.slot) * 3u;
     const float4 h0 = hd[0], h1 = hd[1], h2 = hd[2];
     s.material = __float_as_uint(h0.w);
+    s.prim = __float_as_uint(tp[0].w);
+    s.light = __float_as_uint(h2.w);
     s.ns = s.ng;
     if (__float_as_uint(h1.w) & kTriHasNormals) {
         const float b0 = 1.0f - h.b1 - h.b2;
@@ -208,11 +212,59 @@ DEV void handle_miss(const DevScene& S, const DevSettings& st, uint32_t depth, P
     }
 }
 
+// Emissive-triangle lights (CpuPathTracer.cpp:1443-1495, 1578-1628): entry
+// i of the list is S.emissive[5 i .. 5 i + 4].
+struct EmissiveLight {
+    V3 p0, p1, p2, n, radiance;
+    float area, weight;
+    uint32_t prim;
+};
+
+DEV EmissiveLight load_light(const DevScene& S, uint32_t i) {
+    const float4* q = S.emissive + static_cast<size_t>(i) * 5u;
+    const float4 a = q[0], b = q[1], c = q[2], d = q[3], e = q[4];
+    return EmissiveLight{xyz(a), xyz(b), xyz(c), xyz(d), xyz(e), a.w, b.w, __float_as_uint(c.w)};
+}
+
+// PickEmissiveTriangle: the first entry whose cumulative weight reaches
+// clamp(u, 0, 1 - 2^-24) * sum (lower_bound), the last one if none does.
+DEV uint32_t pick_light(const DevScene& S, float u) {
+    const float target = clampf(u, 0.0f, 0.99999994f) * S.emissiveSum;
+    uint32_t lo = 0, n = S.emissiveCount;
+    while (n > 0) {
+        const uint32_t half = n >> 1;
+        if (S.emissiveCdf[lo + half] < target) {
+            lo += half + 1u;
+            n -= half + 1u;
+        } else {
+            n = half;
+        }
+    }
+    return lo < S.emissiveCount ? lo : S.emissiveCount - 1u;
+}
+
+// EmissiveLightPdf: solid-angle pdf of reaching `pos` on light `light`
+// (list index + 1) from `origin`; 0 for triangles that are not lights.
+DEV float emissive_pdf(const DevScene& S, uint32_t light, V3 origin, V3 pos) {
+    if (light == 0u || S.emissiveSum <= 0.0f) return 0.0f;
+    const EmissiveLight e = load_light(S, light - 1u);
+    if (e.area <= 0.0f) return 0.0f;
+    const V3 d = pos - origin;
+    const float d2 = dot(d, d);
+    if (d2 <= 1.0e-10f) return 0.0f;
+    const V3 ld = normalize(d);
+    const float lc = fmaxs(fabsf(dot(e.n, ld * -1.0f)), 1.0e-6f);
+    const float sel = e.weight / S.emissiveSum;
+    return sel / e.area * d2 / lc;
+}
+
 // Shading frame at a closest hit.
 struct Surface {
     V3 p, ng, n, cn, v;
     float t;
     uint32_t material;
+    uint32_t prim;
+
     V2 uv;            // interpolated UV (textured materials only)
     Mat M;
 };
@@ -222,10 +274,11 @@ struct Surface {
 // Returns false when the opacity test lets the ray pass through (the ray is
 // advanced and the bounce ends without lighting, as the reference's
 // `continue`).  TEX: the scene has textured materials; their per-hit
-// constants are derived into *ts, which sf.M then points to.
+// constants are derived into *ts, which sf.M then points to.  `emitted`
+// (optional) receives the radiance added per unit of throughput.
 template <bool TEX>
 DEV bool shade_hit(const DevScene& S, uint32_t depth, const Sampler& smp, const Ray& ray, const HitRec& h,
-                   PathState& ps, Surface& sf, TexState* ts) {
+                   PathState& ps, Surface& sf, TexState* ts, V3* emitted = nullptr) {
     const SurfaceHit hit = fetch_hit(S, ray, h);
     if (ps.bits & kBitsMediumOn) {
         ps.T = ps.T * V3{expf(-ps.ext.x * h.t), expf(-ps.ext.y * h.t), expf(-ps.ext.z * h.t)};
@@ -262,13 +315,21 @@ DEV bool shade_hit(const DevScene& S, uint32_t depth, const Sampler& smp, const 
         ps.normal = n * 0.5f + V3{0.5f, 0.5f, 0.5f};
         ps.depth = h.t;
     }
+    if (emitted) *emitted = V3{0.0f, 0.0f, 0.0f};
     if (md.flags & kMatEmissive) {
-        // Emission seen directly, through a delta or transmission bounce
-        // (CpuPathTracer.cpp:1728-1748); other cases need emissive-surface
-        // lights, which the uploader reports as unsupported.
+        // Emission (CpuPathTracer.cpp:1728-1748): in full when seen directly
+        // or through a delta or transmission bounce, otherwise MIS-weighted
+        // against the emissive-triangle light sampling.
         const uint32_t prevType = ps.bits & kBitsTypeMask;
+        const V3 Le = ld3(md.emission);
         if (depth == 0 || (ps.bits & kBitsPrevDelta) || prevType == kBounceTransmission) {
-            ps.L = ps.L + ps.T * ld3(md.emission);
+            ps.L = ps.L + ps.T * Le;
+            if (emitted) *emitted = Le;
+        } else {
+            const float lp = emissive_pdf(S, hit.light, ray.o, hit.p);
+            const float w = lp > 0.0f ? power_heuristic(ps.prevPdf, lp) : 1.0f;
+            ps.L = ps.L + ps.T * Le * w;
+            if (emitted) *emitted = Le * w;
         }
     }
     sf.p = hit.p;
@@ -278,25 +339,30 @@ DEV bool shade_hit(const DevScene& S, uint32_t depth, const Sampler& smp, const 
     sf.v = ray.d * -1.0f;
     sf.t = h.t;
     sf.material = hit.material;
+    sf.prim = hit.prim;
     return true;
 }
 
 // Number of light samples of DirectLighting at one hit: each directional
-// light once, each dome domeLightSamples times (CpuPathTracer.cpp:1529-1576).
+// light once, each dome domeLightSamples times, one emissive triangle
+// (CpuPathTracer.cpp:1529-1631).  The wavefront's shadow-record capacity
+// per path (enqueue_wavefront) uses the same count.
 DEV uint32_t direct_sample_count(const DevScene& S, const DevSettings& st) {
     const uint32_t K = st.domeLightSamples > 1u ? st.domeLightSamples : 1u;
-    return S.dirCount + S.domeCount * K;
+    return S.dirCount + S.domeCount * K + (S.emissiveCount > 0u && S.emissiveSum > 0.0f ? 1u : 0u);
 }
 
-// Light sample j of DirectLighting: the shadow ray (from p + n*eps) and the
-// contribution it carries if unoccluded.  group 0 = directional light,
-// 1 + i = dome i.  Returns false when the sample needs no shadow ray.
-// `fOut` / `lfOut` (optional) receive the BSDF value and the light factor,
+// Light sample j of DirectLighting: the shadow ray (from p + n*eps, up to
+// *maxDist) and the contribution it carries if unoccluded.  group 0 =
+// directional light or emissive triangle (added as is), 1 + i = dome i.
+// Returns false when the sample needs no shadow ray.  `fOut` / `lfOut`
+// (optional) receive the BSDF value and the light factor,
 // contrib = clamp_max_component(f * lf, 24).
 template <bool COUNT>
 DEV bool direct_sample(const DevScene& S, const DevSettings& st, const Surface& sf, const Mat& M, const Sampler& smp,
-                       uint32_t depth, uint32_t j, V3* dirOut, V3* contrib, uint32_t* group, Counters<COUNT>& c,
-                       V3* fOut = nullptr, V3* lfOut = nullptr) {
+                       uint32_t depth, uint32_t j, V3* dirOut, V3* contrib, uint32_t* group, float* maxDist,
+                       Counters<COUNT>& c, V3* fOut = nullptr, V3* lfOut = nullptr) {
+    *maxDist = kFltMax;
     if (j < S.dirCount) {
         const DevDirLight& dl = S.dirs[j];
         const V3 ld = normalize(ld3(dl.direction) * -1.0f);
@@ -312,6 +378,40 @@ DEV bool direct_sample(const DevScene& S, const DevSettings& st, const Surface& 
     }
     const uint32_t K = st.domeLightSamples > 1u ? st.domeLightSamples : 1u;
     const uint32_t jj = j - S.dirCount;
+    if (jj >= S.domeCount * K) {
+        // Emissive triangle (CpuPathTracer.cpp:1578-1628).
+        const uint32_t base = 8192u + depth * 8u;
+        const EmissiveLight e = load_light(S, pick_light(S, smp.next1(base)));
+        if (e.prim == sf.prim) return false;
+        const V2 u = smp.next2(base + 1u);
+        const float su = sqrtf(clampf(u.x, 0.0f, 1.0f));
+        const float b0 = 1.0f - su;
+        const float b1 = clampf(u.y, 0.0f, 1.0f) * su;
+        const float b2 = 1.0f - b0 - b1;
+        const V3 lpos = e.p0 * b0 + e.p1 * b1 + e.p2 * b2;
+        const V3 tl = lpos - sf.p;
+        const float d2 = dot(tl, tl);
+        if (!(d2 > 1.0e-10f)) return false;
+        const float dist = sqrtf(d2);
+        const V3 ld = tl / dist;
+        const float nl = clampf(dot(sf.n, ld), 0.0f, 1.0f);
+        const float lc = fabsf(dot(e.n, ld * -1.0f));
+        if (nl <= 0.0f || lc <= 0.0f) return false;
+        const float sel = e.weight / S.emissiveSum;
+        const float pa = sel / fmaxs(e.area, 1.0e-6f);
+        const float lp = pa * d2 / fmaxs(lc, 1.0e-6f);
+        if (!(lp > 0.0f)) return false;
+        const float bp = bsdf_pdf(M, sf.n, sf.cn, sf.v, ld);
+        const float w = power_heuristic(lp, bp);
+        const V3 f = bsdf_eval(M, sf.n, sf.cn, sf.v, ld);
+        *dirOut = ld;
+        *contrib = clamp_max_component(f * e.radiance * (nl * w / lp), kDirectClamp);
+        if (fOut) *fOut = f;
+        if (lfOut) *lfOut = e.radiance * (nl * w / lp);
+        *group = 0u;
+        *maxDist = fmaxs(dist - 2.0f * kRayEps, kRayEps);
+        return true;
+    }
     const uint32_t li = jj / K, k = jj - li * K;
     V3 ld, lr;
     float lp = 0.0f;
@@ -349,6 +449,7 @@ struct DirectSum {
     }
     DEV void add(uint32_t group, V3 c, bool visible) {
         if (group == 0u) {
+            flush();
             if (visible) L = L + c;
             return;
         }
@@ -557,8 +658,9 @@ DEV PathState trace_path(const DevScene& S, const DevSettings& st, const DevCame
         for (uint32_t j = 0; j < nLight; ++j) {
             V3 ld, contrib;
             uint32_t group;
-            if (!direct_sample(S, st, sf, M, smp, depth, j, &ld, &contrib, &group, c)) continue;
-            ds.add(group, contrib, !occluded(S, origin, ld, kFltMax, c));
+            float maxDist;
+            if (!direct_sample(S, st, sf, M, smp, depth, j, &ld, &contrib, &group, &maxDist, c)) continue;
+            ds.add(group, contrib, !occluded(S, origin, ld, maxDist, c));
         }
         ds.flush();
         ps.L = ps.L + ps.T * ds.L;

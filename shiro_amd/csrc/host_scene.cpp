@@ -198,6 +198,49 @@ bool build_env_tables(const float* px, uint32_t w, uint32_t h, uint32_t layout, 
     return true;
 }
 
+void build_emissive(const shiro_hip_scene& s, std::vector<EmissiveEntry>* out, float* sum) {
+    out->clear();
+    *sum = 0.0f;
+    uint32_t base = 0;  // global index of the mesh's first triangle
+    for (uint32_t mi = 0; mi < s.meshCount; ++mi) {
+        const shiro_hip_mesh& m = s.meshes[mi];
+        const uint32_t nt = m.indexCount / 3u;
+        const uint32_t first = base;
+        base += nt;
+        if (m.indexCount < 3u || m.indexCount % 3u != 0u || !m.positions || !m.indices) continue;
+        const shiro_hip_material mat = m.materialIndex < s.materialCount ? s.materials[m.materialIndex] : default_material();
+        const H3 rad{std::max(mat.emissionColor[0] * mat.emissionStrength, 0.0f),
+                     std::max(mat.emissionColor[1] * mat.emissionStrength, 0.0f),
+                     std::max(mat.emissionColor[2] * mat.emissionStrength, 0.0f)};
+        const float rw = std::max(0.2126f * rad.x + 0.7152f * rad.y + 0.0722f * rad.z, 0.0f);
+        if (rw <= 0.0f) continue;
+        for (uint32_t p = 0; p < nt; ++p) {
+            const uint32_t* ix = m.indices + 3u * p;
+            if (ix[0] >= m.vertexCount || ix[1] >= m.vertexCount || ix[2] >= m.vertexCount) continue;
+            const H3 p0 = h3(m.positions + 3u * ix[0]), p1 = h3(m.positions + 3u * ix[1]),
+                     p2 = h3(m.positions + 3u * ix[2]);
+            const H3 e1 = p1 - p0, e2 = p2 - p0;
+            const H3 av{e1.y * e2.z - e1.z * e2.y, e1.z * e2.x - e1.x * e2.z, e1.x * e2.y - e1.y * e2.x};
+            const float twice = std::sqrt(av.x * av.x + av.y * av.y + av.z * av.z);
+            if (twice <= 1.0e-8f) continue;
+            EmissiveEntry e;
+            e.tri = first + p;
+            e.area = twice * 0.5f;
+            e.weight = e.area * rw;
+            *sum += e.weight;
+            e.cumulative = *sum;
+            e.normal[0] = av.x / twice;
+            e.normal[1] = av.y / twice;
+            e.normal[2] = av.z / twice;
+            st3(e.radiance, rad);
+            st3(e.p, p0);
+            st3(e.p + 3, p1);
+            st3(e.p + 6, p2);
+            out->push_back(e);
+        }
+    }
+}
+
 bool flatten_scene(const shiro_hip_scene& s, FlatScene* out, std::string* err) {
     *out = FlatScene();
     if (s.materialCount > 0 && !s.materials) {

@@ -37,6 +37,19 @@ struct FlatScene {
     bool emissive = false;          // any emissive material in use
 };
 
+// Emissive-triangle light list of BuildAccelerationScene
+// (CpuPathTracer.cpp:482-522): meshes in scene order whose index count is a
+// multiple of 3 and whose material emits (luminance of emissionColor *
+// strength > 0), their triangles of non-zero area in order; weight = area *
+// luminance, cumulative = running fp32 sum of the weights.
+struct EmissiveEntry {
+    uint32_t tri;          // global triangle index (scene order)
+    float area, weight, cumulative;
+    float normal[3], radiance[3];
+    float p[9];            // vertices
+};
+void build_emissive(const shiro_hip_scene& s, std::vector<EmissiveEntry>* out, float* sum);
+
 // Validates and flattens; returns false with a message on invalid input
 // (index out of range, null arrays, too many lights, a material naming a
 // missing texture, an empty texture).

@@ -17,8 +17,11 @@
 //              {v2.xyz, 0}
 //            read only by the watertight test during traversal.
 //   hitdata  48 B per triangle in leaf order = 3 x float4:
-//              {n0.xyz, material index bits}, {n1.xyz, flags}, {n2.xyz, 0}
+//              {n0.xyz, material index bits}, {n1.xyz, flags},
+//              {n2.xyz, emissive-list index + 1 bits (0: not a light)}
 //            read once per closest hit (shading normal + material).
+//   emissive the emissive-triangle light list (80 B per entry) and its
+//            cumulative weights, for light sampling and MIS at hits.
 //   materials  shiro_hip_material array (276 B each) + a derived table
 //              (MaterialDerived) precomputed on the host.
 //   uvs      optional, 24 B per triangle in leaf order (3 x float2), read
@@ -119,6 +122,10 @@ struct DevScene {
     const shiro_hip_material* materials;
     const MaterialDerived* derived;
     const float2* uvs;            // 3 per triangle in leaf order, null without UVs
+    const float4* emissive;       // 5 per emissive triangle: {p0, area} {p1, weight} {p2, tri bits} {normal} {radiance}
+    const float* emissiveCdf;     // cumulative weights (emissiveCount)
+    uint32_t emissiveCount;
+    float emissiveSum;
     const DevTexture* textures;   // null without textures
     uint32_t textureCount;
     uint32_t textured;            // any material with kMatTextured

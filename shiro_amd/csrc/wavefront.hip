@@ -207,11 +207,13 @@ __global__ __launch_bounds__(64) void k_wf_trace(WfParams P, uint32_t depth) {
                 item = idx;
                 Ray ray;
                 bool valid = true;
+                float rayMax = kFltMax;
                 if (ANY) {
                     const uint32_t rid = P.B.qsr[idx];
                     const uint32_t p = rid / P.B.maxShadow;
                     ray.o = xyz(P.B.so[p]);
                     ray.d = xyz(P.B.sr[2u * static_cast<size_t>(rid)]);
+                    rayMax = P.B.sr[2u * static_cast<size_t>(rid) + 1u].w;
                     item = rid;
                 } else if (depth == 0u) {
                     const PathCoord c = path_coord(P, idx);
@@ -224,9 +226,9 @@ __global__ __launch_bounds__(64) void k_wf_trace(WfParams P, uint32_t depth) {
                 }
                 if (valid) {
                     ray.tmin = kRayEps;
-                    ray.tmax = kFltMax;
+                    ray.tmax = rayMax;
                     r = precompute(ray);
-                    tmax = kFltMax;
+                    tmax = rayMax;
                     best.t = kFltMax;
                     best.b1 = best.b2 = 0.0f;
                     best.slot = best.prim = kNoHit;
@@ -337,12 +339,17 @@ __global__ __launch_bounds__(64) void k_wf_shade(WfParams P, uint32_t depth) {
                     const Mat& M = sf.M;
                     uint32_t ns = 0;
                     float4* sr = P.B.sr + static_cast<size_t>(p) * P.B.maxShadow * 2u;
-                    for (uint32_t j = 0; j < nLight; ++j) {
+                    // nLight <= maxShadow by construction (enqueue_wavefront); the
+                    // bound keeps the records inside the path's slots regardless.
+                    for (uint32_t j = 0; j < nLight && ns < P.B.maxShadow; ++j) {
                         V3 ld, contrib;
                         uint32_t group;
-                        if (!direct_sample(P.scene, P.settings, sf, M, smp, depth, j, &ld, &contrib, &group, c)) continue;
+                        float maxDist;
+                        if (!direct_sample(P.scene, P.settings, sf, M, smp, depth, j, &ld, &contrib, &group, &maxDist,
+                                           c))
+                            continue;
                         sr[2u * ns] = make_float4(ld.x, ld.y, ld.z, __uint_as_float(group));
-                        sr[2u * ns + 1u] = make_float4(contrib.x, contrib.y, contrib.z, 0.0f);
+                        sr[2u * ns + 1u] = make_float4(contrib.x, contrib.y, contrib.z, maxDist);
                         wave_push(true, p * P.B.maxShadow + ns, P.B.qsr, &P.B.counts[P.B.shadowRayBase + depth]);
                         ++ns;
                     }

@@ -396,6 +396,29 @@ def c3(scale: float = 1.0, spp: int = 16, seed: int = BASE_SEED, objects: int = 
     return Workload("c3", scene, cam, _settings(w, h, spp, 16, rr=3))
 
 
+def emissive(scale: float = 1.0, spp: int = 16, seed: int = BASE_SEED) -> Workload:
+    """c1 with area lights: an emissive quad above the sphere (2 triangles,
+    warm white, strength 4) and a small emissive icosphere (80 triangles,
+    blue) beside it, next to c1's directional light.  Exercises the
+    emissive-triangle light list, its sampling and the MIS of emission at
+    hits."""
+    wl = c1(scale=scale, spp=spp, seed=seed)
+    lamp = usd_preview_surface((0.0, 0.0, 0.0), roughness=0.5, emissiveColor=(1.0, 0.9, 0.7))
+    lamp.emissionStrength = 4.0
+    blue = usd_preview_surface((0.1, 0.1, 0.1), roughness=0.3, emissiveColor=(0.3, 0.5, 1.0))
+    blue.emissionStrength = 2.0
+    scene = wl.scene
+    scene.materials = list(scene.materials) + [lamp, blue]
+    q = np.array([[-1.2, 3.2, -1.0], [1.2, 3.2, -1.0], [1.2, 3.4, 1.0], [-1.2, 3.4, 1.0]], np.float32)
+    qi = np.array([0, 1, 2, 0, 2, 3], np.uint32)  # facing down
+    sp, si = geodesic_sphere(2)
+    scene.meshes = list(scene.meshes) + [make_mesh(q, qi, len(scene.materials) - 2),
+                                         make_mesh(sp * 0.35 + np.array([1.8, 0.35, 0.6]), si,
+                                                   len(scene.materials) - 1)]
+    wl.name = "emissive"
+    return wl
+
+
 def c4_scene(index: int, seed: int = BASE_SEED, env_size=(1024, 512), scale: float = 1.0,
              spp_scale: float = 1.0, tri_scale: float = 1.0) -> Workload:
     """Scene `index` of the c4 batch (parameters drawn from a per-index
@@ -506,4 +529,4 @@ def c5(scale: float = 1.0, spp: int = 32, seed: int = BASE_SEED, frequency: int 
     return wl
 
 
-WORKLOADS = {"c1": c1, "c2": c2, "c3": c3, "c5": c5}
+WORKLOADS = {"c1": c1, "c2": c2, "c3": c3, "c5": c5, "emissive": emissive}

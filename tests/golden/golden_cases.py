@@ -118,6 +118,7 @@ def image_workloads():
         "c1": workloads.c1(scale=48 / 512, spp=4),
         "c2": workloads.c2(scale=40 / 1920, spp=2, env_size=(64, 32)),
         "c3": c3,
+        "emissive": workloads.emissive(scale=48 / 512, spp=4),
     }
 
 

@@ -16,6 +16,11 @@ def c1_mini(size=64, spp=8):
     return workloads.c1(scale=size / 512, spp=spp)
 
 
+def emissive_mini(size=64, spp=8):
+    """c1 with an emissive quad and an emissive sphere (area lights)."""
+    return workloads.emissive(scale=size / 512, spp=spp)
+
+
 def c2_mini(width=96, spp=8, env=(256, 128)):
     return workloads.c2(scale=width / 1920, spp=spp, env_size=env)
 

@@ -91,9 +91,13 @@ def _tri_scene(**mat):
 def test_supports_scene():
     L = load_library()
     assert L.shiro_hip_supports_scene(marshal(_tri_scene()).ptr) == 1
-    # Emissive surfaces are light sources the device does not sample yet:
-    # the renderer must fall back to the CPU backend for them.
-    assert L.shiro_hip_supports_scene(marshal(_tri_scene(emissionColor=(1, 1, 1), emissionStrength=1.0)).ptr) == 0
+    # Emissive surfaces are sampled as triangle lights on the device.
+    assert L.shiro_hip_supports_scene(marshal(_tri_scene(emissionColor=(1, 1, 1), emissionStrength=1.0)).ptr) == 1
+    # More directional or dome lights than the device holds fall back.
+    from shiro_amd.scene import DirectionalLight
+    many = _tri_scene()
+    many.distantLights = [DirectionalLight() for _ in range(9)]
+    assert L.shiro_hip_supports_scene(marshal(many).ptr) == 0
     assert L.shiro_hip_supports_scene(None) == 0
 
 

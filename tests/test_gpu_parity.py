@@ -14,7 +14,7 @@ import ctypes as C
 import numpy as np
 import pytest
 
-from helpers import c1_mini, c2_mini, c3_mini, random_rays, relmse
+from helpers import c1_mini, c2_mini, c3_mini, emissive_mini, random_rays, relmse
 from oracle.bindings import Accumulator, OracleScene, oracle_fns
 from shiro_amd.scene import Camera, DirectionalLight, PbrMaterial, RenderSettings, Scene, TriangleMesh
 
@@ -136,7 +136,7 @@ def test_env_vs_oracle(hip_ctx, c2m):
     assert np.mean(np.all(np.isclose(gc2, oc2, rtol=2e-4, atol=1e-5), axis=1)) >= 0.999
 
 
-@pytest.mark.parametrize("make", [c1_mini, c2_mini, c3_mini], ids=["c1", "c2", "c3"])
+@pytest.mark.parametrize("make", [c1_mini, c2_mini, c3_mini, emissive_mini], ids=["c1", "c2", "c3", "emissive"])
 def test_render_relmse(hip_ctx, make):
     wl = make()
     _log(f"render {wl.name} {wl.settings.width}x{wl.settings.height} {wl.settings.samplesPerPixel}spp")
@@ -255,7 +255,7 @@ def test_shiro_adapter_probe_binary():
     assert r.stdout.startswith("probe center")
 
 
-@pytest.mark.parametrize("make", [c1_mini, c2_mini, c3_mini], ids=["c1", "c2", "c3"])
+@pytest.mark.parametrize("make", [c1_mini, c2_mini, c3_mini, emissive_mini], ids=["c1", "c2", "c3", "emissive"])
 def test_wavefront_equals_megakernel(hip_ctx, make):
     """Both transports evaluate the same per-bounce functions in the same
     order and accumulate samples in order, so their frames are identical."""

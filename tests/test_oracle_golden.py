@@ -84,7 +84,7 @@ def _scene_hash(scene):
     return mg.scene_hash(scene)
 
 
-@pytest.mark.parametrize("name", ["c1", "c2", "c3"])
+@pytest.mark.parametrize("name", ["c1", "c2", "c3", "emissive"])
 @pytest.mark.parametrize("bvh", [False, True], ids=["brute", "bvh"])
 def test_images_bit_exact(name, bvh):
     z = _load(f"img_{name}.npz")
