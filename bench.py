@@ -224,6 +224,7 @@ def main():
     ap.add_argument("--no-count", action="store_true")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default wavefront tuning")
     ap.add_argument("--fetch-mode", type=int, default=None, help="force the traversal fetch mode (0, 1, 2)")
+    ap.add_argument("--paths-per-pass", type=int, default=None, help="force the wavefront paths per pass")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device 3 times (calibrates HBM byte counters)")
     args = ap.parse_args()
@@ -267,8 +268,13 @@ def main():
     npx = st.width * st.height
     tuning = None
     if args.transport == "wavefront":
-        if args.fetch_mode is not None:
-            ctx.set_tuning(fetchMode=args.fetch_mode)
+        if args.fetch_mode is not None or args.paths_per_pass is not None:
+            kw = {}
+            if args.fetch_mode is not None:
+                kw["fetchMode"] = args.fetch_mode
+            if args.paths_per_pass is not None:
+                kw["pathsPerPass"] = args.paths_per_pass
+            ctx.set_tuning(**kw)
         elif not args.no_autotune:
             trial = max(1, min(st.samplesPerPixel, (16 << 20) // npx))
             t0 = time.perf_counter()
@@ -277,7 +283,8 @@ def main():
                   flush=True)
         t = ctx.get_tuning()
         tuning = {"pathsPerPass": t.pathsPerPass, "fetchMode": t.fetchMode, "traceBlocks": t.traceBlocks,
-                  "stageBlocks": t.stageBlocks, "autotuned": args.fetch_mode is None and not args.no_autotune}
+                  "stageBlocks": t.stageBlocks,
+                  "autotuned": args.fetch_mode is None and args.paths_per_pass is None and not args.no_autotune}
     dev = torch.device("cuda:0")
     bufs = [torch.empty(npx * k, dtype=torch.float32, device=dev) for k in (4, 3, 3, 1)]
     ptrs = [b.data_ptr() for b in bufs]

@@ -148,6 +148,69 @@ struct WavePool {
     }
 };
 
+// Traversal stack of the persistent kernels.  The newest kRing entries of
+// each lane live in an LDS ring (slot k of lane i at [k * 64 + i]); older
+// entries move to a private overflow array, touched only while a lane holds
+// more than kRing entries.  Entries [0, base) are in the overflow array,
+// [base, sp) in the ring.  Any-hit traversal keeps node words only.
+constexpr int kRing = 8;
+__shared__ uint32_t gRingNode[kRing * 64];
+__shared__ float gRingT[kRing * 64];
+
+DEV uint32_t ring_slot(int i, uint32_t lane) { return static_cast<uint32_t>(i & (kRing - 1)) * 64u + lane; }
+
+template <bool KEEP_T>
+DEV void ring_push(uint32_t n, float t, uint32_t lane, uint32_t* ovNode, float* ovT, int& sp, int& base) {
+    if (sp - base == kRing) {
+        const uint32_t old = ring_slot(base, lane);
+        ovNode[base] = gRingNode[old];
+        if (KEEP_T) ovT[base] = gRingT[old];
+        ++base;
+    }
+    const uint32_t at = ring_slot(sp, lane);
+    gRingNode[at] = n;
+    if (KEEP_T) gRingT[at] = t;
+    ++sp;
+}
+
+// push_sorted (dev_bvh.h) onto the ring stack.
+template <bool KEEP_T>
+DEV uint32_t ring_push_sorted(uint32_t mask, const float tn[4], const uint32_t w[4], uint32_t lane, uint32_t* ovNode,
+                              float* ovT, int& sp, int& base) {
+    float k[4] = {(mask & 1u) ? tn[0] : kInf, (mask & 2u) ? tn[1] : kInf, (mask & 4u) ? tn[2] : kInf,
+                  (mask & 8u) ? tn[3] : kInf};
+    uint32_t v[4] = {w[0], w[1], w[2], w[3]};
+    cswap(k[0], v[0], k[1], v[1]);
+    cswap(k[2], v[2], k[3], v[3]);
+    cswap(k[0], v[0], k[2], v[2]);
+    cswap(k[1], v[1], k[3], v[3]);
+    cswap(k[1], v[1], k[2], v[2]);
+    const int nh = __popc(mask);
+    for (int i = nh - 1; i >= 1; --i) ring_push<KEEP_T>(v[i], k[i], lane, ovNode, ovT, sp, base);
+    return v[0];
+}
+
+// pop_entry (dev_bvh.h) from the ring stack.
+template <bool ANY>
+DEV uint32_t ring_pop(float tcull, uint32_t lane, const uint32_t* ovNode, const float* ovT, int& sp, int& base) {
+    while (sp > 0) {
+        --sp;
+        uint32_t n;
+        float t = 0.0f;
+        if (sp >= base) {
+            const uint32_t at = ring_slot(sp, lane);
+            n = gRingNode[at];
+            if (!ANY) t = gRingT[at];
+        } else {
+            n = ovNode[sp];
+            if (!ANY) t = ovT[sp];
+            base = sp;
+        }
+        if (ANY || t <= tcull) return n;
+    }
+    return kEmpty;
+}
+
 // Persistent traversal.  ANY = false: closest hit of ray-queue entries (all
 // paths at depth 0), result in H.  ANY = true: any hit of shadow-queue
 // entries, result in svis.  The traversal itself is the one of dev_bvh.h
@@ -175,9 +238,10 @@ __global__ __launch_bounds__(64) void k_wf_trace(WfParams P, uint32_t depth) {
         }
         return;
     }
-    uint32_t stackNode[kStackSize];
-    float stackT[kStackSize];
-    int sp = 0;
+    uint32_t ovNode[kStackSize];
+    float ovT[ANY ? 1 : kStackSize];
+    int sp = 0, base = 0;
+    const uint32_t lane = threadIdx.x;
     uint32_t cur = kEmpty;
     uint32_t item = 0;
     bool active = false, exhausted = false;
@@ -232,7 +296,7 @@ __global__ __launch_bounds__(64) void k_wf_trace(WfParams P, uint32_t depth) {
                     best.t = kFltMax;
                     best.b1 = best.b2 = 0.0f;
                     best.slot = best.prim = kNoHit;
-                    sp = 0;
+                    sp = base = 0;
                     cur = 0u;
                     active = true;
                 }
@@ -255,14 +319,14 @@ __global__ __launch_bounds__(64) void k_wf_trace(WfParams P, uint32_t depth) {
                 uint32_t w[4];
                 const uint32_t mask = node_boxes(nd, r, ANY ? tmax : best.t, tn, w);
                 if (mask) {
-                    cur = push_sorted(mask, tn, w, stackNode, stackT, sp);
+                    cur = ring_push_sorted<!ANY>(mask, tn, w, lane, ovNode, ovT, sp, base);
                 } else {
-                    cur = pop_entry<ANY>(stackNode, stackT, sp, best.t);
+                    cur = ring_pop<ANY>(best.t, lane, ovNode, ovT, sp, base);
                 }
             }
             if (leaf == kEmpty && cur != kEmpty && (cur & kLeafBit)) {
                 leaf = cur;
-                cur = pop_entry<ANY>(stackNode, stackT, sp, best.t);
+                cur = ring_pop<ANY>(best.t, lane, ovNode, ovT, sp, base);
             }
             if (!__any(leaf == kEmpty && cur != kEmpty)) break;
         }

@@ -39,6 +39,7 @@ from typing import Tuple
 
 import numpy as np
 
+from . import abi
 from .scene import (Camera, DirectionalLight, DomeLight, PbrMaterial, RenderSettings, Scene, Texture,
                     TriangleMesh, standard_surface, usd_preview_surface)
 
@@ -524,6 +525,9 @@ def c5(scale: float = 1.0, spp: int = 32, seed: int = BASE_SEED, frequency: int 
     w = h = max(8, int(round(256 * scale)))
     cam = Camera.look_at((0.0, 1.6, 3.6), (0.0, 0.9, 0.0), fov=40.0, aspect=1.0)
     wl = Workload("c5", scene, cam, _settings(w, h, spp, 4, rr=4))
+    # 65,536 pixels: the wavefront packs all 32 samples into one pass, the
+    # megakernel (one thread per pixel) would leave most of the GPU idle.
+    wl.settings.transport = abi.TRANSPORT_WAVEFRONT
     wl.textures_truth = (albedo, rough)
     wl.textures_init = (init_albedo, init_rough)
     return wl
