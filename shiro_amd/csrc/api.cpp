@@ -61,7 +61,7 @@ struct DevBuf {
 };
 
 struct EnvBuffers {
-    DevBuf texels, row, cond;
+    DevBuf texels, row, cond, rowGuide, condGuide;
 };
 
 // Texture extension: the texels, and a second copy that sampling reads
@@ -622,17 +622,30 @@ int shiro_hip_upload_scene(shiro_hip_ctx* ctx, const shiro_hip_scene* s) {
             if (table) {
                 HIPC(upload(ctx->env[i].row, row.data(), row.size(), ctx->stream), "upload env");
                 HIPC(upload(ctx->env[i].cond, cond.data(), cond.size(), ctx->stream), "upload env");
+                std::vector<uint32_t> rg(kEnvRowGuide + 1u), cg(static_cast<size_t>(L.height) * (kEnvColGuide + 1u));
+                build_cdf_guide(row.data(), 0u, L.height, kEnvRowGuide, rg.data());
+                for (uint32_t y = 0; y < L.height; ++y) {
+                    const uint32_t off = y * (L.width + 1u);
+                    build_cdf_guide(cond.data(), off, off + L.width, kEnvColGuide,
+                                    cg.data() + static_cast<size_t>(y) * (kEnvColGuide + 1u));
+                }
+                HIPC(upload(ctx->env[i].rowGuide, rg.data(), rg.size(), ctx->stream), "upload env");
+                HIPC(upload(ctx->env[i].condGuide, cg.data(), cg.size(), ctx->stream), "upload env");
             }
             DevEnv& E = d.envs[i];
             E.texels = ctx->env[i].texels.as<float4>();
             E.rowCdf = table ? ctx->env[i].row.as<float>() : nullptr;
             E.condCdf = table ? ctx->env[i].cond.as<float>() : nullptr;
+            E.rowGuide = table ? ctx->env[i].rowGuide.as<uint32_t>() : nullptr;
+            E.condGuide = table ? ctx->env[i].condGuide.as<uint32_t>() : nullptr;
             E.w = L.width;
             E.h = L.height;
             E.layout = layout;
             E.hasTable = table ? 1u : 0u;
             D.env = static_cast<int32_t>(i);
-            bytes += ctx->env[i].texels.bytes + (table ? ctx->env[i].row.bytes + ctx->env[i].cond.bytes : 0);
+            bytes += ctx->env[i].texels.bytes + (table ? ctx->env[i].row.bytes + ctx->env[i].cond.bytes +
+                                                             ctx->env[i].rowGuide.bytes + ctx->env[i].condGuide.bytes
+                                                       : 0);
         }
     }
     std::memcpy(d.envTop, s->environmentTop, 12);

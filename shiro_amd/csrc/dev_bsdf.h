@@ -162,7 +162,7 @@ DEV V3 apply_thin_film(V3 lobe, const shiro_hip_material& m, V3 n, V3 v, V3 l) {
 }
 
 // Reflective mixture pdf (CpuPathTracer.cpp:1232-1273).
-DEV float bsdf_pdf(const Mat& M, V3 n, V3 cn, V3 v, V3 l) {
+DEV_CALL float bsdf_pdf(Mat M, V3 n, V3 cn, V3 v, V3 l) {
     const float nl = clampf(dot(n, l), 0.0f, 1.0f);
     if (nl <= 0.0f) return 0.0f;
     const shiro_hip_material& m = *M.m;
@@ -180,7 +180,7 @@ DEV float bsdf_pdf(const Mat& M, V3 n, V3 cn, V3 v, V3 l) {
 }
 
 // Layered BSDF value (CpuPathTracer.cpp:1275-1354).
-DEV V3 bsdf_eval(const Mat& M, V3 n, V3 cn, V3 v, V3 l) {
+DEV_CALL V3 bsdf_eval(Mat M, V3 n, V3 cn, V3 v, V3 l) {
     const float nv = clampf(dot(n, v), 0.0f, 1.0f);
     const float nl = clampf(dot(n, l), 0.0f, 1.0f);
     if (nv <= 0.0f || nl <= 0.0f) return V3{0.0f, 0.0f, 0.0f};
@@ -323,7 +323,7 @@ DEV BsdfGrad bsdf_grad(const Mat& M, V3 n, V3 cn, V3 v, V3 l) {
 }
 
 // Rough dielectric transmission (CpuPathTracer.cpp:1136-1182).
-DEV V3 transmission_lobe(const Mat& M, V3 n, V3 cn, V3 v, V3 l, float etaI, float etaT, float r) {
+DEV_CALL V3 transmission_lobe(Mat M, V3 n, V3 cn, V3 v, V3 l, float etaI, float etaT, float r) {
     const float cwo = dot(n, v);
     const float cwi = dot(n, l);
     const float acwo = fabsf(cwo), acwi = fabsf(cwi);

@@ -75,12 +75,21 @@ DEV V3 env_eval(const DevEnv& e, V3 dir, const EnvCount<COUNT>& c) {
 // the bucket relative to begin and its mass (EnvironmentMap.cpp:315-330).
 // The bucket is clamped to the table so that a malformed table (NaN
 // weights) cannot index past it.
+// `guide` (G + 1 positions, build_cdf_guide) narrows the search to the
+// positions of the keys (g - 1) / G and (g + 2) / G around the key's
+// bucket g (one bucket of slack on each side for the rounding of key * G),
+// which bracket the key's lower_bound; the result is that of the full
+// search.
 template <bool COUNT>
 DEV uint32_t env_sample_cdf(const float* cdf, uint32_t begin, uint32_t end, float u, float* mass,
-                            const EnvCount<COUNT>& c) {
+                            const uint32_t* guide, uint32_t G, const EnvCount<COUNT>& c) {
     const float key = clampf(u, 0.0f, 0x1.fffffep-1f);
-    uint32_t first = begin + 1u;
-    uint32_t count = end - begin;
+    uint32_t g = static_cast<uint32_t>(key * static_cast<float>(G));
+    g = g < G - 1u ? g : G - 1u;
+    const uint32_t lo = guide[g > 0u ? g - 1u : 0u];
+    const uint32_t hi = guide[g + 2u < G ? g + 2u : G];
+    uint32_t first = lo;
+    uint32_t count = hi - lo;
     int steps = 0;
     while (count > 0u) {
         const uint32_t half = count >> 1;
@@ -112,9 +121,11 @@ template <bool COUNT>
 DEV V3 env_sample(const DevEnv& e, V2 u, V3* dirOut, float* pdfOut, const EnvCount<COUNT>& c) {
     if (e.layout == SHIRO_HIP_ENV_LATLONG && e.hasTable) {
         float rowMass = 0.0f, colMass = 0.0f;
-        const uint32_t row = env_sample_cdf(e.rowCdf, 0u, e.h, u.y, &rowMass, c);
+        const uint32_t row = env_sample_cdf(e.rowCdf, 0u, e.h, u.y, &rowMass, e.rowGuide, kEnvRowGuide, c);
         const uint32_t off = row * (e.w + 1u);
-        const uint32_t col = env_sample_cdf(e.condCdf, off, off + e.w, u.x, &colMass, c);
+        const uint32_t col = env_sample_cdf(e.condCdf, off, off + e.w, u.x, &colMass,
+                                            e.condGuide + static_cast<size_t>(row) * (kEnvColGuide + 1u),
+                                            kEnvColGuide, c);
         const float uu = (static_cast<float>(col) + 0.5f) / static_cast<float>(e.w);
         const float vv = (static_cast<float>(row) + 0.5f) / static_cast<float>(e.h);
         const float theta = vv * kPi;

@@ -8,6 +8,9 @@
 #include <stdint.h>
 
 #define DEV __device__ __forceinline__
+// Large functions called from several places in one kernel are kept out of
+// line so that the shading kernels stay within the instruction cache.
+#define DEV_CALL __device__ __noinline__
 
 namespace shiro_hip {
 

@@ -198,6 +198,30 @@ bool build_env_tables(const float* px, uint32_t w, uint32_t h, uint32_t layout, 
     return true;
 }
 
+void build_cdf_guide(const float* cdf, uint32_t begin, uint32_t end, uint32_t G, uint32_t* out) {
+    bool monotone = true;
+    for (uint32_t i = begin; i < end && monotone; ++i) monotone = cdf[i] <= cdf[i + 1u];
+    for (uint32_t g = 0; g <= G; ++g) {
+        if (!monotone) {
+            out[g] = g == G ? end + 1u : begin + 1u;
+            continue;
+        }
+        const float key = static_cast<float>(g) / static_cast<float>(G);
+        uint32_t first = begin + 1u, count = end - begin;
+        while (count > 0u) {
+            const uint32_t half = count >> 1;
+            const uint32_t mid = first + half;
+            if (cdf[mid] < key) {
+                first = mid + 1u;
+                count -= half + 1u;
+            } else {
+                count = half;
+            }
+        }
+        out[g] = first;
+    }
+}
+
 void build_emissive(const shiro_hip_scene& s, std::vector<EmissiveEntry>* out, float* sum) {
     out->clear();
     *sum = 0.0f;

@@ -24,6 +24,14 @@ MaterialDerived derive_material(const shiro_hip_material& m);
 bool build_env_tables(const float* pixels, uint32_t w, uint32_t h, uint32_t layout, std::vector<float>* rowCdf,
                       std::vector<float>* condCdf);
 
+// Search guides of a CDF table for the device's lower_bound: for a table
+// cdf[begin .. end] and G buckets, guide[g] = the lower_bound position of
+// key g / G over cdf[begin + 1 .. end] (end + 1 if none), g = 0..G.  A key
+// in [a / G, b / G] then has its lower_bound in [guide[a], guide[b]].  If
+// the table is not non-decreasing (NaN weights), the guide brackets the
+// whole range instead.  `out` receives G + 1 entries per table.
+void build_cdf_guide(const float* cdf, uint32_t begin, uint32_t end, uint32_t G, uint32_t* out);
+
 // Flattened triangle soup of a scene in scene order (meshes concatenated,
 // full index triples only, CpuPathTracer.cpp:434).
 struct FlatScene {

@@ -28,7 +28,7 @@
 //            only at hits on textured materials.
 //   textures optional float texels (w x h x channels, row 0 at v = 0).
 //   env      per dome: texels as float4 (RGB + pad), rowCdf (H+1),
-//            condCdf (H x (W+1)), all fp32.
+//            condCdf (H x (W+1)), all fp32; search guides of both (u32).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -51,6 +51,11 @@ constexpr uint32_t kMaxBvhDepth = (kStackSize - 1) / 3;
 
 constexpr size_t kNodeFloats = 16;   // floats per BVH4 node (64 B)
 
+// Buckets of the environment CDF search guides (row table, each column
+// table).
+constexpr uint32_t kEnvRowGuide = 256;
+constexpr uint32_t kEnvColGuide = 64;
+
 constexpr int kMaxDomes = 4;
 constexpr int kMaxDirLights = 8;
 
@@ -58,6 +63,8 @@ struct DevEnv {
     const float4* texels;   // w*h, row 0 = theta 0
     const float* rowCdf;    // h+1 (null if no table)
     const float* condCdf;   // h*(w+1)
+    const uint32_t* rowGuide;   // kEnvRowGuide + 1 search guide entries (build_cdf_guide)
+    const uint32_t* condGuide;  // kEnvColGuide + 1 per row
     uint32_t w, h;
     uint32_t layout;        // SHIRO_HIP_ENV_LATLONG or _ANGULAR
     uint32_t hasTable;
