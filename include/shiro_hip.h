@@ -499,6 +499,30 @@ typedef struct shiro_hip_stage_times {
 int shiro_hip_set_stage_timing(shiro_hip_ctx* ctx, uint32_t enable);
 int shiro_hip_get_stage_times(shiro_hip_ctx* ctx, shiro_hip_stage_times* out);
 
+/* ---- Environment image files: the pixel decoding EnvironmentMap::Load
+ * gets from OpenImageIO (reference src/render/EnvironmentMap.cpp:49-92),
+ * for hosts built without it.  Formats: PFM ("PF"/"Pf", either byte order),
+ * Radiance RGBE (flat or run-length scanlines, any [+-]Y [+-]X orientation)
+ * and single-part scanline OpenEXR (NONE/RLE/ZIPS/ZIP; HALF/FLOAT/UINT; R,G,B
+ * or Y channels).  `pixels` receives width*height RGB floats, row 0 the top
+ * of the image, with Load's channel expansion (g = r and b = g where the
+ * file lacks them): the layout shiro_hip_dome_light.pixels takes.  Needs no
+ * device. ---- */
+enum { SHIRO_HIP_IMAGE_PFM = 1, SHIRO_HIP_IMAGE_RGBE = 2, SHIRO_HIP_IMAGE_EXR = 3 };
+typedef struct shiro_hip_image {
+    float* pixels;     /* owned by the library: release with shiro_hip_free_image */
+    uint32_t width;
+    uint32_t height;
+    uint32_t channels; /* channels stored in the file */
+    uint32_t format;   /* SHIRO_HIP_IMAGE_* */
+} shiro_hip_image;
+/* Returns SHIRO_HIP_OK or SHIRO_HIP_INVALID (unreadable, malformed or
+ * unsupported file; `err` receives the reason) or SHIRO_HIP_OOM. */
+int shiro_hip_load_image(const char* path, shiro_hip_image* out, char* err, size_t errSize);
+/* The same decoding from a file image held in memory. */
+int shiro_hip_load_image_memory(const void* data, size_t size, shiro_hip_image* out, char* err, size_t errSize);
+void shiro_hip_free_image(shiro_hip_image* img);
+
 #ifdef __cplusplus
 }
 #endif

@@ -156,6 +156,15 @@ class StageTimes(C.Structure):
     ]
 
 
+class Image(C.Structure):
+    _fields_ = [
+        ("pixels", C.POINTER(C.c_float)), ("width", C.c_uint32), ("height", C.c_uint32),
+        ("channels", C.c_uint32), ("format", C.c_uint32),
+    ]
+
+
+IMAGE_PFM, IMAGE_RGBE, IMAGE_EXR = 1, 2, 3
+
 RESULT_NAMES = {
     0: "OK", 1: "NO_DEVICE", 2: "OOM", 3: "CANCELLED", 4: "INVALID",
     5: "NO_SCENE", 6: "LAUNCH",
@@ -180,6 +189,7 @@ EXPORTS = [
     "shiro_hip_set_stage_timing", "shiro_hip_get_stage_times",
     "shiro_hip_update_texture", "shiro_hip_render_adjoint",
     "shiro_hip_render_many", "shiro_hip_autotune_many",
+    "shiro_hip_load_image", "shiro_hip_load_image_memory", "shiro_hip_free_image",
 ]
 
 

@@ -94,8 +94,43 @@ def load_library(path: str = LIB_PATH):
     d("shiro_hip_render_adjoint", C.c_int, V, _P(abi.Camera), _P(abi.Settings), _u, _u, _fp, _P(_fp), _u)
     d("shiro_hip_render_many", C.c_int, V, _P(abi.BatchItem), _u, _u)
     d("shiro_hip_autotune_many", C.c_int, V, _P(abi.BatchItem), _u, _P(abi.Tuning), C.c_char_p, C.c_size_t)
+    d("shiro_hip_load_image", C.c_int, C.c_char_p, _P(abi.Image), C.c_char_p, C.c_size_t)
+    d("shiro_hip_load_image_memory", C.c_int, C.c_void_p, C.c_size_t, _P(abi.Image), C.c_char_p, C.c_size_t)
+    d("shiro_hip_free_image", None, _P(abi.Image))
     _lib = L
     return L
+
+
+def load_image(source) -> tuple:
+    """Decodes an environment image file (PFM, Radiance RGBE or scanline
+    OpenEXR) the way EnvironmentMap::Load reads it through OpenImageIO
+    (EnvironmentMap.cpp:49-92): returns ((H, W, 3) float32 RGB with row 0 the
+    top of the image, channels stored in the file, abi.IMAGE_* format).
+    `source` is a path or the file's bytes.  Needs no device."""
+    L = load_library()
+    img = abi.Image()
+    err = C.create_string_buffer(512)
+    if isinstance(source, (bytes, bytearray, memoryview)):
+        buf = bytes(source)
+        rc = L.shiro_hip_load_image_memory(buf, len(buf), C.byref(img), err, len(err))
+    else:
+        rc = L.shiro_hip_load_image(os.fsencode(source), C.byref(img), err, len(err))
+    if rc:
+        raise HipError(rc, err.value.decode(errors="replace"))
+    try:
+        n = img.width * img.height * 3
+        px = np.ctypeslib.as_array(img.pixels, shape=(n,)).copy().reshape(img.height, img.width, 3)
+        return px, int(img.channels), int(img.format)
+    finally:
+        L.shiro_hip_free_image(C.byref(img))
+
+
+def load_dome_light(path, layout: int = abi.ENV_AUTOMATIC, **kw):
+    """A DomeLight whose pixels come from an image file (the reference's
+    dome lights hold EnvironmentMap::Load(path, layout))."""
+    from .scene import DomeLight
+    px, _, _ = load_image(path)
+    return DomeLight(pixels=px, layout=layout, **kw)
 
 
 def scene_signature(scene: Scene) -> str:

@@ -109,6 +109,26 @@ def env_queries(n=4000):
     return u, d
 
 
+def angular_dome():
+    """A 48x48 angular (light-probe) dome whose basis is turned 0.6 rad about
+    +y (WorldToLight/LightToWorld, CpuPathTracer.cpp:315-325)."""
+    from shiro_amd import abi
+    from shiro_amd.scene import DomeLight
+    from shiro_amd.workloads import procedural_dome
+    c, s = float(np.cos(0.6)), float(np.sin(0.6))
+    px = procedural_dome(48, 48, 777, hot_spots=2, peak=200.0, spot_width_deg=10.0)
+    return DomeLight(radiance=(1.0, 0.9, 0.8), pixels=px, layout=abi.ENV_ANGULAR, right=(c, 0.0, -s),
+                     up=(0.0, 1.0, 0.0), forward=(s, 0.0, c))
+
+
+def angular_workload(scale=40 / 1920, spp=2):
+    """c2's geometry and materials under the angular dome."""
+    wl = workloads.c2(scale=scale, spp=spp, env_size=(16, 8))
+    wl.scene.domeLights[0] = angular_dome()
+    wl.name = "angular"
+    return wl
+
+
 def image_workloads():
     """Small c1/c2/c3 scenes.  Roulette starts at depth 2 in all of them: the
     reference hardcodes that depth (CpuPathTracer.cpp:2006)."""
@@ -119,6 +139,7 @@ def image_workloads():
         "c2": workloads.c2(scale=40 / 1920, spp=2, env_size=(64, 32)),
         "c3": c3,
         "emissive": workloads.emissive(scale=48 / 512, spp=4),
+        "angular": angular_workload(),
     }
 
 

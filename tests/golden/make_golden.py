@@ -13,13 +13,21 @@ Fixtures (numpy .npz, float32/uint32 arrays):
   env.npz       BuildImportanceTable rows/conditional CDFs, Sample and
                 Pdf/Evaluate of a 64x32 procedural lat-long map
                 (EnvironmentMap.cpp:109-236)
+  env_angular.npz  Sample / Pdf / Evaluate of a 48x48 angular (light-probe)
+                map, which has no importance table (EnvironmentMap.cpp:109-183,
+                247-264)
   img_*.npz     CpuPathTracer::RenderSampleBatch frames of small c1/c2/c3
-                scenes (brute-force intersection), plus a two-batch
-                FrameAccumulator result with the path guide active
+                scenes (brute-force intersection), of a c1 scene with emissive
+                area lights and of a c2 scene under a rotated angular dome,
+                plus a two-batch FrameAccumulator result (the second batch
+                runs with the path-guide state the first one left)
 Each image fixture stores a hash of the scene arrays it was rendered from,
 so the test detects a change of the workload generator.
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py [section ...]
+
+Sections: qmc bsdf env env_angular images accumulated, or img:<name> for
+one image fixture; all sections when none is named.
 """
 import ctypes as C
 import hashlib
@@ -51,14 +59,45 @@ def scene_hash(scene):
     for d in scene.domeLights:
         if d.pixels is not None:
             h.update(np.ascontiguousarray(d.pixels, np.float32).tobytes())
+        if d.layout != abi.ENV_AUTOMATIC:
+            h.update(np.array([d.layout, *d.right, *d.up, *d.forward], np.float32).tobytes())
     return h.hexdigest()
 
 
-def main():
+def main(sections):
     if not reference_available():
         sys.exit("oracle/_ref/libshiro_ref.so is not built (needs the reference checkout)")
     f = reference_fns()
+
+    def want(name):
+        return not sections or name in sections
+
     # --- sampler
+    if want("qmc"):
+        write_qmc(f)
+    if want("bsdf"):
+        write_bsdf(f)
+    if want("env"):
+        px_env = procedural_dome(64, 32, 12345, hot_spots=2, peak=500.0, spot_width_deg=6.0)
+        write_env("env.npz", Scene(domeLights=[DomeLight(pixels=px_env)]))
+    if want("env_angular"):
+        write_env("env_angular.npz", Scene(domeLights=[cases.angular_dome()]))
+    for name, wl in cases.image_workloads().items():
+        if want("images") or want(f"img:{name}"):
+            print("rendering", name, wl.settings.width, wl.settings.height, wl.settings.samplesPerPixel, flush=True)
+            fb = ReferenceScene(wl.scene).render(wl.camera, wl.settings)
+            np.savez_compressed(os.path.join(HERE, f"img_{name}.npz"), beauty=fb.beauty, albedo=fb.albedo,
+                                normal=fb.normal, depth=fb.depth, scene_hash=scene_hash(wl.scene))
+    if want("accumulated"):
+        name, wl, batches = cases.accumulated_workload()
+        fb = ReferenceScene(wl.scene).render_accumulated(wl.camera, wl.settings, batches)
+        np.savez_compressed(os.path.join(HERE, f"img_{name}.npz"), beauty=fb.beauty, albedo=fb.albedo,
+                            normal=fb.normal, depth=fb.depth, scene_hash=scene_hash(wl.scene),
+                            batches=np.array(batches, np.uint32))
+    print("done")
+
+
+def write_qmc(f):
     px, py, ss, dims = cases.qmc_grid()
     vals = np.zeros((len(py), len(px), len(ss), len(dims)), np.float32)
     for j, y in enumerate(py):
@@ -70,7 +109,9 @@ def main():
     ri = np.array([[f.radical_inverse(int(b), int(n)) for n in idx] for b in bases], np.float32)
     np.savez_compressed(os.path.join(HERE, "qmc.npz"), px=px, py=py, s=ss, dims=dims, values=vals,
                         bases=bases, index=idx, radical_inverse=ri)
-    # --- BSDF
+
+
+def write_bsdf(f):
     mats = cases.bsdf_materials()
     mi, n, cn, v, l = cases.bsdf_tuples(len(mats))
     val, pdf = eval_bsdf(f, mats, mi, n, cn, v, l)
@@ -87,31 +128,22 @@ def main():
     np.savez_compressed(os.path.join(HERE, "bsdf.npz"), mat_index=mi, n=n, cn=cn, v=v, l=l, value=val, pdf=pdf,
                         weights=w4, diffuse=dc, nh=nh, rough=rough, cosi=cosi, etai=etai, etat=etat, ggx_d=ggx,
                         ggx_g2=g2, fresnel=fres)
-    # --- environment map
-    px_env = procedural_dome(64, 32, 12345, hot_spots=2, peak=500.0, spot_width_deg=6.0)
-    sc = Scene(domeLights=[DomeLight(pixels=px_env)])
+
+
+def write_env(fname, sc):
+    d = sc.domeLights[0]
     r = ReferenceScene(sc)
-    row, cond = r.env_tables()
+    if d.layout == abi.ENV_ANGULAR:  # no importance table (EnvironmentMap.cpp:189-191)
+        row, cond = np.zeros(0, np.float32), np.zeros((0, 0), np.float32)
+    else:
+        row, cond = r.env_tables()
     u, dirs = cases.env_queries()
     sd, sp, sc_ = r.env_sample(u)
     ep, ec = r.env_eval(dirs)
-    np.savez_compressed(os.path.join(HERE, "env.npz"), pixels=px_env, row_cdf=row, cond_cdf=cond, u=u,
-                        sample_dir=sd, sample_pdf=sp, sample_rgb=sc_, dirs=dirs, eval_pdf=ep, eval_rgb=ec)
-    # --- images
-    for name, wl in cases.image_workloads().items():
-        print("rendering", name, wl.settings.width, wl.settings.height, wl.settings.samplesPerPixel, flush=True)
-        rs = ReferenceScene(wl.scene)
-        fb = rs.render(wl.camera, wl.settings)
-        np.savez_compressed(os.path.join(HERE, f"img_{name}.npz"), beauty=fb.beauty, albedo=fb.albedo,
-                            normal=fb.normal, depth=fb.depth, scene_hash=scene_hash(wl.scene))
-    name, wl, batches = cases.accumulated_workload()
-    rs = ReferenceScene(wl.scene)
-    fb = rs.render_accumulated(wl.camera, wl.settings, batches)
-    np.savez_compressed(os.path.join(HERE, f"img_{name}.npz"), beauty=fb.beauty, albedo=fb.albedo,
-                        normal=fb.normal, depth=fb.depth, scene_hash=scene_hash(wl.scene),
-                        batches=np.array(batches, np.uint32))
-    print("done")
+    np.savez_compressed(os.path.join(HERE, fname), pixels=d.pixels, layout=np.uint32(d.layout), row_cdf=row,
+                        cond_cdf=cond, u=u, sample_dir=sd, sample_pdf=sp, sample_rgb=sc_, dirs=dirs, eval_pdf=ep,
+                        eval_rgb=ec)
 
 
 if __name__ == "__main__":
-    main()
+    main(sys.argv[1:])

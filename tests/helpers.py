@@ -25,6 +25,12 @@ def c2_mini(width=96, spp=8, env=(256, 128)):
     return workloads.c2(scale=width / 1920, spp=spp, env_size=env)
 
 
+def angular_mini(width=96, spp=8):
+    """c2's geometry under a rotated angular (light-probe) dome."""
+    from golden import golden_cases
+    return golden_cases.angular_workload(scale=width / 1920, spp=spp)
+
+
 def c3_mini(width=96, spp=8, objects=12, env=(128, 64)):
     """c3's material mix and lighting on a few low-resolution objects."""
     return workloads.c3(scale=width / 3840, spp=spp, objects=objects, sphere_freq=8, box_n=6, env_size=env)

@@ -10,11 +10,12 @@ Bars (tolerances written in each test):
   - Images: beauty relMSE <= 1e-3 against the oracle at equal spp and seeds.
 """
 import ctypes as C
+import os
 
 import numpy as np
 import pytest
 
-from helpers import c1_mini, c2_mini, c3_mini, emissive_mini, random_rays, relmse
+from helpers import angular_mini, c1_mini, c2_mini, c3_mini, emissive_mini, random_rays, relmse
 from oracle.bindings import Accumulator, OracleScene, oracle_fns
 from shiro_amd.scene import Camera, DirectionalLight, PbrMaterial, RenderSettings, Scene, TriangleMesh
 
@@ -136,7 +137,32 @@ def test_env_vs_oracle(hip_ctx, c2m):
     assert np.mean(np.all(np.isclose(gc2, oc2, rtol=2e-4, atol=1e-5), axis=1)) >= 0.999
 
 
-@pytest.mark.parametrize("make", [c1_mini, c2_mini, c3_mini, emissive_mini], ids=["c1", "c2", "c3", "emissive"])
+MINI = [c1_mini, c2_mini, c3_mini, emissive_mini, angular_mini]
+MINI_IDS = ["c1", "c2", "c3", "emissive", "angular"]
+
+
+def test_env_angular_vs_oracle(hip_ctx):
+    """Angular dome: evaluation through the light-probe mapping, uniform
+    sphere sampling with pdf 1/(4 pi) (no importance table)."""
+    wl = angular_mini()
+    _log("angular dome sample/eval")
+    hip_ctx.upload(wl.scene)
+    orc = OracleScene(wl.scene)
+    rng = np.random.default_rng(8)
+    u = rng.random((50000, 2)).astype(np.float32)
+    gd, gp, gc = hip_ctx.env_sample(u)
+    od, op, oc = orc.env_sample(u)
+    assert np.allclose(gd, od, rtol=2e-5, atol=2e-6)
+    assert np.array_equal(gp, op)
+    assert np.mean(np.all(np.isclose(gc, oc, rtol=2e-4, atol=1e-5), axis=1)) >= 0.999
+    d = rng.normal(size=(50000, 3)).astype(np.float32)
+    gp2, gc2 = hip_ctx.env_eval(d)
+    op2, oc2 = orc.env_eval(d)
+    assert np.array_equal(gp2, op2)
+    assert np.mean(np.all(np.isclose(gc2, oc2, rtol=2e-4, atol=1e-5), axis=1)) >= 0.999
+
+
+@pytest.mark.parametrize("make", MINI, ids=MINI_IDS)
 def test_render_relmse(hip_ctx, make):
     wl = make()
     _log(f"render {wl.name} {wl.settings.width}x{wl.settings.height} {wl.settings.samplesPerPixel}spp")
@@ -193,6 +219,36 @@ def test_device_accumulator(hip_ctx):
     ref = acc.resolve()
     assert np.array_equal(out[0].cpu().numpy().reshape(-1, 4), ref.beauty)
     assert np.array_equal(out[3].cpu().numpy(), ref.depth)
+
+
+def test_progressive_batches_vs_reference_fixture(hip_ctx):
+    """Two progressive batches (the second with sampleStart > 0, where the
+    CPU backend carries its path-guide state) accumulated on the device,
+    against the frame the reference's own CPU backend wrote for the same
+    scene and batches: beauty relMSE <= 1e-3."""
+    import torch
+    from golden import golden_cases as cases
+    name, wl, batches = cases.accumulated_workload()
+    z = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", f"img_{name}.npz"))
+    _log(f"progressive {name} batches {batches}")
+    hip_ctx.upload(wl.scene)
+    w, h = wl.settings.width, wl.settings.height
+    dev = torch.device("cuda:0")
+    bufs = [torch.zeros(w * h * k, device=dev) for k in (4, 3, 3, 1)]
+    out = [torch.zeros(w * h * k, device=dev) for k in (4, 3, 3, 1)]
+    hip_ctx.accum_reset(w, h)
+    start = 0
+    for count in batches:
+        hip_ctx.render_device(wl.camera, wl.settings, [b.data_ptr() for b in bufs], start, count)
+        hip_ctx.accum_add([b.data_ptr() for b in bufs], w, h, count)
+        start += count
+    hip_ctx.accum_resolve([b.data_ptr() for b in out], w, h)
+    torch.cuda.synchronize()
+    beauty = out[0].cpu().numpy().reshape(-1, 4)
+    e = relmse(beauty[:, :3], z["beauty"][:, :3])
+    _log(f"relMSE vs reference fixture = {e:.3e}")
+    assert e <= 1e-3
+    assert np.mean(beauty[:, 3] == z["beauty"][:, 3]) >= 0.999
 
 
 def test_cancel_returns_cleared_frame(hip_ctx):
@@ -255,7 +311,7 @@ def test_shiro_adapter_probe_binary():
     assert r.stdout.startswith("probe center")
 
 
-@pytest.mark.parametrize("make", [c1_mini, c2_mini, c3_mini, emissive_mini], ids=["c1", "c2", "c3", "emissive"])
+@pytest.mark.parametrize("make", MINI, ids=MINI_IDS)
 def test_wavefront_equals_megakernel(hip_ctx, make):
     """Both transports evaluate the same per-bounce functions in the same
     order and accumulate samples in order, so their frames are identical."""

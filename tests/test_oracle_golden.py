@@ -79,12 +79,27 @@ def test_env_tables_sample_eval_bit_exact():
     assert _same(ep, z["eval_pdf"]) and _same(ec, z["eval_rgb"])
 
 
+def test_env_angular_sample_eval_bit_exact():
+    """Angular (light-probe) layout: Evaluate through EvaluateAngular, no
+    importance table, so Sample is uniform over the sphere with pdf 1/(4 pi)
+    and Pdf is 1/(4 pi) (EnvironmentMap.cpp:109-183, 247-264)."""
+    z = _load("env_angular.npz")
+    d = cases.angular_dome()
+    assert _same(d.pixels, z["pixels"]), "dome generator changed"
+    o = OracleScene(Scene(domeLights=[d]))
+    sd, sp, sc = o.env_sample(z["u"])
+    assert _same(sd, z["sample_dir"]) and _same(sp, z["sample_pdf"]) and _same(sc, z["sample_rgb"])
+    ep, ec = o.env_eval(z["dirs"])
+    assert _same(ep, z["eval_pdf"]) and _same(ec, z["eval_rgb"])
+    assert np.all(z["sample_pdf"] == np.float32(1.0 / (4.0 * np.pi)))
+
+
 def _scene_hash(scene):
     import golden.make_golden as mg
     return mg.scene_hash(scene)
 
 
-@pytest.mark.parametrize("name", ["c1", "c2", "c3", "emissive"])
+@pytest.mark.parametrize("name", ["c1", "c2", "c3", "emissive", "angular"])
 @pytest.mark.parametrize("bvh", [False, True], ids=["brute", "bvh"])
 def test_images_bit_exact(name, bvh):
     z = _load(f"img_{name}.npz")
@@ -95,9 +110,10 @@ def test_images_bit_exact(name, bvh):
         assert _same(getattr(fb, k), z[k]), k
 
 
-def test_accumulated_with_path_guide_bit_exact():
-    """Two batches through FrameAccumulator; the second batch uses the path
-    guide learned from the first (CpuPathTracer.cpp:2054-2062, 2158-2167)."""
+def test_accumulated_two_batches_bit_exact():
+    """Two batches through FrameAccumulator; the second batch starts from the
+    path-guide state the first one left (CpuPathTracer.cpp:2054-2062,
+    2158-2167)."""
     from oracle.bindings import Accumulator
     z = _load("img_c1_accumulated.npz")
     name, wl, batches = cases.accumulated_workload()
@@ -111,3 +127,21 @@ def test_accumulated_with_path_guide_bit_exact():
     fb = acc.resolve()
     for k in ("beauty", "albedo", "normal", "depth"):
         assert _same(getattr(fb, k), z[k]), k
+
+
+def test_path_guide_never_activates():
+    """The path guide of CpuPathTracer is inert: its weights start at zero
+    (CpuPathTracer.cpp:2057-2060), records are pushed only while the guide is
+    active, i.e. its total weight exceeds 1 (:1954, :1983-1985), and weights
+    grow only from records (:820-834, :2158-2167).  By induction the weights
+    stay zero, so every batch samples diffuse bounces from the cosine lobe.
+    Check: a second batch rendered after a first one (state carried over)
+    equals the same batch rendered by a fresh tracer, bit for bit, on the
+    scene of the two-batch reference fixture."""
+    name, wl, batches = cases.accumulated_workload()
+    carried = OracleScene(wl.scene, bvh=True)
+    carried.render(wl.camera, wl.settings, 0, batches[0])
+    second = carried.render(wl.camera, wl.settings, batches[0], batches[1])
+    fresh = OracleScene(wl.scene, bvh=True).render(wl.camera, wl.settings, batches[0], batches[1])
+    for k in ("beauty", "albedo", "normal", "depth"):
+        assert _same(getattr(second, k), getattr(fresh, k)), k
