@@ -225,6 +225,8 @@ def main():
     ap.add_argument("--no-autotune", action="store_true", help="keep the default wavefront tuning")
     ap.add_argument("--fetch-mode", type=int, default=None, help="force the traversal fetch mode (0, 1, 2)")
     ap.add_argument("--paths-per-pass", type=int, default=None, help="force the wavefront paths per pass")
+    ap.add_argument("--shade-order", type=int, default=None, choices=[0, 1],
+                    help="force shading-class ordering of the hit queue off (0) or on (1)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device 3 times (calibrates HBM byte counters)")
     args = ap.parse_args()
@@ -276,14 +278,16 @@ def main():
                 kw["pathsPerPass"] = args.paths_per_pass
             ctx.set_tuning(**kw)
         elif not args.no_autotune:
-            trial = max(1, min(st.samplesPerPixel, (16 << 20) // npx))
+            trial = max(1, min(st.samplesPerPixel, (64 << 20) // npx))
             t0 = time.perf_counter()
             _, report = ctx.autotune(wl.camera, st, trial)
             print(f"# autotune ({trial} spp trials, {time.perf_counter() - t0:.1f} s):\n" + report, file=sys.stderr,
                   flush=True)
+        if args.shade_order is not None:
+            ctx.set_tuning(shadeOrder=args.shade_order)
         t = ctx.get_tuning()
         tuning = {"pathsPerPass": t.pathsPerPass, "fetchMode": t.fetchMode, "traceBlocks": t.traceBlocks,
-                  "stageBlocks": t.stageBlocks,
+                  "stageBlocks": t.stageBlocks, "shadeOrder": t.shadeOrder,
                   "autotuned": args.fetch_mode is None and args.paths_per_pass is None and not args.no_autotune}
     dev = torch.device("cuda:0")
     bufs = [torch.empty(npx * k, dtype=torch.float32, device=dev) for k in (4, 3, 3, 1)]

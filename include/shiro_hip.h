@@ -424,20 +424,25 @@ int shiro_hip_count_fetches(
 /* ---- Wavefront tuning (no counterpart in the reference: knobs of the
  * wavefront transport, set by the caller or chosen by shiro_hip_autotune). ---- */
 typedef struct shiro_hip_tuning {
-    uint32_t pathsPerPass;  /* paths in flight per pass (default 4M) */
+    uint32_t pathsPerPass;  /* paths in flight per pass (default 64M; the path
+                               state takes about (176 + 32 x light samples) B per
+                               path, capped at 32 GiB) */
     uint32_t fetchMode;     /* traversal work distribution: 0 wave rounds, 1 lane refill,
                                2 lane refill from per-wave pools of the queue (default 0) */
     uint32_t traceBlocks;   /* grid of the fetchMode 2 traversal kernels (default 32 x CUs) */
     uint32_t stageBlocks;   /* grid cap of the grid-stride stage kernels (default 16384) */
     uint32_t batchBlocksPerCU; /* one-wave blocks per CU of the batched persistent kernel (default 8) */
-    uint32_t reserved[3];
+    uint32_t shadeOrder;    /* 1: shade bounces after the first in shading-class order (default 1) */
+    uint32_t reserved[2];
 } shiro_hip_tuning;
 int shiro_hip_set_tuning(shiro_hip_ctx* ctx, const shiro_hip_tuning* tuning);
 int shiro_hip_get_tuning(const shiro_hip_ctx* ctx, shiro_hip_tuning* tuning);
 
 /* Times candidate tunings on `trialSamples` samples of the given frame
  * (wavefront transport, uploaded scene), keeps the fastest in the context
- * and returns it; `report` (may be NULL) receives one line per candidate. */
+ * and returns it; `report` (may be NULL) receives one line per candidate.
+ * Pass sizes above trialSamples x pixels time like the whole trial in one
+ * pass, so trials should span the pass sizes of interest. */
 int shiro_hip_autotune(
     shiro_hip_ctx* ctx, const shiro_hip_camera* camera,
     const shiro_hip_settings* settings, uint32_t trialSamples,
@@ -485,6 +490,7 @@ enum {
     SHIRO_HIP_STAGE_RESOLVE = 4,    /* ordered sum of unoccluded light samples */
     SHIRO_HIP_STAGE_ACCUM = 5,      /* per-pixel sample sums of the pass */
     SHIRO_HIP_STAGE_MEGAKERNEL = 6, /* one megakernel launch */
+    SHIRO_HIP_STAGE_ORDER = 7,      /* hit queue sorted by shading class */
     SHIRO_HIP_STAGE_COUNT = 8
 };
 typedef struct shiro_hip_stage_times {

@@ -132,7 +132,8 @@ class FetchCounts(C.Structure):
 class Tuning(C.Structure):
     _fields_ = [
         ("pathsPerPass", C.c_uint32), ("fetchMode", C.c_uint32), ("traceBlocks", C.c_uint32),
-        ("stageBlocks", C.c_uint32), ("batchBlocksPerCU", C.c_uint32), ("reserved", C.c_uint32 * 3),
+        ("stageBlocks", C.c_uint32), ("batchBlocksPerCU", C.c_uint32), ("shadeOrder", C.c_uint32),
+        ("reserved", C.c_uint32 * 2),
     ]
 
 
@@ -146,7 +147,7 @@ class BatchItem(C.Structure):
 MANY_PERSISTENT, MANY_GRAPH, MANY_SEQUENTIAL, MANY_WAVEFRONT = 0, 1, 2, 3
 
 
-STAGE_NAMES = ["reset", "extend", "shade", "shadow", "resolve", "accum", "megakernel", "unused"]
+STAGE_NAMES = ["reset", "extend", "shade", "shadow", "resolve", "accum", "megakernel", "order"]
 
 
 class StageTimes(C.Structure):

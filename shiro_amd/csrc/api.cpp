@@ -85,7 +85,7 @@ struct shiro_hip_ctx {
     // Scene.
     bool hasScene = false;
     bool emissive = false;
-    DevBuf nodes, tris, hitdata, materials, derived, uvs, lightList, lightCdf;
+    DevBuf nodes, tris, hitdata, materials, derived, uvs, lightList, lightCdf, triClass;
     EnvBuffers env[kMaxDomes];
     std::vector<std::unique_ptr<TexBuffers>> tex;
     DevBuf texTable;
@@ -97,12 +97,12 @@ struct shiro_hip_ctx {
     DevBuf frB, frA, frN, frD;
     DevBuf counters;
     // Wavefront state (WfBuffers arrays) and queues.
-    DevBuf wf[16];
+    DevBuf wf[20];
     // Accumulator.
     DevBuf acB, acA, acN, acD, acC;
     uint32_t acW = 0, acH = 0, acTotal = 0;
     // Wavefront tuning (shiro_hip_set_tuning / shiro_hip_autotune).
-    shiro_hip_tuning tuning{4u << 20, 0u, 8192u, 16384u, 8u, {0, 0, 0}};
+    shiro_hip_tuning tuning{64u << 20, 0u, 8192u, 16384u, 8u, 1u, {0, 0}};
     uint32_t computeUnits = 256;
     // Batched rendering: items, work list and the work counter.
     DevBuf manyItems, manyWork, manyCounter;
@@ -190,9 +190,12 @@ int check_settings(shiro_hip_ctx* ctx, const shiro_hip_settings* s) {
 }
 
 // Wavefront form of one batch: passes of S samples for every pixel, S
-// chosen so that a pass holds about kWavefrontPaths paths (and at most
-// kWavefrontStateBytes of path state).
-constexpr uint64_t kWavefrontStateBytes = 6ull << 30;
+// chosen so that a pass holds about tuning.pathsPerPass paths (and at most
+// kWavefrontStateBytes of path state).  Large passes pay: the last bounces
+// of a pass hold few paths and run latency-bound on a mostly idle GPU, so
+// fewer passes means fewer such tails (c3: 4M -> 64M paths per pass cuts a
+// frame from 245 to 189 ms).
+constexpr uint64_t kWavefrontStateBytes = 32ull << 30;
 
 // Starts the event list of a render call in stage-timing mode: the first
 // event marks the start of the first kernel.
@@ -234,6 +237,13 @@ int enqueue_wavefront(shiro_hip_ctx* ctx, const DevScene& scene, const shiro_hip
     HIPC(ctx->wf[13].reserve(cap * 4), "alloc wavefront queues");
     HIPC(ctx->wf[14].reserve(cap * M * 4), "alloc wavefront queues");
     HIPC(ctx->wf[15].reserve(cap * M * 4), "alloc wavefront queues");
+    const bool order = ctx->tuning.shadeOrder != 0u;
+    if (order) {
+        HIPC(ctx->wf[16].reserve(cap), "alloc wavefront queues");
+        HIPC(ctx->wf[17].reserve(cap * 4), "alloc wavefront queues");
+        HIPC(ctx->wf[18].reserve(kShadeClasses * kOrderBlocks * 4), "alloc wavefront queues");
+        HIPC(ctx->wf[19].reserve((kShadeClasses + 1u) * kOrderBlocks * 4), "alloc wavefront queues");
+    }
     const uint32_t slots = 5u * st.maxDepth + 1u;
     HIPC(ctx->counters.reserve(std::max<size_t>(sizeof(DevCounters), slots * 4u)), "alloc counters");
     if (S < count) {
@@ -272,6 +282,11 @@ int enqueue_wavefront(shiro_hip_ctx* ctx, const DevScene& scene, const shiro_hip
     B.fetchBase = B.shadowPathBase + st.maxDepth;
     B.maxShadow = M;
     B.cap = static_cast<uint32_t>(cap);
+    B.shadeOrder = order ? 1u : 0u;
+    B.qkey = order ? ctx->wf[16].as<uint8_t>() : nullptr;
+    B.qsorted = order ? ctx->wf[17].as<uint32_t>() : nullptr;
+    B.blockCount = order ? ctx->wf[18].as<uint32_t>() : nullptr;
+    B.blockStart = order ? ctx->wf[19].as<uint32_t>() : nullptr;
     P.sampleStart = start;
     P.batchCount = count;
     P.tilesX = tilesX;
@@ -526,6 +541,14 @@ int shiro_hip_upload_scene(shiro_hip_ctx* ctx, const shiro_hip_scene* s) {
     ctx->mats = mats;
     std::vector<MaterialDerived> der(mats.size());
     for (size_t i = 0; i < mats.size(); ++i) der[i] = derive_material(mats[i]);
+    assign_shade_classes(mats.data(), der.data(), der.size());
+    std::vector<uint8_t> cls(nt);
+    for (size_t j = 0; j < nt; ++j) {
+        uint32_t mat;
+        std::memcpy(&mat, &hit[j * 12 + 3], 4);
+        cls[j] = static_cast<uint8_t>(der[std::min<size_t>(mat, der.size() - 1)].shadeClass);
+    }
+    if (nt) HIPC(upload(ctx->triClass, cls.data(), cls.size(), ctx->stream), "upload triangle classes");
     HIPC(upload(ctx->nodes, bvh.nodes.data(), bvh.nodes.size(), ctx->stream), "upload nodes");
     HIPC(upload(ctx->tris, tris.data(), tris.size(), ctx->stream), "upload triangles");
     HIPC(upload(ctx->hitdata, hit.data(), hit.size(), ctx->stream), "upload hit data");
@@ -572,6 +595,7 @@ int shiro_hip_upload_scene(shiro_hip_ctx* ctx, const shiro_hip_scene* s) {
     if (!table.empty()) HIPC(upload(ctx->texTable, table.data(), table.size(), ctx->stream), "upload textures");
     DevScene d{};
     d.uvs = flat.hasUvs ? ctx->uvs.as<float2>() : nullptr;
+    d.triClass = nt ? ctx->triClass.as<uint8_t>() : nullptr;
     d.emissive = lights.empty() ? nullptr : ctx->lightList.as<float4>();
     d.emissiveCdf = lights.empty() ? nullptr : ctx->lightCdf.as<float>();
     d.emissiveCount = static_cast<uint32_t>(lights.size());
@@ -960,7 +984,7 @@ int shiro_hip_set_tuning(shiro_hip_ctx* ctx, const shiro_hip_tuning* t) {
     if (!ctx || !t) return SHIRO_HIP_INVALID;
     std::lock_guard<std::mutex> lock(ctx->mu);
     if (t->pathsPerPass == 0 || t->fetchMode > 2u || t->traceBlocks == 0 || t->stageBlocks == 0 ||
-        t->batchBlocksPerCU == 0 || t->batchBlocksPerCU > 64u)
+        t->batchBlocksPerCU == 0 || t->batchBlocksPerCU > 64u || t->shadeOrder > 1u)
         return fail(ctx, SHIRO_HIP_INVALID, "invalid tuning");
     ctx->tuning = *t;
     return SHIRO_HIP_OK;
@@ -992,7 +1016,7 @@ int shiro_hip_autotune(shiro_hip_ctx* ctx, const shiro_hip_camera* camera, const
     const shiro_hip_tuning base = ctx->tuning;
     const uint32_t cus = std::max(1u, base.traceBlocks / 32u);
     std::vector<shiro_hip_tuning> cands;
-    for (uint32_t paths : {1u << 20, 2u << 20, 4u << 20, 8u << 20}) {
+    for (uint32_t paths : {4u << 20, 16u << 20, 64u << 20, 128u << 20}) {
         for (uint32_t mode : {0u, 1u, 2u}) {
             for (uint32_t tb : {16u, 32u, 64u}) {
                 if (mode != 2u && tb != 32u) continue;

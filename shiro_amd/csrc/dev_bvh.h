@@ -162,6 +162,11 @@ DEV uint32_t node_boxes(const float4* __restrict__ nd, const RayPre& r, float tm
     w[1] = __float_as_uint(cw.y);
     w[2] = __float_as_uint(cw.z);
     w[3] = __float_as_uint(cw.w);
+    // The child words are used only when a child is hit; without this the
+    // compiler defers their load past the box test, which costs a second
+    // memory round trip per node.  The empty asm makes them operands here so
+    // the load is issued together with the boxes'.
+    asm volatile("" : "+v"(w[0]), "+v"(w[1]), "+v"(w[2]), "+v"(w[3]));
     return mask & (ex >> 24);
 }
 

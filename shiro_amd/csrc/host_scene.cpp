@@ -162,6 +162,26 @@ MaterialDerived derive_material(const shiro_hip_material& m) {
     return d;
 }
 
+void assign_shade_classes(const shiro_hip_material* m, MaterialDerived* d, size_t n) {
+    std::vector<uint32_t> seen;
+    for (size_t i = 0; i < n; ++i) {
+        const MaterialDerived& x = d[i];
+        uint32_t sig = x.flags;
+        sig |= (x.wDiffuse > 0.0f ? 1u : 0u) << 16;
+        sig |= (x.wBaseSpecular > 0.0f ? 1u : 0u) << 17;
+        sig |= (x.wCoat > 0.0f ? 1u : 0u) << 18;
+        sig |= (x.wTransmission > 0.0f ? 1u : 0u) << 19;
+        sig |= (x.sampleRoughnessBase <= 0.02f ? 1u : 0u) << 20;
+        sig |= (x.sampleRoughnessCoat <= 0.02f ? 1u : 0u) << 21;
+        sig |= (x.sampleRoughnessTrans <= 0.02f ? 1u : 0u) << 22;
+        sig |= (m[i].thinWalled ? 1u : 0u) << 23;
+        size_t k = 0;
+        while (k < seen.size() && seen[k] != sig) ++k;
+        if (k == seen.size()) seen.push_back(sig);
+        d[i].shadeClass = static_cast<uint32_t>(std::min<size_t>(k, kShadeMissClass - 1u));
+    }
+}
+
 bool build_env_tables(const float* px, uint32_t w, uint32_t h, uint32_t layout, std::vector<float>* rowCdf,
                       std::vector<float>* condCdf) {
     rowCdf->clear();

@@ -17,6 +17,11 @@ shiro_hip_material default_material();
 // (CpuPathTracer.cpp:841-1441): lobe weights, F0s, tints, isotropic
 // sampling roughnesses, medium extinction.
 MaterialDerived derive_material(const shiro_hip_material& m);
+// Assigns MaterialDerived::shadeClass: materials whose shading takes the
+// same branches (feature flags, which lobes have weight, which lobes are
+// delta) share a class, numbered in order of first appearance; classes past
+// kShadeClasses - 2 share the last material class.
+void assign_shade_classes(const shiro_hip_material* m, MaterialDerived* d, size_t n);
 
 // EnvironmentMap::BuildImportanceTable (EnvironmentMap.cpp:185-236):
 // sequential fp32 prefix sums in the reference's order.  pixels is w*h RGB.

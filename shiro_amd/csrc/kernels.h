@@ -35,6 +35,11 @@ struct WfBuffers {
     uint32_t fetchBase;                     // counts[fetchBase + d], [+ maxDepth + d]: work fetched
     uint32_t maxShadow;
     uint32_t cap;
+    uint8_t* qkey;                          // shading class per queue position
+    uint32_t* qsorted;                      // the hit queue in class order
+    uint32_t* blockCount;                   // [class][block]: hits of the class in the block's range
+    uint32_t* blockStart;                   // [class][block]: first slot within the class; [64][..]: totals
+    uint32_t shadeOrder;                    // 1: shade depth > 0 hits in class order
 };
 
 struct WfParams {

@@ -100,7 +100,19 @@ struct MaterialDerived {
     float sampleScaleBase;       // 1 - 0.35 * anisotropy * cos(2 pi rotation), base lobe
     float sampleScaleTrans;      // the same for the transmission lobe
     uint32_t flags;              // kMat* bits
+    uint32_t shadeClass;         // materials that take the same branches of the
+                                 // shading code share a class (< kShadeClasses - 1)
 };
+
+// Shading classes for ordering the wavefront's hit queue: materials are
+// grouped by the branches the BSDF and bounce code take; misses get the last
+// class.
+constexpr uint32_t kShadeClasses = 64;
+constexpr uint32_t kShadeMissClass = kShadeClasses - 1u;
+// The class sort splits a queue into kOrderBlocks contiguous ranges, one per
+// 256-thread block.
+constexpr uint32_t kOrderBlocks = 512;
+constexpr uint32_t kOrderThreads = 256;
 
 constexpr uint32_t kMatAnisotropic = 1u;   // specular or coat anisotropy != 0
 constexpr uint32_t kMatThinFilm = 2u;
@@ -129,6 +141,7 @@ struct DevScene {
     const shiro_hip_material* materials;
     const MaterialDerived* derived;
     const float2* uvs;            // 3 per triangle in leaf order, null without UVs
+    const uint8_t* triClass;      // shading class of each triangle's material, leaf order
     const float4* emissive;       // 5 per emissive triangle: {p0, area} {p1, weight} {p2, tri bits} {normal} {radiance}
     const float* emissiveCdf;     // cumulative weights (emissiveCount)
     uint32_t emissiveCount;

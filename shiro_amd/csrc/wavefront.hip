@@ -7,6 +7,10 @@
 //     k_wf_trace<closest>  closest hit of every queued ray; at depth 0 every
 //                  (pixel, sample) path of the pass, its camera ray computed
 //                  in place (no queue, no stored ray)
+//     k_wf_classify, k_wf_order (depth > 0, optional)  counting sort of
+//                  the hit queue by shading class (misses, and materials
+//                  that take the same shading branches), so that the lanes
+//                  of a shade wave follow the same code path
 //     k_wf_shade   surface terms, light samples -> shadow rays, BSDF
 //                  sampling and roulette -> next ray queue
 //     k_wf_trace<any>  any-hit test of every queued shadow ray
@@ -220,7 +224,11 @@ DEV uint32_t ring_pop(float tcull, uint32_t lane, const uint32_t* ovNode, const 
 // its lanes are done), 1 = lanes refill as their rays finish (static item
 // order), 2 = lanes refill from per-wave pools of the global queue.
 template <bool ANY, int FETCH>
-__global__ __launch_bounds__(64) void k_wf_trace(WfParams P, uint32_t depth) {
+__global__ __launch_bounds__(64)
+#ifdef SHIRO_TRACE_WAVES
+__attribute__((amdgpu_waves_per_eu(SHIRO_TRACE_WAVES)))
+#endif
+void k_wf_trace(WfParams P, uint32_t depth) {
     const uint32_t md = P.settings.maxDepth;
     const uint32_t n = ANY ? P.B.counts[P.B.shadowRayBase + depth] : stage_items(P, depth);
     uint32_t* counter = &P.B.counts[P.B.fetchBase + (ANY ? md : 0u) + depth];
@@ -255,14 +263,29 @@ __global__ __launch_bounds__(64) void k_wf_trace(WfParams P, uint32_t depth) {
     WavePool pool;
     // Static assignment: lane i of the grid takes items i, i + G, i + 2G...
     uint32_t nextStatic = blockIdx.x * 64u + threadIdx.x;
+#ifdef SHIRO_XCD_MAP
+    const bool xcdMap = (gridDim.x & 7u) == 0u;
+    const uint32_t xcdPer = gridDim.x >> 3;
+    const uint32_t xcdSeg = ((n + 63u) / 64u + 7u) / 8u;
+    const uint32_t xcdBase = (blockIdx.x & 7u) * xcdSeg;
+    uint32_t xcdLocal = blockIdx.x >> 3;
+#endif
     for (;;) {
         const bool want = !active && !exhausted && (FETCH != 0 || !__any(active));
         uint32_t idx;
         if (FETCH == 2) {
             idx = pool.fetch(want, counter);
         } else {
-            idx = nextStatic;
-            if (want) nextStatic += gridDim.x * 64u;
+#ifdef SHIRO_XCD_MAP
+            if (xcdMap) {
+                idx = xcdLocal < xcdSeg ? (xcdBase + xcdLocal) * 64u + lane : 0xffffffffu;
+                if (want) xcdLocal += xcdPer;
+            } else
+#endif
+            {
+                idx = nextStatic;
+                if (want) nextStatic += gridDim.x * 64u;
+            }
         }
         if (want) {
             if (idx >= n) {
@@ -363,9 +386,141 @@ __global__ __launch_bounds__(64) void k_wf_trace(WfParams P, uint32_t depth) {
     }
 }
 
+// Shading class of the hit of path p (kShadeMissClass for a miss).
+DEV uint32_t hit_class(const WfParams& P, uint32_t p) {
+    const uint32_t slot = __float_as_uint(P.B.H[p].w);
+    return slot == kNoHit ? kShadeMissClass : P.scene.triClass[slot];
+}
+
+// Range of queue positions of block b in the class sort: kOrderBlocks
+// contiguous ranges, whole multiples of the block size.
+DEV void order_range(uint32_t n, uint32_t b, uint32_t* lo, uint32_t* hi) {
+    uint32_t r = (n + kOrderBlocks - 1u) / kOrderBlocks;
+    r = (r + kOrderThreads - 1u) / kOrderThreads * kOrderThreads;
+    const uint64_t l = static_cast<uint64_t>(b) * r;
+    *lo = static_cast<uint32_t>(l < n ? l : n);
+    *hi = static_cast<uint32_t>(l + r < n ? l + r : n);
+}
+
+// Adds `valid` lanes to the LDS counter of their class, one LDS atomic per
+// distinct class of the wave; returns each lane's rank among its wave's
+// lanes of the same class, and in *base the counter's value before the add.
+DEV uint32_t wave_class_add(bool valid, uint32_t k, uint32_t* counter, uint32_t* base) {
+    const uint32_t lane = lane_id();
+    bool pending = valid;
+    uint32_t rank = 0, b = 0;
+    for (;;) {
+        const unsigned long long live = __ballot(pending);
+        if (live == 0ull) break;
+        const int leader = __ffsll(static_cast<long long>(live)) - 1;
+        const uint32_t lk = __shfl(k, leader);
+        const bool mine = pending && k == lk;
+        const unsigned long long same = __ballot(mine);
+        uint32_t got = 0;
+        if (static_cast<int>(lane) == leader) got = atomicAdd(counter + lk, static_cast<uint32_t>(__popcll(same)));
+        got = __shfl(got, leader);
+        if (mine) {
+            rank = static_cast<uint32_t>(__popcll(same & ((1ull << lane) - 1ull)));
+            b = got;
+            pending = false;
+        }
+    }
+    *base = b;
+    return rank;
+}
+
+// Class sort of the depth's hit queue, pass 1: class of every position and
+// the per-block class counts.
+__global__ __launch_bounds__(kOrderThreads) void k_wf_classify(WfParams P, uint32_t depth) {
+    __shared__ uint32_t hist[kShadeClasses];
+    const uint32_t n = P.B.counts[depth];
+    const uint32_t* in = (depth & 1u) ? P.B.qb : P.B.qa;
+    if (threadIdx.x < kShadeClasses) hist[threadIdx.x] = 0u;
+    __syncthreads();
+    uint32_t lo, hi;
+    order_range(n, blockIdx.x, &lo, &hi);
+    for (uint32_t base = lo; base < hi; base += kOrderThreads) {
+        const uint32_t i = base + threadIdx.x;
+        const bool valid = i < hi;
+        uint32_t k = 0;
+        if (valid) {
+            k = hit_class(P, in[i]);
+            P.B.qkey[i] = static_cast<uint8_t>(k);
+        }
+        uint32_t unused;
+        wave_class_add(valid, k, hist, &unused);
+    }
+    __syncthreads();
+    if (threadIdx.x < kShadeClasses) P.B.blockCount[threadIdx.x * kOrderBlocks + blockIdx.x] = hist[threadIdx.x];
+}
+
+// Pass 2: block k scans class k's counts over the blocks (first slot of each
+// block within the class) and stores the class total.
+__global__ __launch_bounds__(kOrderThreads) void k_wf_order_scan(WfParams P) {
+    constexpr uint32_t kPer = kOrderBlocks / kOrderThreads;
+    __shared__ uint32_t part[kOrderThreads];
+    const uint32_t k = blockIdx.x;
+    const uint32_t* c = P.B.blockCount + k * kOrderBlocks + threadIdx.x * kPer;
+    uint32_t v[kPer];
+    uint32_t sum = 0;
+    for (uint32_t j = 0; j < kPer; ++j) {
+        v[j] = c[j];
+        sum += v[j];
+    }
+    part[threadIdx.x] = sum;
+    __syncthreads();
+    for (uint32_t off = 1; off < kOrderThreads; off <<= 1) {
+        const uint32_t add = threadIdx.x >= off ? part[threadIdx.x - off] : 0u;
+        __syncthreads();
+        part[threadIdx.x] += add;
+        __syncthreads();
+    }
+    uint32_t run = part[threadIdx.x] - sum;
+    uint32_t* out = P.B.blockStart + k * kOrderBlocks + threadIdx.x * kPer;
+    for (uint32_t j = 0; j < kPer; ++j) {
+        out[j] = run;
+        run += v[j];
+    }
+    if (threadIdx.x == kOrderThreads - 1u) P.B.blockStart[kShadeClasses * kOrderBlocks + k] = part[threadIdx.x];
+}
+
+// Pass 3: scatters the queue into class order.  Class k starts after the
+// totals of the smaller classes; the block's hits of class k take
+// consecutive slots from its scanned start, in the order of the wave
+// atomics.  Every path's result is independent of where it is shaded.
+__global__ __launch_bounds__(kOrderThreads) void k_wf_order(WfParams P, uint32_t depth) {
+    __shared__ uint32_t start[kShadeClasses];
+    __shared__ uint32_t cursor[kShadeClasses];
+    const uint32_t n = P.B.counts[depth];
+    const uint32_t* in = (depth & 1u) ? P.B.qb : P.B.qa;
+    if (threadIdx.x < 64u) {
+        // Exclusive scan of the class totals, one class per lane of wave 0.
+        const uint32_t lane = threadIdx.x;
+        const uint32_t total = P.B.blockStart[kShadeClasses * kOrderBlocks + lane];
+        uint32_t incl = total;
+        for (uint32_t off = 1; off < 64u; off <<= 1) {
+            const uint32_t t = __shfl_up(incl, off);
+            if (lane >= off) incl += t;
+        }
+        start[lane] = incl - total + P.B.blockStart[lane * kOrderBlocks + blockIdx.x];
+        cursor[lane] = 0u;
+    }
+    __syncthreads();
+    uint32_t lo, hi;
+    order_range(n, blockIdx.x, &lo, &hi);
+    for (uint32_t base = lo; base < hi; base += kOrderThreads) {
+        const uint32_t i = base + threadIdx.x;
+        const bool valid = i < hi;
+        const uint32_t k = valid ? P.B.qkey[i] : 0u;
+        uint32_t got;
+        const uint32_t rank = wave_class_add(valid, k, cursor, &got);
+        if (valid) P.B.qsorted[start[k] + got + rank] = in[i];
+    }
+}
+
 template <bool TEX>
 __global__ __launch_bounds__(64) void k_wf_shade(WfParams P, uint32_t depth) {
-    const uint32_t* in = (depth & 1u) ? P.B.qb : P.B.qa;
+    const uint32_t* in = depth > 0u && P.B.shadeOrder ? P.B.qsorted : ((depth & 1u) ? P.B.qb : P.B.qa);
     uint32_t* out = (depth & 1u) ? P.B.qa : P.B.qb;
     const uint32_t n = stage_items(P, depth);
     const uint32_t nLight = direct_sample_count(P.scene, P.settings);
@@ -537,6 +692,12 @@ hipError_t launch_wavefront_pass(const WfParams& P, const WfLaunch& L, hipStream
             default: hipLaunchKernelGGL((k_wf_trace<false, 0>), dim3(blocks), dim3(64), 0, s, P, d); break;
         }
         m.mark(SHIRO_HIP_STAGE_EXTEND, s);
+        if (d > 0u && P.B.shadeOrder) {
+            hipLaunchKernelGGL(k_wf_classify, dim3(kOrderBlocks), dim3(kOrderThreads), 0, s, P, d);
+            hipLaunchKernelGGL(k_wf_order_scan, dim3(kShadeClasses), dim3(kOrderThreads), 0, s, P);
+            hipLaunchKernelGGL(k_wf_order, dim3(kOrderBlocks), dim3(kOrderThreads), 0, s, P, d);
+            m.mark(SHIRO_HIP_STAGE_ORDER, s);
+        }
         if (P.scene.textured) {
             hipLaunchKernelGGL(k_wf_shade<true>, dim3(blocks), dim3(64), 0, s, P, d);
         } else {

@@ -359,6 +359,30 @@ def test_wavefront_fetch_modes_identical(hip_ctx, mode):
     assert np.array_equal(wf.beauty, mk.beauty)
 
 
+@pytest.mark.parametrize("make", [c2_mini, c3_mini, emissive_mini], ids=["c2", "c3", "emissive"])
+def test_shade_order_identical(hip_ctx, make):
+    """Shading the hit queue in shading-class order permutes where paths are
+    shaded, not what they compute: frames equal those without the ordering
+    and those of the megakernel, bit for bit."""
+    from shiro_amd import abi
+    wl = make()
+    _log(f"shade order {wl.name}")
+    hip_ctx.upload(wl.scene)
+    mk = hip_ctx.render(wl.camera, wl.settings)
+    st = RenderSettings(**{**wl.settings.__dict__, "transport": abi.TRANSPORT_WAVEFRONT})
+    saved = hip_ctx.get_tuning()
+    frames = []
+    try:
+        for order in (0, 1):
+            hip_ctx.set_tuning(shadeOrder=order, pathsPerPass=1 << 16)
+            frames.append(hip_ctx.render(wl.camera, st))
+    finally:
+        hip_ctx.set_tuning(**{k: getattr(saved, k) for k, _ in abi.Tuning._fields_ if k != "reserved"})
+    for k in ("beauty", "albedo", "normal", "depth"):
+        assert np.array_equal(getattr(frames[0], k), getattr(frames[1], k)), k
+        assert np.array_equal(getattr(frames[1], k), getattr(mk, k)), k
+
+
 def test_autotune_keeps_result(hip_ctx):
     from shiro_amd import abi
     wl = c2_mini(width=128, spp=4)
