@@ -41,17 +41,23 @@ def cpu_baseline(wl, target_seconds, threads):
     build_s = time.perf_counter() - t0
     st = wl.settings
     spp = st.samplesPerPixel
-    rows = 4
-    y0 = st.height // 2
-    orc.render_region(wl.camera, st, 0, y0, st.width, 1, 0, 1, threads)  # warm caches and threads
-    t0 = time.perf_counter()
-    orc.render_region(wl.camera, st, 0, y0, st.width, rows, 0, spp, threads)
-    dt = max(time.perf_counter() - t0, 1e-3)
-    rows2 = int(min(st.height // 2, max(rows, rows * target_seconds / dt)))
-    y0 = (st.height - rows2) // 2
-    t0 = time.perf_counter()
-    orc.render_region(wl.camera, st, 0, y0, st.width, rows2, 0, spp, threads)
-    dt = time.perf_counter() - t0
+    orc.render_region(wl.camera, st, 0, st.height // 2, st.width, 1, 0, 1, threads)  # warm caches and threads
+
+    def timed(rows):
+        y0 = (st.height - rows) // 2
+        t0 = time.perf_counter()
+        orc.render_region(wl.camera, st, 0, y0, st.width, rows, 0, spp, threads)
+        return y0, max(time.perf_counter() - t0, 1e-3)
+
+    # Grow the band of rows until it takes about target_seconds; the last
+    # band timed is the one reported.
+    rows2 = 4
+    y0, dt = timed(rows2)
+    for _ in range(3):
+        if dt >= 0.6 * target_seconds or rows2 >= st.height:
+            break
+        rows2 = int(min(st.height, max(rows2 + 1, rows2 * target_seconds / dt)))
+        y0, dt = timed(rows2)
     samples = st.width * rows2 * spp
     orc.close()
     return {
