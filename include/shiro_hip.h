@@ -266,6 +266,16 @@ int shiro_hip_supports_scene(const shiro_hip_scene* scene);
  * the caller may free it after the call. */
 int shiro_hip_upload_scene(shiro_hip_ctx* ctx, const shiro_hip_scene* scene);
 
+/* Re-uploads a scene whose meshes and index arrays are those of the
+ * uploaded one (vertex positions, normals, UVs, materials, lights and domes
+ * may change) and refits the BVH to the moved vertices, keeping its tree,
+ * instead of building a new one.  The reference rebuilds on any change of
+ * its scene signature (OptixBackend.cpp:150-232, CpuPathTracer.cpp:2030-2041);
+ * for deforming meshes this skips the SAH build (c3: 1.3 s).  Returns
+ * SHIRO_HIP_INVALID if the topology differs.  The tree's quality degrades
+ * with large deformations; upload anew to rebuild it. */
+int shiro_hip_refit_scene(shiro_hip_ctx* ctx, const shiro_hip_scene* scene);
+
 /* RenderSampleBatch, RenderBackend.h:33 / CpuPathTracer.cpp:2043-2170:
  * renders samples [sampleStart, sampleStart + sampleCount) of every pixel and
  * writes the batch means into host buffers `out` (sized settings->width x
@@ -504,6 +514,25 @@ typedef struct shiro_hip_stage_times {
  * per-stage sums of the last synchronised render call. */
 int shiro_hip_set_stage_timing(shiro_hip_ctx* ctx, uint32_t enable);
 int shiro_hip_get_stage_times(shiro_hip_ctx* ctx, shiro_hip_stage_times* out);
+
+/* ---- Mesh ingestion of Shiro's Hydra scene bridge (host code, no device):
+ * what HdShiroSceneBridge does to a USD polygon mesh before it reaches a
+ * render backend. ---- */
+/* Fan triangulation (reference src/hydra/SceneBridge.cpp:1009-1020): a face
+ * of c >= 3 vertices v0..v(c-1) gives triangles (v0, vk, vk+1), k = 1..c-2;
+ * faces of fewer vertices are skipped (their indices consumed).  *outCount
+ * receives the triangle index count; with outIndices NULL only the count is
+ * computed.  SHIRO_HIP_INVALID for a negative count, faces needing more
+ * than indexCount indices, or outCapacity below the count. */
+int shiro_hip_triangulate(const int32_t* faceVertexCounts, uint32_t faceCount, const int32_t* faceVertexIndices,
+                          uint32_t indexCount, uint32_t* outIndices, uint64_t outCapacity, uint64_t* outCount);
+/* ComputeSmoothNormals (reference SceneBridge.cpp:147-180), used when a
+ * mesh has no normal per point: cross(p1 - p0, p2 - p0) of each triangle
+ * added to its three vertices (triangles with an out-of-range index or a
+ * cross product of length <= 1e-8 skipped), then normalised; a vertex with
+ * a zero sum gets (0, 1, 0).  outNormals: 3 x vertexCount floats. */
+int shiro_hip_smooth_normals(const float* positions, uint32_t vertexCount, const uint32_t* indices,
+                             uint32_t indexCount, float* outNormals);
 
 /* ---- Environment image files: the pixel decoding EnvironmentMap::Load
  * gets from OpenImageIO (reference src/render/EnvironmentMap.cpp:49-92),

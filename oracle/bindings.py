@@ -255,7 +255,34 @@ def oracle_fns():
     f.accum_resolve = _bind(lib, "oracle_accum_resolve", None, _u, _u, _fp, _fp, _fp, _fp, _up, _fp, _fp, _fp,
                             _fp)
     f.bsdf_grad = _bind(lib, "oracle_bsdf_grad", None, _P(abi.Material), _up, _fp, _fp, _fp, _fp, _u, _fp, _fp)
+    f.triangulate = _bind(lib, "oracle_triangulate", C.c_int64, _P(C.c_int32), _u, _P(C.c_int32), _u, _up,
+                          C.c_uint64)
+    f.smooth_normals = _bind(lib, "oracle_smooth_normals", None, _fp, _u, _up, _u, _fp)
     return f
+
+
+def triangulate(face_vertex_counts, face_vertex_indices):
+    """The oracle's fan triangulation (SceneBridge.cpp:1009-1020); None for
+    malformed input."""
+    f = oracle_fns()
+    counts = np.ascontiguousarray(face_vertex_counts, np.int32)
+    idx = np.ascontiguousarray(face_vertex_indices, np.int32)
+    cp, ip = counts.ctypes.data_as(_P(C.c_int32)), idx.ctypes.data_as(_P(C.c_int32))
+    n = f.triangulate(cp, len(counts), ip, len(idx), None, 0)
+    if n < 0:
+        return None
+    out = np.zeros(n, np.uint32)
+    f.triangulate(cp, len(counts), ip, len(idx), abi.up(out), n)
+    return out
+
+
+def smooth_normals(positions, indices):
+    """The oracle's ComputeSmoothNormals (SceneBridge.cpp:147-180)."""
+    pos = np.ascontiguousarray(positions, np.float32).reshape(-1, 3)
+    idx = np.ascontiguousarray(indices, np.uint32).reshape(-1)
+    out = np.zeros_like(pos)
+    oracle_fns().smooth_normals(abi.fp(pos), len(pos), abi.up(idx), len(idx), abi.fp(out))
+    return out
 
 
 def reference_fns():

@@ -2095,11 +2095,80 @@ bool render_rows(const Scene& s, const Settings& st, const Camera& cam, uint32_t
 }  // namespace orc
 
 // ---------------------------------------------------------------------------
+// Mesh ingestion of the Hydra scene bridge (src/hydra/SceneBridge.cpp).
+// ---------------------------------------------------------------------------
+namespace orc {
+
+// Fan triangulation (SceneBridge.cpp:1009-1020): face f of c >= 3 vertices
+// gives (v0, vk, vk+1) for k = 1..c-2; smaller faces are skipped but their
+// indices consumed.  Returns the triangle index count, or -1 if a count is
+// negative or the faces need more indices than given.
+int64_t triangulate(const int32_t* counts, uint32_t faces, const int32_t* idx, uint32_t nidx,
+                    std::vector<uint32_t>* out) {
+    size_t off = 0;
+    for (uint32_t f = 0; f < faces; ++f) {
+        const int32_t c = counts[f];
+        if (c < 0 || off + static_cast<size_t>(c) > nidx) return -1;
+        if (c >= 3) {
+            const uint32_t root = static_cast<uint32_t>(idx[off]);
+            for (int32_t k = 1; k < c - 1; ++k) {
+                out->push_back(root);
+                out->push_back(static_cast<uint32_t>(idx[off + k]));
+                out->push_back(static_cast<uint32_t>(idx[off + k + 1]));
+            }
+        }
+        off += static_cast<size_t>(c);
+    }
+    return static_cast<int64_t>(out->size());
+}
+
+// ComputeSmoothNormals (SceneBridge.cpp:147-180): unnormalised face normals
+// cross(p1 - p0, p2 - p0) summed per vertex in triangle order, skipping
+// triangles with an out-of-range index or |cross| <= 1e-8; then normalised,
+// (0, 1, 0) where the sum is zero.
+void smooth_normals(const float* pos, uint32_t nv, const uint32_t* idx, uint32_t ni, float* out) {
+    std::vector<V3> acc(nv, V3{0.0f, 0.0f, 0.0f});
+    for (size_t t = 0; t + 2 < ni; t += 3) {
+        const uint32_t i0 = idx[t], i1 = idx[t + 1], i2 = idx[t + 2];
+        if (i0 >= nv || i1 >= nv || i2 >= nv) continue;
+        const V3 p0{pos[3 * i0], pos[3 * i0 + 1], pos[3 * i0 + 2]};
+        const V3 p1{pos[3 * i1], pos[3 * i1 + 1], pos[3 * i1 + 2]};
+        const V3 p2{pos[3 * i2], pos[3 * i2 + 1], pos[3 * i2 + 2]};
+        const V3 f = cross(p1 - p0, p2 - p0);
+        if (length(f) <= 1.0e-8f) continue;
+        acc[i0] = acc[i0] + f;
+        acc[i1] = acc[i1] + f;
+        acc[i2] = acc[i2] + f;
+    }
+    for (uint32_t v = 0; v < nv; ++v) {
+        const V3 n = length(acc[v]) > 0.0f ? normalize(acc[v]) : V3{0.0f, 1.0f, 0.0f};
+        out[3 * v] = n.x;
+        out[3 * v + 1] = n.y;
+        out[3 * v + 2] = n.z;
+    }
+}
+
+}  // namespace orc
+
+// ---------------------------------------------------------------------------
 // C ABI used by the tests (ctypes) and the CPU-baseline leg of the benchmark.
 // ---------------------------------------------------------------------------
 using namespace orc;
 
 extern "C" {
+
+int64_t oracle_triangulate(const int32_t* counts, uint32_t faces, const int32_t* idx, uint32_t nidx, uint32_t* out,
+                           uint64_t cap) {
+    std::vector<uint32_t> tri;
+    const int64_t n = triangulate(counts, faces, idx, nidx, &tri);
+    if (n < 0) return n;
+    if (out) std::copy(tri.begin(), tri.begin() + static_cast<int64_t>(std::min<uint64_t>(cap, tri.size())), out);
+    return n;
+}
+
+void oracle_smooth_normals(const float* pos, uint32_t nv, const uint32_t* idx, uint32_t ni, float* out) {
+    smooth_normals(pos, nv, idx, ni, out);
+}
 
 float oracle_qmc(uint32_t px, uint32_t py, uint32_t s, uint32_t dim) { return qmc_value(px, py, s, dim); }
 

@@ -173,11 +173,34 @@ uint64_t Signature(const Scene& s) {
     return h;
 }
 
+// FNV-1a over the meshes' vertex counts and index arrays: scenes that agree
+// on it differ only in vertex data, materials or lights, so the device BVH
+// can be refitted instead of rebuilt.
+uint64_t Topology(const Scene& s) {
+    uint64_t h = 1469598103934665603ull;
+    auto mix = [&](const void* p, size_t n) {
+        const unsigned char* b = static_cast<const unsigned char*>(p);
+        for (size_t i = 0; i < n; ++i) {
+            h ^= b[i];
+            h *= 1099511628211ull;
+        }
+    };
+    const size_t meshes = s.meshes.size();
+    mix(&meshes, sizeof(meshes));
+    for (const TriangleMesh& m : s.meshes) {
+        const size_t n = m.positions.size();
+        mix(&n, sizeof(n));
+        mix(m.indices.data(), m.indices.size() * sizeof(uint32_t));
+    }
+    return h;
+}
+
 }  // namespace
 
 struct HipBackend::RuntimeState {
     shiro_hip_ctx* ctx = nullptr;
     uint64_t signature = 0;
+    uint64_t topology = 0;
     bool hasScene = false;
     ~RuntimeState() {
         if (ctx) shiro_hip_destroy(ctx);
@@ -227,12 +250,18 @@ FrameBuffer HipBackend::RenderSampleBatch(const RenderRequest& request) const {
     if (!runtime_->hasScene || sig != runtime_->signature) {
         SceneView view;
         BuildView(request.scene, &view);
-        if (shiro_hip_upload_scene(runtime_->ctx, &view.scene) != SHIRO_HIP_OK) {
+        // Moved vertices on unchanged meshes (deformation): refit the BVH;
+        // anything else, or a failed refit: upload and rebuild.
+        const uint64_t topo = Topology(request.scene);
+        const bool refitted = runtime_->hasScene && topo == runtime_->topology &&
+                              shiro_hip_refit_scene(runtime_->ctx, &view.scene) == SHIRO_HIP_OK;
+        if (!refitted && shiro_hip_upload_scene(runtime_->ctx, &view.scene) != SHIRO_HIP_OK) {
             std::fprintf(stderr, "shiro hip upload failed: %s\n", shiro_hip_last_error(runtime_->ctx));
             runtime_->hasScene = false;
             return frame;
         }
         runtime_->signature = sig;
+        runtime_->topology = topo;
         runtime_->hasScene = true;
     }
     const Camera& c = request.camera;

@@ -86,6 +86,8 @@ struct shiro_hip_ctx {
     bool hasScene = false;
     bool emissive = false;
     DevBuf nodes, tris, hitdata, materials, derived, uvs, lightList, lightCdf, triClass;
+    Bvh4 bvh;               // host copy of the uploaded tree, for refits
+    uint64_t topology = 0;  // topology_hash of the uploaded scene
     EnvBuffers env[kMaxDomes];
     std::vector<std::unique_ptr<TexBuffers>> tex;
     DevBuf texTable;
@@ -492,11 +494,37 @@ int shiro_hip_supports_scene(const shiro_hip_scene* s) {
     return 1;
 }
 
-int shiro_hip_upload_scene(shiro_hip_ctx* ctx, const shiro_hip_scene* s) {
+// FNV-1a over what fixes the triangle list of a scene: mesh count, and per
+// mesh its vertex count and index array.
+uint64_t topology_hash(const shiro_hip_scene& s) {
+    uint64_t h = 1469598103934665603ull;
+    auto mix = [&h](const void* p, size_t n) {
+        const uint8_t* b = static_cast<const uint8_t*>(p);
+        for (size_t i = 0; i < n; ++i) {
+            h ^= b[i];
+            h *= 1099511628211ull;
+        }
+    };
+    mix(&s.meshCount, 4);
+    for (uint32_t m = 0; m < s.meshCount && s.meshes; ++m) {
+        const shiro_hip_mesh& M = s.meshes[m];
+        mix(&M.vertexCount, 4);
+        mix(&M.indexCount, 4);
+        if (M.indices) mix(M.indices, static_cast<size_t>(M.indexCount) * 4u);
+    }
+    return h;
+}
+
+// Scene upload; with `refit` the BVH of the uploaded scene is refitted to
+// the new vertex positions instead of rebuilt (same topology required).
+int upload_scene(shiro_hip_ctx* ctx, const shiro_hip_scene* s, bool refit) {
     if (!ctx) return SHIRO_HIP_INVALID;
     if (!s) return fail(ctx, SHIRO_HIP_INVALID, "scene is null");
     std::lock_guard<std::mutex> lock(ctx->mu);
     HIPC(hipSetDevice(ctx->device), "set device");
+    const uint64_t topo = topology_hash(*s);
+    if (refit && (!ctx->hasScene || topo != ctx->topology))
+        return fail(ctx, SHIRO_HIP_INVALID, "refit needs the uploaded scene's meshes and indices unchanged");
     ctx->hasScene = false;
     const auto t0 = std::chrono::steady_clock::now();
     FlatScene flat;
@@ -504,7 +532,12 @@ int shiro_hip_upload_scene(shiro_hip_ctx* ctx, const shiro_hip_scene* s) {
     if (!flatten_scene(*s, &flat, &err)) return fail(ctx, SHIRO_HIP_INVALID, err);
     const size_t nt = flat.material.size();
     Bvh4 bvh;
-    if (!build_bvh4(flat.tri.data(), nt, kMaxBvhDepth, &bvh, &err)) return fail(ctx, SHIRO_HIP_INVALID, err);
+    if (refit) {
+        bvh = std::move(ctx->bvh);
+        if (!refit_bvh4(&bvh, flat.tri.data(), nt, &err)) return fail(ctx, SHIRO_HIP_INVALID, err);
+    } else if (!build_bvh4(flat.tri.data(), nt, kMaxBvhDepth, &bvh, &err)) {
+        return fail(ctx, SHIRO_HIP_INVALID, err);
+    }
     if (!validate_bvh4(bvh, flat.tri.data(), nt, kMaxBvhDepth, &err)) return fail(ctx, SHIRO_HIP_INVALID, "BVH check: " + err);
     const auto t1 = std::chrono::steady_clock::now();
     // Emissive-triangle lights and the triangle -> light map.
@@ -685,9 +718,15 @@ int shiro_hip_upload_scene(shiro_hip_ctx* ctx, const shiro_hip_scene* s) {
     ctx->info.deviceBytes = bytes;
     ctx->info.buildSeconds = std::chrono::duration<double>(t1 - t0).count();
     ctx->info.uploadSeconds = std::chrono::duration<double>(t2 - t1).count();
+    ctx->bvh = std::move(bvh);
+    ctx->topology = topo;
     ctx->hasScene = true;
     return SHIRO_HIP_OK;
 }
+
+int shiro_hip_upload_scene(shiro_hip_ctx* ctx, const shiro_hip_scene* s) { return upload_scene(ctx, s, false); }
+
+int shiro_hip_refit_scene(shiro_hip_ctx* ctx, const shiro_hip_scene* s) { return upload_scene(ctx, s, true); }
 
 int shiro_hip_update_texture(shiro_hip_ctx* ctx, uint32_t index, const float* texels, uint32_t flags) {
     if (!ctx) return SHIRO_HIP_INVALID;

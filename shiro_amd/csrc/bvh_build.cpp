@@ -324,6 +324,78 @@ bool build_bvh4(const float* tris, size_t n, uint32_t maxDepth, Bvh4* out, std::
 
 namespace {
 
+struct Refitter {
+    Bvh4& b;
+    const float* tris;
+    float slack;
+
+    Box tri_box(uint32_t slot) const {
+        Box r;
+        const float* t = tris + 9u * static_cast<size_t>(b.order[slot]);
+        for (int a = 0; a < 3; ++a) {
+            r.lo[a] = std::min(t[a], std::min(t[3 + a], t[6 + a]));
+            r.hi[a] = std::max(t[a], std::max(t[3 + a], t[6 + a]));
+        }
+        return r;
+    }
+
+    // Requantises node id from its children's exact boxes; returns the union
+    // of those boxes (the node's exact box, before padding).
+    Box fit(uint32_t id) {
+        float* nd = &b.nodes[static_cast<size_t>(id) * kNodeFloats];
+        uint32_t ex, words[4];
+        std::memcpy(&ex, &nd[3], 4);
+        std::memcpy(words, &nd[4], 16);
+        const uint32_t valid = ex >> 24;
+        int nk = 0;
+        while (nk < 4 && (valid >> nk) & 1u) ++nk;
+        Box boxes[4], u;
+        for (int k = 0; k < nk; ++k) {
+            Box c;
+            if (words[k] & kLeafBit) {
+                for (uint32_t slot = words[k] & ~kLeafBit;; ++slot) {
+                    c.grow(tri_box(slot));
+                    if (b.leafEnd[slot]) break;
+                }
+            } else {
+                c = fit(words[k]);
+            }
+            u.grow(c);
+            boxes[k] = c;
+            pad(boxes[k]);
+            for (int a = 0; a < 3; ++a) {
+                boxes[k].lo[a] -= slack;
+                boxes[k].hi[a] += slack;
+            }
+        }
+        quantize_node(boxes, nk, words, nd);
+        return u;
+    }
+};
+
+}  // namespace
+
+bool refit_bvh4(Bvh4* b, const float* tris, size_t n, std::string* err) {
+    if (n != b->order.size()) {
+        *err = "refit: triangle count differs from the built tree's";
+        return false;
+    }
+    if (n == 0) return true;
+    Box all;
+    for (size_t i = 0; i < 3 * n; ++i)
+        for (int a = 0; a < 3; ++a) {
+            all.lo[a] = std::min(all.lo[a], tris[3 * i + a]);
+            all.hi[a] = std::max(all.hi[a], tris[3 * i + a]);
+        }
+    float scale = 0.0f;
+    for (int a = 0; a < 3; ++a) scale = std::max(scale, std::max(std::fabs(all.lo[a]), std::fabs(all.hi[a])));
+    Refitter R{*b, tris, scale * 16.0f * FLT_EPSILON};
+    R.fit(0);
+    return true;
+}
+
+namespace {
+
 // Dequantised box of child k of a node, in double precision.
 void child_box(const float* nd, int k, double lo[3], double hi[3]) {
     uint32_t ex, q[6];

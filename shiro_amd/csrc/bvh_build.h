@@ -32,6 +32,12 @@ struct Bvh4 {
 // levels.
 bool build_bvh4(const float* tris, size_t n, uint32_t maxDepth, Bvh4* out, std::string* err);
 
+// Refit: recomputes every box of a built tree for moved vertices of the
+// same n triangles (same order as given to build_bvh4), keeping the tree's
+// topology and leaf order.  Boxes are padded and quantised as in the build,
+// so refitting unmoved triangles reproduces the built node array exactly.
+bool refit_bvh4(Bvh4* b, const float* tris, size_t n, std::string* err);
+
 // Check of a built tree before upload: every internal child word names an
 // existing node other than the root, every leaf word names a triangle slot
 // whose run ends with an end-of-leaf flag inside the array, every node is

@@ -94,6 +94,9 @@ def load_library(path: str = LIB_PATH):
     d("shiro_hip_render_adjoint", C.c_int, V, _P(abi.Camera), _P(abi.Settings), _u, _u, _fp, _P(_fp), _u)
     d("shiro_hip_render_many", C.c_int, V, _P(abi.BatchItem), _u, _u)
     d("shiro_hip_autotune_many", C.c_int, V, _P(abi.BatchItem), _u, _P(abi.Tuning), C.c_char_p, C.c_size_t)
+    d("shiro_hip_refit_scene", C.c_int, V, _P(abi.Scene))
+    d("shiro_hip_triangulate", C.c_int, _P(C.c_int32), _u, _P(C.c_int32), _u, _up, C.c_uint64, _P(C.c_uint64))
+    d("shiro_hip_smooth_normals", C.c_int, _fp, _u, _up, _u, _fp)
     d("shiro_hip_load_image", C.c_int, C.c_char_p, _P(abi.Image), C.c_char_p, C.c_size_t)
     d("shiro_hip_load_image_memory", C.c_int, C.c_void_p, C.c_size_t, _P(abi.Image), C.c_char_p, C.c_size_t)
     d("shiro_hip_free_image", None, _P(abi.Image))
@@ -123,6 +126,58 @@ def load_image(source) -> tuple:
         return px, int(img.channels), int(img.format)
     finally:
         L.shiro_hip_free_image(C.byref(img))
+
+
+def triangulate(face_vertex_counts, face_vertex_indices) -> np.ndarray:
+    """Fan triangulation of polygon faces as Shiro's Hydra scene bridge does
+    it (SceneBridge.cpp:1009-1020); returns the triangle indices (u32)."""
+    L = load_library()
+    counts = np.ascontiguousarray(face_vertex_counts, np.int32)
+    idx = np.ascontiguousarray(face_vertex_indices, np.int32)
+    n = C.c_uint64(0)
+    cp, ip = counts.ctypes.data_as(_P(C.c_int32)), idx.ctypes.data_as(_P(C.c_int32))
+    rc = L.shiro_hip_triangulate(cp, len(counts), ip, len(idx), None, 0, C.byref(n))
+    if rc:
+        raise HipError(rc, "bad polygon face counts or indices")
+    out = np.zeros(n.value, np.uint32)
+    rc = L.shiro_hip_triangulate(cp, len(counts), ip, len(idx), abi.up(out), len(out), C.byref(n))
+    if rc:
+        raise HipError(rc, "triangulation failed")
+    return out
+
+
+def smooth_normals(positions, indices) -> np.ndarray:
+    """Area-weighted vertex normals of a triangle mesh
+    (ComputeSmoothNormals, SceneBridge.cpp:147-180); (V, 3) float32."""
+    L = load_library()
+    pos = np.ascontiguousarray(positions, np.float32).reshape(-1, 3)
+    idx = np.ascontiguousarray(indices, np.uint32).reshape(-1)
+    out = np.zeros_like(pos)
+    rc = L.shiro_hip_smooth_normals(abi.fp(pos), len(pos), abi.up(idx), len(idx), abi.fp(out))
+    if rc:
+        raise HipError(rc, "smooth normals failed")
+    return out
+
+
+def polygon_mesh(points, face_vertex_counts, face_vertex_indices, normals=None, material_index: int = 0):
+    """A TriangleMesh from a polygon mesh the way the Hydra scene bridge
+    builds one (SceneBridge.cpp:967-1032, points already in world space):
+    fan-triangulated faces, the given per-point normals normalised, or smooth
+    normals when there is not one per point; None when nothing renderable
+    (fewer than 3 points or triangle indices) is left."""
+    from .scene import TriangleMesh
+    pts = np.ascontiguousarray(points, np.float32).reshape(-1, 3)
+    idx = triangulate(face_vertex_counts, face_vertex_indices)
+    if len(pts) < 3 or len(idx) < 3:
+        return None
+    if normals is not None and len(normals) == len(pts):
+        n = np.asarray(normals, np.float32).reshape(-1, 3)
+        ln = np.sqrt(n[:, 0] * n[:, 0] + n[:, 1] * n[:, 1] + n[:, 2] * n[:, 2])
+        safe = np.where(ln > 0, ln, np.float32(1))[:, None]
+        nrm = np.where((ln > 0)[:, None], n / safe, np.float32(0)).astype(np.float32)
+    else:
+        nrm = smooth_normals(pts, idx)
+    return TriangleMesh(pts, nrm, idx, material_index)
 
 
 def load_dome_light(path, layout: int = abi.ENV_AUTOMATIC, **kw):
@@ -155,6 +210,17 @@ def scene_signature(scene: Scene) -> str:
     for t in scene.textures:
         h.update(np.asarray(np.shape(t.texels), np.int64).tobytes())
         h.update(np.ascontiguousarray(t.texels, np.float32).tobytes())
+    return h.hexdigest()
+
+
+def topology_signature(scene: Scene) -> str:
+    """Hash of what fixes a scene's triangle list (mesh count, vertex counts,
+    index arrays): scenes that share it can be refitted into each other."""
+    h = hashlib.blake2b(digest_size=16)
+    h.update(len(scene.meshes).to_bytes(8, "little"))
+    for m in scene.meshes:
+        h.update(len(m.positions).to_bytes(8, "little"))
+        h.update(np.ascontiguousarray(m.indices, np.uint32).tobytes())
     return h.hexdigest()
 
 
@@ -215,6 +281,14 @@ class Context:
         m = marshal(scene)
         self._check(self.L.shiro_hip_upload_scene(self.h, m.ptr))
         self._sig = sig
+
+    def refit(self, scene: Scene):
+        """Uploads a scene whose meshes and indices are the uploaded scene's
+        (moved vertices, other materials or lights) and refits the BVH
+        instead of rebuilding it (shiro_hip_refit_scene)."""
+        m = marshal(scene)
+        self._check(self.L.shiro_hip_refit_scene(self.h, m.ptr))
+        self._sig = scene_signature(scene)
 
     def scene_info(self) -> abi.SceneInfo:
         info = abi.SceneInfo()
@@ -464,7 +538,20 @@ class HipBackend:
         if ctx is None:
             return FrameBuffer(st.width, st.height)
         try:
-            ctx.upload(request.scene)
+            if scene_signature(request.scene) != ctx._sig:
+                # Moved vertices on unchanged meshes: refit the BVH; else
+                # upload and rebuild.
+                topo = topology_signature(request.scene)
+                refitted = False
+                if ctx._sig is not None and topo == getattr(ctx, "_topo", None):
+                    try:
+                        ctx.refit(request.scene)
+                        refitted = True
+                    except HipError:
+                        refitted = False
+                if not refitted:
+                    ctx.upload(request.scene, force=True)
+                ctx._topo = topo
             return ctx.render(request.camera, st, request.sampleStart, request.sampleCount, request.cancel)
         except HipError as e:
             import sys
