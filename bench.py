@@ -228,9 +228,12 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-count", action="store_true")
+    ap.add_argument("--no-parity", action="store_true", help="skip the other transport's comparison frame")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default wavefront tuning")
     ap.add_argument("--fetch-mode", type=int, default=None, help="force the traversal fetch mode (0, 1, 2)")
     ap.add_argument("--paths-per-pass", type=int, default=None, help="force the wavefront paths per pass")
+    ap.add_argument("--primary-packets", type=int, default=None, choices=[0, 1],
+                    help="force tile-packet tracing of camera rays off (0) or on (1)")
     ap.add_argument("--shade-order", type=int, default=None, choices=[0, 1],
                     help="force shading-class ordering of the hit queue off (0) or on (1)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -258,10 +261,9 @@ def main():
         bench_c4(args)
         return
     import torch
-    from shiro_amd import workloads
+    from shiro_amd import abi, workloads
     from shiro_amd.hip import Context
-
-    from shiro_amd import abi
+    from shiro_amd.scene import RenderSettings
     wl = workloads.WORKLOADS[args.workload]()
     st = wl.settings
     st.transport = abi.TRANSPORT_WAVEFRONT if args.transport == "wavefront" else abi.TRANSPORT_MEGAKERNEL
@@ -291,9 +293,11 @@ def main():
                   flush=True)
         if args.shade_order is not None:
             ctx.set_tuning(shadeOrder=args.shade_order)
+        if args.primary_packets is not None:
+            ctx.set_tuning(primaryPackets=args.primary_packets)
         t = ctx.get_tuning()
         tuning = {"pathsPerPass": t.pathsPerPass, "fetchMode": t.fetchMode, "traceBlocks": t.traceBlocks,
-                  "stageBlocks": t.stageBlocks, "shadeOrder": t.shadeOrder,
+                  "stageBlocks": t.stageBlocks, "shadeOrder": t.shadeOrder, "primaryPackets": t.primaryPackets,
                   "autotuned": args.fetch_mode is None and args.paths_per_pass is None and not args.no_autotune}
     dev = torch.device("cuda:0")
     bufs = [torch.empty(npx * k, dtype=torch.float32, device=dev) for k in (4, 3, 3, 1)]
@@ -371,6 +375,29 @@ def main():
                                    "shadow_node_fetches": c.shadowNodeFetches / ns,
                                    "shadow_tri_tests": c.shadowTriFetches / ns,
                                    "env_taps": c.envTaps / ns, "cdf_steps": c.cdfSteps / ns}}
+    # Outside the timed region: the other transport's frame on the same
+    # samples (the throughput comparison is at matched error only if the
+    # frames agree; both transports evaluate the same per-sample operations,
+    # so they are expected to be identical) and the end-to-end time of a
+    # host-buffer batch, readback included.
+    parity = None
+    if not args.no_parity:
+        other = RenderSettings(**{**st.__dict__, "transport": abi.TRANSPORT_MEGAKERNEL
+                                  if st.transport == abi.TRANSPORT_WAVEFRONT else abi.TRANSPORT_WAVEFRONT})
+        obufs = [torch.empty_like(b) for b in bufs]
+        ctx.render_device(wl.camera, other, [b.data_ptr() for b in obufs], 0, st.samplesPerPixel, synchronize=True)
+        other_ms = ctx.last_render_ms()
+        torch.cuda.synchronize()
+        diff = (bufs[0] != obufs[0]).view(-1, 4).any(dim=1).float().mean().item()
+        parity = {"other_transport": "megakernel" if other.transport == abi.TRANSPORT_MEGAKERNEL else "wavefront",
+                  "other_ms": round(other_ms, 2), "speedup_vs_other": round(other_ms / (sum(gpu_ms) / len(gpu_ms)), 3),
+                  "beauty_identical": all(bool(torch.equal(a, b)) for a, b in zip(bufs, obufs)),
+                  "pixels_differing": diff}
+        del obufs
+    t0 = time.perf_counter()
+    host = ctx.render(wl.camera, st)
+    e2e_ms = (time.perf_counter() - t0) * 1e3
+    del host
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(wl, args.cpu_seconds, args.cpu_threads)
@@ -383,7 +410,8 @@ def main():
         "config": {"workload": workload_desc(wl), "transport": args.transport, "tuning": tuning,
                    "samples_per_step": samples,
                    "bvh": f"BVH4 {info.bvhNodeCount} nodes, depth {info.bvhDepth}",
-                   "scene_upload_s": round(upload_s, 2), "finite_output": ok},
+                   "scene_upload_s": round(upload_s, 2), "finite_output": ok,
+                   "end_to_end_ms_with_readback": round(e2e_ms, 2), "parity": parity},
         "roofline": roofline, "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

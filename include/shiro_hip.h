@@ -443,7 +443,8 @@ typedef struct shiro_hip_tuning {
     uint32_t stageBlocks;   /* grid cap of the grid-stride stage kernels (default 16384) */
     uint32_t batchBlocksPerCU; /* one-wave blocks per CU of the batched persistent kernel (default 8) */
     uint32_t shadeOrder;    /* 1: shade bounces after the first in shading-class order (default 1) */
-    uint32_t reserved[2];
+    uint32_t primaryPackets; /* 1: camera rays traced as 64-ray tile packets (default 1) */
+    uint32_t reserved[1];
 } shiro_hip_tuning;
 int shiro_hip_set_tuning(shiro_hip_ctx* ctx, const shiro_hip_tuning* tuning);
 int shiro_hip_get_tuning(const shiro_hip_ctx* ctx, shiro_hip_tuning* tuning);

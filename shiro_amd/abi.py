@@ -133,7 +133,7 @@ class Tuning(C.Structure):
     _fields_ = [
         ("pathsPerPass", C.c_uint32), ("fetchMode", C.c_uint32), ("traceBlocks", C.c_uint32),
         ("stageBlocks", C.c_uint32), ("batchBlocksPerCU", C.c_uint32), ("shadeOrder", C.c_uint32),
-        ("reserved", C.c_uint32 * 2),
+        ("primaryPackets", C.c_uint32), ("reserved", C.c_uint32 * 1),
     ]
 
 

@@ -104,7 +104,7 @@ struct shiro_hip_ctx {
     DevBuf acB, acA, acN, acD, acC;
     uint32_t acW = 0, acH = 0, acTotal = 0;
     // Wavefront tuning (shiro_hip_set_tuning / shiro_hip_autotune).
-    shiro_hip_tuning tuning{64u << 20, 0u, 8192u, 16384u, 8u, 1u, {0, 0}};
+    shiro_hip_tuning tuning{64u << 20, 0u, 8192u, 16384u, 8u, 1u, 1u, {0}};
     uint32_t computeUnits = 256;
     // Batched rendering: items, work list and the work counter.
     DevBuf manyItems, manyWork, manyCounter;
@@ -297,6 +297,7 @@ int enqueue_wavefront(shiro_hip_ctx* ctx, const DevScene& scene, const shiro_hip
     L.maxBlocks = ctx->tuning.stageBlocks;
     L.traceBlocks = ctx->tuning.traceBlocks;
     L.fetchMode = ctx->tuning.fetchMode;
+    L.primaryPackets = ctx->tuning.primaryPackets;
     if (events) {
         HIPC(hipEventRecord(ctx->ev0, ctx->stream), "event");
         begin_marks(ctx);
@@ -1023,7 +1024,8 @@ int shiro_hip_set_tuning(shiro_hip_ctx* ctx, const shiro_hip_tuning* t) {
     if (!ctx || !t) return SHIRO_HIP_INVALID;
     std::lock_guard<std::mutex> lock(ctx->mu);
     if (t->pathsPerPass == 0 || t->fetchMode > 2u || t->traceBlocks == 0 || t->stageBlocks == 0 ||
-        t->batchBlocksPerCU == 0 || t->batchBlocksPerCU > 64u || t->shadeOrder > 1u)
+        t->batchBlocksPerCU == 0 || t->batchBlocksPerCU > 64u || t->shadeOrder > 1u ||
+        t->primaryPackets > 1u)
         return fail(ctx, SHIRO_HIP_INVALID, "invalid tuning");
     ctx->tuning = *t;
     return SHIRO_HIP_OK;

@@ -57,6 +57,7 @@ struct WfLaunch {
     uint32_t maxBlocks;    // grid of the grid-stride stage kernels
     uint32_t traceBlocks;  // grid of the persistent traversal kernels
     uint32_t fetchMode;    // traversal work distribution, see k_wf_trace
+    uint32_t primaryPackets;  // 1: camera rays traced as tile packets (k_wf_primary)
 };
 
 // Optional per-kernel timing: an event recorded after each launch, tagged
