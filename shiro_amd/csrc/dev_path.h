@@ -512,6 +512,7 @@ DEV bool sample_bounce(const DevScene& S, const DevSettings& st, uint32_t depth,
     float pdf = 1.0f;
     uint32_t type = kBounceDiffuse;
     bool delta = false, pickT = false, pickS = false, pickC = false;
+    bool needEval = false, needPdf = false;
     if (sel < wt) {
         pickT = true;
     } else {
@@ -554,8 +555,7 @@ DEV bool sample_bounce(const DevScene& S, const DevSettings& st, uint32_t depth,
             if (!refract_dir(rd, mn, etaRatio, &next)) {
                 next = normalize(reflect(rd, mn));
                 if (wr <= 0.0f) return false;
-                f = bsdf_eval(M, n, cn, v, next);
-                pdf = fmaxs((wr / wsum) * bsdf_pdf(M, n, cn, v, next), 1.0e-6f);
+                needEval = needPdf = true;
                 type = kBounceReflective;
                 delta = rT <= kDeltaRoughness;
                 pickT = false;
@@ -592,8 +592,7 @@ DEV bool sample_bounce(const DevScene& S, const DevSettings& st, uint32_t depth,
         next = normalize(reflect(rd, hv));
         const float rc = clampf(dot(sn, next), 0.0f, 1.0f);
         if (rc <= 0.0f) return false;
-        pdf = fmaxs((wr / wsum) * bsdf_pdf(M, n, cn, v, next), 1.0e-6f);
-        f = bsdf_eval(M, n, cn, v, next);
+        needEval = needPdf = true;
         delta = sr <= kDeltaRoughness;
     } else {
         if (nDiffuse >= st.diffuseDepth) return false;
@@ -603,8 +602,13 @@ DEV bool sample_bounce(const DevScene& S, const DevSettings& st, uint32_t depth,
         next = sample_cosine(n, du);
         const float cpdf = clampf(dot(n, next), 0.0f, 1.0f) * kInvPi;
         pdf = fmaxs((wr / wsum) * (1.0f * cpdf + 0.0f * 0.0f), 1.0e-6f);
-        f = bsdf_eval(M, n, cn, v, next);
+        needEval = true;
     }
+    // The reflective lobes evaluate the BSDF (and its mixture pdf) once,
+    // after the lobe branches, so that a wave whose lanes picked different
+    // lobes runs these calls once rather than once per branch.
+    if (needPdf) pdf = fmaxs((wr / wsum) * bsdf_pdf(M, n, cn, v, next), 1.0e-6f);
+    if (needEval) f = bsdf_eval(M, n, cn, v, next);
 
     const float cosT = pickT ? fmaxs(fabsf(dot(ng, next)), 1.0e-4f) : clampf(dot(n, next), 0.0f, 1.0f);
     if (cosT <= 0.0f) return false;
