@@ -357,6 +357,27 @@ int shiro_hip_render_adjoint(
     const shiro_hip_settings* settings, uint32_t sampleStart,
     uint32_t sampleCount, const float* dBeauty, float* const* dTexels,
     uint32_t dTexelsCount);
+/* shiro_hip_render_adjoint that also ADDS to dEnv[i] (device, W*H*3 floats
+ * in the layout of dome light i's pixels; NULL entries skipped) the gradient
+ * with respect to dome i's map texels.  The maps enter the render through
+ * the radiance escaped rays and dome light samples read (bilinear lookups);
+ * their importance tables are sampling and are detached, so this is the
+ * gradient of the render whose tables stay those of the current texels
+ * (shiro_hip_update_environment with SHIRO_HIP_TEXELS_EVAL_ONLY). */
+int shiro_hip_render_adjoint_env(
+    shiro_hip_ctx* ctx, const shiro_hip_camera* camera,
+    const shiro_hip_settings* settings, uint32_t sampleStart,
+    uint32_t sampleCount, const float* dBeauty, float* const* dTexels,
+    uint32_t dTexelsCount, float* const* dEnv, uint32_t dEnvCount);
+/* Replaces the texels of dome light i's map (W*H*3 floats in the layout of
+ * its pixels; host memory, or device memory with SHIRO_HIP_TEXELS_DEVICE)
+ * without re-uploading the scene, the EnvironmentMap of a DomeLight changing
+ * between renders (no counterpart in the reference, which reloads the map).
+ * The importance tables are rebuilt on the device with the host build's
+ * operation order (the same map samples exactly as if uploaded), unless
+ * SHIRO_HIP_TEXELS_EVAL_ONLY keeps the current tables (frozen sampling, for
+ * finite differences of the adjoint).  Synchronous. */
+int shiro_hip_update_environment(shiro_hip_ctx* ctx, uint32_t dome, const float* texels, uint32_t flags);
 
 int shiro_hip_scene_info_get(const shiro_hip_ctx* ctx, shiro_hip_scene_info* out);
 

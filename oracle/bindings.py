@@ -180,17 +180,33 @@ class OracleScene(_SceneBase):
         if rc:
             raise IndexError(index)
 
-    def render_adjoint(self, camera, settings, dbeauty, wanted, sample_start=0, sample_count=None, threads=8):
+    def update_environment(self, dome, texels, eval_only=False):
+        arr = np.ascontiguousarray(texels, np.float32)
+        rc = self._fn("update_environment", C.c_int, C.c_void_p, _u, _fp, C.c_int)(self.h, dome, abi.fp(arr),
+                                                                                    int(eval_only))
+        if rc:
+            raise IndexError(dome)
+
+    def render_adjoint(self, camera, settings, dbeauty, wanted, sample_start=0, sample_count=None, threads=8,
+                       env_wanted=()):
         """Texel gradients of sum(dbeauty * beauty); `wanted` lists texture
-        indices; returns {index: gradient array shaped like the texels}."""
+        indices, `env_wanted` dome indices; returns {index: gradient shaped
+        like the texels} and, with env_wanted, also {("env", dome): gradient
+        shaped like the dome's pixels}."""
         sample_count = settings.samplesPerPixel if sample_count is None else sample_count
         db = np.ascontiguousarray(dbeauty, np.float32).reshape(-1)
         ntex = len(self.scene.textures)
         grads = {i: np.zeros(np.shape(self.scene.textures[i].texels), np.float32) for i in wanted}
         ptrs = (_fp * max(1, ntex))(*[abi.fp(grads[i]) if i in grads else None for i in range(ntex)])
+        ndome = len(self.scene.domeLights)
+        env = {d: np.zeros(np.shape(self.scene.domeLights[d].pixels), np.float32) for d in env_wanted}
+        eptrs = (_fp * max(1, ndome))(*[abi.fp(env[d]) if d in env else None for d in range(ndome)])
         cam, st = camera.to_c(), settings.to_c()
-        self._fn("render_adjoint", C.c_int, C.c_void_p, _P(abi.Camera), _P(abi.Settings), _u, _u, _fp, _P(_fp), _u,
-                 _u)(self.h, C.byref(cam), C.byref(st), sample_start, sample_count, abi.fp(db), ptrs, ntex, threads)
+        self._fn("render_adjoint_env", C.c_int, C.c_void_p, _P(abi.Camera), _P(abi.Settings), _u, _u, _fp, _P(_fp),
+                 _u, _P(_fp), _u, _u)(self.h, C.byref(cam), C.byref(st), sample_start, sample_count, abi.fp(db), ptrs,
+                                      ntex, eptrs, ndome, threads)
+        for d, g in env.items():
+            grads[("env", d)] = g
         return grads
 
 

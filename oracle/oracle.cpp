@@ -223,6 +223,61 @@ V3 env_eval(const Env& e, V3 dir) {
     return e.layout == SHIRO_HIP_ENV_ANGULAR ? env_eval_angular(e, dir) : env_eval_latlong(e, dir);
 }
 
+// Texel indices and weights of the bilinear lookup env_eval makes at a
+// map-space direction (texture extension: the dome-map adjoint); false
+// where the map adds nothing.  d env_eval / d px[idx[k]] = w[k].
+struct EnvTaps {
+    size_t idx[4];
+    float w[4];
+};
+
+void env_bilinear_taps(const Env& e, float u, float v, EnvTaps* t) {
+    u = wrap01(u) * static_cast<float>(e.w) - 0.5f;
+    v = clampf(v, 0.0f, 1.0f) * static_cast<float>(e.h) - 0.5f;
+    const int x0 = static_cast<int>(std::floor(u));
+    const int y0 = static_cast<int>(std::floor(v));
+    const int y1 = std::min(y0 + 1, static_cast<int>(e.h) - 1);
+    const float fx = u - static_cast<float>(x0);
+    const float fy = v - static_cast<float>(y0);
+    const int W = static_cast<int>(e.w);
+    auto at = [&](int x, int y) {
+        int xx = x % W;
+        if (xx < 0) xx += W;
+        const int yy = std::clamp(y, 0, static_cast<int>(e.h) - 1);
+        return static_cast<size_t>(yy) * e.w + static_cast<size_t>(xx);
+    };
+    t->idx[0] = at(x0, y0);
+    t->idx[1] = at(x0 + 1, y0);
+    t->idx[2] = at(x0, y1);
+    t->idx[3] = at(x0 + 1, y1);
+    t->w[0] = (1.0f - fx) * (1.0f - fy);
+    t->w[1] = fx * (1.0f - fy);
+    t->w[2] = (1.0f - fx) * fy;
+    t->w[3] = fx * fy;
+}
+
+bool env_taps(const Env& e, V3 dir, EnvTaps* t) {
+    if (e.px.empty() || e.w == 0 || e.h == 0) return false;
+    const V3 d = normalize(dir);
+    if (e.layout == SHIRO_HIP_ENV_ANGULAR) {
+        const float radial = std::sqrt(d.x * d.x + d.y * d.y);
+        if (radial <= 1.0e-6f) {
+            env_bilinear_taps(e, 0.5f, 0.5f, t);
+            return true;
+        }
+        const float r = std::acos(clampf(d.z, -1.0f, 1.0f)) / kPi;
+        const float u = 0.5f + 0.5f * r * (d.x / radial);
+        const float v = 0.5f + 0.5f * r * (d.y / radial);
+        if (u < 0.0f || u > 1.0f || v < 0.0f || v > 1.0f) return false;
+        env_bilinear_taps(e, u, v, t);
+        return true;
+    }
+    const float phi = std::atan2(d.x, d.z);
+    const float theta = std::acos(clampf(d.y, -1.0f, 1.0f));
+    env_bilinear_taps(e, wrap01(phi / kTwoPi), clampf(theta / kPi, 0.0f, 1.0f), t);
+    return true;
+}
+
 // EnvironmentMap.cpp:315-330: lower_bound over cdf[begin+1 .. end] of the
 // clamped sample; returns the bucket (relative to begin) and its mass.
 uint32_t env_sample_cdf(const std::vector<float>& cdf, uint32_t begin, uint32_t end, float u, float* mass) {
@@ -503,7 +558,7 @@ float dome_pdf(const Dome& d, V3 dir) {
     return kInvFourPi;
 }
 
-bool dome_sample(const Dome& d, V2 u, V3* dir, V3* rad, float* pdf) {
+bool dome_sample(const Dome& d, V2 u, V3* dir, V3* rad, float* pdf, V3* localOut = nullptr) {
     V3 ld{};
     float lp = 0.0f;
     V3 lr{};
@@ -514,6 +569,7 @@ bool dome_sample(const Dome& d, V2 u, V3* dir, V3* rad, float* pdf) {
         lp = kInvFourPi;
         lr = {1.0f, 1.0f, 1.0f};
     }
+    if (localOut) *localOut = ld;
     if (lp <= 0.0f) return false;
     *dir = dome_to_world(d, ld);
     *rad = lr * d.radiance;
@@ -1533,7 +1589,12 @@ struct Adjoint {
     V3 T;                                   // throughput at the current vertex
     std::vector<AdjVtx> tape;
     std::vector<std::vector<float>>* grads; // per texture (empty: not wanted)
+    std::vector<std::vector<float>>* envGrads = nullptr;  // per dome: RGB texel gradients (empty: not wanted)
 };
+
+// Adds g (per unit of dome li's map radiance at map-space direction
+// `local`) to the map's texel gradients through the lookup's footprint.
+void scatter_env(const Scene& s, Adjoint* adj, size_t li, V3 local, V3 g);
 
 V3 clamp_max_vjp(V3 v, float lim, V3 g) {
     const float c = max3(v);
@@ -1591,6 +1652,21 @@ void scatter_grad(const Scene& s, Adjoint& adj, const Material& m, V2 uv, V3 gBa
     }
 }
 
+void scatter_env(const Scene& s, Adjoint* adj, size_t li, V3 local, V3 g) {
+    if (!adj->envGrads || li >= adj->envGrads->size() || (*adj->envGrads)[li].empty()) return;
+    const Dome& d = s.domes[li];
+    if (!d.env) return;
+    EnvTaps t;
+    if (!env_taps(*d.env, local, &t)) return;
+    const V3 gr = g * d.radiance;
+    std::vector<float>& out = (*adj->envGrads)[li];
+    for (int k = 0; k < 4; ++k) {
+        out[t.idx[k] * 3] += gr.x * t.w[k];
+        out[t.idx[k] * 3 + 1] += gr.y * t.w[k];
+        out[t.idx[k] * 3 + 2] += gr.z * t.w[k];
+    }
+}
+
 // Gradient of one visible light sample contrib = clamp(f * lf, 24) that
 // enters the sum with weight 1 / K.
 void light_grad(Adjoint* adj, const Material& me, V3 n, V3 cn, V3 v, V3 ld, V3 f, V3 lf, float invK, V3* gBase,
@@ -1620,11 +1696,11 @@ V3 direct_lighting(const Scene& s, const Settings& st, const Hit& hit, V3 n, V3 
     for (size_t li = 0; li < s.domes.size(); ++li) {
         V3 acc{};
         for (uint32_t k = 0; k < K; ++k) {
-            V3 ld{}, lr{};
+            V3 ld{}, lr{}, local{};
             float lp = 0.0f;
             const uint32_t branch =
                 kDomeBranchBase + depth * kDomeBranchDepth + static_cast<uint32_t>(li) * kDomeBranchLight + k * kDomeBranchSample;
-            if (!dome_sample(s.domes[li], smp.next2(branch), &ld, &lr, &lp) || lp <= 0.0f) continue;
+            if (!dome_sample(s.domes[li], smp.next2(branch), &ld, &lr, &lp, &local) || lp <= 0.0f) continue;
             const float nl = clampf(dot(n, ld), 0.0f, 1.0f);
             if (nl <= 0.0f || occluded(s, origin, ld)) continue;
             const float bp = bsdf_pdf(ms, n, cn, v, ld);
@@ -1632,6 +1708,12 @@ V3 direct_lighting(const Scene& s, const Settings& st, const Hit& hit, V3 n, V3 
             const V3 f = bsdf_eval(m, n, cn, v, ld);
             acc = acc + clamp_max_component(f * lr * (nl * w / lp), kDirectClamp);
             light_grad(adj, m, n, cn, v, ld, f, lr * (nl * w / lp), 1.0f / static_cast<float>(K), gBase, gRough);
+            if (adj && adj->envGrads) {
+                // lr = env(local) * radiance: the clamp's VJP times f times nl*w/lp.
+                const V3 gc = adj->a * adj->T * (1.0f / static_cast<float>(K));
+                const V3 gy = clamp_max_vjp(f * lr * (nl * w / lp), kDirectClamp, gc);
+                scatter_env(s, adj, li, local, gy * f * (nl * w / lp));
+            }
         }
         L = L + acc / static_cast<float>(K);
     }
@@ -1717,6 +1799,7 @@ PathResult trace_path(const Scene& s, const Settings& st, Ray ray, const Sampler
             if (depth > 0 || st.backgroundVisible) {
                 const V3 Le = environment_radiance(s, ray.d);
                 V3 dL;
+                float mis = 1.0f;
                 if (s.domes.empty() || depth == 0 || prevDelta || prevType == kTransmission) {
                     dL = T * Le;
                     factor = Le;
@@ -1724,7 +1807,11 @@ PathResult trace_path(const Scene& s, const Settings& st, Ray ray, const Sampler
                     const float w = power_heuristic_many(prevPdf, env_pdf_sq_sum(s, ray.d));
                     dL = T * Le * w;
                     factor = Le * w;
+                    mis = w;
                 }
+                if (adj && adj->envGrads)
+                    for (size_t li = 0; li < s.domes.size(); ++li)
+                        scatter_env(s, adj, li, world_to_dome(s.domes[li], ray.d), adj->a * T * mis);
                 guide_accumulate(recs, dL, upd);
                 res.radiance = res.radiance + dL;
             }
@@ -2566,34 +2653,64 @@ int oracle_update_texture(void* h, uint32_t index, const float* texels, int eval
 
 // Adds to grads[i] (size of texture i; null = skip) the gradient of
 // sum_{pixel, c} dBeauty[pixel*4 + c] * beauty[pixel][c] over samples
-// [sampleStart, sampleStart + sampleCount), beauty being the batch mean.
-// Row bands over `threads` threads, each with its own gradient buffers,
-// summed in thread order at the end.
+// [sampleStart, sampleStart + sampleCount), beauty being the batch mean,
+// and to envGrads[i] (w*h*3 of dome i's map; null = skip) its gradient
+// w.r.t. that map's texels, `threads` threads each with its own buffers.
+int oracle_render_adjoint_env(void* h, const shiro_hip_camera* camera, const shiro_hip_settings* settings,
+                              uint32_t sampleStart, uint32_t sampleCount, const float* dBeauty, float* const* grads,
+                              uint32_t gradCount, float* const* envGrads, uint32_t envCount, uint32_t threads);
+
 int oracle_render_adjoint(void* h, const shiro_hip_camera* camera, const shiro_hip_settings* settings,
                           uint32_t sampleStart, uint32_t sampleCount, const float* dBeauty, float* const* grads,
                           uint32_t gradCount, uint32_t threads) {
+    return oracle_render_adjoint_env(h, camera, settings, sampleStart, sampleCount, dBeauty, grads, gradCount,
+                                     nullptr, 0, threads);
+}
+
+// Replaces dome `dome`'s map texels (w*h*3); without evalOnly the importance
+// tables are rebuilt from them, with it they stay (frozen sampling).
+int oracle_update_environment(void* h, uint32_t dome, const float* texels, int evalOnly) {
+    Scene& s = *static_cast<Scene*>(h);
+    if (dome >= s.domes.size() || !s.domes[dome].env) return 1;
+    Env& e = *s.domes[dome].env;
+    for (size_t p = 0; p < e.px.size(); ++p) e.px[p] = {texels[3 * p], texels[3 * p + 1], texels[3 * p + 2]};
+    if (!evalOnly) env_build_tables(e);
+    return 0;
+}
+
+int oracle_render_adjoint_env(void* h, const shiro_hip_camera* camera, const shiro_hip_settings* settings,
+                              uint32_t sampleStart, uint32_t sampleCount, const float* dBeauty, float* const* grads,
+                              uint32_t gradCount, float* const* envGrads, uint32_t envCount, uint32_t threads) {
     const Scene& s = *static_cast<Scene*>(h);
     const Settings st = to_settings(*settings);
     const Camera cam = to_camera(*camera);
     if (sampleCount == 0) return 0;
-    const uint32_t n = std::max(1u, threads);
-    const uint32_t rows = std::max(1u, (st.height + n - 1) / n);
-    std::vector<std::vector<std::vector<float>>> local(n);
-    for (auto& L : local) {
+    const uint32_t n = std::max(1u, std::min(threads, st.height));
+    std::vector<std::vector<std::vector<float>>> local(n), localEnv(n);
+    bool envWanted = false;
+    for (uint32_t i = 0; i < envCount && i < s.domes.size(); ++i) envWanted |= envGrads[i] && s.domes[i].env;
+    for (uint32_t t = 0; t < n; ++t) {
+        auto& L = local[t];
         L.resize(s.textures.size());
         for (uint32_t i = 0; i < gradCount && i < s.textures.size(); ++i)
             if (grads[i]) L[i].assign(s.textures[i].eval.size(), 0.0f);
+        auto& E = localEnv[t];
+        E.resize(s.domes.size());
+        for (uint32_t i = 0; i < envCount && i < s.domes.size(); ++i)
+            if (envGrads[i] && s.domes[i].env) E[i].assign(s.domes[i].env->px.size() * 3, 0.0f);
     }
     std::vector<std::thread> pool;
+    // Rows are dealt out round robin (thread t takes rows t, t + n, ...), so
+    // that a band of weighted rows still spreads over every thread, and each
+    // thread's partial sums are added in thread order: the result does not
+    // depend on timing.
     for (uint32_t t = 0; t < n; ++t) {
-        const uint32_t y0 = t * rows;
-        if (y0 >= st.height) break;
-        const uint32_t y1 = std::min(st.height, y0 + rows);
-        pool.emplace_back([&, t, y0, y1]() {
+        pool.emplace_back([&, t]() {
             Adjoint adj;
             adj.grads = &local[t];
+            adj.envGrads = envWanted ? &localEnv[t] : nullptr;
             const float inv = 1.0f / static_cast<float>(sampleCount);
-            for (uint32_t y = y0; y < y1; ++y)
+            for (uint32_t y = t; y < st.height; y += n)
                 for (uint32_t x = 0; x < st.width; ++x) {
                     const float* db = dBeauty + (static_cast<size_t>(y) * st.width + x) * 4;
                     adj.a = {db[0] * inv, db[1] * inv, db[2] * inv};
@@ -2613,6 +2730,11 @@ int oracle_render_adjoint(void* h, const shiro_hip_camera* camera, const shiro_h
         if (!grads[i]) continue;
         for (const auto& L : local)
             for (size_t k = 0; k < L[i].size(); ++k) grads[i][k] += L[i][k];
+    }
+    for (uint32_t i = 0; i < envCount && i < s.domes.size(); ++i) {
+        if (!envGrads[i]) continue;
+        for (const auto& E : localEnv)
+            for (size_t k = 0; k < E[i].size(); ++k) envGrads[i][k] += E[i][k];
     }
     return 0;
 }

@@ -192,6 +192,7 @@ EXPORTS = [
     "shiro_hip_render_many", "shiro_hip_autotune_many",
     "shiro_hip_load_image", "shiro_hip_load_image_memory", "shiro_hip_free_image",
     "shiro_hip_refit_scene", "shiro_hip_triangulate", "shiro_hip_smooth_normals",
+    "shiro_hip_render_adjoint_env", "shiro_hip_update_environment",
 ]
 
 

@@ -1,5 +1,5 @@
-// Path-replay adjoint of the render with respect to texture texels (texture
-// extension; no counterpart in the reference).
+// Path-replay adjoint of the render with respect to texture texels and
+// dome-map texels (texture extension; no counterpart in the reference).
 //
 // The forward estimate of one sample is L = sum_j T_j * C_j, where C_j is
 // what vertex j adds per unit of throughput (emission, the clamped and
@@ -17,7 +17,12 @@
 // the sampled direction), adds the light-sample gradients on the way, then
 // walks the vertices backwards propagating dLoss/dT_j through the clamp and
 // the roulette division.  Texel gradients are scattered with atomics through
-// the bilinear footprint of each hit.
+// the bilinear footprint of each hit.  The dome maps enter only through the
+// radiance an escaped ray or a dome light sample reads (the importance
+// tables are sampling, hence detached), so their gradient is scattered where
+// that radiance is looked up, weighted by what the lookup adds to the pixel:
+// throughput x MIS weight on a miss, and the clamp's vector-Jacobian product
+// x BSDF x (cos x MIS / pdf) / K for a visible light sample.
 #include "kernels.h"
 
 #include "dev_path.h"
@@ -107,6 +112,26 @@ DEV void scatter_grad(const AdjointParams& P, uint32_t material, V2 uv, V3 gBase
     }
 }
 
+// Adds g (per channel, per unit of the map's radiance at map-space
+// direction `local`) to the texel gradients of dome i's map through the
+// bilinear footprint of the lookup.
+DEV void scatter_env(const AdjointParams& P, uint32_t i, V3 local, V3 g) {
+    const DevScene& S = P.scene;
+    const DevDome& d = S.domes[i];
+    if (d.env < 0 || !P.envGrad[i]) return;
+    EnvTaps t;
+    if (!env_taps(S.envs[d.env], local, &t)) return;
+    const V3 gr = g * ld3(d.radiance);
+    float* out = P.envGrad[i];
+    for (int k = 0; k < 4; ++k) {
+        if (t.w[k] == 0.0f) continue;
+        float* q = out + static_cast<size_t>(t.idx[k]) * 3u;
+        if (gr.x != 0.0f) atomicAdd(q, gr.x * t.w[k]);
+        if (gr.y != 0.0f) atomicAdd(q + 1, gr.y * t.w[k]);
+        if (gr.z != 0.0f) atomicAdd(q + 2, gr.z * t.w[k]);
+    }
+}
+
 DEV V3 ratio(V3 a, V3 b) {
     return V3{b.x != 0.0f ? a.x / b.x : 0.0f, b.y != 0.0f ? a.y / b.y : 0.0f, b.z != 0.0f ? a.z / b.z : 0.0f};
 }
@@ -114,7 +139,8 @@ DEV V3 ratio(V3 a, V3 b) {
 }  // namespace
 
 // Grid: one 64-thread block per (8x8 tile, sample); tile-major within a
-// sample.
+// sample.  MAXD bounds the path length, sizing the per-thread vertex tape.
+template <uint32_t MAXD>
 __global__ __launch_bounds__(64) void k_adjoint(AdjointParams P) {
     const uint32_t tile = blockIdx.x % P.tiles;
     const uint32_t s = blockIdx.x / P.tiles;
@@ -131,9 +157,9 @@ __global__ __launch_bounds__(64) void k_adjoint(AdjointParams P) {
     PathState ps = init_path(P.camera, st, x, y, smp);
     const uint32_t nLight = direct_sample_count(S, st);
     Counters<false> cnt;
-    Vtx tape[kAdjointMaxDepth];
+    Vtx tape[MAXD];
     uint32_t nv = 0;
-    for (uint32_t depth = 0; depth < st.maxDepth; ++depth) {
+    for (uint32_t depth = 0; depth < st.maxDepth && depth < MAXD; ++depth) {
         const Ray ray = path_ray(ps);
         const HitRec h = intersect_closest(S, ray, cnt.trav());
         Vtx& V = tape[nv++];
@@ -142,7 +168,13 @@ __global__ __launch_bounds__(64) void k_adjoint(AdjointParams P) {
             V.kind = kVtxMiss;
             V.T = ps.T;
             V.att = V3{1.0f, 1.0f, 1.0f};
-            handle_miss(S, st, depth, ps, cnt, &V.C);
+            float mis = 0.0f;
+            handle_miss(S, st, depth, ps, cnt, &V.C, &mis);
+            // The escaped ray adds T * mis * sum_i env_i(dir) * radiance_i.
+            if (P.envWanted && mis != 0.0f) {
+                const V3 g = a * V.T * mis;
+                for (uint32_t i = 0; i < S.domeCount; ++i) scatter_env(P, i, world_to_dome(S.domes[i], ps.d), g);
+            }
             break;
         }
         const V3 Tin = ps.T;
@@ -171,15 +203,22 @@ __global__ __launch_bounds__(64) void k_adjoint(AdjointParams P) {
             V3 ld, contrib, f, lf;
             uint32_t group;
             float maxDist;
-            if (!direct_sample(S, st, sf, M, smp, depth, j, &ld, &contrib, &group, &maxDist, cnt, &f, &lf)) continue;
+            DomeSampleInfo dome;
+            if (!direct_sample(S, st, sf, M, smp, depth, j, &ld, &contrib, &group, &maxDist, cnt, &f, &lf, &dome))
+                continue;
             const bool visible = !occluded(S, origin, ld, maxDist, cnt);
             ds.add(group, contrib, visible);
-            if (visible && textured) {
+            if (visible && (textured || (P.envWanted && group > 0u))) {
                 const V3 gc = group == 0u ? aT : aT / ds.K;
-                const V3 gf = clamp_max_vjp(f * lf, kDirectClamp, gc) * lf;
-                const BsdfGrad bg = bsdf_grad(M, sf.n, sf.cn, sf.v, ld);
-                gBase = gBase + gf * bg.dBase;
-                gRough += dot(gf, bg.dRough);
+                const V3 gy = clamp_max_vjp(f * lf, kDirectClamp, gc);
+                if (textured) {
+                    const V3 gf = gy * lf;
+                    const BsdfGrad bg = bsdf_grad(M, sf.n, sf.cn, sf.v, ld);
+                    gBase = gBase + gf * bg.dBase;
+                    gRough += dot(gf, bg.dRough);
+                }
+                // lf = env(local) * radiance * s for the sample of dome group - 1.
+                if (P.envWanted && group > 0u) scatter_env(P, group - 1u, dome.local, gy * f * dome.s);
             }
         }
         ds.flush();
@@ -232,7 +271,17 @@ hipError_t launch_adjoint(const AdjointParams& P, hipStream_t s) {
     const uint64_t blocks = static_cast<uint64_t>(P.tiles) * P.sampleCount;
     if (blocks == 0) return hipSuccess;
     if (blocks > 0x7fffffffull) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_adjoint, dim3(static_cast<uint32_t>(blocks)), dim3(64), 0, s, P);
+    const dim3 grid(static_cast<uint32_t>(blocks));
+    const uint32_t md = P.settings.maxDepth;
+    if (md <= 4u) {
+        hipLaunchKernelGGL(k_adjoint<4>, grid, dim3(64), 0, s, P);
+    } else if (md <= 8u) {
+        hipLaunchKernelGGL(k_adjoint<8>, grid, dim3(64), 0, s, P);
+    } else if (md <= kAdjointMaxDepth) {
+        hipLaunchKernelGGL(k_adjoint<kAdjointMaxDepth>, grid, dim3(64), 0, s, P);
+    } else {
+        return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 

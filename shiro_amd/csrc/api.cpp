@@ -62,6 +62,9 @@ struct DevBuf {
 
 struct EnvBuffers {
     DevBuf texels, row, cond, rowGuide, condGuide;
+    DevBuf rowSin, rowW, rgb;   // device updates: row factors, row totals, staging of host texels
+    uint32_t w = 0, h = 0;
+    bool table = false;
 };
 
 // Texture extension: the texels, and a second copy that sampling reads
@@ -677,7 +680,13 @@ int upload_scene(shiro_hip_ctx* ctx, const shiro_hip_scene* s, bool refit) {
             std::vector<float> row, cond;
             const bool table = build_env_tables(L.pixels, L.width, L.height, layout, &row, &cond);
             HIPC(upload(ctx->env[i].texels, tex.data(), tex.size(), ctx->stream), "upload env");
+            ctx->env[i].w = L.width;
+            ctx->env[i].h = L.height;
+            ctx->env[i].table = table;
             if (table) {
+                std::vector<float> sines(L.height);
+                for (uint32_t y = 0; y < L.height; ++y) sines[y] = env_row_sine(y, L.height);
+                HIPC(upload(ctx->env[i].rowSin, sines.data(), sines.size(), ctx->stream), "upload env");
                 HIPC(upload(ctx->env[i].row, row.data(), row.size(), ctx->stream), "upload env");
                 HIPC(upload(ctx->env[i].cond, cond.data(), cond.size(), ctx->stream), "upload env");
                 std::vector<uint32_t> rg(kEnvRowGuide + 1u), cg(static_cast<size_t>(L.height) * (kEnvColGuide + 1u));
@@ -1165,9 +1174,44 @@ int shiro_hip_get_stage_times(shiro_hip_ctx* ctx, shiro_hip_stage_times* out) {
     return SHIRO_HIP_OK;
 }
 
-int shiro_hip_render_adjoint(shiro_hip_ctx* ctx, const shiro_hip_camera* camera, const shiro_hip_settings* settings,
-                             uint32_t sampleStart, uint32_t sampleCount, const float* dBeauty, float* const* dTexels,
-                             uint32_t dTexelsCount) {
+int shiro_hip_update_environment(shiro_hip_ctx* ctx, uint32_t dome, const float* texels, uint32_t flags) {
+    if (!ctx) return SHIRO_HIP_INVALID;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    if (!ctx->hasScene) return fail(ctx, SHIRO_HIP_NO_SCENE, "no scene uploaded");
+    if (dome >= ctx->dev.domeCount || ctx->dev.domes[dome].env < 0 || !texels)
+        return fail(ctx, SHIRO_HIP_INVALID, "bad dome index, dome without a map, or null texels");
+    HIPC(hipSetDevice(ctx->device), "set device");
+    EnvBuffers& E = ctx->env[ctx->dev.domes[dome].env];
+    const size_t n = static_cast<size_t>(E.w) * E.h;
+    const float* rgb = texels;
+    if (!(flags & SHIRO_HIP_TEXELS_DEVICE)) {
+        HIPC(E.rgb.reserve(n * 12), "alloc env staging");
+        HIPC(hipMemcpyAsync(E.rgb.p, texels, n * 12, hipMemcpyHostToDevice, ctx->stream), "upload env texels");
+        rgb = E.rgb.as<float>();
+    }
+    EnvUpdate U{};
+    U.rgb = rgb;
+    U.texels = E.texels.as<float4>();
+    U.w = E.w;
+    U.h = E.h;
+    U.rebuild = E.table && !(flags & SHIRO_HIP_TEXELS_EVAL_ONLY) ? 1u : 0u;
+    if (U.rebuild) {
+        HIPC(E.rowW.reserve(static_cast<size_t>(E.h) * 4), "alloc env scratch");
+        U.rowSin = E.rowSin.as<float>();
+        U.rowW = E.rowW.as<float>();
+        U.rowCdf = E.row.as<float>();
+        U.cond = E.cond.as<float>();
+        U.rowGuide = E.rowGuide.as<uint32_t>();
+        U.condGuide = E.condGuide.as<uint32_t>();
+    }
+    HIPC(launch_env_update(U, ctx->stream), "env update");
+    HIPC(hipStreamSynchronize(ctx->stream), "env update");
+    return SHIRO_HIP_OK;
+}
+
+int render_adjoint(shiro_hip_ctx* ctx, const shiro_hip_camera* camera, const shiro_hip_settings* settings,
+                   uint32_t sampleStart, uint32_t sampleCount, const float* dBeauty, float* const* dTexels,
+                   uint32_t dTexelsCount, float* const* dEnv, uint32_t dEnvCount) {
     if (!ctx) return SHIRO_HIP_INVALID;
     std::lock_guard<std::mutex> lock(ctx->mu);
     if (!camera || !dBeauty) return fail(ctx, SHIRO_HIP_INVALID, "null camera or adjoint image");
@@ -1181,6 +1225,11 @@ int shiro_hip_render_adjoint(shiro_hip_ctx* ctx, const shiro_hip_camera* camera,
     if (ctx->tex.size() > kAdjointMaxTextures)
         return fail(ctx, SHIRO_HIP_INVALID, "adjoint supports at most " + std::to_string(kAdjointMaxTextures) +
                                                 " textures");
+    if (dEnvCount > ctx->dev.domeCount || (dEnvCount > 0 && !dEnv))
+        return fail(ctx, SHIRO_HIP_INVALID, "more environment gradients than dome lights");
+    for (uint32_t i = 0; i < dEnvCount; ++i)
+        if (dEnv[i] && ctx->dev.domes[i].env < 0)
+            return fail(ctx, SHIRO_HIP_INVALID, "environment gradient for dome " + std::to_string(i) + " without a map");
     for (size_t i = 0; i < ctx->mats.size(); ++i) {
         const shiro_hip_material& m = ctx->mats[i];
         if (m.baseColorTexture < 0 && m.roughnessTexture < 0) continue;
@@ -1198,6 +1247,10 @@ int shiro_hip_render_adjoint(shiro_hip_ctx* ctx, const shiro_hip_camera* camera,
     P.settings = to_dev(*settings);
     P.dBeauty = reinterpret_cast<const float4*>(dBeauty);
     for (uint32_t i = 0; i < dTexelsCount; ++i) P.grad[i] = dTexels[i];
+    for (uint32_t i = 0; i < dEnvCount; ++i) {
+        P.envGrad[i] = dEnv[i];
+        if (dEnv[i]) P.envWanted = 1u;
+    }
     P.sampleStart = sampleStart;
     P.sampleCount = sampleCount;
     P.tilesX = (settings->width + 7u) / 8u;
@@ -1206,6 +1259,20 @@ int shiro_hip_render_adjoint(shiro_hip_ctx* ctx, const shiro_hip_camera* camera,
     HIPC(launch_adjoint(P, ctx->stream), "adjoint launch");
     HIPC(hipEventRecord(ctx->ev1, ctx->stream), "event");
     return finish_timing(ctx);
+}
+
+int shiro_hip_render_adjoint(shiro_hip_ctx* ctx, const shiro_hip_camera* camera, const shiro_hip_settings* settings,
+                             uint32_t sampleStart, uint32_t sampleCount, const float* dBeauty, float* const* dTexels,
+                             uint32_t dTexelsCount) {
+    return render_adjoint(ctx, camera, settings, sampleStart, sampleCount, dBeauty, dTexels, dTexelsCount, nullptr, 0);
+}
+
+int shiro_hip_render_adjoint_env(shiro_hip_ctx* ctx, const shiro_hip_camera* camera,
+                                 const shiro_hip_settings* settings, uint32_t sampleStart, uint32_t sampleCount,
+                                 const float* dBeauty, float* const* dTexels, uint32_t dTexelsCount,
+                                 float* const* dEnv, uint32_t dEnvCount) {
+    return render_adjoint(ctx, camera, settings, sampleStart, sampleCount, dBeauty, dTexels, dTexelsCount, dEnv,
+                          dEnvCount);
 }
 
 namespace {

@@ -54,6 +54,65 @@ DEV V3 env_bilinear(const DevEnv& e, float u, float v, const EnvCount<COUNT>& c)
     return lerp(lerp(c00, c10, fx), lerp(c01, c11, fx), fy);
 }
 
+// The four texels and bilinear weights env_bilinear blends at (u, v):
+// d env_bilinear / d texel[idx[k]] = w[k] (a clamped row may repeat a texel;
+// its weights then add up).
+struct EnvTaps {
+    uint32_t idx[4];
+    float w[4];
+};
+
+DEV EnvTaps env_bilinear_taps(const DevEnv& e, float u, float v) {
+    u = wrap01(u) * static_cast<float>(e.w) - 0.5f;
+    v = clampf(v, 0.0f, 1.0f) * static_cast<float>(e.h) - 0.5f;
+    const int x0 = static_cast<int>(floorf(u));
+    const int y0 = static_cast<int>(floorf(v));
+    const int H = static_cast<int>(e.h), W = static_cast<int>(e.w);
+    const int y1 = y0 + 1 < H - 1 ? y0 + 1 : H - 1;
+    const float fx = u - static_cast<float>(x0);
+    const float fy = v - static_cast<float>(y0);
+    auto wx = [&](int x) {
+        int r = x % W;
+        return r < 0 ? r + W : r;
+    };
+    auto cy = [&](int y) { return y < 0 ? 0 : (y > H - 1 ? H - 1 : y); };
+    const uint32_t r0 = static_cast<uint32_t>(cy(y0)) * e.w, r1 = static_cast<uint32_t>(cy(y1)) * e.w;
+    const uint32_t a = static_cast<uint32_t>(wx(x0)), b = static_cast<uint32_t>(wx(x0 + 1));
+    EnvTaps t;
+    t.idx[0] = r0 + a;
+    t.idx[1] = r0 + b;
+    t.idx[2] = r1 + a;
+    t.idx[3] = r1 + b;
+    t.w[0] = (1.0f - fx) * (1.0f - fy);
+    t.w[1] = fx * (1.0f - fy);
+    t.w[2] = (1.0f - fx) * fy;
+    t.w[3] = fx * fy;
+    return t;
+}
+
+// Taps of env_eval at a map-space direction; false where the map adds
+// nothing (outside an angular map's disc).
+DEV bool env_taps(const DevEnv& e, V3 dir, EnvTaps* t) {
+    const V3 d = normalize(dir);
+    if (e.layout == SHIRO_HIP_ENV_ANGULAR) {
+        const float radial = sqrtf(d.x * d.x + d.y * d.y);
+        if (radial <= 1.0e-6f) {
+            *t = env_bilinear_taps(e, 0.5f, 0.5f);
+            return true;
+        }
+        const float r = acosf(clampf(d.z, -1.0f, 1.0f)) / kPi;
+        const float u = 0.5f + 0.5f * r * (d.x / radial);
+        const float v = 0.5f + 0.5f * r * (d.y / radial);
+        if (u < 0.0f || u > 1.0f || v < 0.0f || v > 1.0f) return false;
+        *t = env_bilinear_taps(e, u, v);
+        return true;
+    }
+    const float phi = atan2f(d.x, d.z);
+    const float theta = acosf(clampf(d.y, -1.0f, 1.0f));
+    *t = env_bilinear_taps(e, wrap01(phi / kTwoPi), clampf(theta / kPi, 0.0f, 1.0f));
+    return true;
+}
+
 template <bool COUNT>
 DEV V3 env_eval(const DevEnv& e, V3 dir, const EnvCount<COUNT>& c) {
     const V3 d = normalize(dir);

@@ -79,7 +79,8 @@ DEV float env_pdf_sq_sum(const DevScene& S, V3 dir, Counters<COUNT>& c) {
 
 // SampleDomeLight (CpuPathTracer.cpp:354-384).
 template <bool COUNT>
-DEV bool dome_sample(const DevScene& S, const DevDome& d, V2 u, V3* dir, V3* rad, float* pdf, Counters<COUNT>& c) {
+DEV bool dome_sample(const DevScene& S, const DevDome& d, V2 u, V3* dir, V3* rad, float* pdf, Counters<COUNT>& c,
+                     V3* localOut = nullptr) {
     V3 ld;
     float lp;
     V3 lr;
@@ -90,6 +91,7 @@ DEV bool dome_sample(const DevScene& S, const DevDome& d, V2 u, V3* dir, V3* rad
         lp = kInvFourPi;
         lr = V3{1.0f, 1.0f, 1.0f};
     }
+    if (localOut) *localOut = ld;
     if (lp <= 0.0f) return false;
     *dir = dome_to_world(d, ld);
     *rad = lr * ld3(d.radiance);
@@ -191,7 +193,8 @@ DEV Ray path_ray(const PathState& ps) {
 // per unit of throughput.
 template <bool COUNT>
 DEV void handle_miss(const DevScene& S, const DevSettings& st, uint32_t depth, PathState& ps, Counters<COUNT>& c,
-                     V3* factor = nullptr) {
+                     V3* factor = nullptr, float* misOut = nullptr) {
+    if (misOut) *misOut = 0.0f;
     if (depth > 0 || st.backgroundVisible) {
         const V3 Le = environment_radiance(S, ps.d, c);
         const uint32_t prevType = ps.bits & kBitsTypeMask;
@@ -199,12 +202,14 @@ DEV void handle_miss(const DevScene& S, const DevSettings& st, uint32_t depth, P
         if (S.domeCount == 0 || depth == 0 || (ps.bits & kBitsPrevDelta) || prevType == kBounceTransmission) {
             dL = ps.T * Le;
             if (factor) *factor = Le;
+            if (misOut) *misOut = 1.0f;
         } else {
             const float sq2 = env_pdf_sq_sum(S, ps.d, c);
             const float b2 = ps.prevPdf * ps.prevPdf;
             const float w = b2 / fmaxs(b2 + sq2, 1.0e-6f);
             dL = ps.T * Le * w;
             if (factor) *factor = Le * w;
+            if (misOut) *misOut = w;
         }
         ps.L = ps.L + dL;
     } else if (factor) {
@@ -358,10 +363,19 @@ DEV uint32_t direct_sample_count(const DevScene& S, const DevSettings& st) {
 // Returns false when the sample needs no shadow ray.  `fOut` / `lfOut`
 // (optional) receive the BSDF value and the light factor,
 // contrib = clamp_max_component(f * lf, 24).
+// `domeOut` (optional) receives, for a dome sample, the map-space
+// direction the radiance was looked up at and the factor s of
+// lf = env radiance * dome radiance * s.
+struct DomeSampleInfo {
+    V3 local;
+    float s;
+};
+
 template <bool COUNT>
 DEV bool direct_sample(const DevScene& S, const DevSettings& st, const Surface& sf, const Mat& M, const Sampler& smp,
                        uint32_t depth, uint32_t j, V3* dirOut, V3* contrib, uint32_t* group, float* maxDist,
-                       Counters<COUNT>& c, V3* fOut = nullptr, V3* lfOut = nullptr) {
+                       Counters<COUNT>& c, V3* fOut = nullptr, V3* lfOut = nullptr,
+                       DomeSampleInfo* domeOut = nullptr) {
     *maxDist = kFltMax;
     if (j < S.dirCount) {
         const DevDirLight& dl = S.dirs[j];
@@ -416,7 +430,9 @@ DEV bool direct_sample(const DevScene& S, const DevSettings& st, const Surface& 
     V3 ld, lr;
     float lp = 0.0f;
     const uint32_t branch = 256u + depth * 512u + li * 128u + k * 2u;
-    if (!dome_sample(S, S.domes[li], smp.next2(branch), &ld, &lr, &lp, c) || lp <= 0.0f) return false;
+    if (!dome_sample(S, S.domes[li], smp.next2(branch), &ld, &lr, &lp, c, domeOut ? &domeOut->local : nullptr) ||
+        lp <= 0.0f)
+        return false;
     const float nl = clampf(dot(sf.n, ld), 0.0f, 1.0f);
     if (nl <= 0.0f) return false;
     const float bp = bsdf_pdf(M, sf.n, sf.cn, sf.v, ld);
@@ -426,6 +442,7 @@ DEV bool direct_sample(const DevScene& S, const DevSettings& st, const Surface& 
     *contrib = clamp_max_component(f * lr * (nl * w / lp), kDirectClamp);
     if (fOut) *fOut = f;
     if (lfOut) *lfOut = lr * (nl * w / lp);
+    if (domeOut) domeOut->s = nl * w / lp;
     *group = 1u + li;
     return true;
 }

@@ -182,6 +182,11 @@ void assign_shade_classes(const shiro_hip_material* m, MaterialDerived* d, size_
     }
 }
 
+float env_row_sine(uint32_t y, uint32_t h) {
+    const float theta = (static_cast<float>(y) + 0.5f) * kPiH / static_cast<float>(h);
+    return std::max(std::sin(theta), 1.0e-6f);
+}
+
 bool build_env_tables(const float* px, uint32_t w, uint32_t h, uint32_t layout, std::vector<float>* rowCdf,
                       std::vector<float>* condCdf) {
     rowCdf->clear();
@@ -191,8 +196,7 @@ bool build_env_tables(const float* px, uint32_t w, uint32_t h, uint32_t layout, 
     condCdf->assign(static_cast<size_t>(h) * (w + 1u), 0.0f);
     float total = 0.0f;
     for (uint32_t y = 0; y < h; ++y) {
-        const float theta = (static_cast<float>(y) + 0.5f) * kPiH / static_cast<float>(h);
-        const float st = std::max(std::sin(theta), 1.0e-6f);
+        const float st = env_row_sine(y, h);
         float* row = condCdf->data() + static_cast<size_t>(y) * (w + 1u);
         float sum = 0.0f;
         for (uint32_t x = 0; x < w; ++x) {

@@ -23,6 +23,10 @@ MaterialDerived derive_material(const shiro_hip_material& m);
 // kShadeClasses - 2 share the last material class.
 void assign_shade_classes(const shiro_hip_material* m, MaterialDerived* d, size_t n);
 
+// Row weight factor of the importance table, max(sin(theta_y), 1e-6) at
+// the row centre (EnvironmentMap.cpp:197-199); the device rebuild of the
+// tables takes these host values so that it matches build_env_tables.
+float env_row_sine(uint32_t y, uint32_t h);
 // EnvironmentMap::BuildImportanceTable (EnvironmentMap.cpp:185-236):
 // sequential fp32 prefix sums in the reference's order.  pixels is w*h RGB.
 // Returns false (tables empty) when the layout is not lat-long.

@@ -87,6 +87,8 @@ struct AdjointParams {
     DevSettings settings;
     const float4* dBeauty;                // W*H adjoint of the beauty mean (rgb; a ignored)
     float* grad[kAdjointMaxTextures];     // texel gradients per texture (null: not wanted)
+    float* envGrad[kMaxDomes];            // per dome light: gradient of its map's RGB texels (null: not wanted)
+    uint32_t envWanted;                   // 1 if any envGrad is set
     uint32_t sampleStart, sampleCount;
     uint32_t tilesX, tiles;
 };
@@ -96,6 +98,21 @@ hipError_t launch_render(const RenderParams& P, bool count, hipStream_t stream);
 hipError_t launch_render_many(const RenderParams* items, const uint2* work, uint32_t workCount, uint32_t* counter,
                               uint32_t blocks, bool tex, hipStream_t s);
 hipError_t launch_adjoint(const AdjointParams& P, hipStream_t s);
+
+// A dome map's texels replaced on the device: rgb (w*h*3, device) into
+// texels (float4); with rebuild, the importance tables rebuilt from them.
+struct EnvUpdate {
+    const float* rgb;
+    float4* texels;
+    const float* rowSin;   // env_row_sine per row (host values)
+    float* rowW;           // h floats of scratch
+    float* rowCdf;         // h + 1
+    float* cond;           // h * (w + 1)
+    uint32_t* rowGuide;    // kEnvRowGuide + 1
+    uint32_t* condGuide;   // h * (kEnvColGuide + 1)
+    uint32_t w, h, rebuild;
+};
+hipError_t launch_env_update(const EnvUpdate& U, hipStream_t s);
 hipError_t launch_wavefront_pass(const WfParams& P, const WfLaunch& L, hipStream_t s, StageMarks* marks);
 hipError_t launch_accum_add(const AccumParams& A, const DevFrame& f, uint32_t count, uint32_t np, hipStream_t s);
 hipError_t launch_accum_resolve(const AccumParams& A, const DevFrame& f, uint32_t total, uint32_t np, hipStream_t s);

@@ -376,4 +376,105 @@ hipError_t launch_env_eval(const DevEnv& e, const float* dir, uint32_t n, float*
     return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------
+// Dome maps updated on the device (texture extension): RGB texels into the
+// float4 layout, and the importance tables rebuilt from them with the host
+// build's operation order (build_env_tables, build_cdf_guide), so that a map
+// updated on the device samples exactly like the same map uploaded.
+
+__global__ void k_env_texels(const float* rgb, float4* out, uint32_t n) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        out[i] = make_float4(rgb[3u * i], rgb[3u * i + 1u], rgb[3u * i + 2u], 0.0f);
+}
+
+// One thread per row: sequential prefix sum of max(0, luminance) * sin, then
+// normalised (uniform where the row is black); the row's total to rowW.
+__global__ void k_env_cond(const float4* tex, uint32_t w, uint32_t h, const float* rowSin, float* cond, float* rowW) {
+    const uint32_t y = blockIdx.x * blockDim.x + threadIdx.x;
+    if (y >= h) return;
+    const float st = rowSin[y];
+    float* row = cond + static_cast<size_t>(y) * (w + 1u);
+    row[0] = 0.0f;
+    float sum = 0.0f;
+    for (uint32_t x = 0; x < w; ++x) {
+        const float4 c = tex[static_cast<size_t>(y) * w + x];
+        const float lum = c.x * 0.2126f + c.y * 0.7152f + c.z * 0.0722f;
+        sum += fmaxs(0.0f, lum) * st;
+        row[x + 1u] = sum;
+    }
+    if (sum > 0.0f) {
+        for (uint32_t x = 1; x <= w; ++x) row[x] /= sum;
+    } else {
+        for (uint32_t x = 1; x <= w; ++x) row[x] = static_cast<float>(x) / static_cast<float>(w);
+        sum = 1.0f;
+    }
+    rowW[y] = sum;
+}
+
+// One thread: the row CDF from the row totals, in row order.
+__global__ void k_env_rows(const float* rowW, uint32_t h, float* rowCdf) {
+    if (blockIdx.x != 0u || threadIdx.x != 0u) return;
+    float total = 0.0f;
+    rowCdf[0] = 0.0f;
+    for (uint32_t y = 0; y < h; ++y) {
+        total += rowW[y];
+        rowCdf[y + 1u] = total;
+    }
+    if (total <= 0.0f) {
+        for (uint32_t y = 1; y <= h; ++y) rowCdf[y] = static_cast<float>(y) / static_cast<float>(h);
+        return;
+    }
+    for (uint32_t y = 1; y <= h; ++y) rowCdf[y] /= total;
+}
+
+// build_cdf_guide of cdf[begin .. end] into out[0 .. G].
+DEV void cdf_guide(const float* cdf, uint32_t begin, uint32_t end, uint32_t G, uint32_t* out) {
+    bool monotone = true;
+    for (uint32_t i = begin; i < end && monotone; ++i) monotone = cdf[i] <= cdf[i + 1u];
+    for (uint32_t g = 0; g <= G; ++g) {
+        if (!monotone) {
+            out[g] = g == G ? end + 1u : begin + 1u;
+            continue;
+        }
+        const float key = static_cast<float>(g) / static_cast<float>(G);
+        uint32_t first = begin + 1u, count = end - begin;
+        while (count > 0u) {
+            const uint32_t half = count >> 1;
+            const uint32_t mid = first + half;
+            if (cdf[mid] < key) {
+                first = mid + 1u;
+                count -= half + 1u;
+            } else {
+                count = half;
+            }
+        }
+        out[g] = first;
+    }
+}
+
+// Threads 0..h-1: the guides of row y's conditional CDF; thread h: the row
+// CDF's guide.
+__global__ void k_env_guides(const float* rowCdf, const float* cond, uint32_t w, uint32_t h, uint32_t* rowGuide,
+                             uint32_t* condGuide) {
+    const uint32_t y = blockIdx.x * blockDim.x + threadIdx.x;
+    if (y < h) {
+        const uint32_t off = y * (w + 1u);
+        cdf_guide(cond, off, off + w, kEnvColGuide, condGuide + static_cast<size_t>(y) * (kEnvColGuide + 1u));
+    } else if (y == h) {
+        cdf_guide(rowCdf, 0u, h, kEnvRowGuide, rowGuide);
+    }
+}
+
+hipError_t launch_env_update(const EnvUpdate& U, hipStream_t s) {
+    const uint32_t n = U.w * U.h;
+    hipLaunchKernelGGL(k_env_texels, dim3(blocks_for(n, 256)), dim3(256), 0, s, U.rgb, U.texels, n);
+    if (!U.rebuild) return hipGetLastError();
+    hipLaunchKernelGGL(k_env_cond, dim3(blocks_for(U.h, 64)), dim3(64), 0, s, U.texels, U.w, U.h, U.rowSin, U.cond,
+                       U.rowW);
+    hipLaunchKernelGGL(k_env_rows, dim3(1), dim3(64), 0, s, U.rowW, U.h, U.rowCdf);
+    hipLaunchKernelGGL(k_env_guides, dim3(blocks_for(U.h + 1u, 64)), dim3(64), 0, s, U.rowCdf, U.cond, U.w, U.h,
+                       U.rowGuide, U.condGuide);
+    return hipGetLastError();
+}
+
 }  // namespace shiro_hip

@@ -92,6 +92,9 @@ def load_library(path: str = LIB_PATH):
     d("shiro_hip_get_stage_times", C.c_int, V, _P(abi.StageTimes))
     d("shiro_hip_update_texture", C.c_int, V, _u, _fp, _u)
     d("shiro_hip_render_adjoint", C.c_int, V, _P(abi.Camera), _P(abi.Settings), _u, _u, _fp, _P(_fp), _u)
+    d("shiro_hip_render_adjoint_env", C.c_int, V, _P(abi.Camera), _P(abi.Settings), _u, _u, _fp, _P(_fp), _u,
+      _P(_fp), _u)
+    d("shiro_hip_update_environment", C.c_int, V, _u, _fp, _u)
     d("shiro_hip_render_many", C.c_int, V, _P(abi.BatchItem), _u, _u)
     d("shiro_hip_autotune_many", C.c_int, V, _P(abi.BatchItem), _u, _P(abi.Tuning), C.c_char_p, C.c_size_t)
     d("shiro_hip_refit_scene", C.c_int, V, _P(abi.Scene))
@@ -478,15 +481,34 @@ class Context:
         del keep
         self._sig = None  # the device scene no longer matches the uploaded one
 
-    def render_adjoint(self, camera, settings, dbeauty_ptr, grad_ptrs, sample_start=0, sample_count=None):
+    def update_environment(self, dome, texels, eval_only=False):
+        """Replaces dome light `dome`'s map texels: an (H, W, 3) float32
+        numpy array, or a device address (int) of H*W*3 floats.  The
+        importance tables are rebuilt on the device unless eval_only."""
+        flags = abi.TEXELS_EVAL_ONLY if eval_only else 0
+        if isinstance(texels, int):
+            ptr = C.cast(C.c_void_p(texels), _fp)
+            flags |= abi.TEXELS_DEVICE
+            keep = None
+        else:
+            keep = np.ascontiguousarray(texels, np.float32)
+            ptr = abi.fp(keep)
+        self._check(self.L.shiro_hip_update_environment(self.h, dome, ptr, flags))
+        del keep
+        self._sig = None
+
+    def render_adjoint(self, camera, settings, dbeauty_ptr, grad_ptrs, sample_start=0, sample_count=None,
+                       env_grad_ptrs=()):
         """Adds the texel gradients of sum(dbeauty * beauty) to the device
-        buffers grad_ptrs[i] (0 = skip) for texture i; dbeauty_ptr is a
-        device W*H*4 float image."""
+        buffers grad_ptrs[i] (0 = skip) for texture i, and those of dome i's
+        map to env_grad_ptrs[i]; dbeauty_ptr is a device W*H*4 float image."""
         sample_count = settings.samplesPerPixel if sample_count is None else sample_count
         arr = (_fp * max(1, len(grad_ptrs)))(*[C.cast(C.c_void_p(p), _fp) if p else None for p in grad_ptrs])
+        env = (_fp * max(1, len(env_grad_ptrs)))(*[C.cast(C.c_void_p(p), _fp) if p else None for p in env_grad_ptrs])
         cam, st = camera.to_c(), settings.to_c()
-        self._check(self.L.shiro_hip_render_adjoint(self.h, C.byref(cam), C.byref(st), sample_start, sample_count,
-                                                    C.cast(C.c_void_p(dbeauty_ptr), _fp), arr, len(grad_ptrs)))
+        self._check(self.L.shiro_hip_render_adjoint_env(self.h, C.byref(cam), C.byref(st), sample_start, sample_count,
+                                                        C.cast(C.c_void_p(dbeauty_ptr), _fp), arr, len(grad_ptrs),
+                                                        env, len(env_grad_ptrs)))
 
     def set_stage_timing(self, enable=True):
         self._check(self.L.shiro_hip_set_stage_timing(self.h, 1 if enable else 0))

@@ -520,8 +520,11 @@ def c5(scale: float = 1.0, spp: int = 32, seed: int = BASE_SEED, frequency: int 
     scene = Scene(materials=[mat, ground], meshes=[sphere, make_mesh(quad, qi, 1)],
                   textures=[Texture(albedo.copy()), Texture(rough.copy())])
     ew, eh = env_size
-    scene.domeLights.append(DomeLight(pixels=procedural_dome(ew, eh, seed + 6, hot_spots=2, peak=40.0,
-                                                             spot_width_deg=6.0)))
+    env = procedural_dome(ew, eh, seed + 6, hot_spots=2, peak=40.0, spot_width_deg=6.0)
+    scene.domeLights.append(DomeLight(pixels=env.copy()))
+    # Starting dome map: the target's contrast reduced towards its mean
+    # colour (hot spots dimmed, still bright where the target is).
+    init_env = (0.6 * env + 0.4 * env.mean(axis=(0, 1), keepdims=True)).astype(np.float32)
     w = h = max(8, int(round(256 * scale)))
     cam = Camera.look_at((0.0, 1.6, 3.6), (0.0, 0.9, 0.0), fov=40.0, aspect=1.0)
     wl = Workload("c5", scene, cam, _settings(w, h, spp, 4, rr=4))
@@ -530,6 +533,8 @@ def c5(scale: float = 1.0, spp: int = 32, seed: int = BASE_SEED, frequency: int 
     wl.settings.transport = abi.TRANSPORT_WAVEFRONT
     wl.textures_truth = (albedo, rough)
     wl.textures_init = (init_albedo, init_rough)
+    wl.env_truth = env
+    wl.env_init = init_env
     return wl
 
 

@@ -13,6 +13,11 @@ Bars (tolerances written in each test):
   - 100 Adam steps on the MSE to a target image rendered from seeded texels
     (same samples) lower the loss by a quarter or more and move the texels
     towards the target.
+Dome-map texels as parameters, to the same bars: a map replaced on the
+device renders what a fresh upload of it renders, bit for bit; the device
+map gradient vs the oracle's; vs central differences on 64 random seen texel
+channels of the full workload (importance tables frozen); Adam over textures
+and map together lowers the loss.
 """
 import numpy as np
 import pytest
@@ -195,3 +200,158 @@ def test_adam_through_torch_lowers_the_loss():
     e1 = float(torch.abs(albedo - torch.from_numpy(alb_t).cuda()).mean())
     _log(f"mean |base colour - target texels| {e0:.4f} -> {e1:.4f}")
     assert e1 < e0
+
+
+# ---------------------------------------------------------------------------
+# Dome-map texels as parameters.
+
+
+def _device_env_adjoint(ctx, wl, weights):
+    import torch
+    st = wl.settings
+    w = torch.from_numpy(np.ascontiguousarray(weights, np.float32)).cuda()
+    g = torch.zeros(np.shape(wl.scene.domeLights[0].pixels), dtype=torch.float32, device="cuda")
+    torch.cuda.synchronize()
+    ctx.render_adjoint(wl.camera, st, w.data_ptr(), [], env_grad_ptrs=[g.data_ptr()])
+    return g.cpu().numpy()
+
+
+def test_env_update_samples_like_upload(hip_ctx, c5m):
+    """A dome map replaced on the device (tables rebuilt there) renders the
+    frame a fresh upload of the same map renders, bit for bit, from host and
+    from device texels."""
+    import copy
+    import torch
+    from shiro_amd.hip import Context
+    from shiro_amd.scene import DomeLight
+    wl, scene = c5m
+    st = wl.settings
+    px = np.array(scene.domeLights[0].pixels, np.float32)
+    other = np.ascontiguousarray(px[::-1, ::-1] * 1.7 + 0.05)
+    _log("env update vs upload")
+    hip_ctx.upload(scene, force=True)
+    hip_ctx.update_environment(0, other)
+    a = hip_ctx.render(wl.camera, st)
+    dev = torch.from_numpy(other).cuda()
+    hip_ctx.upload(scene, force=True)
+    hip_ctx.update_environment(0, dev.data_ptr())
+    torch.cuda.synchronize()
+    b = hip_ctx.render(wl.camera, st)
+    s2 = copy.copy(scene)
+    d = scene.domeLights[0]
+    s2.domeLights = [DomeLight(radiance=d.radiance, pixels=other, layout=d.layout, right=d.right, up=d.up,
+                               forward=d.forward)]
+    fresh = Context(0)
+    try:
+        fresh.upload(s2)
+        c = fresh.render(wl.camera, st)
+    finally:
+        fresh.close()
+    assert np.array_equal(a.beauty, c.beauty) and np.array_equal(b.beauty, c.beauty)
+    hip_ctx.upload(scene, force=True)
+
+
+def test_env_adjoint_vs_oracle_adjoint(hip_ctx, c5m):
+    wl, scene = c5m
+    st = wl.settings
+    rng = np.random.default_rng(13)
+    weights = rng.normal(size=(st.height, st.width, 4)).astype(np.float32)
+    weights[..., 3] = 0.0
+    _log("env adjoint c5-mini vs oracle")
+    hip_ctx.upload(scene, force=True)
+    dev = _device_env_adjoint(hip_ctx, wl, weights)
+    ref = OracleScene(scene).render_adjoint(wl.camera, st, weights, wanted=[], env_wanted=[0], threads=8)[("env", 0)]
+    err = np.linalg.norm(dev - ref) / np.linalg.norm(ref)
+    _log(f"env gradient relative L2 {err:.2e}")
+    assert np.linalg.norm(ref) > 0
+    assert err <= 1e-3, err
+
+
+def test_env_adjoint_vs_frozen_sampling_differences_c5(hip_ctx):
+    """Full c5 at the starting parameters: 64 random dome-map texel channels
+    among those the image sees; central differences of renders whose
+    importance tables stay those of the starting map, with h = 1e-3 times
+    the texel's magnitude (at least 1e-3: the map holds radiances up to 40)."""
+    import copy
+    from shiro_amd.scene import DomeLight
+    wl = workloads.c5()
+    st = wl.settings
+    alb_t, rough_t = wl.textures_truth
+    alb0, rough0 = wl.textures_init
+    _log("c5 env target render")
+    hip_ctx.upload(c5_scene_with(wl, alb_t, rough_t), force=True)
+    target = hip_ctx.render(wl.camera, st).beauty.reshape(st.height, st.width, 4)[..., :3].astype(np.float64)
+    scene0 = c5_scene_with(wl, alb0, rough0)
+    d = scene0.domeLights[0]
+    scene0.domeLights = [DomeLight(radiance=d.radiance, pixels=wl.env_init.copy(), layout=d.layout, right=d.right,
+                                   up=d.up, forward=d.forward)]
+    hip_ctx.upload(scene0, force=True)
+    img0 = hip_ctx.render(wl.camera, st).beauty.reshape(st.height, st.width, 4)[..., :3].astype(np.float64)
+    n = img0.size
+
+    def loss(img):
+        return float(np.sum((img - target) ** 2) / n)
+
+    dimg = np.zeros((st.height, st.width, 4), np.float32)
+    dimg[..., :3] = (2.0 * (img0 - target) / n).astype(np.float32)
+    wl0 = copy.copy(wl)
+    wl0.scene = scene0
+    g = _device_env_adjoint(hip_ctx, wl0, dimg)
+    rng = np.random.default_rng(6)
+    flat = np.abs(g.ravel())
+    seen = np.nonzero(flat > 1e-3 * flat.max())[0]
+    picks = rng.choice(seen, 64, replace=False)
+    base = wl.env_init.copy()
+
+    def loss_at(k, dv):
+        t = base.copy().ravel()
+        t[k] += dv
+        hip_ctx.update_environment(0, t.reshape(base.shape), eval_only=True)
+        img = hip_ctx.render(wl.camera, st).beauty.reshape(st.height, st.width, 4)[..., :3]
+        return loss(img.astype(np.float64))
+
+    ad, fd = [], []
+    for k in picks:
+        h = 1e-3 * max(1.0, abs(float(base.ravel()[k])))
+        fd.append((loss_at(k, h) - loss_at(k, -h)) / (2 * h))
+        ad.append(float(g.ravel()[k]))
+    hip_ctx.update_environment(0, base)
+    ad, fd = np.array(ad), np.array(fd)
+    err = np.linalg.norm(ad - fd) / np.linalg.norm(fd)
+    _log(f"c5 env adjoint vs central differences over {len(picks)} texel channels: relative L2 error {err:.2e}")
+    assert err <= 1e-3, (err, ad[:5], fd[:5])
+
+
+def test_adam_with_the_dome_map_lowers_the_loss():
+    """100 Adam steps on base colour, roughness and the dome map together."""
+    import torch
+    from shiro_amd.diff import TextureRenderer
+    wl = workloads.c5(scale=0.5, spp=8)
+    st = wl.settings
+    alb_t, rough_t = wl.textures_truth
+    alb0, rough0 = wl.textures_init
+    _log("c5 half-size Adam with the dome map")
+    r = TextureRenderer(c5_scene_with(wl, alb_t, rough_t), wl.camera, st, env_domes=[0])
+    env_t = torch.from_numpy(wl.env_truth).cuda()
+    with torch.no_grad():
+        target = r.render(torch.from_numpy(alb_t).cuda(), torch.from_numpy(rough_t).cuda(), env=[env_t]).clone()
+    albedo = torch.from_numpy(alb0.copy()).cuda().requires_grad_(True)
+    rough = torch.from_numpy(rough0.copy()).cuda().requires_grad_(True)
+    sky = torch.from_numpy(wl.env_init.copy()).cuda().requires_grad_(True)
+    opt = torch.optim.Adam([{"params": [albedo, rough], "lr": 0.02}, {"params": [sky], "lr": 0.05}])
+    losses = []
+    for step in range(100):
+        opt.zero_grad()
+        img = r.render(albedo, rough, env=[sky])
+        loss = ((img - target) ** 2).mean()
+        loss.backward()
+        assert sky.grad is not None and torch.isfinite(sky.grad).all()
+        opt.step()
+        with torch.no_grad():
+            albedo.clamp_(0.0, 1.0)
+            rough.clamp_(0.05, 1.0)
+            sky.clamp_(min=0.0)
+        losses.append(loss.item())
+    _log(f"loss {losses[0]:.4e} -> {losses[-1]:.4e}")
+    assert np.isfinite(losses).all()
+    assert losses[-1] < 0.75 * losses[0]

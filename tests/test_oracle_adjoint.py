@@ -181,3 +181,69 @@ def test_adjoint_matches_frozen_sampling_differences(which):
     assert np.linalg.norm(fd) > 0
     print(f"adjoint vs differences: relative L2 error {err:.2e} over {len(picks)} texels")
     assert err < 1e-3, (err, ad[:6], fd[:6])
+
+
+def test_env_adjoint_matches_frozen_sampling_differences():
+    """Dome-map texels: the oracle's adjoint against Richardson-extrapolated
+    central differences of the render whose importance tables stay those of
+    the unperturbed map (eval-only updates).  The render is linear in the map
+    except where the direct-light clamp binds, so a step proportional to the
+    texel's magnitude keeps the differences accurate."""
+    wl = c5_mini(size=32, spp=4)
+    st = wl.settings
+    rng = np.random.default_rng(12)
+    alb, rough = wl.textures_truth
+    scene = c5_scene_with(wl, alb, rough)
+    base = np.array(scene.domeLights[0].pixels, np.float32)
+    osc = OracleScene(scene)
+    weights = rng.normal(size=(st.height, st.width, 4)).astype(np.float32)
+    weights[..., 3] = 0.0
+    grads = osc.render_adjoint(wl.camera, st, weights, wanted=[], env_wanted=[0], threads=8)
+    g = grads[("env", 0)]
+    assert g.shape == base.shape
+    flat = np.abs(g).ravel()
+    order = np.argsort(-flat)
+    picks = np.concatenate([order[:12], rng.choice(order[:300], 8, replace=False)])
+
+    def loss_at(k, d):
+        t = base.copy().ravel()
+        t[k] += d
+        osc.update_environment(0, t.reshape(base.shape), eval_only=True)
+        img = osc.render(wl.camera, st, threads=8).beauty.reshape(st.height, st.width, 4)
+        return float(np.sum(img[..., :3].astype(np.float64) * weights[..., :3]))
+
+    ad, fd = [], []
+    for k in picks:
+        h = 1e-2 * max(1.0, abs(float(base.ravel()[k])))
+        d1 = (loss_at(k, h) - loss_at(k, -h)) / (2 * h)
+        d2 = (loss_at(k, h / 2) - loss_at(k, -h / 2)) / h
+        fd.append((4 * d2 - d1) / 3)
+        ad.append(float(g.ravel()[k]))
+    osc.update_environment(0, base, eval_only=False)
+    ad, fd = np.array(ad), np.array(fd)
+    err = np.linalg.norm(ad - fd) / np.linalg.norm(fd)
+    assert np.linalg.norm(fd) > 0
+    print(f"env adjoint vs differences: relative L2 error {err:.2e} over {len(picks)} texels")
+    assert err < 1e-3, (err, ad[:6], fd[:6])
+
+
+def test_env_update_rebuilds_tables_like_upload():
+    """An environment update without eval_only samples exactly as the map
+    uploaded with the scene (same tables): frames bit-identical."""
+    wl = c5_mini(size=24, spp=2)
+    st = wl.settings
+    alb, rough = wl.textures_truth
+    scene = c5_scene_with(wl, alb, rough)
+    px = np.array(scene.domeLights[0].pixels, np.float32)
+    other = np.ascontiguousarray(px[::-1, ::-1] * 1.5)
+    a = OracleScene(scene)
+    a.update_environment(0, other)
+    fa = a.render(wl.camera, st, threads=8)
+    import copy
+    from shiro_amd.scene import DomeLight
+    s2 = copy.copy(scene)
+    d = scene.domeLights[0]
+    s2.domeLights = [DomeLight(radiance=d.radiance, pixels=other, layout=d.layout, right=d.right, up=d.up,
+                               forward=d.forward)]
+    fb = OracleScene(s2).render(wl.camera, st, threads=8)
+    assert np.array_equal(fa.beauty, fb.beauty)
