@@ -32,6 +32,22 @@ def workload_desc(wl):
             f"{len(wl.scene.distantLights)} directional lights")
 
 
+def host_cpu():
+    """CPU model and logical CPU counts of this host (the CPU baseline's
+    machine), from /proc/cpuinfo and the process's affinity mask."""
+    import os
+    model = "unknown CPU"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return f"{model}; {os.cpu_count()} logical CPUs, {len(os.sched_getaffinity(0))} available to this process"
+
+
 def cpu_baseline(wl, target_seconds, threads):
     """Times the CPU oracle (binary SAH BVH, this host's cores) on a band of
     full-width rows of the same frame at the full spp."""
@@ -68,6 +84,8 @@ def cpu_baseline(wl, target_seconds, threads):
         "sample": f"rows {y0}-{y0 + rows2 - 1} of the same frame ({st.width}x{rows2} px x {spp} spp = "
                   f"{samples} samples) in {dt:.1f} s; CPU oracle with a binary SAH BVH "
                   f"(built in {build_s:.1f} s, not timed), row-parallel threads",
+        "per_core": samples / dt / 1e6 / threads,
+        "host": host_cpu(),
     }
 
 
@@ -221,7 +239,8 @@ def bench_c5(args):
         cpu = {"value": reps * rows / st.height / dt, "unit": "grads/s", "cores": args.cpu_threads, "kind": "port",
                "sample": f"{reps} x rows {y0}-{y0 + rows - 1} ({st.width}x{rows} px, {spp} spp forward + {spp} spp "
                          f"adjoint into both textures and the map) in {dt:.1f} s, scaled to the full frame; "
-                         f"CPU oracle, row-parallel threads"}
+                         f"CPU oracle, row-parallel threads",
+               "per_core": reps * rows / st.height / dt / args.cpu_threads, "host": host_cpu()}
     out = {
         "metric": "grads/s", "value": round(value, 3), "unit": "grads/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(wall * 1e3 / args.steps, 3), "higher_is_better": True,
@@ -253,6 +272,8 @@ def main():
     ap.add_argument("--paths-per-pass", type=int, default=None, help="force the wavefront paths per pass")
     ap.add_argument("--primary-packets", type=int, default=None, choices=[0, 1],
                     help="force tile-packet tracing of camera rays off (0) or on (1)")
+    ap.add_argument("--bvh-width", type=int, default=None, choices=[4, 8],
+                    help="force the tree width the wavefront traversal walks")
     ap.add_argument("--shade-order", type=int, default=None, choices=[0, 1],
                     help="force shading-class ordering of the hit queue off (0) or on (1)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -292,7 +313,8 @@ def main():
     ctx.upload(wl.scene)
     upload_s = time.perf_counter() - t0
     info = ctx.scene_info()
-    print(f"# BVH4 nodes {info.bvhNodeCount} leaves {info.bvhLeafCount} depth {info.bvhDepth} "
+    print(f"# BVH4 nodes {info.bvhNodeCount} leaves {info.bvhLeafCount} depth {info.bvhDepth} stack {info.bvhStack}; "
+          f"8-wide nodes {info.bvh8NodeCount} depth {info.bvh8Depth} stack {info.bvh8Stack}; "
           f"build {info.buildSeconds:.2f}s upload {info.uploadSeconds:.2f}s", file=sys.stderr, flush=True)
     npx = st.width * st.height
     tuning = None
@@ -314,9 +336,12 @@ def main():
             ctx.set_tuning(shadeOrder=args.shade_order)
         if args.primary_packets is not None:
             ctx.set_tuning(primaryPackets=args.primary_packets)
+        if args.bvh_width is not None:
+            ctx.set_tuning(bvhWidth=args.bvh_width)
         t = ctx.get_tuning()
         tuning = {"pathsPerPass": t.pathsPerPass, "fetchMode": t.fetchMode, "traceBlocks": t.traceBlocks,
                   "stageBlocks": t.stageBlocks, "shadeOrder": t.shadeOrder, "primaryPackets": t.primaryPackets,
+                  "bvhWidth": t.bvhWidth if info.bvh8NodeCount else 4,
                   "autotuned": args.fetch_mode is None and args.paths_per_pass is None and not args.no_autotune}
     dev = torch.device("cuda:0")
     bufs = [torch.empty(npx * k, dtype=torch.float32, device=dev) for k in (4, 3, 3, 1)]
@@ -428,7 +453,8 @@ def main():
         "data": "synthetic (seeded procedural scene and dome map)",
         "config": {"workload": workload_desc(wl), "transport": args.transport, "tuning": tuning,
                    "samples_per_step": samples,
-                   "bvh": f"BVH4 {info.bvhNodeCount} nodes, depth {info.bvhDepth}",
+                   "bvh": f"BVH4 {info.bvhNodeCount} nodes, depth {info.bvhDepth}; 8-wide {info.bvh8NodeCount} nodes, "
+                          f"depth {info.bvh8Depth}",
                    "scene_upload_s": round(upload_s, 2), "finite_output": ok,
                    "end_to_end_ms_with_readback": round(e2e_ms, 2), "parity": parity},
         "roofline": roofline, "cpu_baseline": cpu,

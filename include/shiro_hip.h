@@ -327,6 +327,11 @@ typedef struct shiro_hip_scene_info {
     uint64_t deviceBytes;
     double buildSeconds;
     double uploadSeconds;
+    uint64_t bvh8NodeCount;  /* 8-wide tree over the same leaves (0: not built, see bvh8Stack) */
+    uint32_t bvh8Depth;
+    uint32_t bvhStack;       /* most traversal stack entries of the 4-wide tree */
+    uint32_t bvh8Stack;      /* of the 8-wide tree; above 95 it is not uploaded */
+    uint32_t reserved;
 } shiro_hip_scene_info;
 /* Replaces the texels of texture `index` of the uploaded scene (same size
  * and channel count) without rebuilding anything.  flags:
@@ -465,14 +470,17 @@ typedef struct shiro_hip_tuning {
     uint32_t batchBlocksPerCU; /* one-wave blocks per CU of the batched persistent kernel (default 8) */
     uint32_t shadeOrder;    /* 1: shade bounces after the first in shading-class order (default 1) */
     uint32_t primaryPackets; /* 1: camera rays traced as 64-ray tile packets (default 1) */
-    uint32_t reserved[1];
+    uint32_t bvhWidth;      /* 4 or 8: tree the wavefront traversal kernels walk (8 falls
+                               back to 4 for a scene without an 8-wide tree) */
 } shiro_hip_tuning;
 int shiro_hip_set_tuning(shiro_hip_ctx* ctx, const shiro_hip_tuning* tuning);
 int shiro_hip_get_tuning(const shiro_hip_ctx* ctx, shiro_hip_tuning* tuning);
 
 /* Times candidate tunings on `trialSamples` samples of the given frame
- * (wavefront transport, uploaded scene), keeps the fastest in the context
- * and returns it; `report` (may be NULL) receives one line per candidate.
+ * (wavefront transport, uploaded scene): pass sizes, fetch modes and trace
+ * grids, then the best of those with the other tree width.  Keeps the
+ * fastest in the context and returns it; `report` (may be NULL) receives
+ * one line per candidate.
  * Pass sizes above trialSamples x pixels time like the whole trial in one
  * pass, so trials should span the pass sizes of interest. */
 int shiro_hip_autotune(

@@ -117,6 +117,8 @@ class SceneInfo(C.Structure):
         ("bvhLeafCount", C.c_uint64), ("bvhDepth", C.c_uint32),
         ("maxLeafSize", C.c_uint32), ("deviceBytes", C.c_uint64),
         ("buildSeconds", C.c_double), ("uploadSeconds", C.c_double),
+        ("bvh8NodeCount", C.c_uint64), ("bvh8Depth", C.c_uint32), ("bvhStack", C.c_uint32),
+        ("bvh8Stack", C.c_uint32), ("reserved", C.c_uint32),
     ]
 
 
@@ -133,7 +135,7 @@ class Tuning(C.Structure):
     _fields_ = [
         ("pathsPerPass", C.c_uint32), ("fetchMode", C.c_uint32), ("traceBlocks", C.c_uint32),
         ("stageBlocks", C.c_uint32), ("batchBlocksPerCU", C.c_uint32), ("shadeOrder", C.c_uint32),
-        ("primaryPackets", C.c_uint32), ("reserved", C.c_uint32 * 1),
+        ("primaryPackets", C.c_uint32), ("bvhWidth", C.c_uint32),
     ]
 
 

@@ -88,8 +88,8 @@ struct shiro_hip_ctx {
     // Scene.
     bool hasScene = false;
     bool emissive = false;
-    DevBuf nodes, tris, hitdata, materials, derived, uvs, lightList, lightCdf, triClass;
-    Bvh4 bvh;               // host copy of the uploaded tree, for refits
+    DevBuf nodes, nodes8, tris, hitdata, materials, derived, uvs, lightList, lightCdf, triClass;
+    Bvh bvh;                // host copy of the uploaded trees, for refits
     uint64_t topology = 0;  // topology_hash of the uploaded scene
     EnvBuffers env[kMaxDomes];
     std::vector<std::unique_ptr<TexBuffers>> tex;
@@ -107,7 +107,7 @@ struct shiro_hip_ctx {
     DevBuf acB, acA, acN, acD, acC;
     uint32_t acW = 0, acH = 0, acTotal = 0;
     // Wavefront tuning (shiro_hip_set_tuning / shiro_hip_autotune).
-    shiro_hip_tuning tuning{64u << 20, 0u, 8192u, 16384u, 8u, 1u, 1u, {0}};
+    shiro_hip_tuning tuning{64u << 20, 0u, 8192u, 16384u, 8u, 1u, 1u, 4u};
     uint32_t computeUnits = 256;
     // Batched rendering: items, work list and the work counter.
     DevBuf manyItems, manyWork, manyCounter;
@@ -301,6 +301,7 @@ int enqueue_wavefront(shiro_hip_ctx* ctx, const DevScene& scene, const shiro_hip
     L.traceBlocks = ctx->tuning.traceBlocks;
     L.fetchMode = ctx->tuning.fetchMode;
     L.primaryPackets = ctx->tuning.primaryPackets;
+    L.bvhWidth = ctx->tuning.bvhWidth == 8u && ctx->dev.nodes8 ? 8u : 4u;
     if (events) {
         HIPC(hipEventRecord(ctx->ev0, ctx->stream), "event");
         begin_marks(ctx);
@@ -535,14 +536,14 @@ int upload_scene(shiro_hip_ctx* ctx, const shiro_hip_scene* s, bool refit) {
     std::string err;
     if (!flatten_scene(*s, &flat, &err)) return fail(ctx, SHIRO_HIP_INVALID, err);
     const size_t nt = flat.material.size();
-    Bvh4 bvh;
+    Bvh bvh;
     if (refit) {
         bvh = std::move(ctx->bvh);
-        if (!refit_bvh4(&bvh, flat.tri.data(), nt, &err)) return fail(ctx, SHIRO_HIP_INVALID, err);
-    } else if (!build_bvh4(flat.tri.data(), nt, kMaxBvhDepth, &bvh, &err)) {
+        if (!refit_bvh(&bvh, flat.tri.data(), nt, &err)) return fail(ctx, SHIRO_HIP_INVALID, err);
+    } else if (!build_bvh(flat.tri.data(), nt, kMaxBvhDepth, &bvh, &err)) {
         return fail(ctx, SHIRO_HIP_INVALID, err);
     }
-    if (!validate_bvh4(bvh, flat.tri.data(), nt, kMaxBvhDepth, &err)) return fail(ctx, SHIRO_HIP_INVALID, "BVH check: " + err);
+    if (!validate_bvh(bvh, flat.tri.data(), nt, kMaxBvhDepth, &err)) return fail(ctx, SHIRO_HIP_INVALID, "BVH check: " + err);
     const auto t1 = std::chrono::steady_clock::now();
     // Emissive-triangle lights and the triangle -> light map.
     std::vector<EmissiveEntry> lights;
@@ -587,6 +588,8 @@ int upload_scene(shiro_hip_ctx* ctx, const shiro_hip_scene* s, bool refit) {
     }
     if (nt) HIPC(upload(ctx->triClass, cls.data(), cls.size(), ctx->stream), "upload triangle classes");
     HIPC(upload(ctx->nodes, bvh.nodes.data(), bvh.nodes.size(), ctx->stream), "upload nodes");
+    if (!bvh.nodes8.empty())
+        HIPC(upload(ctx->nodes8, bvh.nodes8.data(), bvh.nodes8.size(), ctx->stream), "upload 8-wide nodes");
     HIPC(upload(ctx->tris, tris.data(), tris.size(), ctx->stream), "upload triangles");
     HIPC(upload(ctx->hitdata, hit.data(), hit.size(), ctx->stream), "upload hit data");
     HIPC(upload(ctx->materials, mats.data(), mats.size(), ctx->stream), "upload materials");
@@ -646,9 +649,11 @@ int upload_scene(shiro_hip_ctx* ctx, const shiro_hip_scene* s, bool refit) {
     d.materials = ctx->materials.as<shiro_hip_material>();
     d.derived = ctx->derived.as<MaterialDerived>();
     d.nodeCount = static_cast<uint32_t>(bvh.nodeCount());
+    d.nodes8 = bvh.nodes8.empty() ? nullptr : ctx->nodes8.as<float4>();
+    d.node8Count = static_cast<uint32_t>(bvh.node8Count());
     d.triCount = static_cast<uint32_t>(nt);
     d.materialCount = static_cast<uint32_t>(mats.size());
-    uint64_t bytes = ctx->nodes.bytes + ctx->tris.bytes + ctx->hitdata.bytes + ctx->materials.bytes + ctx->derived.bytes;
+    uint64_t bytes = ctx->nodes.bytes + (bvh.nodes8.empty() ? 0 : ctx->nodes8.bytes) + ctx->tris.bytes + ctx->hitdata.bytes + ctx->materials.bytes + ctx->derived.bytes;
     if (flat.hasUvs) bytes += ctx->uvs.bytes;
     if (!lights.empty()) bytes += ctx->lightList.bytes + ctx->lightCdf.bytes;
     for (const auto& b : ctx->tex) bytes += b->eval.bytes;
@@ -724,6 +729,10 @@ int upload_scene(shiro_hip_ctx* ctx, const shiro_hip_scene* s, bool refit) {
     ctx->info.bvhNodeCount = bvh.nodeCount();
     ctx->info.bvhLeafCount = bvh.leafCount;
     ctx->info.bvhDepth = bvh.depth;
+    ctx->info.bvh8NodeCount = bvh.node8Count();
+    ctx->info.bvh8Depth = bvh.depth8;
+    ctx->info.bvhStack = bvh.stack;
+    ctx->info.bvh8Stack = bvh.stack8;
     ctx->info.maxLeafSize = bvh.maxLeafSize;
     ctx->info.deviceBytes = bytes;
     ctx->info.buildSeconds = std::chrono::duration<double>(t1 - t0).count();
@@ -1034,7 +1043,7 @@ int shiro_hip_set_tuning(shiro_hip_ctx* ctx, const shiro_hip_tuning* t) {
     std::lock_guard<std::mutex> lock(ctx->mu);
     if (t->pathsPerPass == 0 || t->fetchMode > 2u || t->traceBlocks == 0 || t->stageBlocks == 0 ||
         t->batchBlocksPerCU == 0 || t->batchBlocksPerCU > 64u || t->shadeOrder > 1u ||
-        t->primaryPackets > 1u)
+        t->primaryPackets > 1u || (t->bvhWidth != 4u && t->bvhWidth != 8u))
         return fail(ctx, SHIRO_HIP_INVALID, "invalid tuning");
     ctx->tuning = *t;
     return SHIRO_HIP_OK;
@@ -1081,30 +1090,45 @@ int shiro_hip_autotune(shiro_hip_ctx* ctx, const shiro_hip_camera* camera, const
     std::string log;
     float bestMs = 0.0f;
     shiro_hip_tuning bestT = base;
-    for (size_t i = 0; i < cands.size(); ++i) {
-        ctx->tuning = cands[i];
-        // One untimed warm run, then the timed run.
-        float ms = 0.0f;
+    // One untimed warm run, then the timed run; appends the report line.
+    auto timeCandidate = [&](const shiro_hip_tuning& t, float* ms) -> int {
+        ctx->tuning = t;
         for (int rep = 0; rep < 2; ++rep) {
-            rc = enqueue_batch(ctx, *camera, st, 0, spp, f, false, nullptr);
-            if (rc) {
+            int r = enqueue_batch(ctx, *camera, st, 0, spp, f, false, nullptr);
+            if (!r) r = finish_timing(ctx);
+            if (r) {
                 ctx->tuning = base;
-                return rc;
+                return r;
             }
-            rc = finish_timing(ctx);
-            if (rc) {
-                ctx->tuning = base;
-                return rc;
-            }
-            ms = ctx->lastMs;
+            *ms = ctx->lastMs;
         }
-        char line[160];
-        std::snprintf(line, sizeof(line), "pathsPerPass %u fetchMode %u traceBlocks %u: %.3f ms\n",
-                      cands[i].pathsPerPass, cands[i].fetchMode, cands[i].traceBlocks, ms);
+        const uint32_t width = t.bvhWidth == 8u && ctx->dev.nodes8 ? 8u : 4u;
+        char line[192];
+        std::snprintf(line, sizeof(line), "pathsPerPass %u fetchMode %u traceBlocks %u bvhWidth %u: %.3f ms\n",
+                      t.pathsPerPass, t.fetchMode, t.traceBlocks, width, *ms);
         log += line;
+        return 0;
+    };
+    for (size_t i = 0; i < cands.size(); ++i) {
+        float ms = 0.0f;
+        rc = timeCandidate(cands[i], &ms);
+        if (rc) return rc;
         if (i == 0 || ms < bestMs) {
             bestMs = ms;
             bestT = cands[i];
+        }
+    }
+    // Second stage: the best candidate with the other tree width, when the
+    // scene has an 8-wide tree.
+    if (ctx->dev.nodes8) {
+        shiro_hip_tuning t = bestT;
+        t.bvhWidth = bestT.bvhWidth == 8u ? 4u : 8u;
+        float ms = 0.0f;
+        rc = timeCandidate(t, &ms);
+        if (rc) return rc;
+        if (ms < bestMs) {
+            bestMs = ms;
+            bestT = t;
         }
     }
     ctx->tuning = bestT;

@@ -1,4 +1,4 @@
-// Host BVH4 builder; see bvh_build.h.
+// Host BVH builder (4- and 8-wide); see bvh_build.h.
 #include "bvh_build.h"
 
 #include <algorithm>
@@ -161,14 +161,40 @@ void pad(Box& b) {
     }
 }
 
+// Node layouts of the two widths (layout.h): floats per node, where the
+// child words start, and the uint holding plane byte k of axis a (hi: the
+// hi plane) with the byte's position in it.
+template <int W>
+struct NodeLayout;
+template <>
+struct NodeLayout<4> {
+    static constexpr size_t kFloats = kNodeFloats;
+    static int plane(int a, int hi, int k, int* shift) {
+        *shift = 8 * k;
+        return 8 + 2 * a + hi;
+    }
+};
+template <>
+struct NodeLayout<8> {
+    static constexpr size_t kFloats = kNode8Floats;
+    static int plane(int a, int hi, int k, int* shift) {
+        *shift = 8 * (k & 3);
+        return 12 + 4 * a + 2 * hi + (k >> 2);
+    }
+};
+
 // Quantises a node's child boxes to 8 bits per plane relative to their
 // union (layout.h): origin = union.lo, per-axis scale 2^e with
 // 255 * 2^e >= extent, lo rounded down and hi rounded up, so every
-// dequantised box contains the exact one.
+// dequantised box contains the exact one.  Empty children get inverted
+// planes (lo 255, hi 0) and are masked out by the valid bits.
+template <int W>
 void quantize_node(const Box* boxes, int nk, const uint32_t* words, float* nd) {
+    using L = NodeLayout<W>;
     Box u;
     for (int k = 0; k < nk; ++k) u.grow(boxes[k]);
-    uint32_t q[6] = {0, 0, 0, 0, 0, 0};  // lo.x hi.x lo.y hi.y lo.z hi.z, byte k = child k
+    std::fill(nd, nd + L::kFloats, 0.0f);
+    uint32_t q[L::kFloats] = {};
     uint32_t expo = 0;
     for (int a = 0; a < 3; ++a) {
         const double origin = u.lo[a];
@@ -177,8 +203,8 @@ void quantize_node(const Box* boxes, int nk, const uint32_t* words, float* nd) {
         e = std::max(-100, std::min(100, e));
         while (std::ldexp(255.0, e) < extent) ++e;
         const double s = std::ldexp(1.0, e);
-        for (int k = 0; k < 4; ++k) {
-            uint32_t lo = 0xffu, hi = 0u;  // empty: inverted, and masked out by the valid bits
+        for (int k = 0; k < W; ++k) {
+            uint32_t lo = 0xffu, hi = 0u;
             if (k < nk) {
                 double ql = std::floor((static_cast<double>(boxes[k].lo[a]) - origin) / s);
                 double qh = std::ceil((static_cast<double>(boxes[k].hi[a]) - origin) / s);
@@ -189,8 +215,11 @@ void quantize_node(const Box* boxes, int nk, const uint32_t* words, float* nd) {
                 lo = static_cast<uint32_t>(ql);
                 hi = static_cast<uint32_t>(qh);
             }
-            q[2 * a] |= lo << (8 * k);
-            q[2 * a + 1] |= hi << (8 * k);
+            int sh;
+            const int il = L::plane(a, 0, k, &sh);
+            q[il] |= lo << sh;
+            const int ih = L::plane(a, 1, k, &sh);
+            q[ih] |= hi << sh;
         }
         expo |= static_cast<uint32_t>(e + 128) << (8 * a);
         nd[a] = u.lo[a];
@@ -199,26 +228,50 @@ void quantize_node(const Box* boxes, int nk, const uint32_t* words, float* nd) {
     for (int k = 0; k < nk; ++k) valid |= 1u << k;
     expo |= valid << 24;
     std::memcpy(&nd[3], &expo, 4);
-    std::memcpy(&nd[4], words, 16);
-    std::memcpy(&nd[8], q, 24);
-    nd[14] = nd[15] = 0.0f;
+    uint32_t w[W];
+    for (int k = 0; k < W; ++k) w[k] = k < nk ? words[k] : kEmpty;
+    std::memcpy(&nd[4], w, sizeof(w));
+    for (int a = 0; a < 3; ++a)
+        for (int hi = 0; hi < 2; ++hi)
+            for (int k = 0; k < W; k += 4) {
+                int sh;
+                const int i = L::plane(a, hi, k, &sh);
+                std::memcpy(&nd[i], &q[i], 4);
+            }
 }
 
+// Child words of a node (W of them; kEmpty past the valid ones) and its
+// valid mask.
+template <int W>
+uint32_t node_words(const float* nd, uint32_t* words) {
+    uint32_t ex;
+    std::memcpy(&ex, &nd[3], 4);
+    std::memcpy(words, &nd[4], 4 * W);
+    return ex >> 24;
+}
+
+// Collapses the binary tree into W-wide nodes: each node adopts its two
+// children and repeatedly opens the internal child of largest surface area
+// until it has W.  Leaves are the binary tree's, so every width shares the
+// triangle order.
+template <int W>
 struct Collapser {
     const Builder& B;
-    Bvh4* out;
-    float slack = 0.0f;   // absolute widening of every box (scene-scale rounding margin)
-    uint32_t maxDepth = 0;
+    std::vector<float>* nodes;
+    Bvh* out;              // leaf statistics (filled by the 4-wide pass only)
+    float slack = 0.0f;    // absolute widening of every box (scene-scale rounding margin)
 
     uint32_t alloc() {
-        const uint32_t id = static_cast<uint32_t>(out->nodes.size() / kNodeFloats);
-        out->nodes.resize(out->nodes.size() + kNodeFloats, 0.0f);
+        const uint32_t id = static_cast<uint32_t>(nodes->size() / NodeLayout<W>::kFloats);
+        nodes->resize(nodes->size() + NodeLayout<W>::kFloats, 0.0f);
         return id;
     }
 
-    // Fills 4-wide node `id` from binary node `bi`; returns the subtree depth.
-    uint32_t emit(uint32_t id, uint32_t bi) {
-        uint32_t kids[4];
+    // Fills node `id` from binary node `bi`; returns the subtree depth and,
+    // in *stack, the most stack entries a traversal below it holds (each
+    // node on the way down leaves at most its other children pushed).
+    uint32_t emit(uint32_t id, uint32_t bi, uint32_t* stack) {
+        uint32_t kids[W];
         int nk = 0;
         const Node2& n = B.nodes[bi];
         if (n.count > 0) {
@@ -226,7 +279,7 @@ struct Collapser {
         } else {
             kids[nk++] = n.left;
             kids[nk++] = n.left + 1;
-            while (nk < 4) {
+            while (nk < W) {
                 int best = -1;
                 float bestArea = -1.0f;
                 for (int k = 0; k < nk; ++k) {
@@ -243,11 +296,12 @@ struct Collapser {
             }
         }
         // Allocate the internal children first so they are contiguous.
-        uint32_t childId[4] = {kEmpty, kEmpty, kEmpty, kEmpty};
+        uint32_t childId[W];
+        for (int k = 0; k < W; ++k) childId[k] = kEmpty;
         for (int k = 0; k < nk; ++k)
             if (B.nodes[kids[k]].count == 0) childId[k] = alloc();
-        Box boxes[4];
-        uint32_t words[4] = {kEmpty, kEmpty, kEmpty, kEmpty};
+        Box boxes[W];
+        uint32_t words[W];
         for (int k = 0; k < nk; ++k) {
             const Node2& c = B.nodes[kids[k]];
             boxes[k] = c.box;
@@ -258,28 +312,37 @@ struct Collapser {
             }
             if (c.count > 0) {
                 words[k] = kLeafBit | c.left;
-                out->leafCount += 1;
-                out->maxLeafSize = std::max(out->maxLeafSize, c.count);
-                out->leafEnd[c.left + c.count - 1] = 1;
+                if (W == 4) {
+                    out->leafCount += 1;
+                    out->maxLeafSize = std::max(out->maxLeafSize, c.count);
+                    out->leafEnd[c.left + c.count - 1] = 1;
+                }
             } else {
                 words[k] = childId[k];
             }
         }
-        quantize_node(boxes, nk, words, &out->nodes[static_cast<size_t>(id) * kNodeFloats]);
-        uint32_t sub = 0;
+        quantize_node<W>(boxes, nk, words, &(*nodes)[static_cast<size_t>(id) * NodeLayout<W>::kFloats]);
+        uint32_t sub = 0, below = 0;
         for (int k = 0; k < nk; ++k)
-            if (childId[k] != kEmpty) sub = std::max(sub, emit(childId[k], kids[k]));
+            if (childId[k] != kEmpty) {
+                uint32_t st = 0;
+                sub = std::max(sub, emit(childId[k], kids[k], &st));
+                below = std::max(below, st);
+            }
+        *stack = static_cast<uint32_t>(nk - 1) + below;
         return sub + 1;
     }
 };
 
 }  // namespace
 
-bool build_bvh4(const float* tris, size_t n, uint32_t maxDepth, Bvh4* out, std::string* err) {
+bool build_bvh(const float* tris, size_t n, uint32_t maxDepth, Bvh* out, std::string* err) {
     out->nodes.clear();
+    out->nodes8.clear();
     out->order.clear();
     out->leafEnd.clear();
-    out->depth = 0;
+    out->depth = out->depth8 = 0;
+    out->stack = out->stack8 = 0;
     out->leafCount = 0;
     out->maxLeafSize = 0;
     if (n == 0) return true;
@@ -310,22 +373,30 @@ bool build_bvh4(const float* tris, size_t n, uint32_t maxDepth, Bvh4* out, std::
     float scale = 0.0f;
     for (int a = 0; a < 3; ++a)
         scale = std::max(scale, std::max(std::fabs(B.nodes[0].box.lo[a]), std::fabs(B.nodes[0].box.hi[a])));
-    Collapser C{B, out, scale * 16.0f * FLT_EPSILON};
+    const float slack = scale * 16.0f * FLT_EPSILON;
+    Collapser<4> C{B, &out->nodes, out, slack};
     out->nodes.reserve(kNodeFloats * (n / 2 + 4));
-    const uint32_t root = C.alloc();
-    out->depth = C.emit(root, 0);
+    out->depth = C.emit(C.alloc(), 0, &out->stack);
     if (out->depth > maxDepth) {
         *err = "BVH4 depth " + std::to_string(out->depth) + " exceeds the traversal stack bound " +
                std::to_string(maxDepth);
         return false;
+    }
+    Collapser<8> C8{B, &out->nodes8, out, slack};
+    out->depth8 = C8.emit(C8.alloc(), 0, &out->stack8);
+    if (out->stack8 + 1 > static_cast<uint32_t>(kStackSize)) {
+        out->nodes8.clear();
+        out->nodes8.shrink_to_fit();
     }
     return true;
 }
 
 namespace {
 
+template <int W>
 struct Refitter {
-    Bvh4& b;
+    Bvh& b;
+    std::vector<float>& nodes;
     const float* tris;
     float slack;
 
@@ -342,14 +413,12 @@ struct Refitter {
     // Requantises node id from its children's exact boxes; returns the union
     // of those boxes (the node's exact box, before padding).
     Box fit(uint32_t id) {
-        float* nd = &b.nodes[static_cast<size_t>(id) * kNodeFloats];
-        uint32_t ex, words[4];
-        std::memcpy(&ex, &nd[3], 4);
-        std::memcpy(words, &nd[4], 16);
-        const uint32_t valid = ex >> 24;
+        float* nd = &nodes[static_cast<size_t>(id) * NodeLayout<W>::kFloats];
+        uint32_t words[W];
+        const uint32_t valid = node_words<W>(nd, words);
         int nk = 0;
-        while (nk < 4 && (valid >> nk) & 1u) ++nk;
-        Box boxes[4], u;
+        while (nk < W && (valid >> nk) & 1u) ++nk;
+        Box boxes[W], u;
         for (int k = 0; k < nk; ++k) {
             Box c;
             if (words[k] & kLeafBit) {
@@ -368,14 +437,14 @@ struct Refitter {
                 boxes[k].hi[a] += slack;
             }
         }
-        quantize_node(boxes, nk, words, nd);
+        quantize_node<W>(boxes, nk, words, nd);
         return u;
     }
 };
 
 }  // namespace
 
-bool refit_bvh4(Bvh4* b, const float* tris, size_t n, std::string* err) {
+bool refit_bvh(Bvh* b, const float* tris, size_t n, std::string* err) {
     if (n != b->order.size()) {
         *err = "refit: triangle count differs from the built tree's";
         return false;
@@ -389,27 +458,38 @@ bool refit_bvh4(Bvh4* b, const float* tris, size_t n, std::string* err) {
         }
     float scale = 0.0f;
     for (int a = 0; a < 3; ++a) scale = std::max(scale, std::max(std::fabs(all.lo[a]), std::fabs(all.hi[a])));
-    Refitter R{*b, tris, scale * 16.0f * FLT_EPSILON};
+    const float slack = scale * 16.0f * FLT_EPSILON;
+    Refitter<4> R{*b, b->nodes, tris, slack};
     R.fit(0);
+    if (!b->nodes8.empty()) {
+        Refitter<8> R8{*b, b->nodes8, tris, slack};
+        R8.fit(0);
+    }
     return true;
 }
 
 namespace {
 
 // Dequantised box of child k of a node, in double precision.
+template <int W>
 void child_box(const float* nd, int k, double lo[3], double hi[3]) {
-    uint32_t ex, q[6];
+    uint32_t ex;
     std::memcpy(&ex, &nd[3], 4);
-    std::memcpy(q, &nd[8], 24);
     for (int a = 0; a < 3; ++a) {
         const double s = std::ldexp(1.0, static_cast<int>((ex >> (8 * a)) & 0xffu) - 128);
-        lo[a] = nd[a] + static_cast<double>((q[2 * a] >> (8 * k)) & 0xffu) * s;
-        hi[a] = nd[a] + static_cast<double>((q[2 * a + 1] >> (8 * k)) & 0xffu) * s;
+        int sh;
+        uint32_t ql, qh;
+        std::memcpy(&ql, &nd[NodeLayout<W>::plane(a, 0, k, &sh)], 4);
+        std::memcpy(&qh, &nd[NodeLayout<W>::plane(a, 1, k, &sh)], 4);
+        lo[a] = nd[a] + static_cast<double>((ql >> sh) & 0xffu) * s;
+        hi[a] = nd[a] + static_cast<double>((qh >> sh) & 0xffu) * s;
     }
 }
 
+template <int W>
 struct Checker {
-    const Bvh4& b;
+    const Bvh& b;
+    const std::vector<float>& nodes;
     const float* tris;
     size_t n;
     uint32_t maxDepth;
@@ -426,13 +506,12 @@ struct Checker {
     // it and checks that each child's box contains its triangles.
     bool visit(uint32_t id, uint32_t depth, double lo[3], double hi[3]) {
         if (depth > maxDepth) return fail("depth bound exceeded");
-        const float* nd = &b.nodes[static_cast<size_t>(id) * kNodeFloats];
-        uint32_t ex;
-        std::memcpy(&ex, &nd[3], 4);
-        for (int k = 0; k < 4; ++k) {
-            uint32_t w;
-            std::memcpy(&w, &nd[4 + k], 4);
-            const bool valid = (ex >> (24 + k)) & 1u;
+        const float* nd = &nodes[static_cast<size_t>(id) * NodeLayout<W>::kFloats];
+        uint32_t words[W];
+        const uint32_t validMask = node_words<W>(nd, words);
+        for (int k = 0; k < W; ++k) {
+            const uint32_t w = words[k];
+            const bool valid = (validMask >> k) & 1u;
             if (w == kEmpty) {
                 if (valid) return fail("empty child marked valid");
                 continue;
@@ -460,7 +539,7 @@ struct Checker {
                 if (!visit(w, depth + 1, clo, chi)) return false;
             }
             double qlo[3], qhi[3];
-            child_box(nd, k, qlo, qhi);
+            child_box<W>(nd, k, qlo, qhi);
             for (int a = 0; a < 3; ++a) {
                 if (!(qlo[a] <= clo[a] && qhi[a] >= chi[a])) return fail("quantised child box does not contain its triangles");
                 lo[a] = std::min(lo[a], clo[a]);
@@ -471,16 +550,16 @@ struct Checker {
     }
 };
 
-}  // namespace
-
-bool validate_bvh4(const Bvh4& b, const float* tris, size_t n, uint32_t maxDepth, std::string* err) {
-    const uint64_t nn = b.nodeCount();
+template <int W>
+bool validate_width(const Bvh& b, const std::vector<float>& nodes, const float* tris, size_t n, uint32_t maxDepth,
+                    std::string* err) {
+    const uint64_t nn = nodes.size() / NodeLayout<W>::kFloats;
     if (n == 0) return nn == 0 || (*err = "nodes without triangles", false);
     if (nn == 0) {
         *err = "no nodes";
         return false;
     }
-    Checker c{b, tris, n, maxDepth, nn, std::vector<uint8_t>(nn, 0), std::vector<uint8_t>(n, 0), err};
+    Checker<W> c{b, nodes, tris, n, maxDepth, nn, std::vector<uint8_t>(nn, 0), std::vector<uint8_t>(n, 0), err};
     c.seen[0] = 1;
     double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
     if (!c.visit(0u, 1u, lo, hi)) return false;
@@ -489,6 +568,23 @@ bool validate_bvh4(const Bvh4& b, const float* tris, size_t n, uint32_t maxDepth
             *err = "triangle slot not referenced by any leaf";
             return false;
         }
+    return true;
+}
+
+}  // namespace
+
+bool validate_bvh(const Bvh& b, const float* tris, size_t n, uint32_t maxDepth, std::string* err) {
+    if (!validate_width<4>(b, b.nodes, tris, n, maxDepth, err)) return false;
+    if (b.nodes8.empty()) return true;
+    if (b.stack8 + 1 > static_cast<uint32_t>(kStackSize)) {
+        *err = "8-wide tree: traversal stack bound exceeded";
+        return false;
+    }
+    std::string e8;
+    if (!validate_width<8>(b, b.nodes8, tris, n, 1000u, &e8)) {
+        *err = "8-wide tree: " + e8;
+        return false;
+    }
     return true;
 }
 

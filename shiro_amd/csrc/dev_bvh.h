@@ -1,4 +1,4 @@
-// BVH4 traversal on gfx950 with a watertight ray/triangle test.
+// BVH traversal on gfx950 with a watertight ray/triangle test.
 //
 // Replaces Embree's rtcIntersect1 / rtcOccluded1 (reference
 // CpuPathTracer.cpp:601-681) and the brute-force fallback (:386-475).
@@ -18,6 +18,8 @@
 //    edges and vertices of a closed mesh cannot slip through.
 //  - The stack holds kStackSize entries per thread; the uploader rejects
 //    trees deeper than kMaxBvhDepth so that it cannot overflow.
+//  - The persistent traversal kernels can walk an 8-wide tree over the
+//    same leaves instead (node_boxes8, sort8): fewer, wider node steps.
 #pragma once
 
 #include "dev_math.h"
@@ -201,6 +203,96 @@ DEV uint32_t push_sorted(uint32_t mask, const float tn[4], const uint32_t w[4], 
         ++sp;
     }
     return v[0];
+}
+
+// Slab test of the eight children of an 8-wide node (layout.h, nodes8):
+// per child the same arithmetic as node_boxes, so both widths accept the
+// same (conservative) boxes.
+DEV uint32_t node_boxes8(const float4* __restrict__ nd, const RayPre& r, float tmax, float tn[8], uint32_t w[8]) {
+    const float4 h = nd[0];
+    const float4 c0 = nd[1], c1 = nd[2];
+    const float4 qx = nd[3], qy = nd[4], qz = nd[5];
+    const uint32_t ex = __float_as_uint(h.w);
+    const float ax = ldexpf(r.inv.x, static_cast<int>(ex & 0xffu) - 128);
+    const float ay = ldexpf(r.inv.y, static_cast<int>((ex >> 8) & 0xffu) - 128);
+    const float az = ldexpf(r.inv.z, static_cast<int>((ex >> 16) & 0xffu) - 128);
+    const float bx = (h.x - r.o.x) * r.inv.x;
+    const float by = (h.y - r.o.y) * r.inv.y;
+    const float bz = (h.z - r.o.z) * r.inv.z;
+    // Plane words: [lo 0-3, lo 4-7, hi 0-3, hi 4-7] per axis; near = lo
+    // for a non-negative direction component.
+    const bool px = r.inv.x >= 0.0f, py = r.inv.y >= 0.0f, pz = r.inv.z >= 0.0f;
+    const uint32_t nx[2] = {__float_as_uint(px ? qx.x : qx.z), __float_as_uint(px ? qx.y : qx.w)};
+    const uint32_t fx[2] = {__float_as_uint(px ? qx.z : qx.x), __float_as_uint(px ? qx.w : qx.y)};
+    const uint32_t ny[2] = {__float_as_uint(py ? qy.x : qy.z), __float_as_uint(py ? qy.y : qy.w)};
+    const uint32_t fy[2] = {__float_as_uint(py ? qy.z : qy.x), __float_as_uint(py ? qy.w : qy.y)};
+    const uint32_t nz[2] = {__float_as_uint(pz ? qz.x : qz.z), __float_as_uint(pz ? qz.y : qz.w)};
+    const uint32_t fz[2] = {__float_as_uint(pz ? qz.z : qz.x), __float_as_uint(pz ? qz.w : qz.y)};
+    uint32_t mask = 0;
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+        const int hw = c >> 2, b = c & 3;
+        const float t0 = fmaxf(fmaxf(__builtin_fmaf(byte_as_float(nx[hw], b), ax, bx),
+                                     __builtin_fmaf(byte_as_float(ny[hw], b), ay, by)),
+                               fmaxf(__builtin_fmaf(byte_as_float(nz[hw], b), az, bz), r.tmin));
+        const float t1 = fminf(fminf(__builtin_fmaf(byte_as_float(fx[hw], b), ax, bx),
+                                     __builtin_fmaf(byte_as_float(fy[hw], b), ay, by)),
+                               fminf(__builtin_fmaf(byte_as_float(fz[hw], b), az, bz), tmax));
+        tn[c] = t0;
+        mask |= (t0 <= t1 ? 1u : 0u) << c;
+    }
+    w[0] = __float_as_uint(c0.x);
+    w[1] = __float_as_uint(c0.y);
+    w[2] = __float_as_uint(c0.z);
+    w[3] = __float_as_uint(c0.w);
+    w[4] = __float_as_uint(c1.x);
+    w[5] = __float_as_uint(c1.y);
+    w[6] = __float_as_uint(c1.z);
+    w[7] = __float_as_uint(c1.w);
+    // Issued with the boxes' loads, as in node_boxes.
+    asm volatile("" : "+v"(w[0]), "+v"(w[1]), "+v"(w[2]), "+v"(w[3]));
+    asm volatile("" : "+v"(w[4]), "+v"(w[5]), "+v"(w[6]), "+v"(w[7]));
+    return mask & (ex >> 24);
+}
+
+// Sorts eight (key, value) pairs by key: the 19-comparator network of
+// depth 6 for eight inputs (Knuth, TAOCP vol. 3, 5.3.4).
+DEV void sort8(float k[8], uint32_t v[8]) {
+    cswap(k[0], v[0], k[2], v[2]);
+    cswap(k[1], v[1], k[3], v[3]);
+    cswap(k[4], v[4], k[6], v[6]);
+    cswap(k[5], v[5], k[7], v[7]);
+    cswap(k[0], v[0], k[4], v[4]);
+    cswap(k[1], v[1], k[5], v[5]);
+    cswap(k[2], v[2], k[6], v[6]);
+    cswap(k[3], v[3], k[7], v[7]);
+    cswap(k[0], v[0], k[1], v[1]);
+    cswap(k[2], v[2], k[3], v[3]);
+    cswap(k[4], v[4], k[5], v[5]);
+    cswap(k[6], v[6], k[7], v[7]);
+    cswap(k[2], v[2], k[4], v[4]);
+    cswap(k[3], v[3], k[5], v[5]);
+    cswap(k[1], v[1], k[4], v[4]);
+    cswap(k[3], v[3], k[6], v[6]);
+    cswap(k[1], v[1], k[2], v[2]);
+    cswap(k[3], v[3], k[4], v[4]);
+    cswap(k[5], v[5], k[6], v[6]);
+}
+
+// Node access by tree width: W = 4 walks `nodes` (64 B nodes), W = 8
+// walks `nodes8` (128 B nodes) over the same leaves.
+template <int W>
+DEV const float4* node_at(const DevScene& S, uint32_t id) {
+    return W == 8 ? S.nodes8 + static_cast<size_t>(id) * 8u : S.nodes + static_cast<size_t>(id) * 4u;
+}
+
+template <int W>
+DEV uint32_t node_test(const float4* __restrict__ nd, const RayPre& r, float tmax, float* tn, uint32_t* w) {
+    if constexpr (W == 8) {
+        return node_boxes8(nd, r, tmax, tn, w);
+    } else {
+        return node_boxes(nd, r, tmax, tn, w);
+    }
 }
 
 // Pops the next stack entry; a closest-hit traversal drops entries whose

@@ -58,6 +58,7 @@ struct WfLaunch {
     uint32_t traceBlocks;  // grid of the persistent traversal kernels
     uint32_t fetchMode;    // traversal work distribution, see k_wf_trace
     uint32_t primaryPackets;  // 1: camera rays traced as tile packets (k_wf_primary)
+    uint32_t bvhWidth;     // 4 or 8: tree of the traversal kernels (8 only if the scene has one)
 };
 
 // Optional per-kernel timing: an event recorded after each launch, tagged

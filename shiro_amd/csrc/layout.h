@@ -12,6 +12,14 @@
 //              [3] lo.z, hi.z, 0, 0
 //            A child's box is origin + q * 2^e per axis, rounded outwards
 //            on the host.  Two sibling nodes share one 128-byte L2 line.
+//   nodes8   the same leaves under 8-wide nodes, 128 B (one L2 line) per
+//            node = 8 x float4:
+//              [0] origin.xyz, w = exponent bytes + valid-child mask (byte 3)
+//              [1], [2] child words 0-3, 4-7
+//              [3] x planes: lo children 0-3, lo 4-7, hi 0-3, hi 4-7
+//              [4] y planes, [5] z planes (same order), [6], [7] zero
+//            Absent (no allocation) when the 8-wide tree's worst-case
+//            traversal stack exceeds kStackSize.
 //   tris     48 B per triangle in leaf order = 3 x float4:
 //              {v0.xyz, global prim id bits}, {v1.xyz, end-of-leaf flag},
 //              {v2.xyz, 0}
@@ -50,6 +58,7 @@ constexpr int kStackSize = 96;
 constexpr uint32_t kMaxBvhDepth = (kStackSize - 1) / 3;
 
 constexpr size_t kNodeFloats = 16;   // floats per BVH4 node (64 B)
+constexpr size_t kNode8Floats = 32;  // floats per 8-wide node (128 B)
 
 // Buckets of the environment CDF search guides (row table, each column
 // table).
@@ -150,6 +159,8 @@ struct DevScene {
     uint32_t textureCount;
     uint32_t textured;            // any material with kMatTextured
     uint32_t nodeCount;
+    const float4* nodes8;         // 8-wide tree over the same leaves, null if absent
+    uint32_t node8Count;
     uint32_t triCount;
     uint32_t materialCount;
     uint32_t rootIsLeaf;          // single-leaf scenes: root word in rootWord

@@ -178,21 +178,36 @@ DEV void ring_push(uint32_t n, float t, uint32_t lane, uint32_t* ovNode, float* 
     ++sp;
 }
 
-// push_sorted (dev_bvh.h) onto the ring stack.
-template <bool KEEP_T>
-DEV uint32_t ring_push_sorted(uint32_t mask, const float tn[4], const uint32_t w[4], uint32_t lane, uint32_t* ovNode,
+// push_sorted (dev_bvh.h) onto the ring stack, for W-wide nodes.
+template <bool KEEP_T, int W>
+DEV uint32_t ring_push_sorted(uint32_t mask, const float* tn, const uint32_t* w, uint32_t lane, uint32_t* ovNode,
                               float* ovT, int& sp, int& base) {
-    float k[4] = {(mask & 1u) ? tn[0] : kInf, (mask & 2u) ? tn[1] : kInf, (mask & 4u) ? tn[2] : kInf,
-                  (mask & 8u) ? tn[3] : kInf};
-    uint32_t v[4] = {w[0], w[1], w[2], w[3]};
-    cswap(k[0], v[0], k[1], v[1]);
-    cswap(k[2], v[2], k[3], v[3]);
-    cswap(k[0], v[0], k[2], v[2]);
-    cswap(k[1], v[1], k[3], v[3]);
-    cswap(k[1], v[1], k[2], v[2]);
     const int nh = __popc(mask);
-    for (int i = nh - 1; i >= 1; --i) ring_push<KEEP_T>(v[i], k[i], lane, ovNode, ovT, sp, base);
-    return v[0];
+    if constexpr (W == 4) {
+        float k[4] = {(mask & 1u) ? tn[0] : kInf, (mask & 2u) ? tn[1] : kInf, (mask & 4u) ? tn[2] : kInf,
+                      (mask & 8u) ? tn[3] : kInf};
+        uint32_t v[4] = {w[0], w[1], w[2], w[3]};
+        cswap(k[0], v[0], k[1], v[1]);
+        cswap(k[2], v[2], k[3], v[3]);
+        cswap(k[0], v[0], k[2], v[2]);
+        cswap(k[1], v[1], k[3], v[3]);
+        cswap(k[1], v[1], k[2], v[2]);
+        for (int i = nh - 1; i >= 1; --i) ring_push<KEEP_T>(v[i], k[i], lane, ovNode, ovT, sp, base);
+        return v[0];
+    } else {
+        float k[8];
+        uint32_t v[8];
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            k[c] = (mask >> c) & 1u ? tn[c] : kInf;
+            v[c] = w[c];
+        }
+        sort8(k, v);
+#pragma unroll
+        for (int i = 7; i >= 1; --i)
+            if (i < nh) ring_push<KEEP_T>(v[i], k[i], lane, ovNode, ovT, sp, base);
+        return v[0];
+    }
 }
 
 // pop_entry (dev_bvh.h) from the ring stack.
@@ -229,9 +244,10 @@ DEV uint32_t ring_pop(float tcull, uint32_t lane, const uint32_t* ovNode, const 
 // every lane on c3, 12-32 are within 1%).
 constexpr int kLeafWaitLanes = 24;
 
-// At most 64 VGPRs, so that eight waves fit on each SIMD.
-template <bool ANY, int FETCH>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8))) void k_wf_trace(WfParams P, uint32_t depth) {
+// W: tree width (4: `nodes`, 8: `nodes8`).  At most 64 VGPRs for the
+// 4-wide tree, so that eight waves fit on each SIMD.
+template <bool ANY, int FETCH, int W>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W == 8 ? (ANY ? 6 : 4) : 8))) void k_wf_trace(WfParams P, uint32_t depth) {
     const uint32_t md = P.settings.maxDepth;
     const uint32_t n = ANY ? P.B.counts[P.B.shadowRayBase + depth] : stage_items(P, depth);
     uint32_t* counter = &P.B.counts[P.B.fetchBase + (ANY ? md : 0u) + depth];
@@ -326,12 +342,12 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8))) void k_
         uint32_t leaf = kEmpty;
         for (;;) {
             if (cur != kEmpty && !(cur & kLeafBit)) {
-                const float4* nd = S.nodes + static_cast<size_t>(cur) * 4u;
-                float tn[4];
-                uint32_t w[4];
-                const uint32_t mask = node_boxes(nd, r, ANY ? tmax : best.t, tn, w);
+                const float4* nd = node_at<W>(S, cur);
+                float tn[W];
+                uint32_t w[W];
+                const uint32_t mask = node_test<W>(nd, r, ANY ? tmax : best.t, tn, w);
                 if (mask) {
-                    cur = ring_push_sorted<!ANY>(mask, tn, w, lane, ovNode, ovT, sp, base);
+                    cur = ring_push_sorted<!ANY, W>(mask, tn, w, lane, ovNode, ovT, sp, base);
                 } else {
                     cur = ring_pop<ANY>(best.t, lane, ovNode, ovT, sp, base);
                 }
@@ -388,6 +404,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8))) void k_
 // each lane still meets every triangle whose box its ray enters with an
 // entry distance up to its closest hit, and the closest-hit rule (smallest
 // t, then smallest primitive index) does not depend on the order.
+template <int W>
 __global__ __launch_bounds__(64) void k_wf_primary(WfParams P) {
     __shared__ uint32_t wstack[kStackSize];
     const DevScene& S = P.scene;
@@ -415,32 +432,41 @@ __global__ __launch_bounds__(64) void k_wf_primary(WfParams P) {
                 if (cur & kLeafBit) {
                     if (valid) leaf_closest(S, r, cur & ~kLeafBit, best, cnt);
                 } else {
-                    const float4* nd = S.nodes + static_cast<size_t>(cur) * 4u;
-                    float tn[4] = {kInf, kInf, kInf, kInf};
-                    uint32_t wd[4] = {kEmpty, kEmpty, kEmpty, kEmpty};
-                    const uint32_t mask = valid ? node_boxes(nd, r, best.t, tn, wd) : 0u;
+                    const float4* nd = node_at<W>(S, cur);
+                    float tn[W];
+                    uint32_t wd[W];
+#pragma unroll
+                    for (int k = 0; k < W; ++k) {
+                        tn[k] = kInf;
+                        wd[k] = kEmpty;
+                    }
+                    const uint32_t mask = valid ? node_test<W>(nd, r, best.t, tn, wd) : 0u;
                     const unsigned long long hitters = __ballot(mask != 0u);
                     if (hitters != 0ull) {
                         uint32_t any = 0u;
-                        for (uint32_t k = 0; k < 4u; ++k)
+                        for (uint32_t k = 0; k < static_cast<uint32_t>(W); ++k)
                             if (__ballot((mask >> k) & 1u)) any |= 1u << k;
                         // Children ordered by the first hitting lane's entry
                         // distances; children only other lanes hit follow
                         // (key FLT_MAX), unhit ones sort last (key +inf).
                         const int leader = __ffsll(static_cast<long long>(hitters)) - 1;
                         const uint32_t lm = __shfl(mask, leader);
-                        float key[4];
-                        uint32_t val[4];
-                        for (int k = 0; k < 4; ++k) {
+                        float key[W];
+                        uint32_t val[W];
+                        for (int k = 0; k < W; ++k) {
                             const float lt = __shfl(tn[k], leader);
                             key[k] = (lm >> k) & 1u ? lt : ((any >> k) & 1u ? kFltMax : kInf);
                             val[k] = __shfl(wd[k], leader);
                         }
-                        cswap(key[0], val[0], key[1], val[1]);
-                        cswap(key[2], val[2], key[3], val[3]);
-                        cswap(key[0], val[0], key[2], val[2]);
-                        cswap(key[1], val[1], key[3], val[3]);
-                        cswap(key[1], val[1], key[2], val[2]);
+                        if constexpr (W == 8) {
+                            sort8(key, val);
+                        } else {
+                            cswap(key[0], val[0], key[1], val[1]);
+                            cswap(key[2], val[2], key[3], val[3]);
+                            cswap(key[0], val[0], key[2], val[2]);
+                            cswap(key[1], val[1], key[3], val[3]);
+                            cswap(key[1], val[1], key[2], val[2]);
+                        }
                         const int n = __popc(any);
                         for (int k = n - 1; k >= 1; --k) {
                             if (sp < kStackSize) wstack[sp] = val[k];
@@ -749,6 +775,20 @@ __global__ __launch_bounds__(64) void k_wf_accum(WfParams P) {
     P.frame.depth[pix] = depth;
 }
 
+// Launches the traversal kernel for the fetch mode and tree width.
+template <bool ANY>
+void launch_trace(const WfParams& P, const WfLaunch& L, uint32_t d, uint32_t blocks, uint32_t tblocks, hipStream_t s) {
+    const bool w8 = L.bvhWidth == 8u && P.scene.nodes8 != nullptr;
+    switch (L.fetchMode * 2u + (w8 ? 1u : 0u)) {
+        case 5: hipLaunchKernelGGL((k_wf_trace<ANY, 2, 8>), dim3(tblocks), dim3(64), 0, s, P, d); break;
+        case 4: hipLaunchKernelGGL((k_wf_trace<ANY, 2, 4>), dim3(tblocks), dim3(64), 0, s, P, d); break;
+        case 3: hipLaunchKernelGGL((k_wf_trace<ANY, 1, 8>), dim3(blocks), dim3(64), 0, s, P, d); break;
+        case 2: hipLaunchKernelGGL((k_wf_trace<ANY, 1, 4>), dim3(blocks), dim3(64), 0, s, P, d); break;
+        case 1: hipLaunchKernelGGL((k_wf_trace<ANY, 0, 8>), dim3(blocks), dim3(64), 0, s, P, d); break;
+        default: hipLaunchKernelGGL((k_wf_trace<ANY, 0, 4>), dim3(blocks), dim3(64), 0, s, P, d); break;
+    }
+}
+
 hipError_t launch_wavefront_pass(const WfParams& P, const WfLaunch& L, hipStream_t s, StageMarks* marks) {
     const uint32_t total = P.sliceCount * P.tiles * 64u;
     if (total == 0) return hipSuccess;
@@ -761,13 +801,13 @@ hipError_t launch_wavefront_pass(const WfParams& P, const WfLaunch& L, hipStream
     const uint32_t tblocks = fullBlocks < L.traceBlocks ? fullBlocks : L.traceBlocks;
     for (uint32_t d = 0; d < P.settings.maxDepth; ++d) {
         if (d == 0u && L.primaryPackets) {
-            hipLaunchKernelGGL(k_wf_primary, dim3(blocks), dim3(64), 0, s, P);
-        } else {
-            switch (L.fetchMode) {
-                case 2: hipLaunchKernelGGL((k_wf_trace<false, 2>), dim3(tblocks), dim3(64), 0, s, P, d); break;
-                case 1: hipLaunchKernelGGL((k_wf_trace<false, 1>), dim3(blocks), dim3(64), 0, s, P, d); break;
-                default: hipLaunchKernelGGL((k_wf_trace<false, 0>), dim3(blocks), dim3(64), 0, s, P, d); break;
+            if (L.bvhWidth == 8u && P.scene.nodes8 != nullptr) {
+                hipLaunchKernelGGL(k_wf_primary<8>, dim3(blocks), dim3(64), 0, s, P);
+            } else {
+                hipLaunchKernelGGL(k_wf_primary<4>, dim3(blocks), dim3(64), 0, s, P);
             }
+        } else {
+            launch_trace<false>(P, L, d, blocks, tblocks, s);
         }
         m.mark(SHIRO_HIP_STAGE_EXTEND, s);
         if (d > 0u && P.B.shadeOrder) {
@@ -782,11 +822,7 @@ hipError_t launch_wavefront_pass(const WfParams& P, const WfLaunch& L, hipStream
             hipLaunchKernelGGL(k_wf_shade<false>, dim3(blocks), dim3(64), 0, s, P, d);
         }
         m.mark(SHIRO_HIP_STAGE_SHADE, s);
-        switch (L.fetchMode) {
-            case 2: hipLaunchKernelGGL((k_wf_trace<true, 2>), dim3(tblocks), dim3(64), 0, s, P, d); break;
-            case 1: hipLaunchKernelGGL((k_wf_trace<true, 1>), dim3(blocks), dim3(64), 0, s, P, d); break;
-            default: hipLaunchKernelGGL((k_wf_trace<true, 0>), dim3(blocks), dim3(64), 0, s, P, d); break;
-        }
+        launch_trace<true>(P, L, d, blocks, tblocks, s);
         m.mark(SHIRO_HIP_STAGE_SHADOW, s);
         hipLaunchKernelGGL(k_wf_resolve, dim3(blocks), dim3(64), 0, s, P, d);
         m.mark(SHIRO_HIP_STAGE_RESOLVE, s);

@@ -418,7 +418,8 @@ class Context:
 
     def autotune(self, camera, settings, trial_samples=2):
         """Chooses paths per pass, traversal fetch mode and grid for this
-        frame; returns (tuning, report)."""
+        frame, then tries the best with the other tree width; returns
+        (tuning, report)."""
         t = abi.Tuning()
         buf = C.create_string_buffer(8192)
         cam, st = camera.to_c(), settings.to_c()

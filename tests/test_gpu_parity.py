@@ -377,10 +377,36 @@ def test_shade_order_identical(hip_ctx, make):
             hip_ctx.set_tuning(shadeOrder=order, pathsPerPass=1 << 16)
             frames.append(hip_ctx.render(wl.camera, st))
     finally:
-        hip_ctx.set_tuning(**{k: getattr(saved, k) for k, _ in abi.Tuning._fields_ if k != "reserved"})
+        hip_ctx.set_tuning(**{k: getattr(saved, k) for k, _ in abi.Tuning._fields_})
     for k in ("beauty", "albedo", "normal", "depth"):
         assert np.array_equal(getattr(frames[0], k), getattr(frames[1], k)), k
         assert np.array_equal(getattr(frames[1], k), getattr(mk, k)), k
+
+
+@pytest.mark.parametrize("make", [c2_mini, c3_mini, emissive_mini], ids=["c2", "c3", "emissive"])
+def test_bvh_width_identical(hip_ctx, make):
+    """The wavefront traversal over the 8-wide tree (same leaves, same
+    watertight test, same closest-hit rule) renders the frames of the 4-wide
+    megakernel bit for bit, with every fetch mode, with and without
+    camera-ray packets."""
+    from shiro_amd import abi
+    wl = make()
+    _log(f"bvh width 8 {wl.name}")
+    hip_ctx.upload(wl.scene)
+    info = hip_ctx.scene_info()
+    assert info.bvh8NodeCount > 0 and info.bvh8Depth <= info.bvhDepth
+    mk = hip_ctx.render(wl.camera, wl.settings)
+    st = RenderSettings(**{**wl.settings.__dict__, "transport": abi.TRANSPORT_WAVEFRONT})
+    saved = hip_ctx.get_tuning()
+    try:
+        for mode in (0, 1, 2):
+            for packets in (0, 1):
+                hip_ctx.set_tuning(bvhWidth=8, fetchMode=mode, primaryPackets=packets, pathsPerPass=1 << 16)
+                wf = hip_ctx.render(wl.camera, st)
+                for k in ("beauty", "albedo", "normal", "depth"):
+                    assert np.array_equal(getattr(wf, k), getattr(mk, k)), (mode, packets, k)
+    finally:
+        hip_ctx.set_tuning(**{k: getattr(saved, k) for k, _ in abi.Tuning._fields_})
 
 
 def test_autotune_keeps_result(hip_ctx):
@@ -393,6 +419,7 @@ def test_autotune_keeps_result(hip_ctx):
     t, report = hip_ctx.autotune(wl.camera, st, 2)
     _log(report)
     assert report.count("ms") >= 10 and t.fetchMode in (0, 1, 2)
+    assert "bvhWidth 8" in report and t.bvhWidth in (4, 8)
     out = hip_ctx.render(wl.camera, st)
-    hip_ctx.set_tuning(**{k: getattr(saved, k) for k in ("pathsPerPass", "fetchMode", "traceBlocks", "stageBlocks")})
+    hip_ctx.set_tuning(**{k: getattr(saved, k) for k, _ in abi.Tuning._fields_})
     assert np.array_equal(out.beauty, ref.beauty)
