@@ -250,21 +250,107 @@ uint32_t node_words(const float* nd, uint32_t* words) {
     return ex >> 24;
 }
 
-// Collapses the binary tree into W-wide nodes: each node adopts its two
-// children and repeatedly opens the internal child of largest surface area
-// until it has W.  Leaves are the binary tree's, so every width shares the
-// triangle order.
+// Collapses the binary tree into W-wide nodes over the binary tree's leaves
+// (so every width shares the triangle order).  Which binary nodes become
+// wide nodes is chosen by dynamic programming over the binary tree (the
+// surface-area cost of Ylitie, Karras and Laine, "Efficient incoherent ray
+// traversal on GPUs through compressed wide BVHs", HPG 2017, with the
+// leaves fixed): with every leaf kept, the expected traversal cost of a
+// random ray is, up to a constant, the sum of the surface areas of the wide
+// nodes, so for binary node n and i = 1..W
+//   cost(n, i) = least area sum of representing n's subtree by at most i
+//                roots (a leaf is one root of cost 0),
+//   split(n, j) = min over k of cost(left, k) + cost(right, j - k),
+//   cost(n, 1) = A(n) + split(n, W)           (n becomes one wide node),
+//   cost(n, i) = min(cost(n, 1), split(n, i)) for i >= 2,
+// and the root is cost(root, 1).
 template <int W>
 struct Collapser {
     const Builder& B;
     std::vector<float>* nodes;
     Bvh* out;              // leaf statistics (filled by the 4-wide pass only)
     float slack = 0.0f;    // absolute widening of every box (scene-scale rounding margin)
+    // Per binary node and slot count i (index i - 1): the cost, and how it
+    // is reached: 0 = the node itself is one root, k > 0 = k slots to the
+    // left child and i - k to the right.
+    std::vector<float> cost;
+    std::vector<uint8_t> how;
 
     uint32_t alloc() {
         const uint32_t id = static_cast<uint32_t>(nodes->size() / NodeLayout<W>::kFloats);
         nodes->resize(nodes->size() + NodeLayout<W>::kFloats, 0.0f);
         return id;
+    }
+
+    void plan() {
+        const uint32_t nb = B.nodeCount.load();
+        cost.assign(static_cast<size_t>(nb) * W, 0.0f);
+        how.assign(static_cast<size_t>(nb) * W, 0);
+        // Children have larger indices than their parent.
+        for (uint32_t n = nb; n-- > 0;) {
+            const Node2& nd = B.nodes[n];
+            float* c = &cost[static_cast<size_t>(n) * W];
+            uint8_t* h = &how[static_cast<size_t>(n) * W];
+            if (nd.count > 0) continue;  // a leaf: cost 0 with any slot count
+            const float* cl = &cost[static_cast<size_t>(nd.left) * W];
+            const float* cr = &cost[static_cast<size_t>(nd.left + 1) * W];
+            float split[W + 1];
+            uint8_t arg[W + 1];
+            for (int j = 2; j <= W; ++j) {
+                split[j] = FLT_MAX;
+                arg[j] = 1;
+                for (int k = 1; k < j; ++k) {
+                    const float v = cl[k - 1] + cr[j - k - 1];
+                    if (v < split[j]) {
+                        split[j] = v;
+                        arg[j] = static_cast<uint8_t>(k);
+                    }
+                }
+            }
+            c[0] = nd.box.area() + split[W];
+            h[0] = 0;
+            for (int i = 2; i <= W; ++i) {
+                if (split[i] < c[0]) {
+                    c[i - 1] = split[i];
+                    h[i - 1] = arg[i];
+                } else {
+                    c[i - 1] = c[0];
+                    h[i - 1] = 0;
+                }
+            }
+        }
+    }
+
+    // Roots representing binary node m with i slots.
+    void gather(uint32_t m, int i, uint32_t* kids, int* nk) const {
+        const Node2& nd = B.nodes[m];
+        const uint8_t h = how[static_cast<size_t>(m) * W + (i - 1)];
+        if (nd.count > 0 || h == 0) {
+            kids[(*nk)++] = m;
+            return;
+        }
+        gather(nd.left, h, kids, nk);
+        gather(nd.left + 1, i - h, kids, nk);
+    }
+
+    // Children of the wide node made from internal binary node bi.
+    int children(uint32_t bi, uint32_t* kids) const {
+        const Node2& nd = B.nodes[bi];
+        const float* cl = &cost[static_cast<size_t>(nd.left) * W];
+        const float* cr = &cost[static_cast<size_t>(nd.left + 1) * W];
+        int bestK = 1;
+        float best = FLT_MAX;
+        for (int k = 1; k < W; ++k) {
+            const float v = cl[k - 1] + cr[W - k - 1];
+            if (v < best) {
+                best = v;
+                bestK = k;
+            }
+        }
+        int nk = 0;
+        gather(nd.left, bestK, kids, &nk);
+        gather(nd.left + 1, W - bestK, kids, &nk);
+        return nk;
     }
 
     // Fills node `id` from binary node `bi`; returns the subtree depth and,
@@ -273,27 +359,10 @@ struct Collapser {
     uint32_t emit(uint32_t id, uint32_t bi, uint32_t* stack) {
         uint32_t kids[W];
         int nk = 0;
-        const Node2& n = B.nodes[bi];
-        if (n.count > 0) {
+        if (B.nodes[bi].count > 0) {
             kids[nk++] = bi;  // a leaf root: one leaf child
         } else {
-            kids[nk++] = n.left;
-            kids[nk++] = n.left + 1;
-            while (nk < W) {
-                int best = -1;
-                float bestArea = -1.0f;
-                for (int k = 0; k < nk; ++k) {
-                    const Node2& c = B.nodes[kids[k]];
-                    if (c.count == 0 && c.box.area() > bestArea) {
-                        bestArea = c.box.area();
-                        best = k;
-                    }
-                }
-                if (best < 0) break;
-                const uint32_t open = kids[best];
-                kids[best] = B.nodes[open].left;
-                kids[nk++] = B.nodes[open].left + 1;
-            }
+            nk = children(bi, kids);
         }
         // Allocate the internal children first so they are contiguous.
         uint32_t childId[W];
@@ -375,6 +444,7 @@ bool build_bvh(const float* tris, size_t n, uint32_t maxDepth, Bvh* out, std::st
         scale = std::max(scale, std::max(std::fabs(B.nodes[0].box.lo[a]), std::fabs(B.nodes[0].box.hi[a])));
     const float slack = scale * 16.0f * FLT_EPSILON;
     Collapser<4> C{B, &out->nodes, out, slack};
+    C.plan();
     out->nodes.reserve(kNodeFloats * (n / 2 + 4));
     out->depth = C.emit(C.alloc(), 0, &out->stack);
     if (out->depth > maxDepth) {
@@ -382,7 +452,10 @@ bool build_bvh(const float* tris, size_t n, uint32_t maxDepth, Bvh* out, std::st
                std::to_string(maxDepth);
         return false;
     }
+    C.cost = std::vector<float>();
+    C.how = std::vector<uint8_t>();
     Collapser<8> C8{B, &out->nodes8, out, slack};
+    C8.plan();
     out->depth8 = C8.emit(C8.alloc(), 0, &out->stack8);
     if (out->stack8 + 1 > static_cast<uint32_t>(kStackSize)) {
         out->nodes8.clear();

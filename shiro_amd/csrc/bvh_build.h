@@ -27,9 +27,10 @@ struct Bvh {
 // Binned SAH (32 bins) on centroids builds a binary tree with leaves of at
 // most 4 triangles (more only where centroids coincide or at the depth
 // limit); subtrees above 32K triangles are built on worker threads.  The
-// binary tree is then collapsed twice: each W-wide node (W = 4, 8) adopts
-// its two children and repeatedly opens the internal child of largest
-// surface area until it has W.  Boxes are padded by a few ulps (and by a
+// binary tree is then collapsed twice, into W-wide nodes (W = 4, 8) over its
+// leaves, choosing which binary nodes become wide nodes by dynamic
+// programming to the least sum of wide-node surface areas (the SAH cost
+// with the leaves fixed).  Boxes are padded by a few ulps (and by a
 // rounding margin of the scene's coordinate scale) so that the device slab
 // test is conservative, then quantised to 8 bits per plane relative to their
 // parent node.  Returns false (with a message) if the 4-wide tree is deeper
