@@ -356,7 +356,8 @@ int shiro_hip_update_texture(
  * MIS weights are constants), so this is the exact gradient of the frozen-
  * sampling render (SHIRO_HIP_TEXELS_EVAL_ONLY) at the current texels.
  * Textured materials must have no specular anisotropy, thin film, sheen or
- * transmission.  Synchronous. */
+ * transmission; textures with gradients hold fewer than 2^27 texels.
+ * Synchronous. */
 int shiro_hip_render_adjoint(
     shiro_hip_ctx* ctx, const shiro_hip_camera* camera,
     const shiro_hip_settings* settings, uint32_t sampleStart,
@@ -368,7 +369,8 @@ int shiro_hip_render_adjoint(
  * the radiance escaped rays and dome light samples read (bilinear lookups);
  * their importance tables are sampling and are detached, so this is the
  * gradient of the render whose tables stay those of the current texels
- * (shiro_hip_update_environment with SHIRO_HIP_TEXELS_EVAL_ONLY). */
+ * (shiro_hip_update_environment with SHIRO_HIP_TEXELS_EVAL_ONLY).  Maps with
+ * gradients hold fewer than 2^27 texels. */
 int shiro_hip_render_adjoint_env(
     shiro_hip_ctx* ctx, const shiro_hip_camera* camera,
     const shiro_hip_settings* settings, uint32_t sampleStart,

@@ -82,53 +82,117 @@ DEV V3 roulette_vjp(V3 y, V3 g) {
     return r;
 }
 
-DEV void scatter(float* grad, const DevTexture& t, V2 uv, const float* g, uint32_t n) {
+// Per-wave sums of texel gradients in LDS.  A wave's dome light samples are
+// importance-sampled onto the same few bright texels, the escaped rays of
+// its 8x8 tile read neighbouring map texels and its hits neighbouring
+// texture texels, so adding them in LDS first leaves one global atomic per
+// distinct texel and channel instead of one per lookup, which contended on
+// the hot texels.  Open addressing over kGradCacheSlots keys (array <<
+// kTexelBits | texel), where array < kMaxDomes names dome `array`'s map (3
+// floats per texel) and kMaxDomes + t texture t (its channel count per
+// texel); a lookup that finds no slot in kGradCacheProbes probes adds to
+// global memory directly.
+constexpr uint32_t kGradCacheSlots = 256;
+constexpr int kGradCacheProbes = 4;
+constexpr uint32_t kTexelBits = 27;
+
+struct GradCache {
+    uint32_t key[kGradCacheSlots];
+    float val[kGradCacheSlots * 3];
+};
+
+// Every lane of the block (one wave) calls this before any adds.
+DEV void cache_clear(GradCache& c) {
+    for (uint32_t s = threadIdx.x; s < kGradCacheSlots; s += 64u) {
+        c.key[s] = kEmpty;
+        c.val[3u * s] = c.val[3u * s + 1u] = c.val[3u * s + 2u] = 0.0f;
+    }
+    __syncthreads();
+}
+
+// Adds channels 0..n-1 of v to texel `texel` of gradient array `array`.
+DEV void add_global(const AdjointParams& P, uint32_t array, uint32_t texel, V3 v, uint32_t n) {
+    float* q;
+    if (array < static_cast<uint32_t>(kMaxDomes)) {
+        q = P.envGrad[array] + static_cast<size_t>(texel) * 3u;
+    } else {
+        const uint32_t t = array - kMaxDomes;
+        q = P.grad[t] + static_cast<size_t>(texel) * P.scene.textures[t].channels;
+    }
+    if (v.x != 0.0f) atomicAdd(q, v.x);
+    if (n > 1u && v.y != 0.0f) atomicAdd(q + 1, v.y);
+    if (n > 2u && v.z != 0.0f) atomicAdd(q + 2, v.z);
+}
+
+DEV void cache_add(GradCache& c, const AdjointParams& P, uint32_t array, uint32_t texel, V3 v, uint32_t n) {
+    const uint32_t key = (array << kTexelBits) | texel;
+    uint32_t slot = (key * 2654435761u) >> 24;  // 8-bit multiplicative hash
+    for (int probe = 0; probe < kGradCacheProbes; ++probe) {
+        const uint32_t prev = atomicCAS(&c.key[slot], kEmpty, key);
+        if (prev == kEmpty || prev == key) {
+            float* q = &c.val[3u * slot];
+            if (v.x != 0.0f) atomicAdd(q, v.x);
+            if (n > 1u && v.y != 0.0f) atomicAdd(q + 1, v.y);
+            if (n > 2u && v.z != 0.0f) atomicAdd(q + 2, v.z);
+            return;
+        }
+        slot = (slot + 1u) & (kGradCacheSlots - 1u);
+    }
+    add_global(P, array, texel, v, n);
+}
+
+// Every lane of the block calls this after its last add.
+DEV void cache_flush(GradCache& c, const AdjointParams& P) {
+    __syncthreads();
+    for (uint32_t s = threadIdx.x; s < kGradCacheSlots; s += 64u) {
+        const uint32_t key = c.key[s];
+        if (key == kEmpty) continue;
+        add_global(P, key >> kTexelBits, key & ((1u << kTexelBits) - 1u),
+                   V3{c.val[3u * s], c.val[3u * s + 1u], c.val[3u * s + 2u]}, 3u);
+    }
+}
+
+// Adds g[0..n-1] per unit of the bilinear lookup of texture `index` at uv
+// to its texel gradients.
+DEV void scatter(const AdjointParams& P, GradCache& cache, uint32_t index, V2 uv, V3 g, uint32_t n) {
+    const DevTexture& t = P.scene.textures[index];
     const TexTap tap = tex_tap(t.w, t.h, uv);
     for (int k = 0; k < 4; ++k) {
         if (tap.w[k] == 0.0f) continue;
-        for (uint32_t c = 0; c < n; ++c)
-            if (g[c] != 0.0f) atomicAdd(grad + static_cast<size_t>(tap.i[k]) * t.channels + c, g[c] * tap.w[k]);
+        cache_add(cache, P, kMaxDomes + index, tap.i[k], g * tap.w[k], n);
     }
 }
 
 // Adds dLoss/dbaseColor and dLoss/droughness at a hit to the texel
 // gradients of the material's textures.
-DEV void scatter_grad(const AdjointParams& P, uint32_t material, V2 uv, V3 gBase, float gRough) {
+DEV void scatter_grad(const AdjointParams& P, GradCache& cache, uint32_t material, V2 uv, V3 gBase, float gRough) {
     const DevScene& S = P.scene;
     const shiro_hip_material& m = S.materials[material];
     if (m.baseColorTexture >= 0 && P.grad[m.baseColorTexture]) {
-        const DevTexture& t = S.textures[m.baseColorTexture];
-        if (t.channels >= 3u) {
-            const float g[3] = {gBase.x, gBase.y, gBase.z};
-            scatter(P.grad[m.baseColorTexture], t, uv, g, 3u);
+        const uint32_t t = static_cast<uint32_t>(m.baseColorTexture);
+        if (S.textures[t].channels >= 3u) {
+            scatter(P, cache, t, uv, gBase, 3u);
         } else {
-            const float g[1] = {gBase.x + gBase.y + gBase.z};
-            scatter(P.grad[m.baseColorTexture], t, uv, g, 1u);
+            scatter(P, cache, t, uv, V3{gBase.x + gBase.y + gBase.z, 0.0f, 0.0f}, 1u);
         }
     }
-    if (m.roughnessTexture >= 0 && P.grad[m.roughnessTexture]) {
-        const float g[1] = {gRough};
-        scatter(P.grad[m.roughnessTexture], S.textures[m.roughnessTexture], uv, g, 1u);
-    }
+    if (m.roughnessTexture >= 0 && P.grad[m.roughnessTexture])
+        scatter(P, cache, static_cast<uint32_t>(m.roughnessTexture), uv, V3{gRough, 0.0f, 0.0f}, 1u);
 }
 
 // Adds g (per channel, per unit of the map's radiance at map-space
 // direction `local`) to the texel gradients of dome i's map through the
 // bilinear footprint of the lookup.
-DEV void scatter_env(const AdjointParams& P, uint32_t i, V3 local, V3 g) {
+DEV void scatter_env(const AdjointParams& P, GradCache& cache, uint32_t i, V3 local, V3 g) {
     const DevScene& S = P.scene;
     const DevDome& d = S.domes[i];
     if (d.env < 0 || !P.envGrad[i]) return;
     EnvTaps t;
     if (!env_taps(S.envs[d.env], local, &t)) return;
     const V3 gr = g * ld3(d.radiance);
-    float* out = P.envGrad[i];
     for (int k = 0; k < 4; ++k) {
         if (t.w[k] == 0.0f) continue;
-        float* q = out + static_cast<size_t>(t.idx[k]) * 3u;
-        if (gr.x != 0.0f) atomicAdd(q, gr.x * t.w[k]);
-        if (gr.y != 0.0f) atomicAdd(q + 1, gr.y * t.w[k]);
-        if (gr.z != 0.0f) atomicAdd(q + 2, gr.z * t.w[k]);
+        cache_add(cache, P, i, t.idx[k], gr * t.w[k], 3u);
     }
 }
 
@@ -138,21 +202,14 @@ DEV V3 ratio(V3 a, V3 b) {
 
 }  // namespace
 
-// Grid: one 64-thread block per (8x8 tile, sample); tile-major within a
-// sample.  MAXD bounds the path length, sizing the per-thread vertex tape.
+// Replays the path of pixel (x, y), sample s with pixel adjoint a (the
+// method described at the top of this file): adds its light-sample and
+// texel gradients, and its dome-map ones to `cache`.  MAXD bounds the path
+// length, sizing the vertex tape.
 template <uint32_t MAXD>
-__global__ __launch_bounds__(64) void k_adjoint(AdjointParams P) {
-    const uint32_t tile = blockIdx.x % P.tiles;
-    const uint32_t s = blockIdx.x / P.tiles;
-    const uint32_t x = (tile % P.tilesX) * 8u + (threadIdx.x & 7u);
-    const uint32_t y = (tile / P.tilesX) * 8u + (threadIdx.x >> 3);
+DEV void adjoint_path(const AdjointParams& P, GradCache& cache, uint32_t x, uint32_t y, uint32_t s, V3 a) {
     const DevSettings& st = P.settings;
     const DevScene& S = P.scene;
-    if (x >= st.width || y >= st.height || s >= P.sampleCount) return;
-    const float4 db = P.dBeauty[static_cast<size_t>(y) * st.width + x];
-    const float inv = 1.0f / static_cast<float>(P.sampleCount);
-    const V3 a{db.x * inv, db.y * inv, db.z * inv};
-    if (a.x == 0.0f && a.y == 0.0f && a.z == 0.0f) return;
     const Sampler smp{x, y, P.sampleStart + s};
     PathState ps = init_path(P.camera, st, x, y, smp);
     const uint32_t nLight = direct_sample_count(S, st);
@@ -173,7 +230,8 @@ __global__ __launch_bounds__(64) void k_adjoint(AdjointParams P) {
             // The escaped ray adds T * mis * sum_i env_i(dir) * radiance_i.
             if (P.envWanted && mis != 0.0f) {
                 const V3 g = a * V.T * mis;
-                for (uint32_t i = 0; i < S.domeCount; ++i) scatter_env(P, i, world_to_dome(S.domes[i], ps.d), g);
+                for (uint32_t i = 0; i < S.domeCount; ++i)
+                    scatter_env(P, cache, i, world_to_dome(S.domes[i], ps.d), g);
             }
             break;
         }
@@ -218,13 +276,13 @@ __global__ __launch_bounds__(64) void k_adjoint(AdjointParams P) {
                     gRough += dot(gf, bg.dRough);
                 }
                 // lf = env(local) * radiance * s for the sample of dome group - 1.
-                if (P.envWanted && group > 0u) scatter_env(P, group - 1u, dome.local, gy * f * dome.s);
+                if (P.envWanted && group > 0u) scatter_env(P, cache, group - 1u, dome.local, gy * f * dome.s);
             }
         }
         ds.flush();
         ps.L = ps.L + ps.T * ds.L;
         V.C = V.C + ds.L;
-        if (textured) scatter_grad(P, sf.material, sf.uv, gBase, gRough);
+        if (textured) scatter_grad(P, cache, sf.material, sf.uv, gBase, gRough);
         BounceRec br;
         const bool alive = sample_bounce(S, st, depth, smp, sf, ps, &br);
         if (!alive) {
@@ -257,7 +315,7 @@ __global__ __launch_bounds__(64) void k_adjoint(AdjointParams P) {
             gT = gy * V.f * V.c;
             if (V.flags & 2u) {
                 const V3 gf = gy * V.T * V.c;
-                scatter_grad(P, V.material, V.uv, gf * V.dfa, dot(gf, V.dfr));
+                scatter_grad(P, cache, V.material, V.uv, gf * V.dfa, dot(gf, V.dfr));
             }
         } else if (V.kind == kVtxPass) {
             gT = g;
@@ -265,6 +323,28 @@ __global__ __launch_bounds__(64) void k_adjoint(AdjointParams P) {
         gT = gT + a * V.C;
         g = gT * V.att;
     }
+}
+
+// Grid: one 64-thread block per (8x8 tile, sample); tile-major within a
+// sample.  MAXD bounds the path length, sizing the per-thread vertex tape.
+// At most 128 VGPRs (4 waves per SIMD): the replay spills some registers
+// there, and still runs faster than at 2 or 3 waves per SIMD.
+template <uint32_t MAXD>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_adjoint(AdjointParams P) {
+    __shared__ GradCache cache;
+    cache_clear(cache);
+    const uint32_t tile = blockIdx.x % P.tiles;
+    const uint32_t s = blockIdx.x / P.tiles;
+    const uint32_t x = (tile % P.tilesX) * 8u + (threadIdx.x & 7u);
+    const uint32_t y = (tile / P.tilesX) * 8u + (threadIdx.x >> 3);
+    const DevSettings& st = P.settings;
+    if (x < st.width && y < st.height && s < P.sampleCount) {
+        const float4 db = P.dBeauty[static_cast<size_t>(y) * st.width + x];
+        const float inv = 1.0f / static_cast<float>(P.sampleCount);
+        const V3 a{db.x * inv, db.y * inv, db.z * inv};
+        if (a.x != 0.0f || a.y != 0.0f || a.z != 0.0f) adjoint_path<MAXD>(P, cache, x, y, s, a);
+    }
+    cache_flush(cache, P);
 }
 
 hipError_t launch_adjoint(const AdjointParams& P, hipStream_t s) {

@@ -1251,9 +1251,16 @@ int render_adjoint(shiro_hip_ctx* ctx, const shiro_hip_camera* camera, const shi
                                                 " textures");
     if (dEnvCount > ctx->dev.domeCount || (dEnvCount > 0 && !dEnv))
         return fail(ctx, SHIRO_HIP_INVALID, "more environment gradients than dome lights");
-    for (uint32_t i = 0; i < dEnvCount; ++i)
+    for (uint32_t i = 0; i < dEnvCount; ++i) {
         if (dEnv[i] && ctx->dev.domes[i].env < 0)
             return fail(ctx, SHIRO_HIP_INVALID, "environment gradient for dome " + std::to_string(i) + " without a map");
+        if (dEnv[i] && static_cast<uint64_t>(ctx->env[ctx->dev.domes[i].env].w) * ctx->env[ctx->dev.domes[i].env].h >=
+                           (1ull << 27))
+            return fail(ctx, SHIRO_HIP_INVALID, "environment gradient of a map with 2^27 texels or more");
+    }
+    for (uint32_t i = 0; i < dTexelsCount; ++i)
+        if (dTexels[i] && static_cast<uint64_t>(ctx->tex[i]->w) * ctx->tex[i]->h >= (1ull << 27))
+            return fail(ctx, SHIRO_HIP_INVALID, "texel gradient of a texture with 2^27 texels or more");
     for (size_t i = 0; i < ctx->mats.size(); ++i) {
         const shiro_hip_material& m = ctx->mats[i];
         if (m.baseColorTexture < 0 && m.roughnessTexture < 0) continue;

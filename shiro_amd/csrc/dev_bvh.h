@@ -385,6 +385,88 @@ DEV HitRec intersect_closest(const DevScene& S, const Ray& ray, const TravCount<
     return best;
 }
 
+// Closest hit of nearly parallel rays as a packet: every active lane of
+// the wave calls this together (lanes with valid = false only take part in
+// the wave's decisions).  The wave holds one stack (`wstack`, kStackSize
+// entries in LDS) and one current node; every lane tests its own ray
+// against the node's boxes (culled by its own closest hit), the wave
+// descends into the children any lane hits, nearest first by the entry
+// distances of the first lane that hits one, and at a leaf every lane tests
+// the leaf's triangles.  Camera rays of an 8x8 tile barely diverge, so the
+// union of their paths through the tree is barely larger than one ray's and
+// every node test keeps all lanes busy, where the per-lane traversal loses
+// most lanes to divergence.  Results equal intersect_closest's: each lane
+// still meets every triangle whose box its ray enters with an entry
+// distance up to its closest hit, and the closest-hit rule (smallest t,
+// then smallest primitive index) does not depend on the order.
+template <int W>
+DEV HitRec intersect_closest_packet(const DevScene& S, const Ray& ray, bool valid, uint32_t* wstack) {
+    HitRec best;
+    best.t = valid ? ray.tmax : kFltMax;
+    best.b1 = best.b2 = 0.0f;
+    best.slot = best.prim = kNoHit;
+    const TravCount<false> cnt{nullptr, nullptr};
+    RayPre r{};
+    if (valid) r = precompute(ray);
+    if (S.nodeCount == 0u || !__any(valid)) return best;
+    int sp = 0;
+    uint32_t cur = 0u;
+    for (;;) {
+        if (cur & kLeafBit) {
+            if (valid) leaf_closest(S, r, cur & ~kLeafBit, best, cnt);
+        } else {
+            const float4* nd = node_at<W>(S, cur);
+            float tn[W];
+            uint32_t wd[W];
+#pragma unroll
+            for (int k = 0; k < W; ++k) {
+                tn[k] = kInf;
+                wd[k] = kEmpty;
+            }
+            const uint32_t mask = valid ? node_test<W>(nd, r, best.t, tn, wd) : 0u;
+            const unsigned long long hitters = __ballot(mask != 0u);
+            if (hitters != 0ull) {
+                uint32_t any = 0u;
+                for (uint32_t k = 0; k < static_cast<uint32_t>(W); ++k)
+                    if (__ballot((mask >> k) & 1u)) any |= 1u << k;
+                // Children ordered by the first hitting lane's entry
+                // distances; children only other lanes hit follow (key
+                // FLT_MAX), unhit ones sort last (key +inf).
+                const int leader = __ffsll(static_cast<long long>(hitters)) - 1;
+                const uint32_t lm = __shfl(mask, leader);
+                float key[W];
+                uint32_t val[W];
+                for (int k = 0; k < W; ++k) {
+                    const float lt = __shfl(tn[k], leader);
+                    key[k] = (lm >> k) & 1u ? lt : ((any >> k) & 1u ? kFltMax : kInf);
+                    val[k] = __shfl(wd[k], leader);
+                }
+                if constexpr (W == 8) {
+                    sort8(key, val);
+                } else {
+                    cswap(key[0], val[0], key[1], val[1]);
+                    cswap(key[2], val[2], key[3], val[3]);
+                    cswap(key[0], val[0], key[2], val[2]);
+                    cswap(key[1], val[1], key[3], val[3]);
+                    cswap(key[1], val[1], key[2], val[2]);
+                }
+                const int n = __popc(any);
+                for (int k = n - 1; k >= 1; --k) {
+                    if (sp < kStackSize) wstack[sp] = val[k];
+                    ++sp;
+                }
+                cur = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(val[0])));
+                continue;
+            }
+        }
+        if (sp == 0) break;
+        --sp;
+        cur = static_cast<uint32_t>(
+            __builtin_amdgcn_readfirstlane(static_cast<int>(wstack[sp < kStackSize ? sp : kStackSize - 1])));
+    }
+    return best;
+}
+
 // Any hit in [tmin, tmax]: Occluded (CpuPathTracer.cpp:659-681, 697-709).
 // Same while-while structure and near-first child order as the closest-hit
 // traversal (an occluder near the origin is found early), no distance
