@@ -249,7 +249,11 @@ int enqueue_wavefront(shiro_hip_ctx* ctx, const DevScene& scene, const shiro_hip
         HIPC(ctx->wf[18].reserve(kShadeClasses * kOrderBlocks * 4), "alloc wavefront queues");
         HIPC(ctx->wf[19].reserve((kShadeClasses + 1u) * kOrderBlocks * 4), "alloc wavefront queues");
     }
-    const uint32_t slots = 5u * st.maxDepth + 1u;
+    const uint32_t shadowRayBase = st.maxDepth + 1u;
+    const uint32_t shadowPathBase = shadowRayBase + st.maxDepth;
+    // The striped work counters start on a 128-byte line.
+    const uint32_t fetchBase = (shadowPathBase + st.maxDepth + kPoolStride - 1u) / kPoolStride * kPoolStride;
+    const uint32_t slots = fetchBase + 2u * st.maxDepth * kPoolStripes * kPoolStride;
     HIPC(ctx->counters.reserve(std::max<size_t>(sizeof(DevCounters), slots * 4u)), "alloc counters");
     if (S < count) {
         HIPC(ctx->sumB.reserve(np * 16), "alloc sums");
@@ -282,9 +286,9 @@ int enqueue_wavefront(shiro_hip_ctx* ctx, const DevScene& scene, const shiro_hip
     B.qsr = ctx->wf[14].as<uint32_t>();
     B.svis = ctx->wf[15].as<uint32_t>();
     B.countSlots = slots;
-    B.shadowRayBase = st.maxDepth + 1u;
-    B.shadowPathBase = B.shadowRayBase + st.maxDepth;
-    B.fetchBase = B.shadowPathBase + st.maxDepth;
+    B.shadowRayBase = shadowRayBase;
+    B.shadowPathBase = shadowPathBase;
+    B.fetchBase = fetchBase;
     B.maxShadow = M;
     B.cap = static_cast<uint32_t>(cap);
     B.shadeOrder = order ? 1u : 0u;

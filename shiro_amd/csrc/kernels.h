@@ -18,6 +18,12 @@ struct AccumParams {
     uint32_t* depthCount;   // 1
 };
 
+// Striped work counters of the persistent traversal kernels (WavePool,
+// wavefront.hip): per stage kPoolStripes counters, each on its own 128-byte
+// line.
+constexpr uint32_t kPoolStripes = 16;
+constexpr uint32_t kPoolStride = 32;
+
 // Wavefront path state (structure of arrays, `cap` paths) and queues.
 struct WfBuffers {
     float4 *o, *d, *T, *L, *E, *A, *N, *H;  // ray origin|prevPdf, dir|bits, throughput, radiance,
@@ -32,7 +38,9 @@ struct WfBuffers {
     uint32_t countSlots;
     uint32_t shadowRayBase;                 // counts[shadowRayBase + d]: shadow rays at depth d
     uint32_t shadowPathBase;                // counts[shadowPathBase + d]: paths with shadow rays
-    uint32_t fetchBase;                     // counts[fetchBase + d], [+ maxDepth + d]: work fetched
+    uint32_t fetchBase;                     // work fetched by the persistent traversal kernels: kPoolStripes
+                                            // counters kPoolStride apart from counts[fetchBase + (k * maxDepth
+                                            // + d) * kPoolStripes * kPoolStride], k = 0 closest, 1 any hit
     uint32_t maxShadow;
     uint32_t cap;
     uint8_t* qkey;                          // shading class per queue position

@@ -1,6 +1,7 @@
 // Wavefront transport: the path integrator split into stage kernels over
 // path state kept in HBM (structure of arrays), with ray queues compacted by
-// wave ballots.  Per pass (a slice of samples for every pixel):
+// wave ballots and block-wide prefix sums.  Per pass (a slice of samples for
+// every pixel):
 //
 //   k_wf_reset     clear the queue counters of the pass
 //   per depth d:
@@ -19,9 +20,10 @@
 //                  order and added to the path radiance
 //
 // The traversal kernels are persistent: a fixed grid whose lanes fetch rays
-// from the queue with one wave-aggregated atomic, and a lane whose ray is
-// done fetches the next one at once instead of idling until the longest ray
-// of its wave ends (dynamic ray fetch, Aila and Laine 2009).
+// from per-wave pools of the queue (one atomic per kPoolChunk rays), and a
+// lane whose ray is done fetches the next one at once instead of idling
+// until the longest ray of its wave ends (dynamic ray fetch, Aila and Laine
+// 2009).
 //   k_wf_accum     per-pixel sums over the pass's samples, in sample order
 //
 // Every stage uses the same device functions as the megakernel
@@ -39,21 +41,6 @@
 namespace shiro_hip {
 
 DEV uint32_t lane_id() { return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u)); }
-
-// Wave-aggregated append: one atomic per wave, slots in lane order.
-DEV void wave_push(bool push, uint32_t value, uint32_t* queue, uint32_t* count) {
-    const unsigned long long mask = __ballot(push);
-    if (mask == 0ull) return;
-    const uint32_t lane = lane_id();
-    const int leader = __ffsll(static_cast<long long>(mask)) - 1;
-    uint32_t base = 0;
-    if (static_cast<int>(lane) == leader) base = atomicAdd(count, static_cast<uint32_t>(__popcll(mask)));
-    base = __shfl(base, leader);
-    if (push) {
-        const unsigned long long below = mask & ((1ull << lane) - 1ull);
-        queue[base + static_cast<uint32_t>(__popcll(below))] = value;
-    }
-}
 
 struct PathCoord {
     uint32_t x, y, s;
@@ -123,13 +110,24 @@ DEV uint32_t stage_items(const WfParams& P, uint32_t depth) {
 }
 
 // Work distribution of the persistent kernels: each wave owns a pool of
-// queue positions [next, end) taken 64 at a time from the global counter, so
-// lanes refill from the pool without a global atomic most of the time.
+// queue positions [next, end), a chunk of 64, so lanes refill from the pool
+// without a global atomic most of the time.  Chunks come from kPoolStripes
+// counters (kPoolStride words apart, one 128-byte line each), counter s
+// handing out the chunks of the s-th of kPoolStripes contiguous ranges of
+// the queue; a wave starts at stripe blockIdx % kPoolStripes and moves to
+// the next stripe when its own is used up.  With one counter for the whole
+// grid, its atomics serialised on its address (about 88M per second were
+// measured): 64-position chunks from one counter took c2's closest-hit /
+// any-hit stages 33.4 / 20.3 ms, 256-position chunks 29.4 / 14.5 but slowed
+// c3 (45.8 -> 49.7 ms closest hit: a wave's lanes then hold rays from a
+// wider span of the queue, which is less coherent).
 struct WavePool {
     uint32_t next = 0, end = 0;
+    uint32_t stripe = 0;   // wave-uniform
+    uint32_t tries = 0;    // stripes found used up (wave-uniform)
     // Hands the wanting lanes consecutive positions; positions >= n mean the
-    // queue is exhausted.
-    DEV uint32_t fetch(bool want, uint32_t* counter) {
+    // queue is exhausted.  `counters` holds the kPoolStripes counters.
+    DEV uint32_t fetch(bool want, uint32_t* counters, uint32_t n) {
         const unsigned long long mask = __ballot(want);
         if (mask == 0ull) return 0xffffffffu;
         const uint32_t lane = lane_id();
@@ -138,8 +136,26 @@ struct WavePool {
         uint32_t chunk = 0;
         if (nWant > avail) {
             const int leader = __ffsll(static_cast<long long>(mask)) - 1;
-            if (static_cast<int>(lane) == leader) chunk = atomicAdd(counter, 64u);
+            const uint32_t chunks = (n + 63u) / 64u;
+            if (static_cast<int>(lane) == leader) {
+                // Chunks [chunks * s / S, chunks * (s + 1) / S) belong to stripe s.
+                chunk = chunks * 64u;  // past the queue: exhausted
+                while (tries < kPoolStripes) {
+                    const uint32_t lo = static_cast<uint32_t>(static_cast<uint64_t>(chunks) * stripe / kPoolStripes);
+                    const uint32_t hi =
+                        static_cast<uint32_t>(static_cast<uint64_t>(chunks) * (stripe + 1u) / kPoolStripes);
+                    const uint32_t c = lo < hi ? atomicAdd(counters + stripe * kPoolStride, 1u) : hi - lo;
+                    if (c < hi - lo) {
+                        chunk = (lo + c) * 64u;
+                        break;
+                    }
+                    stripe = stripe + 1u == kPoolStripes ? 0u : stripe + 1u;
+                    ++tries;
+                }
+            }
             chunk = __shfl(chunk, leader);
+            stripe = __shfl(stripe, leader);
+            tries = __shfl(tries, leader);
         }
         const uint32_t rank = static_cast<uint32_t>(__popcll(mask & ((1ull << lane) - 1ull)));
         const uint32_t idx = rank < avail ? next + rank : chunk + (rank - avail);
@@ -250,7 +266,7 @@ template <bool ANY, int FETCH, int W>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W == 8 ? (ANY ? 6 : 4) : 8))) void k_wf_trace(WfParams P, uint32_t depth) {
     const uint32_t md = P.settings.maxDepth;
     const uint32_t n = ANY ? P.B.counts[P.B.shadowRayBase + depth] : stage_items(P, depth);
-    uint32_t* counter = &P.B.counts[P.B.fetchBase + (ANY ? md : 0u) + depth];
+    uint32_t* counter = &P.B.counts[P.B.fetchBase + ((ANY ? md : 0u) + depth) * kPoolStripes * kPoolStride];
     const uint32_t* in = (depth & 1u) ? P.B.qb : P.B.qa;
     const DevScene& S = P.scene;
     if (S.nodeCount == 0) {
@@ -280,13 +296,14 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W == 8 ? (AN
     best.slot = best.prim = kNoHit;
     TravCount<false> cnt{nullptr, nullptr};
     WavePool pool;
+    pool.stripe = blockIdx.x % kPoolStripes;
     // Static assignment: lane i of the grid takes items i, i + G, i + 2G...
     uint32_t nextStatic = blockIdx.x * 64u + threadIdx.x;
     for (;;) {
         const bool want = !active && !exhausted && (FETCH != 0 || !__any(active));
         uint32_t idx;
         if (FETCH == 2) {
-            idx = pool.fetch(want, counter);
+            idx = pool.fetch(want, counter, n);
         } else {
             idx = nextStatic;
             if (want) nextStatic += gridDim.x * 64u;
@@ -545,16 +562,64 @@ __global__ __launch_bounds__(kOrderThreads) void k_wf_order(WfParams P, uint32_t
     }
 }
 
+// Inclusive prefix sum over the lanes of a wave.
+DEV uint32_t wave_incl_scan(uint32_t v) {
+    const uint32_t lane = lane_id();
+    for (uint32_t d = 1u; d < 64u; d <<= 1) {
+        const uint32_t t = __shfl_up(v, d);
+        if (lane >= d) v += t;
+    }
+    return v;
+}
+
+// Shade blocks: four waves, whose queue appends are reserved together.
+constexpr uint32_t kShadeThreads = 256;
+constexpr uint32_t kShadeWaves = kShadeThreads / 64u;
+
+// Reserves c[k] consecutive slots of queue k (k = 0, 1, 2; counters
+// count[k]) for every thread of the block and returns the first in pos[k]:
+// one global atomic per queue and block.  Appends to one queue counter
+// from every wave of the grid serialise on that counter's address (about
+// 88M atomics per second on one address were measured), which bounded the
+// first bounce's shade stage; a block of four waves needs a quarter of
+// them.  Every thread of the block calls this, outside divergent code.
+DEV void block_reserve3(const uint32_t c[3], uint32_t* count0, uint32_t* count1, uint32_t* count2, uint32_t pos[3]) {
+    __shared__ uint32_t waveTotal[3][kShadeWaves];
+    __shared__ uint32_t blockBase[3];
+    const uint32_t wave = threadIdx.x >> 6, lane = lane_id();
+    uint32_t incl[3];
+    for (int k = 0; k < 3; ++k) {
+        incl[k] = wave_incl_scan(c[k]);
+        if (lane == 63u) waveTotal[k][wave] = incl[k];
+    }
+    __syncthreads();
+    if (threadIdx.x < 3u) {
+        const uint32_t k = threadIdx.x;
+        uint32_t total = 0;
+        for (uint32_t w = 0; w < kShadeWaves; ++w) total += waveTotal[k][w];
+        uint32_t* count = k == 0u ? count0 : (k == 1u ? count1 : count2);
+        blockBase[k] = total ? atomicAdd(count, total) : 0u;
+    }
+    __syncthreads();
+    for (int k = 0; k < 3; ++k) {
+        uint32_t off = blockBase[k];
+        for (uint32_t w = 0; w < wave; ++w) off += waveTotal[k][w];
+        pos[k] = off + incl[k] - c[k];
+    }
+    __syncthreads();  // the shared totals are rewritten by the next call
+}
+
 template <bool TEX>
-__global__ __launch_bounds__(64) void k_wf_shade(WfParams P, uint32_t depth) {
+__global__ __launch_bounds__(kShadeThreads) void k_wf_shade(WfParams P, uint32_t depth) {
     const uint32_t* in = depth > 0u && P.B.shadeOrder ? P.B.qsorted : ((depth & 1u) ? P.B.qb : P.B.qa);
     uint32_t* out = (depth & 1u) ? P.B.qa : P.B.qb;
     const uint32_t n = stage_items(P, depth);
     const uint32_t nLight = direct_sample_count(P.scene, P.settings);
     const bool more = depth + 1u < P.settings.maxDepth;
-    for (uint32_t base = blockIdx.x * 64u; base < n; base += gridDim.x * 64u) {
+    for (uint32_t base = blockIdx.x * kShadeThreads; base < n; base += gridDim.x * kShadeThreads) {
         const uint32_t i = base + threadIdx.x;
         bool alive = false, shadow = false, extChanged = false;
+        uint32_t ns = 0;
         uint32_t p = 0;
         bool work = i < n;
         PathCoord pc{};
@@ -583,7 +648,6 @@ __global__ __launch_bounds__(64) void k_wf_shade(WfParams P, uint32_t depth) {
                     alive = true;
                 } else {
                     const Mat& M = sf.M;
-                    uint32_t ns = 0;
                     float4* sr = P.B.sr + static_cast<size_t>(p) * P.B.maxShadow * 2u;
                     // nLight <= maxShadow by construction (enqueue_wavefront); the
                     // bound keeps the records inside the path's slots regardless.
@@ -596,7 +660,6 @@ __global__ __launch_bounds__(64) void k_wf_shade(WfParams P, uint32_t depth) {
                             continue;
                         sr[2u * ns] = make_float4(ld.x, ld.y, ld.z, __uint_as_float(group));
                         sr[2u * ns + 1u] = make_float4(contrib.x, contrib.y, contrib.z, maxDist);
-                        wave_push(true, p * P.B.maxShadow + ns, P.B.qsr, &P.B.counts[P.B.shadowRayBase + depth]);
                         ++ns;
                     }
                     if (ns > 0u) {
@@ -612,8 +675,15 @@ __global__ __launch_bounds__(64) void k_wf_shade(WfParams P, uint32_t depth) {
             }
             store_state(P.B, p, ps, depth, alive, extChanged);
         }
-        wave_push(shadow, p, P.B.qs, &P.B.counts[P.B.shadowPathBase + depth]);
-        wave_push(alive && more, p, out, &P.B.counts[depth + 1u]);
+        // Queue appends: the path's shadow rays, the path to the shadow
+        // queue, the path to the next bounce's ray queue.
+        const uint32_t c[3] = {ns, shadow ? 1u : 0u, alive && more ? 1u : 0u};
+        uint32_t pos[3];
+        block_reserve3(c, &P.B.counts[P.B.shadowRayBase + depth], &P.B.counts[P.B.shadowPathBase + depth],
+                       &P.B.counts[depth + 1u], pos);
+        for (uint32_t j = 0; j < ns; ++j) P.B.qsr[pos[0] + j] = p * P.B.maxShadow + j;
+        if (shadow) P.B.qs[pos[1]] = p;
+        if (alive && more) out[pos[2]] = p;
     }
 }
 
@@ -743,10 +813,11 @@ hipError_t launch_wavefront_pass(const WfParams& P, const WfLaunch& L, hipStream
             hipLaunchKernelGGL(k_wf_order, dim3(kOrderBlocks), dim3(kOrderThreads), 0, s, P, d);
             m.mark(SHIRO_HIP_STAGE_ORDER, s);
         }
+        const uint32_t sblocks = (blocks + kShadeWaves - 1u) / kShadeWaves;
         if (P.scene.textured) {
-            hipLaunchKernelGGL(k_wf_shade<true>, dim3(blocks), dim3(64), 0, s, P, d);
+            hipLaunchKernelGGL(k_wf_shade<true>, dim3(sblocks), dim3(kShadeThreads), 0, s, P, d);
         } else {
-            hipLaunchKernelGGL(k_wf_shade<false>, dim3(blocks), dim3(64), 0, s, P, d);
+            hipLaunchKernelGGL(k_wf_shade<false>, dim3(sblocks), dim3(kShadeThreads), 0, s, P, d);
         }
         m.mark(SHIRO_HIP_STAGE_SHADE, s);
         launch_trace<true>(P, L, d, blocks, tblocks, s);
