@@ -462,7 +462,7 @@ int shiro_hip_count_fetches(
 /* ---- Wavefront tuning (no counterpart in the reference: knobs of the
  * wavefront transport, set by the caller or chosen by shiro_hip_autotune). ---- */
 typedef struct shiro_hip_tuning {
-    uint32_t pathsPerPass;  /* paths in flight per pass (default 64M; the path
+    uint32_t pathsPerPass;  /* paths in flight per pass (default 128M; the path
                                state takes about (176 + 32 x light samples) B per
                                path, capped at 32 GiB) */
     uint32_t fetchMode;     /* traversal work distribution: 0 wave rounds, 1 lane refill,

@@ -107,7 +107,7 @@ struct shiro_hip_ctx {
     DevBuf acB, acA, acN, acD, acC;
     uint32_t acW = 0, acH = 0, acTotal = 0;
     // Wavefront tuning (shiro_hip_set_tuning / shiro_hip_autotune).
-    shiro_hip_tuning tuning{64u << 20, 0u, 8192u, 16384u, 8u, 1u, 1u, 4u};
+    shiro_hip_tuning tuning{128u << 20, 0u, 8192u, 16384u, 8u, 1u, 1u, 4u};
     uint32_t computeUnits = 256;
     // Batched rendering: items, work list and the work counter.
     DevBuf manyItems, manyWork, manyCounter;
@@ -199,7 +199,8 @@ int check_settings(shiro_hip_ctx* ctx, const shiro_hip_settings* s) {
 // kWavefrontStateBytes of path state).  Large passes pay: the last bounces
 // of a pass hold few paths and run latency-bound on a mostly idle GPU, so
 // fewer passes means fewer such tails (c3: 4M -> 64M paths per pass cuts a
-// frame from 245 to 189 ms).
+// frame from 245 to 189 ms; 64M -> 128M, one pass for the 16-spp frame
+// instead of two, 122 -> 114 ms).
 constexpr uint64_t kWavefrontStateBytes = 32ull << 30;
 
 // Starts the event list of a render call in stage-timing mode: the first
