@@ -278,8 +278,8 @@ def main():
     ap.add_argument("--no-autotune", action="store_true", help="keep the default wavefront tuning")
     ap.add_argument("--fetch-mode", type=int, default=None, help="force the traversal fetch mode (0, 1, 2)")
     ap.add_argument("--paths-per-pass", type=int, default=None, help="force the wavefront paths per pass")
-    ap.add_argument("--primary-packets", type=int, default=None, choices=[0, 1],
-                    help="force tile-packet tracing of camera rays off (0) or on (1)")
+    ap.add_argument("--primary-packets", type=int, default=None, choices=[0, 1, 4, 16, 64],
+                    help="camera-ray packets off (0) or of 1, 4, 16 or 64 samples per packet")
     ap.add_argument("--bvh-width", type=int, default=None, choices=[4, 8],
                     help="force the tree width the wavefront traversal walks")
     ap.add_argument("--shade-order", type=int, default=None, choices=[0, 1],

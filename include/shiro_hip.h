@@ -471,7 +471,12 @@ typedef struct shiro_hip_tuning {
     uint32_t stageBlocks;   /* grid cap of the grid-stride stage kernels (default 16384) */
     uint32_t batchBlocksPerCU; /* one-wave blocks per CU of the batched persistent kernel (default 8) */
     uint32_t shadeOrder;    /* 1: shade bounces after the first in shading-class order (default 1) */
-    uint32_t primaryPackets; /* 1: camera rays traced as 64-ray tile packets (default 1) */
+    uint32_t primaryPackets; /* G: a pass's paths are laid out in groups of 64, each G
+                                samples of an 8/sqrt(G)-pixel square (G = 1: 8x8 pixels,
+                                4: 4x4, 16: 2x2, 64: one pixel; a smaller G when the
+                                pass's samples per pixel are not a multiple), and each
+                                group's camera rays are traced as one packet; 0: G = 1
+                                and per-lane camera-ray traversal (default 16) */
     uint32_t bvhWidth;      /* 4 or 8: tree the wavefront traversal kernels walk (8 falls
                                back to 4 for a scene without an 8-wide tree) */
 } shiro_hip_tuning;

@@ -107,7 +107,7 @@ struct shiro_hip_ctx {
     DevBuf acB, acA, acN, acD, acC;
     uint32_t acW = 0, acH = 0, acTotal = 0;
     // Wavefront tuning (shiro_hip_set_tuning / shiro_hip_autotune).
-    shiro_hip_tuning tuning{128u << 20, 0u, 8192u, 16384u, 8u, 1u, 1u, 4u};
+    shiro_hip_tuning tuning{128u << 20, 0u, 8192u, 16384u, 8u, 1u, 16u, 4u};
     uint32_t computeUnits = 256;
     // Batched rendering: items, work list and the work counter.
     DevBuf manyItems, manyWork, manyCounter;
@@ -318,6 +318,11 @@ int enqueue_wavefront(shiro_hip_ctx* ctx, const DevScene& scene, const shiro_hip
         }
         P.sliceBegin = b;
         P.sliceCount = std::min(static_cast<uint32_t>(S), count - b);
+        // Path groups of G samples as asked, or the largest smaller choice
+        // that divides the pass's samples per pixel.
+        uint32_t g = L.primaryPackets ? L.primaryPackets : 1u;
+        while (g > 1u && P.sliceCount % g != 0u) g /= 4u;
+        P.groupLog = g == 64u ? 6u : (g == 16u ? 4u : (g == 4u ? 2u : 0u));
         HIPC(launch_wavefront_pass(P, L, ctx->stream, ctx->stageTiming && events ? &ctx->marks : nullptr),
              "wavefront launch");
         if (cancel) HIPC(hipStreamSynchronize(ctx->stream), "wavefront");
@@ -1048,7 +1053,8 @@ int shiro_hip_set_tuning(shiro_hip_ctx* ctx, const shiro_hip_tuning* t) {
     std::lock_guard<std::mutex> lock(ctx->mu);
     if (t->pathsPerPass == 0 || t->fetchMode > 2u || t->traceBlocks == 0 || t->stageBlocks == 0 ||
         t->batchBlocksPerCU == 0 || t->batchBlocksPerCU > 64u || t->shadeOrder > 1u ||
-        t->primaryPackets > 1u || (t->bvhWidth != 4u && t->bvhWidth != 8u))
+        (t->primaryPackets > 1u && t->primaryPackets != 4u && t->primaryPackets != 16u && t->primaryPackets != 64u) ||
+        (t->bvhWidth != 4u && t->bvhWidth != 8u))
         return fail(ctx, SHIRO_HIP_INVALID, "invalid tuning");
     ctx->tuning = *t;
     return SHIRO_HIP_OK;
@@ -1109,8 +1115,9 @@ int shiro_hip_autotune(shiro_hip_ctx* ctx, const shiro_hip_camera* camera, const
         }
         const uint32_t width = t.bvhWidth == 8u && ctx->dev.nodes8 ? 8u : 4u;
         char line[192];
-        std::snprintf(line, sizeof(line), "pathsPerPass %u fetchMode %u traceBlocks %u bvhWidth %u: %.3f ms\n",
-                      t.pathsPerPass, t.fetchMode, t.traceBlocks, width, *ms);
+        std::snprintf(line, sizeof(line),
+                      "pathsPerPass %u fetchMode %u traceBlocks %u primaryPackets %u bvhWidth %u: %.3f ms\n",
+                      t.pathsPerPass, t.fetchMode, t.traceBlocks, t.primaryPackets, width, *ms);
         log += line;
         return 0;
     };
@@ -1123,7 +1130,24 @@ int shiro_hip_autotune(shiro_hip_ctx* ctx, const shiro_hip_camera* camera, const
             bestT = cands[i];
         }
     }
-    // Second stage: the best candidate with the other tree width, when the
+    // Second stage: the best candidate with camera-ray packets of 1, 4, 16
+    // and 64 samples per packet (when packets are on).
+    if (bestT.primaryPackets) {
+        const shiro_hip_tuning first = bestT;
+        for (uint32_t g : {1u, 4u, 16u, 64u}) {
+            if (g == first.primaryPackets) continue;
+            shiro_hip_tuning t = first;
+            t.primaryPackets = g;
+            float ms = 0.0f;
+            rc = timeCandidate(t, &ms);
+            if (rc) return rc;
+            if (ms < bestMs) {
+                bestMs = ms;
+                bestT = t;
+            }
+        }
+    }
+    // Third stage: the best candidate with the other tree width, when the
     // scene has an 8-wide tree.
     if (ctx->dev.nodes8) {
         shiro_hip_tuning t = bestT;

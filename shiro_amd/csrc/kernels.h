@@ -59,13 +59,15 @@ struct WfParams {
     WfBuffers B;
     uint32_t sampleStart, sliceBegin, sliceCount, batchCount;
     uint32_t tilesX, tiles;
+    uint32_t groupLog;  // path layout: groups of 2^groupLog samples (path_coord, wavefront.hip)
 };
 
 struct WfLaunch {
     uint32_t maxBlocks;    // grid of the grid-stride stage kernels
     uint32_t traceBlocks;  // grid of the persistent traversal kernels
     uint32_t fetchMode;    // traversal work distribution, see k_wf_trace
-    uint32_t primaryPackets;  // 1: camera rays traced as tile packets (k_wf_primary)
+    uint32_t primaryPackets;  // samples per path group (1, 4, 16, 64; path_coord), 0: G = 1 and no
+                              // camera-ray packets
     uint32_t bvhWidth;     // 4 or 8: tree of the traversal kernels (8 only if the scene has one)
 };
 

@@ -46,12 +46,37 @@ struct PathCoord {
     uint32_t x, y, s;
 };
 
+// Path layout of a pass: path p = 64 * w + lane, where group w holds G =
+// 2^groupLog consecutive samples of an R x R pixel square of an 8x8 tile,
+// R = 8 / sqrt(G) (G = 1: the whole tile at one sample; 4: a 4x4 quarter at
+// four samples; 16: a 2x2 square at sixteen; 64: one pixel at sixty-four).
+// Group w covers samples (w / (tiles * G)) * G + [0, G), tile
+// (w / G) % tiles and square w % G (squares row-major in the tile); lane =
+// (sample in group) * R * R + (pixel in square, row-major).  The samples of
+// one pixel differ only by their jitter, so a wave of consecutive paths
+// holds nearly identical camera rays and surfaces, and queues filled in
+// path order keep them together for the later stages.  sliceCount % G == 0.
 DEV PathCoord path_coord(const WfParams& P, uint32_t p) {
-    const uint32_t per = P.tiles * 64u;
-    const uint32_t s = p / per;
-    const uint32_t q = p - s * per;
-    const uint32_t tile = q >> 6, lane = q & 63u;
-    return PathCoord{(tile % P.tilesX) * 8u + (lane & 7u), (tile / P.tilesX) * 8u + (lane >> 3), s};
+    const uint32_t gl = P.groupLog, rl = 3u - (gl >> 1u), ql = 3u - rl;
+    const uint32_t w = p >> 6, lane = p & 63u;
+    const uint32_t perGroup = P.tiles << gl;
+    const uint32_t sg = w / perGroup;
+    const uint32_t rem = w - sg * perGroup;
+    const uint32_t t = rem >> gl, q = rem & ((1u << gl) - 1u);
+    const uint32_t sub = lane >> (2u * rl), j = lane & ((1u << (2u * rl)) - 1u);
+    const uint32_t xin = ((q & ((1u << ql) - 1u)) << rl) + (j & ((1u << rl) - 1u));
+    const uint32_t yin = ((q >> ql) << rl) + (j >> rl);
+    return PathCoord{(t % P.tilesX) * 8u + xin, (t / P.tilesX) * 8u + yin, (sg << gl) + sub};
+}
+
+// Inverse of path_coord: the path of sample s of pixel (xin, yin) of tile t.
+DEV uint32_t path_index(const WfParams& P, uint32_t s, uint32_t t, uint32_t xin, uint32_t yin) {
+    const uint32_t gl = P.groupLog, rl = 3u - (gl >> 1u), ql = 3u - rl;
+    const uint32_t sg = s >> gl, sub = s & ((1u << gl) - 1u);
+    const uint32_t q = ((yin >> rl) << ql) + (xin >> rl);
+    const uint32_t j = ((yin & ((1u << rl) - 1u)) << rl) + (xin & ((1u << rl) - 1u));
+    const uint32_t w = sg * (P.tiles << gl) + (t << gl) + q;
+    return (w << 6) + (sub << (2u * rl)) + j;
 }
 
 DEV Sampler path_sampler(const WfParams& P, const PathCoord& c) {
@@ -408,11 +433,13 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W == 8 ? (AN
     }
 }
 
-// Closest hit of the pass's camera rays as packets: one wave per (8x8
-// tile, sample), the 64 rays of a tile traversing the tree together
-// (intersect_closest_packet, dev_bvh.h).
+// Closest hit of the pass's camera rays as packets
+// (intersect_closest_packet, dev_bvh.h): one wave per group of 64
+// consecutive paths (path_coord: G samples of an 8/sqrt(G)-pixel square);
+// every lane tests each node the packet visits against its own ray and
+// keeps its own closest hit.
 template <int W>
-__global__ __launch_bounds__(64) void k_wf_primary(WfParams P) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W == 8 ? 6 : 8))) void k_wf_primary(WfParams P) {
     __shared__ uint32_t wstack[kStackSize];
     const uint32_t packets = P.sliceCount * P.tiles;
     for (uint32_t w = blockIdx.x; w < packets; w += gridDim.x) {
@@ -731,9 +758,8 @@ __global__ __launch_bounds__(64) void k_wf_accum(WfParams P) {
         normal = P.sums.normal[pix];
         depth = P.sums.depth[pix];
     }
-    const uint32_t per = P.tiles * 64u;
     for (uint32_t s = 0; s < P.sliceCount; ++s) {
-        const uint32_t p = s * per + tile * 64u + lane;
+        const uint32_t p = path_index(P, s, tile, lane & 7u, lane >> 3);
         const float4 L = P.B.L[p], A = P.B.A[p], N = P.B.N[p];
         beauty.x = beauty.x + L.x;
         beauty.y = beauty.y + L.y;

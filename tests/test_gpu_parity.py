@@ -400,11 +400,39 @@ def test_bvh_width_identical(hip_ctx, make):
     saved = hip_ctx.get_tuning()
     try:
         for mode in (0, 1, 2):
-            for packets in (0, 1):
+            for packets in (0, 1, 4, 16, 64):
                 hip_ctx.set_tuning(bvhWidth=8, fetchMode=mode, primaryPackets=packets, pathsPerPass=1 << 16)
                 wf = hip_ctx.render(wl.camera, st)
                 for k in ("beauty", "albedo", "normal", "depth"):
                     assert np.array_equal(getattr(wf, k), getattr(mk, k)), (mode, packets, k)
+    finally:
+        hip_ctx.set_tuning(**{k: getattr(saved, k) for k, _ in abi.Tuning._fields_})
+
+
+@pytest.mark.parametrize("make", [c2_mini, c3_mini, emissive_mini], ids=["c2", "c3", "emissive"])
+def test_packet_sample_groups_identical(hip_ctx, make):
+    """Camera-ray packets of 4 samples of a 4x4 pixel square, of 16 samples
+    of a 2x2 square and of 64 samples of one pixel (and the fallback to a
+    smaller group when the pass's samples per pixel are not a multiple)
+    render the megakernel's frame bit for bit: each lane keeps its own ray's
+    closest hit, over the 4-wide and the 8-wide tree."""
+    from shiro_amd import abi
+    wl = make()
+    st = RenderSettings(**{**wl.settings.__dict__, "samplesPerPixel": 64})
+    _log(f"packet groups {wl.name} {st.width}x{st.height} 64 spp")
+    hip_ctx.upload(wl.scene)
+    mk = hip_ctx.render(wl.camera, st)
+    wst = RenderSettings(**{**st.__dict__, "transport": abi.TRANSPORT_WAVEFRONT})
+    saved = hip_ctx.get_tuning()
+    try:
+        # 1 << 8 paths per pass: one sample per pass, so G falls back to 1.
+        cases = [(4, 1 << 26, 4), (16, 1 << 26, 4), (64, 1 << 26, 4), (64, 1 << 8, 4), (16, 1 << 26, 8),
+                 (64, 1 << 26, 8)]
+        for packets, paths, width in cases:
+            hip_ctx.set_tuning(primaryPackets=packets, pathsPerPass=paths, bvhWidth=width)
+            wf = hip_ctx.render(wl.camera, wst)
+            for k in ("beauty", "albedo", "normal", "depth"):
+                assert np.array_equal(getattr(wf, k), getattr(mk, k)), (packets, paths, width, k)
     finally:
         hip_ctx.set_tuning(**{k: getattr(saved, k) for k, _ in abi.Tuning._fields_})
 
