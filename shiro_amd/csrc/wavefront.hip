@@ -636,8 +636,11 @@ DEV void block_reserve3(const uint32_t c[3], uint32_t* count0, uint32_t* count1,
     __syncthreads();  // the shared totals are rewritten by the next call
 }
 
-template <bool TEX>
-__global__ __launch_bounds__(kShadeThreads) void k_wf_shade(WfParams P, uint32_t depth) {
+// FIRST: the first bounce (depth 0), compiled apart so that the depth
+// tests of the shading code fold; `depthArg` is the depth otherwise.
+template <bool TEX, bool FIRST>
+__global__ __launch_bounds__(kShadeThreads) void k_wf_shade(WfParams P, uint32_t depthArg) {
+    const uint32_t depth = FIRST ? 0u : depthArg;
     const uint32_t* in = depth > 0u && P.B.shadeOrder ? P.B.qsorted : ((depth & 1u) ? P.B.qb : P.B.qa);
     uint32_t* out = (depth & 1u) ? P.B.qa : P.B.qb;
     const uint32_t n = stage_items(P, depth);
@@ -841,9 +844,15 @@ hipError_t launch_wavefront_pass(const WfParams& P, const WfLaunch& L, hipStream
         }
         const uint32_t sblocks = (blocks + kShadeWaves - 1u) / kShadeWaves;
         if (P.scene.textured) {
-            hipLaunchKernelGGL(k_wf_shade<true>, dim3(sblocks), dim3(kShadeThreads), 0, s, P, d);
+            if (d == 0u) {
+                hipLaunchKernelGGL((k_wf_shade<true, true>), dim3(sblocks), dim3(kShadeThreads), 0, s, P, d);
+            } else {
+                hipLaunchKernelGGL((k_wf_shade<true, false>), dim3(sblocks), dim3(kShadeThreads), 0, s, P, d);
+            }
+        } else if (d == 0u) {
+            hipLaunchKernelGGL((k_wf_shade<false, true>), dim3(sblocks), dim3(kShadeThreads), 0, s, P, d);
         } else {
-            hipLaunchKernelGGL(k_wf_shade<false>, dim3(sblocks), dim3(kShadeThreads), 0, s, P, d);
+            hipLaunchKernelGGL((k_wf_shade<false, false>), dim3(sblocks), dim3(kShadeThreads), 0, s, P, d);
         }
         m.mark(SHIRO_HIP_STAGE_SHADE, s);
         launch_trace<true>(P, L, d, blocks, tblocks, s);
