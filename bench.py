@@ -421,6 +421,10 @@ def main():
             rec = json.load(open(tpath)).get(f"{args.workload}/{args.transport}/{dom}")
             if rec:
                 traffic, traffic_src = rec["bytes_per_launch"], rec["source"]
+        # Algorithmic bytes count every ray's own node and triangle fetches;
+        # camera-ray packets and same-pixel path groups share fetches between
+        # rays (LDS stack, L2, last-level cache), so the closest-hit stage can
+        # exceed the HBM peak on this measure; `traffic` is what HBM moved.
         roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic, "traffic_source": traffic_src,
                     "kernel": names[dom], "launches_per_step": launches,

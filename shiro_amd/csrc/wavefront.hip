@@ -638,8 +638,13 @@ DEV void block_reserve3(const uint32_t c[3], uint32_t* count0, uint32_t* count1,
 
 // FIRST: the first bounce (depth 0), compiled apart so that the depth
 // tests of the shading code fold; `depthArg` is the depth otherwise.
+// At most 168 VGPRs, three waves per SIMD (a few registers spill): the
+// kernel is bound by its instruction stream, and the third wave hides more
+// of its latencies than the spills cost (c3 shade stage 34.4 -> 29.4 ms;
+// capped at 128 VGPRs, four waves, it spills 38 registers and loses).
 template <bool TEX, bool FIRST>
-__global__ __launch_bounds__(kShadeThreads) void k_wf_shade(WfParams P, uint32_t depthArg) {
+__global__ __launch_bounds__(kShadeThreads) __attribute__((amdgpu_waves_per_eu(3))) void k_wf_shade(WfParams P,
+                                                                                                   uint32_t depthArg) {
     const uint32_t depth = FIRST ? 0u : depthArg;
     const uint32_t* in = depth > 0u && P.B.shadeOrder ? P.B.qsorted : ((depth & 1u) ? P.B.qb : P.B.qa);
     uint32_t* out = (depth & 1u) ? P.B.qa : P.B.qb;
