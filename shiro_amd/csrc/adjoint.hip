@@ -325,18 +325,29 @@ DEV void adjoint_path(const AdjointParams& P, GradCache& cache, uint32_t x, uint
     }
 }
 
-// Grid: one 64-thread block per (8x8 tile, sample); tile-major within a
-// sample.  MAXD bounds the path length, sizing the per-thread vertex tape.
+// Grid: one 64-thread block per group of G = 2^groupLog samples of an
+// 8/sqrt(G)-pixel square of an 8x8 tile, laid out as the wavefront's path
+// groups (block w: samples (w / (tiles * G)) * G + [0, G), tile (w / G) %
+// tiles, square w % G; thread = sample in group * R * R + pixel in square):
+// the lanes of a wave replay nearly identical paths of one pixel or a few
+// neighbouring ones, and their texel gradients land on the same entries of
+// the block's cache.  MAXD bounds the path length, sizing the per-thread vertex tape.
 // At most 128 VGPRs (4 waves per SIMD): the replay spills some registers
 // there, and still runs faster than at 2 or 3 waves per SIMD.
 template <uint32_t MAXD>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_adjoint(AdjointParams P) {
     __shared__ GradCache cache;
     cache_clear(cache);
-    const uint32_t tile = blockIdx.x % P.tiles;
-    const uint32_t s = blockIdx.x / P.tiles;
-    const uint32_t x = (tile % P.tilesX) * 8u + (threadIdx.x & 7u);
-    const uint32_t y = (tile / P.tilesX) * 8u + (threadIdx.x >> 3);
+    const uint32_t gl = P.groupLog, rl = 3u - (gl >> 1u), ql = 3u - rl;
+    const uint32_t perGroup = P.tiles << gl;
+    const uint32_t sg = blockIdx.x / perGroup;
+    const uint32_t rem = blockIdx.x - sg * perGroup;
+    const uint32_t tile = rem >> gl, q = rem & ((1u << gl) - 1u);
+    const uint32_t lane = threadIdx.x;
+    const uint32_t sub = lane >> (2u * rl), j = lane & ((1u << (2u * rl)) - 1u);
+    const uint32_t s = (sg << gl) + sub;
+    const uint32_t x = (tile % P.tilesX) * 8u + ((q & ((1u << ql) - 1u)) << rl) + (j & ((1u << rl) - 1u));
+    const uint32_t y = (tile / P.tilesX) * 8u + ((q >> ql) << rl) + (j >> rl);
     const DevSettings& st = P.settings;
     if (x < st.width && y < st.height && s < P.sampleCount) {
         const float4 db = P.dBeauty[static_cast<size_t>(y) * st.width + x];

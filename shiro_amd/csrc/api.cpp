@@ -1315,6 +1315,9 @@ int render_adjoint(shiro_hip_ctx* ctx, const shiro_hip_camera* camera, const shi
     P.sampleCount = sampleCount;
     P.tilesX = (settings->width + 7u) / 8u;
     P.tiles = P.tilesX * ((settings->height + 7u) / 8u);
+    // Blocks of 16 samples of a 2x2 pixel square (4 of a 4x4 square, or
+    // one sample of an 8x8 tile, when the sample count is not a multiple).
+    P.groupLog = sampleCount % 16u == 0u ? 4u : (sampleCount % 4u == 0u ? 2u : 0u);
     HIPC(hipEventRecord(ctx->ev0, ctx->stream), "event");
     HIPC(launch_adjoint(P, ctx->stream), "adjoint launch");
     HIPC(hipEventRecord(ctx->ev1, ctx->stream), "event");

@@ -102,6 +102,7 @@ struct AdjointParams {
     uint32_t envWanted;                   // 1 if any envGrad is set
     uint32_t sampleStart, sampleCount;
     uint32_t tilesX, tiles;
+    uint32_t groupLog;  // blocks of 2^groupLog samples of a pixel square, as the wavefront path groups
 };
 
 hipError_t launch_render(const RenderParams& P, bool count, hipStream_t stream);
