@@ -30,10 +30,24 @@ public:
     bool SupportsScene(const render::Scene& scene) const;
     render::FrameBuffer RenderSampleBatch(const RenderRequest& request) const override;
 
+    // Progressive rendering with the accumulator on the device: one call
+    // stands for the Hydra worker's RenderSampleBatch, FrameAccumulator::
+    // Accumulate and Resolve (src/hydra/RenderParam.cpp:789-803,
+    // src/render/Renderer.cpp:47-110), with the same results.  `reset`
+    // clears the accumulator first (the worker resets at sampleStart == 0);
+    // a size change also clears it.  With `resolve` the returned frame is
+    // the resolved running frame; without it the call reads nothing back and
+    // returns an empty frame.  A cancelled batch is not accumulated and
+    // returns an empty frame.  accumulatedSamples (may be null) receives the
+    // accumulator's sample count (0 on failure).
+    render::FrameBuffer RenderProgressiveBatch(const RenderRequest& request, bool reset, bool resolve = true,
+                                               uint32_t* accumulatedSamples = nullptr) const;
+
     struct RuntimeState;
 
 private:
     bool EnsureRuntime() const;
+    bool PrepareScene(const render::Scene& scene) const;
 
     mutable std::mutex mutex_;
     mutable std::unique_ptr<RuntimeState> runtime_;

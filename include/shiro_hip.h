@@ -311,6 +311,32 @@ int shiro_hip_accum_reset(shiro_hip_ctx* ctx, uint32_t width, uint32_t height);
 int shiro_hip_accum_add(shiro_hip_ctx* ctx, const shiro_hip_frame* batch, uint32_t sampleCount);
 int shiro_hip_accum_resolve(shiro_hip_ctx* ctx, shiro_hip_frame* out);
 
+/* Progressive batch with the accumulator on the device: replaces the Hydra
+ * worker's RenderSampleBatch -> FrameAccumulator::Accumulate -> Resolve
+ * sequence (RenderParam.cpp:789-803 over Renderer.cpp:47-110) for a GPU
+ * backend.  Renders samples [sampleStart, sampleStart + sampleCount) into
+ * the context's device frame, adds them to the device accumulator with
+ * weight sampleCount (cleared first with SHIRO_HIP_PROGRESSIVE_RESET, or
+ * when the frame size changed, as Accumulate resets on a size change) and,
+ * when `resolved` is non-NULL, writes the resolved running frame (the
+ * accumulator's means; depth = mean of the batch minima over the batches
+ * that hit) into the host buffers `resolved`.  A NULL `resolved` skips the
+ * readback (the display is not refreshed for this batch).  A cancelled
+ * batch (cancel as in shiro_hip_render_batch) is not accumulated and the
+ * call returns SHIRO_HIP_CANCELLED.  accumulatedSamples (may be NULL)
+ * receives the accumulator's sample count after the call. */
+enum { SHIRO_HIP_PROGRESSIVE_RESET = 1 };
+int shiro_hip_render_progressive(
+    shiro_hip_ctx* ctx,
+    const shiro_hip_camera* camera,
+    const shiro_hip_settings* settings,
+    uint32_t sampleStart,
+    uint32_t sampleCount,
+    uint32_t flags,
+    const volatile uint8_t* cancel,
+    shiro_hip_frame* resolved,
+    uint32_t* accumulatedSamples);
+
 /* Stream and timing.  last_render_ms is the HIP-event time of the most
  * recent render call's kernels on the context stream. */
 void* shiro_hip_stream(shiro_hip_ctx* ctx);

@@ -274,7 +274,32 @@ def oracle_fns():
     f.triangulate = _bind(lib, "oracle_triangulate", C.c_int64, _P(C.c_int32), _u, _P(C.c_int32), _u, _up,
                           C.c_uint64)
     f.smooth_normals = _bind(lib, "oracle_smooth_normals", None, _fp, _u, _up, _u, _fp)
+    f.plant_mistake = _bind(lib, "oracle_plant_mistake", None, C.c_int)
     return f
+
+
+# Planted mistakes of the oracle (oracle.cpp, enum Plant), for tests that
+# show a fixture reaches the branch a mistake would change.
+PLANTS = {
+    "extinction_sign": 1, "pane_no_denominator": 2, "opacity_no_rescale": 3, "opacity_no_skip": 4,
+    "delta_as_rough": 5, "ignore_normal": 6, "ignore_coat_normal": 7, "no_attenuation": 8,
+    "tint_in_medium": 9, "no_override_flip": 10, "mirror_not_delta": 11,
+}
+
+
+class planted:
+    """Context manager: the oracle computes with one planted mistake."""
+
+    def __init__(self, name):
+        self.code = PLANTS[name]
+
+    def __enter__(self):
+        oracle_fns().plant_mistake(self.code)
+        return self
+
+    def __exit__(self, *exc):
+        oracle_fns().plant_mistake(0)
+        return False
 
 
 def triangulate(face_vertex_counts, face_vertex_indices):

@@ -42,6 +42,25 @@
 
 namespace orc {
 
+// Planted mistakes (tests only, oracle_plant_mistake): 0 = none.  Each value
+// changes one branch the way a plausible slip would, so that a test can show
+// that a fixture reaches that branch (its comparison must then fail).
+enum Plant : int {
+    kPlantNone = 0,
+    kPlantExtinctionSign = 1,    // +ln(tint)/depth instead of -ln(tint)/depth
+    kPlantPaneNoDenominator = 2, // pane transmittance 1-R instead of (1-R)/(1+R)
+    kPlantOpacityNoRescale = 3,  // opaque hits at opacity < 1 not divided by opacity
+    kPlantOpacityNoSkip = 4,     // no pass-through at opacity < 1
+    kPlantDeltaAsRough = 5,      // smooth glass sent through the rough microfacet branch
+    kPlantIgnoreNormal = 6,      // material normal override ignored
+    kPlantIgnoreCoatNormal = 7,  // coat normal override ignored
+    kPlantNoAttenuation = 8,     // interior medium never attenuates
+    kPlantTintInMedium = 9,      // surface tint applied although transmissionDepth > 0
+    kPlantNoOverrideFlip = 10,   // override not flipped to the fallback hemisphere
+    kPlantMirrorNotDelta = 11,   // smooth reflection marked non-delta (MIS at the next miss)
+};
+int g_plant = kPlantNone;
+
 // ---------------------------------------------------------------------------
 // Constants (CpuPathTracer.cpp:59-77, EnvironmentMap.cpp:19-21)
 // ---------------------------------------------------------------------------
@@ -997,10 +1016,11 @@ V3 apply_thin_film(V3 lobe, const Material& m, V3 n, V3 v, V3 l) {
 V3 resolve_normal(const Material& m, V3 fallback, bool coat) {
     const bool has = coat ? m.hasCoatNormalOverride : m.hasNormalOverride;
     if (!has) return fallback;
+    if (g_plant == (coat ? kPlantIgnoreCoatNormal : kPlantIgnoreNormal)) return fallback;
     const V3 c = coat ? v3(m.coatNormalOverride) : v3(m.normalOverride);
     if (length(c) <= 0.0f) return fallback;
     V3 n = normalize(c);
-    if (dot(n, fallback) < 0.0f) n = n * -1.0f;
+    if (dot(n, fallback) < 0.0f && g_plant != kPlantNoOverrideFlip) n = n * -1.0f;
     return n;
 }
 
@@ -1099,7 +1119,7 @@ float specular_pdf(V3 n, V3 v, V3 l, float r) {
 }
 
 V3 transmission_tint(const Material& m) {
-    if (!m.thinWalled && m.transmissionDepth > 0.0f) return {1.0f, 1.0f, 1.0f};
+    if (!m.thinWalled && m.transmissionDepth > 0.0f && g_plant != kPlantTintInMedium) return {1.0f, 1.0f, 1.0f};
     return clamp3(v3(m.transmissionColor), 0.0f, 1.0f);
 }
 
@@ -1413,6 +1433,7 @@ bool refract_dir(V3 i, V3 n, float eta, V3* out) {
 
 float pane_transmittance(float R) {
     const float r = clampf(R, 0.0f, 1.0f);
+    if (g_plant == kPlantPaneNoDenominator) return clampf(1.0f - r, 0.0f, 1.0f);
     return clampf((1.0f - r) / std::max(1.0f + r, 1.0e-6f), 0.0f, 1.0f);
 }
 
@@ -1423,9 +1444,10 @@ V3 medium_extinction(const Material& m) {
     if (m.thinWalled || m.transmissionDepth <= 0.0f) return e;
     const float depth = std::max(m.transmissionDepth, 1.0e-3f);
     const V3 tint = clamp3(v3(m.transmissionColor), 1.0e-4f, 1.0f);
-    e.x += -std::log(tint.x) / depth;
-    e.y += -std::log(tint.y) / depth;
-    e.z += -std::log(tint.z) / depth;
+    const float sgn = g_plant == kPlantExtinctionSign ? 1.0f : -1.0f;
+    e.x += sgn * std::log(tint.x) / depth;
+    e.y += sgn * std::log(tint.y) / depth;
+    e.z += sgn * std::log(tint.z) / depth;
     return e;
 }
 
@@ -1826,7 +1848,7 @@ PathResult trace_path(const Scene& s, const Settings& st, Ray ray, const Sampler
             break;
         }
         const V3 Tin = T;
-        if (mediumOn) T = T * medium_attenuation(ext, hit.t);
+        if (mediumOn && g_plant != kPlantNoAttenuation) T = T * medium_attenuation(ext, hit.t);
         const Material m = hit.material < s.materials.size() ? s.materials[hit.material] : default_material();
         // Texture extension: me is evaluated, ms drives sampling decisions.
         Material me = m, ms = m;
@@ -1851,7 +1873,7 @@ PathResult trace_path(const Scene& s, const Settings& st, Ray ray, const Sampler
         const V3 v = ray.d * -1.0f;
         const float opacity = clampf(m.opacity, 0.0f, 1.0f);
         if (opacity < 0.999f) {
-            if (smp.next1(b0) > opacity) {
+            if (smp.next1(b0) > opacity && g_plant != kPlantOpacityNoSkip) {
                 ray.o = hit.p + ray.d * kEps;
                 if (vx) {
                     vx->kind = 2;
@@ -1861,7 +1883,7 @@ PathResult trace_path(const Scene& s, const Settings& st, Ray ray, const Sampler
                 }
                 continue;
             }
-            T = T / std::max(opacity, 1.0e-4f);
+            if (g_plant != kPlantOpacityNoRescale) T = T / std::max(opacity, 1.0e-4f);
         }
         if (vx) {
             vx->T = T;
@@ -1962,7 +1984,7 @@ PathResult trace_path(const Scene& s, const Settings& st, Ray ray, const Sampler
                 const float relEta = et / std::max(ei, 1.0f);
                 const float R = fresnel_dielectric(std::fabs(dot(ng, v)), ei, et);
                 const float mt = clampf(m.transmission, 0.0f, 1.0f) * (1.0f - clampf(m.metallic, 0.0f, 1.0f));
-                if (rT <= kDeltaRoughness) {
+                if (rT <= kDeltaRoughness && g_plant != kPlantDeltaAsRough) {
                     if (!refract_dir(ray.d, ng, etaRatio, &next)) {
                         next = normalize(reflect(ray.d, ng));
                         if (wr <= 0.0f) break;
@@ -2026,7 +2048,7 @@ PathResult trace_path(const Scene& s, const Settings& st, Ray ray, const Sampler
             if (rc <= 0.0f) break;
             pdf = std::max((wr / wsum) * bsdf_pdf(ms, n, cn, v, next), 1.0e-6f);
             f = bsdf_eval(me, n, cn, v, next);
-            delta = sr <= kDeltaRoughness;
+            delta = sr <= kDeltaRoughness && g_plant != kPlantMirrorNotDelta;
         } else {
             if (nDiffuse >= st.diffuseDepth) break;
             ++nDiffuse;
@@ -2268,6 +2290,9 @@ void oracle_qmc_fill(uint32_t x0, uint32_t y0, uint32_t w, uint32_t h, uint32_t 
 }
 
 float oracle_radical_inverse(uint32_t base, uint32_t n) { return radical_inverse(base, n); }
+
+// Tests only: switches one planted mistake on (orc::Plant), 0 switches it off.
+void oracle_plant_mistake(int which) { g_plant = which; }
 
 void* oracle_scene_create(const shiro_hip_scene* in, int useBvh) {
     auto* s = new Scene();

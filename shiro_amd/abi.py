@@ -175,6 +175,7 @@ RESULT_NAMES = {
 
 ENV_AUTOMATIC, ENV_LATLONG, ENV_ANGULAR = 0, 1, 2
 TEXELS_DEVICE, TEXELS_EVAL_ONLY = 1, 2
+PROGRESSIVE_RESET = 1
 TRANSPORT_MEGAKERNEL, TRANSPORT_WAVEFRONT = 0, 1
 
 # Every symbol include/shiro_hip.h declares (checked by the ABI test).
@@ -183,7 +184,7 @@ EXPORTS = [
     "shiro_hip_create", "shiro_hip_destroy", "shiro_hip_last_error",
     "shiro_hip_get_status", "shiro_hip_supports_scene", "shiro_hip_upload_scene",
     "shiro_hip_render_batch", "shiro_hip_render_batch_device",
-    "shiro_hip_accum_reset", "shiro_hip_accum_add", "shiro_hip_accum_resolve",
+    "shiro_hip_accum_reset", "shiro_hip_accum_add", "shiro_hip_accum_resolve", "shiro_hip_render_progressive",
     "shiro_hip_stream", "shiro_hip_synchronize", "shiro_hip_last_render_ms",
     "shiro_hip_scene_info_get", "shiro_hip_qmc_fill", "shiro_hip_intersect",
     "shiro_hip_occluded", "shiro_hip_trace_primary", "shiro_hip_eval_bsdf",

@@ -140,37 +140,77 @@ void BuildView(const Scene& s, SceneView* v) {
     Set3(o.environmentBottom, s.environmentBottom);
 }
 
-// FNV-1a over the scene content, like the reference's scene signature
+// Content hash of a scene, like the reference's scene signature
 // (OptixBackend.cpp:150-232): a re-upload happens only when it changes.
-uint64_t Signature(const Scene& s) {
-    uint64_t h = 1469598103934665603ull;
-    auto mix = [&](const void* p, size_t n) {
+// FNV-1a over 8-byte words (the tail bytewise), with every vector's length
+// mixed in before its contents so that split and merged meshes or lights
+// holding the same bytes differ, and the dome maps hashed by size, layout
+// and pixels (not by address: a new map may reuse a freed map's address).
+class Hasher {
+public:
+    void Bytes(const void* p, size_t n) {
         const unsigned char* b = static_cast<const unsigned char*>(p);
-        for (size_t i = 0; i < n; ++i) {
-            h ^= b[i];
-            h *= 1099511628211ull;
+        size_t i = 0;
+        for (; i + 8 <= n; i += 8) {
+            uint64_t w;
+            std::memcpy(&w, b + i, 8);
+            h_ = (h_ ^ w) * 1099511628211ull;
         }
-    };
+        for (; i < n; ++i) h_ = (h_ ^ b[i]) * 1099511628211ull;
+    }
+    void Size(size_t n) {
+        const uint64_t v = n;
+        Bytes(&v, sizeof(v));
+    }
+    template <typename T>
+    void Vector(const std::vector<T>& v) {
+        Size(v.size());
+        Bytes(v.data(), v.size() * sizeof(T));
+    }
+    uint64_t Value() const { return h_; }
+
+private:
+    uint64_t h_ = 1469598103934665603ull;
+};
+
+uint64_t Signature(const Scene& s) {
+    Hasher h;
+    h.Size(s.materials.size());
     for (const PbrMaterial& m : s.materials) {
         const shiro_hip_material c = ToMaterial(m);
-        mix(&c, sizeof(c));
+        h.Bytes(&c, sizeof(c));
     }
+    h.Size(s.meshes.size());
     for (const TriangleMesh& m : s.meshes) {
-        mix(m.positions.data(), m.positions.size() * sizeof(Vec3f));
-        mix(m.normals.data(), m.normals.size() * sizeof(Vec3f));
-        mix(m.indices.data(), m.indices.size() * sizeof(uint32_t));
-        mix(&m.materialIndex, sizeof(m.materialIndex));
+        h.Vector(m.positions);
+        h.Vector(m.normals);
+        h.Vector(m.indices);
+        h.Bytes(&m.materialIndex, sizeof(m.materialIndex));
     }
-    for (const DirectionalLight& d : s.distantLights) mix(&d, sizeof(d));
+    h.Size(s.distantLights.size());
+    for (const DirectionalLight& d : s.distantLights) {
+        h.Bytes(&d.direction, sizeof(Vec3f));
+        h.Bytes(&d.radiance, sizeof(Vec3f));
+    }
+    h.Size(s.domeLights.size());
     for (const DomeLight& d : s.domeLights) {
-        mix(&d.radiance, sizeof(Vec3f));
-        mix(&d.right, sizeof(Vec3f) * 3);
-        const void* env = d.environment.get();
-        mix(&env, sizeof(env));
+        h.Bytes(&d.radiance, sizeof(Vec3f));
+        h.Bytes(&d.right, sizeof(Vec3f));
+        h.Bytes(&d.up, sizeof(Vec3f));
+        h.Bytes(&d.forward, sizeof(Vec3f));
+        const EnvironmentMap* env = d.environment.get();
+        if (env && !env->Pixels().empty()) {
+            const uint32_t dims[3] = {env->Width(), env->Height(),
+                                      env->Layout() == EnvironmentMapLayout::Angular ? 1u : 0u};
+            h.Bytes(dims, sizeof(dims));
+            h.Vector(env->Pixels());
+        } else {
+            h.Size(0);
+        }
     }
-    mix(&s.environmentTop, sizeof(Vec3f));
-    mix(&s.environmentBottom, sizeof(Vec3f));
-    return h;
+    h.Bytes(&s.environmentTop, sizeof(Vec3f));
+    h.Bytes(&s.environmentBottom, sizeof(Vec3f));
+    return h.Value();
 }
 
 // FNV-1a over the meshes' vertex counts and index arrays: scenes that agree
@@ -196,6 +236,12 @@ uint64_t Topology(const Scene& s) {
 }
 
 }  // namespace
+
+// Batches of at least this many samples (pixels x samples) render with the
+// wavefront transport, smaller ones with the megakernel, whose single
+// launch per slice has less fixed cost.  Both transports render the same
+// frame bit for bit, so the choice changes only the time.
+constexpr uint64_t kWavefrontMinSamples = uint64_t{1} << 18;
 
 struct HipBackend::RuntimeState {
     shiro_hip_ctx* ctx = nullptr;
@@ -240,31 +286,30 @@ bool HipBackend::SupportsScene(const Scene& scene) const {
     return shiro_hip_supports_scene(&view.scene) != 0;
 }
 
-FrameBuffer HipBackend::RenderSampleBatch(const RenderRequest& request) const {
-    std::scoped_lock lock(mutex_);
-    FrameBuffer frame;
-    frame.Resize(request.settings.width, request.settings.height);
-    frame.Clear();
-    if (request.sampleCount == 0 || !EnsureRuntime()) return frame;
-    const uint64_t sig = Signature(request.scene);
-    if (!runtime_->hasScene || sig != runtime_->signature) {
-        SceneView view;
-        BuildView(request.scene, &view);
-        // Moved vertices on unchanged meshes (deformation): refit the BVH;
-        // anything else, or a failed refit: upload and rebuild.
-        const uint64_t topo = Topology(request.scene);
-        const bool refitted = runtime_->hasScene && topo == runtime_->topology &&
-                              shiro_hip_refit_scene(runtime_->ctx, &view.scene) == SHIRO_HIP_OK;
-        if (!refitted && shiro_hip_upload_scene(runtime_->ctx, &view.scene) != SHIRO_HIP_OK) {
-            std::fprintf(stderr, "shiro hip upload failed: %s\n", shiro_hip_last_error(runtime_->ctx));
-            runtime_->hasScene = false;
-            return frame;
-        }
-        runtime_->signature = sig;
-        runtime_->topology = topo;
-        runtime_->hasScene = true;
+bool HipBackend::PrepareScene(const Scene& scene) const {
+    const uint64_t sig = Signature(scene);
+    if (runtime_->hasScene && sig == runtime_->signature) return true;
+    SceneView view;
+    BuildView(scene, &view);
+    // Moved vertices on unchanged meshes (deformation): refit the BVH;
+    // anything else, or a failed refit: upload and rebuild.
+    const uint64_t topo = Topology(scene);
+    const bool refitted = runtime_->hasScene && topo == runtime_->topology &&
+                          shiro_hip_refit_scene(runtime_->ctx, &view.scene) == SHIRO_HIP_OK;
+    if (!refitted && shiro_hip_upload_scene(runtime_->ctx, &view.scene) != SHIRO_HIP_OK) {
+        std::fprintf(stderr, "shiro hip upload failed: %s\n", shiro_hip_last_error(runtime_->ctx));
+        runtime_->hasScene = false;
+        return false;
     }
-    const Camera& c = request.camera;
+    runtime_->signature = sig;
+    runtime_->topology = topo;
+    runtime_->hasScene = true;
+    return true;
+}
+
+namespace {
+
+shiro_hip_camera ToCamera(const Camera& c) {
     shiro_hip_camera cam{};
     Set3(cam.position, c.position);
     Set3(cam.forward, c.forward);
@@ -272,7 +317,10 @@ FrameBuffer HipBackend::RenderSampleBatch(const RenderRequest& request) const {
     Set3(cam.up, c.up);
     cam.verticalFovDegrees = c.verticalFovDegrees;
     cam.aspectRatio = c.aspectRatio;
-    const RenderSettings& r = request.settings;
+    return cam;
+}
+
+shiro_hip_settings ToSettings(const RenderSettings& r, uint32_t sampleCount) {
     shiro_hip_settings st;
     shiro_hip_default_settings(&st);
     st.width = r.width;
@@ -289,29 +337,88 @@ FrameBuffer HipBackend::RenderSampleBatch(const RenderRequest& request) const {
     st.enableHeadlight = r.enableHeadlight ? 1u : 0u;
     st.enableCaustics = r.enableCaustics ? 1u : 0u;
     st.enableSss = r.enableSss ? 1u : 0u;
-    const size_t np = static_cast<size_t>(r.width) * r.height;
-    std::vector<float> beauty(np * 4), albedo(np * 3), normal(np * 3), depth(np);
-    shiro_hip_frame out{beauty.data(), albedo.data(), normal.data(), depth.data(), r.width, r.height};
-    // The library polls the request's std::atomic<bool> cancel flag between
-    // launches through its one-byte object representation.
+    const uint64_t samples = static_cast<uint64_t>(r.width) * r.height * sampleCount;
+    st.transport = samples >= kWavefrontMinSamples ? SHIRO_HIP_TRANSPORT_WAVEFRONT : SHIRO_HIP_TRANSPORT_MEGAKERNEL;
+    return st;
+}
+
+// Host AOV buffers in the ABI's layout, copied into a FrameBuffer through
+// its setters (the only way to fill one).
+struct HostFrame {
+    std::vector<float> beauty, albedo, normal, depth;
+    shiro_hip_frame view{};
+    HostFrame(uint32_t w, uint32_t h) {
+        const size_t np = static_cast<size_t>(w) * h;
+        beauty.resize(np * 4);
+        albedo.resize(np * 3);
+        normal.resize(np * 3);
+        depth.resize(np);
+        view = shiro_hip_frame{beauty.data(), albedo.data(), normal.data(), depth.data(), w, h};
+    }
+    void CopyTo(FrameBuffer* frame) const {
+        for (uint32_t y = 0; y < view.height; ++y) {
+            for (uint32_t x = 0; x < view.width; ++x) {
+                const size_t p = static_cast<size_t>(y) * view.width + x;
+                frame->SetBeauty(x, y, {beauty[4 * p], beauty[4 * p + 1], beauty[4 * p + 2], beauty[4 * p + 3]});
+                frame->SetAlbedo(x, y, {albedo[3 * p], albedo[3 * p + 1], albedo[3 * p + 2]});
+                frame->SetNormal(x, y, {normal[3 * p], normal[3 * p + 1], normal[3 * p + 2]});
+                frame->SetDepth(x, y, depth[p]);
+            }
+        }
+    }
+};
+
+// The library polls the request's std::atomic<bool> cancel flag between
+// launches through its one-byte object representation.
+const volatile uint8_t* CancelByte(const std::atomic<bool>* cancel) {
     static_assert(sizeof(std::atomic<bool>) == 1 && std::atomic<bool>::is_always_lock_free, "atomic<bool> layout");
-    const volatile uint8_t* cancel =
-        request.cancel ? reinterpret_cast<const volatile uint8_t*>(request.cancel) : nullptr;
-    const int rc = shiro_hip_render_batch(runtime_->ctx, &cam, &st, request.sampleStart, request.sampleCount, cancel,
-                                          &out);
+    return cancel ? reinterpret_cast<const volatile uint8_t*>(cancel) : nullptr;
+}
+
+}  // namespace
+
+FrameBuffer HipBackend::RenderSampleBatch(const RenderRequest& request) const {
+    std::scoped_lock lock(mutex_);
+    FrameBuffer frame;
+    frame.Resize(request.settings.width, request.settings.height);
+    frame.Clear();
+    if (request.sampleCount == 0 || !EnsureRuntime() || !PrepareScene(request.scene)) return frame;
+    const shiro_hip_camera cam = ToCamera(request.camera);
+    const shiro_hip_settings st = ToSettings(request.settings, request.sampleCount);
+    HostFrame out(st.width, st.height);
+    const int rc = shiro_hip_render_batch(runtime_->ctx, &cam, &st, request.sampleStart, request.sampleCount,
+                                          CancelByte(request.cancel), &out.view);
     if (rc != SHIRO_HIP_OK) {
-        std::fprintf(stderr, "shiro hip render failed: %s\n", shiro_hip_last_error(runtime_->ctx));
+        if (rc != SHIRO_HIP_CANCELLED)
+            std::fprintf(stderr, "shiro hip render failed: %s\n", shiro_hip_last_error(runtime_->ctx));
         return frame;
     }
     if (request.cancel && request.cancel->load(std::memory_order_relaxed)) return frame;
-    for (uint32_t y = 0; y < r.height; ++y) {
-        for (uint32_t x = 0; x < r.width; ++x) {
-            const size_t p = static_cast<size_t>(y) * r.width + x;
-            frame.SetBeauty(x, y, {beauty[4 * p], beauty[4 * p + 1], beauty[4 * p + 2], beauty[4 * p + 3]});
-            frame.SetAlbedo(x, y, {albedo[3 * p], albedo[3 * p + 1], albedo[3 * p + 2]});
-            frame.SetNormal(x, y, {normal[3 * p], normal[3 * p + 1], normal[3 * p + 2]});
-            frame.SetDepth(x, y, depth[p]);
-        }
+    out.CopyTo(&frame);
+    return frame;
+}
+
+FrameBuffer HipBackend::RenderProgressiveBatch(const RenderRequest& request, bool reset, bool resolve,
+                                               uint32_t* accumulatedSamples) const {
+    std::scoped_lock lock(mutex_);
+    FrameBuffer frame;
+    if (accumulatedSamples) *accumulatedSamples = 0;
+    if (!EnsureRuntime() || !PrepareScene(request.scene)) return frame;
+    const shiro_hip_camera cam = ToCamera(request.camera);
+    const shiro_hip_settings st = ToSettings(request.settings, request.sampleCount);
+    std::unique_ptr<HostFrame> out;
+    if (resolve) out = std::make_unique<HostFrame>(st.width, st.height);
+    const int rc = shiro_hip_render_progressive(runtime_->ctx, &cam, &st, request.sampleStart, request.sampleCount,
+                                                reset ? SHIRO_HIP_PROGRESSIVE_RESET : 0u, CancelByte(request.cancel),
+                                                out ? &out->view : nullptr, accumulatedSamples);
+    if (rc != SHIRO_HIP_OK) {
+        if (rc != SHIRO_HIP_CANCELLED)
+            std::fprintf(stderr, "shiro hip render failed: %s\n", shiro_hip_last_error(runtime_->ctx));
+        return frame;
+    }
+    if (out) {
+        frame.Resize(st.width, st.height);
+        out->CopyTo(&frame);
     }
     return frame;
 }

@@ -134,6 +134,8 @@ int hip_fail(shiro_hip_ctx* c, hipError_t e, const char* what) {
     return fail(c, code, std::string(what) + ": " + hipGetErrorString(e));
 }
 
+int accum_reset_locked(shiro_hip_ctx* ctx, uint32_t width, uint32_t height);
+
 #define HIPC(expr, what)                          \
     do {                                          \
         hipError_t e_ = (expr);                   \
@@ -848,6 +850,15 @@ int shiro_hip_render_batch_device(shiro_hip_ctx* ctx, const shiro_hip_camera* ca
 int shiro_hip_accum_reset(shiro_hip_ctx* ctx, uint32_t width, uint32_t height) {
     if (!ctx) return SHIRO_HIP_INVALID;
     std::lock_guard<std::mutex> lock(ctx->mu);
+    return accum_reset_locked(ctx, width, height);
+}
+
+}  // extern "C"
+
+namespace {
+
+// shiro_hip_accum_reset with ctx->mu held.
+int accum_reset_locked(shiro_hip_ctx* ctx, uint32_t width, uint32_t height) {
     HIPC(hipSetDevice(ctx->device), "set device");
     const uint64_t np = static_cast<uint64_t>(width) * height;
     HIPC(ctx->acB.reserve(std::max<uint64_t>(np * 16, 16)), "alloc accumulator");
@@ -866,6 +877,10 @@ int shiro_hip_accum_reset(shiro_hip_ctx* ctx, uint32_t width, uint32_t height) {
     ctx->acTotal = 0;
     return SHIRO_HIP_OK;
 }
+
+}  // namespace
+
+extern "C" {
 
 int shiro_hip_accum_add(shiro_hip_ctx* ctx, const shiro_hip_frame* batch, uint32_t sampleCount) {
     if (!ctx) return SHIRO_HIP_INVALID;
@@ -893,6 +908,60 @@ int shiro_hip_accum_resolve(shiro_hip_ctx* ctx, shiro_hip_frame* out) {
     const DevFrame f{out->beauty, out->albedo, out->normal, out->depth};
     HIPC(launch_accum_resolve(A, f, ctx->acTotal, ctx->acW * ctx->acH, ctx->stream), "resolve");
     HIPC(hipStreamSynchronize(ctx->stream), "resolve");
+    return SHIRO_HIP_OK;
+}
+
+int shiro_hip_render_progressive(shiro_hip_ctx* ctx, const shiro_hip_camera* camera,
+                                 const shiro_hip_settings* settings, uint32_t sampleStart, uint32_t sampleCount,
+                                 uint32_t flags, const volatile uint8_t* cancel, shiro_hip_frame* resolved,
+                                 uint32_t* accumulatedSamples) {
+    if (!ctx) return SHIRO_HIP_INVALID;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    if (!camera) return fail(ctx, SHIRO_HIP_INVALID, "null camera");
+    if (resolved && (!resolved->beauty || !resolved->albedo || !resolved->normal || !resolved->depth))
+        return fail(ctx, SHIRO_HIP_INVALID, "null frame buffer");
+    int rc = check_settings(ctx, settings);
+    if (rc) return rc;
+    if (resolved) clear_host_frame(*settings, resolved);
+    if (!ctx->hasScene) return fail(ctx, SHIRO_HIP_NO_SCENE, "no scene uploaded");
+    HIPC(hipSetDevice(ctx->device), "set device");
+    const uint32_t w = settings->width, h = settings->height;
+    const uint64_t np = static_cast<uint64_t>(w) * h;
+    if ((flags & SHIRO_HIP_PROGRESSIVE_RESET) || w != ctx->acW || h != ctx->acH || !ctx->acB.p) {
+        rc = accum_reset_locked(ctx, w, h);
+        if (rc) return rc;
+    }
+    HIPC(ctx->frB.reserve(np * 16), "alloc frame");
+    HIPC(ctx->frA.reserve(np * 12), "alloc frame");
+    HIPC(ctx->frN.reserve(np * 12), "alloc frame");
+    HIPC(ctx->frD.reserve(np * 4), "alloc frame");
+    const DevFrame f{ctx->frB.as<float>(), ctx->frA.as<float>(), ctx->frN.as<float>(), ctx->frD.as<float>()};
+    const AccumParams A{ctx->acB.as<float>(), ctx->acA.as<float>(), ctx->acN.as<float>(), ctx->acD.as<float>(),
+                        ctx->acC.as<uint32_t>()};
+    if (sampleCount > 0) {
+        rc = enqueue_batch(ctx, *camera, *settings, sampleStart, sampleCount, f, false, cancel);
+        if (rc) {
+            if (resolved) clear_host_frame(*settings, resolved);
+            return rc;
+        }
+        if (cancel && *cancel) {
+            (void)hipStreamSynchronize(ctx->stream);
+            return fail(ctx, SHIRO_HIP_CANCELLED, "cancelled");
+        }
+        HIPC(launch_accum_add(A, f, sampleCount, static_cast<uint32_t>(np), ctx->stream), "accumulate");
+        ctx->acTotal += sampleCount;
+        rc = finish_timing(ctx);
+        if (rc) return rc;
+    }
+    if (accumulatedSamples) *accumulatedSamples = ctx->acTotal;
+    if (!resolved) return SHIRO_HIP_OK;
+    // The batch frame is consumed: resolve into it and read it back.
+    HIPC(launch_accum_resolve(A, f, ctx->acTotal, static_cast<uint32_t>(np), ctx->stream), "resolve");
+    HIPC(hipMemcpyAsync(resolved->beauty, f.beauty, np * 16, hipMemcpyDeviceToHost, ctx->stream), "readback");
+    HIPC(hipMemcpyAsync(resolved->albedo, f.albedo, np * 12, hipMemcpyDeviceToHost, ctx->stream), "readback");
+    HIPC(hipMemcpyAsync(resolved->normal, f.normal, np * 12, hipMemcpyDeviceToHost, ctx->stream), "readback");
+    HIPC(hipMemcpyAsync(resolved->depth, f.depth, np * 4, hipMemcpyDeviceToHost, ctx->stream), "readback");
+    HIPC(hipStreamSynchronize(ctx->stream), "readback");
     return SHIRO_HIP_OK;
 }
 

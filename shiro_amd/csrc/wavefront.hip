@@ -766,9 +766,16 @@ __global__ __launch_bounds__(64) void k_wf_accum(WfParams P) {
         normal = P.sums.normal[pix];
         depth = P.sums.depth[pix];
     }
+    // maxDepth 0: no stage of the pass ran, every sample is the empty path
+    // (radiance, albedo and normal 0, depth +inf: SampleResult's defaults,
+    // CpuPathTracer.cpp:108-113), as the megakernel's init_path leaves it.
+    const bool empty = st.maxDepth == 0u;
     for (uint32_t s = 0; s < P.sliceCount; ++s) {
         const uint32_t p = path_index(P, s, tile, lane & 7u, lane >> 3);
-        const float4 L = P.B.L[p], A = P.B.A[p], N = P.B.N[p];
+        const float4 zero = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        const float4 L = empty ? zero : P.B.L[p];
+        const float4 A = empty ? make_float4(0.0f, 0.0f, 0.0f, kInf) : P.B.A[p];
+        const float4 N = empty ? zero : P.B.N[p];
         beauty.x = beauty.x + L.x;
         beauty.y = beauty.y + L.y;
         beauty.z = beauty.z + L.z;

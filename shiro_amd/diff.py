@@ -31,28 +31,34 @@ class _RenderTextures(torch.autograd.Function):
     @staticmethod
     def forward(ctx, renderer, sample_start, n_tex, *tensors):
         r = renderer
-        texels, envs = tensors[:n_tex], tensors[n_tex:]
-        torch.cuda.current_stream().synchronize()
-        for i, t in zip(r.indices, texels):
-            r.ctx.update_texture(i, t.detach().contiguous().data_ptr())
-        for d, t in zip(r.env_domes, envs):
-            r.ctx.update_environment(d, t.detach().contiguous().data_ptr())
+        r._load(tensors[:n_tex], tensors[n_tex:])
         st = r.settings
         r.ctx.render_device(r.camera, st, [b.data_ptr() for b in r.frame], sample_start, st.samplesPerPixel,
                             synchronize=True)
+        # The adjoint replays the paths at the parameters of this forward
+        # pass: keep them, and the version of the context's contents they
+        # were uploaded as.  Autograd's version counters reject a backward
+        # after an in-place change of a saved tensor.
+        ctx.save_for_backward(*tensors)
         ctx.renderer = r
+        ctx.version = r.version
         ctx.sample_start = sample_start
         ctx.n_tex = n_tex
-        ctx.shapes = [t.shape for t in tensors]
         return r.frame[0].view(st.height, st.width, 4)[..., :3].clone()
 
     @staticmethod
     def backward(ctx, grad_out):
         r = ctx.renderer
         st = r.settings
+        saved = ctx.saved_tensors
+        if r.version != ctx.version:
+            # Another forward pass loaded other parameters since: put this
+            # pass's texels and maps back (the maps' importance tables are
+            # rebuilt on the device, as a fresh upload builds them).
+            r._load(saved[:ctx.n_tex], saved[ctx.n_tex:], version=ctx.version)
         g = torch.zeros(st.height, st.width, 4, dtype=torch.float32, device=grad_out.device)
         g[..., :3] = grad_out
-        grads = [torch.zeros(s, dtype=torch.float32, device=grad_out.device) for s in ctx.shapes]
+        grads = [torch.zeros(t.shape, dtype=torch.float32, device=grad_out.device) for t in saved]
         tex_grads, env_grads = grads[:ctx.n_tex], grads[ctx.n_tex:]
         ptrs = [0] * (max(r.indices) + 1 if r.indices else 0)
         for i, gt in zip(r.indices, tex_grads):
@@ -82,6 +88,24 @@ class TextureRenderer:
         n = settings.width * settings.height
         dev = torch.device("cuda", device)
         self.frame = [torch.empty(n * k, dtype=torch.float32, device=dev) for k in (4, 3, 3, 1)]
+        # Version of the parameters loaded into the context: every forward
+        # pass loads its own and takes a new number.
+        self.version = 0
+        self._versions = 0
+
+    def _load(self, texels, envs, version=None):
+        """Uploads texel and map tensors into the context (synchronously)."""
+        torch.cuda.current_stream().synchronize()
+        for i, t in zip(self.indices, texels):
+            t = t.detach().contiguous()
+            self.ctx.update_texture(i, t.data_ptr())
+        for d, t in zip(self.env_domes, envs):
+            t = t.detach().contiguous()
+            self.ctx.update_environment(d, t.data_ptr())
+        if version is None:
+            self._versions += 1
+            version = self._versions
+        self.version = version
 
     def render(self, *texels, env=(), sample_start: int = 0) -> torch.Tensor:
         if len(texels) != len(self.indices) or len(env) != len(self.env_domes):

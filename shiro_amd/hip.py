@@ -72,6 +72,8 @@ def load_library(path: str = LIB_PATH):
     d("shiro_hip_accum_reset", C.c_int, V, _u, _u)
     d("shiro_hip_accum_add", C.c_int, V, _P(abi.Frame), _u)
     d("shiro_hip_accum_resolve", C.c_int, V, _P(abi.Frame))
+    d("shiro_hip_render_progressive", C.c_int, V, _P(abi.Camera), _P(abi.Settings), _u, _u, _u, _P(C.c_uint8),
+      _P(abi.Frame), _P(C.c_uint32))
     d("shiro_hip_stream", V, V)
     d("shiro_hip_synchronize", C.c_int, V)
     d("shiro_hip_last_render_ms", C.c_int, V, _P(C.c_float))
@@ -348,6 +350,24 @@ class Context:
         f.width, f.height = w, h
         self._check(self.L.shiro_hip_accum_resolve(self.h, C.byref(f)))
 
+    def render_progressive(self, camera: Camera, settings: RenderSettings, sample_start, sample_count,
+                           reset=False, resolve=True, cancel=None):
+        """One progressive batch with the accumulator on the device
+        (shiro_hip_render_progressive): returns (resolved FrameBuffer or None,
+        accumulated sample count)."""
+        fb = FrameBuffer(settings.width, settings.height) if resolve else None
+        f = fb.to_c() if fb is not None else None
+        n = C.c_uint32(0)
+        cam, st = camera.to_c(), settings.to_c()
+        rc = self.L.shiro_hip_render_progressive(self.h, C.byref(cam), C.byref(st), sample_start, sample_count,
+                                                 abi.PROGRESSIVE_RESET if reset else 0, cancel,
+                                                 C.byref(f) if f is not None else None, C.byref(n))
+        if fb is not None:
+            fb.rc = rc
+        if rc not in (0, 3):
+            self._check(rc)
+        return fb, n.value
+
     # ---- single stages ----
     def qmc_fill(self, x0, y0, w, h, sample, dim0, dim_count) -> np.ndarray:
         out = np.zeros(w * h * dim_count, np.float32)
@@ -534,6 +554,17 @@ class HipBackend:
         self._ctx: Optional[Context] = None
         self._attempted = False
 
+    # Batches of at least this many samples (pixels x samples) render with
+    # the wavefront transport, smaller ones with the megakernel (the C++
+    # adapter's rule); both render the same frame bit for bit.
+    WAVEFRONT_MIN_SAMPLES = 1 << 18
+
+    @classmethod
+    def _settings(cls, st: RenderSettings, count: int) -> RenderSettings:
+        import dataclasses
+        big = st.width * st.height * count >= cls.WAVEFRONT_MIN_SAMPLES
+        return dataclasses.replace(st, transport=abi.TRANSPORT_WAVEFRONT if big else abi.TRANSPORT_MEGAKERNEL)
+
     def _runtime(self) -> Optional[Context]:
         if self._ctx is None and not self._attempted:
             self._attempted = True
@@ -575,7 +606,8 @@ class HipBackend:
                 if not refitted:
                     ctx.upload(request.scene, force=True)
                 ctx._topo = topo
-            return ctx.render(request.camera, st, request.sampleStart, request.sampleCount, request.cancel)
+            return ctx.render(request.camera, self._settings(st, request.sampleCount), request.sampleStart,
+                              request.sampleCount, request.cancel)
         except HipError as e:
             import sys
             print(f"shiro_hip: {e}", file=sys.stderr)

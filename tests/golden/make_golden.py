@@ -26,8 +26,11 @@ so the test detects a change of the workload generator.
 
     python tests/golden/make_golden.py [section ...]
 
-Sections: qmc bsdf env env_angular images accumulated, or img:<name> for
-one image fixture; all sections when none is named.
+Sections: qmc bsdf env env_angular images features accumulated, or
+img:<name> for one image fixture; all sections when none is named.
+`features` renders the scenes of golden_cases.feature_workloads (opacity,
+delta glass and mirrors, thin-walled panes, absorbing media, normal
+overrides).
 """
 import ctypes as C
 import hashlib
@@ -84,10 +87,10 @@ def main(sections):
         write_env("env_angular.npz", Scene(domeLights=[cases.angular_dome()]))
     for name, wl in cases.image_workloads().items():
         if want("images") or want(f"img:{name}"):
-            print("rendering", name, wl.settings.width, wl.settings.height, wl.settings.samplesPerPixel, flush=True)
-            fb = ReferenceScene(wl.scene).render(wl.camera, wl.settings)
-            np.savez_compressed(os.path.join(HERE, f"img_{name}.npz"), beauty=fb.beauty, albedo=fb.albedo,
-                                normal=fb.normal, depth=fb.depth, scene_hash=scene_hash(wl.scene))
+            render_image(name, wl)
+    for name, wl in cases.feature_workloads().items():
+        if want("features") or want(f"img:{name}"):
+            render_image(name, wl)
     if want("accumulated"):
         name, wl, batches = cases.accumulated_workload()
         fb = ReferenceScene(wl.scene).render_accumulated(wl.camera, wl.settings, batches)
@@ -95,6 +98,13 @@ def main(sections):
                             normal=fb.normal, depth=fb.depth, scene_hash=scene_hash(wl.scene),
                             batches=np.array(batches, np.uint32))
     print("done")
+
+
+def render_image(name, wl):
+    print("rendering", name, wl.settings.width, wl.settings.height, wl.settings.samplesPerPixel, flush=True)
+    fb = ReferenceScene(wl.scene).render(wl.camera, wl.settings)
+    np.savez_compressed(os.path.join(HERE, f"img_{name}.npz"), beauty=fb.beauty, albedo=fb.albedo,
+                        normal=fb.normal, depth=fb.depth, scene_hash=scene_hash(wl.scene))
 
 
 def write_qmc(f):

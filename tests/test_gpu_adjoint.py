@@ -355,3 +355,54 @@ def test_adam_with_the_dome_map_lowers_the_loss():
     _log(f"loss {losses[0]:.4e} -> {losses[-1]:.4e}")
     assert np.isfinite(losses).all()
     assert losses[-1] < 0.75 * losses[0]
+
+
+def test_two_forwards_then_one_backward():
+    """Two render() calls at different parameters before one backward():
+    each image's gradient is taken at its own parameters (the forward pass
+    keeps them and the backward pass loads them again when another forward
+    pass has replaced them).  The summed gradient equals that of two
+    separate forward/backward pairs (relative L2 <= 1e-5: the adjoint's
+    float atomics may sum in another order), while the gradient of the first
+    image at the second image's parameters is far from it."""
+    import torch
+    from shiro_amd.diff import TextureRenderer
+    wl = c5_mini(size=48, spp=4)
+    alb_t, rough_t = wl.textures_truth
+    alb0, rough0 = wl.textures_init
+    _log("two forwards, one backward")
+    r = TextureRenderer(c5_scene_with(wl, alb_t, rough_t), wl.camera, wl.settings, env_domes=[0])
+    env_t = torch.from_numpy(wl.env_truth).cuda()
+    with torch.no_grad():
+        target = r.render(torch.from_numpy(alb_t).cuda(), torch.from_numpy(rough_t).cuda(), env=[env_t]).clone()
+    pa = [torch.from_numpy(alb0.copy()).cuda(), torch.from_numpy(rough0.copy()).cuda(),
+          torch.from_numpy(wl.env_init.copy()).cuda()]
+    pb = [torch.clamp(pa[0] * 0.7 + 0.2, 0, 1), torch.clamp(pa[1] * 0.5 + 0.3, 0.05, 1), pa[2] * 1.5]
+
+    def leaf(ps):
+        return [p.clone().requires_grad_(True) for p in ps]
+
+    def loss_of(ps):
+        img = r.render(ps[0], ps[1], env=[ps[2]])
+        return ((img - target) ** 2).mean()
+
+    a, b = leaf(pa), leaf(pb)
+    (loss_of(a) + loss_of(b)).backward()
+    a1, b1 = leaf(pa), leaf(pb)
+    loss_of(a1).backward()
+    loss_of(b1).backward()
+    # The old hazard: the first image's adjoint at the second's parameters.
+    a2, b2 = leaf(pa), leaf(pb)
+    la = loss_of(a2)
+    v = r.version
+    with torch.no_grad():
+        r._load([b2[0], b2[1]], [b2[2]])
+    r.version = v  # the context holds b's parameters but claims a's version
+    la.backward()
+    for k in range(3):
+        for x, y in ((a[k].grad, a1[k].grad), (b[k].grad, b1[k].grad)):
+            e = float(torch.linalg.norm(x - y) / torch.linalg.norm(y))
+            assert e <= 1e-5, (k, e)
+        wrong = float(torch.linalg.norm(a2[k].grad - a1[k].grad) / torch.linalg.norm(a1[k].grad))
+        _log(f"parameter {k}: gradient at the other pass's parameters is off by {wrong:.2e}")
+        assert wrong > 1e-2

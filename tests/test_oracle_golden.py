@@ -99,11 +99,14 @@ def _scene_hash(scene):
     return mg.scene_hash(scene)
 
 
-@pytest.mark.parametrize("name", ["c1", "c2", "c3", "emissive", "angular"])
+FEATURES = ["opacity", "delta", "thin", "medium", "overrides"]
+
+
+@pytest.mark.parametrize("name", ["c1", "c2", "c3", "emissive", "angular"] + FEATURES)
 @pytest.mark.parametrize("bvh", [False, True], ids=["brute", "bvh"])
 def test_images_bit_exact(name, bvh):
     z = _load(f"img_{name}.npz")
-    wl = cases.image_workloads()[name]
+    wl = {**cases.image_workloads(), **cases.feature_workloads()}[name]
     assert _scene_hash(wl.scene) == str(z["scene_hash"]), "workload generator changed"
     fb = OracleScene(wl.scene, bvh=bvh).render(wl.camera, wl.settings)
     for k in ("beauty", "albedo", "normal", "depth"):
@@ -145,3 +148,33 @@ def test_path_guide_never_activates():
     fresh = OracleScene(wl.scene, bvh=True).render(wl.camera, wl.settings, batches[0], batches[1])
     for k in ("beauty", "albedo", "normal", "depth"):
         assert _same(getattr(second, k), getattr(fresh, k)), k
+
+
+# Which feature fixture each planted mistake of the oracle must break.
+PLANTED = [
+    ("opacity_no_skip", "opacity"), ("opacity_no_rescale", "opacity"),
+    ("delta_as_rough", "delta"), ("delta_as_rough", "medium"), ("mirror_not_delta", "delta"),
+    ("pane_no_denominator", "thin"),
+    ("extinction_sign", "medium"), ("no_attenuation", "medium"), ("tint_in_medium", "medium"),
+    ("ignore_normal", "overrides"), ("ignore_coat_normal", "overrides"), ("no_override_flip", "overrides"),
+]
+
+
+@pytest.mark.parametrize("plant,name", PLANTED, ids=[f"{p}-{n}" for p, n in PLANTED])
+def test_feature_fixtures_reach_their_branch(plant, name):
+    """Each feature fixture reaches the branch it is written for: with one
+    plausible slip planted in the oracle (a sign, a dropped term, a skipped
+    branch), the oracle no longer reproduces the reference's frame, and the
+    difference is beyond relMSE 1e-4 -- the GPU images are held to 1e-3, and
+    observed within 1e-6, of the oracle."""
+    from helpers import relmse
+    from oracle.bindings import planted
+    z = _load(f"img_{name}.npz")
+    wl = cases.feature_workloads()[name]
+    with planted(plant):
+        fb = OracleScene(wl.scene, bvh=True).render(wl.camera, wl.settings)
+    e = relmse(fb.beauty[:, :3], z["beauty"][:, :3])
+    assert e > 1e-4, (plant, name, e)
+    # Switched off again, the oracle matches the fixture bit for bit.
+    fb = OracleScene(wl.scene, bvh=True).render(wl.camera, wl.settings)
+    assert _same(fb.beauty, z["beauty"])
