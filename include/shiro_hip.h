@@ -304,6 +304,30 @@ int shiro_hip_render_batch_device(
     shiro_hip_frame* out,
     int synchronize);
 
+/* No counterpart in the reference: shiro_hip_render_batch_device for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Renders with the
+ * wavefront transport (the same frame, bit for bit, as either transport of
+ * shiro_hip_render_batch_device) and calls no hipMalloc or hipFree: the path
+ * state, queues, counters and running sums of its passes live in the
+ * library's scratch array, 256 MiB of device memory per GPU (held from the
+ * first context on that GPU until the process ends), and a frame whose
+ * state does not fit is covered in passes over ranges of 8x8 tiles.  The
+ * scene must be uploaded (upload allocates) and `out` holds device
+ * pointers.  Every call on one GPU shares that GPU's array: calls that
+ * overlap on one GPU (from two contexts or two threads' streams) race, so
+ * serialise them.  The counters are cleared by a kernel at every pass, so a
+ * call does not depend on what an earlier call left there.  Returns
+ * SHIRO_HIP_INVALID when not even one tile of path state fits the array. */
+int shiro_hip_render_batch_device_noalloc(
+    shiro_hip_ctx* ctx,
+    const shiro_hip_camera* camera,
+    const shiro_hip_settings* settings,
+    uint32_t sampleStart,
+    uint32_t sampleCount,
+    shiro_hip_frame* out,
+    int synchronize);
+
 /* Device-side FrameAccumulator (Renderer.cpp:47-110).  accumulate adds the
  * batch frame held in device buffers `batch` with weight sampleCount;
  * resolve writes the resolved frame into device buffers `out`. */

@@ -183,7 +183,7 @@ EXPORTS = [
     "shiro_hip_default_settings", "shiro_hip_default_material",
     "shiro_hip_create", "shiro_hip_destroy", "shiro_hip_last_error",
     "shiro_hip_get_status", "shiro_hip_supports_scene", "shiro_hip_upload_scene",
-    "shiro_hip_render_batch", "shiro_hip_render_batch_device",
+    "shiro_hip_render_batch", "shiro_hip_render_batch_device", "shiro_hip_render_batch_device_noalloc",
     "shiro_hip_accum_reset", "shiro_hip_accum_add", "shiro_hip_accum_resolve", "shiro_hip_render_progressive",
     "shiro_hip_stream", "shiro_hip_synchronize", "shiro_hip_last_render_ms",
     "shiro_hip_scene_info_get", "shiro_hip_qmc_fill", "shiro_hip_intersect",

@@ -120,6 +120,10 @@ struct shiro_hip_ctx {
     std::vector<uint8_t> stageKind;
     StageMarks marks;
     shiro_hip_stage_times stageTimes{};
+    // Device address of this device's scratch array (scratch_address), null
+    // if the lookup failed (scratchError says why).
+    void* scratch = nullptr;
+    std::string scratchError;
 };
 
 namespace {
@@ -196,13 +200,16 @@ int check_settings(shiro_hip_ctx* ctx, const shiro_hip_settings* s) {
     return SHIRO_HIP_OK;
 }
 
-// Wavefront form of one batch: passes of S samples for every pixel, S
-// chosen so that a pass holds about tuning.pathsPerPass paths (and at most
-// kWavefrontStateBytes of path state).  Large passes pay: the last bounces
-// of a pass hold few paths and run latency-bound on a mostly idle GPU, so
-// fewer passes means fewer such tails (c3: 4M -> 64M paths per pass cuts a
-// frame from 245 to 189 ms; 64M -> 128M, one pass for the 16-spp frame
-// instead of two, 122 -> 114 ms).
+// Wavefront form of one batch: passes of S samples for the pixels of a
+// range of T 8x8 tiles, S chosen so that a pass holds about
+// tuning.pathsPerPass paths; T is the whole frame unless the path state of
+// one pass would exceed the memory budget (kWavefrontStateBytes, or the
+// scratch array for the entry points that allocate nothing), in which case
+// the frame is covered range by range, all samples of a range before the
+// next.  Large passes pay: the last bounces of a pass hold few paths and run
+// latency-bound on a mostly idle GPU, so fewer passes means fewer such tails
+// (c3: 4M -> 64M paths per pass cuts a frame from 245 to 189 ms; 64M ->
+// 128M, one pass for the 16-spp frame instead of two, 122 -> 114 ms).
 constexpr uint64_t kWavefrontStateBytes = 32ull << 30;
 
 // Starts the event list of a render call in stage-timing mode: the first
@@ -216,78 +223,128 @@ void begin_marks(shiro_hip_ctx* ctx) {
     ctx->marks.mark(0xFF, ctx->stream);
 }
 
+// Where the wavefront's arrays live: the context's buffers (grown on
+// demand) or consecutive 256-byte aligned pieces of the scratch array.
+struct WfMemory {
+    unsigned char* scratch = nullptr;  // null: the context's buffers
+    size_t scratchBytes = 0;
+    size_t used = 0;
+    // One array of n bytes: `own` grown to n, or the next scratch piece.
+    hipError_t get(DevBuf& own, size_t n, void** out) {
+        if (!scratch) {
+            hipError_t e = own.reserve(std::max<size_t>(n, 16));
+            *out = own.p;
+            return e;
+        }
+        const size_t at = (used + 255u) & ~size_t{255};
+        if (at + n > scratchBytes) return hipErrorOutOfMemory;
+        *out = scratch + at;
+        used = at + n;
+        return hipSuccess;
+    }
+    // Bytes the scratch pieces take for the given sizes (with alignment).
+    static size_t footprint(const size_t* n, size_t k) {
+        size_t t = 0;
+        for (size_t i = 0; i < k; ++i) t = ((t + 255u) & ~size_t{255}) + n[i];
+        return t;
+    }
+};
+
+// Sizes of a pass's arrays for cap paths, M light samples per hit, the
+// shading-class order on or off, sum slots and counter slots.  Order:
+// o d T L E A N H so st | sr | qa qb qs | qsr svis | qkey qsorted blockCount
+// blockStart | counters | sums beauty albedo normal depth.
+constexpr size_t kWfArrays = 25;
+void wavefront_sizes(uint64_t cap, uint32_t M, bool order, uint64_t sumSlots, uint32_t slots, size_t* n) {
+    for (int i = 0; i < 10; ++i) n[i] = cap * 16;
+    n[10] = cap * M * 32;
+    n[11] = n[12] = n[13] = cap * 4;
+    n[14] = n[15] = cap * M * 4;
+    n[16] = order ? cap : 0;
+    n[17] = order ? cap * 4 : 0;
+    n[18] = order ? kShadeClasses * kOrderBlocks * 4 : 0;
+    n[19] = order ? (kShadeClasses + 1u) * kOrderBlocks * 4 : 0;
+    n[20] = std::max<size_t>(sizeof(DevCounters), slots * 4u);
+    n[21] = n[22] = n[23] = sumSlots * 16;
+    n[24] = sumSlots * 4;
+}
+
 // `scene` is ctx's own scene except for batched renders; `events` records
-// ev0 / ev1 around the passes.
+// ev0 / ev1 around the passes; a non-null `scratch` (kScratchBytes) holds
+// every array instead of the context's buffers.
 int enqueue_wavefront(shiro_hip_ctx* ctx, const DevScene& scene, const shiro_hip_camera& cam,
                       const shiro_hip_settings& st, uint32_t start, uint32_t count, const DevFrame& frame,
-                      const volatile uint8_t* cancel, bool events = true) {
-    const uint64_t np = static_cast<uint64_t>(st.width) * st.height;
+                      const volatile uint8_t* cancel, bool events = true, void* scratch = nullptr) {
     const uint32_t tilesX = (st.width + 7u) / 8u, tilesY = (st.height + 7u) / 8u;
-    const uint64_t perSample = static_cast<uint64_t>(tilesX) * tilesY * 64u;
+    const uint32_t tilesTotal = tilesX * tilesY;
+    const uint64_t perSample = static_cast<uint64_t>(tilesTotal) * 64u;
     const uint32_t K = st.domeLightSamples > 1u ? st.domeLightSamples : 1u;
     // Light samples per hit: the device's direct_sample_count (dev_path.h).
     const uint32_t lights = scene.dirCount + scene.domeCount * K +
                             (scene.emissiveCount > 0u && scene.emissiveSum > 0.0f ? 1u : 0u);
     const uint32_t M = std::max(1u, lights);
-    const uint64_t bytesPerPath = 10u * 16u + 12u + static_cast<uint64_t>(M) * 32u;
-    uint64_t S = std::max<uint64_t>(1, ctx->tuning.pathsPerPass / perSample);
-    S = std::min<uint64_t>(S, std::max<uint64_t>(1, kWavefrontStateBytes / (bytesPerPath * perSample)));
-    S = std::min<uint64_t>(S, count);
-    const uint64_t cap = S * perSample;
-    if (cap >= (1ull << 32) / std::max<uint64_t>(1, 2ull * M))
-        return fail(ctx, SHIRO_HIP_INVALID, "frame too large for the wavefront path state");
-    for (int i = 0; i < 8; ++i) HIPC(ctx->wf[i].reserve(cap * 16), "alloc wavefront state");
-    HIPC(ctx->wf[8].reserve(cap * 16), "alloc wavefront state");
-    HIPC(ctx->wf[9].reserve(cap * 16), "alloc wavefront state");
-    HIPC(ctx->wf[10].reserve(cap * M * 32), "alloc wavefront state");
-    HIPC(ctx->wf[11].reserve(cap * 4), "alloc wavefront queues");
-    HIPC(ctx->wf[12].reserve(cap * 4), "alloc wavefront queues");
-    HIPC(ctx->wf[13].reserve(cap * 4), "alloc wavefront queues");
-    HIPC(ctx->wf[14].reserve(cap * M * 4), "alloc wavefront queues");
-    HIPC(ctx->wf[15].reserve(cap * M * 4), "alloc wavefront queues");
     const bool order = ctx->tuning.shadeOrder != 0u;
-    if (order) {
-        HIPC(ctx->wf[16].reserve(cap), "alloc wavefront queues");
-        HIPC(ctx->wf[17].reserve(cap * 4), "alloc wavefront queues");
-        HIPC(ctx->wf[18].reserve(kShadeClasses * kOrderBlocks * 4), "alloc wavefront queues");
-        HIPC(ctx->wf[19].reserve((kShadeClasses + 1u) * kOrderBlocks * 4), "alloc wavefront queues");
-    }
     const uint32_t shadowRayBase = st.maxDepth + 1u;
     const uint32_t shadowPathBase = shadowRayBase + st.maxDepth;
     // The striped work counters start on a 128-byte line.
     const uint32_t fetchBase = (shadowPathBase + st.maxDepth + kPoolStride - 1u) / kPoolStride * kPoolStride;
     const uint32_t slots = fetchBase + 2u * st.maxDepth * kPoolStripes * kPoolStride;
-    HIPC(ctx->counters.reserve(std::max<size_t>(sizeof(DevCounters), slots * 4u)), "alloc counters");
-    if (S < count) {
-        HIPC(ctx->sumB.reserve(np * 16), "alloc sums");
-        HIPC(ctx->sumA.reserve(np * 16), "alloc sums");
-        HIPC(ctx->sumN.reserve(np * 16), "alloc sums");
-        HIPC(ctx->sumD.reserve(np * 4), "alloc sums");
+    // Pass plan: S samples per pass, T tiles per range.
+    const uint64_t budget = scratch ? kScratchBytes : kWavefrontStateBytes;
+    const uint64_t maxCap = (1ull << 32) / (2ull * M) - 1u;  // queue entries path * M + index stay below 2^32
+    size_t n[kWfArrays];
+    auto bytes_for = [&](uint64_t S, uint64_t T) {
+        wavefront_sizes(S * T * 64u, M, order, S < count ? T * 64u : 0u, slots, n);
+        return WfMemory::footprint(n, kWfArrays);
+    };
+    auto fits = [&](uint64_t S, uint64_t T) { return bytes_for(S, T) <= budget && S * T * 64u <= maxCap; };
+    // Largest x in [1, hi] with ok(x), or 0 (ok is monotone).
+    auto largest = [](uint64_t hi, const auto& ok) {
+        uint64_t lo = 0;
+        while (lo < hi) {
+            const uint64_t mid = lo + (hi - lo + 1) / 2;
+            if (ok(mid)) lo = mid;
+            else hi = mid - 1;
+        }
+        return lo;
+    };
+    uint64_t S = std::min<uint64_t>(count, std::max<uint64_t>(1, ctx->tuning.pathsPerPass / perSample));
+    uint64_t T = tilesTotal;
+    if (!fits(S, T)) S = std::max<uint64_t>(1, largest(S, [&](uint64_t x) { return fits(x, T); }));
+    if (!fits(S, T)) {
+        // One sample of the whole frame does not fit: ranges of tiles.
+        T = largest(tilesTotal, [&](uint64_t x) { return fits(S, x); });
+        if (T == 0)
+            return fail(ctx, SHIRO_HIP_INVALID,
+                        "one tile of wavefront path state (" + std::to_string(bytes_for(1, 1)) +
+                            " bytes) exceeds the " + std::to_string(budget) + "-byte memory budget");
     }
+    const uint64_t cap = S * T * 64u;
+    wavefront_sizes(cap, M, order, S < count ? T * 64u : 0u, slots, n);
+    WfMemory mem{static_cast<unsigned char*>(scratch), scratch ? kScratchBytes : 0};
+    void* a[kWfArrays];
+    for (size_t i = 0; i < 20; ++i) HIPC(mem.get(ctx->wf[i], n[i], &a[i]), "alloc wavefront state");
+    HIPC(mem.get(ctx->counters, n[20], &a[20]), "alloc counters");
+    HIPC(mem.get(ctx->sumB, n[21], &a[21]), "alloc sums");
+    HIPC(mem.get(ctx->sumA, n[22], &a[22]), "alloc sums");
+    HIPC(mem.get(ctx->sumN, n[23], &a[23]), "alloc sums");
+    HIPC(mem.get(ctx->sumD, n[24], &a[24]), "alloc sums");
     WfParams P{};
     P.scene = scene;
     P.camera = to_dev(cam);
     P.settings = to_dev(st);
-    P.sums = DevBatchSums{ctx->sumB.as<float4>(), ctx->sumA.as<float4>(), ctx->sumN.as<float4>(), ctx->sumD.as<float>()};
+    P.sums = DevBatchSums{static_cast<float4*>(a[21]), static_cast<float4*>(a[22]), static_cast<float4*>(a[23]),
+                          static_cast<float*>(a[24])};
     P.frame = frame;
     WfBuffers& B = P.B;
-    B.o = ctx->wf[0].as<float4>();
-    B.d = ctx->wf[1].as<float4>();
-    B.T = ctx->wf[2].as<float4>();
-    B.L = ctx->wf[3].as<float4>();
-    B.E = ctx->wf[4].as<float4>();
-    B.A = ctx->wf[5].as<float4>();
-    B.N = ctx->wf[6].as<float4>();
-    B.H = ctx->wf[7].as<float4>();
-    B.so = ctx->wf[8].as<float4>();
-    B.st = ctx->wf[9].as<float4>();
-    B.sr = ctx->wf[10].as<float4>();
-    B.qa = ctx->wf[11].as<uint32_t>();
-    B.qb = ctx->wf[12].as<uint32_t>();
-    B.qs = ctx->wf[13].as<uint32_t>();
-    B.counts = ctx->counters.as<uint32_t>();
-    B.qsr = ctx->wf[14].as<uint32_t>();
-    B.svis = ctx->wf[15].as<uint32_t>();
+    float4** f4[11] = {&B.o, &B.d, &B.T, &B.L, &B.E, &B.A, &B.N, &B.H, &B.so, &B.st, &B.sr};
+    for (int i = 0; i < 11; ++i) *f4[i] = static_cast<float4*>(a[i]);
+    B.qa = static_cast<uint32_t*>(a[11]);
+    B.qb = static_cast<uint32_t*>(a[12]);
+    B.qs = static_cast<uint32_t*>(a[13]);
+    B.qsr = static_cast<uint32_t*>(a[14]);
+    B.svis = static_cast<uint32_t*>(a[15]);
+    B.counts = static_cast<uint32_t*>(a[20]);
     B.countSlots = slots;
     B.shadowRayBase = shadowRayBase;
     B.shadowPathBase = shadowPathBase;
@@ -295,14 +352,13 @@ int enqueue_wavefront(shiro_hip_ctx* ctx, const DevScene& scene, const shiro_hip
     B.maxShadow = M;
     B.cap = static_cast<uint32_t>(cap);
     B.shadeOrder = order ? 1u : 0u;
-    B.qkey = order ? ctx->wf[16].as<uint8_t>() : nullptr;
-    B.qsorted = order ? ctx->wf[17].as<uint32_t>() : nullptr;
-    B.blockCount = order ? ctx->wf[18].as<uint32_t>() : nullptr;
-    B.blockStart = order ? ctx->wf[19].as<uint32_t>() : nullptr;
+    B.qkey = order ? static_cast<uint8_t*>(a[16]) : nullptr;
+    B.qsorted = order ? static_cast<uint32_t*>(a[17]) : nullptr;
+    B.blockCount = order ? static_cast<uint32_t*>(a[18]) : nullptr;
+    B.blockStart = order ? static_cast<uint32_t*>(a[19]) : nullptr;
     P.sampleStart = start;
     P.batchCount = count;
     P.tilesX = tilesX;
-    P.tiles = tilesX * tilesY;
     WfLaunch L{};
     L.maxBlocks = ctx->tuning.stageBlocks;
     L.traceBlocks = ctx->tuning.traceBlocks;
@@ -313,21 +369,25 @@ int enqueue_wavefront(shiro_hip_ctx* ctx, const DevScene& scene, const shiro_hip
         HIPC(hipEventRecord(ctx->ev0, ctx->stream), "event");
         begin_marks(ctx);
     }
-    for (uint32_t b = 0; b < count; b += static_cast<uint32_t>(S)) {
-        if (cancel && *cancel) {
-            (void)hipStreamSynchronize(ctx->stream);
-            return fail(ctx, SHIRO_HIP_CANCELLED, "cancelled");
+    for (uint32_t t0 = 0; t0 < tilesTotal; t0 += static_cast<uint32_t>(T)) {
+        P.tileBase = t0;
+        P.tiles = std::min(static_cast<uint32_t>(T), tilesTotal - t0);
+        for (uint32_t b = 0; b < count; b += static_cast<uint32_t>(S)) {
+            if (cancel && *cancel) {
+                (void)hipStreamSynchronize(ctx->stream);
+                return fail(ctx, SHIRO_HIP_CANCELLED, "cancelled");
+            }
+            P.sliceBegin = b;
+            P.sliceCount = std::min(static_cast<uint32_t>(S), count - b);
+            // Path groups of G samples as asked, or the largest smaller
+            // choice that divides the pass's samples per pixel.
+            uint32_t g = L.primaryPackets ? L.primaryPackets : 1u;
+            while (g > 1u && P.sliceCount % g != 0u) g /= 4u;
+            P.groupLog = g == 64u ? 6u : (g == 16u ? 4u : (g == 4u ? 2u : 0u));
+            HIPC(launch_wavefront_pass(P, L, ctx->stream, ctx->stageTiming && events ? &ctx->marks : nullptr),
+                 "wavefront launch");
+            if (cancel) HIPC(hipStreamSynchronize(ctx->stream), "wavefront");
         }
-        P.sliceBegin = b;
-        P.sliceCount = std::min(static_cast<uint32_t>(S), count - b);
-        // Path groups of G samples as asked, or the largest smaller choice
-        // that divides the pass's samples per pixel.
-        uint32_t g = L.primaryPackets ? L.primaryPackets : 1u;
-        while (g > 1u && P.sliceCount % g != 0u) g /= 4u;
-        P.groupLog = g == 64u ? 6u : (g == 16u ? 4u : (g == 4u ? 2u : 0u));
-        HIPC(launch_wavefront_pass(P, L, ctx->stream, ctx->stageTiming && events ? &ctx->marks : nullptr),
-             "wavefront launch");
-        if (cancel) HIPC(hipStreamSynchronize(ctx->stream), "wavefront");
     }
     if (events) HIPC(hipEventRecord(ctx->ev1, ctx->stream), "event");
     return SHIRO_HIP_OK;
@@ -469,6 +529,13 @@ int shiro_hip_create(int device, shiro_hip_ctx** out) {
     if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0) {
         ctx->tuning.traceBlocks = static_cast<uint32_t>(prop.multiProcessorCount) * 32u;
         ctx->computeUnits = static_cast<uint32_t>(prop.multiProcessorCount);
+    }
+    // This device's copy of the scratch array of the no-allocation entry
+    // point; on failure that entry point reports the error, the others work.
+    const hipError_t se = scratch_address(device, &ctx->scratch);
+    if (se != hipSuccess) {
+        ctx->scratch = nullptr;
+        ctx->scratchError = std::string("scratch address: ") + hipGetErrorString(se);
     }
     *out = ctx;
     return SHIRO_HIP_OK;
@@ -842,6 +909,28 @@ int shiro_hip_render_batch_device(shiro_hip_ctx* ctx, const shiro_hip_camera* ca
     HIPC(hipSetDevice(ctx->device), "set device");
     const DevFrame f{out->beauty, out->albedo, out->normal, out->depth};
     rc = enqueue_batch(ctx, *camera, *settings, sampleStart, sampleCount, f, false, nullptr);
+    if (rc) return rc;
+    if (synchronize) return finish_timing(ctx);
+    return SHIRO_HIP_OK;
+}
+
+int shiro_hip_render_batch_device_noalloc(shiro_hip_ctx* ctx, const shiro_hip_camera* camera,
+                                          const shiro_hip_settings* settings, uint32_t sampleStart,
+                                          uint32_t sampleCount, shiro_hip_frame* out, int synchronize) {
+    if (!ctx) return SHIRO_HIP_INVALID;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    if (!camera || !out || !out->beauty || !out->albedo || !out->normal || !out->depth)
+        return fail(ctx, SHIRO_HIP_INVALID, "null camera or frame buffer");
+    int rc = check_settings(ctx, settings);
+    if (rc) return rc;
+    if (!ctx->hasScene) return fail(ctx, SHIRO_HIP_NO_SCENE, "no scene uploaded");
+    if (!ctx->scratch) return fail(ctx, SHIRO_HIP_LAUNCH, "no scratch memory: " + ctx->scratchError);
+    if (sampleCount == 0) return SHIRO_HIP_OK;
+    HIPC(hipSetDevice(ctx->device), "set device");
+    const DevFrame f{out->beauty, out->albedo, out->normal, out->depth};
+    shiro_hip_settings st = *settings;
+    st.transport = SHIRO_HIP_TRANSPORT_WAVEFRONT;
+    rc = enqueue_wavefront(ctx, ctx->dev, *camera, st, sampleStart, sampleCount, f, nullptr, true, ctx->scratch);
     if (rc) return rc;
     if (synchronize) return finish_timing(ctx);
     return SHIRO_HIP_OK;

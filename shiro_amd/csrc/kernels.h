@@ -58,9 +58,17 @@ struct WfParams {
     DevFrame frame;
     WfBuffers B;
     uint32_t sampleStart, sliceBegin, sliceCount, batchCount;
-    uint32_t tilesX, tiles;
+    uint32_t tilesX;    // 8x8 tiles per row of the frame
+    uint32_t tiles;     // tiles of this pass: frame tiles [tileBase, tileBase + tiles) in row-major order
+    uint32_t tileBase;
     uint32_t groupLog;  // path layout: groups of 2^groupLog samples (path_coord, wavefront.hip)
 };
+
+// Device memory of the entry points that allocate nothing (wavefront.hip):
+// the size of the library's scratch array, and the address of `device`'s
+// copy of it (looked up once per device and kept).
+constexpr size_t kScratchBytes = size_t{256} << 20;
+hipError_t scratch_address(int device, void** ptr);
 
 struct WfLaunch {
     uint32_t maxBlocks;    // grid of the grid-stride stage kernels

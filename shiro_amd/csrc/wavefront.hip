@@ -35,6 +35,8 @@
 // between stages.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+
 #include "dev_path.h"
 #include "kernels.h"
 
@@ -66,10 +68,12 @@ DEV PathCoord path_coord(const WfParams& P, uint32_t p) {
     const uint32_t sub = lane >> (2u * rl), j = lane & ((1u << (2u * rl)) - 1u);
     const uint32_t xin = ((q & ((1u << ql) - 1u)) << rl) + (j & ((1u << rl) - 1u));
     const uint32_t yin = ((q >> ql) << rl) + (j >> rl);
-    return PathCoord{(t % P.tilesX) * 8u + xin, (t / P.tilesX) * 8u + yin, (sg << gl) + sub};
+    const uint32_t tg = P.tileBase + t;
+    return PathCoord{(tg % P.tilesX) * 8u + xin, (tg / P.tilesX) * 8u + yin, (sg << gl) + sub};
 }
 
-// Inverse of path_coord: the path of sample s of pixel (xin, yin) of tile t.
+// Inverse of path_coord: the path of sample s of pixel (xin, yin) of the
+// pass's tile t.
 DEV uint32_t path_index(const WfParams& P, uint32_t s, uint32_t t, uint32_t xin, uint32_t yin) {
     const uint32_t gl = P.groupLog, rl = 3u - (gl >> 1u), ql = 3u - rl;
     const uint32_t sg = s >> gl, sub = s & ((1u << gl) - 1u);
@@ -745,15 +749,20 @@ __global__ __launch_bounds__(64) void k_wf_resolve(WfParams P, uint32_t depth) {
 }
 
 // Per-pixel sums over the pass's samples in sample order, then the batch
-// means on the last pass (CpuPathTracer.cpp:2098-2137).
+// means on the last pass of the pixel's tile (CpuPathTracer.cpp:2098-2137).
+// The running sums between passes are kept per pass tile (slot tile * 64 +
+// lane): the passes of one tile range follow each other, so only that
+// range's pixels need them.
 __global__ __launch_bounds__(64) void k_wf_accum(WfParams P) {
     const uint32_t tile = blockIdx.x;
     const uint32_t lane = threadIdx.x;
-    const uint32_t x = (tile % P.tilesX) * 8u + (lane & 7u);
-    const uint32_t y = (tile / P.tilesX) * 8u + (lane >> 3);
+    const uint32_t tg = P.tileBase + tile;
+    const uint32_t x = (tg % P.tilesX) * 8u + (lane & 7u);
+    const uint32_t y = (tg / P.tilesX) * 8u + (lane >> 3);
     const DevSettings& st = P.settings;
     if (x >= st.width || y >= st.height) return;
     const size_t pix = static_cast<size_t>(y) * st.width + x;
+    const size_t slot = static_cast<size_t>(tile) * 64u + lane;
     float4 beauty, albedo, normal;
     float depth;
     if (P.sliceBegin == 0u) {
@@ -761,10 +770,10 @@ __global__ __launch_bounds__(64) void k_wf_accum(WfParams P) {
         albedo = normal = beauty;
         depth = kInf;
     } else {
-        beauty = P.sums.beauty[pix];
-        albedo = P.sums.albedo[pix];
-        normal = P.sums.normal[pix];
-        depth = P.sums.depth[pix];
+        beauty = P.sums.beauty[slot];
+        albedo = P.sums.albedo[slot];
+        normal = P.sums.normal[slot];
+        depth = P.sums.depth[slot];
     }
     // maxDepth 0: no stage of the pass ran, every sample is the empty path
     // (radiance, albedo and normal 0, depth +inf: SampleResult's defaults,
@@ -789,10 +798,10 @@ __global__ __launch_bounds__(64) void k_wf_accum(WfParams P) {
         if (A.w < depth) depth = A.w;
     }
     if (P.sliceBegin + P.sliceCount < P.batchCount) {
-        P.sums.beauty[pix] = beauty;
-        P.sums.albedo[pix] = albedo;
-        P.sums.normal[pix] = normal;
-        P.sums.depth[pix] = depth;
+        P.sums.beauty[slot] = beauty;
+        P.sums.albedo[slot] = albedo;
+        P.sums.normal[slot] = normal;
+        P.sums.depth[slot] = depth;
         return;
     }
     const float nf = static_cast<float>(P.batchCount);
@@ -875,6 +884,40 @@ hipError_t launch_wavefront_pass(const WfParams& P, const WfLaunch& L, hipStream
     hipLaunchKernelGGL(k_wf_accum, dim3(P.tiles), dim3(64), 0, s, P);
     m.mark(SHIRO_HIP_STAGE_ACCUM, s);
     return hipGetLastError();
+}
+
+// Scratch memory of the entry points that allocate nothing
+// (shiro_hip_render_batch_device_noalloc): path state, queues, counters and
+// running sums of a wavefront pass carved out of one file-scope device
+// array.  Every GPU the library runs on holds its own copy until the process
+// ends; the host obtains each device's address with hipGetSymbolAddress and
+// uses only that address (the array's name is not a device address on the
+// host).
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+hipError_t scratch_address(int device, void** ptr) {
+    constexpr int kMaxDevices = 64;
+    static std::mutex mu;
+    static void* address[kMaxDevices] = {};
+    *ptr = nullptr;
+    if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(mu);
+    if (!address[device]) {
+        // Looked up once per device, with that device current.
+        int current = 0;
+        hipError_t e = hipGetDevice(&current);
+        if (e != hipSuccess) return e;
+        e = hipSetDevice(device);
+        if (e != hipSuccess) return e;
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        const hipError_t back = hipSetDevice(current);
+        if (e != hipSuccess) return e;
+        if (back != hipSuccess) return back;
+        address[device] = p;
+    }
+    *ptr = address[device];
+    return hipSuccess;
 }
 
 }  // namespace shiro_hip

@@ -69,6 +69,8 @@ def load_library(path: str = LIB_PATH):
     d("shiro_hip_upload_scene", C.c_int, V, _P(abi.Scene))
     d("shiro_hip_render_batch", C.c_int, V, _P(abi.Camera), _P(abi.Settings), _u, _u, _P(C.c_uint8), _P(abi.Frame))
     d("shiro_hip_render_batch_device", C.c_int, V, _P(abi.Camera), _P(abi.Settings), _u, _u, _P(abi.Frame), C.c_int)
+    d("shiro_hip_render_batch_device_noalloc", C.c_int, V, _P(abi.Camera), _P(abi.Settings), _u, _u, _P(abi.Frame),
+      C.c_int)
     d("shiro_hip_accum_reset", C.c_int, V, _u, _u)
     d("shiro_hip_accum_add", C.c_int, V, _P(abi.Frame), _u)
     d("shiro_hip_accum_resolve", C.c_int, V, _P(abi.Frame))
