@@ -263,6 +263,84 @@ def bench_c5(args):
     print(json.dumps(out), flush=True)
 
 
+def _csv_rows(directory, suffix):
+    import csv
+    import glob
+    rows = []
+    for path in sorted(glob.glob(os.path.join(directory, "**", f"*{suffix}"), recursive=True)):
+        with open(path, newline="") as f:
+            rows.extend(csv.DictReader(f))
+    return rows
+
+
+def _kernel_symbol(name):
+    """rocprofv3's kernel name without the namespace and the argument list."""
+    name = name.strip()
+    if name.startswith("void "):
+        name = name[5:]
+    depth, cut = 0, len(name)
+    for i, ch in enumerate(name):  # the argument list is the first '(' outside template brackets
+        if ch == "<":
+            depth += 1
+        elif ch == ">":
+            depth -= 1
+        elif ch == "(" and depth == 0:
+            cut = i
+            break
+    return name[:cut].replace("shiro_hip::", "").strip()
+
+
+def summarize_profile(args):
+    """Per-kernel HBM bytes per dispatch from rocprofv3 output directories
+    (csv): kernel stats of a --kernel-trace --stats run, FETCH_SIZE and
+    WRITE_SIZE runs of the workload, and the same two counters of a device
+    copy of known size (bench.py --copy-probe), which give the factors that
+    turn the counters into bytes.  Prints a markdown table and the JSON
+    record kept in profiles/hbm_traffic.json."""
+    stats_dir, fetch_dir, write_dir, copyf_dir, copyw_dir = args.summarize_profile
+
+    def counter_avgs(directory, counter):
+        acc = {}
+        for r in _csv_rows(directory, "counter_collection.csv"):
+            if r.get("Counter_Name") != counter:
+                continue
+            k = _kernel_symbol(r["Kernel_Name"])
+            acc.setdefault(k, []).append(float(r["Counter_Value"]))
+        return {k: (sum(v) / len(v), len(v)) for k, v in acc.items()}
+
+    fetch, write = counter_avgs(fetch_dir, "FETCH_SIZE"), counter_avgs(write_dir, "WRITE_SIZE")
+    cf, cw = counter_avgs(copyf_dir, "FETCH_SIZE"), counter_avgs(copyw_dir, "WRITE_SIZE")
+    # The copy kernel reads and writes the buffer (the fill that made it only
+    # writes): the kernel of both runs with the largest read.
+    copy_kernel = max((k for k in cf if k in cw), key=lambda k: cf[k][0])
+    copy_bytes = float(args.copy_bytes)
+    fscale = copy_bytes / (cf[copy_kernel][0] * 1024.0)
+    wscale = copy_bytes / (cw[copy_kernel][0] * 1024.0)
+    stats = {}
+    for r in _csv_rows(stats_dir, "kernel_stats.csv"):
+        stats[_kernel_symbol(r["Name"])] = (int(r["Calls"]), float(r["TotalDurationNs"]) / 1e6,
+                                            float(r["AverageNs"]) / 1e6)
+    kernels = {}
+    lines = ["| kernel | calls (trace) | avg ms (trace) | FETCH_SIZE KiB | WRITE_SIZE KiB | HBM MB per dispatch | GB/s |",
+             "|---|---|---|---|---|---|---|"]
+    for k in sorted(stats, key=lambda k: -stats[k][1]):
+        calls, total, avg = stats[k]
+        if k not in fetch or k not in write:
+            lines.append(f"| `{k}` | {calls} | {avg:.4f} | - | - | - | - |")
+            continue
+        b = fetch[k][0] * 1024.0 * fscale + write[k][0] * 1024.0 * wscale
+        kernels[k] = {"hbm_bytes_per_dispatch": int(b), "fetch_kib": round(fetch[k][0], 1),
+                      "write_kib": round(write[k][0], 1), "avg_ms": round(avg, 4), "calls": calls}
+        lines.append(f"| `{k}` | {calls} | {avg:.4f} | {fetch[k][0]:.0f} | {write[k][0]:.0f} | {b / 1e6:.1f} | "
+                     f"{b / (avg * 1e-3) / 1e9:.0f} |")
+    print(f"Copy calibration ({copy_kernel}, {copy_bytes / 2**30:.2f} GiB): FETCH_SIZE x {fscale:.3f}, "
+          f"WRITE_SIZE x {wscale:.3f}\n")
+    print("\n".join(lines))
+    print()
+    print(json.dumps({"fetch_scale": round(fscale, 4), "write_scale": round(wscale, 4), "kernels": kernels},
+                     indent=1))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -286,7 +364,14 @@ def main():
                     help="force shading-class ordering of the hit queue off (0) or on (1)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device 3 times (calibrates HBM byte counters)")
+    ap.add_argument("--summarize-profile", nargs=5, default=None,
+                    metavar=("STATS", "FETCH", "WRITE", "COPY_FETCH", "COPY_WRITE"),
+                    help="summarise rocprofv3 csv output directories into HBM bytes per dispatch (no GPU)")
+    ap.add_argument("--copy-bytes", type=int, default=1 << 30, help="bytes the --copy-probe run copied")
     args = ap.parse_args()
+    if args.summarize_profile:
+        summarize_profile(args)
+        return
     if args.copy_probe:
         import torch
         n = args.copy_probe * (1 << 20) // 4
@@ -383,56 +468,99 @@ def main():
     # Roofline of the dominant render kernel.  Algorithmic bytes come from
     # the fetch counters (same BVH, same traversal order): BVH4 nodes 64 B,
     # triangles 48 B, hit data 96 B, environment texels 16 B, CDF entries
-    # 4 B, AOV write 44 B per pixel per batch.  The wavefront's closest-hit
-    # kernel is credited with the closest-hit rays' node and triangle bytes,
-    # its any-hit kernel with the shadow rays'; queue and path-state traffic
-    # earns nothing.  Kernel durations come from events recorded after every
-    # launch on the render stream, in one extra (untimed) render call.
+    # 4 B, AOV write 44 B per pixel per batch.  Each wavefront kernel symbol
+    # is credited with its own rays' bytes: the camera-packet kernel with the
+    # camera rays' node and triangle fetches, the closest-hit kernel with
+    # those of the bounce rays, the any-hit kernel with the shadow rays'; the
+    # shade kernels with the hit, texel and CDF bytes; queue and path-state
+    # traffic earns nothing.  Kernel durations come from events recorded
+    # after every launch on the render stream, in one extra (untimed) render
+    # call.  The dominant kernel is the symbol with the largest time per
+    # frame.  Caveat: the algorithmic count charges every ray for its own
+    # fetches, while camera packets (one LDS stack and one node load per
+    # wave) and same-pixel path groups (neighbouring lanes walk the same
+    # nodes, served by L2 and the last-level cache) share them, so a
+    # packet kernel can exceed the HBM peak on this measure; `traffic` is
+    # what HBM moved per launch by the rocprofv3 counters.
     roofline = None
     stages = None
     if not args.no_count:
         cspp = min(4, st.samplesPerPixel)
         c = ctx.count_fetches(wl.camera, st, 0, cspp)
         ns = max(c.samples, 1)
-        closest = (c.nodeFetches * NODE_BYTES + c.triFetches * TRI_BYTES) / ns
+        camera_b = (c.primaryNodeFetches * NODE_BYTES + c.primaryTriFetches * TRI_BYTES) / ns
+        bounce_b = ((c.nodeFetches - c.primaryNodeFetches) * NODE_BYTES +
+                    (c.triFetches - c.primaryTriFetches) * TRI_BYTES) / ns
         shadow = (c.shadowNodeFetches * NODE_BYTES + c.shadowTriFetches * TRI_BYTES) / ns
         other = (c.hitFetches * HIT_BYTES + c.envTaps * TEXEL_BYTES + c.cdfSteps * CDF_BYTES) / ns
-        per_sample = closest + shadow + other + 44.0 / st.samplesPerPixel
+        per_sample = camera_b + bounce_b + shadow + other + 44.0 / st.samplesPerPixel
         ctx.set_stage_timing(True)
         ctx.render_device(wl.camera, st, ptrs, 0, st.samplesPerPixel, synchronize=True)
         stages = ctx.stage_times()
         ctx.set_stage_timing(False)
+        fm, bw = (tuning["fetchMode"], tuning["bvhWidth"]) if tuning else (0, 4)
         if args.transport == "wavefront":
-            dom = max(("extend", "shadow", "shade"), key=lambda k: stages.get(k, {"ms": 0.0})["ms"])
-            names = {"extend": f"k_wf_trace<false,{tuning['fetchMode']}> (closest hit)",
-                     "shadow": f"k_wf_trace<true,{tuning['fetchMode']}> (any hit)",
-                     "shade": "k_wf_shade"}
-            credit = {"extend": closest, "shadow": shadow, "shade": other}[dom]
+            shade_first = "k_wf_shade<true, true>" if wl.scene.textures else "k_wf_shade<false, true>"
+            shade_later = "k_wf_shade<true, false>" if wl.scene.textures else "k_wf_shade<false, false>"
+            kernels = {
+                "primary": (f"k_wf_primary<{bw}>", camera_b),
+                "extend": (f"k_wf_trace<false, {fm}, {bw}>", bounce_b if stages.get("primary") else
+                           bounce_b + camera_b),
+                "shadow": (f"k_wf_trace<true, {fm}, {bw}>", shadow),
+                "shade_first": (shade_first, None),
+                "shade": (shade_later, None),
+            }
         else:
-            dom, credit = "megakernel", per_sample
-            names = {"megakernel": "k_render<false> (megakernel)"}
-        stage_ms, launches = stages[dom]["ms"], stages[dom]["launches"]
-        bytes_per_launch = credit * samples / launches
-        avg_launch_ms = stage_ms / launches
-        achieved = bytes_per_launch / (avg_launch_ms * 1e-3) / 1e9
+            kernels = {"megakernel": ("k_render<false, false>", per_sample)}
+        entries = []
+        for stage, (sym, credit) in kernels.items():
+            if stage not in stages:
+                continue
+            ms, n = stages[stage]["ms"], stages[stage]["launches"]
+            e = {"kernel": sym, "stage": stage, "launches_per_step": n, "ms_per_step": round(ms, 3),
+                 "avg_launch_ms": round(ms / n, 4)}
+            if credit is not None:
+                bpl = credit * samples / n
+                e["bytes_per_sample"] = round(credit, 1)
+                e["bytes_per_launch"] = int(bpl)
+                e["achieved"] = round(bpl / (ms / n * 1e-3) / 1e9, 2)
+                e["frac"] = round(e["achieved"] / HBM_PEAK_GBS, 5)
+            entries.append(e)
+        dom = max(entries, key=lambda e: e["ms_per_step"])
+        if "achieved" not in dom:
+            # A shade kernel: credited with the hit, texel and CDF bytes of
+            # every bounce in proportion to its share of the shade time.
+            sh = sum(e["ms_per_step"] for e in entries if e["stage"] in ("shade", "shade_first"))
+            credit = other * dom["ms_per_step"] / sh
+            bpl = credit * samples / dom["launches_per_step"]
+            dom.update(bytes_per_sample=round(credit, 1), bytes_per_launch=int(bpl),
+                       achieved=round(bpl / (dom["avg_launch_ms"] * 1e-3) / 1e9, 2))
+            dom["frac"] = round(dom["achieved"] / HBM_PEAK_GBS, 5)
+        # Measured HBM bytes per launch of the dominant symbol, from the
+        # committed rocprofv3 counter summary of this workload (FETCH_SIZE and
+        # WRITE_SIZE runs scaled by a device copy of known size).
         traffic, traffic_src = None, None
-        tpath = os.path.join(ROOT, "profiles", "traffic.json")
+        tpath = os.path.join(ROOT, "profiles", "hbm_traffic.json")
         if os.path.exists(tpath):
-            rec = json.load(open(tpath)).get(f"{args.workload}/{args.transport}/{dom}")
-            if rec:
-                traffic, traffic_src = rec["bytes_per_launch"], rec["source"]
-        # Algorithmic bytes count every ray's own node and triangle fetches;
-        # camera-ray packets and same-pixel path groups share fetches between
-        # rays (LDS stack, L2, last-level cache), so the closest-hit stage can
-        # exceed the HBM peak on this measure; `traffic` is what HBM moved.
-        roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic, "traffic_source": traffic_src,
-                    "kernel": names[dom], "launches_per_step": launches,
-                    "avg_launch_ms": round(avg_launch_ms, 4), "bytes_per_launch": int(bytes_per_launch),
-                    "bytes_per_sample": {"kernel": round(credit, 1), "path": round(per_sample, 1)},
+            rec = json.load(open(tpath)).get(args.workload, {})
+            k = rec.get("kernels", {}).get(dom["kernel"])
+            if k:
+                traffic = k["hbm_bytes_per_dispatch"]
+                traffic_src = f"{rec['source']} (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, commit {rec['commit']})"
+        roofline = {"bound": "hbm", "achieved": dom["achieved"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": dom["frac"], "traffic": traffic, "traffic_source": traffic_src,
+                    "traffic_frac": (round(traffic / (dom["avg_launch_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 5)
+                                     if traffic else None),
+                    "kernel": dom["kernel"], "launches_per_step": dom["launches_per_step"],
+                    "avg_launch_ms": dom["avg_launch_ms"], "bytes_per_launch": dom["bytes_per_launch"],
+                    "bytes_per_sample": {"kernel": dom["bytes_per_sample"], "path": round(per_sample, 1)},
+                    "kernels": entries,
                     "stage_ms": {k: round(v["ms"], 2) for k, v in stages.items() if k != "complete"},
-                    "per_sample": {"rays": c.rays / ns, "shadow_rays": c.shadowRays / ns,
+                    "per_sample": {"rays": c.rays / ns, "camera_rays": c.primaryRays / ns,
+                                   "shadow_rays": c.shadowRays / ns,
                                    "node_fetches": c.nodeFetches / ns, "tri_tests": c.triFetches / ns,
+                                   "camera_node_fetches": c.primaryNodeFetches / ns,
+                                   "camera_tri_tests": c.primaryTriFetches / ns,
                                    "shadow_node_fetches": c.shadowNodeFetches / ns,
                                    "shadow_tri_tests": c.shadowTriFetches / ns,
                                    "env_taps": c.envTaps / ns, "cdf_steps": c.cdfSteps / ns}}

@@ -503,6 +503,9 @@ typedef struct shiro_hip_fetch_counts {
     uint64_t hitFetches;    /* per-hit vertex/normal/material fetches */
     uint64_t shadowNodeFetches;  /* BVH4 nodes read by any-hit rays */
     uint64_t shadowTriFetches;   /* triangles tested by any-hit rays */
+    uint64_t primaryRays;        /* camera rays: the depth-0 part of rays, */
+    uint64_t primaryNodeFetches; /* of nodeFetches */
+    uint64_t primaryTriFetches;  /* and of triFetches */
 } shiro_hip_fetch_counts;
 int shiro_hip_count_fetches(
     shiro_hip_ctx* ctx, const shiro_hip_camera* camera,
@@ -582,13 +585,15 @@ int shiro_hip_autotune_many(
 enum {
     SHIRO_HIP_STAGE_RESET = 0,      /* wavefront queue counters cleared */
     SHIRO_HIP_STAGE_EXTEND = 1,     /* closest-hit traversal of the path queue */
-    SHIRO_HIP_STAGE_SHADE = 2,      /* surface/miss terms, light samples, bounce */
+    SHIRO_HIP_STAGE_SHADE = 2,      /* surface/miss terms, light samples, bounce (depth > 0) */
     SHIRO_HIP_STAGE_SHADOW = 3,     /* any-hit traversal of the shadow-ray queue */
     SHIRO_HIP_STAGE_RESOLVE = 4,    /* ordered sum of unoccluded light samples */
     SHIRO_HIP_STAGE_ACCUM = 5,      /* per-pixel sample sums of the pass */
     SHIRO_HIP_STAGE_MEGAKERNEL = 6, /* one megakernel launch */
     SHIRO_HIP_STAGE_ORDER = 7,      /* hit queue sorted by shading class */
-    SHIRO_HIP_STAGE_COUNT = 8
+    SHIRO_HIP_STAGE_PRIMARY = 8,    /* closest hit of the camera rays traced as packets (depth 0) */
+    SHIRO_HIP_STAGE_SHADE_FIRST = 9, /* the first bounce's shading (its own kernel; SHADE: the later bounces) */
+    SHIRO_HIP_STAGE_COUNT = 10
 };
 typedef struct shiro_hip_stage_times {
     float ms[SHIRO_HIP_STAGE_COUNT];          /* summed kernel time per stage */

@@ -128,6 +128,7 @@ class FetchCounts(C.Structure):
         ("nodeFetches", C.c_uint64), ("triFetches", C.c_uint64),
         ("envTaps", C.c_uint64), ("cdfSteps", C.c_uint64), ("hitFetches", C.c_uint64),
         ("shadowNodeFetches", C.c_uint64), ("shadowTriFetches", C.c_uint64),
+        ("primaryRays", C.c_uint64), ("primaryNodeFetches", C.c_uint64), ("primaryTriFetches", C.c_uint64),
     ]
 
 
@@ -149,12 +150,12 @@ class BatchItem(C.Structure):
 MANY_PERSISTENT, MANY_GRAPH, MANY_SEQUENTIAL, MANY_WAVEFRONT = 0, 1, 2, 3
 
 
-STAGE_NAMES = ["reset", "extend", "shade", "shadow", "resolve", "accum", "megakernel", "order"]
+STAGE_NAMES = ["reset", "extend", "shade", "shadow", "resolve", "accum", "megakernel", "order", "primary", "shade_first"]
 
 
 class StageTimes(C.Structure):
     _fields_ = [
-        ("ms", C.c_float * 8), ("launches", C.c_uint32 * 8),
+        ("ms", C.c_float * 10), ("launches", C.c_uint32 * 10),
         ("complete", C.c_uint32), ("reserved", C.c_uint32),
     ]
 

@@ -1356,6 +1356,9 @@ int shiro_hip_count_fetches(shiro_hip_ctx* ctx, const shiro_hip_camera* camera, 
     out->hitFetches = c.hitFetches;
     out->shadowNodeFetches = c.shadowNodeFetches;
     out->shadowTriFetches = c.shadowTriFetches;
+    out->primaryRays = c.primaryRays;
+    out->primaryNodeFetches = c.primaryNodeFetches;
+    out->primaryTriFetches = c.primaryTriFetches;
     return SHIRO_HIP_OK;
 }
 

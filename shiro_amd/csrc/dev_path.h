@@ -23,6 +23,7 @@ template <bool COUNT>
 struct Counters {
     unsigned long long rays = 0, shadowRays = 0, nodes = 0, tris = 0, taps = 0, steps = 0, hits = 0;
     unsigned long long shadowNodes = 0, shadowTris = 0;
+    unsigned long long primaryRays = 0, primaryNodes = 0, primaryTris = 0;  // the depth-0 part of rays/nodes/tris
     DEV TravCount<COUNT> trav() { return TravCount<COUNT>{&nodes, &tris}; }
     DEV TravCount<COUNT> shadow_trav() { return TravCount<COUNT>{&shadowNodes, &shadowTris}; }
     DEV EnvCount<COUNT> env() { return EnvCount<COUNT>{&taps, &steps}; }
@@ -663,7 +664,13 @@ DEV PathState trace_path(const DevScene& S, const DevSettings& st, const DevCame
     for (uint32_t depth = 0; depth < st.maxDepth; ++depth) {
         const Ray ray = path_ray(ps);
         if (COUNT) c.rays += 1;
+        const unsigned long long nodes0 = c.nodes, tris0 = c.tris;
         const HitRec h = intersect_closest(S, ray, c.trav());
+        if (COUNT && depth == 0u) {
+            c.primaryRays += 1;
+            c.primaryNodes += c.nodes - nodes0;
+            c.primaryTris += c.tris - tris0;
+        }
         if (h.slot == kNoHit) {
             handle_miss(S, st, depth, ps, c);
             break;

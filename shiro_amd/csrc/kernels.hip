@@ -86,6 +86,9 @@ DEV void render_tile(const RenderParams& P, uint32_t tile, uint32_t lane) {
         atomicAdd(&c->hitFetches, cnt.hits);
         atomicAdd(&c->shadowNodeFetches, cnt.shadowNodes);
         atomicAdd(&c->shadowTriFetches, cnt.shadowTris);
+        atomicAdd(&c->primaryRays, cnt.primaryRays);
+        atomicAdd(&c->primaryNodeFetches, cnt.primaryNodes);
+        atomicAdd(&c->primaryTriFetches, cnt.primaryTris);
     }
     if (P.sliceBegin + P.sliceCount < P.batchCount) {
         P.sums.beauty[pix] = beauty;

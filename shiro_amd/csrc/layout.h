@@ -207,6 +207,7 @@ struct DevFrame {
 struct DevCounters {
     unsigned long long samples, rays, shadowRays, nodeFetches, triFetches, envTaps, cdfSteps, hitFetches;
     unsigned long long shadowNodeFetches, shadowTriFetches;
+    unsigned long long primaryRays, primaryNodeFetches, primaryTriFetches;
 };
 
 struct RenderParams {

@@ -853,10 +853,11 @@ hipError_t launch_wavefront_pass(const WfParams& P, const WfLaunch& L, hipStream
             } else {
                 hipLaunchKernelGGL(k_wf_primary<4>, dim3(blocks), dim3(64), 0, s, P);
             }
+            m.mark(SHIRO_HIP_STAGE_PRIMARY, s);
         } else {
             launch_trace<false>(P, L, d, blocks, tblocks, s);
+            m.mark(SHIRO_HIP_STAGE_EXTEND, s);
         }
-        m.mark(SHIRO_HIP_STAGE_EXTEND, s);
         if (d > 0u && P.B.shadeOrder) {
             hipLaunchKernelGGL(k_wf_classify, dim3(kOrderBlocks), dim3(kOrderThreads), 0, s, P, d);
             hipLaunchKernelGGL(k_wf_order_scan, dim3(kShadeClasses), dim3(kOrderThreads), 0, s, P);
@@ -875,7 +876,7 @@ hipError_t launch_wavefront_pass(const WfParams& P, const WfLaunch& L, hipStream
         } else {
             hipLaunchKernelGGL((k_wf_shade<false, false>), dim3(sblocks), dim3(kShadeThreads), 0, s, P, d);
         }
-        m.mark(SHIRO_HIP_STAGE_SHADE, s);
+        m.mark(d == 0u ? SHIRO_HIP_STAGE_SHADE_FIRST : SHIRO_HIP_STAGE_SHADE, s);
         launch_trace<true>(P, L, d, blocks, tblocks, s);
         m.mark(SHIRO_HIP_STAGE_SHADOW, s);
         hipLaunchKernelGGL(k_wf_resolve, dim3(blocks), dim3(64), 0, s, P, d);
