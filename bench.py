@@ -102,7 +102,9 @@ def bench_c4(args):
     4/16/64 spp, 10k-500k triangles, own 1024x512 dome each) in the fastest
     batch mode; every mode (persistent work-list megakernel with autotuned
     grid, replayed hipGraph of per-scene megakernel launches, sequential
-    launches, per-scene wavefront) is timed and reported beside it."""
+    launches, per-scene wavefront, and the wavefront of all scenes in one
+    captured graph of autotuned concurrent branches, replayed) is timed and
+    reported beside it."""
     import torch
     from shiro_amd import abi, workloads
     from shiro_amd.hip import Context
@@ -129,7 +131,7 @@ def bench_c4(args):
     print("# batch autotune (persistent grid):\n" + report, file=sys.stderr, flush=True)
     modes = {}
     names = {"persistent": abi.MANY_PERSISTENT, "graph": abi.MANY_GRAPH, "sequential": abi.MANY_SEQUENTIAL,
-             "wavefront": abi.MANY_WAVEFRONT}
+             "wavefront": abi.MANY_WAVEFRONT, "wavefront_graph": abi.MANY_WAVEFRONT_GRAPH}
     for name, mode in names.items():
         launcher.render_many(items, mode)
         modes[name] = round(launcher.render_many(items, mode), 3)
@@ -157,7 +159,8 @@ def bench_c4(args):
         "config": {"workload": f"c4: {len(wls)} scenes, {tris} triangles, {samples} samples per step",
                    "batch_mode": fastest, "batch_ms": modes,
                    "speedup_vs_sequential": round(modes["sequential"] / (sum(gpu_ms) / len(gpu_ms)), 3),
-                   "tuning": {"batchBlocksPerCU": best.batchBlocksPerCU}},
+                   "tuning": {"batchBlocksPerCU": best.batchBlocksPerCU, "manyBranches": best.manyBranches},
+                   "graph_captures": launcher.many_graph_stats()[0]},
         "roofline": None, "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

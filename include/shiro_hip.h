@@ -532,6 +532,8 @@ typedef struct shiro_hip_tuning {
                                 and per-lane camera-ray traversal (default 16) */
     uint32_t bvhWidth;      /* 4 or 8: tree the wavefront traversal kernels walk (8 falls
                                back to 4 for a scene without an 8-wide tree) */
+    uint32_t manyBranches;  /* SHIRO_HIP_MANY_WAVEFRONT_GRAPH: concurrent branches (streams, each
+                               with its own wavefront state) of the batch graph, 1-16 (default 4) */
 } shiro_hip_tuning;
 int shiro_hip_set_tuning(shiro_hip_ctx* ctx, const shiro_hip_tuning* tuning);
 int shiro_hip_get_tuning(const shiro_hip_ctx* ctx, shiro_hip_tuning* tuning);
@@ -563,7 +565,11 @@ enum {
     SHIRO_HIP_MANY_PERSISTENT = 0, /* one persistent megakernel over (item, tile) units, longest first */
     SHIRO_HIP_MANY_GRAPH = 1,      /* one megakernel launch per item, captured in a hipGraph and replayed */
     SHIRO_HIP_MANY_SEQUENTIAL = 2, /* one megakernel launch per item, back to back on the stream */
-    SHIRO_HIP_MANY_WAVEFRONT = 3   /* the wavefront form of each item in turn (ctx's tuning and buffers) */
+    SHIRO_HIP_MANY_WAVEFRONT = 3,  /* the wavefront form of each item in turn (ctx's tuning and buffers) */
+    SHIRO_HIP_MANY_WAVEFRONT_GRAPH = 4 /* the wavefront form of every item in one hipGraph: items
+                                          assigned longest first to tuning.manyBranches concurrent
+                                          branches, the fork/join captured once, instantiated and
+                                          replayed (updated in place when the items change) */
 };
 /* Renders every item's batch (means into its device frame, like
  * shiro_hip_render_batch_device; the frames are identical in every mode)
@@ -573,9 +579,16 @@ enum {
 int shiro_hip_render_many(
     shiro_hip_ctx* ctx, const shiro_hip_batch_item* items, uint32_t count,
     uint32_t mode);
-/* Times the persistent mode for blocks-per-CU in {1, 2, 4, 8, 16, 32} (a
- * warm run and a timed run each), keeps the fastest in ctx's tuning and
- * returns it; `report` (may be NULL) receives one line per candidate. */
+/* Counts of SHIRO_HIP_MANY_WAVEFRONT_GRAPH on this context: graphs captured
+ * (a replay of unchanged items captures nothing) and captures that updated
+ * the instantiated graph in place instead of instantiating a new one. */
+int shiro_hip_many_graph_stats(const shiro_hip_ctx* ctx, uint32_t* captures, uint32_t* updates);
+/* Times the persistent mode for blocks-per-CU in {1, 2, 4, 8, 16, 32}, then
+ * the graph-captured wavefront batch for manyBranches in {1, 2, 3, 4, 6, 8}
+ * (a warm run and a timed run each; the warm run of a branch count captures
+ * and instantiates its graph, the timed run replays it), keeps the fastest
+ * of each in ctx's tuning and returns it; `report` (may be NULL) receives
+ * one line per candidate. */
 int shiro_hip_autotune_many(
     shiro_hip_ctx* ctx, const shiro_hip_batch_item* items, uint32_t count,
     shiro_hip_tuning* best, char* report, size_t reportSize);

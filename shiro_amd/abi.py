@@ -136,7 +136,7 @@ class Tuning(C.Structure):
     _fields_ = [
         ("pathsPerPass", C.c_uint32), ("fetchMode", C.c_uint32), ("traceBlocks", C.c_uint32),
         ("stageBlocks", C.c_uint32), ("batchBlocksPerCU", C.c_uint32), ("shadeOrder", C.c_uint32),
-        ("primaryPackets", C.c_uint32), ("bvhWidth", C.c_uint32),
+        ("primaryPackets", C.c_uint32), ("bvhWidth", C.c_uint32), ("manyBranches", C.c_uint32),
     ]
 
 
@@ -147,7 +147,7 @@ class BatchItem(C.Structure):
     ]
 
 
-MANY_PERSISTENT, MANY_GRAPH, MANY_SEQUENTIAL, MANY_WAVEFRONT = 0, 1, 2, 3
+MANY_PERSISTENT, MANY_GRAPH, MANY_SEQUENTIAL, MANY_WAVEFRONT, MANY_WAVEFRONT_GRAPH = 0, 1, 2, 3, 4
 
 
 STAGE_NAMES = ["reset", "extend", "shade", "shadow", "resolve", "accum", "megakernel", "order", "primary", "shade_first"]
@@ -193,7 +193,7 @@ EXPORTS = [
     "shiro_hip_set_tuning", "shiro_hip_get_tuning", "shiro_hip_autotune",
     "shiro_hip_set_stage_timing", "shiro_hip_get_stage_times",
     "shiro_hip_update_texture", "shiro_hip_render_adjoint",
-    "shiro_hip_render_many", "shiro_hip_autotune_many",
+    "shiro_hip_render_many", "shiro_hip_autotune_many", "shiro_hip_many_graph_stats",
     "shiro_hip_load_image", "shiro_hip_load_image_memory", "shiro_hip_free_image",
     "shiro_hip_refit_scene", "shiro_hip_triangulate", "shiro_hip_smooth_normals",
     "shiro_hip_render_adjoint_env", "shiro_hip_update_environment",

@@ -76,6 +76,13 @@ struct TexBuffers {
     size_t count() const { return static_cast<size_t>(w) * h * channels; }
 };
 
+// The arrays of the wavefront passes of one stream (wavefront_sizes order:
+// path state, shadow records, queues, class order, counters, running sums).
+constexpr size_t kWfArrays = 25;
+struct WfState {
+    DevBuf a[kWfArrays];
+};
+
 }  // namespace
 
 struct shiro_hip_ctx {
@@ -101,18 +108,28 @@ struct shiro_hip_ctx {
     DevBuf sumB, sumA, sumN, sumD;
     DevBuf frB, frA, frN, frD;
     DevBuf counters;
-    // Wavefront state (WfBuffers arrays) and queues.
-    DevBuf wf[20];
+    // Wavefront state of the context stream.
+    WfState wfs;
     // Accumulator.
     DevBuf acB, acA, acN, acD, acC;
     uint32_t acW = 0, acH = 0, acTotal = 0;
     // Wavefront tuning (shiro_hip_set_tuning / shiro_hip_autotune).
-    shiro_hip_tuning tuning{128u << 20, 0u, 8192u, 16384u, 8u, 1u, 16u, 4u};
+    shiro_hip_tuning tuning{128u << 20, 0u, 8192u, 16384u, 8u, 1u, 16u, 4u, 4u};
     uint32_t computeUnits = 256;
     // Batched rendering: items, work list and the work counter.
     DevBuf manyItems, manyWork, manyCounter;
     std::vector<uint32_t> manyDepth;   // BVH depth per item of the current batch (cost estimate)
     std::vector<const shiro_hip_batch_item*> manyItemsHost;  // non-empty items of the current batch
+    // Batched wavefront in one graph: per branch a stream and its wavefront
+    // state; the instantiated graph of the last batch and the key (items,
+    // tuning, buffer addresses) it was captured for.
+    std::vector<hipStream_t> branchStreams;
+    std::vector<std::unique_ptr<WfState>> branchStates;
+    std::vector<hipEvent_t> joinEv;
+    hipEvent_t forkEv = nullptr;
+    hipGraphExec_t manyExec = nullptr;
+    uint64_t manyKey = 0;
+    uint32_t manyCaptures = 0, manyUpdates = 0;  // captures made, and how many of them updated the cached graph
     // Stage timing (shiro_hip_set_stage_timing): events recorded after each
     // launch of the last render call.
     bool stageTiming = false;
@@ -254,7 +271,6 @@ struct WfMemory {
 // shading-class order on or off, sum slots and counter slots.  Order:
 // o d T L E A N H so st | sr | qa qb qs | qsr svis | qkey qsorted blockCount
 // blockStart | counters | sums beauty albedo normal depth.
-constexpr size_t kWfArrays = 25;
 void wavefront_sizes(uint64_t cap, uint32_t M, bool order, uint64_t sumSlots, uint32_t slots, size_t* n) {
     for (int i = 0; i < 10; ++i) n[i] = cap * 16;
     n[10] = cap * M * 32;
@@ -269,33 +285,37 @@ void wavefront_sizes(uint64_t cap, uint32_t M, bool order, uint64_t sumSlots, ui
     n[24] = sumSlots * 4;
 }
 
-// `scene` is ctx's own scene except for batched renders; `events` records
-// ev0 / ev1 around the passes; a non-null `scratch` (kScratchBytes) holds
-// every array instead of the context's buffers.
-int enqueue_wavefront(shiro_hip_ctx* ctx, const DevScene& scene, const shiro_hip_camera& cam,
-                      const shiro_hip_settings& st, uint32_t start, uint32_t count, const DevFrame& frame,
-                      const volatile uint8_t* cancel, bool events = true, void* scratch = nullptr) {
-    const uint32_t tilesX = (st.width + 7u) / 8u, tilesY = (st.height + 7u) / 8u;
-    const uint32_t tilesTotal = tilesX * tilesY;
-    const uint64_t perSample = static_cast<uint64_t>(tilesTotal) * 64u;
+// Pass plan of one batch: S samples per pass over ranges of T tiles, and
+// the array sizes.
+struct WfPlan {
+    uint32_t tilesX = 0, tilesTotal = 0, M = 1, slots = 0, shadowRayBase = 0, shadowPathBase = 0, fetchBase = 0;
+    uint64_t S = 1, T = 1, cap = 0;
+    bool order = false;
+    size_t n[kWfArrays] = {};
+};
+
+int wavefront_plan(shiro_hip_ctx* ctx, const DevScene& scene, const shiro_hip_settings& st, uint32_t count,
+                   uint64_t budget, WfPlan* out) {
+    WfPlan& p = *out;
+    p.tilesX = (st.width + 7u) / 8u;
+    const uint32_t tilesY = (st.height + 7u) / 8u;
+    p.tilesTotal = p.tilesX * tilesY;
+    const uint64_t perSample = static_cast<uint64_t>(p.tilesTotal) * 64u;
     const uint32_t K = st.domeLightSamples > 1u ? st.domeLightSamples : 1u;
     // Light samples per hit: the device's direct_sample_count (dev_path.h).
     const uint32_t lights = scene.dirCount + scene.domeCount * K +
                             (scene.emissiveCount > 0u && scene.emissiveSum > 0.0f ? 1u : 0u);
-    const uint32_t M = std::max(1u, lights);
-    const bool order = ctx->tuning.shadeOrder != 0u;
-    const uint32_t shadowRayBase = st.maxDepth + 1u;
-    const uint32_t shadowPathBase = shadowRayBase + st.maxDepth;
+    p.M = std::max(1u, lights);
+    p.order = ctx->tuning.shadeOrder != 0u;
+    p.shadowRayBase = st.maxDepth + 1u;
+    p.shadowPathBase = p.shadowRayBase + st.maxDepth;
     // The striped work counters start on a 128-byte line.
-    const uint32_t fetchBase = (shadowPathBase + st.maxDepth + kPoolStride - 1u) / kPoolStride * kPoolStride;
-    const uint32_t slots = fetchBase + 2u * st.maxDepth * kPoolStripes * kPoolStride;
-    // Pass plan: S samples per pass, T tiles per range.
-    const uint64_t budget = scratch ? kScratchBytes : kWavefrontStateBytes;
-    const uint64_t maxCap = (1ull << 32) / (2ull * M) - 1u;  // queue entries path * M + index stay below 2^32
-    size_t n[kWfArrays];
+    p.fetchBase = (p.shadowPathBase + st.maxDepth + kPoolStride - 1u) / kPoolStride * kPoolStride;
+    p.slots = p.fetchBase + 2u * st.maxDepth * kPoolStripes * kPoolStride;
+    const uint64_t maxCap = (1ull << 32) / (2ull * p.M) - 1u;  // queue entries path * M + index stay below 2^32
     auto bytes_for = [&](uint64_t S, uint64_t T) {
-        wavefront_sizes(S * T * 64u, M, order, S < count ? T * 64u : 0u, slots, n);
-        return WfMemory::footprint(n, kWfArrays);
+        wavefront_sizes(S * T * 64u, p.M, p.order, S < count ? T * 64u : 0u, p.slots, p.n);
+        return WfMemory::footprint(p.n, kWfArrays);
     };
     auto fits = [&](uint64_t S, uint64_t T) { return bytes_for(S, T) <= budget && S * T * 64u <= maxCap; };
     // Largest x in [1, hi] with ok(x), or 0 (ok is monotone).
@@ -308,27 +328,41 @@ int enqueue_wavefront(shiro_hip_ctx* ctx, const DevScene& scene, const shiro_hip
         }
         return lo;
     };
-    uint64_t S = std::min<uint64_t>(count, std::max<uint64_t>(1, ctx->tuning.pathsPerPass / perSample));
-    uint64_t T = tilesTotal;
-    if (!fits(S, T)) S = std::max<uint64_t>(1, largest(S, [&](uint64_t x) { return fits(x, T); }));
-    if (!fits(S, T)) {
+    p.S = std::min<uint64_t>(count, std::max<uint64_t>(1, ctx->tuning.pathsPerPass / perSample));
+    p.T = p.tilesTotal;
+    if (!fits(p.S, p.T)) p.S = std::max<uint64_t>(1, largest(p.S, [&](uint64_t x) { return fits(x, p.T); }));
+    if (!fits(p.S, p.T)) {
         // One sample of the whole frame does not fit: ranges of tiles.
-        T = largest(tilesTotal, [&](uint64_t x) { return fits(S, x); });
-        if (T == 0)
+        p.T = largest(p.tilesTotal, [&](uint64_t x) { return fits(p.S, x); });
+        if (p.T == 0)
             return fail(ctx, SHIRO_HIP_INVALID,
                         "one tile of wavefront path state (" + std::to_string(bytes_for(1, 1)) +
                             " bytes) exceeds the " + std::to_string(budget) + "-byte memory budget");
     }
-    const uint64_t cap = S * T * 64u;
-    wavefront_sizes(cap, M, order, S < count ? T * 64u : 0u, slots, n);
+    p.cap = p.S * p.T * 64u;
+    wavefront_sizes(p.cap, p.M, p.order, p.S < count ? p.T * 64u : 0u, p.slots, p.n);
+    return SHIRO_HIP_OK;
+}
+
+// `scene` is ctx's own scene except for batched renders; `events` records
+// ev0 / ev1 around the passes; a non-null `scratch` (kScratchBytes) holds
+// every array instead of `state`'s buffers.  Work goes to `stream` (null:
+// the context stream) with the arrays of `state` (null: the context's);
+// `budget` (0: the default for the memory used) bounds the path state.
+int enqueue_wavefront(shiro_hip_ctx* ctx, const DevScene& scene, const shiro_hip_camera& cam,
+                      const shiro_hip_settings& st, uint32_t start, uint32_t count, const DevFrame& frame,
+                      const volatile uint8_t* cancel, bool events = true, void* scratch = nullptr,
+                      hipStream_t stream = nullptr, WfState* state = nullptr, uint64_t budget = 0) {
+    if (!stream) stream = ctx->stream;
+    if (!state) state = &ctx->wfs;
+    if (budget == 0) budget = scratch ? kScratchBytes : kWavefrontStateBytes;
+    WfPlan plan;
+    int rc = wavefront_plan(ctx, scene, st, count, budget, &plan);
+    if (rc) return rc;
+    const size_t* n = plan.n;
     WfMemory mem{static_cast<unsigned char*>(scratch), scratch ? kScratchBytes : 0};
     void* a[kWfArrays];
-    for (size_t i = 0; i < 20; ++i) HIPC(mem.get(ctx->wf[i], n[i], &a[i]), "alloc wavefront state");
-    HIPC(mem.get(ctx->counters, n[20], &a[20]), "alloc counters");
-    HIPC(mem.get(ctx->sumB, n[21], &a[21]), "alloc sums");
-    HIPC(mem.get(ctx->sumA, n[22], &a[22]), "alloc sums");
-    HIPC(mem.get(ctx->sumN, n[23], &a[23]), "alloc sums");
-    HIPC(mem.get(ctx->sumD, n[24], &a[24]), "alloc sums");
+    for (size_t i = 0; i < kWfArrays; ++i) HIPC(mem.get(state->a[i], n[i], &a[i]), "alloc wavefront state");
     WfParams P{};
     P.scene = scene;
     P.camera = to_dev(cam);
@@ -345,51 +379,52 @@ int enqueue_wavefront(shiro_hip_ctx* ctx, const DevScene& scene, const shiro_hip
     B.qsr = static_cast<uint32_t*>(a[14]);
     B.svis = static_cast<uint32_t*>(a[15]);
     B.counts = static_cast<uint32_t*>(a[20]);
-    B.countSlots = slots;
-    B.shadowRayBase = shadowRayBase;
-    B.shadowPathBase = shadowPathBase;
-    B.fetchBase = fetchBase;
-    B.maxShadow = M;
-    B.cap = static_cast<uint32_t>(cap);
-    B.shadeOrder = order ? 1u : 0u;
-    B.qkey = order ? static_cast<uint8_t*>(a[16]) : nullptr;
-    B.qsorted = order ? static_cast<uint32_t*>(a[17]) : nullptr;
-    B.blockCount = order ? static_cast<uint32_t*>(a[18]) : nullptr;
-    B.blockStart = order ? static_cast<uint32_t*>(a[19]) : nullptr;
+    B.countSlots = plan.slots;
+    B.shadowRayBase = plan.shadowRayBase;
+    B.shadowPathBase = plan.shadowPathBase;
+    B.fetchBase = plan.fetchBase;
+    B.maxShadow = plan.M;
+    B.cap = static_cast<uint32_t>(plan.cap);
+    B.shadeOrder = plan.order ? 1u : 0u;
+    B.qkey = plan.order ? static_cast<uint8_t*>(a[16]) : nullptr;
+    B.qsorted = plan.order ? static_cast<uint32_t*>(a[17]) : nullptr;
+    B.blockCount = plan.order ? static_cast<uint32_t*>(a[18]) : nullptr;
+    B.blockStart = plan.order ? static_cast<uint32_t*>(a[19]) : nullptr;
     P.sampleStart = start;
     P.batchCount = count;
-    P.tilesX = tilesX;
+    P.tilesX = plan.tilesX;
     WfLaunch L{};
     L.maxBlocks = ctx->tuning.stageBlocks;
     L.traceBlocks = ctx->tuning.traceBlocks;
     L.fetchMode = ctx->tuning.fetchMode;
     L.primaryPackets = ctx->tuning.primaryPackets;
-    L.bvhWidth = ctx->tuning.bvhWidth == 8u && ctx->dev.nodes8 ? 8u : 4u;
+    L.bvhWidth = ctx->tuning.bvhWidth == 8u && scene.nodes8 ? 8u : 4u;
     if (events) {
-        HIPC(hipEventRecord(ctx->ev0, ctx->stream), "event");
+        HIPC(hipEventRecord(ctx->ev0, stream), "event");
         begin_marks(ctx);
     }
-    for (uint32_t t0 = 0; t0 < tilesTotal; t0 += static_cast<uint32_t>(T)) {
+    const uint32_t S = static_cast<uint32_t>(plan.S), T = static_cast<uint32_t>(plan.T);
+    for (uint32_t t0 = 0; t0 < plan.tilesTotal; t0 += T) {
         P.tileBase = t0;
-        P.tiles = std::min(static_cast<uint32_t>(T), tilesTotal - t0);
-        for (uint32_t b = 0; b < count; b += static_cast<uint32_t>(S)) {
+        P.tiles = std::min(T, plan.tilesTotal - t0);
+        for (uint32_t b = 0; b < count; b += S) {
             if (cancel && *cancel) {
-                (void)hipStreamSynchronize(ctx->stream);
+                (void)hipStreamSynchronize(stream);
                 return fail(ctx, SHIRO_HIP_CANCELLED, "cancelled");
             }
             P.sliceBegin = b;
-            P.sliceCount = std::min(static_cast<uint32_t>(S), count - b);
+            P.sliceCount = std::min(S, count - b);
             // Path groups of G samples as asked, or the largest smaller
             // choice that divides the pass's samples per pixel.
             uint32_t g = L.primaryPackets ? L.primaryPackets : 1u;
             while (g > 1u && P.sliceCount % g != 0u) g /= 4u;
             P.groupLog = g == 64u ? 6u : (g == 16u ? 4u : (g == 4u ? 2u : 0u));
-            HIPC(launch_wavefront_pass(P, L, ctx->stream, ctx->stageTiming && events ? &ctx->marks : nullptr),
+            HIPC(launch_wavefront_pass(P, L, stream, ctx->stageTiming && events ? &ctx->marks : nullptr),
                  "wavefront launch");
-            if (cancel) HIPC(hipStreamSynchronize(ctx->stream), "wavefront");
+            if (cancel) HIPC(hipStreamSynchronize(stream), "wavefront");
         }
     }
-    if (events) HIPC(hipEventRecord(ctx->ev1, ctx->stream), "event");
+    if (events) HIPC(hipEventRecord(ctx->ev1, stream), "event");
     return SHIRO_HIP_OK;
 }
 
@@ -546,6 +581,13 @@ void shiro_hip_destroy(shiro_hip_ctx* ctx) {
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     for (hipEvent_t e : ctx->stageEv) (void)hipEventDestroy(e);
+    if (ctx->manyExec) (void)hipGraphExecDestroy(ctx->manyExec);
+    for (hipStream_t b : ctx->branchStreams) {
+        (void)hipStreamSynchronize(b);
+        (void)hipStreamDestroy(b);
+    }
+    for (hipEvent_t e : ctx->joinEv) (void)hipEventDestroy(e);
+    if (ctx->forkEv) (void)hipEventDestroy(ctx->forkEv);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -1212,7 +1254,7 @@ int shiro_hip_set_tuning(shiro_hip_ctx* ctx, const shiro_hip_tuning* t) {
     if (t->pathsPerPass == 0 || t->fetchMode > 2u || t->traceBlocks == 0 || t->stageBlocks == 0 ||
         t->batchBlocksPerCU == 0 || t->batchBlocksPerCU > 64u || t->shadeOrder > 1u ||
         (t->primaryPackets > 1u && t->primaryPackets != 4u && t->primaryPackets != 16u && t->primaryPackets != 64u) ||
-        (t->bvhWidth != 4u && t->bvhWidth != 8u))
+        (t->bvhWidth != 4u && t->bvhWidth != 8u) || t->manyBranches == 0 || t->manyBranches > 16u)
         return fail(ctx, SHIRO_HIP_INVALID, "invalid tuning");
     ctx->tuning = *t;
     return SHIRO_HIP_OK;
@@ -1532,6 +1574,141 @@ int many_params(shiro_hip_ctx* ctx, const shiro_hip_batch_item* items, uint32_t 
     return SHIRO_HIP_OK;
 }
 
+// FNV-1a over 8-byte words, for the batch graph's key.
+struct KeyHash {
+    uint64_t h = 1469598103934665603ull;
+    void add(const void* p, size_t n) {
+        const unsigned char* b = static_cast<const unsigned char*>(p);
+        size_t i = 0;
+        for (; i + 8 <= n; i += 8) {
+            uint64_t w;
+            std::memcpy(&w, b + i, 8);
+            h = (h ^ w) * 1099511628211ull;
+        }
+        for (; i < n; ++i) h = (h ^ b[i]) * 1099511628211ull;
+    }
+};
+
+// The batch as one graph (SHIRO_HIP_MANY_WAVEFRONT_GRAPH).  Items go
+// longest first (cost: samples x (4 + BVH depth)) to the branch with the
+// least work so far; each branch is a stream that renders its items one
+// after the other through the wavefront with its own state (path state of
+// at most kWavefrontStateBytes / branches).  The fork/join of the branches
+// is captured once into a hipGraph and instantiated; a call on the same
+// items (scenes, cameras, settings, frames, tuning, buffer addresses)
+// replays it without capturing, and a call on other items captures anew and
+// updates the instantiated graph in place when its topology allows, else
+// instantiates it again.  The frames equal single renders bit for bit: every
+// item runs the same passes as alone, on its own arrays.
+int enqueue_many_graph(shiro_hip_ctx* ctx, const std::vector<RenderParams>& P) {
+    const uint32_t nItems = static_cast<uint32_t>(P.size());
+    const uint32_t B = std::max(1u, std::min(ctx->tuning.manyBranches, nItems));
+    while (ctx->branchStreams.size() < B) {
+        hipStream_t st = nullptr;
+        HIPC(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "branch stream");
+        ctx->branchStreams.push_back(st);
+        hipEvent_t e = nullptr;
+        HIPC(hipEventCreateWithFlags(&e, hipEventDisableTiming), "branch event");
+        ctx->joinEv.push_back(e);
+        ctx->branchStates.push_back(std::make_unique<WfState>());
+    }
+    if (!ctx->forkEv) HIPC(hipEventCreateWithFlags(&ctx->forkEv, hipEventDisableTiming), "branch event");
+    // Longest first onto the least loaded branch.
+    std::vector<uint32_t> order(nItems), branch(nItems);
+    for (uint32_t i = 0; i < nItems; ++i) order[i] = i;
+    auto cost = [&](uint32_t i) {
+        const shiro_hip_batch_item& it = *ctx->manyItemsHost[i];
+        return static_cast<double>(it.settings.width) * it.settings.height * it.sampleCount *
+               (4.0 + static_cast<double>(ctx->manyDepth[i]));
+    };
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return cost(a) > cost(b); });
+    std::vector<double> load(B, 0.0);
+    for (uint32_t i : order) {
+        const uint32_t b = static_cast<uint32_t>(std::min_element(load.begin(), load.end()) - load.begin());
+        branch[i] = b;
+        load[b] += cost(i);
+    }
+    // Every array sized before the capture (nothing is allocated inside it).
+    const uint64_t budget = kWavefrontStateBytes / B;
+    for (uint32_t i : order) {
+        const shiro_hip_batch_item& it = *ctx->manyItemsHost[i];
+        WfPlan plan;
+        int rc = wavefront_plan(ctx, P[i].scene, it.settings, it.sampleCount, budget, &plan);
+        if (rc) return rc;
+        WfState& S = *ctx->branchStates[branch[i]];
+        for (size_t k = 0; k < kWfArrays; ++k) HIPC(S.a[k].reserve(std::max<size_t>(plan.n[k], 16)), "alloc state");
+    }
+    KeyHash key;
+    key.add(&B, sizeof(B));
+    key.add(&ctx->tuning, sizeof(ctx->tuning));
+    for (uint32_t i : order) {
+        const shiro_hip_batch_item& it = *ctx->manyItemsHost[i];
+        key.add(&branch[i], sizeof(uint32_t));
+        key.add(&P[i].scene, sizeof(DevScene));
+        key.add(&it.camera, sizeof(it.camera));
+        key.add(&it.settings, sizeof(it.settings));
+        key.add(&it.sampleStart, sizeof(it.sampleStart));
+        key.add(&it.sampleCount, sizeof(it.sampleCount));
+        key.add(&P[i].frame, sizeof(DevFrame));
+    }
+    for (uint32_t b = 0; b < B; ++b)
+        for (size_t k = 0; k < kWfArrays; ++k) key.add(&ctx->branchStates[b]->a[k].p, sizeof(void*));
+    if (!ctx->manyExec || key.h != ctx->manyKey) {
+        hipGraph_t graph = nullptr;
+        HIPC(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal), "graph capture");
+        int rc = SHIRO_HIP_OK;
+        hipError_t e = hipEventRecord(ctx->forkEv, ctx->stream);
+        for (uint32_t b = 0; b < B && e == hipSuccess; ++b) e = hipStreamWaitEvent(ctx->branchStreams[b], ctx->forkEv, 0);
+        for (uint32_t i : order) {
+            if (e != hipSuccess || rc) break;
+            const shiro_hip_batch_item& it = *ctx->manyItemsHost[i];
+            const uint32_t b = branch[i];
+            rc = enqueue_wavefront(ctx, P[i].scene, it.camera, it.settings, it.sampleStart, it.sampleCount,
+                                   P[i].frame, nullptr, false, nullptr, ctx->branchStreams[b],
+                                   ctx->branchStates[b].get(), budget);
+        }
+        for (uint32_t b = 0; b < B && e == hipSuccess && !rc; ++b) {
+            e = hipEventRecord(ctx->joinEv[b], ctx->branchStreams[b]);
+            if (e == hipSuccess) e = hipStreamWaitEvent(ctx->stream, ctx->joinEv[b], 0);
+        }
+        const hipError_t e2 = hipStreamEndCapture(ctx->stream, &graph);
+        if (rc || e != hipSuccess || e2 != hipSuccess) {
+            if (graph) (void)hipGraphDestroy(graph);
+            if (rc) return rc;
+            return hip_fail(ctx, e != hipSuccess ? e : e2, "graph capture");
+        }
+        ++ctx->manyCaptures;
+        bool updated = false;
+        if (ctx->manyExec) {
+            hipGraphNode_t errNode = nullptr;
+            hipGraphExecUpdateResult res = hipGraphExecUpdateSuccess;
+            updated = hipGraphExecUpdate(ctx->manyExec, graph, &errNode, &res) == hipSuccess &&
+                      res == hipGraphExecUpdateSuccess;
+            if (updated) {
+                ++ctx->manyUpdates;
+            } else {
+                (void)hipGetLastError();
+                (void)hipGraphExecDestroy(ctx->manyExec);
+                ctx->manyExec = nullptr;
+            }
+        }
+        if (!updated) {
+            e = hipGraphInstantiate(&ctx->manyExec, graph, nullptr, nullptr, 0);
+            if (e != hipSuccess) {
+                ctx->manyExec = nullptr;
+                (void)hipGraphDestroy(graph);
+                return hip_fail(ctx, e, "graph instantiate");
+            }
+        }
+        (void)hipGraphDestroy(graph);
+        ctx->manyKey = key.h;
+    }
+    HIPC(hipEventRecord(ctx->ev0, ctx->stream), "event");
+    HIPC(hipGraphLaunch(ctx->manyExec, ctx->stream), "graph launch");
+    HIPC(hipEventRecord(ctx->ev1, ctx->stream), "event");
+    return SHIRO_HIP_OK;
+}
+
 // Enqueues one batched render in `mode` between ev0 and ev1.
 int enqueue_many(shiro_hip_ctx* ctx, const std::vector<RenderParams>& P, bool tex, uint32_t mode) {
     if (mode == SHIRO_HIP_MANY_PERSISTENT) {
@@ -1566,6 +1743,7 @@ int enqueue_many(shiro_hip_ctx* ctx, const std::vector<RenderParams>& P, bool te
         HIPC(hipEventRecord(ctx->ev1, ctx->stream), "event");
         return SHIRO_HIP_OK;
     }
+    if (mode == SHIRO_HIP_MANY_WAVEFRONT_GRAPH) return enqueue_many_graph(ctx, P);
     if (mode == SHIRO_HIP_MANY_WAVEFRONT) {
         HIPC(hipEventRecord(ctx->ev0, ctx->stream), "event");
         for (size_t i = 0; i < P.size(); ++i) {
@@ -1610,7 +1788,7 @@ int shiro_hip_render_many(shiro_hip_ctx* ctx, const shiro_hip_batch_item* items,
     if (!ctx) return SHIRO_HIP_INVALID;
     std::lock_guard<std::mutex> lock(ctx->mu);
     if (count > 0 && !items) return fail(ctx, SHIRO_HIP_INVALID, "items is null");
-    if (mode > SHIRO_HIP_MANY_WAVEFRONT) return fail(ctx, SHIRO_HIP_INVALID, "unknown batch mode");
+    if (mode > SHIRO_HIP_MANY_WAVEFRONT_GRAPH) return fail(ctx, SHIRO_HIP_INVALID, "unknown batch mode");
     HIPC(hipSetDevice(ctx->device), "set device");
     std::vector<RenderParams> P;
     bool tex = false;
@@ -1630,6 +1808,13 @@ int shiro_hip_render_many(shiro_hip_ctx* ctx, const shiro_hip_batch_item* items,
     return finish_timing(ctx);
 }
 
+int shiro_hip_many_graph_stats(const shiro_hip_ctx* ctx, uint32_t* captures, uint32_t* updates) {
+    if (!ctx) return SHIRO_HIP_INVALID;
+    if (captures) *captures = ctx->manyCaptures;
+    if (updates) *updates = ctx->manyUpdates;
+    return SHIRO_HIP_OK;
+}
+
 int shiro_hip_autotune_many(shiro_hip_ctx* ctx, const shiro_hip_batch_item* items, uint32_t count,
                             shiro_hip_tuning* best, char* report, size_t reportSize) {
     if (!ctx) return SHIRO_HIP_INVALID;
@@ -1641,8 +1826,12 @@ int shiro_hip_autotune_many(shiro_hip_ctx* ctx, const shiro_hip_batch_item* item
     int rc = many_params(ctx, items, count, &P, &tex);
     if (rc) return rc;
     ctx->manyDepth.clear();
+    ctx->manyItemsHost.clear();
     for (uint32_t i = 0; i < count; ++i)
-        if (items[i].sampleCount > 0) ctx->manyDepth.push_back(items[i].scene->info.bvhDepth);
+        if (items[i].sampleCount > 0) {
+            ctx->manyDepth.push_back(items[i].scene->info.bvhDepth);
+            ctx->manyItemsHost.push_back(items + i);
+        }
     const shiro_hip_tuning base = ctx->tuning;
     shiro_hip_tuning bestT = base;
     float bestMs = 0.0f;
@@ -1666,6 +1855,33 @@ int shiro_hip_autotune_many(shiro_hip_ctx* ctx, const shiro_hip_batch_item* item
         if (first || ms < bestMs) {
             bestMs = ms;
             bestT = ctx->tuning;
+            first = false;
+        }
+    }
+    // Branches of the graph-captured wavefront batch: the first call of a
+    // count captures and instantiates, the second replays and is timed.
+    ctx->tuning = bestT;
+    float bestGraph = 0.0f;
+    first = true;
+    for (uint32_t branches : {1u, 2u, 3u, 4u, 6u, 8u}) {
+        if (branches > std::max<size_t>(1, P.size())) break;
+        ctx->tuning.manyBranches = branches;
+        float ms = 0.0f;
+        for (int rep = 0; rep < 2 && !P.empty(); ++rep) {
+            rc = enqueue_many(ctx, P, tex, SHIRO_HIP_MANY_WAVEFRONT_GRAPH);
+            if (!rc) rc = finish_timing(ctx);
+            if (rc) {
+                ctx->tuning = base;
+                return rc;
+            }
+            ms = ctx->lastMs;
+        }
+        char line[96];
+        std::snprintf(line, sizeof(line), "wavefront graph, manyBranches %u: %.3f ms\n", branches, ms);
+        log += line;
+        if (first || ms < bestGraph) {
+            bestGraph = ms;
+            bestT.manyBranches = branches;
             first = false;
         }
     }

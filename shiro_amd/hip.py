@@ -101,6 +101,7 @@ def load_library(path: str = LIB_PATH):
     d("shiro_hip_update_environment", C.c_int, V, _u, _fp, _u)
     d("shiro_hip_render_many", C.c_int, V, _P(abi.BatchItem), _u, _u)
     d("shiro_hip_autotune_many", C.c_int, V, _P(abi.BatchItem), _u, _P(abi.Tuning), C.c_char_p, C.c_size_t)
+    d("shiro_hip_many_graph_stats", C.c_int, V, _P(C.c_uint32), _P(C.c_uint32))
     d("shiro_hip_refit_scene", C.c_int, V, _P(abi.Scene))
     d("shiro_hip_triangulate", C.c_int, _P(C.c_int32), _u, _P(C.c_int32), _u, _up, C.c_uint64, _P(C.c_uint64))
     d("shiro_hip_smooth_normals", C.c_int, _fp, _u, _up, _u, _fp)
@@ -480,6 +481,13 @@ class Context:
         stream; returns the kernel milliseconds."""
         self._check(self.L.shiro_hip_render_many(self.h, items, len(items), mode))
         return self.last_render_ms()
+
+    def many_graph_stats(self):
+        """(graphs captured, captures that updated the instantiated graph)
+        of the graph-captured wavefront batch mode on this context."""
+        a, b = C.c_uint32(), C.c_uint32()
+        self._check(self.L.shiro_hip_many_graph_stats(self.h, C.byref(a), C.byref(b)))
+        return a.value, b.value
 
     def autotune_many(self, items):
         best = abi.Tuning()
