@@ -260,7 +260,8 @@ def bench_c5(args):
         "config": {"workload": workload_desc(wl),
                    "parameters": "128x128x3 base colour + 128x128x1 roughness + 256x128x3 dome map texels",
                    "forward_ms": round(sum(fwd_ms) / len(fwd_ms), 3), "adjoint_ms": round(sum(adj_ms) / len(adj_ms), 3),
-                   "loss_first": losses[0], "loss_last": losses[-1]},
+                   "loss_first": losses[0], "loss_last": losses[-1],
+                   "adam_steps": len(losses), "loss_curve": [float(f"{x:.5g}") for x in losses]},
         "roofline": None, "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
@@ -347,7 +348,8 @@ def summarize_profile(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=None,
+                    help="timed steps (default 5; c5: 100 Adam steps, whose loss curve is reported)")
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="c3", choices=["c1", "c2", "c3", "c4", "c5"])
     ap.add_argument("--transport", default="wavefront", choices=["megakernel", "wavefront"])
@@ -372,6 +374,8 @@ def main():
                     help="summarise rocprofv3 csv output directories into HBM bytes per dispatch (no GPU)")
     ap.add_argument("--copy-bytes", type=int, default=1 << 30, help="bytes the --copy-probe run copied")
     args = ap.parse_args()
+    if args.steps is None:
+        args.steps = 100 if args.workload == "c5" else 5
     if args.summarize_profile:
         summarize_profile(args)
         return

@@ -178,3 +178,16 @@ def test_feature_fixtures_reach_their_branch(plant, name):
     # Switched off again, the oracle matches the fixture bit for bit.
     fb = OracleScene(wl.scene, bvh=True).render(wl.camera, wl.settings)
     assert _same(fb.beauty, z["beauty"])
+
+
+def test_convergence_fixture_matches_its_workload():
+    """tests/golden/conv_c2.npz (the oracle's 16,384-spp c2 mini frame, the
+    reference of the device's convergence test) belongs to the current
+    workload generator and holds a finite frame that covers the scene."""
+    from helpers import c2_mini
+    z = _load("conv_c2.npz")
+    wl = c2_mini(width=64, spp=16)
+    assert _scene_hash(wl.scene) == str(z["scene_hash"])
+    b = z["beauty"]
+    assert b.shape == (64 * 36, 4) and np.isfinite(b).all() and int(z["spp"]) == 16384
+    assert 0.0 < b[:, 3].mean() <= 1.0
