@@ -352,7 +352,7 @@ def main():
                     help="timed steps (default 5; c5: 100 Adam steps, whose loss curve is reported)")
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="c3", choices=["c1", "c2", "c3", "c4", "c5"])
-    ap.add_argument("--transport", default="wavefront", choices=["megakernel", "wavefront"])
+    ap.add_argument("--transport", default="wavefront", choices=["megakernel", "wavefront", "megakernel_plus"])
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
@@ -406,7 +406,8 @@ def main():
     from shiro_amd.scene import RenderSettings
     wl = workloads.WORKLOADS[args.workload]()
     st = wl.settings
-    st.transport = abi.TRANSPORT_WAVEFRONT if args.transport == "wavefront" else abi.TRANSPORT_MEGAKERNEL
+    st.transport = {"wavefront": abi.TRANSPORT_WAVEFRONT, "megakernel": abi.TRANSPORT_MEGAKERNEL,
+                    "megakernel_plus": abi.TRANSPORT_MEGAKERNEL_PLUS}[args.transport]
     print(f"# workload {workload_desc(wl)}", file=sys.stderr, flush=True)
     ctx = Context(0)
     t0 = time.perf_counter()
@@ -518,7 +519,8 @@ def main():
                 "shade": (shade_later, None),
             }
         else:
-            kernels = {"megakernel": ("k_render<false, false>", per_sample)}
+            sym = "k_render_plus<false>" if args.transport == "megakernel_plus" else "k_render<false, false>"
+            kernels = {"megakernel": (sym, per_sample)}
         entries = []
         for stage, (sym, credit) in kernels.items():
             if stage not in stages:
@@ -589,6 +591,16 @@ def main():
                   "other_ms": round(other_ms, 2), "speedup_vs_other": round(other_ms / (sum(gpu_ms) / len(gpu_ms)), 3),
                   "beauty_identical": all(bool(torch.equal(a, b)) for a, b in zip(bufs, obufs)),
                   "pixels_differing": diff}
+        if st.transport == abi.TRANSPORT_WAVEFRONT:
+            # The megakernel with path regeneration and LDS-ring stacks, a
+            # stronger single-kernel baseline, on the same samples.
+            plus = RenderSettings(**{**st.__dict__, "transport": abi.TRANSPORT_MEGAKERNEL_PLUS})
+            ctx.render_device(wl.camera, plus, [b.data_ptr() for b in obufs], 0, st.samplesPerPixel, synchronize=True)
+            plus_ms = ctx.last_render_ms()
+            torch.cuda.synchronize()
+            parity.update(megakernel_plus_ms=round(plus_ms, 2),
+                          speedup_vs_megakernel_plus=round(plus_ms / (sum(gpu_ms) / len(gpu_ms)), 3),
+                          megakernel_plus_identical=all(bool(torch.equal(a, b)) for a, b in zip(bufs, obufs)))
         del obufs
     t0 = time.perf_counter()
     host = ctx.render(wl.camera, st)

@@ -54,7 +54,10 @@ typedef enum shiro_hip_env_layout {
 /* Transport form used by render calls. */
 typedef enum shiro_hip_transport {
     SHIRO_HIP_TRANSPORT_MEGAKERNEL = 0, /* one fused TracePath per thread */
-    SHIRO_HIP_TRANSPORT_WAVEFRONT = 1   /* queue-driven stage kernels */
+    SHIRO_HIP_TRANSPORT_WAVEFRONT = 1,  /* queue-driven stage kernels */
+    SHIRO_HIP_TRANSPORT_MEGAKERNEL_PLUS = 2 /* the megakernel with path regeneration: persistent lanes
+                                               take the next sample or pixel as soon as a path ends,
+                                               traversal stacks in LDS rings; same frames */
 } shiro_hip_transport;
 
 /* PbrMaterial, reference Renderer.h:56-102; same fields, same order.

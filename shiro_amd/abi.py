@@ -177,7 +177,7 @@ RESULT_NAMES = {
 ENV_AUTOMATIC, ENV_LATLONG, ENV_ANGULAR = 0, 1, 2
 TEXELS_DEVICE, TEXELS_EVAL_ONLY = 1, 2
 PROGRESSIVE_RESET = 1
-TRANSPORT_MEGAKERNEL, TRANSPORT_WAVEFRONT = 0, 1
+TRANSPORT_MEGAKERNEL, TRANSPORT_WAVEFRONT, TRANSPORT_MEGAKERNEL_PLUS = 0, 1, 2
 
 # Every symbol include/shiro_hip.h declares (checked by the ABI test).
 EXPORTS = [

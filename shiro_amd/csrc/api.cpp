@@ -118,6 +118,7 @@ struct shiro_hip_ctx {
     uint32_t computeUnits = 256;
     // Batched rendering: items, work list and the work counter.
     DevBuf manyItems, manyWork, manyCounter;
+    DevBuf plusCounter;  // pixel counter of the megakernel with path regeneration
     std::vector<uint32_t> manyDepth;   // BVH depth per item of the current batch (cost estimate)
     std::vector<const shiro_hip_batch_item*> manyItemsHost;  // non-empty items of the current batch
     // Batched wavefront in one graph: per branch a stream and its wavefront
@@ -210,7 +211,7 @@ int check_settings(shiro_hip_ctx* ctx, const shiro_hip_settings* s) {
     if (s->width == 0 || s->height == 0) return fail(ctx, SHIRO_HIP_INVALID, "zero-sized frame");
     if (static_cast<uint64_t>(s->width) * s->height > (1ull << 31))
         return fail(ctx, SHIRO_HIP_INVALID, "frame larger than 2^31 pixels");
-    if (s->transport > SHIRO_HIP_TRANSPORT_WAVEFRONT) return fail(ctx, SHIRO_HIP_INVALID, "unknown transport");
+    if (s->transport > SHIRO_HIP_TRANSPORT_MEGAKERNEL_PLUS) return fail(ctx, SHIRO_HIP_INVALID, "unknown transport");
     if (s->maxDepth > 255u || s->diffuseDepth > 255u || s->specularDepth > 255u)
         return fail(ctx, SHIRO_HIP_INVALID, "depth limits above 255 are not supported");
     if (s->domeLightSamples > 64u) return fail(ctx, SHIRO_HIP_INVALID, "domeLightSamples above 64 is not supported");
@@ -461,7 +462,13 @@ int enqueue_batch(shiro_hip_ctx* ctx, const shiro_hip_camera& cam, const shiro_h
         }
         P.sliceBegin = b;
         P.sliceCount = std::min(slice, count - b);
-        HIPC(launch_render(P, countFetches, ctx->stream), "render launch");
+        if (st.transport == SHIRO_HIP_TRANSPORT_MEGAKERNEL_PLUS && !countFetches) {
+            HIPC(ctx->plusCounter.reserve(256), "alloc counter");
+            HIPC(launch_render_plus(P, ctx->plusCounter.as<uint32_t>(), ctx->computeUnits * 16u, ctx->stream),
+                 "render launch");
+        } else {
+            HIPC(launch_render(P, countFetches, ctx->stream), "render launch");
+        }
         if (ctx->stageTiming) ctx->marks.mark(SHIRO_HIP_STAGE_MEGAKERNEL, ctx->stream);
         if (cancel) HIPC(hipStreamSynchronize(ctx->stream), "render");
     }

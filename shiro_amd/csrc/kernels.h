@@ -114,6 +114,9 @@ struct AdjointParams {
 };
 
 hipError_t launch_render(const RenderParams& P, bool count, hipStream_t stream);
+// The megakernel with path regeneration: `blocks` persistent one-wave
+// blocks; counter: one device word (cleared by the launch).
+hipError_t launch_render_plus(const RenderParams& P, uint32_t* counter, uint32_t blocks, hipStream_t stream);
 // items / work / counter in device memory; work = (item, tile) pairs.
 hipError_t launch_render_many(const RenderParams* items, const uint2* work, uint32_t workCount, uint32_t* counter,
                               uint32_t blocks, bool tex, hipStream_t s);

@@ -17,6 +17,7 @@
 #include <hip/hip_runtime.h>
 
 #include "dev_path.h"
+#include "dev_ring.h"
 #include "kernels.h"
 
 namespace shiro_hip {
@@ -35,6 +36,35 @@ DEV uint32_t xcd_tile(uint32_t bid, uint32_t nblocks) {
         base = full * per + (xcd - full) * (per - 1u);
     }
     return base + slot;
+}
+
+// The end of a launch for one pixel: the running sums kept for the next
+// slice, or on the last slice the batch means, beauty divided by the count
+// and albedo/normal scaled by its reciprocal (CpuPathTracer.cpp:2132-2137).
+DEV void store_pixel(const RenderParams& P, size_t pix, float4 beauty, float4 albedo, float4 normal, float depth) {
+    if (P.sliceBegin + P.sliceCount < P.batchCount) {
+        P.sums.beauty[pix] = beauty;
+        P.sums.albedo[pix] = albedo;
+        P.sums.normal[pix] = normal;
+        P.sums.depth[pix] = depth;
+        return;
+    }
+    const float n = static_cast<float>(P.batchCount);
+    const float inv = 1.0f / n;
+    float* b = P.frame.beauty + pix * 4u;
+    b[0] = beauty.x / n;
+    b[1] = beauty.y / n;
+    b[2] = beauty.z / n;
+    b[3] = beauty.w / n;
+    float* a = P.frame.albedo + pix * 3u;
+    a[0] = albedo.x * inv;
+    a[1] = albedo.y * inv;
+    a[2] = albedo.z * inv;
+    float* nn = P.frame.normal + pix * 3u;
+    nn[0] = normal.x * inv;
+    nn[1] = normal.y * inv;
+    nn[2] = normal.z * inv;
+    P.frame.depth[pix] = depth;
 }
 
 // The megakernel's work for one 8x8 tile of one batch (one lane per pixel).
@@ -90,36 +120,140 @@ DEV void render_tile(const RenderParams& P, uint32_t tile, uint32_t lane) {
         atomicAdd(&c->primaryNodeFetches, cnt.primaryNodes);
         atomicAdd(&c->primaryTriFetches, cnt.primaryTris);
     }
-    if (P.sliceBegin + P.sliceCount < P.batchCount) {
-        P.sums.beauty[pix] = beauty;
-        P.sums.albedo[pix] = albedo;
-        P.sums.normal[pix] = normal;
-        P.sums.depth[pix] = depth;
-        return;
-    }
-    // Batch means: beauty divided by the count, albedo/normal scaled by its
-    // reciprocal (CpuPathTracer.cpp:2132-2137).
-    const float n = static_cast<float>(P.batchCount);
-    const float inv = 1.0f / n;
-    float* b = P.frame.beauty + pix * 4u;
-    b[0] = beauty.x / n;
-    b[1] = beauty.y / n;
-    b[2] = beauty.z / n;
-    b[3] = beauty.w / n;
-    float* a = P.frame.albedo + pix * 3u;
-    a[0] = albedo.x * inv;
-    a[1] = albedo.y * inv;
-    a[2] = albedo.z * inv;
-    float* nn = P.frame.normal + pix * 3u;
-    nn[0] = normal.x * inv;
-    nn[1] = normal.y * inv;
-    nn[2] = normal.z * inv;
-    P.frame.depth[pix] = depth;
+    store_pixel(P, pix, beauty, albedo, normal, depth);
 }
 
 template <bool COUNT, bool TEX>
 __global__ __launch_bounds__(64) void k_render(RenderParams P) {
     render_tile<COUNT, TEX>(P, xcd_tile(blockIdx.x, gridDim.x), threadIdx.x);
+}
+
+// The megakernel with path regeneration ("megakernel+", the transport
+// SHIRO_HIP_TRANSPORT_MEGAKERNEL_PLUS): persistent one-wave blocks whose
+// lanes each take a pixel (in 8x8-tile order, from one counter, 64 at a time
+// per wave), run its samples one after the other and take the next pixel
+// when they are done.  The loop advances every lane by one bounce per
+// iteration, so a lane whose path ends starts its next sample (or pixel)
+// at once instead of idling until the wave's longest path ends, and the
+// traversal stacks keep their newest kRing entries in LDS as the
+// wavefront's traversal kernels do.  Every sample runs trace_path's
+// bounce body (dev_path.h) with the same functions in the same order, and
+// each pixel sums its samples in order, so frames equal k_render's bit for
+// bit.
+template <bool TEX>
+__global__ __launch_bounds__(64) void k_render_plus(RenderParams P, uint32_t* counter) {
+    const DevSettings& st = P.settings;
+    const DevScene& S = P.scene;
+    const uint32_t lane = threadIdx.x;
+    const uint32_t total = P.tilesX * ((st.height + 7u) / 8u) * 64u;
+    const uint32_t nLight = direct_sample_count(S, st);
+    uint32_t ovNode[kStackSize];
+    float ovT[kStackSize];
+    Counters<false> c;
+    bool active = false, exhausted = false;
+    uint32_t x = 0, y = 0, i = 0, depth = 0;
+    size_t pix = 0;
+    float4 beauty{}, albedo{}, normal{};
+    float dmin = kInf;
+    PathState ps;
+    for (;;) {
+        // Lanes without a pixel take the next ones: one atomic per wave.
+        const bool want = !active && !exhausted;
+        const unsigned long long wm = __ballot(want);
+        if (wm) {
+            const uint32_t cnt = static_cast<uint32_t>(__popcll(wm));
+            const int leader = __ffsll(static_cast<long long>(wm)) - 1;
+            uint32_t first = 0;
+            if (static_cast<int>(lane) == leader) first = atomicAdd(counter, cnt);
+            first = __shfl(first, leader);
+            if (want) {
+                const uint32_t idx = first + static_cast<uint32_t>(__popcll(wm & ((1ull << lane) - 1ull)));
+                if (idx >= total) {
+                    exhausted = true;
+                } else {
+                    const uint32_t tile = idx >> 6, l = idx & 63u;
+                    x = (tile % P.tilesX) * 8u + (l & 7u);
+                    y = (tile / P.tilesX) * 8u + (l >> 3);
+                    if (x < st.width && y < st.height) {
+                        pix = static_cast<size_t>(y) * st.width + x;
+                        if (P.sliceBegin == 0u) {
+                            beauty = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                            albedo = normal = beauty;
+                            dmin = kInf;
+                        } else {
+                            beauty = P.sums.beauty[pix];
+                            albedo = P.sums.albedo[pix];
+                            normal = P.sums.normal[pix];
+                            dmin = P.sums.depth[pix];
+                        }
+                        i = 0;
+                        depth = 0;
+                        ps = init_path(P.camera, st, x, y, Sampler{x, y, P.sampleStart + P.sliceBegin});
+                        active = P.sliceCount > 0u;
+                    }
+                }
+            }
+        }
+        if (__all(exhausted)) break;
+        if (!active) continue;
+        // One bounce of TracePath (dev_path.h:trace_path's loop body).
+        const Sampler smp{x, y, P.sampleStart + P.sliceBegin + i};
+        bool ended = depth >= st.maxDepth;
+        if (!ended) {
+            const Ray ray = path_ray(ps);
+            const HitRec h = intersect_closest_ring(S, ray, lane, ovNode, ovT);
+            if (h.slot == kNoHit) {
+                handle_miss(S, st, depth, ps, c);
+                ended = true;
+            } else {
+                Surface sf;
+                TexState ts;
+                if (shade_hit<TEX>(S, depth, smp, ray, h, ps, sf, &ts)) {
+                    const Mat& M = sf.M;
+                    DirectSum ds;
+                    ds.init(st);
+                    const V3 origin = sf.p + sf.n * kRayEps;
+                    for (uint32_t j = 0; j < nLight; ++j) {
+                        V3 ld, contrib;
+                        uint32_t group;
+                        float maxDist;
+                        if (!direct_sample(S, st, sf, M, smp, depth, j, &ld, &contrib, &group, &maxDist, c)) continue;
+                        Ray sr;
+                        sr.o = origin;
+                        sr.d = ld;
+                        sr.tmin = kRayEps;
+                        sr.tmax = maxDist;
+                        ds.add(group, contrib, !intersect_any_ring(S, sr, lane, ovNode));
+                    }
+                    ds.flush();
+                    ps.L = ps.L + ps.T * ds.L;
+                    if (!sample_bounce(S, st, depth, smp, sf, ps)) ended = true;
+                }
+            }
+            ++depth;
+            if (depth >= st.maxDepth) ended = true;
+        }
+        if (!ended) continue;
+        // The sample is done: add it in order, then the next sample.
+        beauty.x = beauty.x + ps.L.x;
+        beauty.y = beauty.y + ps.L.y;
+        beauty.z = beauty.z + ps.L.z;
+        beauty.w = beauty.w + (ps.depth < kInf ? 1.0f : 0.0f);
+        albedo.x = albedo.x + ps.albedo.x;
+        albedo.y = albedo.y + ps.albedo.y;
+        albedo.z = albedo.z + ps.albedo.z;
+        normal.x = normal.x + ps.normal.x;
+        normal.y = normal.y + ps.normal.y;
+        normal.z = normal.z + ps.normal.z;
+        if (ps.depth < dmin) dmin = ps.depth;
+        if (++i < P.sliceCount) {
+            depth = 0;
+            ps = init_path(P.camera, st, x, y, Sampler{x, y, P.sampleStart + P.sliceBegin + i});
+            continue;
+        }
+        store_pixel(P, pix, beauty, albedo, normal, dmin);
+        active = false;
+    }
 }
 
 // Batched multi-scene megakernel: persistent one-wave blocks take work
@@ -301,6 +435,18 @@ hipError_t launch_render(const RenderParams& P, bool count, hipStream_t stream) 
         } else {
             hipLaunchKernelGGL((k_render<false, false>), dim3(nb), dim3(64), 0, stream, P);
         }
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_render_plus(const RenderParams& P, uint32_t* counter, uint32_t blocks, hipStream_t stream) {
+    const uint32_t tilesY = (P.settings.height + 7u) / 8u;
+    if (P.tilesX * tilesY == 0 || blocks == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_clear_u32, dim3(1), dim3(64), 0, stream, counter, 1u);
+    if (P.scene.textured != 0u) {
+        hipLaunchKernelGGL((k_render_plus<true>), dim3(blocks), dim3(64), 0, stream, P, counter);
+    } else {
+        hipLaunchKernelGGL((k_render_plus<false>), dim3(blocks), dim3(64), 0, stream, P, counter);
     }
     return hipGetLastError();
 }

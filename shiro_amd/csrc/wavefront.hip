@@ -38,6 +38,7 @@
 #include <mutex>
 
 #include "dev_path.h"
+#include "dev_ring.h"
 #include "kernels.h"
 
 namespace shiro_hip {
@@ -197,84 +198,6 @@ struct WavePool {
         return idx;
     }
 };
-
-// Traversal stack of the persistent kernels.  The newest kRing entries of
-// each lane live in an LDS ring (slot k of lane i at [k * 64 + i]); older
-// entries move to a private overflow array, touched only while a lane holds
-// more than kRing entries.  Entries [0, base) are in the overflow array,
-// [base, sp) in the ring.  Any-hit traversal keeps node words only.
-constexpr int kRing = 8;
-__shared__ uint32_t gRingNode[kRing * 64];
-__shared__ float gRingT[kRing * 64];
-
-DEV uint32_t ring_slot(int i, uint32_t lane) { return static_cast<uint32_t>(i & (kRing - 1)) * 64u + lane; }
-
-template <bool KEEP_T>
-DEV void ring_push(uint32_t n, float t, uint32_t lane, uint32_t* ovNode, float* ovT, int& sp, int& base) {
-    if (sp - base == kRing) {
-        const uint32_t old = ring_slot(base, lane);
-        ovNode[base] = gRingNode[old];
-        if (KEEP_T) ovT[base] = gRingT[old];
-        ++base;
-    }
-    const uint32_t at = ring_slot(sp, lane);
-    gRingNode[at] = n;
-    if (KEEP_T) gRingT[at] = t;
-    ++sp;
-}
-
-// push_sorted (dev_bvh.h) onto the ring stack, for W-wide nodes.
-template <bool KEEP_T, int W>
-DEV uint32_t ring_push_sorted(uint32_t mask, const float* tn, const uint32_t* w, uint32_t lane, uint32_t* ovNode,
-                              float* ovT, int& sp, int& base) {
-    const int nh = __popc(mask);
-    if constexpr (W == 4) {
-        float k[4] = {(mask & 1u) ? tn[0] : kInf, (mask & 2u) ? tn[1] : kInf, (mask & 4u) ? tn[2] : kInf,
-                      (mask & 8u) ? tn[3] : kInf};
-        uint32_t v[4] = {w[0], w[1], w[2], w[3]};
-        cswap(k[0], v[0], k[1], v[1]);
-        cswap(k[2], v[2], k[3], v[3]);
-        cswap(k[0], v[0], k[2], v[2]);
-        cswap(k[1], v[1], k[3], v[3]);
-        cswap(k[1], v[1], k[2], v[2]);
-        for (int i = nh - 1; i >= 1; --i) ring_push<KEEP_T>(v[i], k[i], lane, ovNode, ovT, sp, base);
-        return v[0];
-    } else {
-        float k[8];
-        uint32_t v[8];
-#pragma unroll
-        for (int c = 0; c < 8; ++c) {
-            k[c] = (mask >> c) & 1u ? tn[c] : kInf;
-            v[c] = w[c];
-        }
-        sort8(k, v);
-#pragma unroll
-        for (int i = 7; i >= 1; --i)
-            if (i < nh) ring_push<KEEP_T>(v[i], k[i], lane, ovNode, ovT, sp, base);
-        return v[0];
-    }
-}
-
-// pop_entry (dev_bvh.h) from the ring stack.
-template <bool ANY>
-DEV uint32_t ring_pop(float tcull, uint32_t lane, const uint32_t* ovNode, const float* ovT, int& sp, int& base) {
-    while (sp > 0) {
-        --sp;
-        uint32_t n;
-        float t = 0.0f;
-        if (sp >= base) {
-            const uint32_t at = ring_slot(sp, lane);
-            n = gRingNode[at];
-            if (!ANY) t = gRingT[at];
-        } else {
-            n = ovNode[sp];
-            if (!ANY) t = ovT[sp];
-            base = sp;
-        }
-        if (ANY || t <= tcull) return n;
-    }
-    return kEmpty;
-}
 
 // Persistent traversal.  ANY = false: closest hit of ray-queue entries (all
 // paths at depth 0), result in H.  ANY = true: any hit of shadow-queue

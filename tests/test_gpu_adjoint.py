@@ -41,7 +41,8 @@ def c5m():
     return wl, scene
 
 
-@pytest.mark.parametrize("transport", [abi.TRANSPORT_MEGAKERNEL, abi.TRANSPORT_WAVEFRONT])
+@pytest.mark.parametrize("transport", [abi.TRANSPORT_MEGAKERNEL, abi.TRANSPORT_WAVEFRONT,
+                                       abi.TRANSPORT_MEGAKERNEL_PLUS])
 def test_textured_frame_vs_oracle(hip_ctx, c5m, transport):
     wl, scene = c5m
     _log(f"textured c5-mini transport {transport}")

@@ -72,6 +72,9 @@ def test_feature_transports_identical(hip_ctx, name):
     _log(f"feature {name}: megakernel vs wavefront variants")
     hip_ctx.upload(wl.scene)
     mk = hip_ctx.render(wl.camera, st)
+    mp = hip_ctx.render(wl.camera, RenderSettings(**{**st.__dict__, "transport": abi.TRANSPORT_MEGAKERNEL_PLUS}))
+    for k in ("beauty", "albedo", "normal", "depth"):
+        assert np.array_equal(getattr(mp, k), getattr(mk, k)), ("megakernel+", k)
     wst = RenderSettings(**{**st.__dict__, "transport": abi.TRANSPORT_WAVEFRONT})
     saved = hip_ctx.get_tuning()
     try:

@@ -349,6 +349,35 @@ def test_wavefront_equals_megakernel(hip_ctx, make):
         assert np.array_equal(a, b), (k, np.max(np.abs(a - b)))
 
 
+@pytest.mark.parametrize("make", MINI, ids=MINI_IDS)
+def test_megakernel_plus_equals_megakernel(hip_ctx, make):
+    """The megakernel with path regeneration runs the same bounce body per
+    sample and sums each pixel's samples in order: identical frames."""
+    from shiro_amd import abi
+    wl = make()
+    _log(f"megakernel+ vs megakernel {wl.name}")
+    hip_ctx.upload(wl.scene)
+    mk = hip_ctx.render(wl.camera, wl.settings)
+    st = RenderSettings(**{**wl.settings.__dict__, "transport": abi.TRANSPORT_MEGAKERNEL_PLUS})
+    mp = hip_ctx.render(wl.camera, st)
+    for k in ("beauty", "albedo", "normal", "depth"):
+        a, b = getattr(mp, k), getattr(mk, k)
+        assert np.array_equal(a, b), (k, np.max(np.abs(a - b)))
+
+
+def test_megakernel_plus_sliced_and_progressive(hip_ctx):
+    """Slices of one batch (running sums between launches) and batches
+    with sampleStart > 0 render the megakernel's frames."""
+    from shiro_amd import abi
+    wl = c1_mini(size=512, spp=40)  # two launch slices
+    hip_ctx.upload(wl.scene)
+    st = RenderSettings(**{**wl.settings.__dict__, "transport": abi.TRANSPORT_MEGAKERNEL_PLUS})
+    for start, count in ((0, 40), (7, 3)):
+        mk = hip_ctx.render(wl.camera, wl.settings, start, count)
+        mp = hip_ctx.render(wl.camera, st, start, count)
+        assert np.array_equal(mp.beauty, mk.beauty) and np.array_equal(mp.depth, mk.depth), (start, count)
+
+
 def test_wavefront_multi_pass(hip_ctx):
     """A batch spanning several wavefront passes (running sums carried
     between passes) equals the megakernel."""
