@@ -185,10 +185,11 @@ DEV void scatter_grad(const AdjointParams& P, GradCache& cache, uint32_t materia
 // bilinear footprint of the lookup.
 DEV void scatter_env(const AdjointParams& P, GradCache& cache, uint32_t i, V3 local, V3 g) {
     const DevScene& S = P.scene;
-    const DevDome& d = S.domes[i];
-    if (d.env < 0 || !P.envGrad[i]) return;
+    if (i >= static_cast<uint32_t>(kMaxDomes) || !P.envGrad[i]) return;
+    const DevDome d = dome_light(S, i);
+    if (d.env < 0) return;
     EnvTaps t;
-    if (!env_taps(S.envs[d.env], local, &t)) return;
+    if (!env_taps(env_map(S, d.env), local, &t)) return;
     const V3 gr = g * ld3(d.radiance);
     for (int k = 0; k < 4; ++k) {
         if (t.w[k] == 0.0f) continue;
@@ -231,7 +232,7 @@ DEV void adjoint_path(const AdjointParams& P, GradCache& cache, uint32_t x, uint
             if (P.envWanted && mis != 0.0f) {
                 const V3 g = a * V.T * mis;
                 for (uint32_t i = 0; i < S.domeCount; ++i)
-                    scatter_env(P, cache, i, world_to_dome(S.domes[i], ps.d), g);
+                    scatter_env(P, cache, i, world_to_dome(dome_light(S, i), ps.d), g);
             }
             break;
         }

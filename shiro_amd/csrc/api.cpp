@@ -14,6 +14,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <deque>
 #include <limits>
 #include <memory>
 #include <mutex>
@@ -98,7 +99,11 @@ struct shiro_hip_ctx {
     DevBuf nodes, nodes8, tris, hitdata, materials, derived, uvs, lightList, lightCdf, triClass;
     Bvh bvh;                // host copy of the uploaded trees, for refits
     uint64_t topology = 0;  // topology_hash of the uploaded scene
-    EnvBuffers env[kMaxDomes];
+    std::deque<EnvBuffers> env;           // per dome light (index = dome index)
+    std::vector<DevEnv> hostEnvs;         // host copies of the device light arrays
+    std::vector<DevDome> hostDomes;
+    std::vector<DevDirLight> hostDirs;
+    DevBuf envsBuf, domesBuf, dirsBuf;
     std::vector<std::unique_ptr<TexBuffers>> tex;
     DevBuf texTable;
     std::vector<shiro_hip_material> mats;   // host copy of the uploaded materials
@@ -622,8 +627,8 @@ int shiro_hip_get_status(const shiro_hip_ctx* ctx, shiro_hip_status* out) {
 
 int shiro_hip_supports_scene(const shiro_hip_scene* s) {
     if (!s) return 0;
-    if (s->distantLightCount > static_cast<uint32_t>(kMaxDirLights)) return 0;
-    if (s->domeLightCount > static_cast<uint32_t>(kMaxDomes)) return 0;
+    if (s->distantLightCount > kMaxSceneDirLights) return 0;
+    if (s->domeLightCount > kMaxSceneDomes) return 0;
     return 1;
 }
 
@@ -787,14 +792,18 @@ int upload_scene(shiro_hip_ctx* ctx, const shiro_hip_scene* s, bool refit) {
     for (const auto& b : ctx->tex) bytes += b->eval.bytes;
     // Lights.
     d.dirCount = s->distantLightCount;
+    ctx->hostDirs.assign(s->distantLightCount, DevDirLight{});
     for (uint32_t i = 0; i < s->distantLightCount; ++i) {
-        std::memcpy(d.dirs[i].direction, s->distantLights[i].direction, 12);
-        std::memcpy(d.dirs[i].radiance, s->distantLights[i].radiance, 12);
+        std::memcpy(ctx->hostDirs[i].direction, s->distantLights[i].direction, 12);
+        std::memcpy(ctx->hostDirs[i].radiance, s->distantLights[i].radiance, 12);
     }
     d.domeCount = s->domeLightCount;
+    ctx->hostDomes.assign(s->domeLightCount, DevDome{});
+    ctx->hostEnvs.assign(s->domeLightCount, DevEnv{});
+    if (ctx->env.size() < s->domeLightCount) ctx->env.resize(s->domeLightCount);
     for (uint32_t i = 0; i < s->domeLightCount; ++i) {
         const shiro_hip_dome_light& L = s->domeLights[i];
-        DevDome& D = d.domes[i];
+        DevDome& D = ctx->hostDomes[i];
         std::memcpy(D.radiance, L.radiance, 12);
         std::memcpy(D.right, L.right, 12);
         std::memcpy(D.up, L.up, 12);
@@ -832,7 +841,7 @@ int upload_scene(shiro_hip_ctx* ctx, const shiro_hip_scene* s, bool refit) {
                 HIPC(upload(ctx->env[i].rowGuide, rg.data(), rg.size(), ctx->stream), "upload env");
                 HIPC(upload(ctx->env[i].condGuide, cg.data(), cg.size(), ctx->stream), "upload env");
             }
-            DevEnv& E = d.envs[i];
+            DevEnv& E = ctx->hostEnvs[i];
             E.texels = ctx->env[i].texels.as<float4>();
             E.rowCdf = table ? ctx->env[i].row.as<float>() : nullptr;
             E.condCdf = table ? ctx->env[i].cond.as<float>() : nullptr;
@@ -847,6 +856,17 @@ int upload_scene(shiro_hip_ctx* ctx, const shiro_hip_scene* s, bool refit) {
                                                              ctx->env[i].rowGuide.bytes + ctx->env[i].condGuide.bytes
                                                        : 0);
         }
+    }
+    HIPC(upload(ctx->dirsBuf, ctx->hostDirs.data(), ctx->hostDirs.size(), ctx->stream), "upload lights");
+    HIPC(upload(ctx->domesBuf, ctx->hostDomes.data(), ctx->hostDomes.size(), ctx->stream), "upload lights");
+    HIPC(upload(ctx->envsBuf, ctx->hostEnvs.data(), ctx->hostEnvs.size(), ctx->stream), "upload lights");
+    d.dirs = ctx->dirsBuf.as<DevDirLight>();
+    d.domes = ctx->domesBuf.as<DevDome>();
+    d.envs = ctx->envsBuf.as<DevEnv>();
+    for (uint32_t i = 0; i < kInlineDirLights && i < ctx->hostDirs.size(); ++i) d.dirIn[i] = ctx->hostDirs[i];
+    for (uint32_t i = 0; i < kInlineDomes && i < ctx->hostDomes.size(); ++i) {
+        d.domeIn[i] = ctx->hostDomes[i];
+        d.envIn[i] = ctx->hostEnvs[i];
     }
     std::memcpy(d.envTop, s->environmentTop, 12);
     std::memcpy(d.envBottom, s->environmentBottom, 12);
@@ -1216,7 +1236,7 @@ int shiro_hip_env_sample(shiro_hip_ctx* ctx, uint32_t domeIndex, const float* u,
     if (!ctx || !u || !direction || !pdf || !radiance) return SHIRO_HIP_INVALID;
     std::lock_guard<std::mutex> lock(ctx->mu);
     if (!ctx->hasScene) return fail(ctx, SHIRO_HIP_NO_SCENE, "no scene uploaded");
-    if (domeIndex >= ctx->dev.domeCount || ctx->dev.domes[domeIndex].env < 0)
+    if (domeIndex >= ctx->dev.domeCount || ctx->hostDomes[domeIndex].env < 0)
         return fail(ctx, SHIRO_HIP_INVALID, "dome has no environment map");
     HIPC(hipSetDevice(ctx->device), "set device");
     Tmp uu, d, p, r;
@@ -1224,7 +1244,7 @@ int shiro_hip_env_sample(shiro_hip_ctx* ctx, uint32_t domeIndex, const float* u,
     HIPC(d.alloc<float>(3ull * n), "alloc");
     HIPC(p.alloc<float>(n), "alloc");
     HIPC(r.alloc<float>(3ull * n), "alloc");
-    HIPC(launch_env_sample(ctx->dev.envs[ctx->dev.domes[domeIndex].env], uu.b.as<float>(), n, d.b.as<float>(),
+    HIPC(launch_env_sample(ctx->hostEnvs[ctx->hostDomes[domeIndex].env], uu.b.as<float>(), n, d.b.as<float>(),
                            p.b.as<float>(), r.b.as<float>(), ctx->stream),
          "env launch");
     HIPC(d.out(direction, 3ull * n, ctx->stream), "readback");
@@ -1239,14 +1259,14 @@ int shiro_hip_env_eval(shiro_hip_ctx* ctx, uint32_t domeIndex, const float* dire
     if (!ctx || !direction || !pdf || !radiance) return SHIRO_HIP_INVALID;
     std::lock_guard<std::mutex> lock(ctx->mu);
     if (!ctx->hasScene) return fail(ctx, SHIRO_HIP_NO_SCENE, "no scene uploaded");
-    if (domeIndex >= ctx->dev.domeCount || ctx->dev.domes[domeIndex].env < 0)
+    if (domeIndex >= ctx->dev.domeCount || ctx->hostDomes[domeIndex].env < 0)
         return fail(ctx, SHIRO_HIP_INVALID, "dome has no environment map");
     HIPC(hipSetDevice(ctx->device), "set device");
     Tmp d, p, r;
     HIPC(d.in(direction, 3ull * n, ctx->stream), "upload");
     HIPC(p.alloc<float>(n), "alloc");
     HIPC(r.alloc<float>(3ull * n), "alloc");
-    HIPC(launch_env_eval(ctx->dev.envs[ctx->dev.domes[domeIndex].env], d.b.as<float>(), n, p.b.as<float>(),
+    HIPC(launch_env_eval(ctx->hostEnvs[ctx->hostDomes[domeIndex].env], d.b.as<float>(), n, p.b.as<float>(),
                          r.b.as<float>(), ctx->stream),
          "env launch");
     HIPC(p.out(pdf, n, ctx->stream), "readback");
@@ -1441,10 +1461,10 @@ int shiro_hip_update_environment(shiro_hip_ctx* ctx, uint32_t dome, const float*
     if (!ctx) return SHIRO_HIP_INVALID;
     std::lock_guard<std::mutex> lock(ctx->mu);
     if (!ctx->hasScene) return fail(ctx, SHIRO_HIP_NO_SCENE, "no scene uploaded");
-    if (dome >= ctx->dev.domeCount || ctx->dev.domes[dome].env < 0 || !texels)
+    if (dome >= ctx->dev.domeCount || ctx->hostDomes[dome].env < 0 || !texels)
         return fail(ctx, SHIRO_HIP_INVALID, "bad dome index, dome without a map, or null texels");
     HIPC(hipSetDevice(ctx->device), "set device");
-    EnvBuffers& E = ctx->env[ctx->dev.domes[dome].env];
+    EnvBuffers& E = ctx->env[ctx->hostDomes[dome].env];
     const size_t n = static_cast<size_t>(E.w) * E.h;
     const float* rgb = texels;
     if (!(flags & SHIRO_HIP_TEXELS_DEVICE)) {
@@ -1490,10 +1510,13 @@ int render_adjoint(shiro_hip_ctx* ctx, const shiro_hip_camera* camera, const shi
                                                 " textures");
     if (dEnvCount > ctx->dev.domeCount || (dEnvCount > 0 && !dEnv))
         return fail(ctx, SHIRO_HIP_INVALID, "more environment gradients than dome lights");
+    if (dEnvCount > static_cast<uint32_t>(kMaxDomes))
+        return fail(ctx, SHIRO_HIP_INVALID, "the adjoint differentiates the maps of at most " +
+                                                std::to_string(kMaxDomes) + " dome lights");
     for (uint32_t i = 0; i < dEnvCount; ++i) {
-        if (dEnv[i] && ctx->dev.domes[i].env < 0)
+        if (dEnv[i] && ctx->hostDomes[i].env < 0)
             return fail(ctx, SHIRO_HIP_INVALID, "environment gradient for dome " + std::to_string(i) + " without a map");
-        if (dEnv[i] && static_cast<uint64_t>(ctx->env[ctx->dev.domes[i].env].w) * ctx->env[ctx->dev.domes[i].env].h >=
+        if (dEnv[i] && static_cast<uint64_t>(ctx->env[ctx->hostDomes[i].env].w) * ctx->env[ctx->hostDomes[i].env].h >=
                            (1ull << 27))
             return fail(ctx, SHIRO_HIP_INVALID, "environment gradient of a map with 2^27 texels or more");
     }

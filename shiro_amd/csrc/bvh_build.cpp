@@ -53,6 +53,7 @@ struct Builder {
     std::atomic<uint32_t> nodeCount{1};
     std::atomic<int> threads{1};
     int maxThreads = 1;
+    uint32_t maxBinaryDepth = kMaxBinaryDepth;  // nodes at this depth become leaves
 };
 
 void build_rec(Builder& B, uint32_t ni, uint32_t begin, uint32_t end, uint32_t depth) {
@@ -71,7 +72,7 @@ void build_rec(Builder& B, uint32_t ni, uint32_t begin, uint32_t end, uint32_t d
         node.left = begin;
         node.count = count;
     };
-    if (count <= kLeafMax || depth >= kMaxBinaryDepth) {
+    if (count <= kLeafMax || depth >= B.maxBinaryDepth) {
         leaf();
         return;
     }
@@ -419,7 +420,28 @@ bool build_bvh(const float* tris, size_t n, uint32_t maxDepth, Bvh* out, std::st
         *err = "too many triangles for the 31-bit leaf slot index";
         return false;
     }
+    if (build_bvh_capped(tris, n, maxDepth, kMaxBinaryDepth, out, err)) return true;
+    // A tree deeper than the traversal stack allows (inputs whose SAH splits
+    // peel off a few triangles at a time, such as geometrically spaced
+    // ones): build again with binary nodes at depth maxDepth made leaves.
+    // Collapsing never deepens a tree, so the 4-wide tree then fits; its
+    // deepest leaves hold more triangles.
+    err->clear();
+    return build_bvh_capped(tris, n, maxDepth, maxDepth > 1u ? maxDepth - 1u : 1u, out, err);
+}
+
+bool build_bvh_capped(const float* tris, size_t n, uint32_t maxDepth, uint32_t binaryCap, Bvh* out,
+                      std::string* err) {
+    out->nodes.clear();
+    out->nodes8.clear();
+    out->order.clear();
+    out->leafEnd.clear();
+    out->depth = out->depth8 = 0;
+    out->stack = out->stack8 = 0;
+    out->leafCount = 0;
+    out->maxLeafSize = 0;
     Builder B;
+    B.maxBinaryDepth = binaryCap;
     B.refs.resize(n);
     for (size_t i = 0; i < n; ++i) {
         Ref& r = B.refs[i];

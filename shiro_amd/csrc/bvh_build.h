@@ -37,6 +37,10 @@ struct Bvh {
 // than maxDepth levels; the 8-wide tree is dropped (nodes8 empty) if a
 // traversal of it could hold more than kStackSize - 1 stack entries.
 bool build_bvh(const float* tris, size_t n, uint32_t maxDepth, Bvh* out, std::string* err);
+// build_bvh with binary nodes at depth binaryCap made leaves; build_bvh
+// falls back to binaryCap = maxDepth when its tree is too deep.
+bool build_bvh_capped(const float* tris, size_t n, uint32_t maxDepth, uint32_t binaryCap, Bvh* out,
+                      std::string* err);
 
 // Refit: recomputes every box of both built trees for moved vertices of
 // the same n triangles (same order as given to build_bvh), keeping the

@@ -40,6 +40,33 @@ DEV bool occluded(const DevScene& S, V3 o, V3 d, float maxDist, Counters<COUNT>&
     return intersect_any(S, r, c.shadow_trav());
 }
 
+// A wave-uniform record in device memory, read into scalar registers
+// (every lane asks for the same address).
+template <typename T>
+DEV T uniform_load(const T* p) {
+    static_assert(sizeof(T) % 4 == 0, "record of whole words");
+    const uint32_t* w = reinterpret_cast<const uint32_t*>(p);
+    uint32_t v[sizeof(T) / 4];
+#pragma unroll
+    for (size_t k = 0; k < sizeof(T) / 4; ++k) v[k] = __builtin_amdgcn_readfirstlane(w[k]);
+    T out;
+    __builtin_memcpy(&out, v, sizeof(T));
+    return out;
+}
+
+// Light records: the first ones from the kernel arguments, the rest from
+// the scene's device arrays.  Every index here is wave-uniform.
+DEV DevDirLight dir_light(const DevScene& S, uint32_t j) {
+    return j < kInlineDirLights ? S.dirIn[j] : uniform_load(S.dirs + j);
+}
+DEV DevDome dome_light(const DevScene& S, uint32_t i) {
+    return i < kInlineDomes ? S.domeIn[i] : uniform_load(S.domes + i);
+}
+DEV DevEnv env_map(const DevScene& S, int32_t e) {
+    const uint32_t i = static_cast<uint32_t>(e);
+    return i < kInlineDomes ? S.envIn[i] : uniform_load(S.envs + i);
+}
+
 DEV V3 world_to_dome(const DevDome& d, V3 v) {
     return normalize(V3{dot(v, ld3(d.right)), dot(v, ld3(d.up)), dot(v, ld3(d.forward))});
 }
@@ -54,9 +81,9 @@ DEV V3 environment_radiance(const DevScene& S, V3 dir, Counters<COUNT>& c) {
     if (S.domeCount > 0) {
         V3 sum{0.0f, 0.0f, 0.0f};
         for (uint32_t i = 0; i < S.domeCount; ++i) {
-            const DevDome& d = S.domes[i];
+            const DevDome d = dome_light(S, i);
             if (d.env >= 0) {
-                sum = sum + env_eval(S.envs[d.env], world_to_dome(d, dir), c.env()) * ld3(d.radiance);
+                sum = sum + env_eval(env_map(S, d.env), world_to_dome(d, dir), c.env()) * ld3(d.radiance);
             } else {
                 sum = sum + ld3(d.radiance);
             }
@@ -71,8 +98,8 @@ template <bool COUNT>
 DEV float env_pdf_sq_sum(const DevScene& S, V3 dir, Counters<COUNT>& c) {
     float acc = 0.0f;
     for (uint32_t i = 0; i < S.domeCount; ++i) {
-        const DevDome& d = S.domes[i];
-        const float p = d.env >= 0 ? env_pdf(S.envs[d.env], world_to_dome(d, dir), c.env()) : kInvFourPi;
+        const DevDome d = dome_light(S, i);
+        const float p = d.env >= 0 ? env_pdf(env_map(S, d.env), world_to_dome(d, dir), c.env()) : kInvFourPi;
         acc += p * p;
     }
     return acc;
@@ -86,7 +113,7 @@ DEV bool dome_sample(const DevScene& S, const DevDome& d, V2 u, V3* dir, V3* rad
     float lp;
     V3 lr;
     if (d.env >= 0) {
-        lr = env_sample(S.envs[d.env], u, &ld, &lp, c.env());
+        lr = env_sample(env_map(S, d.env), u, &ld, &lp, c.env());
     } else {
         ld = uniform_sphere(u);
         lp = kInvFourPi;
@@ -379,7 +406,7 @@ DEV bool direct_sample(const DevScene& S, const DevSettings& st, const Surface& 
                        DomeSampleInfo* domeOut = nullptr) {
     *maxDist = kFltMax;
     if (j < S.dirCount) {
-        const DevDirLight& dl = S.dirs[j];
+        const DevDirLight dl = dir_light(S, j);
         const V3 ld = normalize(ld3(dl.direction) * -1.0f);
         const float nl = clampf(dot(sf.n, ld), 0.0f, 1.0f);
         if (nl <= 0.0f) return false;
@@ -431,7 +458,7 @@ DEV bool direct_sample(const DevScene& S, const DevSettings& st, const Surface& 
     V3 ld, lr;
     float lp = 0.0f;
     const uint32_t branch = 256u + depth * 512u + li * 128u + k * 2u;
-    if (!dome_sample(S, S.domes[li], smp.next2(branch), &ld, &lr, &lp, c, domeOut ? &domeOut->local : nullptr) ||
+    if (!dome_sample(S, dome_light(S, li), smp.next2(branch), &ld, &lr, &lp, c, domeOut ? &domeOut->local : nullptr) ||
         lp <= 0.0f)
         return false;
     const float nl = clampf(dot(sf.n, ld), 0.0f, 1.0f);

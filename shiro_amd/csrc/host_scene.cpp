@@ -299,12 +299,20 @@ bool flatten_scene(const shiro_hip_scene& s, FlatScene* out, std::string* err) {
         *err = "meshes is null";
         return false;
     }
-    if (s.distantLightCount > static_cast<uint32_t>(kMaxDirLights)) {
-        *err = "too many distant lights (max " + std::to_string(kMaxDirLights) + ")";
+    if (s.distantLightCount > kMaxSceneDirLights) {
+        *err = "too many distant lights (max " + std::to_string(kMaxSceneDirLights) + ")";
         return false;
     }
-    if (s.domeLightCount > static_cast<uint32_t>(kMaxDomes)) {
-        *err = "too many dome lights (max " + std::to_string(kMaxDomes) + ")";
+    if (s.domeLightCount > kMaxSceneDomes) {
+        *err = "too many dome lights (max " + std::to_string(kMaxSceneDomes) + ")";
+        return false;
+    }
+    if (s.distantLightCount > 0 && !s.distantLights) {
+        *err = "distantLights is null";
+        return false;
+    }
+    if (s.domeLightCount > 0 && !s.domeLights) {
+        *err = "domeLights is null";
         return false;
     }
     size_t total = 0;

@@ -65,8 +65,17 @@ constexpr size_t kNode8Floats = 32;  // floats per 8-wide node (128 B)
 constexpr uint32_t kEnvRowGuide = 256;
 constexpr uint32_t kEnvColGuide = 64;
 
+// Light counts a scene may hold (a bound on the per-path light samples of
+// the wavefront's shadow records, not a storage size: the lights live in
+// device arrays).  The adjoint differentiates the maps of the first
+// kMaxDomes dome lights.
+constexpr uint32_t kMaxSceneDirLights = 1024;
+constexpr uint32_t kMaxSceneDomes = 256;
 constexpr int kMaxDomes = 4;
-constexpr int kMaxDirLights = 8;
+// The first lights also ride in the kernel arguments (DevScene), where the
+// kernels read them with scalar loads; the device arrays hold them all.
+constexpr uint32_t kInlineDirLights = 8;
+constexpr uint32_t kInlineDomes = 4;
 
 struct DevEnv {
     const float4* texels;   // w*h, row 0 = theta 0
@@ -166,9 +175,12 @@ struct DevScene {
     uint32_t rootIsLeaf;          // single-leaf scenes: root word in rootWord
     uint32_t rootWord;
     uint32_t rootCount;
-    DevEnv envs[kMaxDomes];
-    DevDome domes[kMaxDomes];
-    DevDirLight dirs[kMaxDirLights];
+    const DevEnv* envs;           // per dome with a map (DevDome::env), device array
+    const DevDome* domes;         // domeCount entries, device array
+    const DevDirLight* dirs;      // dirCount entries, device array
+    DevEnv envIn[kInlineDomes];   // copies of the first entries of those arrays
+    DevDome domeIn[kInlineDomes];
+    DevDirLight dirIn[kInlineDirLights];
     uint32_t domeCount;
     uint32_t dirCount;
     float envTop[3];

@@ -67,5 +67,20 @@ int main() {
                     b.depth, b.depth8, b.stack, b.stack8, static_cast<unsigned long long>(b.leafCount),
                     err.empty() ? "-" : err.c_str());
     }
+    // A depth bound the SAH tree exceeds: build_bvh builds again with
+    // binary nodes at the bound made leaves, and the result passes the
+    // checks at that bound (deeper leaves hold more triangles).
+    for (uint32_t bound : {3u, 5u}) {
+        const size_t n = 20000;
+        std::vector<float> tris = make_case(0, n, 99u + bound);
+        Bvh b;
+        std::string err;
+        bool ok = shiro_hip::build_bvh(tris.data(), n, bound, &b, &err);
+        if (ok) ok = shiro_hip::validate_bvh(b, tris.data(), n, bound, &err);
+        if (!ok || b.depth > bound) ++failures;
+        std::printf("capped bound=%u ok=%d depth=%u stack=%u leaves=%llu maxleaf=%u err=%s\n", bound, ok ? 1 : 0,
+                    b.depth, b.stack, static_cast<unsigned long long>(b.leafCount), b.maxLeafSize,
+                    err.empty() ? "-" : err.c_str());
+    }
     return failures == 0 ? 0 : 1;
 }

@@ -94,10 +94,15 @@ def test_supports_scene():
     assert L.shiro_hip_supports_scene(marshal(_tri_scene()).ptr) == 1
     # Emissive surfaces are sampled as triangle lights on the device.
     assert L.shiro_hip_supports_scene(marshal(_tri_scene(emissionColor=(1, 1, 1), emissionStrength=1.0)).ptr) == 1
-    # More directional or dome lights than the device holds fall back.
-    from shiro_amd.scene import DirectionalLight
+    # Lights live in device arrays: scenes with many directional and dome
+    # lights are supported (the reference has no limit); only counts beyond
+    # the per-path light-sample bound (1024 directional, 256 dome) fall back.
+    from shiro_amd.scene import DirectionalLight, DomeLight
     many = _tri_scene()
-    many.distantLights = [DirectionalLight() for _ in range(9)]
+    many.distantLights = [DirectionalLight() for _ in range(40)]
+    many.domeLights = [DomeLight() for _ in range(12)]
+    assert L.shiro_hip_supports_scene(marshal(many).ptr) == 1
+    many.distantLights = [DirectionalLight() for _ in range(1025)]
     assert L.shiro_hip_supports_scene(marshal(many).ptr) == 0
     assert L.shiro_hip_supports_scene(None) == 0
 

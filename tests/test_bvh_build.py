@@ -32,6 +32,18 @@ def report(tmp_path_factory):
     return out.returncode, rows, out.stdout
 
 
+def test_too_deep_tree_is_built_again_with_capped_depth(report):
+    """A depth bound the SAH tree exceeds does not fail the build: it is
+    built again with its binary nodes at the bound made leaves, and passes
+    every structural check at that bound."""
+    rc, _, text = report
+    capped = [dict(re.findall(r"(\w+)=(\S+)", l)) for l in text.splitlines() if l.startswith("capped ")]
+    assert len(capped) == 2, text
+    for r in capped:
+        assert r["ok"] == "1" and int(r["depth"]) <= int(r["bound"]), r
+        assert int(r["maxleaf"]) > 4, r
+
+
 def test_every_case_builds_checks_and_refits(report):
     rc, rows, text = report
     assert rows, text
