@@ -345,6 +345,64 @@ def summarize_profile(args):
                      indent=1))
 
 
+def counter_table(directory):
+    """Per kernel symbol: dispatches and the mean of every counter of a
+    rocprofv3 --pmc csv directory."""
+    acc = {}
+    for r in _csv_rows(directory, "counter_collection.csv"):
+        k = _kernel_symbol(r["Kernel_Name"])
+        d = acc.setdefault(k, {})
+        d.setdefault(r["Counter_Name"], []).append(float(r["Counter_Value"]))
+    return {k: {c: sum(v) / len(v) for c, v in d.items()} | {"_n": max(len(v) for v in d.values())}
+            for k, d in acc.items()}
+
+
+def summarize_valu(args):
+    """VALU issue rate, lane utilisation, memory waits and measured occupancy
+    per kernel, against the issue rate the FMA probe (bench.py --valu-probe)
+    sustains on the same counters: SQ_INSTS_VALU per second of the probe
+    kernel is the ceiling."""
+    sq_dir, occ_dir, stats_dir, probe_sq_dir, probe_stats_dir = args.summarize_valu
+    sq, occ = counter_table(sq_dir), counter_table(occ_dir)
+    stats = {}
+    for d in (stats_dir, probe_stats_dir):
+        for r in _csv_rows(d, "kernel_stats.csv"):
+            stats[_kernel_symbol(r["Name"])] = (int(r["Calls"]), float(r["AverageNs"]) / 1e6)
+    # The probe runs a short warm-up dispatch and the measured one: take the
+    # largest count and the longest duration (the measured dispatch).
+    pk, n_probe = None, 0.0
+    for r in _csv_rows(probe_sq_dir, "counter_collection.csv"):
+        if r["Counter_Name"] == "SQ_INSTS_VALU" and _kernel_symbol(r["Kernel_Name"]).startswith("k_valu_probe"):
+            if float(r["Counter_Value"]) > n_probe:
+                pk, n_probe = _kernel_symbol(r["Kernel_Name"]), float(r["Counter_Value"])
+    probe_ms = max(float(r["MaxNs"]) for r in _csv_rows(probe_stats_dir, "kernel_stats.csv")
+                   if _kernel_symbol(r["Name"]) == pk) / 1e6
+    ceiling = n_probe / (probe_ms * 1e-3)
+    print(f"VALU issue ceiling: {pk} issues {n_probe:.4g} VALU wave-instructions (SQ_INSTS_VALU) in "
+          f"{probe_ms:.3f} ms = {ceiling:.4g} per second\n")
+    print("| kernel | avg ms | VALU wave-instr per dispatch | VALU issue rate / ceiling | lane utilisation | "
+          "memory-wait share | MeanOccupancyPerActiveCU |")
+    print("|---|---|---|---|---|---|---|")
+    rows = []
+    for k, c in sq.items():
+        if k.startswith("at::") or k not in stats or "SQ_INSTS_VALU" not in c:
+            continue
+        ms = stats[k][1]
+        rate = c["SQ_INSTS_VALU"] / (ms * 1e-3) / ceiling
+        util = c.get("SQ_THREAD_CYCLES_VALU", 0.0) / max(64.0 * c.get("SQ_ACTIVE_INST_VALU", 1.0), 1.0)
+        wait = c.get("SQ_WAIT_INST_ANY", 0.0) / max(c.get("SQ_WAVE_CYCLES", 1.0), 1.0)
+        o = occ.get(k, {}).get("MeanOccupancyPerActiveCU")
+        rows.append((ms * stats[k][0], k, ms, c["SQ_INSTS_VALU"], rate, util, wait, o))
+    out = {}
+    for _, k, ms, n, rate, util, wait, o in sorted(rows, reverse=True):
+        print(f"| `{k}` | {ms:.4f} | {n:.4g} | {rate:.3f} | {util:.2f} | {wait:.2f} | "
+              f"{'-' if o is None else f'{o:.1f}'} |")
+        out[k] = {"valu_issue_frac": round(rate, 4), "lane_utilisation": round(util, 3),
+                  "memory_wait_share": round(wait, 3), "mean_occupancy": None if o is None else round(o, 2)}
+    print()
+    print(json.dumps({"valu_ceiling_per_s": ceiling, "kernels": out}, indent=1))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -373,11 +431,25 @@ def main():
                     metavar=("STATS", "FETCH", "WRITE", "COPY_FETCH", "COPY_WRITE"),
                     help="summarise rocprofv3 csv output directories into HBM bytes per dispatch (no GPU)")
     ap.add_argument("--copy-bytes", type=int, default=1 << 30, help="bytes the --copy-probe run copied")
+    ap.add_argument("--valu-probe", type=int, default=0, metavar="ITERS",
+                    help="only run the FMA issue-rate probe (ITERS iterations of 8 FMAs per thread)")
+    ap.add_argument("--summarize-valu", nargs=5, default=None,
+                    metavar=("SQ", "OCCUPANCY", "STATS", "PROBE_SQ", "PROBE_STATS"),
+                    help="summarise rocprofv3 SQ counter directories into VALU issue rates (no GPU)")
     args = ap.parse_args()
     if args.steps is None:
         args.steps = 100 if args.workload == "c5" else 5
     if args.summarize_profile:
         summarize_profile(args)
+        return
+    if args.summarize_valu:
+        summarize_valu(args)
+        return
+    if args.valu_probe:
+        from shiro_amd.hip import Context
+        ctx = Context(0)
+        ms, n = ctx.valu_probe(args.valu_probe)
+        print(json.dumps({"valu_probe_ms": ms, "wave_instructions": n, "per_second": n / (ms * 1e-3)}), flush=True)
         return
     if args.copy_probe:
         import torch

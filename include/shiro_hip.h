@@ -621,6 +621,13 @@ typedef struct shiro_hip_stage_times {
  * launch on the context's stream; shiro_hip_get_stage_times returns the
  * per-stage sums of the last synchronised render call. */
 int shiro_hip_set_stage_timing(shiro_hip_ctx* ctx, uint32_t enable);
+/* Calibration of the VALU issue rate (measurement; no counterpart in the
+ * reference): a kernel of independent FP32 FMAs at full occupancy, 32 waves
+ * per compute unit for `iterations` iterations of 8 FMAs.  Returns its
+ * HIP-event time and the VALU wave-instructions it issued (FMAs only), whose
+ * ratio is the sustained issue rate the render kernels' VALU counts are
+ * compared with. */
+int shiro_hip_valu_probe(shiro_hip_ctx* ctx, uint32_t iterations, float* ms, uint64_t* waveInstructions);
 int shiro_hip_get_stage_times(shiro_hip_ctx* ctx, shiro_hip_stage_times* out);
 
 /* ---- Mesh ingestion of Shiro's Hydra scene bridge (host code, no device):

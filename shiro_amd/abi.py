@@ -191,7 +191,7 @@ EXPORTS = [
     "shiro_hip_occluded", "shiro_hip_trace_primary", "shiro_hip_eval_bsdf",
     "shiro_hip_env_sample", "shiro_hip_env_eval", "shiro_hip_count_fetches",
     "shiro_hip_set_tuning", "shiro_hip_get_tuning", "shiro_hip_autotune",
-    "shiro_hip_set_stage_timing", "shiro_hip_get_stage_times",
+    "shiro_hip_set_stage_timing", "shiro_hip_get_stage_times", "shiro_hip_valu_probe",
     "shiro_hip_update_texture", "shiro_hip_render_adjoint",
     "shiro_hip_render_many", "shiro_hip_autotune_many", "shiro_hip_many_graph_stats",
     "shiro_hip_load_image", "shiro_hip_load_image_memory", "shiro_hip_free_image",

@@ -1431,6 +1431,23 @@ int shiro_hip_count_fetches(shiro_hip_ctx* ctx, const shiro_hip_camera* camera, 
     return SHIRO_HIP_OK;
 }
 
+int shiro_hip_valu_probe(shiro_hip_ctx* ctx, uint32_t iterations, float* ms, uint64_t* waveInstructions) {
+    if (!ctx || !ms || !waveInstructions || iterations == 0) return SHIRO_HIP_INVALID;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    HIPC(hipSetDevice(ctx->device), "set device");
+    const uint32_t blocks = ctx->computeUnits * 8u;  // 8 blocks of 4 waves: 32 waves per CU
+    Tmp out;
+    HIPC(out.alloc<float>(static_cast<size_t>(blocks) * 256u), "alloc probe");
+    HIPC(launch_valu_probe(out.b.as<float>(), blocks, 16u, ctx->stream), "probe");  // warm-up
+    HIPC(hipEventRecord(ctx->ev0, ctx->stream), "event");
+    HIPC(launch_valu_probe(out.b.as<float>(), blocks, iterations, ctx->stream), "probe");
+    HIPC(hipEventRecord(ctx->ev1, ctx->stream), "event");
+    HIPC(hipEventSynchronize(ctx->ev1), "probe");
+    HIPC(hipEventElapsedTime(ms, ctx->ev0, ctx->ev1), "probe");
+    *waveInstructions = static_cast<uint64_t>(blocks) * 4u * iterations * 8u;
+    return SHIRO_HIP_OK;
+}
+
 int shiro_hip_set_stage_timing(shiro_hip_ctx* ctx, uint32_t enable) {
     if (!ctx) return SHIRO_HIP_INVALID;
     std::lock_guard<std::mutex> lock(ctx->mu);

@@ -114,6 +114,9 @@ struct AdjointParams {
 };
 
 hipError_t launch_render(const RenderParams& P, bool count, hipStream_t stream);
+// VALU issue-rate probe (measurement): `blocks` blocks of four waves, each
+// thread issuing 8 * iters FMAs; out holds blocks * 256 floats.
+hipError_t launch_valu_probe(float* out, uint32_t blocks, uint32_t iters, hipStream_t s);
 // The megakernel with path regeneration: `blocks` persistent one-wave
 // blocks; counter: one device word (cleared by the launch).
 hipError_t launch_render_plus(const RenderParams& P, uint32_t* counter, uint32_t blocks, hipStream_t stream);

@@ -276,6 +276,29 @@ __global__ __launch_bounds__(64) void k_render_many(const RenderParams* __restri
     }
 }
 
+// VALU issue-rate probe: eight independent FMA chains per thread, so that
+// a wave always has an FMA ready and the SIMDs issue VALU instructions
+// back to back.  kValuProbeChains FMAs per iteration.
+constexpr uint32_t kValuProbeChains = 8;
+__global__ __launch_bounds__(256) void k_valu_probe(float* out, uint32_t iters, float seed) {
+    float a[kValuProbeChains];
+#pragma unroll
+    for (uint32_t k = 0; k < kValuProbeChains; ++k) a[k] = seed + static_cast<float>(threadIdx.x + k);
+    for (uint32_t i = 0; i < iters; ++i) {
+#pragma unroll
+        for (uint32_t k = 0; k < kValuProbeChains; ++k) a[k] = __builtin_fmaf(a[k], 0.999999f, 1.0e-7f);
+    }
+    float s = 0.0f;
+#pragma unroll
+    for (uint32_t k = 0; k < kValuProbeChains; ++k) s += a[k];
+    out[blockIdx.x * blockDim.x + threadIdx.x] = s;
+}
+
+hipError_t launch_valu_probe(float* out, uint32_t blocks, uint32_t iters, hipStream_t s) {
+    hipLaunchKernelGGL(k_valu_probe, dim3(blocks), dim3(256), 0, s, out, iters, 1.0f);
+    return hipGetLastError();
+}
+
 __global__ void k_clear_u32(uint32_t* p, uint32_t n) {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0u;
 }

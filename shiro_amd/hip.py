@@ -94,6 +94,7 @@ def load_library(path: str = LIB_PATH):
       C.c_size_t)
     d("shiro_hip_set_stage_timing", C.c_int, V, _u)
     d("shiro_hip_get_stage_times", C.c_int, V, _P(abi.StageTimes))
+    d("shiro_hip_valu_probe", C.c_int, V, _u, _P(C.c_float), _P(C.c_uint64))
     d("shiro_hip_update_texture", C.c_int, V, _u, _fp, _u)
     d("shiro_hip_render_adjoint", C.c_int, V, _P(abi.Camera), _P(abi.Settings), _u, _u, _fp, _P(_fp), _u)
     d("shiro_hip_render_adjoint_env", C.c_int, V, _P(abi.Camera), _P(abi.Settings), _u, _u, _fp, _P(_fp), _u,
@@ -540,6 +541,12 @@ class Context:
         self._check(self.L.shiro_hip_render_adjoint_env(self.h, C.byref(cam), C.byref(st), sample_start, sample_count,
                                                         C.cast(C.c_void_p(dbeauty_ptr), _fp), arr, len(grad_ptrs),
                                                         env, len(env_grad_ptrs)))
+
+    def valu_probe(self, iterations=20000):
+        """(milliseconds, VALU wave-instructions) of the FMA issue-rate probe."""
+        ms, n = C.c_float(), C.c_uint64()
+        self._check(self.L.shiro_hip_valu_probe(self.h, iterations, C.byref(ms), C.byref(n)))
+        return ms.value, n.value
 
     def set_stage_timing(self, enable=True):
         self._check(self.L.shiro_hip_set_stage_timing(self.h, 1 if enable else 0))
