@@ -161,16 +161,19 @@ DEV V3 apply_thin_film(V3 lobe, const shiro_hip_material& m, V3 n, V3 v, V3 l) {
     return lobe * tint;
 }
 
-// Reflective mixture pdf (CpuPathTracer.cpp:1232-1273).
-DEV_CALL float bsdf_pdf(Mat M, V3 n, V3 cn, V3 v, V3 l) {
+// Reflective mixture pdf (CpuPathTracer.cpp:1232-1273).  The out-of-line
+// BSDF functions take the material as separate pointers, which travel in
+// registers (a Mat passed by value goes through the stack).
+DEV_CALL float bsdf_pdf_at(const shiro_hip_material* mp, const MaterialDerived* dsp, V3 n, V3 cn, V3 v, V3 l) {
     const float nl = clampf(dot(n, l), 0.0f, 1.0f);
     if (nl <= 0.0f) return 0.0f;
-    const shiro_hip_material& m = *M.m;
-    const float wd = M.ds->wDiffuse, ws = M.ds->wBaseSpecular, wc = M.ds->wCoat;
+    const shiro_hip_material& m = *mp;
+    const MaterialDerived& ds = *dsp;
+    const float wd = ds.wDiffuse, ws = ds.wBaseSpecular, wc = ds.wCoat;
     const float wr = wd + ws + wc;
     if (wr <= 0.0f) return 0.0f;
     const float pd = wd / wr, ps = ws / wr, pc = wc / wr;
-    const float rs = roughness_eval(M.ds->baseRoughness, m.specularAnisotropy, m.specularRotation, m, n, v, l);
+    const float rs = roughness_eval(ds.baseRoughness, m.specularAnisotropy, m.specularRotation, m, n, v, l);
     float coatTerm = 0.0f;
     if (pc != 0.0f) {
         const float rc = roughness_eval(m.coatRoughness, m.coatAnisotropy, m.coatRotation, m, cn, v, l);
@@ -180,12 +183,12 @@ DEV_CALL float bsdf_pdf(Mat M, V3 n, V3 cn, V3 v, V3 l) {
 }
 
 // Layered BSDF value (CpuPathTracer.cpp:1275-1354).
-DEV_CALL V3 bsdf_eval(Mat M, V3 n, V3 cn, V3 v, V3 l) {
+DEV_CALL V3 bsdf_eval_at(const shiro_hip_material* mp, const MaterialDerived* dp, V3 n, V3 cn, V3 v, V3 l) {
     const float nv = clampf(dot(n, v), 0.0f, 1.0f);
     const float nl = clampf(dot(n, l), 0.0f, 1.0f);
     if (nv <= 0.0f || nl <= 0.0f) return V3{0.0f, 0.0f, 0.0f};
-    const shiro_hip_material& m = *M.m;
-    const MaterialDerived& d = *M.d;
+    const shiro_hip_material& m = *mp;
+    const MaterialDerived& d = *dp;
     const V3 f0b = ld3(d.specularF0);
     const V3 f0c = ld3(d.coatF0);
     const V3 underCoat = ld3(d.coatTint) * layer_transmittance(f0c, dot(cn, v), dot(cn, l));
@@ -222,6 +225,9 @@ DEV_CALL V3 bsdf_eval(Mat M, V3 n, V3 cn, V3 v, V3 l) {
     }
     return out;
 }
+
+DEV float bsdf_pdf(const Mat& M, V3 n, V3 cn, V3 v, V3 l) { return bsdf_pdf_at(M.m, M.ds, n, cn, v, l); }
+DEV V3 bsdf_eval(const Mat& M, V3 n, V3 cn, V3 v, V3 l) { return bsdf_eval_at(M.m, M.d, n, cn, v, l); }
 
 // Derivatives of bsdf_eval at fixed directions with respect to the base
 // colour (per channel: d f_c / d baseColor_c) and the roughness
@@ -323,7 +329,9 @@ DEV BsdfGrad bsdf_grad(const Mat& M, V3 n, V3 cn, V3 v, V3 l) {
 }
 
 // Rough dielectric transmission (CpuPathTracer.cpp:1136-1182).
-DEV_CALL V3 transmission_lobe(Mat M, V3 n, V3 cn, V3 v, V3 l, float etaI, float etaT, float r) {
+DEV_CALL V3 transmission_lobe_at(const shiro_hip_material* mp, const MaterialDerived* dp, V3 n, V3 cn, V3 v, V3 l,
+                                 float etaI, float etaT, float r) {
+    const Mat M{mp, dp, dp};
     const float cwo = dot(n, v);
     const float cwi = dot(n, l);
     const float acwo = fabsf(cwo), acwi = fabsf(cwi);
@@ -346,6 +354,10 @@ DEV_CALL V3 transmission_lobe(Mat M, V3 n, V3 cn, V3 v, V3 l, float etaI, float 
     const float factor = fabsf(D * G * fabsf(ih) * fabsf(oh) / fmaxs(acwo * acwi * sq(sden), 1.0e-6f));
     const float transport = 1.0f / fmaxs(eta * eta, 1.0e-6f);
     return scale * F * (factor * transport);
+}
+
+DEV V3 transmission_lobe(const Mat& M, V3 n, V3 cn, V3 v, V3 l, float etaI, float etaT, float r) {
+    return transmission_lobe_at(M.m, M.d, n, cn, v, l, etaI, etaT, r);
 }
 
 DEV float transmission_pdf(V3 n, V3 v, V3 l, float eta, float r) {
