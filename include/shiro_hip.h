@@ -429,6 +429,18 @@ int shiro_hip_render_adjoint_env(
     const shiro_hip_settings* settings, uint32_t sampleStart,
     uint32_t sampleCount, const float* dBeauty, float* const* dTexels,
     uint32_t dTexelsCount, float* const* dEnv, uint32_t dEnvCount);
+/* Replay log for the adjoint (no counterpart in the reference): with
+ * maxBytes > 0, every wavefront render batch whose log fits in maxBytes
+ * (20 bytes per pixel, sample and depth of maxDepth) and whose vertices
+ * take at most 32 light samples records the closest hit of each path vertex
+ * and which of its light samples were visible.  An adjoint of the same
+ * samples, camera and settings (the transport aside) with the scene content
+ * unchanged since (no upload, refit, texture or map update in between)
+ * reads them instead of tracing the rays again; its gradients are those of
+ * the traced replay.  0 turns the log off and frees it. */
+int shiro_hip_set_replay_log(shiro_hip_ctx* ctx, uint64_t maxBytes);
+/* *replayed = 1 if the last adjoint call read the replay log, else 0. */
+int shiro_hip_last_adjoint_replayed(const shiro_hip_ctx* ctx, uint32_t* replayed);
 /* Replaces the texels of dome light i's map (W*H*3 floats in the layout of
  * its pixels; host memory, or device memory with SHIRO_HIP_TEXELS_DEVICE)
  * without re-uploading the scene, the EnvironmentMap of a DomeLight changing

@@ -197,6 +197,7 @@ EXPORTS = [
     "shiro_hip_load_image", "shiro_hip_load_image_memory", "shiro_hip_free_image",
     "shiro_hip_refit_scene", "shiro_hip_triangulate", "shiro_hip_smooth_normals",
     "shiro_hip_render_adjoint_env", "shiro_hip_update_environment",
+    "shiro_hip_set_replay_log", "shiro_hip_last_adjoint_replayed",
 ]
 
 

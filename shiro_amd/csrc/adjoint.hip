@@ -11,8 +11,10 @@
 // frozen-sampling render, whose finite differences it reproduces.
 //
 // One thread replays one (pixel, sample) path with the forward code
-// (intersect_closest, shade_hit<true>, direct_sample, occluded,
-// sample_bounce: the same decisions as the textured render), records each
+// (closest hit, shade_hit<true>, direct_sample, shadow rays, sample_bounce:
+// the same decisions as the textured render; when the forward wavefront
+// render of the same samples left a replay log, its hits and light-sample
+// visibilities are read instead of tracing the rays again), records each
 // vertex (throughput, attenuation, C_j, bounce factors, d f_j / d theta at
 // the sampled direction), adds the light-sample gradients on the way, then
 // walks the vertices backwards propagating dLoss/dT_j through the clamp and
@@ -26,6 +28,7 @@
 #include "kernels.h"
 
 #include "dev_path.h"
+#include "dev_ring.h"
 
 namespace shiro_hip {
 
@@ -215,11 +218,30 @@ DEV void adjoint_path(const AdjointParams& P, GradCache& cache, uint32_t x, uint
     PathState ps = init_path(P.camera, st, x, y, smp);
     const uint32_t nLight = direct_sample_count(S, st);
     Counters<false> cnt;
+    // Traversal stacks: the newest entries in the block's LDS ring
+    // (dev_ring.h), older ones here; same traversal and results as
+    // intersect_closest / intersect_any.
+    const uint32_t lane = threadIdx.x;
+    uint32_t ovNode[kStackSize];
+    float ovT[kStackSize];
     Vtx tape[MAXD];
     uint32_t nv = 0;
     for (uint32_t depth = 0; depth < st.maxDepth && depth < MAXD; ++depth) {
         const Ray ray = path_ray(ps);
-        const HitRec h = intersect_closest(S, ray, cnt.trav());
+        const size_t li = replay_index(st.width, st.height, P.sampleCount, x, y, s, depth);
+        HitRec h;
+        if (P.hitLog) {
+            // The forward pass's hit of this vertex (its record as the
+            // wavefront stores it; the triangle index follows from the slot).
+            const float4 hh = P.hitLog[li];
+            h.t = hh.x;
+            h.b1 = hh.y;
+            h.b2 = hh.z;
+            h.slot = __float_as_uint(hh.w);
+            h.prim = kNoHit;
+        } else {
+            h = intersect_closest_ring(S, ray, lane, ovNode, ovT);
+        }
         Vtx& V = tape[nv++];
         V.flags = 0u;
         if (h.slot == kNoHit) {
@@ -258,6 +280,8 @@ DEV void adjoint_path(const AdjointParams& P, GradCache& cache, uint32_t x, uint
         V3 gBase{0.0f, 0.0f, 0.0f};
         float gRough = 0.0f;
         const V3 aT = a * ps.T;
+        const uint32_t visBits = P.visLog ? P.visLog[li] : 0u;
+        uint32_t record = 0;
         for (uint32_t j = 0; j < nLight; ++j) {
             V3 ld, contrib, f, lf;
             uint32_t group;
@@ -265,7 +289,20 @@ DEV void adjoint_path(const AdjointParams& P, GradCache& cache, uint32_t x, uint
             DomeSampleInfo dome;
             if (!direct_sample(S, st, sf, M, smp, depth, j, &ld, &contrib, &group, &maxDist, cnt, &f, &lf, &dome))
                 continue;
-            const bool visible = !occluded(S, origin, ld, maxDist, cnt);
+            bool visible;
+            if (P.visLog) {
+                // Light samples that need a shadow ray are numbered in order,
+                // as the forward pass's shadow records are.
+                visible = ((visBits >> (record & 31u)) & 1u) != 0u;
+            } else {
+                Ray sr;
+                sr.o = origin;
+                sr.d = ld;
+                sr.tmin = kRayEps;
+                sr.tmax = maxDist;
+                visible = !intersect_any_ring(S, sr, lane, ovNode);
+            }
+            ++record;
             ds.add(group, contrib, visible);
             if (visible && (textured || (P.envWanted && group > 0u))) {
                 const V3 gc = group == 0u ? aT : aT / ds.K;

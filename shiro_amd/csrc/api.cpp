@@ -147,6 +147,20 @@ struct shiro_hip_ctx {
     // if the lookup failed (scratchError says why).
     void* scratch = nullptr;
     std::string scratchError;
+    // Scene content version: changes with every upload, refit, texture or
+    // map update (the replay log is valid only for the version it saw).
+    uint64_t contentVersion = 0;
+    // Replay log (shiro_hip_set_replay_log): hits and light-sample
+    // visibility of the last wavefront batch that fitted `replayCap` bytes,
+    // and what that batch was.
+    uint64_t replayCap = 0;
+    DevBuf replayHits, replayVis;
+    bool replayValid = false;
+    uint64_t replayVersion = 0;
+    shiro_hip_camera replayCamera{};
+    shiro_hip_settings replaySettings{};
+    uint32_t replayStart = 0, replayCount = 0;
+    bool lastAdjointReplayed = false;  // the last adjoint read the replay log
 };
 
 namespace {
@@ -358,7 +372,8 @@ int wavefront_plan(shiro_hip_ctx* ctx, const DevScene& scene, const shiro_hip_se
 int enqueue_wavefront(shiro_hip_ctx* ctx, const DevScene& scene, const shiro_hip_camera& cam,
                       const shiro_hip_settings& st, uint32_t start, uint32_t count, const DevFrame& frame,
                       const volatile uint8_t* cancel, bool events = true, void* scratch = nullptr,
-                      hipStream_t stream = nullptr, WfState* state = nullptr, uint64_t budget = 0) {
+                      hipStream_t stream = nullptr, WfState* state = nullptr, uint64_t budget = 0,
+                      float4* hitLog = nullptr, uint32_t* visLog = nullptr) {
     if (!stream) stream = ctx->stream;
     if (!state) state = &ctx->wfs;
     if (budget == 0) budget = scratch ? kScratchBytes : kWavefrontStateBytes;
@@ -399,6 +414,8 @@ int enqueue_wavefront(shiro_hip_ctx* ctx, const DevScene& scene, const shiro_hip
     P.sampleStart = start;
     P.batchCount = count;
     P.tilesX = plan.tilesX;
+    P.hitLog = hitLog;
+    P.visLog = visLog;
     WfLaunch L{};
     L.maxBlocks = ctx->tuning.stageBlocks;
     L.traceBlocks = ctx->tuning.traceBlocks;
@@ -438,8 +455,31 @@ int enqueue_wavefront(shiro_hip_ctx* ctx, const DevScene& scene, const shiro_hip
 // pointers.  If cancel is non-null the host flag is polled between slices.
 int enqueue_batch(shiro_hip_ctx* ctx, const shiro_hip_camera& cam, const shiro_hip_settings& st, uint32_t start,
                   uint32_t count, const DevFrame& frame, bool countFetches, const volatile uint8_t* cancel) {
-    if (st.transport == SHIRO_HIP_TRANSPORT_WAVEFRONT && !countFetches)
-        return enqueue_wavefront(ctx, ctx->dev, cam, st, start, count, frame, cancel);
+    if (st.transport == SHIRO_HIP_TRANSPORT_WAVEFRONT && !countFetches) {
+        // The replay log: one hit record (16 bytes) and one visibility word
+        // per vertex, for batches within the cap whose vertices take at
+        // most 32 light samples.
+        const uint64_t entries = static_cast<uint64_t>(st.width) * st.height * count * st.maxDepth;
+        const DevScene& S = ctx->dev;
+        const uint64_t lights = static_cast<uint64_t>(S.dirCount) +
+                                static_cast<uint64_t>(S.domeCount) * std::max(1u, st.domeLightSamples) +
+                                (S.emissiveCount > 0u && S.emissiveSum > 0.0f ? 1u : 0u);
+        const bool record = ctx->replayCap > 0 && entries > 0 && entries * 20u <= ctx->replayCap && lights <= 32u;
+        ctx->replayValid = false;
+        if (!record) return enqueue_wavefront(ctx, ctx->dev, cam, st, start, count, frame, cancel);
+        HIPC(ctx->replayHits.reserve(entries * 16u), "alloc replay log");
+        HIPC(ctx->replayVis.reserve(entries * 4u), "alloc replay log");
+        const int rc = enqueue_wavefront(ctx, ctx->dev, cam, st, start, count, frame, cancel, true, nullptr, nullptr,
+                                         nullptr, 0, ctx->replayHits.as<float4>(), ctx->replayVis.as<uint32_t>());
+        if (rc) return rc;
+        ctx->replayValid = true;
+        ctx->replayVersion = ctx->contentVersion;
+        ctx->replayCamera = cam;
+        ctx->replaySettings = st;
+        ctx->replayStart = start;
+        ctx->replayCount = count;
+        return SHIRO_HIP_OK;
+    }
     const uint64_t np = static_cast<uint64_t>(st.width) * st.height;
     const uint32_t slice = slice_samples(np, count);
     RenderParams P{};
@@ -659,6 +699,7 @@ int upload_scene(shiro_hip_ctx* ctx, const shiro_hip_scene* s, bool refit) {
     if (!ctx) return SHIRO_HIP_INVALID;
     if (!s) return fail(ctx, SHIRO_HIP_INVALID, "scene is null");
     std::lock_guard<std::mutex> lock(ctx->mu);
+    ++ctx->contentVersion;
     HIPC(hipSetDevice(ctx->device), "set device");
     const uint64_t topo = topology_hash(*s);
     if (refit && (!ctx->hasScene || topo != ctx->topology))
@@ -900,6 +941,7 @@ int shiro_hip_update_texture(shiro_hip_ctx* ctx, uint32_t index, const float* te
     std::lock_guard<std::mutex> lock(ctx->mu);
     if (!ctx->hasScene) return fail(ctx, SHIRO_HIP_NO_SCENE, "no scene uploaded");
     if (index >= ctx->tex.size() || !texels) return fail(ctx, SHIRO_HIP_INVALID, "bad texture index or null texels");
+    ++ctx->contentVersion;
     HIPC(hipSetDevice(ctx->device), "set device");
     TexBuffers& b = *ctx->tex[index];
     const size_t bytes = b.count() * sizeof(float);
@@ -1480,6 +1522,7 @@ int shiro_hip_update_environment(shiro_hip_ctx* ctx, uint32_t dome, const float*
     if (!ctx->hasScene) return fail(ctx, SHIRO_HIP_NO_SCENE, "no scene uploaded");
     if (dome >= ctx->dev.domeCount || ctx->hostDomes[dome].env < 0 || !texels)
         return fail(ctx, SHIRO_HIP_INVALID, "bad dome index, dome without a map, or null texels");
+    ++ctx->contentVersion;
     HIPC(hipSetDevice(ctx->device), "set device");
     EnvBuffers& E = ctx->env[ctx->hostDomes[dome].env];
     const size_t n = static_cast<size_t>(E.w) * E.h;
@@ -1568,6 +1611,18 @@ int render_adjoint(shiro_hip_ctx* ctx, const shiro_hip_camera* camera, const shi
     // Blocks of 16 samples of a 2x2 pixel square (4 of a 4x4 square, or
     // one sample of an 8x8 tile, when the sample count is not a multiple).
     P.groupLog = sampleCount % 16u == 0u ? 4u : (sampleCount % 4u == 0u ? 2u : 0u);
+    // Replay the log of the forward pass if it rendered these samples of
+    // this frame from the scene content as it is now.
+    if (ctx->replayValid && ctx->replayVersion == ctx->contentVersion && ctx->replayStart == sampleStart &&
+        ctx->replayCount == sampleCount && std::memcmp(&ctx->replayCamera, camera, sizeof(*camera)) == 0) {
+        shiro_hip_settings a = ctx->replaySettings, b = *settings;
+        a.transport = b.transport = 0u;
+        if (std::memcmp(&a, &b, sizeof(a)) == 0) {
+            P.hitLog = ctx->replayHits.as<float4>();
+            P.visLog = ctx->replayVis.as<uint32_t>();
+        }
+    }
+    ctx->lastAdjointReplayed = P.hitLog != nullptr;
     HIPC(hipEventRecord(ctx->ev0, ctx->stream), "event");
     HIPC(launch_adjoint(P, ctx->stream), "adjoint launch");
     HIPC(hipEventRecord(ctx->ev1, ctx->stream), "event");
@@ -1586,6 +1641,24 @@ int shiro_hip_render_adjoint_env(shiro_hip_ctx* ctx, const shiro_hip_camera* cam
                                  float* const* dEnv, uint32_t dEnvCount) {
     return render_adjoint(ctx, camera, settings, sampleStart, sampleCount, dBeauty, dTexels, dTexelsCount, dEnv,
                           dEnvCount);
+}
+
+int shiro_hip_set_replay_log(shiro_hip_ctx* ctx, uint64_t maxBytes) {
+    if (!ctx) return SHIRO_HIP_INVALID;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    ctx->replayCap = maxBytes;
+    ctx->replayValid = false;
+    if (maxBytes == 0) {
+        ctx->replayHits.release();
+        ctx->replayVis.release();
+    }
+    return SHIRO_HIP_OK;
+}
+
+int shiro_hip_last_adjoint_replayed(const shiro_hip_ctx* ctx, uint32_t* replayed) {
+    if (!ctx || !replayed) return SHIRO_HIP_INVALID;
+    *replayed = ctx->lastAdjointReplayed ? 1u : 0u;
+    return SHIRO_HIP_OK;
 }
 
 namespace {

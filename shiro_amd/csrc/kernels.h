@@ -62,7 +62,19 @@ struct WfParams {
     uint32_t tiles;     // tiles of this pass: frame tiles [tileBase, tileBase + tiles) in row-major order
     uint32_t tileBase;
     uint32_t groupLog;  // path layout: groups of 2^groupLog samples (path_coord, wavefront.hip)
+    // Replay log (null: off): the closest hit of every path vertex and the
+    // visibility bits of its light samples, at replay_index, for the adjoint
+    // to replay the batch without tracing.
+    float4* hitLog;
+    uint32_t* visLog;
 };
+
+// Entry of the replay log for vertex `depth` of sample s (of the batch) of
+// pixel (x, y): depth-major planes of width * height * samples entries.
+__host__ __device__ inline size_t replay_index(uint32_t width, uint32_t height, uint32_t samples, uint32_t x,
+                                               uint32_t y, uint32_t s, uint32_t depth) {
+    return ((static_cast<size_t>(depth) * samples + s) * height + y) * width + x;
+}
 
 // Device memory of the entry points that allocate nothing (wavefront.hip):
 // the size of the library's scratch array, and the address of `device`'s
@@ -111,6 +123,10 @@ struct AdjointParams {
     uint32_t sampleStart, sampleCount;
     uint32_t tilesX, tiles;
     uint32_t groupLog;  // blocks of 2^groupLog samples of a pixel square, as the wavefront path groups
+    // Replay log of the forward render of these samples (null: trace the
+    // paths again): hits and light-sample visibility at replay_index.
+    const float4* hitLog;
+    const uint32_t* visLog;
 };
 
 hipError_t launch_render(const RenderParams& P, bool count, hipStream_t stream);

@@ -594,6 +594,9 @@ __global__ __launch_bounds__(kShadeThreads) __attribute__((amdgpu_waves_per_eu(3
             const Sampler smp = path_sampler(P, pc);
             PathState ps = depth == 0u ? init_path(P.camera, P.settings, pc.x, pc.y, smp) : load_state(P.B, p);
             const float4 hh = P.B.H[p];
+            if (P.hitLog)
+                P.hitLog[replay_index(P.settings.width, P.settings.height, P.batchCount, pc.x, pc.y,
+                                      P.sliceBegin + pc.s, depth)] = hh;
             HitRec h;
             h.t = hh.x;
             h.b1 = hh.y;
@@ -650,7 +653,8 @@ __global__ __launch_bounds__(kShadeThreads) __attribute__((amdgpu_waves_per_eu(3
 }
 
 // Adds the visible light samples of each shaded path, in DirectLighting's
-// order, times the throughput at the hit (CpuPathTracer.cpp:1750-1762).
+// order, times the throughput at the hit (CpuPathTracer.cpp:1750-1762);
+// with the replay log on, records which of them were visible.
 __global__ __launch_bounds__(64) void k_wf_resolve(WfParams P, uint32_t depth) {
     const uint32_t n = P.B.counts[P.B.shadowPathBase + depth];
     for (uint32_t i = blockIdx.x * 64u + threadIdx.x; i < n; i += gridDim.x * 64u) {
@@ -660,11 +664,20 @@ __global__ __launch_bounds__(64) void k_wf_resolve(WfParams P, uint32_t depth) {
         const uint32_t* vis = P.B.svis + static_cast<size_t>(p) * P.B.maxShadow;
         DirectSum ds;
         ds.init(P.settings);
+        uint32_t bits = 0;
         for (uint32_t j = 0; j < ns; ++j) {
             const float4 a = sr[2u * j], b = sr[2u * j + 1u];
-            ds.add(__float_as_uint(a.w), xyz(b), vis[j] != 0u);
+            const bool visible = vis[j] != 0u;
+            ds.add(__float_as_uint(a.w), xyz(b), visible);
+            bits |= (visible ? 1u : 0u) << (j & 31u);
         }
         ds.flush();
+        if (P.visLog) {
+            // At most 32 light samples per vertex when the log is on.
+            const PathCoord pc = path_coord(P, p);
+            P.visLog[replay_index(P.settings.width, P.settings.height, P.batchCount, pc.x, pc.y, P.sliceBegin + pc.s,
+                                  depth)] = bits;
+        }
         const float4 L = P.B.L[p];
         const V3 Lv = xyz(L) + xyz(P.B.st[p]) * ds.L;
         P.B.L[p] = make_float4(Lv.x, Lv.y, Lv.z, L.w);

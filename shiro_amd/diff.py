@@ -75,12 +75,18 @@ class _RenderTextures(torch.autograd.Function):
 class TextureRenderer:
     """Renders `scene` with texture `indices` (default: all) and the maps of
     the dome lights `env_domes` (default: none) taken from tensors.  One
-    context per renderer; calls are synchronous."""
+    context per renderer; calls are synchronous.  Wavefront forward passes
+    needing at most `replay_log_bytes` of replay log (20 bytes per pixel,
+    sample and depth; 0: never) keep their hits and shadow-ray visibilities
+    for the backward pass."""
 
     def __init__(self, scene: Scene, camera: Camera, settings: RenderSettings, indices=None, device: int = 0,
-                 env_domes=()):
+                 env_domes=(), replay_log_bytes: int = 2 << 30):
         self.ctx = Context(device)
         self.ctx.upload(scene)
+        # The backward pass replays the forward pass's hits and shadow-ray
+        # visibilities when nothing changed in between (wavefront forward).
+        self.ctx.set_replay_log(replay_log_bytes)
         self.camera = camera
         self.settings = settings
         self.indices = list(range(len(scene.textures))) if indices is None else list(indices)

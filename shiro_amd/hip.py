@@ -100,6 +100,8 @@ def load_library(path: str = LIB_PATH):
     d("shiro_hip_render_adjoint_env", C.c_int, V, _P(abi.Camera), _P(abi.Settings), _u, _u, _fp, _P(_fp), _u,
       _P(_fp), _u)
     d("shiro_hip_update_environment", C.c_int, V, _u, _fp, _u)
+    d("shiro_hip_set_replay_log", C.c_int, V, C.c_uint64)
+    d("shiro_hip_last_adjoint_replayed", C.c_int, V, _P(C.c_uint32))
     d("shiro_hip_render_many", C.c_int, V, _P(abi.BatchItem), _u, _u)
     d("shiro_hip_autotune_many", C.c_int, V, _P(abi.BatchItem), _u, _P(abi.Tuning), C.c_char_p, C.c_size_t)
     d("shiro_hip_many_graph_stats", C.c_int, V, _P(C.c_uint32), _P(C.c_uint32))
@@ -541,6 +543,17 @@ class Context:
         self._check(self.L.shiro_hip_render_adjoint_env(self.h, C.byref(cam), C.byref(st), sample_start, sample_count,
                                                         C.cast(C.c_void_p(dbeauty_ptr), _fp), arr, len(grad_ptrs),
                                                         env, len(env_grad_ptrs)))
+
+    def set_replay_log(self, max_bytes):
+        """Keeps the hits and light-sample visibility of wavefront batches
+        needing at most max_bytes (0: off) for the adjoint of the same
+        samples to read instead of tracing again."""
+        self._check(self.L.shiro_hip_set_replay_log(self.h, int(max_bytes)))
+
+    def last_adjoint_replayed(self) -> bool:
+        out = C.c_uint32()
+        self._check(self.L.shiro_hip_last_adjoint_replayed(self.h, C.byref(out)))
+        return bool(out.value)
 
     def valu_probe(self, iterations=20000):
         """(milliseconds, VALU wave-instructions) of the FMA issue-rate probe."""

@@ -12,7 +12,9 @@ Bars (tolerances written in each test):
     of the full 256x256, 32 spp workload: relative L2 error <= 1e-3;
   - 100 Adam steps on the MSE to a target image rendered from seeded texels
     (same samples) lower the loss by a quarter or more and move the texels
-    towards the target.
+    towards the target; the full 256x256, 32 spp workload with textures and
+    dome map as parameters, 100 Adam steps: the loss falls below 0.3 of its
+    start.
 Dome-map texels as parameters, to the same bars: a map replaced on the
 device renders what a fresh upload of it renders, bit for bit; the device
 map gradient vs the oracle's; vs central differences on 64 random seen texel
@@ -25,6 +27,7 @@ import pytest
 from helpers import c5_mini, c5_scene_with, relmse
 from oracle.bindings import OracleScene
 from shiro_amd import abi, workloads
+from shiro_amd.scene import RenderSettings
 
 pytestmark = pytest.mark.gpu
 
@@ -167,6 +170,65 @@ def test_adjoint_vs_frozen_sampling_differences_c5(hip_ctx):
     assert err <= 1e-3, (err, ad[:5], fd[:5])
 
 
+def test_replay_log_adjoint_equals_traced_adjoint():
+    """The adjoint replaying the forward wavefront pass's hits and shadow-ray
+    visibilities (shiro_hip_set_replay_log) against the adjoint that traces
+    the rays again, on the half-size c5 workload with texture and dome-map
+    gradients: relative L2 difference <= 1e-5 per array (float atomics add in
+    another order).  The log is used only for the samples, camera and scene
+    content of the logged pass."""
+    import torch
+    from shiro_amd.hip import Context
+    wl = workloads.c5(scale=0.5, spp=8)
+    st = RenderSettings(**{**wl.settings.__dict__, "transport": abi.TRANSPORT_WAVEFRONT})
+    alb0, rough0 = wl.textures_init
+    scene = c5_scene_with(wl, alb0, rough0)
+    ctx = Context(0)
+    try:
+        ctx.upload(scene)
+        n = st.width * st.height
+        frame = [torch.empty(n * k, dtype=torch.float32, device="cuda") for k in (4, 3, 3, 1)]
+        rng = np.random.default_rng(11)
+        w = torch.from_numpy(rng.normal(size=(st.height, st.width, 4)).astype(np.float32)).cuda()
+
+        def adjoint():
+            grads = [torch.zeros(np.shape(t.texels), dtype=torch.float32, device="cuda") for t in scene.textures]
+            env = [torch.zeros(np.shape(scene.domeLights[0].pixels), dtype=torch.float32, device="cuda")]
+            torch.cuda.synchronize()
+            ctx.render_adjoint(wl.camera, st, w.data_ptr(), [g.data_ptr() for g in grads],
+                               env_grad_ptrs=[e.data_ptr() for e in env])
+            return [g.cpu().numpy() for g in grads + env], ctx.last_adjoint_replayed()
+
+        def forward(start=0):
+            ctx.render_device(wl.camera, st, [b.data_ptr() for b in frame], start, st.samplesPerPixel)
+
+        _log("c5 half-size: traced adjoint, then the replayed one")
+        forward()
+        traced, replayed = adjoint()
+        assert not replayed
+        ctx.set_replay_log(1 << 30)
+        forward()
+        logged, replayed = adjoint()
+        assert replayed
+        for k, (a, b) in enumerate(zip(traced, logged)):
+            err = np.linalg.norm(a - b) / np.linalg.norm(a)
+            _log(f"  array {k}: relative L2 difference {err:.2e}")
+            assert np.linalg.norm(a) > 0 and err <= 1e-5, (k, err)
+        # Another batch of samples, then a texture update: the log no longer
+        # matches and the adjoint traces.
+        forward(start=st.samplesPerPixel)
+        assert not adjoint()[1]
+        forward()
+        assert adjoint()[1]
+        ctx.update_texture(0, alb0)
+        assert not adjoint()[1]
+        ctx.set_replay_log(0)
+        forward()
+        assert not adjoint()[1]
+    finally:
+        ctx.close()
+
+
 def test_adam_through_torch_lowers_the_loss():
     import torch
     from shiro_amd.diff import TextureRenderer
@@ -201,6 +263,59 @@ def test_adam_through_torch_lowers_the_loss():
     e1 = float(torch.abs(albedo - torch.from_numpy(alb_t).cuda()).mean())
     _log(f"mean |base colour - target texels| {e0:.4f} -> {e1:.4f}")
     assert e1 < e0
+
+
+def test_adam_full_size_c5_textures_and_dome():
+    """The full c5 workload as the benchmark runs it: 256x256, 32 spp,
+    base-colour and roughness textures and the dome map as parameters,
+    100 Adam steps through torch.autograd.  The image loss falls below 0.3
+    of its start.  The parameters themselves need not approach the targets:
+    a brighter map and darker base colours render alike (the lighting and
+    albedo scale ambiguity), so their errors are only reported."""
+    import copy
+    import torch
+    from shiro_amd.diff import TextureRenderer
+    from shiro_amd.scene import Texture
+    wl = workloads.c5()
+    st = wl.settings
+    assert (st.width, st.height, st.samplesPerPixel) == (256, 256, 32)
+    alb_t, rough_t = wl.textures_truth
+    alb0, rough0 = wl.textures_init
+    scene = copy.copy(wl.scene)
+    scene.textures = [Texture(alb_t), Texture(rough_t)]
+    _log("c5 full-size Adam: textures and dome map")
+    r = TextureRenderer(scene, wl.camera, st, env_domes=[0])
+    with torch.no_grad():
+        target = r.render(torch.from_numpy(alb_t).cuda(), torch.from_numpy(rough_t).cuda(),
+                          env=[torch.from_numpy(wl.env_truth).cuda()]).clone()
+    albedo = torch.from_numpy(alb0.copy()).cuda().requires_grad_(True)
+    rough = torch.from_numpy(rough0.copy()).cuda().requires_grad_(True)
+    sky = torch.from_numpy(wl.env_init.copy()).cuda().requires_grad_(True)
+    opt = torch.optim.Adam([{"params": [albedo, rough], "lr": 0.02}, {"params": [sky], "lr": 0.05}])
+    losses = []
+    for step in range(100):
+        opt.zero_grad()
+        img = r.render(albedo, rough, env=[sky])
+        loss = ((img - target) ** 2).mean()
+        loss.backward()
+        opt.step()
+        with torch.no_grad():
+            albedo.clamp_(0.0, 1.0)
+            rough.clamp_(0.05, 1.0)
+            sky.clamp_(min=0.0)
+        losses.append(loss.item())
+        if step % 25 == 0:
+            _log(f"  step {step}: loss {losses[-1]:.4e}")
+    _log(f"loss {losses[0]:.4e} -> {losses[-1]:.4e}")
+    assert np.isfinite(losses).all()
+    assert losses[-1] < 0.3 * losses[0]
+    with torch.no_grad():
+        ea0 = float(np.abs(alb0 - alb_t).mean())
+        ea1 = float(torch.abs(albedo - torch.from_numpy(alb_t).cuda()).mean())
+        es0 = float(np.abs(wl.env_init - wl.env_truth).mean())
+        es1 = float(torch.abs(sky - torch.from_numpy(wl.env_truth).cuda()).mean())
+    _log(f"mean |base colour - target| {ea0:.4f} -> {ea1:.4f}; mean |map - target| {es0:.4f} -> {es1:.4f}")
+    assert np.isfinite([ea1, es1]).all()
 
 
 # ---------------------------------------------------------------------------
