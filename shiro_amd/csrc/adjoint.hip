@@ -209,8 +209,9 @@ DEV V3 ratio(V3 a, V3 b) {
 // Replays the path of pixel (x, y), sample s with pixel adjoint a (the
 // method described at the top of this file): adds its light-sample and
 // texel gradients, and its dome-map ones to `cache`.  MAXD bounds the path
-// length, sizing the vertex tape.
-template <uint32_t MAXD>
+// length, sizing the vertex tape.  LOG: read hits and visibilities from the
+// replay log (P.hitLog, P.visLog) instead of tracing.
+template <uint32_t MAXD, bool LOG>
 DEV void adjoint_path(const AdjointParams& P, GradCache& cache, uint32_t x, uint32_t y, uint32_t s, V3 a) {
     const DevSettings& st = P.settings;
     const DevScene& S = P.scene;
@@ -230,7 +231,7 @@ DEV void adjoint_path(const AdjointParams& P, GradCache& cache, uint32_t x, uint
         const Ray ray = path_ray(ps);
         const size_t li = replay_index(st.width, st.height, P.sampleCount, x, y, s, depth);
         HitRec h;
-        if (P.hitLog) {
+        if constexpr (LOG) {
             // The forward pass's hit of this vertex (its record as the
             // wavefront stores it; the triangle index follows from the slot).
             const float4 hh = P.hitLog[li];
@@ -280,7 +281,7 @@ DEV void adjoint_path(const AdjointParams& P, GradCache& cache, uint32_t x, uint
         V3 gBase{0.0f, 0.0f, 0.0f};
         float gRough = 0.0f;
         const V3 aT = a * ps.T;
-        const uint32_t visBits = P.visLog ? P.visLog[li] : 0u;
+        const uint32_t visBits = LOG ? P.visLog[li] : 0u;
         uint32_t record = 0;
         for (uint32_t j = 0; j < nLight; ++j) {
             V3 ld, contrib, f, lf;
@@ -290,7 +291,7 @@ DEV void adjoint_path(const AdjointParams& P, GradCache& cache, uint32_t x, uint
             if (!direct_sample(S, st, sf, M, smp, depth, j, &ld, &contrib, &group, &maxDist, cnt, &f, &lf, &dome))
                 continue;
             bool visible;
-            if (P.visLog) {
+            if constexpr (LOG) {
                 // Light samples that need a shadow ray are numbered in order,
                 // as the forward pass's shadow records are.
                 visible = ((visBits >> (record & 31u)) & 1u) != 0u;
@@ -370,10 +371,12 @@ DEV void adjoint_path(const AdjointParams& P, GradCache& cache, uint32_t x, uint
 // the lanes of a wave replay nearly identical paths of one pixel or a few
 // neighbouring ones, and their texel gradients land on the same entries of
 // the block's cache.  MAXD bounds the path length, sizing the per-thread vertex tape.
-// At most 128 VGPRs (4 waves per SIMD): the replay spills some registers
-// there, and still runs faster than at 2 or 3 waves per SIMD.
-template <uint32_t MAXD>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_adjoint(AdjointParams P) {
+// Register caps (c5, measured): the tracing replay at 128 VGPRs (4 waves
+// per SIMD; it spills some registers there, and still runs faster than at
+// 2 or 3 waves); the replay from the log at 168 (3 waves: 2.16 ms, against
+// 2.35 at 4, 2.23 at 2 and 2.90 at 6).
+template <uint32_t MAXD, bool LOG>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(LOG ? 3 : 4))) void k_adjoint(AdjointParams P) {
     __shared__ GradCache cache;
     cache_clear(cache);
     const uint32_t gl = P.groupLog, rl = 3u - (gl >> 1u), ql = 3u - rl;
@@ -391,7 +394,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
         const float4 db = P.dBeauty[static_cast<size_t>(y) * st.width + x];
         const float inv = 1.0f / static_cast<float>(P.sampleCount);
         const V3 a{db.x * inv, db.y * inv, db.z * inv};
-        if (a.x != 0.0f || a.y != 0.0f || a.z != 0.0f) adjoint_path<MAXD>(P, cache, x, y, s, a);
+        if (a.x != 0.0f || a.y != 0.0f || a.z != 0.0f) adjoint_path<MAXD, LOG>(P, cache, x, y, s, a);
     }
     cache_flush(cache, P);
 }
@@ -402,14 +405,16 @@ hipError_t launch_adjoint(const AdjointParams& P, hipStream_t s) {
     if (blocks > 0x7fffffffull) return hipErrorInvalidValue;
     const dim3 grid(static_cast<uint32_t>(blocks));
     const uint32_t md = P.settings.maxDepth;
-    if (md <= 4u) {
-        hipLaunchKernelGGL(k_adjoint<4>, grid, dim3(64), 0, s, P);
-    } else if (md <= 8u) {
-        hipLaunchKernelGGL(k_adjoint<8>, grid, dim3(64), 0, s, P);
-    } else if (md <= kAdjointMaxDepth) {
-        hipLaunchKernelGGL(k_adjoint<kAdjointMaxDepth>, grid, dim3(64), 0, s, P);
-    } else {
-        return hipErrorInvalidValue;
+    if (md > kAdjointMaxDepth) return hipErrorInvalidValue;
+    const bool log = P.hitLog != nullptr && P.visLog != nullptr;
+    const uint32_t v = (md <= 4u ? 0u : (md <= 8u ? 2u : 4u)) + (log ? 1u : 0u);
+    switch (v) {
+        case 0: hipLaunchKernelGGL((k_adjoint<4, false>), grid, dim3(64), 0, s, P); break;
+        case 1: hipLaunchKernelGGL((k_adjoint<4, true>), grid, dim3(64), 0, s, P); break;
+        case 2: hipLaunchKernelGGL((k_adjoint<8, false>), grid, dim3(64), 0, s, P); break;
+        case 3: hipLaunchKernelGGL((k_adjoint<8, true>), grid, dim3(64), 0, s, P); break;
+        case 4: hipLaunchKernelGGL((k_adjoint<kAdjointMaxDepth, false>), grid, dim3(64), 0, s, P); break;
+        default: hipLaunchKernelGGL((k_adjoint<kAdjointMaxDepth, true>), grid, dim3(64), 0, s, P); break;
     }
     return hipGetLastError();
 }

@@ -46,7 +46,25 @@ DEV uint32_t ring_push_sorted(uint32_t mask, const float* tn, const uint32_t* w,
         cswap(k[0], v[0], k[2], v[2]);
         cswap(k[1], v[1], k[3], v[3]);
         cswap(k[1], v[1], k[2], v[2]);
-        for (int i = nh - 1; i >= 1; --i) ring_push<KEEP_T>(v[i], k[i], lane, ovNode, ovT, sp, base);
+        // Pushes v[nh-1] .. v[1] (farthest first) without a loop: entry i
+        // goes to stack position sp + nh - 1 - i.  Room is made first by
+        // moving the oldest ring entries to the overflow array (rare).
+        const int m = nh - 1;
+        while (sp + m - base > kRing) {
+            const uint32_t old = ring_slot(base, lane);
+            ovNode[base] = gRingNode[old];
+            if (KEEP_T) ovT[base] = gRingT[old];
+            ++base;
+        }
+#pragma unroll
+        for (int i = 1; i < 4; ++i) {
+            if (i <= m) {
+                const uint32_t at = ring_slot(sp + m - i, lane);
+                gRingNode[at] = v[i];
+                if (KEEP_T) gRingT[at] = k[i];
+            }
+        }
+        sp += m;
         return v[0];
     } else {
         float k[8];
