@@ -384,7 +384,8 @@ typedef struct shiro_hip_scene_info {
     uint32_t bvh8Depth;
     uint32_t bvhStack;       /* most traversal stack entries of the 4-wide tree */
     uint32_t bvh8Stack;      /* of the 8-wide tree; above 95 it is not uploaded */
-    uint32_t reserved;
+    uint32_t plainBsdf;      /* 1: no material needs anisotropy, thin film, sheen or subsurface
+                                wrap, so the wavefront shades with the BSDF code that leaves them out */
 } shiro_hip_scene_info;
 /* Replaces the texels of texture `index` of the uploaded scene (same size
  * and channel count) without rebuilding anything.  flags:

@@ -118,7 +118,7 @@ class SceneInfo(C.Structure):
         ("maxLeafSize", C.c_uint32), ("deviceBytes", C.c_uint64),
         ("buildSeconds", C.c_double), ("uploadSeconds", C.c_double),
         ("bvh8NodeCount", C.c_uint64), ("bvh8Depth", C.c_uint32), ("bvhStack", C.c_uint32),
-        ("bvh8Stack", C.c_uint32), ("reserved", C.c_uint32),
+        ("bvh8Stack", C.c_uint32), ("plainBsdf", C.c_uint32),
     ]
 
 

@@ -817,6 +817,7 @@ int upload_scene(shiro_hip_ctx* ctx, const shiro_hip_scene* s, bool refit) {
     d.textures = table.empty() ? nullptr : ctx->texTable.as<DevTexture>();
     d.textureCount = s->textureCount;
     d.textured = flat.textured ? 1u : 0u;
+    d.plainBsdf = flat.plainBsdf ? 1u : 0u;
     d.nodes = ctx->nodes.as<float4>();
     d.tris = ctx->tris.as<float4>();
     d.hitdata = ctx->hitdata.as<float4>();
@@ -923,6 +924,7 @@ int upload_scene(shiro_hip_ctx* ctx, const shiro_hip_scene* s, bool refit) {
     ctx->info.bvhStack = bvh.stack;
     ctx->info.bvh8Stack = bvh.stack8;
     ctx->info.maxLeafSize = bvh.maxLeafSize;
+    ctx->info.plainBsdf = flat.plainBsdf ? 1u : 0u;
     ctx->info.deviceBytes = bytes;
     ctx->info.buildSeconds = std::chrono::duration<double>(t1 - t0).count();
     ctx->info.uploadSeconds = std::chrono::duration<double>(t2 - t1).count();

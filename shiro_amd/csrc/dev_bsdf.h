@@ -51,9 +51,14 @@ DEV float anisotropy_phase(V3 n, V3 t, V3 v, V3 l, float rot) {
 
 // CpuPathTracer.cpp:881-893.  With zero anisotropy the phase is multiplied
 // by zero and the result is clamp(r, 0.02, 1) exactly, so it is skipped.
+// PLAIN: the scene's materials have no specular or coat anisotropy, no thin
+// film, no sheen and no subsurface wrap (DevScene::plainBsdf), so those
+// terms are known and their code is left out; the results are those of the
+// general evaluation for such materials.
+template <bool PLAIN = false>
 DEV float roughness_eval(float r, float aniso, float rot, const shiro_hip_material& m, V3 n, V3 v, V3 l) {
     const float a = clampf(aniso, -0.95f, 0.95f);
-    if (a == 0.0f) return clampf(r * (1.0f - 0.35f * a * 1.0f), 0.02f, 1.0f);
+    if (PLAIN || a == 0.0f) return clampf(r * (1.0f - 0.35f * a * 1.0f), 0.02f, 1.0f);
     const float phase = anisotropy_phase(n, projected_tangent(m, n), v, l, rot);
     return clampf(r * (1.0f - 0.35f * a * phase), 0.02f, 1.0f);
 }
@@ -146,8 +151,9 @@ DEV float specular_pdf(V3 n, V3 v, V3 l, float r) {
 }
 
 // CpuPathTracer.cpp:934-968.
+template <bool PLAIN = false>
 DEV V3 apply_thin_film(V3 lobe, const shiro_hip_material& m, V3 n, V3 v, V3 l) {
-    if (m.thinFilmThickness <= 0.0f) return lobe;
+    if (PLAIN || m.thinFilmThickness <= 0.0f) return lobe;
     const V3 h = safe_normalize(v + l, n);
     const float vDotH = clampf(dot(v, h), 0.0f, 1.0f);
     const float thickness = m.thinFilmThickness;
@@ -164,6 +170,7 @@ DEV V3 apply_thin_film(V3 lobe, const shiro_hip_material& m, V3 n, V3 v, V3 l) {
 // Reflective mixture pdf (CpuPathTracer.cpp:1232-1273).  The out-of-line
 // BSDF functions take the material as separate pointers, which travel in
 // registers (a Mat passed by value goes through the stack).
+template <bool PLAIN>
 DEV_CALL float bsdf_pdf_at(const shiro_hip_material* mp, const MaterialDerived* dsp, V3 n, V3 cn, V3 v, V3 l) {
     const float nl = clampf(dot(n, l), 0.0f, 1.0f);
     if (nl <= 0.0f) return 0.0f;
@@ -173,16 +180,17 @@ DEV_CALL float bsdf_pdf_at(const shiro_hip_material* mp, const MaterialDerived* 
     const float wr = wd + ws + wc;
     if (wr <= 0.0f) return 0.0f;
     const float pd = wd / wr, ps = ws / wr, pc = wc / wr;
-    const float rs = roughness_eval(ds.baseRoughness, m.specularAnisotropy, m.specularRotation, m, n, v, l);
+    const float rs = roughness_eval<PLAIN>(ds.baseRoughness, m.specularAnisotropy, m.specularRotation, m, n, v, l);
     float coatTerm = 0.0f;
     if (pc != 0.0f) {
-        const float rc = roughness_eval(m.coatRoughness, m.coatAnisotropy, m.coatRotation, m, cn, v, l);
+        const float rc = roughness_eval<PLAIN>(m.coatRoughness, m.coatAnisotropy, m.coatRotation, m, cn, v, l);
         coatTerm = pc * specular_pdf(cn, v, l, rc);
     }
     return pd * nl * kInvPi + ps * specular_pdf(n, v, l, rs) + coatTerm;
 }
 
 // Layered BSDF value (CpuPathTracer.cpp:1275-1354).
+template <bool PLAIN>
 DEV_CALL V3 bsdf_eval_at(const shiro_hip_material* mp, const MaterialDerived* dp, V3 n, V3 cn, V3 v, V3 l) {
     const float nv = clampf(dot(n, v), 0.0f, 1.0f);
     const float nl = clampf(dot(n, l), 0.0f, 1.0f);
@@ -204,19 +212,23 @@ DEV_CALL V3 bsdf_eval_at(const shiro_hip_material* mp, const MaterialDerived* dp
     const float sl = sqrtf(fmaxs(0.0f, 1.0f - nl * nl));
     const float sinAlpha = fmaxs(sv, sl);
     const float tanBeta = fmins(sv / fmaxs(nv, 1.0e-6f), sl / fmaxs(nl, 1.0e-6f));
-    const float wrap = clampf(m.subsurface * m.subsurfaceScale *
-                                  (0.35f + 0.15f * clampf(m.subsurfaceAnisotropy, -1.0f, 1.0f) +
-                                   0.1f * clampf(max3(ld3(m.subsurfaceRadius)) / 3.0f, 0.0f, 1.0f)),
-                              0.0f, 0.9f);
-    const float wnl = clampf((nl + wrap) / (1.0f + wrap), 0.0f, 1.0f);
+    // With no subsurface wrap (0 < nl <= 1 here), wnl = clamp((nl + 0) / 1) = nl.
+    float wnl = nl;
+    if (!PLAIN) {
+        const float wrap = clampf(m.subsurface * m.subsurfaceScale *
+                                      (0.35f + 0.15f * clampf(m.subsurfaceAnisotropy, -1.0f, 1.0f) +
+                                       0.1f * clampf(max3(ld3(m.subsurfaceRadius)) / 3.0f, 0.0f, 1.0f)),
+                                  0.0f, 0.9f);
+        wnl = clampf((nl + wrap) / (1.0f + wrap), 0.0f, 1.0f);
+    }
     const float dterm = (A + B * fmaxs(0.0f, cosPhi) * sinAlpha * tanBeta) * (wnl / fmaxs(nl, 1.0e-6f));
     const V3 diffuse = ld3(d.diffuseColor) * underCoat * underBase * (kInvPi * dterm);
-    const float rs = roughness_eval(d.baseRoughness, m.specularAnisotropy, m.specularRotation, m, n, v, l);
-    const V3 spec = underCoat * apply_thin_film(specular_lobe(n, v, l, f0b, rs), m, n, v, l);
-    const float rc = roughness_eval(m.coatRoughness, m.coatAnisotropy, m.coatRotation, m, cn, v, l);
-    const V3 coat = apply_thin_film(specular_lobe(cn, v, l, f0c, rc), m, cn, v, l);
+    const float rs = roughness_eval<PLAIN>(d.baseRoughness, m.specularAnisotropy, m.specularRotation, m, n, v, l);
+    const V3 spec = underCoat * apply_thin_film<PLAIN>(specular_lobe(n, v, l, f0b, rs), m, n, v, l);
+    const float rc = roughness_eval<PLAIN>(m.coatRoughness, m.coatAnisotropy, m.coatRotation, m, cn, v, l);
+    const V3 coat = apply_thin_film<PLAIN>(specular_lobe(cn, v, l, f0c, rc), m, cn, v, l);
     V3 out = diffuse + spec + coat;
-    if (d.flags & kMatSheen) {
+    if (!PLAIN && (d.flags & kMatSheen)) {
         const V3 h = safe_normalize(v + l, n);
         const float sheenW = clampf(m.sheen, 0.0f, 1.0f);
         const float lh = clampf(dot(l, h), 0.0f, 1.0f);
@@ -226,8 +238,14 @@ DEV_CALL V3 bsdf_eval_at(const shiro_hip_material* mp, const MaterialDerived* dp
     return out;
 }
 
-DEV float bsdf_pdf(const Mat& M, V3 n, V3 cn, V3 v, V3 l) { return bsdf_pdf_at(M.m, M.ds, n, cn, v, l); }
-DEV V3 bsdf_eval(const Mat& M, V3 n, V3 cn, V3 v, V3 l) { return bsdf_eval_at(M.m, M.d, n, cn, v, l); }
+template <bool PLAIN = false>
+DEV float bsdf_pdf(const Mat& M, V3 n, V3 cn, V3 v, V3 l) {
+    return bsdf_pdf_at<PLAIN>(M.m, M.ds, n, cn, v, l);
+}
+template <bool PLAIN = false>
+DEV V3 bsdf_eval(const Mat& M, V3 n, V3 cn, V3 v, V3 l) {
+    return bsdf_eval_at<PLAIN>(M.m, M.d, n, cn, v, l);
+}
 
 // Derivatives of bsdf_eval at fixed directions with respect to the base
 // colour (per channel: d f_c / d baseColor_c) and the roughness

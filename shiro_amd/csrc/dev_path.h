@@ -399,7 +399,7 @@ struct DomeSampleInfo {
     float s;
 };
 
-template <bool COUNT>
+template <bool COUNT, bool PLAIN = false>
 DEV bool direct_sample(const DevScene& S, const DevSettings& st, const Surface& sf, const Mat& M, const Sampler& smp,
                        uint32_t depth, uint32_t j, V3* dirOut, V3* contrib, uint32_t* group, float* maxDist,
                        Counters<COUNT>& c, V3* fOut = nullptr, V3* lfOut = nullptr,
@@ -411,7 +411,7 @@ DEV bool direct_sample(const DevScene& S, const DevSettings& st, const Surface& 
         const float nl = clampf(dot(sf.n, ld), 0.0f, 1.0f);
         if (nl <= 0.0f) return false;
         *dirOut = ld;
-        const V3 f = bsdf_eval(M, sf.n, sf.cn, sf.v, ld);
+        const V3 f = bsdf_eval<PLAIN>(M, sf.n, sf.cn, sf.v, ld);
         *contrib = clamp_max_component(f * ld3(dl.radiance) * nl, kDirectClamp);
         if (fOut) *fOut = f;
         if (lfOut) *lfOut = ld3(dl.radiance) * nl;
@@ -443,9 +443,9 @@ DEV bool direct_sample(const DevScene& S, const DevSettings& st, const Surface& 
         const float pa = sel / fmaxs(e.area, 1.0e-6f);
         const float lp = pa * d2 / fmaxs(lc, 1.0e-6f);
         if (!(lp > 0.0f)) return false;
-        const float bp = bsdf_pdf(M, sf.n, sf.cn, sf.v, ld);
+        const float bp = bsdf_pdf<PLAIN>(M, sf.n, sf.cn, sf.v, ld);
         const float w = power_heuristic(lp, bp);
-        const V3 f = bsdf_eval(M, sf.n, sf.cn, sf.v, ld);
+        const V3 f = bsdf_eval<PLAIN>(M, sf.n, sf.cn, sf.v, ld);
         *dirOut = ld;
         *contrib = clamp_max_component(f * e.radiance * (nl * w / lp), kDirectClamp);
         if (fOut) *fOut = f;
@@ -463,10 +463,10 @@ DEV bool direct_sample(const DevScene& S, const DevSettings& st, const Surface& 
         return false;
     const float nl = clampf(dot(sf.n, ld), 0.0f, 1.0f);
     if (nl <= 0.0f) return false;
-    const float bp = bsdf_pdf(M, sf.n, sf.cn, sf.v, ld);
+    const float bp = bsdf_pdf<PLAIN>(M, sf.n, sf.cn, sf.v, ld);
     const float w = power_heuristic(lp, bp);
     *dirOut = ld;
-    const V3 f = bsdf_eval(M, sf.n, sf.cn, sf.v, ld);
+    const V3 f = bsdf_eval<PLAIN>(M, sf.n, sf.cn, sf.v, ld);
     *contrib = clamp_max_component(f * lr * (nl * w / lp), kDirectClamp);
     if (fOut) *fOut = f;
     if (lfOut) *lfOut = lr * (nl * w / lp);
@@ -520,6 +520,7 @@ struct BounceRec {
 // Lobe choice, BSDF sampling, throughput update and Russian roulette
 // (CpuPathTracer.cpp:1750-2013).  Returns false when the path ends.  `rec`
 // (optional) receives the throughput factors of a surviving bounce.
+template <bool PLAIN = false>
 DEV bool sample_bounce(const DevScene& S, const DevSettings& st, uint32_t depth, const Sampler& smp,
                        const Surface& sf, PathState& ps, BounceRec* rec = nullptr) {
     const Mat& M = sf.M;
@@ -652,8 +653,8 @@ DEV bool sample_bounce(const DevScene& S, const DevSettings& st, uint32_t depth,
     // The reflective lobes evaluate the BSDF (and its mixture pdf) once,
     // after the lobe branches, so that a wave whose lanes picked different
     // lobes runs these calls once rather than once per branch.
-    if (needPdf) pdf = fmaxs((wr / wsum) * bsdf_pdf(M, n, cn, v, next), 1.0e-6f);
-    if (needEval) f = bsdf_eval(M, n, cn, v, next);
+    if (needPdf) pdf = fmaxs((wr / wsum) * bsdf_pdf<PLAIN>(M, n, cn, v, next), 1.0e-6f);
+    if (needEval) f = bsdf_eval<PLAIN>(M, n, cn, v, next);
 
     const float cosT = pickT ? fmaxs(fabsf(dot(ng, next)), 1.0e-4f) : clampf(dot(n, next), 0.0f, 1.0f);
     if (cosT <= 0.0f) return false;

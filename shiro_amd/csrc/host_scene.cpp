@@ -289,6 +289,18 @@ void build_emissive(const shiro_hip_scene& s, std::vector<EmissiveEntry>* out, f
     }
 }
 
+// True if the BSDF evaluation of m needs none of its optional terms: no
+// specular or coat anisotropy, no thin film, no sheen, and a subsurface wrap
+// of exactly 0 (subsurface 0 with finite wrap parameters), so that the
+// device's plain evaluation gives the general one's results.
+bool plain_bsdf(const shiro_hip_material& m) {
+    const bool finite = std::isfinite(m.subsurfaceScale) && std::isfinite(m.subsurfaceAnisotropy) &&
+                        std::isfinite(m.subsurfaceRadius[0]) && std::isfinite(m.subsurfaceRadius[1]) &&
+                        std::isfinite(m.subsurfaceRadius[2]);
+    return m.specularAnisotropy == 0.0f && m.coatAnisotropy == 0.0f && m.thinFilmThickness <= 0.0f &&
+           !(m.sheen > 0.0f) && m.subsurface == 0.0f && finite;
+}
+
 bool flatten_scene(const shiro_hip_scene& s, FlatScene* out, std::string* err) {
     *out = FlatScene();
     if (s.materialCount > 0 && !s.materials) {
@@ -343,7 +355,9 @@ bool flatten_scene(const shiro_hip_scene& s, FlatScene* out, std::string* err) {
             return false;
         }
         if (m.baseColorTexture >= 0 || m.roughnessTexture >= 0) out->textured = true;
+        if (!plain_bsdf(m)) out->plainBsdf = false;
     }
+    if (!plain_bsdf(default_material())) out->plainBsdf = false;
     for (uint32_t mi = 0; mi < s.meshCount; ++mi) {
         const shiro_hip_mesh& m = s.meshes[mi];
         const uint32_t nt = m.indexCount / 3u;

@@ -51,6 +51,7 @@ struct FlatScene {
     std::vector<float> uv;          // 6 floats per triangle (vertex UVs, 0 where a mesh has none), when hasUvs
     bool hasUvs = false;            // any mesh with UVs
     bool textured = false;          // any material with a texture
+    bool plainBsdf = true;          // no material with anisotropy, thin film, sheen or subsurface wrap
     bool emissive = false;          // any emissive material in use
 };
 

@@ -167,6 +167,7 @@ struct DevScene {
     const DevTexture* textures;   // null without textures
     uint32_t textureCount;
     uint32_t textured;            // any material with kMatTextured
+    uint32_t plainBsdf;           // no material has anisotropy, thin film, sheen or subsurface wrap
     uint32_t nodeCount;
     const float4* nodes8;         // 8-wide tree over the same leaves, null if absent
     uint32_t node8Count;

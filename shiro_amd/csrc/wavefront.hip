@@ -569,7 +569,10 @@ DEV void block_reserve3(const uint32_t c[3], uint32_t* count0, uint32_t* count1,
 // kernel is bound by its instruction stream, and the third wave hides more
 // of its latencies than the spills cost (c3 shade stage 34.4 -> 29.4 ms;
 // capped at 128 VGPRs, four waves, it spills 38 registers and loses).
-template <bool TEX, bool FIRST>
+// PLAIN: the scene's materials need no anisotropy, thin-film, sheen or
+// subsurface-wrap terms (DevScene::plainBsdf): the BSDF evaluation leaves
+// them out.
+template <bool TEX, bool FIRST, bool PLAIN>
 __global__ __launch_bounds__(kShadeThreads) __attribute__((amdgpu_waves_per_eu(3))) void k_wf_shade(WfParams P,
                                                                                                    uint32_t depthArg) {
     const uint32_t depth = FIRST ? 0u : depthArg;
@@ -620,8 +623,8 @@ __global__ __launch_bounds__(kShadeThreads) __attribute__((amdgpu_waves_per_eu(3
                         V3 ld, contrib;
                         uint32_t group;
                         float maxDist;
-                        if (!direct_sample(P.scene, P.settings, sf, M, smp, depth, j, &ld, &contrib, &group, &maxDist,
-                                           c))
+                        if (!direct_sample<false, PLAIN>(P.scene, P.settings, sf, M, smp, depth, j, &ld, &contrib,
+                                                         &group, &maxDist, c))
                             continue;
                         sr[2u * ns] = make_float4(ld.x, ld.y, ld.z, __uint_as_float(group));
                         sr[2u * ns + 1u] = make_float4(contrib.x, contrib.y, contrib.z, maxDist);
@@ -634,7 +637,7 @@ __global__ __launch_bounds__(kShadeThreads) __attribute__((amdgpu_waves_per_eu(3
                         shadow = true;
                     }
                     const uint32_t insideBefore = ps.bits & kBitsInside;
-                    alive = sample_bounce(P.scene, P.settings, depth, smp, sf, ps);
+                    alive = sample_bounce<PLAIN>(P.scene, P.settings, depth, smp, sf, ps);
                     extChanged = (ps.bits & kBitsInside) != insideBefore && (ps.bits & kBitsInside);
                 }
             }
@@ -772,6 +775,22 @@ void launch_trace(const WfParams& P, const WfLaunch& L, uint32_t d, uint32_t blo
     }
 }
 
+// The shade kernel for the scene's textures and materials and the depth.
+void launch_shade(const WfParams& P, uint32_t d, uint32_t blocks, hipStream_t s) {
+    const uint32_t v = (P.scene.textured ? 4u : 0u) + (d == 0u ? 2u : 0u) + (P.scene.plainBsdf ? 1u : 0u);
+    const dim3 g(blocks), b(kShadeThreads);
+    switch (v) {
+        case 7: hipLaunchKernelGGL((k_wf_shade<true, true, true>), g, b, 0, s, P, d); break;
+        case 6: hipLaunchKernelGGL((k_wf_shade<true, true, false>), g, b, 0, s, P, d); break;
+        case 5: hipLaunchKernelGGL((k_wf_shade<true, false, true>), g, b, 0, s, P, d); break;
+        case 4: hipLaunchKernelGGL((k_wf_shade<true, false, false>), g, b, 0, s, P, d); break;
+        case 3: hipLaunchKernelGGL((k_wf_shade<false, true, true>), g, b, 0, s, P, d); break;
+        case 2: hipLaunchKernelGGL((k_wf_shade<false, true, false>), g, b, 0, s, P, d); break;
+        case 1: hipLaunchKernelGGL((k_wf_shade<false, false, true>), g, b, 0, s, P, d); break;
+        default: hipLaunchKernelGGL((k_wf_shade<false, false, false>), g, b, 0, s, P, d); break;
+    }
+}
+
 hipError_t launch_wavefront_pass(const WfParams& P, const WfLaunch& L, hipStream_t s, StageMarks* marks) {
     const uint32_t total = P.sliceCount * P.tiles * 64u;
     if (total == 0) return hipSuccess;
@@ -801,17 +820,7 @@ hipError_t launch_wavefront_pass(const WfParams& P, const WfLaunch& L, hipStream
             m.mark(SHIRO_HIP_STAGE_ORDER, s);
         }
         const uint32_t sblocks = (blocks + kShadeWaves - 1u) / kShadeWaves;
-        if (P.scene.textured) {
-            if (d == 0u) {
-                hipLaunchKernelGGL((k_wf_shade<true, true>), dim3(sblocks), dim3(kShadeThreads), 0, s, P, d);
-            } else {
-                hipLaunchKernelGGL((k_wf_shade<true, false>), dim3(sblocks), dim3(kShadeThreads), 0, s, P, d);
-            }
-        } else if (d == 0u) {
-            hipLaunchKernelGGL((k_wf_shade<false, true>), dim3(sblocks), dim3(kShadeThreads), 0, s, P, d);
-        } else {
-            hipLaunchKernelGGL((k_wf_shade<false, false>), dim3(sblocks), dim3(kShadeThreads), 0, s, P, d);
-        }
+        launch_shade(P, d, sblocks, s);
         m.mark(d == 0u ? SHIRO_HIP_STAGE_SHADE_FIRST : SHIRO_HIP_STAGE_SHADE, s);
         launch_trace<true>(P, L, d, blocks, tblocks, s);
         m.mark(SHIRO_HIP_STAGE_SHADOW, s);
