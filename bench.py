@@ -329,6 +329,8 @@ def summarize_profile(args):
              "|---|---|---|---|---|---|---|"]
     for k in sorted(stats, key=lambda k: -stats[k][1]):
         calls, total, avg = stats[k]
+        if k.startswith("at::"):
+            continue  # torch's own kernels (the frame comparisons), not the render
         if k not in fetch or k not in write:
             lines.append(f"| `{k}` | {calls} | {avg:.4f} | - | - | - | - |")
             continue
@@ -381,8 +383,8 @@ def summarize_valu(args):
     print(f"VALU issue ceiling: {pk} issues {n_probe:.4g} VALU wave-instructions (SQ_INSTS_VALU) in "
           f"{probe_ms:.3f} ms = {ceiling:.4g} per second\n")
     print("| kernel | avg ms | VALU wave-instr per dispatch | VALU issue rate / ceiling | lane utilisation | "
-          "memory-wait share | MeanOccupancyPerActiveCU |")
-    print("|---|---|---|---|---|---|---|")
+          "wait share | issue-wait share | MeanOccupancyPerActiveCU |")
+    print("|---|---|---|---|---|---|---|---|")
     rows = []
     for k, c in sq.items():
         if k.startswith("at::") or k not in stats or "SQ_INSTS_VALU" not in c:
@@ -390,15 +392,21 @@ def summarize_valu(args):
         ms = stats[k][1]
         rate = c["SQ_INSTS_VALU"] / (ms * 1e-3) / ceiling
         util = c.get("SQ_THREAD_CYCLES_VALU", 0.0) / max(64.0 * c.get("SQ_ACTIVE_INST_VALU", 1.0), 1.0)
-        wait = c.get("SQ_WAIT_INST_ANY", 0.0) / max(c.get("SQ_WAVE_CYCLES", 1.0), 1.0)
+        # Shares of resident wave time: waiting on anything (memory,
+        # dependencies, barriers: SQ_WAIT_ANY) and ready but waiting to be
+        # issued (SQ_WAIT_INST_ANY).
+        cyc = max(c.get("SQ_WAVE_CYCLES", 1.0), 1.0)
+        wait = c["SQ_WAIT_ANY"] / cyc if "SQ_WAIT_ANY" in c else None
+        iwait = c.get("SQ_WAIT_INST_ANY", 0.0) / cyc
         o = occ.get(k, {}).get("MeanOccupancyPerActiveCU")
-        rows.append((ms * stats[k][0], k, ms, c["SQ_INSTS_VALU"], rate, util, wait, o))
+        rows.append((ms * stats[k][0], k, ms, c["SQ_INSTS_VALU"], rate, util, wait, iwait, o))
     out = {}
-    for _, k, ms, n, rate, util, wait, o in sorted(rows, reverse=True):
-        print(f"| `{k}` | {ms:.4f} | {n:.4g} | {rate:.3f} | {util:.2f} | {wait:.2f} | "
-              f"{'-' if o is None else f'{o:.1f}'} |")
+    for _, k, ms, n, rate, util, wait, iwait, o in sorted(rows, reverse=True):
+        print(f"| `{k}` | {ms:.4f} | {n:.4g} | {rate:.3f} | {util:.2f} | {'-' if wait is None else f'{wait:.2f}'} | "
+              f"{iwait:.2f} | {'-' if o is None else f'{o:.1f}'} |")
         out[k] = {"valu_issue_frac": round(rate, 4), "lane_utilisation": round(util, 3),
-                  "memory_wait_share": round(wait, 3), "mean_occupancy": None if o is None else round(o, 2)}
+                  "wait_share": None if wait is None else round(wait, 3), "issue_wait_share": round(iwait, 3),
+                  "mean_occupancy": None if o is None else round(o, 2)}
     print()
     print(json.dumps({"valu_ceiling_per_s": ceiling, "kernels": out}, indent=1))
 
@@ -580,8 +588,10 @@ def main():
         ctx.set_stage_timing(False)
         fm, bw = (tuning["fetchMode"], tuning["bvhWidth"]) if tuning else (0, 4)
         if args.transport == "wavefront":
-            shade_first = "k_wf_shade<true, true>" if wl.scene.textures else "k_wf_shade<false, true>"
-            shade_later = "k_wf_shade<true, false>" if wl.scene.textures else "k_wf_shade<false, false>"
+            tex = "true" if wl.scene.textures else "false"
+            plain = "true" if ctx.scene_info().plainBsdf else "false"
+            shade_first = f"k_wf_shade<{tex}, true, {plain}>"
+            shade_later = f"k_wf_shade<{tex}, false, {plain}>"
             kernels = {
                 "primary": (f"k_wf_primary<{bw}>", camera_b),
                 "extend": (f"k_wf_trace<false, {fm}, {bw}>", bounce_b if stages.get("primary") else
@@ -628,8 +638,18 @@ def main():
             if k:
                 traffic = k["hbm_bytes_per_dispatch"]
                 traffic_src = f"{rec['source']} (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, commit {rec['commit']})"
+        # VALU issue rate of the dominant symbol against the FMA probe's
+        # measured ceiling, from the committed rocprofv3 SQ-counter summary.
+        valu = None
+        vpath = os.path.join(ROOT, "profiles", "valu_issue.json")
+        if os.path.exists(vpath):
+            rec = json.load(open(vpath)).get(args.workload, {})
+            k = rec.get("kernels", {}).get(dom["kernel"])
+            if k:
+                valu = {"issue_frac": k["valu_issue_frac"], "lane_utilisation": k["lane_utilisation"],
+                        "source": f"{rec['source']} (rocprofv3 --pmc SQ_INSTS_VALU ..., commit {rec['commit']})"}
         roofline = {"bound": "hbm", "achieved": dom["achieved"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": dom["frac"], "traffic": traffic, "traffic_source": traffic_src,
+                    "frac": dom["frac"], "traffic": traffic, "traffic_source": traffic_src, "valu": valu,
                     "traffic_frac": (round(traffic / (dom["avg_launch_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 5)
                                      if traffic else None),
                     "kernel": dom["kernel"], "launches_per_step": dom["launches_per_step"],
@@ -645,6 +665,10 @@ def main():
                                    "shadow_node_fetches": c.shadowNodeFetches / ns,
                                    "shadow_tri_tests": c.shadowTriFetches / ns,
                                    "env_taps": c.envTaps / ns, "cdf_steps": c.cdfSteps / ns}}
+        if dom["frac"] > 1.0:
+            roofline["note"] = ("frac > 1: the algorithmic bytes charge every ray for its own node and triangle "
+                                "fetches, which the rays of one wave (same-pixel path groups) share through the "
+                                "caches; traffic_frac is what HBM moved, valu.issue_frac the binding limit")
     # Outside the timed region: the other transport's frame on the same
     # samples (the throughput comparison is at matched error only if the
     # frames agree; both transports evaluate the same per-sample operations,
