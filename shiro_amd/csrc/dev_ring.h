@@ -82,23 +82,26 @@ DEV uint32_t ring_push_sorted(uint32_t mask, const float* tn, const uint32_t* w,
     }
 }
 
-// pop_entry (dev_bvh.h) from the ring stack.
+// pop_entry (dev_bvh.h) from the ring stack.  Entries are always read from
+// the ring: when it runs empty while older entries wait in the overflow
+// array, up to kRing of them move back first (rare).
 template <bool ANY>
 DEV uint32_t ring_pop(float tcull, uint32_t lane, const uint32_t* ovNode, const float* ovT, int& sp, int& base) {
     while (sp > 0) {
-        --sp;
-        uint32_t n;
-        float t = 0.0f;
-        if (sp >= base) {
-            const uint32_t at = ring_slot(sp, lane);
-            n = gRingNode[at];
-            if (!ANY) t = gRingT[at];
-        } else {
-            n = ovNode[sp];
-            if (!ANY) t = ovT[sp];
-            base = sp;
+        if (sp == base) {
+            const int k = base < kRing ? base : kRing;
+            for (int i = base - k; i < base; ++i) {
+                const uint32_t at = ring_slot(i, lane);
+                gRingNode[at] = ovNode[i];
+                if (!ANY) gRingT[at] = ovT[i];
+            }
+            base -= k;
         }
-        if (ANY || t <= tcull) return n;
+        --sp;
+        const uint32_t at = ring_slot(sp, lane);
+        const uint32_t n = gRingNode[at];
+        if (ANY) return n;
+        if (gRingT[at] <= tcull) return n;
     }
     return kEmpty;
 }
