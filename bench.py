@@ -554,22 +554,26 @@ def main():
     ok = bool(torch.isfinite(frame).all().item())
 
     # Roofline of the dominant render kernel.  Algorithmic bytes come from
-    # the fetch counters (same BVH, same traversal order): BVH4 nodes 64 B,
-    # triangles 48 B, hit data 96 B, environment texels 16 B, CDF entries
-    # 4 B, AOV write 44 B per pixel per batch.  Each wavefront kernel symbol
-    # is credited with its own rays' bytes: the camera-packet kernel with the
-    # camera rays' node and triangle fetches, the closest-hit kernel with
-    # those of the bounce rays, the any-hit kernel with the shadow rays'; the
-    # shade kernels with the hit, texel and CDF bytes; queue and path-state
+    # fetch counters: BVH4 nodes 64 B (8-wide 128 B), triangles 48 B, hit
+    # data 96 B, environment texels 16 B, CDF entries 4 B, AOV write 44 B per
+    # pixel per batch.  The wavefront's traversal kernels count their own
+    # fetches (counting instantiations of the same kernels over the whole
+    # frame); hit, texel and CDF reads come from the megakernel's counters
+    # (the shading is the same in both transports).  Each wavefront kernel
+    # symbol is credited with its own bytes: each traversal kernel with the
+    # nodes and triangles its load instructions request, each record once
+    # per wave instruction that reads it (the camera packets read each node
+    # once for 64 rays; in the per-lane kernels the lanes of one instruction
+    # that read the same node, rays of one path group walking in step, share
+    # one request), with every ray's own fetches summed beside it
+    # (frac_per_ray_sum: can exceed 1, since the rays of a path group share
+    # their nodes through L2 and the last-level cache as well); the shade
+    # kernels with the hit, texel and CDF bytes; queue and path-state
     # traffic earns nothing.  Kernel durations come from events recorded
     # after every launch on the render stream, in one extra (untimed) render
     # call.  The dominant kernel is the symbol with the largest time per
-    # frame.  Caveat: the algorithmic count charges every ray for its own
-    # fetches, while camera packets (one LDS stack and one node load per
-    # wave) and same-pixel path groups (neighbouring lanes walk the same
-    # nodes, served by L2 and the last-level cache) share them, so a
-    # packet kernel can exceed the HBM peak on this measure; `traffic` is
-    # what HBM moved per launch by the rocprofv3 counters.
+    # frame.  `traffic` is what HBM moved per launch by the rocprofv3
+    # counters.
     roofline = None
     stages = None
     if not args.no_count:
@@ -581,6 +585,34 @@ def main():
                     (c.triFetches - c.primaryTriFetches) * TRI_BYTES) / ns
         shadow = (c.shadowNodeFetches * NODE_BYTES + c.shadowTriFetches * TRI_BYTES) / ns
         other = (c.hitFetches * HIT_BYTES + c.envTaps * TEXEL_BYTES + c.cdfSteps * CDF_BYTES) / ns
+        wf = None
+        if args.transport == "wavefront":
+            # The traversal kernels' own counts over the whole frame (the
+            # counting instantiations: same traversal, same work
+            # distribution).  The
+            # camera packets fetch each node and triangle once per packet of
+            # 64 rays, so k_wf_primary is credited with its per-packet
+            # fetches; the per-lane kernels with every ray's own fetches.
+            w = ctx.count_wavefront(wl.camera, st, 0, st.samplesPerPixel)
+            nsw = max(w.samples, 1)
+            nb = NODE_BYTES * (2 if w.bvhWidth == 8 else 1)
+
+            def per_ray(k):
+                return (w.nodeFetches[k] * nb + w.triFetches[k] * TRI_BYTES) / nsw
+
+            def per_wave(k):
+                return (w.waveNodeFetches[k] * nb + w.waveTriFetches[k] * TRI_BYTES) / nsw
+
+            camera_b, bounce_b, shadow = per_ray(0), per_ray(1), per_ray(2)
+            wf = {"camera_rays": w.rays[0] / nsw, "bounce_rays": w.rays[1] / nsw, "shadow_rays": w.rays[2] / nsw,
+                  "camera_packet_bytes": round(per_wave(0), 1),
+                  "bytes_per_ray_sum": {"camera": round(camera_b, 1), "bounce": round(bounce_b, 1),
+                                        "shadow": round(shadow, 1)},
+                  "bytes_per_wave_instruction": {"camera": round(per_wave(0), 1), "bounce": round(per_wave(1), 1),
+                                                 "shadow": round(per_wave(2), 1)},
+                  "node_fetches_per_ray": [round(w.nodeFetches[k] / max(w.rays[k], 1), 2) for k in range(3)],
+                  "tri_tests_per_ray": [round(w.triFetches[k] / max(w.rays[k], 1), 2) for k in range(3)],
+                  "shadow_occluded": round(w.shadowOccluded / max(w.rays[2], 1), 4)}
         per_sample = camera_b + bounce_b + shadow + other + 44.0 / st.samplesPerPixel
         ctx.set_stage_timing(True)
         ctx.render_device(wl.camera, st, ptrs, 0, st.samplesPerPixel, synchronize=True)
@@ -592,19 +624,22 @@ def main():
             plain = "true" if ctx.scene_info().plainBsdf else "false"
             shade_first = f"k_wf_shade<{tex}, true, {plain}>"
             shade_later = f"k_wf_shade<{tex}, false, {plain}>"
+            wv = wf["bytes_per_wave_instruction"]
+            packets = bool(stages.get("primary"))
             kernels = {
-                "primary": (f"k_wf_primary<{bw}>", camera_b),
-                "extend": (f"k_wf_trace<false, {fm}, {bw}>", bounce_b if stages.get("primary") else
-                           bounce_b + camera_b),
-                "shadow": (f"k_wf_trace<true, {fm}, {bw}>", shadow),
-                "shade_first": (shade_first, None),
-                "shade": (shade_later, None),
+                "primary": (f"k_wf_primary<{bw}, false>", wv["camera"], camera_b),
+                "extend": (f"k_wf_trace<false, {fm}, {bw}, false>",
+                           wv["bounce"] if packets else wv["bounce"] + wv["camera"],
+                           bounce_b if packets else bounce_b + camera_b),
+                "shadow": (f"k_wf_trace<true, {fm}, {bw}, false>", wv["shadow"], shadow),
+                "shade_first": (shade_first, None, None),
+                "shade": (shade_later, None, None),
             }
         else:
             sym = "k_render_plus<false>" if args.transport == "megakernel_plus" else "k_render<false, false>"
-            kernels = {"megakernel": (sym, per_sample)}
+            kernels = {"megakernel": (sym, per_sample, per_sample)}
         entries = []
-        for stage, (sym, credit) in kernels.items():
+        for stage, (sym, credit, ray_credit) in kernels.items():
             if stage not in stages:
                 continue
             ms, n = stages[stage]["ms"], stages[stage]["launches"]
@@ -616,6 +651,10 @@ def main():
                 e["bytes_per_launch"] = int(bpl)
                 e["achieved"] = round(bpl / (ms / n * 1e-3) / 1e9, 2)
                 e["frac"] = round(e["achieved"] / HBM_PEAK_GBS, 5)
+                # Every ray charged its own fetches (no merging of lanes
+                # that read the same record in one instruction).
+                e["bytes_per_sample_per_ray_sum"] = round(ray_credit, 1)
+                e["frac_per_ray_sum"] = round(ray_credit * samples / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 5)
             entries.append(e)
         dom = max(entries, key=lambda e: e["ms_per_step"])
         if "achieved" not in dom:
@@ -664,11 +703,15 @@ def main():
                                    "camera_tri_tests": c.primaryTriFetches / ns,
                                    "shadow_node_fetches": c.shadowNodeFetches / ns,
                                    "shadow_tri_tests": c.shadowTriFetches / ns,
-                                   "env_taps": c.envTaps / ns, "cdf_steps": c.cdfSteps / ns}}
-        if dom["frac"] > 1.0:
-            roofline["note"] = ("frac > 1: the algorithmic bytes charge every ray for its own node and triangle "
-                                "fetches, which the rays of one wave (same-pixel path groups) share through the "
-                                "caches; traffic_frac is what HBM moved, valu.issue_frac the binding limit")
+                                   "env_taps": c.envTaps / ns, "cdf_steps": c.cdfSteps / ns},
+                    "wavefront_counts": wf}
+        if args.transport == "wavefront":
+            roofline["note"] = ("achieved: bytes the traversal kernel's load instructions request (each node "
+                                "and triangle once per wave instruction that reads it, as the lanes of one "
+                                "instruction reading one record share the request), counted by the kernel's "
+                                "own counting instantiation; frac_per_ray_sum in `kernels` charges every ray "
+                                "its own fetches instead; traffic_frac is what HBM moved (rocprofv3 counters), "
+                                "valu.issue_frac the binding limit")
     # Outside the timed region: the other transport's frame on the same
     # samples (the throughput comparison is at matched error only if the
     # frames agree; both transports evaluate the same per-sample operations,

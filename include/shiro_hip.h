@@ -528,6 +528,33 @@ int shiro_hip_count_fetches(
     const shiro_hip_settings* settings, uint32_t sampleStart,
     uint32_t sampleCount, shiro_hip_fetch_counts* out);
 
+/* Fetch counters of the wavefront transport's traversal kernels (no
+ * counterpart in the reference; measurement): renders the samples once
+ * through the wavefront with the context's tuning, running the counting
+ * instantiations of the traversal kernels (same traversal, same work
+ * distribution), and returns per kernel kind -- [0] camera rays
+ * (k_wf_primary's packets, or the depth-0 k_wf_trace), [1] closest-hit rays
+ * of the later bounces (k_wf_trace<false>), [2] shadow rays
+ * (k_wf_trace<true>) -- the rays, the nodes and triangles read summed over
+ * rays, and the distinct nodes and triangles per wave load instruction
+ * summed over instructions (a camera packet reads each node once for its
+ * 64 rays); for shadow rays also those found occluded. */
+typedef struct shiro_hip_wavefront_counts {
+    uint64_t samples;
+    uint64_t rays[3];
+    uint64_t nodeFetches[3];      /* 64 B (4-wide) or 128 B (8-wide) each */
+    uint64_t triFetches[3];       /* 48 B each */
+    uint64_t waveNodeFetches[3];
+    uint64_t waveTriFetches[3];
+    uint64_t shadowOccluded;
+    uint32_t bvhWidth;            /* tree the kernels walked */
+    uint32_t reserved;
+} shiro_hip_wavefront_counts;
+int shiro_hip_count_wavefront(
+    shiro_hip_ctx* ctx, const shiro_hip_camera* camera,
+    const shiro_hip_settings* settings, uint32_t sampleStart,
+    uint32_t sampleCount, shiro_hip_wavefront_counts* out);
+
 /* ---- Wavefront tuning (no counterpart in the reference: knobs of the
  * wavefront transport, set by the caller or chosen by shiro_hip_autotune). ---- */
 typedef struct shiro_hip_tuning {
@@ -550,6 +577,9 @@ typedef struct shiro_hip_tuning {
                                back to 4 for a scene without an 8-wide tree) */
     uint32_t manyBranches;  /* SHIRO_HIP_MANY_WAVEFRONT_GRAPH: concurrent branches (streams, each
                                with its own wavefront state) of the batch graph, 1-16 (default 4) */
+    uint32_t leafWaitLanes; /* closest-hit traversal: a wave turns to its leaf tests once at most
+                               this many of its 64 lanes still search for a leaf (0-64, default 24) */
+    uint32_t shadowLeafWaitLanes; /* the same for shadow rays (0-64, default 24) */
 } shiro_hip_tuning;
 int shiro_hip_set_tuning(shiro_hip_ctx* ctx, const shiro_hip_tuning* tuning);
 int shiro_hip_get_tuning(const shiro_hip_ctx* ctx, shiro_hip_tuning* tuning);

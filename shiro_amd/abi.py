@@ -132,11 +132,21 @@ class FetchCounts(C.Structure):
     ]
 
 
+class WavefrontCounts(C.Structure):
+    _fields_ = [
+        ("samples", C.c_uint64), ("rays", C.c_uint64 * 3), ("nodeFetches", C.c_uint64 * 3),
+        ("triFetches", C.c_uint64 * 3), ("waveNodeFetches", C.c_uint64 * 3), ("waveTriFetches", C.c_uint64 * 3),
+        ("shadowOccluded", C.c_uint64),
+        ("bvhWidth", C.c_uint32), ("reserved", C.c_uint32),
+    ]
+
+
 class Tuning(C.Structure):
     _fields_ = [
         ("pathsPerPass", C.c_uint32), ("fetchMode", C.c_uint32), ("traceBlocks", C.c_uint32),
         ("stageBlocks", C.c_uint32), ("batchBlocksPerCU", C.c_uint32), ("shadeOrder", C.c_uint32),
         ("primaryPackets", C.c_uint32), ("bvhWidth", C.c_uint32), ("manyBranches", C.c_uint32),
+        ("leafWaitLanes", C.c_uint32), ("shadowLeafWaitLanes", C.c_uint32)
     ]
 
 
@@ -189,7 +199,7 @@ EXPORTS = [
     "shiro_hip_stream", "shiro_hip_synchronize", "shiro_hip_last_render_ms",
     "shiro_hip_scene_info_get", "shiro_hip_qmc_fill", "shiro_hip_intersect",
     "shiro_hip_occluded", "shiro_hip_trace_primary", "shiro_hip_eval_bsdf",
-    "shiro_hip_env_sample", "shiro_hip_env_eval", "shiro_hip_count_fetches",
+    "shiro_hip_env_sample", "shiro_hip_env_eval", "shiro_hip_count_fetches", "shiro_hip_count_wavefront",
     "shiro_hip_set_tuning", "shiro_hip_get_tuning", "shiro_hip_autotune",
     "shiro_hip_set_stage_timing", "shiro_hip_get_stage_times", "shiro_hip_valu_probe",
     "shiro_hip_update_texture", "shiro_hip_render_adjoint",

@@ -119,7 +119,11 @@ struct shiro_hip_ctx {
     DevBuf acB, acA, acN, acD, acC;
     uint32_t acW = 0, acH = 0, acTotal = 0;
     // Wavefront tuning (shiro_hip_set_tuning / shiro_hip_autotune).
-    shiro_hip_tuning tuning{128u << 20, 0u, 8192u, 16384u, 8u, 1u, 16u, 4u, 4u};
+    shiro_hip_tuning tuning{128u << 20, 0u, 8192u, 16384u, 8u, 1u, 16u, 4u, 4u, 24u, 24u};
+    // Fetch counters of the wavefront traversal kernels
+    // (shiro_hip_count_wavefront): on only for that call's render.
+    DevBuf wfCount;
+    bool wfCountOn = false;
     uint32_t computeUnits = 256;
     // Batched rendering: items, work list and the work counter.
     DevBuf manyItems, manyWork, manyCounter;
@@ -416,6 +420,9 @@ int enqueue_wavefront(shiro_hip_ctx* ctx, const DevScene& scene, const shiro_hip
     P.tilesX = plan.tilesX;
     P.hitLog = hitLog;
     P.visLog = visLog;
+    P.leafWait = ctx->tuning.leafWaitLanes;
+    P.shadowLeafWait = ctx->tuning.shadowLeafWaitLanes;
+    P.wfCount = ctx->wfCountOn ? ctx->wfCount.as<unsigned long long>() : nullptr;
     WfLaunch L{};
     L.maxBlocks = ctx->tuning.stageBlocks;
     L.traceBlocks = ctx->tuning.traceBlocks;
@@ -1325,7 +1332,8 @@ int shiro_hip_set_tuning(shiro_hip_ctx* ctx, const shiro_hip_tuning* t) {
     if (t->pathsPerPass == 0 || t->fetchMode > 2u || t->traceBlocks == 0 || t->stageBlocks == 0 ||
         t->batchBlocksPerCU == 0 || t->batchBlocksPerCU > 64u || t->shadeOrder > 1u ||
         (t->primaryPackets > 1u && t->primaryPackets != 4u && t->primaryPackets != 16u && t->primaryPackets != 64u) ||
-        (t->bvhWidth != 4u && t->bvhWidth != 8u) || t->manyBranches == 0 || t->manyBranches > 16u)
+        (t->bvhWidth != 4u && t->bvhWidth != 8u) || t->manyBranches == 0 || t->manyBranches > 16u ||
+        t->leafWaitLanes > 64u || t->shadowLeafWaitLanes > 64u)
         return fail(ctx, SHIRO_HIP_INVALID, "invalid tuning");
     ctx->tuning = *t;
     return SHIRO_HIP_OK;
@@ -1472,6 +1480,47 @@ int shiro_hip_count_fetches(shiro_hip_ctx* ctx, const shiro_hip_camera* camera, 
     out->primaryRays = c.primaryRays;
     out->primaryNodeFetches = c.primaryNodeFetches;
     out->primaryTriFetches = c.primaryTriFetches;
+    return SHIRO_HIP_OK;
+}
+
+int shiro_hip_count_wavefront(shiro_hip_ctx* ctx, const shiro_hip_camera* camera,
+                              const shiro_hip_settings* settings, uint32_t sampleStart, uint32_t sampleCount,
+                              shiro_hip_wavefront_counts* out) {
+    if (!ctx || !camera || !out) return SHIRO_HIP_INVALID;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    int rc = check_settings(ctx, settings);
+    if (rc) return rc;
+    if (!ctx->hasScene) return fail(ctx, SHIRO_HIP_NO_SCENE, "no scene uploaded");
+    std::memset(out, 0, sizeof(*out));
+    out->bvhWidth = ctx->tuning.bvhWidth == 8u && ctx->dev.nodes8 ? 8u : 4u;
+    if (sampleCount == 0) return SHIRO_HIP_OK;
+    HIPC(hipSetDevice(ctx->device), "set device");
+    const uint64_t np = static_cast<uint64_t>(settings->width) * settings->height;
+    HIPC(ctx->frB.reserve(np * 16), "alloc frame");
+    HIPC(ctx->frA.reserve(np * 12), "alloc frame");
+    HIPC(ctx->frN.reserve(np * 12), "alloc frame");
+    HIPC(ctx->frD.reserve(np * 4), "alloc frame");
+    constexpr size_t kBytes = sizeof(unsigned long long) * kWfCountKinds * kWfCountFields;
+    HIPC(ctx->wfCount.reserve(kBytes), "alloc counters");
+    HIPC(hipMemsetAsync(ctx->wfCount.p, 0, kBytes, ctx->stream), "clear counters");
+    const DevFrame f{ctx->frB.as<float>(), ctx->frA.as<float>(), ctx->frN.as<float>(), ctx->frD.as<float>()};
+    ctx->wfCountOn = true;
+    rc = enqueue_wavefront(ctx, ctx->dev, *camera, *settings, sampleStart, sampleCount, f, nullptr);
+    ctx->wfCountOn = false;
+    if (rc) return rc;
+    unsigned long long c[kWfCountKinds * kWfCountFields];
+    HIPC(hipMemcpyAsync(c, ctx->wfCount.p, kBytes, hipMemcpyDeviceToHost, ctx->stream), "readback");
+    HIPC(hipStreamSynchronize(ctx->stream), "count");
+    out->samples = np * sampleCount;
+    for (int k = 0; k < kWfCountKinds; ++k) {
+        const unsigned long long* v = c + k * kWfCountFields;
+        out->rays[k] = v[kWfCountRays];
+        out->nodeFetches[k] = v[kWfCountNodes];
+        out->triFetches[k] = v[kWfCountTris];
+        out->waveNodeFetches[k] = v[kWfCountWaveNodes];
+        out->waveTriFetches[k] = v[kWfCountWaveTris];
+    }
+    out->shadowOccluded = c[2 * kWfCountFields + kWfCountOccluded];
     return SHIRO_HIP_OK;
 }
 

@@ -118,6 +118,8 @@ struct TravCount {
     DEV void tri() const {
         if (COUNT) *tris += 1;
     }
+    DEV void node(uint32_t) const { node(); }
+    DEV void tri(uint32_t) const { tri(); }
 };
 
 DEV float byte_as_float(uint32_t w, int k) { return static_cast<float>((w >> (8 * k)) & 0xffu); }
@@ -308,12 +310,13 @@ DEV uint32_t pop_entry(const uint32_t* stackNode, const float* stackT, int& sp, 
 
 // Tests the triangles of the leaf starting at `slot` (the last one carries
 // the end-of-leaf flag in tris[3*slot+1].w) against the current best hit.
-template <bool COUNT>
-DEV void leaf_closest(const DevScene& S, const RayPre& r, uint32_t slot, HitRec& best, const TravCount<COUNT>& cnt) {
+// CNT: a fetch counter (TravCount, or the wavefront kernels' WaveCount).
+template <class CNT>
+DEV void leaf_closest(const DevScene& S, const RayPre& r, uint32_t slot, HitRec& best, CNT& cnt) {
     for (;;) {
         const float4* tp = S.tris + static_cast<size_t>(slot) * 3u;
         const float4 a = tp[0], b = tp[1], c = tp[2];
-        cnt.tri();
+        cnt.tri(slot);
         float t, b1, b2;
         if (tri_test(r, xyz(a), xyz(b), xyz(c), best.t, &t, &b1, &b2)) {
             const uint32_t prim = __float_as_uint(a.w);
@@ -399,13 +402,13 @@ DEV HitRec intersect_closest(const DevScene& S, const Ray& ray, const TravCount<
 // still meets every triangle whose box its ray enters with an entry
 // distance up to its closest hit, and the closest-hit rule (smallest t,
 // then smallest primitive index) does not depend on the order.
-template <int W>
-DEV HitRec intersect_closest_packet(const DevScene& S, const Ray& ray, bool valid, uint32_t* wstack) {
+// CNT counts the node and triangle fetches (node(id) / tri(slot) per lane).
+template <int W, class CNT>
+DEV HitRec intersect_closest_packet(const DevScene& S, const Ray& ray, bool valid, uint32_t* wstack, CNT& cnt) {
     HitRec best;
     best.t = valid ? ray.tmax : kFltMax;
     best.b1 = best.b2 = 0.0f;
     best.slot = best.prim = kNoHit;
-    const TravCount<false> cnt{nullptr, nullptr};
     RayPre r{};
     if (valid) r = precompute(ray);
     if (S.nodeCount == 0u || !__any(valid)) return best;
@@ -416,6 +419,7 @@ DEV HitRec intersect_closest_packet(const DevScene& S, const Ray& ray, bool vali
             if (valid) leaf_closest(S, r, cur & ~kLeafBit, best, cnt);
         } else {
             const float4* nd = node_at<W>(S, cur);
+            if (valid) cnt.node(cur);
             float tn[W];
             uint32_t wd[W];
 #pragma unroll

@@ -50,6 +50,21 @@ struct WfBuffers {
     uint32_t shadeOrder;                    // 1: shade depth > 0 hits in class order
 };
 
+// Fetch counters of the traversal kernels' measurement instantiations:
+// WfParams::wfCount[kind * kWfCountFields + field], kind 0 camera rays
+// (k_wf_primary, or k_wf_trace at depth 0), 1 closest-hit rays of later
+// bounces, 2 shadow rays.
+constexpr int kWfCountKinds = 3;
+enum : int {
+    kWfCountRays,       // rays traced
+    kWfCountNodes,      // nodes read, summed over rays
+    kWfCountTris,       // triangles tested, summed over rays
+    kWfCountWaveNodes,  // distinct nodes per wave load instruction, summed
+    kWfCountWaveTris,   // distinct triangles per wave load instruction, summed
+    kWfCountOccluded,   // shadow rays found occluded
+    kWfCountFields
+};
+
 struct WfParams {
     DevScene scene;
     DevCamera camera;
@@ -67,6 +82,13 @@ struct WfParams {
     // to replay the batch without tracing.
     float4* hitLog;
     uint32_t* visLog;
+    // Traversal knobs: lanes that may still look for a leaf when a wave
+    // turns to its leaf tests (closest hit, any hit).
+    uint32_t leafWait, shadowLeafWait;
+    // Measurement runs only (null otherwise): fetch counters, see
+    // kWfCountFields; the counting instantiations of the traversal kernels
+    // run when it is set.
+    unsigned long long* wfCount;
 };
 
 // Entry of the replay log for vertex `depth` of sample s (of the batch) of

@@ -45,6 +45,60 @@ namespace shiro_hip {
 
 DEV uint32_t lane_id() { return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u)); }
 
+// Number of distinct values of `id` over the active lanes, returned to the
+// lowest active lane (0 to the others).
+DEV uint32_t wave_distinct(uint32_t id) {
+    const unsigned long long active = __ballot(true);
+    unsigned long long m = active;
+    uint32_t n = 0;
+    while (m != 0ull) {
+        const int l = __ffsll(static_cast<long long>(m)) - 1;
+        const uint32_t v = __shfl(id, l);
+        m &= ~__ballot(id == v);
+        ++n;
+    }
+    return static_cast<int>(lane_id()) == __ffsll(static_cast<long long>(active)) - 1 ? n : 0u;
+}
+
+// Fetch counters of the measurement instantiations of the traversal kernels
+// (ON = true, results in WfParams::wfCount): per lane, the nodes and
+// triangles each ray's traversal reads; per wave, the distinct nodes and
+// triangles of each load instruction (the lanes of one instruction that
+// read the same record need it once: every node of a camera packet, and
+// the rays of one path group that walk the same nodes in step).
+template <bool ON>
+struct WaveCount {
+    uint32_t v[kWfCountFields] = {};
+    DEV void ray() {
+        if (ON) ++v[kWfCountRays];
+    }
+    DEV void node(uint32_t id) {
+        if (ON) {
+            ++v[kWfCountNodes];
+            v[kWfCountWaveNodes] += wave_distinct(id);
+        }
+    }
+    DEV void tri(uint32_t slot) {
+        if (ON) {
+            ++v[kWfCountTris];
+            v[kWfCountWaveTris] += wave_distinct(slot);
+        }
+    }
+    DEV void add(int field, bool c) {
+        if (ON && c) ++v[field];
+    }
+    // Sums the wave's counters into out[kind * kWfCountFields + field];
+    // called by every lane of the wave.
+    DEV void flush(unsigned long long* out, uint32_t kind) {
+        if (!ON || !out) return;
+        for (int f = 0; f < kWfCountFields; ++f) {
+            uint32_t x = v[f];
+            for (int d = 32; d >= 1; d >>= 1) x += __shfl_xor(x, d);
+            if (lane_id() == 0u && x) atomicAdd(out + kind * kWfCountFields + f, static_cast<unsigned long long>(x));
+        }
+    }
+};
+
 struct PathCoord {
     uint32_t x, y, s;
 };
@@ -207,14 +261,14 @@ struct WavePool {
 // FETCH: 0 = wave-synchronous rounds (a wave takes new rays only when all
 // its lanes are done), 1 = lanes refill as their rays finish (static item
 // order), 2 = lanes refill from per-wave pools of the global queue.
-// Lanes of a wave that may still be searching for a leaf when the wave
-// turns to its leaf tests (measured: 24 of 64 is 5% faster than waiting for
-// every lane on c3, 12-32 are within 1%).
-constexpr int kLeafWaitLanes = 24;
-
+// The node phase ends when at most P.leafWait (closest hit) or
+// P.shadowLeafWait (any hit) lanes of the wave are still searching for a
+// leaf (closest hit, measured: 24 of 64 is 5% faster than waiting for every
+// lane on c3, 12-32 are within 1%).
+// COUNT: the measurement instantiation (fetch counters in P.wfCount).
 // W: tree width (4: `nodes`, 8: `nodes8`).  At most 64 VGPRs for the
 // 4-wide tree, so that eight waves fit on each SIMD.
-template <bool ANY, int FETCH, int W>
+template <bool ANY, int FETCH, int W, bool COUNT>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W == 8 ? (ANY ? 6 : 4) : 8))) void k_wf_trace(WfParams P, uint32_t depth) {
     const uint32_t md = P.settings.maxDepth;
     const uint32_t n = ANY ? P.B.counts[P.B.shadowRayBase + depth] : stage_items(P, depth);
@@ -246,7 +300,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W == 8 ? (AN
     best.t = kFltMax;
     best.b1 = best.b2 = 0.0f;
     best.slot = best.prim = kNoHit;
-    TravCount<false> cnt{nullptr, nullptr};
+    WaveCount<COUNT> cnt;
+    const uint32_t leafWait = ANY ? P.shadowLeafWait : P.leafWait;
     WavePool pool;
     pool.stripe = blockIdx.x % kPoolStripes;
     // Static assignment: lane i of the grid takes items i, i + G, i + 2G...
@@ -295,6 +350,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W == 8 ? (AN
                     sp = base = 0;
                     cur = 0u;
                     active = true;
+                    cnt.ray();
                 }
             }
         }
@@ -312,6 +368,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W == 8 ? (AN
         for (;;) {
             if (cur != kEmpty && !(cur & kLeafBit)) {
                 const float4* nd = node_at<W>(S, cur);
+                cnt.node(cur);
                 float tn[W];
                 uint32_t w[W];
                 const uint32_t mask = node_test<W>(nd, r, ANY ? tmax : best.t, tn, w);
@@ -325,7 +382,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W == 8 ? (AN
                 leaf = cur;
                 cur = ring_pop<ANY>(best.t, lane, ovNode, ovT, sp, base);
             }
-            if (__popcll(__ballot(leaf == kEmpty && cur != kEmpty)) <= kLeafWaitLanes) break;
+            if (__popcll(__ballot(leaf == kEmpty && cur != kEmpty)) <= leafWait) break;
         }
         // Leaf phase.
         if (leaf != kEmpty) {
@@ -334,6 +391,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W == 8 ? (AN
                 for (;;) {
                     const float4* tp = S.tris + static_cast<size_t>(slot) * 3u;
                     const float4 a = tp[0], b = tp[1], c = tp[2];
+                    cnt.tri(slot);
                     float t, b1, b2;
                     if (tri_test(r, xyz(a), xyz(b), xyz(c), tmax, &t, &b1, &b2)) {
                         occluded = true;
@@ -352,12 +410,14 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W == 8 ? (AN
         if (finished) {
             if (ANY) {
                 P.B.svis[item] = occluded ? 0u : 1u;
+                cnt.add(kWfCountOccluded, occluded);
             } else {
                 P.B.H[item] = make_float4(best.t, best.b1, best.b2, __uint_as_float(best.slot));
             }
             active = false;
         }
     }
+    cnt.flush(P.wfCount, ANY ? 2u : (depth == 0u ? 0u : 1u));
 }
 
 // Closest hit of the pass's camera rays as packets
@@ -365,9 +425,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W == 8 ? (AN
 // consecutive paths (path_coord: G samples of an 8/sqrt(G)-pixel square);
 // every lane tests each node the packet visits against its own ray and
 // keeps its own closest hit.
-template <int W>
+template <int W, bool COUNT>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W == 8 ? 6 : 8))) void k_wf_primary(WfParams P) {
     __shared__ uint32_t wstack[kStackSize];
+    WaveCount<COUNT> cnt;
     const uint32_t packets = P.sliceCount * P.tiles;
     for (uint32_t w = blockIdx.x; w < packets; w += gridDim.x) {
         const uint32_t p = w * 64u + threadIdx.x;
@@ -379,9 +440,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W == 8 ? 6 :
             ray.tmin = kRayEps;
             ray.tmax = kFltMax;
         }
-        const HitRec best = intersect_closest_packet<W>(P.scene, ray, valid, wstack);
+        if (valid) cnt.ray();
+        const HitRec best = intersect_closest_packet<W>(P.scene, ray, valid, wstack, cnt);
         if (valid) P.B.H[p] = make_float4(best.t, best.b1, best.b2, __uint_as_float(best.slot));
     }
+    cnt.flush(P.wfCount, 0u);
 }
 
 // Shading class of the hit of path p (kShadeMissClass for a miss).
@@ -761,17 +824,28 @@ __global__ __launch_bounds__(64) void k_wf_accum(WfParams P) {
     P.frame.depth[pix] = depth;
 }
 
-// Launches the traversal kernel for the fetch mode and tree width.
-template <bool ANY>
-void launch_trace(const WfParams& P, const WfLaunch& L, uint32_t d, uint32_t blocks, uint32_t tblocks, hipStream_t s) {
+// Launches the traversal kernel for the fetch mode and tree width (the
+// counting instantiation when P.wfCount is set).
+template <bool ANY, bool COUNT>
+void launch_trace_as(const WfParams& P, const WfLaunch& L, uint32_t d, uint32_t blocks, uint32_t tblocks,
+                     hipStream_t s) {
     const bool w8 = L.bvhWidth == 8u && P.scene.nodes8 != nullptr;
     switch (L.fetchMode * 2u + (w8 ? 1u : 0u)) {
-        case 5: hipLaunchKernelGGL((k_wf_trace<ANY, 2, 8>), dim3(tblocks), dim3(64), 0, s, P, d); break;
-        case 4: hipLaunchKernelGGL((k_wf_trace<ANY, 2, 4>), dim3(tblocks), dim3(64), 0, s, P, d); break;
-        case 3: hipLaunchKernelGGL((k_wf_trace<ANY, 1, 8>), dim3(blocks), dim3(64), 0, s, P, d); break;
-        case 2: hipLaunchKernelGGL((k_wf_trace<ANY, 1, 4>), dim3(blocks), dim3(64), 0, s, P, d); break;
-        case 1: hipLaunchKernelGGL((k_wf_trace<ANY, 0, 8>), dim3(blocks), dim3(64), 0, s, P, d); break;
-        default: hipLaunchKernelGGL((k_wf_trace<ANY, 0, 4>), dim3(blocks), dim3(64), 0, s, P, d); break;
+        case 5: hipLaunchKernelGGL((k_wf_trace<ANY, 2, 8, COUNT>), dim3(tblocks), dim3(64), 0, s, P, d); break;
+        case 4: hipLaunchKernelGGL((k_wf_trace<ANY, 2, 4, COUNT>), dim3(tblocks), dim3(64), 0, s, P, d); break;
+        case 3: hipLaunchKernelGGL((k_wf_trace<ANY, 1, 8, COUNT>), dim3(blocks), dim3(64), 0, s, P, d); break;
+        case 2: hipLaunchKernelGGL((k_wf_trace<ANY, 1, 4, COUNT>), dim3(blocks), dim3(64), 0, s, P, d); break;
+        case 1: hipLaunchKernelGGL((k_wf_trace<ANY, 0, 8, COUNT>), dim3(blocks), dim3(64), 0, s, P, d); break;
+        default: hipLaunchKernelGGL((k_wf_trace<ANY, 0, 4, COUNT>), dim3(blocks), dim3(64), 0, s, P, d); break;
+    }
+}
+
+template <bool ANY>
+void launch_trace(const WfParams& P, const WfLaunch& L, uint32_t d, uint32_t blocks, uint32_t tblocks, hipStream_t s) {
+    if (P.wfCount) {
+        launch_trace_as<ANY, true>(P, L, d, blocks, tblocks, s);
+    } else {
+        launch_trace_as<ANY, false>(P, L, d, blocks, tblocks, s);
     }
 }
 
@@ -803,10 +877,17 @@ hipError_t launch_wavefront_pass(const WfParams& P, const WfLaunch& L, hipStream
     const uint32_t tblocks = fullBlocks < L.traceBlocks ? fullBlocks : L.traceBlocks;
     for (uint32_t d = 0; d < P.settings.maxDepth; ++d) {
         if (d == 0u && L.primaryPackets) {
-            if (L.bvhWidth == 8u && P.scene.nodes8 != nullptr) {
-                hipLaunchKernelGGL(k_wf_primary<8>, dim3(blocks), dim3(64), 0, s, P);
+            const bool w8 = L.bvhWidth == 8u && P.scene.nodes8 != nullptr;
+            if (P.wfCount) {
+                if (w8) {
+                    hipLaunchKernelGGL((k_wf_primary<8, true>), dim3(blocks), dim3(64), 0, s, P);
+                } else {
+                    hipLaunchKernelGGL((k_wf_primary<4, true>), dim3(blocks), dim3(64), 0, s, P);
+                }
+            } else if (w8) {
+                hipLaunchKernelGGL((k_wf_primary<8, false>), dim3(blocks), dim3(64), 0, s, P);
             } else {
-                hipLaunchKernelGGL(k_wf_primary<4>, dim3(blocks), dim3(64), 0, s, P);
+                hipLaunchKernelGGL((k_wf_primary<4, false>), dim3(blocks), dim3(64), 0, s, P);
             }
             m.mark(SHIRO_HIP_STAGE_PRIMARY, s);
         } else {

@@ -88,6 +88,7 @@ def load_library(path: str = LIB_PATH):
     d("shiro_hip_env_sample", C.c_int, V, _u, _fp, _u, _fp, _fp, _fp)
     d("shiro_hip_env_eval", C.c_int, V, _u, _fp, _u, _fp, _fp)
     d("shiro_hip_count_fetches", C.c_int, V, _P(abi.Camera), _P(abi.Settings), _u, _u, _P(abi.FetchCounts))
+    d("shiro_hip_count_wavefront", C.c_int, V, _P(abi.Camera), _P(abi.Settings), _u, _u, _P(abi.WavefrontCounts))
     d("shiro_hip_set_tuning", C.c_int, V, _P(abi.Tuning))
     d("shiro_hip_get_tuning", C.c_int, V, _P(abi.Tuning))
     d("shiro_hip_autotune", C.c_int, V, _P(abi.Camera), _P(abi.Settings), _u, _P(abi.Tuning), C.c_char_p,
@@ -459,6 +460,17 @@ class Context:
         cam, st = camera.to_c(), settings.to_c()
         self._check(self.L.shiro_hip_count_fetches(self.h, C.byref(cam), C.byref(st), sample_start, sample_count,
                                                    C.byref(out)))
+        return out
+
+    def count_wavefront(self, camera, settings, sample_start=0, sample_count=None) -> abi.WavefrontCounts:
+        """Fetch counters of the wavefront traversal kernels (camera, bounce
+        and shadow rays) over one render of the samples with this context's
+        tuning."""
+        sample_count = settings.samplesPerPixel if sample_count is None else sample_count
+        out = abi.WavefrontCounts()
+        cam, st = camera.to_c(), settings.to_c()
+        self._check(self.L.shiro_hip_count_wavefront(self.h, C.byref(cam), C.byref(st), sample_start, sample_count,
+                                                     C.byref(out)))
         return out
 
     # ---- batched multi-scene rendering ----

@@ -38,6 +38,7 @@ def test_struct_layouts_match_ctypes(tmp_path):
                "shiro_hip_texture": abi.Texture, "shiro_hip_scene": abi.Scene, "shiro_hip_camera": abi.Camera,
                "shiro_hip_settings": abi.Settings, "shiro_hip_frame": abi.Frame, "shiro_hip_status": abi.Status,
                "shiro_hip_scene_info": abi.SceneInfo, "shiro_hip_fetch_counts": abi.FetchCounts,
+               "shiro_hip_wavefront_counts": abi.WavefrontCounts,
                "shiro_hip_tuning": abi.Tuning, "shiro_hip_stage_times": abi.StageTimes,
                "shiro_hip_batch_item": abi.BatchItem, "shiro_hip_image": abi.Image}
     lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HEADER}"', "int main(void) {"]

@@ -502,3 +502,70 @@ def test_autotune_keeps_result(hip_ctx):
     out = hip_ctx.render(wl.camera, st)
     hip_ctx.set_tuning(**{k: getattr(saved, k) for k, _ in abi.Tuning._fields_})
     assert np.array_equal(out.beauty, ref.beauty)
+
+
+@pytest.mark.parametrize("make", [c2_mini, c3_mini, emissive_mini], ids=["c2", "c3", "emissive"])
+def test_traversal_knobs_identical(hip_ctx, make):
+    """The leaf-wait thresholds of the closest-hit and any-hit traversal
+    change when lanes test their leaves, not which triangles a ray meets:
+    the frames equal the megakernel's bit for bit, over both tree widths and
+    every fetch mode."""
+    from shiro_amd import abi
+    wl = make()
+    _log(f"traversal knobs {wl.name}")
+    hip_ctx.upload(wl.scene)
+    mk = hip_ctx.render(wl.camera, wl.settings)
+    st = RenderSettings(**{**wl.settings.__dict__, "transport": abi.TRANSPORT_WAVEFRONT})
+    saved = hip_ctx.get_tuning()
+    try:
+        for width in (4, 8):
+            for mode in (0, 2):
+                for wait, swait in ((0, 0), (24, 24), (64, 64), (40, 8), (8, 40)):
+                    hip_ctx.set_tuning(bvhWidth=width, fetchMode=mode, leafWaitLanes=wait,
+                                       shadowLeafWaitLanes=swait, pathsPerPass=1 << 16)
+                    wf = hip_ctx.render(wl.camera, st)
+                    for k in ("beauty", "albedo", "normal", "depth"):
+                        assert np.array_equal(getattr(wf, k), getattr(mk, k)), (width, mode, wait, swait, k)
+    finally:
+        hip_ctx.set_tuning(**{k: getattr(saved, k) for k, _ in abi.Tuning._fields_})
+
+
+@pytest.mark.parametrize("make", [c2_mini, c3_mini], ids=["c2", "c3"])
+def test_wavefront_fetch_counts(hip_ctx, make):
+    """The counting instantiations of the wavefront traversal kernels trace
+    the rays the megakernel traces (same camera, bounce and shadow rays);
+    with leaf tests as soon as a lane reaches a leaf the any-hit traversal
+    visits exactly the megakernel's nodes and triangles and finds the same
+    occluded rays at any leaf-wait threshold; a camera packet fetches each
+    node once for its rays."""
+    from shiro_amd import abi
+    wl = make()
+    hip_ctx.upload(wl.scene)
+    st = RenderSettings(**{**wl.settings.__dict__, "transport": abi.TRANSPORT_WAVEFRONT})
+    spp = st.samplesPerPixel
+    mkc = hip_ctx.count_fetches(wl.camera, wl.settings, 0, spp)
+    saved = hip_ctx.get_tuning()
+    try:
+        counts = {}
+        for swait in (64, 24):
+            hip_ctx.set_tuning(shadowLeafWaitLanes=swait, bvhWidth=4, primaryPackets=16, fetchMode=2)
+            counts[swait] = hip_ctx.count_wavefront(wl.camera, st, 0, spp)
+    finally:
+        hip_ctx.set_tuning(**{k: getattr(saved, k) for k, _ in abi.Tuning._fields_})
+    w0, w1 = counts[64], counts[24]
+    _log(f"{wl.name}: rays {list(w0.rays)} vs megakernel {mkc.primaryRays} {mkc.rays - mkc.primaryRays} "
+         f"{mkc.shadowRays}; shadow nodes {w0.nodeFetches[2]} vs {mkc.shadowNodeFetches} (leaf wait 24: "
+         f"{w1.nodeFetches[2]}); occluded {w0.shadowOccluded}/{w1.shadowOccluded}; camera nodes per ray "
+         f"{w0.nodeFetches[0]} per packet {w0.waveNodeFetches[0]}")
+    assert w0.samples == st.width * st.height * spp
+    assert w0.rays[0] == mkc.primaryRays == w0.samples
+    assert w0.rays[1] == mkc.rays - mkc.primaryRays
+    assert w0.rays[2] == mkc.shadowRays and w1.rays[2] == w0.rays[2]
+    assert w0.nodeFetches[2] == mkc.shadowNodeFetches
+    assert w0.triFetches[2] == mkc.shadowTriFetches
+    assert w1.shadowOccluded == w0.shadowOccluded
+    for w in (w0, w1):
+        for k in range(3):
+            assert w.waveNodeFetches[k] <= w.nodeFetches[k] and w.waveTriFetches[k] <= w.triFetches[k]
+    # Packets of 16 samples of 2x2 pixels: each node once per 64 rays.
+    assert w0.waveNodeFetches[0] * 8 < w0.nodeFetches[0]
