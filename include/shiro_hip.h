@@ -696,8 +696,10 @@ int shiro_hip_smooth_normals(const float* positions, uint32_t vertexCount, const
  * gets from OpenImageIO (reference src/render/EnvironmentMap.cpp:49-92),
  * for hosts built without it.  Formats: PFM ("PF"/"Pf", either byte order),
  * Radiance RGBE (flat or run-length scanlines, any [+-]Y [+-]X orientation)
- * and single-part scanline OpenEXR (NONE/RLE/ZIPS/ZIP; HALF/FLOAT/UINT; R,G,B
- * or Y channels).  `pixels` receives width*height RGB floats, row 0 the top
+ * and OpenEXR (scanline or tiled, single- or multi-part: the first part,
+ * level 0 of mipmapped or ripmapped tiles; NONE/RLE/ZIPS/ZIP/PIZ/PXR24/
+ * B44/B44A; HALF/FLOAT/UINT; R,G,B or Y channels; deep files and DWAA/DWAB
+ * are rejected).  `pixels` receives width*height RGB floats, row 0 the top
  * of the image, with Load's channel expansion (g = r and b = g where the
  * file lacks them): the layout shiro_hip_dome_light.pixels takes.  Needs no
  * device. ---- */
