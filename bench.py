@@ -411,6 +411,76 @@ def summarize_valu(args):
     print(json.dumps({"valu_ceiling_per_s": ceiling, "kernels": out}, indent=1))
 
 
+def _profile_child(out_dir, counters, child_args, seconds=300):
+    """Runs this script under `rocprofv3 --pmc <counters>` as a child process
+    (csv output in out_dir) and returns the child's last JSON stdout line."""
+    import subprocess
+    cmd = ["timeout", "-s", "KILL", str(seconds), "rocprofv3", "--pmc", *counters, "--output-format", "csv",
+           "-d", out_dir, "-o", "run", "--", sys.executable, os.path.join(ROOT, "bench.py"), *child_args]
+    r = subprocess.run(cmd, cwd=ROOT, capture_output=True, text=True, env={**os.environ, "TMPDIR": "/tmp"})
+    if r.returncode != 0:
+        raise RuntimeError(f"rocprofv3 {' '.join(counters)}: exit {r.returncode}: {r.stderr[-400:]}")
+    lines = [l for l in r.stdout.splitlines() if l.startswith("{")]
+    return json.loads(lines[-1]) if lines else None
+
+
+def measure_counters(args, tuning, kernel, probe_ms):
+    """Counter measurements of this run for `kernel` (a kernel symbol of
+    the workload): HBM bytes per dispatch (FETCH_SIZE and WRITE_SIZE child
+    runs of one frame with the same tuning, scaled by what the same counters
+    read for a 1 GiB device copy) and VALU wave-instructions per dispatch
+    (SQ counters) against those of the FMA issue-rate probe.  probe_ms(n)
+    times the probe of n iterations outside the profiler (a profiled
+    dispatch's events include the counter set-up)."""
+    import shutil
+    import tempfile
+    work = tempfile.mkdtemp(prefix="shiro_prof_", dir="/tmp")
+    frame = ["--workload", args.workload, "--transport", args.transport, "--no-cpu", "--no-count", "--no-parity",
+             "--no-traffic", "--no-e2e", "--steps", "1", "--warmup", "0", "--tuning-json", json.dumps(tuning)]
+    copy_mib = 1024
+    out = {}
+    try:
+        for name, counters, child in (("fetch", ["FETCH_SIZE"], frame), ("write", ["WRITE_SIZE"], frame),
+                                      ("copyf", ["FETCH_SIZE"], ["--copy-probe", str(copy_mib)]),
+                                      ("copyw", ["WRITE_SIZE"], ["--copy-probe", str(copy_mib)])):
+            d = os.path.join(work, name)
+            _profile_child(d, counters, child)
+            out[name] = counter_table(d)
+            print(f"# counters {name}: {len(out[name])} kernels", file=sys.stderr, flush=True)
+        copy_bytes = copy_mib * (1 << 20)
+        copy_kernel = max((k for k in out["copyf"] if k in out["copyw"]),
+                          key=lambda k: out["copyf"][k]["FETCH_SIZE"])
+        fscale = copy_bytes / (out["copyf"][copy_kernel]["FETCH_SIZE"] * 1024.0)
+        wscale = copy_bytes / (out["copyw"][copy_kernel]["WRITE_SIZE"] * 1024.0)
+        res = {"fetch_scale": round(fscale, 4), "write_scale": round(wscale, 4)}
+        f, w = out["fetch"].get(kernel), out["write"].get(kernel)
+        if f and w:
+            res["hbm_bytes_per_dispatch"] = int(f["FETCH_SIZE"] * 1024.0 * fscale + w["WRITE_SIZE"] * 1024.0 * wscale)
+            res["dispatches_profiled"] = int(f["_n"])
+        sq = ["SQ_INSTS_VALU", "SQ_ACTIVE_INST_VALU", "SQ_THREAD_CYCLES_VALU", "SQ_WAVE_CYCLES", "SQ_WAIT_ANY",
+              "SQ_WAIT_INST_ANY"]
+        d = os.path.join(work, "sq")
+        _profile_child(d, sq, frame)
+        k = counter_table(d).get(kernel)
+        print("# counters sq", file=sys.stderr, flush=True)
+        d = os.path.join(work, "probe")
+        _profile_child(d, ["SQ_INSTS_VALU"], ["--valu-probe", "100000"])
+        # The probe's timed dispatch is its larger one (a short warm-up runs
+        # first).
+        pn = max((float(r["Counter_Value"]) for r in _csv_rows(d, "counter_collection.csv")
+                  if r["Counter_Name"] == "SQ_INSTS_VALU"
+                  and _kernel_symbol(r["Kernel_Name"]).startswith("k_valu_probe")), default=0.0)
+        if k and pn > 0:
+            res["valu_wave_instructions_per_dispatch"] = k["SQ_INSTS_VALU"]
+            res["valu_ceiling_per_s"] = pn / (min(probe_ms(100000) for _ in range(3)) * 1e-3)
+            res["lane_utilisation"] = round(k["SQ_THREAD_CYCLES_VALU"] / max(64.0 * k["SQ_ACTIVE_INST_VALU"], 1.0), 3)
+            res["wait_share"] = round(k["SQ_WAIT_ANY"] / max(k["SQ_WAVE_CYCLES"], 1.0), 3)
+            res["issue_wait_share"] = round(k["SQ_WAIT_INST_ANY"] / max(k["SQ_WAVE_CYCLES"], 1.0), 3)
+        return res
+    finally:
+        shutil.rmtree(work, ignore_errors=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -425,6 +495,10 @@ def main():
     ap.add_argument("--no-count", action="store_true")
     ap.add_argument("--no-parity", action="store_true", help="skip the other transport's comparison frame")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default wavefront tuning")
+    ap.add_argument("--no-traffic", action="store_true",
+                    help="skip the rocprofv3 child runs that measure the dominant kernel's HBM bytes and VALU issue")
+    ap.add_argument("--no-e2e", action="store_true", help="skip the host-buffer render with readback")
+    ap.add_argument("--tuning-json", default=None, help="set the wavefront tuning (JSON of its fields), no autotune")
     ap.add_argument("--fetch-mode", type=int, default=None, help="force the traversal fetch mode (0, 1, 2)")
     ap.add_argument("--paths-per-pass", type=int, default=None, help="force the wavefront paths per pass")
     ap.add_argument("--primary-packets", type=int, default=None, choices=[0, 1, 4, 16, 64],
@@ -500,7 +574,9 @@ def main():
     npx = st.width * st.height
     tuning = None
     if args.transport == "wavefront":
-        if args.fetch_mode is not None or args.paths_per_pass is not None:
+        if args.tuning_json:
+            ctx.set_tuning(**json.loads(args.tuning_json))
+        elif args.fetch_mode is not None or args.paths_per_pass is not None:
             kw = {}
             if args.fetch_mode is not None:
                 kw["fetchMode"] = args.fetch_mode
@@ -528,7 +604,8 @@ def main():
         tuning = {"pathsPerPass": t.pathsPerPass, "fetchMode": t.fetchMode, "traceBlocks": t.traceBlocks,
                   "stageBlocks": t.stageBlocks, "shadeOrder": t.shadeOrder, "primaryPackets": t.primaryPackets,
                   "bvhWidth": t.bvhWidth if info.bvh8NodeCount else 4,
-                  "autotuned": args.fetch_mode is None and args.paths_per_pass is None and not args.no_autotune}
+                  "autotuned": (args.fetch_mode is None and args.paths_per_pass is None and not args.no_autotune
+                                and not args.tuning_json)}
     dev = torch.device("cuda:0")
     bufs = [torch.empty(npx * k, dtype=torch.float32, device=dev) for k in (4, 3, 3, 1)]
     ptrs = [b.data_ptr() for b in bufs]
@@ -666,12 +743,34 @@ def main():
             dom.update(bytes_per_sample=round(credit, 1), bytes_per_launch=int(bpl),
                        achieved=round(bpl / (dom["avg_launch_ms"] * 1e-3) / 1e9, 2))
             dom["frac"] = round(dom["achieved"] / HBM_PEAK_GBS, 5)
-        # Measured HBM bytes per launch of the dominant symbol, from the
-        # committed rocprofv3 counter summary of this workload (FETCH_SIZE and
-        # WRITE_SIZE runs scaled by a device copy of known size).
-        traffic, traffic_src = None, None
+        # Measured HBM bytes and VALU issue of the dominant symbol: counted in
+        # this run by rocprofv3 child runs of one frame with the same tuning
+        # (FETCH_SIZE and WRITE_SIZE scaled by a device copy of known size;
+        # SQ counters against the FMA probe's issue rate); the committed
+        # counter summaries of this workload only if that fails.
+        traffic, traffic_src, valu = None, None, None
+        measured = None
+        if not args.no_traffic and args.transport == "wavefront":
+            try:
+                measured = measure_counters(args, {k: getattr(ctx.get_tuning(), k) for k, _ in abi.Tuning._fields_},
+                                            dom["kernel"], lambda n: ctx.valu_probe(n)[0])
+            except Exception as exc:  # noqa: BLE001 - reported in the line, the committed summary is used
+                print(f"# counter runs failed: {exc}", file=sys.stderr, flush=True)
+                measured = {"error": str(exc)[:300]}
+        if measured and "hbm_bytes_per_dispatch" in measured:
+            traffic = measured["hbm_bytes_per_dispatch"]
+            traffic_src = (f"measured in this run: rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of one frame "
+                           f"({measured['dispatches_profiled']} dispatches), x{measured['fetch_scale']} / "
+                           f"x{measured['write_scale']} from a 1 GiB device copy")
+        if measured and "valu_wave_instructions_per_dispatch" in measured:
+            rate = measured["valu_wave_instructions_per_dispatch"] / (dom["avg_launch_ms"] * 1e-3)
+            valu = {"issue_frac": round(rate / measured["valu_ceiling_per_s"], 4),
+                    "lane_utilisation": measured["lane_utilisation"], "wait_share": measured["wait_share"],
+                    "issue_wait_share": measured["issue_wait_share"],
+                    "source": "measured in this run: rocprofv3 --pmc SQ_INSTS_VALU ... of one frame, against "
+                              "the SQ_INSTS_VALU rate of k_valu_probe"}
         tpath = os.path.join(ROOT, "profiles", "hbm_traffic.json")
-        if os.path.exists(tpath):
+        if traffic is None and os.path.exists(tpath):
             rec = json.load(open(tpath)).get(args.workload, {})
             k = rec.get("kernels", {}).get(dom["kernel"])
             if k:
@@ -679,9 +778,8 @@ def main():
                 traffic_src = f"{rec['source']} (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, commit {rec['commit']})"
         # VALU issue rate of the dominant symbol against the FMA probe's
         # measured ceiling, from the committed rocprofv3 SQ-counter summary.
-        valu = None
         vpath = os.path.join(ROOT, "profiles", "valu_issue.json")
-        if os.path.exists(vpath):
+        if valu is None and os.path.exists(vpath):
             rec = json.load(open(vpath)).get(args.workload, {})
             k = rec.get("kernels", {}).get(dom["kernel"])
             if k:
@@ -704,7 +802,7 @@ def main():
                                    "shadow_node_fetches": c.shadowNodeFetches / ns,
                                    "shadow_tri_tests": c.shadowTriFetches / ns,
                                    "env_taps": c.envTaps / ns, "cdf_steps": c.cdfSteps / ns},
-                    "wavefront_counts": wf}
+                    "wavefront_counts": wf, "counter_runs": measured}
         if args.transport == "wavefront":
             roofline["note"] = ("achieved: bytes the traversal kernel's load instructions request (each node "
                                 "and triangle once per wave instruction that reads it, as the lanes of one "
@@ -741,10 +839,12 @@ def main():
                           speedup_vs_megakernel_plus=round(plus_ms / (sum(gpu_ms) / len(gpu_ms)), 3),
                           megakernel_plus_identical=all(bool(torch.equal(a, b)) for a, b in zip(bufs, obufs)))
         del obufs
-    t0 = time.perf_counter()
-    host = ctx.render(wl.camera, st)
-    e2e_ms = (time.perf_counter() - t0) * 1e3
-    del host
+    e2e_ms = None
+    if not args.no_e2e:
+        t0 = time.perf_counter()
+        host = ctx.render(wl.camera, st)
+        e2e_ms = round((time.perf_counter() - t0) * 1e3, 2)
+        del host
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(wl, args.cpu_seconds, args.cpu_threads)
@@ -759,7 +859,7 @@ def main():
                    "bvh": f"BVH4 {info.bvhNodeCount} nodes, depth {info.bvhDepth}; 8-wide {info.bvh8NodeCount} nodes, "
                           f"depth {info.bvh8Depth}",
                    "scene_upload_s": round(upload_s, 2), "finite_output": ok,
-                   "end_to_end_ms_with_readback": round(e2e_ms, 2), "parity": parity},
+                   "end_to_end_ms_with_readback": e2e_ms, "parity": parity},
         "roofline": roofline, "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

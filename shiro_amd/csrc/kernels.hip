@@ -599,11 +599,23 @@ __global__ void k_env_rows(const float* rowW, uint32_t h, float* rowCdf) {
     for (uint32_t y = 1; y <= h; ++y) rowCdf[y] /= total;
 }
 
-// build_cdf_guide of cdf[begin .. end] into out[0 .. G].
-DEV void cdf_guide(const float* cdf, uint32_t begin, uint32_t end, uint32_t G, uint32_t* out) {
-    bool monotone = true;
-    for (uint32_t i = begin; i < end && monotone; ++i) monotone = cdf[i] <= cdf[i + 1u];
-    for (uint32_t g = 0; g <= G; ++g) {
+// build_cdf_guide on the device: block y < h the guide of row y's
+// conditional CDF, block h the row CDF's; the block's threads check the
+// table's monotonicity together, then take one guide position each.
+__global__ __launch_bounds__(256) void k_env_guides(const float* rowCdf, const float* cond, uint32_t w, uint32_t h,
+                                                    uint32_t* rowGuide, uint32_t* condGuide) {
+    const uint32_t y = blockIdx.x;
+    const bool row = y == h;
+    const float* cdf = row ? rowCdf : cond;
+    const uint32_t begin = row ? 0u : y * (w + 1u);
+    const uint32_t end = row ? h : begin + w;
+    const uint32_t G = row ? kEnvRowGuide : kEnvColGuide;
+    uint32_t* out = row ? rowGuide : condGuide + static_cast<size_t>(y) * (kEnvColGuide + 1u);
+    int ok = 1;
+    for (uint32_t i = begin + threadIdx.x; i < end; i += blockDim.x)
+        if (!(cdf[i] <= cdf[i + 1u])) ok = 0;
+    const bool monotone = __syncthreads_and(ok) != 0;
+    for (uint32_t g = threadIdx.x; g <= G; g += blockDim.x) {
         if (!monotone) {
             out[g] = g == G ? end + 1u : begin + 1u;
             continue;
@@ -624,19 +636,6 @@ DEV void cdf_guide(const float* cdf, uint32_t begin, uint32_t end, uint32_t G, u
     }
 }
 
-// Threads 0..h-1: the guides of row y's conditional CDF; thread h: the row
-// CDF's guide.
-__global__ void k_env_guides(const float* rowCdf, const float* cond, uint32_t w, uint32_t h, uint32_t* rowGuide,
-                             uint32_t* condGuide) {
-    const uint32_t y = blockIdx.x * blockDim.x + threadIdx.x;
-    if (y < h) {
-        const uint32_t off = y * (w + 1u);
-        cdf_guide(cond, off, off + w, kEnvColGuide, condGuide + static_cast<size_t>(y) * (kEnvColGuide + 1u));
-    } else if (y == h) {
-        cdf_guide(rowCdf, 0u, h, kEnvRowGuide, rowGuide);
-    }
-}
-
 hipError_t launch_env_update(const EnvUpdate& U, hipStream_t s) {
     const uint32_t n = U.w * U.h;
     hipLaunchKernelGGL(k_env_texels, dim3(blocks_for(n, 256)), dim3(256), 0, s, U.rgb, U.texels, n);
@@ -644,8 +643,8 @@ hipError_t launch_env_update(const EnvUpdate& U, hipStream_t s) {
     hipLaunchKernelGGL(k_env_cond, dim3(blocks_for(U.h, 64)), dim3(64), 0, s, U.texels, U.w, U.h, U.rowSin, U.cond,
                        U.rowW);
     hipLaunchKernelGGL(k_env_rows, dim3(1), dim3(64), 0, s, U.rowW, U.h, U.rowCdf);
-    hipLaunchKernelGGL(k_env_guides, dim3(blocks_for(U.h + 1u, 64)), dim3(64), 0, s, U.rowCdf, U.cond, U.w, U.h,
-                       U.rowGuide, U.condGuide);
+    hipLaunchKernelGGL(k_env_guides, dim3(U.h + 1u), dim3(256), 0, s, U.rowCdf, U.cond, U.w, U.h, U.rowGuide,
+                       U.condGuide);
     return hipGetLastError();
 }
 

@@ -61,9 +61,10 @@ constexpr size_t kNodeFloats = 16;   // floats per BVH4 node (64 B)
 constexpr size_t kNode8Floats = 32;  // floats per 8-wide node (128 B)
 
 // Buckets of the environment CDF search guides (row table, each column
-// table).
-constexpr uint32_t kEnvRowGuide = 256;
-constexpr uint32_t kEnvColGuide = 64;
+// table): a sample's lower_bound search covers about three buckets' span
+// (c2's 2048x1024 map: 1-2 rows, 6 columns; 4 MB of column guides).
+constexpr uint32_t kEnvRowGuide = 4096;
+constexpr uint32_t kEnvColGuide = 1024;
 
 // Light counts a scene may hold (a bound on the per-path light samples of
 // the wavefront's shadow records, not a storage size: the lights live in
