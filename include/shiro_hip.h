@@ -580,13 +580,17 @@ typedef struct shiro_hip_tuning {
     uint32_t leafWaitLanes; /* closest-hit traversal: a wave turns to its leaf tests once at most
                                this many of its 64 lanes still search for a leaf (0-64, default 24) */
     uint32_t shadowLeafWaitLanes; /* the same for shadow rays (0-64, default 24) */
+    uint32_t tailDepth;     /* from this bounce on (1 or more; 0: never) the paths still alive are
+                               finished by one persistent kernel instead of a launch of each
+                               stage per bounce (frames unchanged; default 0) */
 } shiro_hip_tuning;
 int shiro_hip_set_tuning(shiro_hip_ctx* ctx, const shiro_hip_tuning* tuning);
 int shiro_hip_get_tuning(const shiro_hip_ctx* ctx, shiro_hip_tuning* tuning);
 
 /* Times candidate tunings on `trialSamples` samples of the given frame
  * (wavefront transport, uploaded scene): pass sizes, fetch modes and trace
- * grids, then the best of those with the other tree width.  Keeps the
+ * grids, then the best of those with other camera-packet sizes, the other
+ * tree width, and the tail kernel from bounce 3, 4, 5, 6 or 8.  Keeps the
  * fastest in the context and returns it; `report` (may be NULL) receives
  * one line per candidate.
  * Pass sizes above trialSamples x pixels time like the whole trial in one
@@ -652,7 +656,8 @@ enum {
     SHIRO_HIP_STAGE_ORDER = 7,      /* hit queue sorted by shading class */
     SHIRO_HIP_STAGE_PRIMARY = 8,    /* closest hit of the camera rays traced as packets (depth 0) */
     SHIRO_HIP_STAGE_SHADE_FIRST = 9, /* the first bounce's shading (its own kernel; SHADE: the later bounces) */
-    SHIRO_HIP_STAGE_COUNT = 10
+    SHIRO_HIP_STAGE_TAIL = 10,      /* the bounces from tuning.tailDepth on, in one persistent kernel */
+    SHIRO_HIP_STAGE_COUNT = 11
 };
 typedef struct shiro_hip_stage_times {
     float ms[SHIRO_HIP_STAGE_COUNT];          /* summed kernel time per stage */

@@ -146,7 +146,7 @@ class Tuning(C.Structure):
         ("pathsPerPass", C.c_uint32), ("fetchMode", C.c_uint32), ("traceBlocks", C.c_uint32),
         ("stageBlocks", C.c_uint32), ("batchBlocksPerCU", C.c_uint32), ("shadeOrder", C.c_uint32),
         ("primaryPackets", C.c_uint32), ("bvhWidth", C.c_uint32), ("manyBranches", C.c_uint32),
-        ("leafWaitLanes", C.c_uint32), ("shadowLeafWaitLanes", C.c_uint32)
+        ("leafWaitLanes", C.c_uint32), ("shadowLeafWaitLanes", C.c_uint32), ("tailDepth", C.c_uint32),
     ]
 
 
@@ -160,12 +160,13 @@ class BatchItem(C.Structure):
 MANY_PERSISTENT, MANY_GRAPH, MANY_SEQUENTIAL, MANY_WAVEFRONT, MANY_WAVEFRONT_GRAPH = 0, 1, 2, 3, 4
 
 
-STAGE_NAMES = ["reset", "extend", "shade", "shadow", "resolve", "accum", "megakernel", "order", "primary", "shade_first"]
+STAGE_NAMES = ["reset", "extend", "shade", "shadow", "resolve", "accum", "megakernel", "order", "primary", "shade_first",
+               "tail"]
 
 
 class StageTimes(C.Structure):
     _fields_ = [
-        ("ms", C.c_float * 10), ("launches", C.c_uint32 * 10),
+        ("ms", C.c_float * 11), ("launches", C.c_uint32 * 11),
         ("complete", C.c_uint32), ("reserved", C.c_uint32),
     ]
 

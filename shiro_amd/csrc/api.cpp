@@ -119,7 +119,7 @@ struct shiro_hip_ctx {
     DevBuf acB, acA, acN, acD, acC;
     uint32_t acW = 0, acH = 0, acTotal = 0;
     // Wavefront tuning (shiro_hip_set_tuning / shiro_hip_autotune).
-    shiro_hip_tuning tuning{128u << 20, 0u, 8192u, 16384u, 8u, 1u, 16u, 4u, 4u, 24u, 24u};
+    shiro_hip_tuning tuning{128u << 20, 0u, 8192u, 16384u, 8u, 1u, 16u, 4u, 4u, 24u, 24u, 0u};
     // Fetch counters of the wavefront traversal kernels
     // (shiro_hip_count_wavefront): on only for that call's render.
     DevBuf wfCount;
@@ -312,7 +312,8 @@ void wavefront_sizes(uint64_t cap, uint32_t M, bool order, uint64_t sumSlots, ui
 // Pass plan of one batch: S samples per pass over ranges of T tiles, and
 // the array sizes.
 struct WfPlan {
-    uint32_t tilesX = 0, tilesTotal = 0, M = 1, slots = 0, shadowRayBase = 0, shadowPathBase = 0, fetchBase = 0;
+    uint32_t tilesX = 0, tilesTotal = 0, M = 1, slots = 0, shadowRayBase = 0, shadowPathBase = 0, fetchBase = 0,
+             tailBase = 0;
     uint64_t S = 1, T = 1, cap = 0;
     bool order = false;
     size_t n[kWfArrays] = {};
@@ -336,6 +337,8 @@ int wavefront_plan(shiro_hip_ctx* ctx, const DevScene& scene, const shiro_hip_se
     // The striped work counters start on a 128-byte line.
     p.fetchBase = (p.shadowPathBase + st.maxDepth + kPoolStride - 1u) / kPoolStride * kPoolStride;
     p.slots = p.fetchBase + 2u * st.maxDepth * kPoolStripes * kPoolStride;
+    p.tailBase = p.slots;  // the tail kernel's path counter, on a line of its own
+    p.slots += kPoolStride;
     const uint64_t maxCap = (1ull << 32) / (2ull * p.M) - 1u;  // queue entries path * M + index stay below 2^32
     auto bytes_for = [&](uint64_t S, uint64_t T) {
         wavefront_sizes(S * T * 64u, p.M, p.order, S < count ? T * 64u : 0u, p.slots, p.n);
@@ -408,6 +411,7 @@ int enqueue_wavefront(shiro_hip_ctx* ctx, const DevScene& scene, const shiro_hip
     B.shadowRayBase = plan.shadowRayBase;
     B.shadowPathBase = plan.shadowPathBase;
     B.fetchBase = plan.fetchBase;
+    B.tailBase = plan.tailBase;
     B.maxShadow = plan.M;
     B.cap = static_cast<uint32_t>(plan.cap);
     B.shadeOrder = plan.order ? 1u : 0u;
@@ -429,6 +433,8 @@ int enqueue_wavefront(shiro_hip_ctx* ctx, const DevScene& scene, const shiro_hip
     L.fetchMode = ctx->tuning.fetchMode;
     L.primaryPackets = ctx->tuning.primaryPackets;
     L.bvhWidth = ctx->tuning.bvhWidth == 8u && scene.nodes8 ? 8u : 4u;
+    L.tailDepth = ctx->tuning.tailDepth;
+    L.tailBlocks = ctx->computeUnits * 16u;
     if (events) {
         HIPC(hipEventRecord(ctx->ev0, stream), "event");
         begin_marks(ctx);
@@ -1333,7 +1339,7 @@ int shiro_hip_set_tuning(shiro_hip_ctx* ctx, const shiro_hip_tuning* t) {
         t->batchBlocksPerCU == 0 || t->batchBlocksPerCU > 64u || t->shadeOrder > 1u ||
         (t->primaryPackets > 1u && t->primaryPackets != 4u && t->primaryPackets != 16u && t->primaryPackets != 64u) ||
         (t->bvhWidth != 4u && t->bvhWidth != 8u) || t->manyBranches == 0 || t->manyBranches > 16u ||
-        t->leafWaitLanes > 64u || t->shadowLeafWaitLanes > 64u)
+        t->leafWaitLanes > 64u || t->shadowLeafWaitLanes > 64u || t->tailDepth > 1024u)
         return fail(ctx, SHIRO_HIP_INVALID, "invalid tuning");
     ctx->tuning = *t;
     return SHIRO_HIP_OK;
@@ -1395,8 +1401,9 @@ int shiro_hip_autotune(shiro_hip_ctx* ctx, const shiro_hip_camera* camera, const
         const uint32_t width = t.bvhWidth == 8u && ctx->dev.nodes8 ? 8u : 4u;
         char line[192];
         std::snprintf(line, sizeof(line),
-                      "pathsPerPass %u fetchMode %u traceBlocks %u primaryPackets %u bvhWidth %u: %.3f ms\n",
-                      t.pathsPerPass, t.fetchMode, t.traceBlocks, t.primaryPackets, width, *ms);
+                      "pathsPerPass %u fetchMode %u traceBlocks %u primaryPackets %u bvhWidth %u tailDepth %u: "
+                      "%.3f ms\n",
+                      t.pathsPerPass, t.fetchMode, t.traceBlocks, t.primaryPackets, width, t.tailDepth, *ms);
         log += line;
         return 0;
     };
@@ -1437,6 +1444,23 @@ int shiro_hip_autotune(shiro_hip_ctx* ctx, const shiro_hip_camera* camera, const
         if (ms < bestMs) {
             bestMs = ms;
             bestT = t;
+        }
+    }
+    // Fourth stage: the bounce from which one persistent kernel finishes
+    // the paths still alive (the latency-bound tail of the pass).
+    {
+        const shiro_hip_tuning first = bestT;
+        for (uint32_t d : {3u, 4u, 5u, 6u, 8u}) {
+            if (d >= settings->maxDepth || d == first.tailDepth) continue;
+            shiro_hip_tuning t = first;
+            t.tailDepth = d;
+            float ms = 0.0f;
+            rc = timeCandidate(t, &ms);
+            if (rc) return rc;
+            if (ms < bestMs) {
+                bestMs = ms;
+                bestT = t;
+            }
         }
     }
     ctx->tuning = bestT;

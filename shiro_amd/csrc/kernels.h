@@ -41,6 +41,7 @@ struct WfBuffers {
     uint32_t fetchBase;                     // work fetched by the persistent traversal kernels: kPoolStripes
                                             // counters kPoolStride apart from counts[fetchBase + (k * maxDepth
                                             // + d) * kPoolStripes * kPoolStride], k = 0 closest, 1 any hit
+    uint32_t tailBase;                      // counts[tailBase]: paths taken by the tail kernel
     uint32_t maxShadow;
     uint32_t cap;
     uint8_t* qkey;                          // shading class per queue position
@@ -111,6 +112,8 @@ struct WfLaunch {
     uint32_t primaryPackets;  // samples per path group (1, 4, 16, 64; path_coord), 0: G = 1 and no
                               // camera-ray packets
     uint32_t bvhWidth;     // 4 or 8: tree of the traversal kernels (8 only if the scene has one)
+    uint32_t tailDepth;    // bounce from which k_wf_tail finishes the pass (0: never)
+    uint32_t tailBlocks;   // grid of k_wf_tail (one-wave blocks)
 };
 
 // Optional per-kernel timing: an event recorded after each launch, tagged

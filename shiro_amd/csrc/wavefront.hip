@@ -750,6 +750,99 @@ __global__ __launch_bounds__(64) void k_wf_resolve(WfParams P, uint32_t depth) {
     }
 }
 
+// The tail of a pass: from bounce depth0 on few paths are alive, and each
+// stage launch per bounce is bound by the latency of its slowest ray (c3:
+// 0.4-0.6 ms per bounce for depths 4-15, 6.4 ms of a 88 ms frame, for a
+// few percent of the rays).  k_wf_tail finishes those paths in one launch:
+// persistent one-wave blocks whose lanes take paths from the depth's ray
+// queue (one atomic per wave) and run trace_path's bounce body
+// (dev_path.h) on each until it ends -- closest hit and shadow rays on the
+// LDS ring stacks as in k_render_plus -- then store its radiance; a lane
+// whose path ended takes the next.  Every path runs the operations the
+// stage kernels would have run, in the same order, so frames are
+// unchanged.  Not used with the replay log on (the tail records no hits).
+template <bool TEX, bool PLAIN>
+__global__ __launch_bounds__(64) void k_wf_tail(WfParams P, uint32_t depth0) {
+    const DevSettings& st = P.settings;
+    const DevScene& S = P.scene;
+    const uint32_t lane = threadIdx.x;
+    const uint32_t n = P.B.counts[depth0];
+    const uint32_t* in = (depth0 & 1u) ? P.B.qb : P.B.qa;
+    uint32_t* counter = &P.B.counts[P.B.tailBase];
+    const uint32_t nLight = direct_sample_count(S, st);
+    uint32_t ovNode[kStackSize];
+    float ovT[kStackSize];
+    Counters<false> c;
+    bool active = false, exhausted = false;
+    uint32_t p = 0, depth = 0;
+    PathCoord pc{};
+    PathState ps;
+    for (;;) {
+        const bool want = !active && !exhausted;
+        const unsigned long long wm = __ballot(want);
+        if (wm) {
+            const int leader = __ffsll(static_cast<long long>(wm)) - 1;
+            uint32_t first = 0;
+            if (static_cast<int>(lane) == leader) first = atomicAdd(counter, static_cast<uint32_t>(__popcll(wm)));
+            first = __shfl(first, leader);
+            if (want) {
+                const uint32_t idx = first + static_cast<uint32_t>(__popcll(wm & ((1ull << lane) - 1ull)));
+                if (idx >= n) {
+                    exhausted = true;
+                } else {
+                    p = in[idx];
+                    pc = path_coord(P, p);
+                    ps = load_state(P.B, p);
+                    depth = depth0;
+                    active = true;
+                }
+            }
+        }
+        if (__all(exhausted)) break;
+        if (!active) continue;
+        // One bounce (trace_path's loop body).
+        const Sampler smp = path_sampler(P, pc);
+        bool ended = false;
+        const Ray ray = path_ray(ps);
+        const HitRec h = intersect_closest_ring(S, ray, lane, ovNode, ovT);
+        if (h.slot == kNoHit) {
+            handle_miss(S, st, depth, ps, c);
+            ended = true;
+        } else {
+            Surface sf;
+            TexState ts;
+            if (shade_hit<TEX>(S, depth, smp, ray, h, ps, sf, &ts)) {
+                const Mat& M = sf.M;
+                DirectSum ds;
+                ds.init(st);
+                const V3 origin = sf.p + sf.n * kRayEps;
+                for (uint32_t j = 0; j < nLight; ++j) {
+                    V3 ld, contrib;
+                    uint32_t group;
+                    float maxDist;
+                    if (!direct_sample<false, PLAIN>(S, st, sf, M, smp, depth, j, &ld, &contrib, &group, &maxDist, c))
+                        continue;
+                    Ray sr;
+                    sr.o = origin;
+                    sr.d = ld;
+                    sr.tmin = kRayEps;
+                    sr.tmax = maxDist;
+                    ds.add(group, contrib, !intersect_any_ring(S, sr, lane, ovNode));
+                }
+                ds.flush();
+                ps.L = ps.L + ps.T * ds.L;
+                if (!sample_bounce<PLAIN>(S, st, depth, smp, sf, ps)) ended = true;
+            }
+        }
+        ++depth;
+        if (depth >= st.maxDepth) ended = true;
+        if (ended) {
+            P.B.L[p] = make_float4(ps.L.x, ps.L.y, ps.L.z, 0.0f);
+            active = false;
+        }
+    }
+}
+
 // Per-pixel sums over the pass's samples in sample order, then the batch
 // means on the last pass of the pixel's tile (CpuPathTracer.cpp:2098-2137).
 // The running sums between passes are kept per pass tile (slot tile * 64 +
@@ -876,6 +969,18 @@ hipError_t launch_wavefront_pass(const WfParams& P, const WfLaunch& L, hipStream
     m.mark(SHIRO_HIP_STAGE_RESET, s);
     const uint32_t tblocks = fullBlocks < L.traceBlocks ? fullBlocks : L.traceBlocks;
     for (uint32_t d = 0; d < P.settings.maxDepth; ++d) {
+        if (d > 0u && d == L.tailDepth && !P.hitLog && !P.wfCount) {
+            // The rest of the pass in one persistent kernel.
+            const dim3 g(L.tailBlocks), b(64);
+            switch ((P.scene.textured ? 2u : 0u) + (P.scene.plainBsdf ? 1u : 0u)) {
+                case 3: hipLaunchKernelGGL((k_wf_tail<true, true>), g, b, 0, s, P, d); break;
+                case 2: hipLaunchKernelGGL((k_wf_tail<true, false>), g, b, 0, s, P, d); break;
+                case 1: hipLaunchKernelGGL((k_wf_tail<false, true>), g, b, 0, s, P, d); break;
+                default: hipLaunchKernelGGL((k_wf_tail<false, false>), g, b, 0, s, P, d); break;
+            }
+            m.mark(SHIRO_HIP_STAGE_TAIL, s);
+            break;
+        }
         if (d == 0u && L.primaryPackets) {
             const bool w8 = L.bvhWidth == 8u && P.scene.nodes8 != nullptr;
             if (P.wfCount) {

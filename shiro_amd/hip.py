@@ -445,7 +445,8 @@ class Context:
 
     def autotune(self, camera, settings, trial_samples=2):
         """Chooses paths per pass, traversal fetch mode and grid for this
-        frame, then tries the best with the other tree width; returns
+        frame, then tries the best with other camera-packet sizes, the other
+        tree width and the tail kernel from several bounces; returns
         (tuning, report)."""
         t = abi.Tuning()
         buf = C.create_string_buffer(8192)

@@ -79,12 +79,12 @@ def test_feature_transports_identical(hip_ctx, name):
     saved = hip_ctx.get_tuning()
     try:
         for width in (4, 8):
-            for order in (0, 1):
-                hip_ctx.set_tuning(bvhWidth=width, shadeOrder=order, pathsPerPass=1 << 16)
+            for order, tail in ((0, 0), (1, 0), (1, 1), (1, 2)):
+                hip_ctx.set_tuning(bvhWidth=width, shadeOrder=order, pathsPerPass=1 << 16, tailDepth=tail)
                 wf = hip_ctx.render(wl.camera, wst)
                 for k in ("beauty", "albedo", "normal", "depth"):
                     a, b = getattr(wf, k), getattr(mk, k)
-                    assert np.array_equal(a, b), (width, order, k, np.max(np.abs(a - b)))
+                    assert np.array_equal(a, b), (width, order, tail, k, np.max(np.abs(a - b)))
     finally:
         hip_ctx.set_tuning(**{k: getattr(saved, k) for k, _ in abi.Tuning._fields_})
 

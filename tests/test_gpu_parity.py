@@ -15,7 +15,7 @@ import os
 import numpy as np
 import pytest
 
-from helpers import angular_mini, c1_mini, c2_mini, c3_mini, emissive_mini, random_rays, relmse
+from helpers import angular_mini, c1_mini, c2_mini, c3_mini, c5_mini, emissive_mini, random_rays, relmse
 from oracle.bindings import Accumulator, OracleScene, oracle_fns
 from shiro_amd.scene import Camera, DirectionalLight, PbrMaterial, RenderSettings, Scene, TriangleMesh
 
@@ -569,3 +569,29 @@ def test_wavefront_fetch_counts(hip_ctx, make):
             assert w.waveNodeFetches[k] <= w.nodeFetches[k] and w.waveTriFetches[k] <= w.triFetches[k]
     # Packets of 16 samples of 2x2 pixels: each node once per 64 rays.
     assert w0.waveNodeFetches[0] * 8 < w0.nodeFetches[0]
+
+
+@pytest.mark.parametrize("make", [c1_mini, c2_mini, c3_mini, emissive_mini, angular_mini, c5_mini],
+                         ids=["c1", "c2", "c3", "emissive", "angular", "c5-textured"])
+def test_tail_kernel_identical(hip_ctx, make):
+    """Finishing a pass's paths from bounce d on in one persistent kernel
+    (tuning.tailDepth) runs each path's remaining bounces with the stage
+    kernels' operations in their order: frames equal the megakernel's bit
+    for bit, from every start depth, over both tree widths and with several
+    passes per frame."""
+    from shiro_amd import abi
+    wl = make()
+    _log(f"tail kernel {wl.name}")
+    hip_ctx.upload(wl.scene)
+    mk = hip_ctx.render(wl.camera, wl.settings)
+    st = RenderSettings(**{**wl.settings.__dict__, "transport": abi.TRANSPORT_WAVEFRONT})
+    saved = hip_ctx.get_tuning()
+    try:
+        for tail in range(1, st.maxDepth + 1):
+            for width, paths in ((4, 1 << 16), (8, 1 << 12)):
+                hip_ctx.set_tuning(tailDepth=tail, bvhWidth=width, pathsPerPass=paths, fetchMode=2)
+                wf = hip_ctx.render(wl.camera, st)
+                for k in ("beauty", "albedo", "normal", "depth"):
+                    assert np.array_equal(getattr(wf, k), getattr(mk, k)), (tail, width, k)
+    finally:
+        hip_ctx.set_tuning(**{k: getattr(saved, k) for k, _ in abi.Tuning._fields_})
