@@ -159,7 +159,8 @@ def bench_c4(args):
         "config": {"workload": f"c4: {len(wls)} scenes, {tris} triangles, {samples} samples per step",
                    "batch_mode": fastest, "batch_ms": modes,
                    "speedup_vs_sequential": round(modes["sequential"] / (sum(gpu_ms) / len(gpu_ms)), 3),
-                   "tuning": {"batchBlocksPerCU": best.batchBlocksPerCU, "manyBranches": best.manyBranches},
+                   "tuning": {"batchBlocksPerCU": best.batchBlocksPerCU, "manyBranches": best.manyBranches,
+                              "tailDepth": best.tailDepth},
                    "graph_captures": launcher.many_graph_stats()[0]},
         "roofline": None, "cpu_baseline": cpu,
     }
@@ -603,7 +604,7 @@ def main():
         t = ctx.get_tuning()
         tuning = {"pathsPerPass": t.pathsPerPass, "fetchMode": t.fetchMode, "traceBlocks": t.traceBlocks,
                   "stageBlocks": t.stageBlocks, "shadeOrder": t.shadeOrder, "primaryPackets": t.primaryPackets,
-                  "bvhWidth": t.bvhWidth if info.bvh8NodeCount else 4,
+                  "bvhWidth": t.bvhWidth if info.bvh8NodeCount else 4, "tailDepth": t.tailDepth,
                   "autotuned": (args.fetch_mode is None and args.paths_per_pass is None and not args.no_autotune
                                 and not args.tuning_json)}
     dev = torch.device("cuda:0")

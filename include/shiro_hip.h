@@ -635,10 +635,11 @@ int shiro_hip_render_many(
 int shiro_hip_many_graph_stats(const shiro_hip_ctx* ctx, uint32_t* captures, uint32_t* updates);
 /* Times the persistent mode for blocks-per-CU in {1, 2, 4, 8, 16, 32}, then
  * the graph-captured wavefront batch for manyBranches in {1, 2, 3, 4, 6, 8}
- * (a warm run and a timed run each; the warm run of a branch count captures
- * and instantiates its graph, the timed run replays it), keeps the fastest
- * of each in ctx's tuning and returns it; `report` (may be NULL) receives
- * one line per candidate. */
+ * and, with the best of those, for tailDepth in {0, 3, 4, 5, 6} (a warm run
+ * and a timed run each; the warm run of a setting captures and instantiates
+ * its graph, the timed run replays it), keeps the fastest of each in ctx's
+ * tuning and returns it; `report` (may be NULL) receives one line per
+ * candidate. */
 int shiro_hip_autotune_many(
     shiro_hip_ctx* ctx, const shiro_hip_batch_item* items, uint32_t count,
     shiro_hip_tuning* best, char* report, size_t reportSize);

@@ -2080,6 +2080,29 @@ int shiro_hip_autotune_many(shiro_hip_ctx* ctx, const shiro_hip_batch_item* item
             first = false;
         }
     }
+    // The tail kernel from a few bounces on, with the best branch count.
+    for (uint32_t d : {0u, 3u, 4u, 5u, 6u}) {
+        if (P.empty() || d == bestT.tailDepth) continue;
+        ctx->tuning = bestT;
+        ctx->tuning.tailDepth = d;
+        float ms = 0.0f;
+        for (int rep = 0; rep < 2; ++rep) {
+            rc = enqueue_many(ctx, P, tex, SHIRO_HIP_MANY_WAVEFRONT_GRAPH);
+            if (!rc) rc = finish_timing(ctx);
+            if (rc) {
+                ctx->tuning = base;
+                return rc;
+            }
+            ms = ctx->lastMs;
+        }
+        char line[96];
+        std::snprintf(line, sizeof(line), "wavefront graph, tailDepth %u: %.3f ms\n", d, ms);
+        log += line;
+        if (ms < bestGraph) {
+            bestGraph = ms;
+            bestT.tailDepth = d;
+        }
+    }
     ctx->tuning = bestT;
     if (best) *best = bestT;
     if (report && reportSize > 0) std::snprintf(report, reportSize, "%s", log.c_str());
