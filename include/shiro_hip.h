@@ -532,7 +532,8 @@ int shiro_hip_count_fetches(
  * counterpart in the reference; measurement): renders the samples once
  * through the wavefront with the context's tuning, running the counting
  * instantiations of the traversal kernels (same traversal, same work
- * distribution), and returns per kernel kind -- [0] camera rays
+ * distribution and tail kernel: the rays the tail kernel traces are not
+ * counted), and returns per kernel kind -- [0] camera rays
  * (k_wf_primary's packets, or the depth-0 k_wf_trace), [1] closest-hit rays
  * of the later bounces (k_wf_trace<false>), [2] shadow rays
  * (k_wf_trace<true>) -- the rays, the nodes and triangles read summed over

@@ -969,7 +969,7 @@ hipError_t launch_wavefront_pass(const WfParams& P, const WfLaunch& L, hipStream
     m.mark(SHIRO_HIP_STAGE_RESET, s);
     const uint32_t tblocks = fullBlocks < L.traceBlocks ? fullBlocks : L.traceBlocks;
     for (uint32_t d = 0; d < P.settings.maxDepth; ++d) {
-        if (d > 0u && d == L.tailDepth && !P.hitLog && !P.wfCount) {
+        if (d > 0u && d == L.tailDepth && !P.hitLog) {
             // The rest of the pass in one persistent kernel.
             const dim3 g(L.tailBlocks), b(64);
             switch ((P.scene.textured ? 2u : 0u) + (P.scene.plainBsdf ? 1u : 0u)) {

@@ -548,7 +548,7 @@ def test_wavefront_fetch_counts(hip_ctx, make):
     try:
         counts = {}
         for swait in (64, 24):
-            hip_ctx.set_tuning(shadowLeafWaitLanes=swait, bvhWidth=4, primaryPackets=16, fetchMode=2)
+            hip_ctx.set_tuning(shadowLeafWaitLanes=swait, bvhWidth=4, primaryPackets=16, fetchMode=2, tailDepth=0)
             counts[swait] = hip_ctx.count_wavefront(wl.camera, st, 0, spp)
     finally:
         hip_ctx.set_tuning(**{k: getattr(saved, k) for k, _ in abi.Tuning._fields_})
