@@ -28,8 +28,10 @@ constexpr uint32_t kPoolStride = 32;
 struct WfBuffers {
     float4 *o, *d, *T, *L, *E, *A, *N, *H;  // ray origin|prevPdf, dir|bits, throughput, radiance,
                                             // extinction, albedo|first-hit t, normal, hit record
-    float4 *so, *st;                        // shadow header: origin|count, throughput at the hit
-    float4* sr;                             // shadow rays: maxShadow x {dir|group, contribution}
+    float4 *so, *st;                        // shadow header: origin|count, throughput at the hit|count
+    float4* sr;                             // shadow rays (path * maxShadow + index): 2 x float4 each,
+                                            // direction|max distance (read by the any-hit kernel) and
+                                            // contribution|group (read by the resolve)
     uint32_t *qa, *qb;                      // ray queues (ping-pong)
     uint32_t* qs;                           // paths with shadow rays
     uint32_t* qsr;                          // shadow rays (path * maxShadow + index)

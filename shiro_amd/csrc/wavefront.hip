@@ -327,8 +327,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W == 8 ? (AN
                     const uint32_t rid = P.B.qsr[idx];
                     const uint32_t p = rid / P.B.maxShadow;
                     ray.o = xyz(P.B.so[p]);
-                    ray.d = xyz(P.B.sr[2u * static_cast<size_t>(rid)]);
-                    rayMax = P.B.sr[2u * static_cast<size_t>(rid) + 1u].w;
+                    const float4 dm = P.B.sr[2u * static_cast<size_t>(rid)];
+                    ray.d = xyz(dm);
+                    rayMax = dm.w;
                     item = rid;
                 } else if (depth == 0u) {
                     const PathCoord c = path_coord(P, idx);
@@ -679,7 +680,7 @@ __global__ __launch_bounds__(kShadeThreads) __attribute__((amdgpu_waves_per_eu(3
                     alive = true;
                 } else {
                     const Mat& M = sf.M;
-                    float4* sr = P.B.sr + static_cast<size_t>(p) * P.B.maxShadow * 2u;
+                    const size_t r0 = static_cast<size_t>(p) * P.B.maxShadow;
                     // nLight <= maxShadow by construction (enqueue_wavefront); the
                     // bound keeps the records inside the path's slots regardless.
                     for (uint32_t j = 0; j < nLight && ns < P.B.maxShadow; ++j) {
@@ -689,14 +690,14 @@ __global__ __launch_bounds__(kShadeThreads) __attribute__((amdgpu_waves_per_eu(3
                         if (!direct_sample<false, PLAIN>(P.scene, P.settings, sf, M, smp, depth, j, &ld, &contrib,
                                                          &group, &maxDist, c))
                             continue;
-                        sr[2u * ns] = make_float4(ld.x, ld.y, ld.z, __uint_as_float(group));
-                        sr[2u * ns + 1u] = make_float4(contrib.x, contrib.y, contrib.z, maxDist);
+                        P.B.sr[2u * (r0 + ns)] = make_float4(ld.x, ld.y, ld.z, maxDist);
+                        P.B.sr[2u * (r0 + ns) + 1u] = make_float4(contrib.x, contrib.y, contrib.z, __uint_as_float(group));
                         ++ns;
                     }
                     if (ns > 0u) {
                         const V3 origin = sf.p + sf.n * kRayEps;
                         P.B.so[p] = make_float4(origin.x, origin.y, origin.z, __uint_as_float(ns));
-                        P.B.st[p] = make_float4(ps.T.x, ps.T.y, ps.T.z, 0.0f);
+                        P.B.st[p] = make_float4(ps.T.x, ps.T.y, ps.T.z, __uint_as_float(ns));
                         shadow = true;
                     }
                     const uint32_t insideBefore = ps.bits & kBitsInside;
@@ -725,16 +726,17 @@ __global__ __launch_bounds__(64) void k_wf_resolve(WfParams P, uint32_t depth) {
     const uint32_t n = P.B.counts[P.B.shadowPathBase + depth];
     for (uint32_t i = blockIdx.x * 64u + threadIdx.x; i < n; i += gridDim.x * 64u) {
         const uint32_t p = P.B.qs[i];
-        const uint32_t ns = __float_as_uint(P.B.so[p].w);
-        const float4* sr = P.B.sr + static_cast<size_t>(p) * P.B.maxShadow * 2u;
+        const float4 tn = P.B.st[p];  // throughput at the hit | record count
+        const uint32_t ns = __float_as_uint(tn.w);
+        const float4* sc = P.B.sr + 2u * static_cast<size_t>(p) * P.B.maxShadow;
         const uint32_t* vis = P.B.svis + static_cast<size_t>(p) * P.B.maxShadow;
         DirectSum ds;
         ds.init(P.settings);
         uint32_t bits = 0;
         for (uint32_t j = 0; j < ns; ++j) {
-            const float4 a = sr[2u * j], b = sr[2u * j + 1u];
+            const float4 a = sc[2u * j + 1u];
             const bool visible = vis[j] != 0u;
-            ds.add(__float_as_uint(a.w), xyz(b), visible);
+            ds.add(__float_as_uint(a.w), xyz(a), visible);
             bits |= (visible ? 1u : 0u) << (j & 31u);
         }
         ds.flush();
@@ -745,7 +747,7 @@ __global__ __launch_bounds__(64) void k_wf_resolve(WfParams P, uint32_t depth) {
                                   depth)] = bits;
         }
         const float4 L = P.B.L[p];
-        const V3 Lv = xyz(L) + xyz(P.B.st[p]) * ds.L;
+        const V3 Lv = xyz(L) + xyz(tn) * ds.L;
         P.B.L[p] = make_float4(Lv.x, Lv.y, Lv.z, L.w);
     }
 }
