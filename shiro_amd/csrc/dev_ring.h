@@ -82,6 +82,32 @@ DEV uint32_t ring_push_sorted(uint32_t mask, const float* tn, const uint32_t* w,
     }
 }
 
+// push_in_order (dev_bvh.h) onto the ring stack: the hit children in node
+// order, the first returned, the others pushed last first, written to
+// their final ring slots after room is made (any-hit traversal keeps node
+// words only).  `mask` is non-zero.
+template <int W>
+DEV uint32_t ring_push_in_order(uint32_t mask, const uint32_t* w, uint32_t lane, uint32_t* ovNode, int& sp,
+                                int& base) {
+    const int m = __popc(mask) - 1;
+    while (sp + m - base > kRing) {
+        const uint32_t old = ring_slot(base, lane);
+        ovNode[base] = gRingNode[old];
+        ++base;
+    }
+    uint32_t cur = kEmpty;
+    int i = sp;
+#pragma unroll
+    for (int k = W - 1; k >= 0; --k) {
+        if ((mask >> k) & 1u) {
+            if (cur != kEmpty) gRingNode[ring_slot(i++, lane)] = cur;
+            cur = w[k];
+        }
+    }
+    sp += m;
+    return cur;
+}
+
 // pop_entry (dev_bvh.h) from the ring stack.  Entries are always read from
 // the ring: when it runs empty while older entries wait in the overflow
 // array, up to kRing of them move back first (rare).
@@ -148,7 +174,7 @@ DEV bool intersect_any_ring(const DevScene& S, const Ray& ray, uint32_t lane, ui
             float tn[4];
             uint32_t w[4];
             const uint32_t mask = node_boxes(S.nodes + static_cast<size_t>(cur) * 4u, r, ray.tmax, tn, w);
-            cur = mask ? ring_push_sorted<false, 4>(mask, tn, w, lane, ovNode, nullptr, sp, base)
+            cur = mask ? ring_push_in_order<4>(mask, w, lane, ovNode, sp, base)
                        : ring_pop<true>(0.0f, lane, ovNode, nullptr, sp, base);
         }
         if (cur == kEmpty) break;

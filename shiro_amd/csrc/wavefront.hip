@@ -374,7 +374,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W == 8 ? (AN
                 uint32_t w[W];
                 const uint32_t mask = node_test<W>(nd, r, ANY ? tmax : best.t, tn, w);
                 if (mask) {
-                    cur = ring_push_sorted<!ANY, W>(mask, tn, w, lane, ovNode, ovT, sp, base);
+                    cur = ANY ? ring_push_in_order<W>(mask, w, lane, ovNode, sp, base)
+                              : ring_push_sorted<true, W>(mask, tn, w, lane, ovNode, ovT, sp, base);
                 } else {
                     cur = ring_pop<ANY>(best.t, lane, ovNode, ovT, sp, base);
                 }
