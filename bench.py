@@ -40,6 +40,12 @@ def workload_desc(wl):
             f"{len(wl.scene.distantLights)} directional lights")
 
 
+def _tail_name(depth):
+    """tuning.tailDepth as reported: a bounce, 0 (never) or "auto"."""
+    from shiro_amd import abi
+    return "auto" if depth == abi.TAIL_AUTO else depth
+
+
 def host_cpu():
     """CPU model and logical CPU counts of this host (the CPU baseline's
     machine), from /proc/cpuinfo and the process's affinity mask."""
@@ -160,7 +166,7 @@ def bench_c4(args):
                    "batch_mode": fastest, "batch_ms": modes,
                    "speedup_vs_sequential": round(modes["sequential"] / (sum(gpu_ms) / len(gpu_ms)), 3),
                    "tuning": {"batchBlocksPerCU": best.batchBlocksPerCU, "manyBranches": best.manyBranches,
-                              "tailDepth": best.tailDepth},
+                              "tailDepth": _tail_name(best.tailDepth)},
                    "graph_captures": launcher.many_graph_stats()[0]},
         "roofline": None, "cpu_baseline": cpu,
     }
@@ -604,7 +610,7 @@ def main():
         t = ctx.get_tuning()
         tuning = {"pathsPerPass": t.pathsPerPass, "fetchMode": t.fetchMode, "traceBlocks": t.traceBlocks,
                   "stageBlocks": t.stageBlocks, "shadeOrder": t.shadeOrder, "primaryPackets": t.primaryPackets,
-                  "bvhWidth": t.bvhWidth if info.bvh8NodeCount else 4, "tailDepth": t.tailDepth,
+                  "bvhWidth": t.bvhWidth if info.bvh8NodeCount else 4, "tailDepth": _tail_name(t.tailDepth),
                   "autotuned": (args.fetch_mode is None and args.paths_per_pass is None and not args.no_autotune
                                 and not args.tuning_json)}
     dev = torch.device("cuda:0")

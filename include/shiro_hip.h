@@ -563,7 +563,7 @@ typedef struct shiro_hip_tuning {
                                state takes about (176 + 32 x light samples) B per
                                path, capped at 32 GiB) */
     uint32_t fetchMode;     /* traversal work distribution: 0 wave rounds, 1 lane refill,
-                               2 lane refill from per-wave pools of the queue (default 0) */
+                               2 lane refill from per-wave pools of the queue (default 2) */
     uint32_t traceBlocks;   /* grid of the fetchMode 2 traversal kernels (default 32 x CUs) */
     uint32_t stageBlocks;   /* grid cap of the grid-stride stage kernels (default 16384) */
     uint32_t batchBlocksPerCU; /* one-wave blocks per CU of the batched persistent kernel (default 8) */
@@ -583,8 +583,10 @@ typedef struct shiro_hip_tuning {
     uint32_t shadowLeafWaitLanes; /* the same for shadow rays (0-64, default 24) */
     uint32_t tailDepth;     /* from this bounce on (1 or more; 0: never) the paths still alive are
                                finished by one persistent kernel instead of a launch of each
-                               stage per bounce (frames unchanged; default 0) */
+                               stage per bounce (frames unchanged); SHIRO_HIP_TAIL_AUTO (the
+                               default): two bounces after roulette starts */
 } shiro_hip_tuning;
+#define SHIRO_HIP_TAIL_AUTO 0xffffffffu
 int shiro_hip_set_tuning(shiro_hip_ctx* ctx, const shiro_hip_tuning* tuning);
 int shiro_hip_get_tuning(const shiro_hip_ctx* ctx, shiro_hip_tuning* tuning);
 

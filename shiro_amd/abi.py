@@ -189,6 +189,7 @@ ENV_AUTOMATIC, ENV_LATLONG, ENV_ANGULAR = 0, 1, 2
 TEXELS_DEVICE, TEXELS_EVAL_ONLY = 1, 2
 PROGRESSIVE_RESET = 1
 TRANSPORT_MEGAKERNEL, TRANSPORT_WAVEFRONT, TRANSPORT_MEGAKERNEL_PLUS = 0, 1, 2
+TAIL_AUTO = 0xFFFFFFFF  # shiro_hip_tuning.tailDepth: two bounces after roulette starts
 
 # Every symbol include/shiro_hip.h declares (checked by the ABI test).
 EXPORTS = [

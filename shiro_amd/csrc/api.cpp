@@ -119,7 +119,10 @@ struct shiro_hip_ctx {
     DevBuf acB, acA, acN, acD, acC;
     uint32_t acW = 0, acH = 0, acTotal = 0;
     // Wavefront tuning (shiro_hip_set_tuning / shiro_hip_autotune).
-    shiro_hip_tuning tuning{128u << 20, 0u, 8192u, 16384u, 8u, 1u, 16u, 4u, 4u, 24u, 24u, 0u};
+    // Defaults: what the autotuner picks on the 4K frames of the large
+    // scenes (one pass of 128M paths, lanes refilled from per-wave pools,
+    // the tail kernel two bounces after roulette starts).
+    shiro_hip_tuning tuning{128u << 20, 2u, 8192u, 16384u, 8u, 1u, 16u, 4u, 4u, 24u, 24u, SHIRO_HIP_TAIL_AUTO};
     // Fetch counters of the wavefront traversal kernels
     // (shiro_hip_count_wavefront): on only for that call's render.
     DevBuf wfCount;
@@ -433,7 +436,7 @@ int enqueue_wavefront(shiro_hip_ctx* ctx, const DevScene& scene, const shiro_hip
     L.fetchMode = ctx->tuning.fetchMode;
     L.primaryPackets = ctx->tuning.primaryPackets;
     L.bvhWidth = ctx->tuning.bvhWidth == 8u && scene.nodes8 ? 8u : 4u;
-    L.tailDepth = ctx->tuning.tailDepth;
+    L.tailDepth = ctx->tuning.tailDepth == SHIRO_HIP_TAIL_AUTO ? st.russianRouletteDepth + 2u : ctx->tuning.tailDepth;
     L.tailBlocks = ctx->computeUnits * 16u;
     if (events) {
         HIPC(hipEventRecord(ctx->ev0, stream), "event");
@@ -1339,7 +1342,8 @@ int shiro_hip_set_tuning(shiro_hip_ctx* ctx, const shiro_hip_tuning* t) {
         t->batchBlocksPerCU == 0 || t->batchBlocksPerCU > 64u || t->shadeOrder > 1u ||
         (t->primaryPackets > 1u && t->primaryPackets != 4u && t->primaryPackets != 16u && t->primaryPackets != 64u) ||
         (t->bvhWidth != 4u && t->bvhWidth != 8u) || t->manyBranches == 0 || t->manyBranches > 16u ||
-        t->leafWaitLanes > 64u || t->shadowLeafWaitLanes > 64u || t->tailDepth > 1024u)
+        t->leafWaitLanes > 64u || t->shadowLeafWaitLanes > 64u ||
+        (t->tailDepth > 1024u && t->tailDepth != SHIRO_HIP_TAIL_AUTO))
         return fail(ctx, SHIRO_HIP_INVALID, "invalid tuning");
     ctx->tuning = *t;
     return SHIRO_HIP_OK;
@@ -1450,7 +1454,7 @@ int shiro_hip_autotune(shiro_hip_ctx* ctx, const shiro_hip_camera* camera, const
     // the paths still alive (the latency-bound tail of the pass).
     {
         const shiro_hip_tuning first = bestT;
-        for (uint32_t d : {3u, 4u, 5u, 6u, 8u}) {
+        for (uint32_t d : {0u, 3u, 4u, 5u, 6u, 8u}) {
             if (d >= settings->maxDepth || d == first.tailDepth) continue;
             shiro_hip_tuning t = first;
             t.tailDepth = d;
@@ -2081,7 +2085,7 @@ int shiro_hip_autotune_many(shiro_hip_ctx* ctx, const shiro_hip_batch_item* item
         }
     }
     // The tail kernel from a few bounces on, with the best branch count.
-    for (uint32_t d : {0u, 3u, 4u, 5u, 6u}) {
+    for (uint32_t d : {0u, 3u, 4u, 5u, 6u, SHIRO_HIP_TAIL_AUTO}) {
         if (P.empty() || d == bestT.tailDepth) continue;
         ctx->tuning = bestT;
         ctx->tuning.tailDepth = d;
