@@ -638,7 +638,7 @@ DEV void block_reserve3(const uint32_t c[3], uint32_t* count0, uint32_t* count1,
 // subsurface-wrap terms (DevScene::plainBsdf): the BSDF evaluation leaves
 // them out.
 template <bool TEX, bool FIRST, bool PLAIN>
-__global__ __launch_bounds__(kShadeThreads) __attribute__((amdgpu_waves_per_eu(3))) void k_wf_shade(WfParams P,
+__global__ __launch_bounds__(kShadeThreads) __attribute__((amdgpu_waves_per_eu(PLAIN && FIRST ? 4 : 3))) void k_wf_shade(WfParams P,
                                                                                                    uint32_t depthArg) {
     const uint32_t depth = FIRST ? 0u : depthArg;
     const uint32_t* in = depth > 0u && P.B.shadeOrder ? P.B.qsorted : ((depth & 1u) ? P.B.qb : P.B.qa);
