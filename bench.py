@@ -41,7 +41,7 @@ def workload_desc(wl):
 
 
 def _tail_name(depth):
-    """tuning.tailDepth as reported: a bounce, 0 (never) or "auto"."""
+    """tuning.tailDepth or tuning.fetchMode as reported: the value, or "auto"."""
     from shiro_amd import abi
     return "auto" if depth == abi.TAIL_AUTO else depth
 
@@ -608,7 +608,7 @@ def main():
         if args.bvh_width is not None:
             ctx.set_tuning(bvhWidth=args.bvh_width)
         t = ctx.get_tuning()
-        tuning = {"pathsPerPass": t.pathsPerPass, "fetchMode": t.fetchMode, "traceBlocks": t.traceBlocks,
+        tuning = {"pathsPerPass": t.pathsPerPass, "fetchMode": _tail_name(t.fetchMode), "traceBlocks": t.traceBlocks,
                   "stageBlocks": t.stageBlocks, "shadeOrder": t.shadeOrder, "primaryPackets": t.primaryPackets,
                   "bvhWidth": t.bvhWidth if info.bvh8NodeCount else 4, "tailDepth": _tail_name(t.tailDepth),
                   "autotuned": (args.fetch_mode is None and args.paths_per_pass is None and not args.no_autotune

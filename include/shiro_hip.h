@@ -563,7 +563,9 @@ typedef struct shiro_hip_tuning {
                                state takes about (176 + 32 x light samples) B per
                                path, capped at 32 GiB) */
     uint32_t fetchMode;     /* traversal work distribution: 0 wave rounds, 1 lane refill,
-                               2 lane refill from per-wave pools of the queue (default 2) */
+                               2 lane refill from per-wave pools of the queue;
+                               SHIRO_HIP_FETCH_AUTO (the default): 2 for passes of at least 8M
+                               paths, 0 below */
     uint32_t traceBlocks;   /* grid of the fetchMode 2 traversal kernels (default 32 x CUs) */
     uint32_t stageBlocks;   /* grid cap of the grid-stride stage kernels (default 16384) */
     uint32_t batchBlocksPerCU; /* one-wave blocks per CU of the batched persistent kernel (default 8) */
@@ -587,6 +589,7 @@ typedef struct shiro_hip_tuning {
                                default): two bounces after roulette starts */
 } shiro_hip_tuning;
 #define SHIRO_HIP_TAIL_AUTO 0xffffffffu
+#define SHIRO_HIP_FETCH_AUTO 0xffffffffu
 int shiro_hip_set_tuning(shiro_hip_ctx* ctx, const shiro_hip_tuning* tuning);
 int shiro_hip_get_tuning(const shiro_hip_ctx* ctx, shiro_hip_tuning* tuning);
 

@@ -190,6 +190,7 @@ TEXELS_DEVICE, TEXELS_EVAL_ONLY = 1, 2
 PROGRESSIVE_RESET = 1
 TRANSPORT_MEGAKERNEL, TRANSPORT_WAVEFRONT, TRANSPORT_MEGAKERNEL_PLUS = 0, 1, 2
 TAIL_AUTO = 0xFFFFFFFF  # shiro_hip_tuning.tailDepth: two bounces after roulette starts
+FETCH_AUTO = 0xFFFFFFFF  # shiro_hip_tuning.fetchMode: per-wave pools for passes of 8M paths or more
 
 # Every symbol include/shiro_hip.h declares (checked by the ABI test).
 EXPORTS = [

@@ -210,8 +210,11 @@ DEV V3 ratio(V3 a, V3 b) {
 // method described at the top of this file): adds its light-sample and
 // texel gradients, and its dome-map ones to `cache`.  MAXD bounds the path
 // length, sizing the vertex tape.  LOG: read hits and visibilities from the
-// replay log (P.hitLog, P.visLog) instead of tracing.
-template <uint32_t MAXD, bool LOG>
+// replay log (P.hitLog, P.visLog) instead of tracing.  PLAIN: the scene's
+// materials need no anisotropy, thin film, sheen or subsurface wrap
+// (DevScene::plainBsdf), and the BSDF code leaves those terms out, as the
+// wavefront's shade kernels do; the values are those of the general code.
+template <uint32_t MAXD, bool LOG, bool PLAIN>
 DEV void adjoint_path(const AdjointParams& P, GradCache& cache, uint32_t x, uint32_t y, uint32_t s, V3 a) {
     const DevSettings& st = P.settings;
     const DevScene& S = P.scene;
@@ -288,7 +291,8 @@ DEV void adjoint_path(const AdjointParams& P, GradCache& cache, uint32_t x, uint
             uint32_t group;
             float maxDist;
             DomeSampleInfo dome;
-            if (!direct_sample(S, st, sf, M, smp, depth, j, &ld, &contrib, &group, &maxDist, cnt, &f, &lf, &dome))
+            if (!direct_sample<false, PLAIN>(S, st, sf, M, smp, depth, j, &ld, &contrib, &group, &maxDist, cnt, &f,
+                                             &lf, &dome))
                 continue;
             bool visible;
             if constexpr (LOG) {
@@ -323,7 +327,7 @@ DEV void adjoint_path(const AdjointParams& P, GradCache& cache, uint32_t x, uint
         V.C = V.C + ds.L;
         if (textured) scatter_grad(P, cache, sf.material, sf.uv, gBase, gRough);
         BounceRec br;
-        const bool alive = sample_bounce(S, st, depth, smp, sf, ps, &br);
+        const bool alive = sample_bounce<PLAIN>(S, st, depth, smp, sf, ps, &br);
         if (!alive) {
             V.kind = kVtxEnd;
             break;
@@ -374,8 +378,9 @@ DEV void adjoint_path(const AdjointParams& P, GradCache& cache, uint32_t x, uint
 // Register caps (c5, measured): the tracing replay at 128 VGPRs (4 waves
 // per SIMD; it spills some registers there, and still runs faster than at
 // 2 or 3 waves); the replay from the log at 168 (3 waves: 2.16 ms, against
-// 2.35 at 4, 2.23 at 2 and 2.90 at 6).
-template <uint32_t MAXD, bool LOG>
+// 2.35 at 4, 2.23 at 2 and 2.90 at 6).  PLAIN (adjoint_path) is instantiated
+// for the replay from the log only.
+template <uint32_t MAXD, bool LOG, bool PLAIN>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(LOG ? 3 : 4))) void k_adjoint(AdjointParams P) {
     __shared__ GradCache cache;
     cache_clear(cache);
@@ -394,7 +399,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(LOG ? 3 : 4)
         const float4 db = P.dBeauty[static_cast<size_t>(y) * st.width + x];
         const float inv = 1.0f / static_cast<float>(P.sampleCount);
         const V3 a{db.x * inv, db.y * inv, db.z * inv};
-        if (a.x != 0.0f || a.y != 0.0f || a.z != 0.0f) adjoint_path<MAXD, LOG>(P, cache, x, y, s, a);
+        if (a.x != 0.0f || a.y != 0.0f || a.z != 0.0f) adjoint_path<MAXD, LOG, PLAIN>(P, cache, x, y, s, a);
     }
     cache_flush(cache, P);
 }
@@ -407,14 +412,18 @@ hipError_t launch_adjoint(const AdjointParams& P, hipStream_t s) {
     const uint32_t md = P.settings.maxDepth;
     if (md > kAdjointMaxDepth) return hipErrorInvalidValue;
     const bool log = P.hitLog != nullptr && P.visLog != nullptr;
-    const uint32_t v = (md <= 4u ? 0u : (md <= 8u ? 2u : 4u)) + (log ? 1u : 0u);
+    const bool plain = log && P.scene.plainBsdf != 0u;
+    const uint32_t v = (md <= 4u ? 0u : (md <= 8u ? 3u : 6u)) + (plain ? 2u : (log ? 1u : 0u));
     switch (v) {
-        case 0: hipLaunchKernelGGL((k_adjoint<4, false>), grid, dim3(64), 0, s, P); break;
-        case 1: hipLaunchKernelGGL((k_adjoint<4, true>), grid, dim3(64), 0, s, P); break;
-        case 2: hipLaunchKernelGGL((k_adjoint<8, false>), grid, dim3(64), 0, s, P); break;
-        case 3: hipLaunchKernelGGL((k_adjoint<8, true>), grid, dim3(64), 0, s, P); break;
-        case 4: hipLaunchKernelGGL((k_adjoint<kAdjointMaxDepth, false>), grid, dim3(64), 0, s, P); break;
-        default: hipLaunchKernelGGL((k_adjoint<kAdjointMaxDepth, true>), grid, dim3(64), 0, s, P); break;
+        case 0: hipLaunchKernelGGL((k_adjoint<4, false, false>), grid, dim3(64), 0, s, P); break;
+        case 1: hipLaunchKernelGGL((k_adjoint<4, true, false>), grid, dim3(64), 0, s, P); break;
+        case 2: hipLaunchKernelGGL((k_adjoint<4, true, true>), grid, dim3(64), 0, s, P); break;
+        case 3: hipLaunchKernelGGL((k_adjoint<8, false, false>), grid, dim3(64), 0, s, P); break;
+        case 4: hipLaunchKernelGGL((k_adjoint<8, true, false>), grid, dim3(64), 0, s, P); break;
+        case 5: hipLaunchKernelGGL((k_adjoint<8, true, true>), grid, dim3(64), 0, s, P); break;
+        case 6: hipLaunchKernelGGL((k_adjoint<kAdjointMaxDepth, false, false>), grid, dim3(64), 0, s, P); break;
+        case 7: hipLaunchKernelGGL((k_adjoint<kAdjointMaxDepth, true, false>), grid, dim3(64), 0, s, P); break;
+        default: hipLaunchKernelGGL((k_adjoint<kAdjointMaxDepth, true, true>), grid, dim3(64), 0, s, P); break;
     }
     return hipGetLastError();
 }

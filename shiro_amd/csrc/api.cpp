@@ -121,8 +121,10 @@ struct shiro_hip_ctx {
     // Wavefront tuning (shiro_hip_set_tuning / shiro_hip_autotune).
     // Defaults: what the autotuner picks on the 4K frames of the large
     // scenes (one pass of 128M paths, lanes refilled from per-wave pools,
-    // the tail kernel two bounces after roulette starts).
-    shiro_hip_tuning tuning{128u << 20, 2u, 8192u, 16384u, 8u, 1u, 16u, 4u, 4u, 24u, 24u, SHIRO_HIP_TAIL_AUTO};
+    // the tail kernel two bounces after roulette starts), with wave rounds
+    // for small passes.
+    shiro_hip_tuning tuning{128u << 20, SHIRO_HIP_FETCH_AUTO, 8192u, 16384u, 8u, 1u, 16u, 4u, 4u, 24u, 24u,
+                            SHIRO_HIP_TAIL_AUTO};
     // Fetch counters of the wavefront traversal kernels
     // (shiro_hip_count_wavefront): on only for that call's render.
     DevBuf wfCount;
@@ -433,7 +435,14 @@ int enqueue_wavefront(shiro_hip_ctx* ctx, const DevScene& scene, const shiro_hip
     WfLaunch L{};
     L.maxBlocks = ctx->tuning.stageBlocks;
     L.traceBlocks = ctx->tuning.traceBlocks;
-    L.fetchMode = ctx->tuning.fetchMode;
+    // Automatic work distribution: per-wave pools pay off on large passes
+    // (c2/c3, 133M paths: 75 -> 63 and 105 -> 83 ms per frame against wave
+    // rounds) and cost on small ones (c1, 4.2M paths: 1.53 -> 1.82 ms; c5,
+    // 2.1M: 1.98 -> 2.66 ms), where most waves get a chunk or two and then
+    // probe every stripe's counter.
+    const uint64_t passPaths = static_cast<uint64_t>(plan.S) * plan.T * 64u;
+    L.fetchMode = ctx->tuning.fetchMode == SHIRO_HIP_FETCH_AUTO ? (passPaths >= kFetchPoolPaths ? 2u : 0u)
+                                                                : ctx->tuning.fetchMode;
     L.primaryPackets = ctx->tuning.primaryPackets;
     L.bvhWidth = ctx->tuning.bvhWidth == 8u && scene.nodes8 ? 8u : 4u;
     L.tailDepth = ctx->tuning.tailDepth == SHIRO_HIP_TAIL_AUTO ? st.russianRouletteDepth + 2u : ctx->tuning.tailDepth;
@@ -1338,7 +1347,8 @@ int shiro_hip_env_eval(shiro_hip_ctx* ctx, uint32_t domeIndex, const float* dire
 int shiro_hip_set_tuning(shiro_hip_ctx* ctx, const shiro_hip_tuning* t) {
     if (!ctx || !t) return SHIRO_HIP_INVALID;
     std::lock_guard<std::mutex> lock(ctx->mu);
-    if (t->pathsPerPass == 0 || t->fetchMode > 2u || t->traceBlocks == 0 || t->stageBlocks == 0 ||
+    if (t->pathsPerPass == 0 || (t->fetchMode > 2u && t->fetchMode != SHIRO_HIP_FETCH_AUTO) || t->traceBlocks == 0 ||
+        t->stageBlocks == 0 ||
         t->batchBlocksPerCU == 0 || t->batchBlocksPerCU > 64u || t->shadeOrder > 1u ||
         (t->primaryPackets > 1u && t->primaryPackets != 4u && t->primaryPackets != 16u && t->primaryPackets != 64u) ||
         (t->bvhWidth != 4u && t->bvhWidth != 8u) || t->manyBranches == 0 || t->manyBranches > 16u ||

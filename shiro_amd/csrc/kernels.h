@@ -23,6 +23,9 @@ struct AccumParams {
 // line.
 constexpr uint32_t kPoolStripes = 16;
 constexpr uint32_t kPoolStride = 32;
+// Passes of at least this many paths take traversal work from per-wave pools
+// (fetch mode 2) under SHIRO_HIP_FETCH_AUTO, smaller ones in wave rounds.
+constexpr uint64_t kFetchPoolPaths = 8ull << 20;
 
 // Wavefront path state (structure of arrays, `cap` paths) and queues.
 struct WfBuffers {
