@@ -611,6 +611,7 @@ def main():
         tuning = {"pathsPerPass": t.pathsPerPass, "fetchMode": _tail_name(t.fetchMode), "traceBlocks": t.traceBlocks,
                   "stageBlocks": t.stageBlocks, "shadeOrder": t.shadeOrder, "primaryPackets": t.primaryPackets,
                   "bvhWidth": t.bvhWidth if info.bvh8NodeCount else 4, "tailDepth": _tail_name(t.tailDepth),
+                  "shadowOverlap": t.shadowOverlap,
                   "autotuned": (args.fetch_mode is None and args.paths_per_pass is None and not args.no_autotune
                                 and not args.tuning_json)}
     dev = torch.device("cuda:0")

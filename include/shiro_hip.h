@@ -587,6 +587,10 @@ typedef struct shiro_hip_tuning {
                                finished by one persistent kernel instead of a launch of each
                                stage per bounce (frames unchanged); SHIRO_HIP_TAIL_AUTO (the
                                default): two bounces after roulette starts */
+    uint32_t shadowOverlap; /* 1: a bounce's shadow rays and their resolve run on a second stream
+                               beside the next bounce's closest hits (frames unchanged; not under
+                               the caller's stream capture or in the entry point that allocates
+                               nothing); 0: every stage in turn on one stream (default 1) */
 } shiro_hip_tuning;
 #define SHIRO_HIP_TAIL_AUTO 0xffffffffu
 #define SHIRO_HIP_FETCH_AUTO 0xffffffffu

@@ -147,6 +147,7 @@ class Tuning(C.Structure):
         ("stageBlocks", C.c_uint32), ("batchBlocksPerCU", C.c_uint32), ("shadeOrder", C.c_uint32),
         ("primaryPackets", C.c_uint32), ("bvhWidth", C.c_uint32), ("manyBranches", C.c_uint32),
         ("leafWaitLanes", C.c_uint32), ("shadowLeafWaitLanes", C.c_uint32), ("tailDepth", C.c_uint32),
+        ("shadowOverlap", C.c_uint32),
     ]
 
 

@@ -90,6 +90,10 @@ struct shiro_hip_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // Second stream of the wavefront passes (WfLaunch::aux) and its
+    // fork / join events, created on first use.
+    hipStream_t auxStream = nullptr;
+    hipEvent_t auxFork = nullptr, auxJoin = nullptr;
     std::mutex mu;
     std::string err;
     float lastMs = 0.0f;
@@ -124,7 +128,7 @@ struct shiro_hip_ctx {
     // the tail kernel two bounces after roulette starts), with wave rounds
     // for small passes.
     shiro_hip_tuning tuning{128u << 20, SHIRO_HIP_FETCH_AUTO, 8192u, 16384u, 8u, 1u, 16u, 4u, 4u, 24u, 24u,
-                            SHIRO_HIP_TAIL_AUTO};
+                            SHIRO_HIP_TAIL_AUTO, 1u};
     // Fetch counters of the wavefront traversal kernels
     // (shiro_hip_count_wavefront): on only for that call's render.
     DevBuf wfCount;
@@ -447,6 +451,24 @@ int enqueue_wavefront(shiro_hip_ctx* ctx, const DevScene& scene, const shiro_hip
     L.bvhWidth = ctx->tuning.bvhWidth == 8u && scene.nodes8 ? 8u : 4u;
     L.tailDepth = ctx->tuning.tailDepth == SHIRO_HIP_TAIL_AUTO ? st.russianRouletteDepth + 2u : ctx->tuning.tailDepth;
     L.tailBlocks = ctx->computeUnits * 16u;
+    {
+        // Shadow rays beside the next bounce's closest hits, except under a
+        // caller's stream capture (the graph keeps the serial order) and in
+        // the entry point that allocates nothing (no stream is created
+        // there).
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        HIPC(hipStreamIsCapturing(stream, &cs), "capture status");
+        if (cs == hipStreamCaptureStatusNone && !scratch && ctx->tuning.shadowOverlap) {
+            if (!ctx->auxStream) {
+                HIPC(hipStreamCreateWithFlags(&ctx->auxStream, hipStreamNonBlocking), "aux stream");
+                HIPC(hipEventCreateWithFlags(&ctx->auxFork, hipEventDisableTiming), "aux event");
+                HIPC(hipEventCreateWithFlags(&ctx->auxJoin, hipEventDisableTiming), "aux event");
+            }
+            L.aux = ctx->auxStream;
+            L.fork = ctx->auxFork;
+            L.join = ctx->auxJoin;
+        }
+    }
     if (events) {
         HIPC(hipEventRecord(ctx->ev0, stream), "event");
         begin_marks(ctx);
@@ -665,6 +687,12 @@ void shiro_hip_destroy(shiro_hip_ctx* ctx) {
     }
     for (hipEvent_t e : ctx->joinEv) (void)hipEventDestroy(e);
     if (ctx->forkEv) (void)hipEventDestroy(ctx->forkEv);
+    if (ctx->auxStream) {
+        (void)hipStreamSynchronize(ctx->auxStream);
+        (void)hipStreamDestroy(ctx->auxStream);
+    }
+    if (ctx->auxFork) (void)hipEventDestroy(ctx->auxFork);
+    if (ctx->auxJoin) (void)hipEventDestroy(ctx->auxJoin);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -1353,7 +1381,7 @@ int shiro_hip_set_tuning(shiro_hip_ctx* ctx, const shiro_hip_tuning* t) {
         (t->primaryPackets > 1u && t->primaryPackets != 4u && t->primaryPackets != 16u && t->primaryPackets != 64u) ||
         (t->bvhWidth != 4u && t->bvhWidth != 8u) || t->manyBranches == 0 || t->manyBranches > 16u ||
         t->leafWaitLanes > 64u || t->shadowLeafWaitLanes > 64u ||
-        (t->tailDepth > 1024u && t->tailDepth != SHIRO_HIP_TAIL_AUTO))
+        (t->tailDepth > 1024u && t->tailDepth != SHIRO_HIP_TAIL_AUTO) || t->shadowOverlap > 1u)
         return fail(ctx, SHIRO_HIP_INVALID, "invalid tuning");
     ctx->tuning = *t;
     return SHIRO_HIP_OK;

@@ -119,6 +119,12 @@ struct WfLaunch {
     uint32_t bvhWidth;     // 4 or 8: tree of the traversal kernels (8 only if the scene has one)
     uint32_t tailDepth;    // bounce from which k_wf_tail finishes the pass (0: never)
     uint32_t tailBlocks;   // grid of k_wf_tail (one-wave blocks)
+    // Non-null: the any-hit traversal and resolve of bounce d run on `aux`
+    // (after `fork`, recorded on the pass's stream once bounce d is shaded)
+    // while the closest hit of bounce d + 1 runs on the pass's stream, which
+    // waits on `join` before the next shading.
+    hipStream_t aux;
+    hipEvent_t fork, join;
 };
 
 // Optional per-kernel timing: an event recorded after each launch, tagged

@@ -971,9 +971,28 @@ hipError_t launch_wavefront_pass(const WfParams& P, const WfLaunch& L, hipStream
     hipLaunchKernelGGL(k_wf_reset, dim3(1), dim3(256), 0, s, P.B.counts, P.B.countSlots);
     m.mark(SHIRO_HIP_STAGE_RESET, s);
     const uint32_t tblocks = fullBlocks < L.traceBlocks ? fullBlocks : L.traceBlocks;
+    // Bounce d's shadow rays and bounce d + 1's closest hits both need only
+    // bounce d's shading, and touch disjoint arrays (svis and the radiance
+    // against the hit records), so with an aux stream they run side by side:
+    // each persistent grid's last waves drain while the other's run, and
+    // the memory-bound resolve overlaps the traversal.  Per-stage timing
+    // keeps the serial order.
+    const bool overlap = L.aux != nullptr && marks == nullptr;
+    hipError_t err = hipSuccess;
+    bool joinPending = false;
+    auto join = [&]() {
+        if (joinPending) {
+            const hipError_t e = hipStreamWaitEvent(s, L.join, 0);
+            if (e != hipSuccess && err == hipSuccess) err = e;
+            joinPending = false;
+        }
+    };
+    auto tailAt = [&](uint32_t d) { return d > 0u && d == L.tailDepth && !P.hitLog; };
+    bool traced = false;  // the closest hit of bounce d was launched with bounce d - 1's shadow rays
     for (uint32_t d = 0; d < P.settings.maxDepth; ++d) {
-        if (d > 0u && d == L.tailDepth && !P.hitLog) {
+        if (tailAt(d)) {
             // The rest of the pass in one persistent kernel.
+            join();
             const dim3 g(L.tailBlocks), b(64);
             switch ((P.scene.textured ? 2u : 0u) + (P.scene.plainBsdf ? 1u : 0u)) {
                 case 3: hipLaunchKernelGGL((k_wf_tail<true, true>), g, b, 0, s, P, d); break;
@@ -984,7 +1003,9 @@ hipError_t launch_wavefront_pass(const WfParams& P, const WfLaunch& L, hipStream
             m.mark(SHIRO_HIP_STAGE_TAIL, s);
             break;
         }
-        if (d == 0u && L.primaryPackets) {
+        if (traced) {
+            traced = false;
+        } else if (d == 0u && L.primaryPackets) {
             const bool w8 = L.bvhWidth == 8u && P.scene.nodes8 != nullptr;
             if (P.wfCount) {
                 if (w8) {
@@ -1008,14 +1029,33 @@ hipError_t launch_wavefront_pass(const WfParams& P, const WfLaunch& L, hipStream
             hipLaunchKernelGGL(k_wf_order, dim3(kOrderBlocks), dim3(kOrderThreads), 0, s, P, d);
             m.mark(SHIRO_HIP_STAGE_ORDER, s);
         }
+        join();  // bounce d - 1's resolve updates the radiance bounce d's shading reads
         const uint32_t sblocks = (blocks + kShadeWaves - 1u) / kShadeWaves;
         launch_shade(P, d, sblocks, s);
         m.mark(d == 0u ? SHIRO_HIP_STAGE_SHADE_FIRST : SHIRO_HIP_STAGE_SHADE, s);
-        launch_trace<true>(P, L, d, blocks, tblocks, s);
-        m.mark(SHIRO_HIP_STAGE_SHADOW, s);
-        hipLaunchKernelGGL(k_wf_resolve, dim3(blocks), dim3(64), 0, s, P, d);
-        m.mark(SHIRO_HIP_STAGE_RESOLVE, s);
+        if (overlap) {
+            hipError_t e = hipEventRecord(L.fork, s);
+            if (e == hipSuccess) e = hipStreamWaitEvent(L.aux, L.fork, 0);
+            if (e != hipSuccess) return e;
+            launch_trace<true>(P, L, d, blocks, tblocks, L.aux);
+            hipLaunchKernelGGL(k_wf_resolve, dim3(blocks), dim3(64), 0, L.aux, P, d);
+            e = hipEventRecord(L.join, L.aux);
+            if (e != hipSuccess) return e;
+            joinPending = true;
+            const uint32_t next = d + 1u;
+            if (next < P.settings.maxDepth && !tailAt(next)) {
+                launch_trace<false>(P, L, next, blocks, tblocks, s);
+                traced = true;
+            }
+        } else {
+            launch_trace<true>(P, L, d, blocks, tblocks, s);
+            m.mark(SHIRO_HIP_STAGE_SHADOW, s);
+            hipLaunchKernelGGL(k_wf_resolve, dim3(blocks), dim3(64), 0, s, P, d);
+            m.mark(SHIRO_HIP_STAGE_RESOLVE, s);
+        }
     }
+    join();
+    if (err != hipSuccess) return err;
     hipLaunchKernelGGL(k_wf_accum, dim3(P.tiles), dim3(64), 0, s, P);
     m.mark(SHIRO_HIP_STAGE_ACCUM, s);
     return hipGetLastError();

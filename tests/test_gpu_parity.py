@@ -595,3 +595,31 @@ def test_tail_kernel_identical(hip_ctx, make):
                     assert np.array_equal(getattr(wf, k), getattr(mk, k)), (tail, width, k)
     finally:
         hip_ctx.set_tuning(**{k: getattr(saved, k) for k, _ in abi.Tuning._fields_})
+
+
+@pytest.mark.parametrize("make", [c1_mini, c2_mini, c3_mini, emissive_mini, c5_mini],
+                         ids=["c1", "c2", "c3", "emissive", "c5-textured"])
+def test_shadow_overlap_identical(hip_ctx, make):
+    """A bounce's shadow rays and resolve on a second stream beside the next
+    bounce's closest hits (tuning.shadowOverlap) touch arrays the closest
+    hits do not, and the next shading waits for both: frames equal the
+    serial order's and the megakernel's bit for bit, with and without the
+    tail kernel, in one and in several passes per frame, with lane refill
+    from pools and with wave rounds.  (The adjoint tests run the forward
+    pass with the replay log under the default, overlapped, order.)"""
+    from shiro_amd import abi
+    wl = make()
+    _log(f"shadow overlap {wl.name}")
+    hip_ctx.upload(wl.scene)
+    mk = hip_ctx.render(wl.camera, wl.settings)
+    st = RenderSettings(**{**wl.settings.__dict__, "transport": abi.TRANSPORT_WAVEFRONT})
+    saved = hip_ctx.get_tuning()
+    try:
+        for overlap in (0, 1):
+            for tail, paths, fetch in ((0, 1 << 16, 2), (abi.TAIL_AUTO, 1 << 14, 0), (3, 1 << 20, 2)):
+                hip_ctx.set_tuning(shadowOverlap=overlap, tailDepth=tail, pathsPerPass=paths, fetchMode=fetch)
+                wf = hip_ctx.render(wl.camera, st)
+                for k in ("beauty", "albedo", "normal", "depth"):
+                    assert np.array_equal(getattr(wf, k), getattr(mk, k)), (overlap, tail, paths, k)
+    finally:
+        hip_ctx.set_tuning(**{k: getattr(saved, k) for k, _ in abi.Tuning._fields_})
