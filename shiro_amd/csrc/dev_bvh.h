@@ -275,9 +275,24 @@ DEV uint32_t node_boxes8(const float4* __restrict__ nd, const RayPre& r, float t
     return mask & (ex >> 24);
 }
 
+// cswap on integer keys: the bit patterns of non-negative floats (entry
+// distances, FLT_MAX, +inf) order as the floats do, and on wave-uniform
+// operands the compiler keeps the comparison on the scalar unit.
+DEV void cswap(uint32_t& ka, uint32_t& va, uint32_t& kb, uint32_t& vb) {
+    if (kb < ka) {
+        const uint32_t tk = ka;
+        ka = kb;
+        kb = tk;
+        const uint32_t tv = va;
+        va = vb;
+        vb = tv;
+    }
+}
+
 // Sorts eight (key, value) pairs by key: the 19-comparator network of
 // depth 6 for eight inputs (Knuth, TAOCP vol. 3, 5.3.4).
-DEV void sort8(float k[8], uint32_t v[8]) {
+template <typename K>
+DEV void sort8(K k[8], uint32_t v[8]) {
     cswap(k[0], v[0], k[2], v[2]);
     cswap(k[1], v[1], k[3], v[3]);
     cswap(k[4], v[4], k[6], v[6]);
@@ -454,14 +469,18 @@ DEV HitRec intersect_closest_packet(const DevScene& S, const Ray& ray, bool vali
                 // Children ordered by the first hitting lane's entry
                 // distances; children only other lanes hit follow (key
                 // FLT_MAX), unhit ones sort last (key +inf).
+                // The leader's values are read into scalar registers
+                // (readlane) and sorted there as integer keys: the sort is
+                // the same for every lane, so it runs on the scalar unit
+                // beside the vector instructions of other waves.
                 const int leader = __ffsll(static_cast<long long>(hitters)) - 1;
-                const uint32_t lm = __shfl(mask, leader);
-                float key[W];
-                uint32_t val[W];
+                const uint32_t lm = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(mask), leader));
+                uint32_t key[W], val[W];
                 for (int k = 0; k < W; ++k) {
-                    const float lt = __shfl(tn[k], leader);
-                    key[k] = (lm >> k) & 1u ? lt : ((any >> k) & 1u ? kFltMax : kInf);
-                    val[k] = __shfl(wd[k], leader);
+                    const uint32_t lt = static_cast<uint32_t>(
+                        __builtin_amdgcn_readlane(static_cast<int>(__float_as_uint(tn[k])), leader));
+                    key[k] = (lm >> k) & 1u ? lt : ((any >> k) & 1u ? __float_as_uint(kFltMax) : __float_as_uint(kInf));
+                    val[k] = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(wd[k]), leader));
                 }
                 if constexpr (W == 8) {
                     sort8(key, val);
