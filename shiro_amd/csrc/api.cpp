@@ -23,6 +23,7 @@
 
 #include "../../include/shiro_hip.h"
 #include "bvh_build.h"
+#include "dev_math.h"
 #include "host_scene.h"
 #include "kernels.h"
 
@@ -202,7 +203,7 @@ DevCamera to_dev(const shiro_hip_camera& c) {
     std::memcpy(d.forward, c.forward, 12);
     std::memcpy(d.right, c.right, 12);
     std::memcpy(d.up, c.up, 12);
-    d.vfov = c.verticalFovDegrees;
+    d.tanHalfFov = std::tan(c.verticalFovDegrees * 0.5f * kPi / 180.0f);
     d.aspect = c.aspectRatio;
     return d;
 }

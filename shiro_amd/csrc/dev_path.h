@@ -131,7 +131,7 @@ DEV bool dome_sample(const DevScene& S, const DevDome& d, V2 u, V3* dir, V3* rad
 DEV Ray camera_ray(const DevCamera& c, const DevSettings& st, uint32_t x, uint32_t y, V2 j) {
     const float fx = ((static_cast<float>(x) + j.x) / static_cast<float>(st.width)) * 2.0f - 1.0f;
     const float fy = ((static_cast<float>(y) + j.y) / static_cast<float>(st.height)) * 2.0f - 1.0f;
-    const float th = tanf(c.vfov * 0.5f * kPi / 180.0f);
+    const float th = c.tanHalfFov;
     const V3 d = ld3(c.forward) + ld3(c.right) * (fx * c.aspect * th) + ld3(c.up) * (fy * th);
     Ray r;
     r.o = ld3(c.position);

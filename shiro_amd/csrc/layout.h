@@ -191,7 +191,10 @@ struct DevScene {
 
 struct DevCamera {
     float position[3], forward[3], right[3], up[3];
-    float vfov, aspect;
+    // tan(verticalFovDegrees / 2 in radians), computed once on the host
+    // with the float tan of the CPU path (CpuPathTracer.cpp:1636) instead
+    // of once per camera ray.
+    float tanHalfFov, aspect;
 };
 
 struct DevSettings {
