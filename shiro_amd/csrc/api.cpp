@@ -307,6 +307,9 @@ struct WfMemory {
 // blockStart | counters | sums beauty albedo normal depth.
 void wavefront_sizes(uint64_t cap, uint32_t M, bool order, uint64_t sumSlots, uint32_t slots, size_t* n) {
     for (int i = 0; i < 10; ++i) n[i] = cap * 16;
+    // o, d, T and E: one array of 64-byte path records (WfBuffers).
+    n[0] = cap * 16 * kPathRecord;
+    n[1] = n[2] = n[4] = 0;
     n[10] = cap * M * 32;
     n[11] = n[12] = n[13] = cap * 4;
     n[14] = n[15] = cap * M * 4;
@@ -411,6 +414,9 @@ int enqueue_wavefront(shiro_hip_ctx* ctx, const DevScene& scene, const shiro_hip
     WfBuffers& B = P.B;
     float4** f4[11] = {&B.o, &B.d, &B.T, &B.L, &B.E, &B.A, &B.N, &B.H, &B.so, &B.st, &B.sr};
     for (int i = 0; i < 11; ++i) *f4[i] = static_cast<float4*>(a[i]);
+    B.d = B.o + 1;
+    B.T = B.o + 2;
+    B.E = B.o + 3;
     B.qa = static_cast<uint32_t*>(a[11]);
     B.qb = static_cast<uint32_t*>(a[12]);
     B.qs = static_cast<uint32_t*>(a[13]);

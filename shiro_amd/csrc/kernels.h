@@ -27,10 +27,19 @@ constexpr uint32_t kPoolStride = 32;
 // (fetch mode 2) under SHIRO_HIP_FETCH_AUTO, smaller ones in wave rounds.
 constexpr uint64_t kFetchPoolPaths = 8ull << 20;
 
-// Wavefront path state (structure of arrays, `cap` paths) and queues.
+// Wavefront path state (`cap` paths) and queues.  Ray origin, direction,
+// throughput and extinction of path p are one 64-byte record (o, d, T and
+// E point at its four float4 slots; entry p of each is at kPathRecord * p):
+// after the first bounce the stages visit paths in queue and shading-class
+// order, scattered over the pass, and a record costs one 64-byte memory
+// sector per path where separate arrays cost one each.  The radiance L is
+// an array of its own: the resolve of the first bounce and the
+// accumulation read it in path order, four paths to a sector.
+constexpr uint32_t kPathRecord = 4;
 struct WfBuffers {
-    float4 *o, *d, *T, *L, *E, *A, *N, *H;  // ray origin|prevPdf, dir|bits, throughput, radiance,
-                                            // extinction, albedo|first-hit t, normal, hit record
+    float4 *o, *d, *T, *L, *E, *A, *N, *H;  // ray origin|prevPdf, dir|bits, throughput (the path
+                                            // record), radiance, extinction (the record's fourth
+                                            // slot), albedo|first-hit t, normal, hit record
     float4 *so, *st;                        // shadow header: origin|count, throughput at the hit|count
     float4* sr;                             // shadow rays (path * maxShadow + index): 2 x float4 each,
                                             // direction|max distance (read by the any-hit kernel) and

@@ -147,7 +147,8 @@ DEV Sampler path_sampler(const WfParams& P, const PathCoord& c) {
 // 0 and never read back here.
 DEV PathState load_state(const WfBuffers& B, uint32_t p) {
     PathState ps;
-    const float4 o = B.o[p], d = B.d[p], T = B.T[p], L = B.L[p];
+    const size_t r = static_cast<size_t>(p) * kPathRecord;
+    const float4 o = B.o[r], d = B.d[r], T = B.T[r], L = B.L[p];
     ps.o = xyz(o);
     ps.prevPdf = o.w;
     ps.d = xyz(d);
@@ -155,7 +156,7 @@ DEV PathState load_state(const WfBuffers& B, uint32_t p) {
     ps.T = xyz(T);
     ps.L = xyz(L);
     ps.ext = V3{0.0f, 0.0f, 0.0f};
-    if (ps.bits & kBitsMediumOn) ps.ext = xyz(B.E[p]);
+    if (ps.bits & kBitsMediumOn) ps.ext = xyz(B.E[r]);
     ps.albedo = ps.normal = V3{0.0f, 0.0f, 0.0f};
     ps.depth = kInf;
     return ps;
@@ -166,11 +167,12 @@ DEV PathState load_state(const WfBuffers& B, uint32_t p) {
 // the extinction only when it was (re)set by entering a medium.
 DEV void store_state(const WfBuffers& B, uint32_t p, const PathState& ps, uint32_t depth, bool alive,
                      bool extChanged) {
+    const size_t r = static_cast<size_t>(p) * kPathRecord;
     if (alive) {
-        B.o[p] = make_float4(ps.o.x, ps.o.y, ps.o.z, ps.prevPdf);
-        B.d[p] = make_float4(ps.d.x, ps.d.y, ps.d.z, __uint_as_float(ps.bits));
-        B.T[p] = make_float4(ps.T.x, ps.T.y, ps.T.z, 0.0f);
-        if (extChanged) B.E[p] = make_float4(ps.ext.x, ps.ext.y, ps.ext.z, 0.0f);
+        B.o[r] = make_float4(ps.o.x, ps.o.y, ps.o.z, ps.prevPdf);
+        B.d[r] = make_float4(ps.d.x, ps.d.y, ps.d.z, __uint_as_float(ps.bits));
+        B.T[r] = make_float4(ps.T.x, ps.T.y, ps.T.z, 0.0f);
+        if (extChanged) B.E[r] = make_float4(ps.ext.x, ps.ext.y, ps.ext.z, 0.0f);
     }
     B.L[p] = make_float4(ps.L.x, ps.L.y, ps.L.z, 0.0f);
     if (depth == 0u) {
@@ -337,8 +339,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W == 8 ? (AN
                     if (valid) ray = camera_ray(P.camera, P.settings, c.x, c.y, path_sampler(P, c).next2(0u));
                 } else {
                     item = in[idx];
-                    ray.o = xyz(P.B.o[item]);
-                    ray.d = xyz(P.B.d[item]);
+                    ray.o = xyz(P.B.o[static_cast<size_t>(item) * kPathRecord]);
+                    ray.d = xyz(P.B.d[static_cast<size_t>(item) * kPathRecord]);
                 }
                 if (valid) {
                     ray.tmin = kRayEps;
